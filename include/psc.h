@@ -1,0 +1,201 @@
+/*
+ * psc.h -- C ABI of the MI355X-native PSpectCosmo lattice time step.
+ *
+ * The reference (PSpectCosmo) has no plugin or FFI: its boundary is the
+ * compile-time configuration (parameters.h, model.h), the step function
+ * `void evolve_VV_k_nopad(double dt)` acting on global state
+ * (pspectcosmo.h:241-242, evolution.cpp:1131-1198), and the output routines
+ * that read the same globals (output.cpp).  Every entry point below states the
+ * reference routine or data it replaces.
+ *
+ * Conventions (shared by every function):
+ *   - plain C types only; every function returns an int status (PSC_OK == 0)
+ *     and never exits the process (the reference prints and calls exit(1),
+ *     output.cpp:450-454, 480-483);
+ *   - k-space arrays crossing the boundary are FP64 half spectra in FFTW r2c
+ *     layout: element [(i*N + j)*(N/2+1) + k] holds (re, im) of the mode with
+ *     lattice momentum (i<=N/2 ? i : i-N, j<=N/2 ? j : j-N, k), unnormalised
+ *     (forward e^{-ikx}), exactly as f_k / pi_k in pspectcosmo.h:73-74;
+ *   - real-space arrays are FP64, index [(i*N + j)*N + k] (pspectcosmo.h:63);
+ *   - one context per host thread; device memory is owned by the context;
+ *     calls on one context are not re-entrant.
+ */
+#ifndef PSC_H
+#define PSC_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PSC_ABI_VERSION 1
+
+/* ---- status codes ------------------------------------------------------ */
+#define PSC_OK 0
+#define PSC_ERR_ARG 1          /* invalid argument or configuration            */
+#define PSC_ERR_DEVICE 2       /* HIP runtime error or missing GPU              */
+#define PSC_ERR_NOMEM 3        /* device or host allocation failed             */
+#define PSC_ERR_STATE 4        /* call out of order (e.g. step before load)    */
+#define PSC_ERR_NONFINITE 5    /* field mean not finite: output.cpp:450-454    */
+#define PSC_ERR_NEGATIVE_A 6   /* scale factor < 0: output.cpp:480-483         */
+#define PSC_ERR_UNSUPPORTED 7  /* configuration outside what the build covers */
+
+/* ---- enumerations ------------------------------------------------------ */
+#define PSC_VV2 2              /* velocity Verlet, pspectcosmo.h:283-285       */
+#define PSC_Y4 4               /* Yoshida 4th order, pspectcosmo.h:286-288     */
+
+#define PSC_FP64 0             /* all fields/FFTs in FP64                      */
+#define PSC_FP32 1             /* FP32 fields/FFTs, FP64 scale factor,
+                                  reductions and spectrum bins               */
+
+#define PSC_EXPANSION_STATIC 0 /* a = 1, da/dt = 0 (Minkowski test mode)       */
+#define PSC_EXPANSION_SELF 2   /* self-consistent Friedmann, parameters.h:65   */
+
+#define PSC_GW_PROJ_REFERENCE 0 /* one-sided P.u as in output.cpp:819-831      */
+#define PSC_GW_PROJ_TT 1        /* Lambda_ij,lm u_lm, true TT projection       */
+
+/* Number of doubles written by psc_energies / psc_oracle energies. */
+#define PSC_ENERGY_LEN 16
+/* Columns of psc_spectra per bin: p, S_f, S_pi, n_k, numpoints. */
+#define PSC_SPECTRA_COLS 5
+/* Columns of psc_gw_spectrum per bin: p, S_h, Omega_gw, prefactor, numpoints. */
+#define PSC_GW_SPECTRA_COLS 5
+
+/* Run configuration: the runtime form of parameters.h (parameters.h:10-134). */
+typedef struct psc_config {
+    int N;             /* lattice points per edge, power of two 16..512 (parameters.h:22) */
+    int nflds;         /* 1 or 2 scalar fields (parameters.h:20)                          */
+    int integrator;    /* PSC_VV2 or PSC_Y4 (parameters.h:10 `VV`)                        */
+    int precision;     /* PSC_FP64 or PSC_FP32                                             */
+    int batch;         /* independent ensemble members advanced together (>= 1)           */
+    int with_gw;       /* evolve the 6 symmetric u_ij and pi_u_ij (parameters.h:15)       */
+    int alpha;         /* time variable exponent; 0 = cosmic time (parameters.h:121)      */
+    int expansion;     /* PSC_EXPANSION_SELF (reference) or PSC_EXPANSION_STATIC          */
+    int yoshida_exact; /* 0: the reference's Yoshida digits (pspectcosmo.h:288);
+                          1: the exact triple-jump weights                               */
+    int use_graph;     /* capture each step in a hipGraph and replay it                   */
+    int reserved_i[6];
+    double kIR;        /* lattice IR wavenumber, L = 2 pi / kIR (parameters.h:28,32)       */
+    double dt;         /* time step in program units (parameters.h:36)                     */
+    double fStar;      /* field rescaling (parameters.h:133)                               */
+    double omegaStar;  /* time rescaling (parameters.h:134)                                */
+    double MPl;        /* reduced Planck mass in program units (parameters.h:122)         */
+    double reserved_d[3];
+} psc_config;
+
+/* Potential V = m2 phi^2 / 2 + g2 phi^2 chi^2 / 2 (the chi term only when
+ * nflds == 2).  The shipped single-field model.h (model.h:35-104) is the
+ * nflds == 1, m2 == 1 case. */
+typedef struct psc_model {
+    double m2;
+    double g2;
+} psc_model;
+
+/* Initial-condition parameters (parameters.h:62,124-125; initialize.cpp:624-668). */
+typedef struct psc_ic_params {
+    double f0[2];      /* homogeneous field values, program units (initfield/fStar)        */
+    double fd0[2];     /* homogeneous field velocities, program units                      */
+    double a0;         /* initial scale factor (parameters.h:29)                           */
+    int seed;          /* Park-Miller seed (parameters.h:62); < 1 is the reference's
+                          debug mode where every draw returns 0.33 (initialize.cpp:15) */
+    int reserved_i;
+    double reserved_d[4];
+} psc_ic_params;
+
+/* ---- host-side utilities (no device needed) --------------------------- */
+
+int psc_abi_version(void);
+const char* psc_status_string(int status);
+
+/* Fill the initial half spectra for all fields, replacing initialize()
+ * (initialize.cpp:562-810, non-inflation Rayleigh branch initialize.cpp:181-197).
+ * f_k and pi_k hold nflds * N*N*(N/2+1) complex values (2 doubles each).
+ * scalars_out[0..3] = a, da/dt, the printed d2a/dt2 (initialize.cpp:667), H. */
+int psc_generate_ic(const psc_config* cfg, const psc_model* model, const psc_ic_params* ic,
+                    double* f_k, double* pi_k, double* scalars_out);
+
+/* Fill the 6 symmetric GW components (11,22,33,12,13,23) of u_ij and pi_u_ij,
+ * replacing initialize_GW() (initialize.cpp:341-470).  The Park-Miller stream
+ * continues after the scalar-field draws of psc_generate_ic with the same seed,
+ * as in the reference (pspectcosmo.cpp:458,488).  Each output holds
+ * 6 * N*N*(N/2+1) complex values. */
+int psc_generate_gw_ic(const psc_config* cfg, const psc_model* model, const psc_ic_params* ic,
+                       double* u_k, double* pi_u_k);
+
+/* ---- device context ----------------------------------------------------- */
+
+typedef struct psc_ctx psc_ctx;
+
+/* Allocate device state for cfg->batch members (pspectcosmo.cpp:253-282). */
+int psc_create(const psc_config* cfg, psc_ctx** out);
+int psc_destroy(psc_ctx* ctx);
+/* Text of the last error on this context (never NULL). */
+const char* psc_last_error(const psc_ctx* ctx);
+
+/* Per-member couplings (model.h:35-98). */
+int psc_set_model(psc_ctx* ctx, int member, const psc_model* model);
+
+/* Copy initial half spectra to the device and compute d2a/dt2 from them as
+ * initialize.cpp:797 does with kernel_a_k_nopad (evolution.cpp:1061-1090). */
+int psc_load_ic_k(psc_ctx* ctx, int member, const double* f_k, const double* pi_k,
+                  double a, double ad);
+/* Overwrite the background scalars t, a, da/dt, d2a/dt2 of one member. */
+int psc_set_scalars(psc_ctx* ctx, int member, double t, double a, double ad, double add);
+/* out[0..3] = t, a, da/dt, d2a/dt2 (pspectcosmo.h:44-45). */
+int psc_get_scalars(psc_ctx* ctx, int member, double* out);
+
+/* Advance every member by nsteps fixed steps of cfg->dt; replaces
+ * `evolve_VV_k_nopad(dt)` called from the main loop (pspectcosmo.cpp:636). */
+int psc_step(psc_ctx* ctx, int nsteps);
+/* Block until all device work of the context is done. */
+int psc_sync(psc_ctx* ctx);
+
+/* Read the state back as FP64 half spectra (f_k, pi_k of pspectcosmo.h:73-74). */
+int psc_get_k(psc_ctx* ctx, int member, double* f_k, double* pi_k);
+/* Real-space fields f and pi (c2r and /N^3, output.cpp:401-425). Either may be NULL. */
+int psc_get_fields(psc_ctx* ctx, int member, double* f, double* pi);
+
+/* Energies, replacing energy_density()/energy() (output.cpp:57-115,1069-1187).
+ * out[PSC_ENERGY_LEN]: K_0, G_0, K_1, G_1, V_term0, V_term1, rho_total,
+ * constraint violation |H^2 - rho/3| / |H^2 + rho/3| (output.cpp:1180-1184),
+ * a, da/dt, d2a/dt2, t, then <f_0>, <f_1>, var f_0, var f_1 (output.cpp:380-461). */
+int psc_energies(psc_ctx* ctx, int member, double* out);
+
+/* Binned scalar spectra of field `fld`, replacing spectra() (output.cpp:490-705).
+ * out holds nbins rows of PSC_SPECTRA_COLS doubles for bins 1..nbins, where
+ * nbins = maxnumbins - 1 and maxnumbins = floor(sqrt(3) * N / 2) + 2
+ * (parameters.h:25). */
+int psc_spectra(psc_ctx* ctx, int member, int fld, double* out, int nbins);
+
+/* GW state in and out (6 symmetric components, FFTW layout, FP64). */
+int psc_load_gw_k(psc_ctx* ctx, int member, const double* u_k, const double* pi_u_k);
+int psc_get_gw_k(psc_ctx* ctx, int member, double* u_k, double* pi_u_k);
+/* GW spectrum (output.cpp:780-1062) with projection PSC_GW_PROJ_REFERENCE or
+ * PSC_GW_PROJ_TT.  out: nbins rows of PSC_GW_SPECTRA_COLS; totals[0..1] =
+ * rho_GW / rho, rho_GW (output.cpp:1054-1060). */
+int psc_gw_spectrum(psc_ctx* ctx, int member, int projection, double* out, int nbins,
+                    double* totals);
+
+/* Number of FP64 bins of psc_spectra for a lattice of N points per edge. */
+int psc_num_bins(int N);
+
+/* ---- instrumentation --------------------------------------------------- */
+
+/* When on, psc_step brackets every kernel with HIP events on the context's
+ * stream and accumulates per-kernel durations (graphs are not used then). */
+int psc_set_kernel_timing(psc_ctx* ctx, int on);
+/* Per-kernel-kind totals since the last reset: names (comma separated, into
+ * names/names_len), total milliseconds, launch counts and the algorithmic bytes
+ * each launch moves.  *nkinds receives the number of kinds (<= cap). */
+int psc_get_kernel_times(psc_ctx* ctx, char* names, int names_len, double* total_ms,
+                         long long* counts, double* bytes_per_launch, int cap, int* nkinds);
+int psc_reset_kernel_times(psc_ctx* ctx);
+/* Bytes of device memory the context holds. */
+int psc_device_bytes(psc_ctx* ctx, double* bytes);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PSC_H */

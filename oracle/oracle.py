@@ -1,0 +1,210 @@
+"""TEST INFRASTRUCTURE ONLY: ctypes binding of the CPU FP64 oracle
+(oracle/psc_oracle.cpp).  Only tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py may import this module; the product library never
+does."""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from pspectcosmo_amd.abi import (PscConfig, PscModel, PscIcParams, PSC_ENERGY_LEN,
+                                 PSC_SPECTRA_COLS, PSC_GW_SPECTRA_COLS, num_bins, half_len)
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+_D = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+
+
+def build(force=False):
+    if force or not os.path.exists(_LIB_PATH):
+        subprocess.check_call(["make", "-C", _HERE, "-B" if force else "liboracle.so"],
+                              stdout=subprocess.DEVNULL)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = C.CDLL(_LIB_PATH)
+        P = C.POINTER
+        L.orc_set_threads.argtypes = [C.c_int]
+        L.orc_park_miller_iterate.restype = C.c_longlong
+        L.orc_park_miller_iterate.argtypes = [C.c_longlong, C.c_longlong]
+        L.orc_rng_draws.argtypes = [C.c_int, C.c_int, _D]
+        L.orc_fft_r2c_3d.argtypes = [C.c_int, _D, _D]
+        L.orc_fft_c2r_3d.argtypes = [C.c_int, _D, _D]
+        L.orc_fft_1d.argtypes = [C.c_int, _D, C.c_int]
+        L.orc_generate_ic.argtypes = [P(PscConfig), P(PscModel), P(PscIcParams), _D, _D, _D]
+        L.orc_generate_gw_ic.argtypes = [P(PscConfig), P(PscModel), P(PscIcParams), _D, _D]
+        L.orc_create.restype = C.c_void_p
+        L.orc_create.argtypes = [P(PscConfig), P(PscModel)]
+        L.orc_destroy.argtypes = [C.c_void_p]
+        L.orc_load.argtypes = [C.c_void_p, _D, _D, C.c_double, C.c_double, C.c_double, C.c_double]
+        L.orc_load_gw.argtypes = [C.c_void_p, _D, _D]
+        L.orc_get_gw.argtypes = [C.c_void_p, _D, _D]
+        L.orc_kernel_a.restype = C.c_double
+        L.orc_kernel_a.argtypes = [C.c_void_p]
+        L.orc_step.argtypes = [C.c_void_p, C.c_int]
+        L.orc_get_k.argtypes = [C.c_void_p, _D, _D]
+        L.orc_get_scalars.argtypes = [C.c_void_p, _D]
+        L.orc_get_fields.argtypes = [C.c_void_p, _D, _D]
+        L.orc_field_derivative.argtypes = [C.c_void_p, C.c_int, C.c_int, _D]
+        L.orc_energies.argtypes = [C.c_void_p, _D]
+        L.orc_spectra.argtypes = [C.c_void_p, C.c_int, _D, C.c_int]
+        L.orc_gw_spectrum.argtypes = [C.c_void_p, C.c_int, _D, C.c_int, _D]
+        _lib = L
+    return _lib
+
+
+def set_threads(n):
+    return lib().orc_set_threads(int(n))
+
+
+def park_miller_iterate(state, steps):
+    return lib().orc_park_miller_iterate(state, steps)
+
+
+def rng_draws(seed, n):
+    out = np.zeros(n)
+    lib().orc_rng_draws(seed, n, out)
+    return out
+
+
+def fft_r2c_3d(x):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    N = x.shape[0]
+    out = np.zeros((N, N, N // 2 + 1), dtype=np.complex128)
+    lib().orc_fft_r2c_3d(N, x.ravel(), out.view(np.float64).ravel())
+    return out
+
+
+def fft_c2r_3d(xk):
+    xk = np.ascontiguousarray(xk, dtype=np.complex128)
+    N = xk.shape[0]
+    out = np.zeros((N, N, N))
+    lib().orc_fft_c2r_3d(N, xk.view(np.float64).ravel(), out.ravel())
+    return out
+
+
+def fft_1d(x, sign=-1):
+    x = np.array(x, dtype=np.complex128)
+    buf = x.view(np.float64).copy()
+    lib().orc_fft_1d(len(x), buf, sign)
+    return buf.view(np.complex128)
+
+
+def generate_ic(cfg, model, ic):
+    """Half spectra (nflds, N, N, N/2+1) complex, and scalars
+    (a, ad, add_printed, H, add_kernel)."""
+    N, nf = cfg.N, cfg.nflds
+    fk = np.zeros((nf, N, N, N // 2 + 1), dtype=np.complex128)
+    pk = np.zeros_like(fk)
+    sc = np.zeros(5)
+    rc = lib().orc_generate_ic(C.byref(cfg), C.byref(model), C.byref(ic),
+                               fk.view(np.float64).ravel(), pk.view(np.float64).ravel(), sc)
+    assert rc == 0, rc
+    return fk, pk, sc
+
+
+def generate_gw_ic(cfg, model, ic):
+    N = cfg.N
+    u = np.zeros((6, N, N, N // 2 + 1), dtype=np.complex128)
+    pu = np.zeros_like(u)
+    rc = lib().orc_generate_gw_ic(C.byref(cfg), C.byref(model), C.byref(ic),
+                                  u.view(np.float64).ravel(), pu.view(np.float64).ravel())
+    assert rc == 0, rc
+    return u, pu
+
+
+class Oracle:
+    """One lattice evolved by the FP64 CPU restatement."""
+
+    def __init__(self, cfg, model):
+        self.cfg, self.model = cfg, model
+        self.N, self.nf = cfg.N, cfg.nflds
+        self._h = lib().orc_create(C.byref(cfg), C.byref(model))
+        if not self._h:
+            raise ValueError("oracle rejected the configuration")
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib().orc_destroy(h)
+            self._h = None
+
+    def load(self, fk, pk, t, a, ad, add):
+        fk = np.ascontiguousarray(fk, dtype=np.complex128)
+        pk = np.ascontiguousarray(pk, dtype=np.complex128)
+        lib().orc_load(self._h, fk.view(np.float64).ravel(), pk.view(np.float64).ravel(),
+                       t, a, ad, add)
+
+    def load_gw(self, u, pu):
+        u = np.ascontiguousarray(u, dtype=np.complex128)
+        pu = np.ascontiguousarray(pu, dtype=np.complex128)
+        assert lib().orc_load_gw(self._h, u.view(np.float64).ravel(),
+                                 pu.view(np.float64).ravel()) == 0
+
+    def get_gw(self):
+        N = self.N
+        u = np.zeros((6, N, N, N // 2 + 1), dtype=np.complex128)
+        pu = np.zeros_like(u)
+        assert lib().orc_get_gw(self._h, u.view(np.float64).ravel(),
+                                pu.view(np.float64).ravel()) == 0
+        return u, pu
+
+    def kernel_a(self):
+        return lib().orc_kernel_a(self._h)
+
+    def step(self, n=1):
+        lib().orc_step(self._h, int(n))
+
+    def get_k(self):
+        N = self.N
+        fk = np.zeros((self.nf, N, N, N // 2 + 1), dtype=np.complex128)
+        pk = np.zeros_like(fk)
+        lib().orc_get_k(self._h, fk.view(np.float64).ravel(), pk.view(np.float64).ravel())
+        return fk, pk
+
+    def scalars(self):
+        s = np.zeros(4)
+        lib().orc_get_scalars(self._h, s)
+        return s
+
+    def fields(self):
+        N = self.N
+        f = np.zeros((self.nf, N, N, N))
+        p = np.zeros_like(f)
+        lib().orc_get_fields(self._h, f.ravel(), p.ravel())
+        return f, p
+
+    def derivative(self, fld, axis):
+        N = self.N
+        out = np.zeros((N, N, N))
+        lib().orc_field_derivative(self._h, fld, axis, out.ravel())
+        return out
+
+    def energies(self):
+        e = np.zeros(PSC_ENERGY_LEN)
+        lib().orc_energies(self._h, e)
+        return e
+
+    def spectra(self, fld):
+        nb = num_bins(self.N)
+        out = np.zeros((nb, PSC_SPECTRA_COLS))
+        assert lib().orc_spectra(self._h, fld, out.ravel(), nb) == 0
+        return out
+
+    def gw_spectrum(self, projection):
+        nb = num_bins(self.N)
+        out = np.zeros((nb, PSC_GW_SPECTRA_COLS))
+        tot = np.zeros(2)
+        assert lib().orc_gw_spectrum(self._h, projection, out.ravel(), nb, tot) == 0
+        return out, tot
+
+
+__all__ = ["Oracle", "generate_ic", "generate_gw_ic", "fft_r2c_3d", "fft_c2r_3d", "fft_1d",
+           "rng_draws", "park_miller_iterate", "set_threads", "half_len"]

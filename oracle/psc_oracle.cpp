@@ -1,0 +1,1102 @@
+// psc_oracle.cpp -- TEST INFRASTRUCTURE ONLY.
+//
+// CPU FP64 restatement of the PSpectCosmo spectral lattice step, used as the
+// correctness checker for the HIP library and as the timed CPU baseline.  It is
+// never linked into, loaded by, or called from the product library.  Each
+// function cites the reference routine it restates (file:line in PSpectCosmo).
+//
+// What pins each part (tests/test_oracle_*.py):
+//   rand_uniform            Park-Miller KAT (state 1 -> 1043618065 after 10000
+//                           iterations) plus the 99-draw warm-up offset;
+//   FFT (own radix-2)       brute-force DFT at N=4/8, numpy.fft (pocketfft) at
+//                           N=16/32, closed forms (DC, single plane wave);
+//   Hubble initialisation   README.md:114-126 example output;
+//   initial modes           exact Hermitian symmetry, DC closed form, shell
+//                           statistics of |f_k|^2;
+//   stepper                 exact discrete leapfrog / Yoshida solution of a free
+//                           field in the static background, time reversal, order
+//                           of convergence, homogeneous background vs scipy ODE;
+//   K, G, V reductions      Parseval against real-space sums;
+//   derivatives             plane-wave eigen-relations incl. the zeroed Nyquist;
+//   spectra bins            brute-force bin-count enumeration;
+//   GW projections          idempotence / transversality / tracelessness of
+//                           Lambda, closed form of the one-sided variant.
+// The exact IC arrays of the reference (RNG consumption order across modes) and
+// its own nonlinear step at the shipped configuration are pinned only through
+// the restated visiting order: the reference itself does not build here (no
+// FFTW3 header or library, and no two-field model.h), so that part is
+// "parity unpinned" beyond the checks listed above.
+
+#include <cmath>
+#include <complex>
+#include <cstdint>
+#include <cstring>
+#include <cstdio>
+#include <vector>
+#include <algorithm>
+#include <string>
+#include <omp.h>
+
+#include "../include/psc.h"
+
+typedef std::complex<double> cd;
+
+namespace {
+
+const double kPi = 3.14159265358979323846264338327950288;
+
+// ---------------------------------------------------------------------------
+// Park-Miller minimal standard generator, restating rand_uniform
+// (initialize.cpp:13-28): a = 16807, m = 2^31 - 1, Schrage q = 127773,
+// r = 2836.  The first call discards 99 draws (initialize.cpp:22-24), so the
+// first value returned is the 100th iterate of the seed.  seed < 1 returns the
+// constant 0.33 (debug mode, initialize.cpp:15).
+struct ParkMiller {
+    int seed;
+    int next;
+    bool warmed;
+    explicit ParkMiller(int s) : seed(s), next(s), warmed(false) {
+        if (!(next > 0)) next = 1;  // initialize.cpp:18-21
+    }
+    void advance() {
+        next = 16807 * (next % 127773) - 2836 * (next / 127773);
+        if (next < 0) next += 2147483647;
+    }
+    double uniform() {
+        if (seed < 1) return 0.33;
+        if (!warmed) {
+            for (int i = 1; i < 100; ++i) advance();
+            warmed = true;
+        }
+        advance();
+        return (double)next / (double)2147483647;
+    }
+};
+
+// ---------------------------------------------------------------------------
+// FFT: iterative radix-2 complex transform of a power-of-two length, used to
+// build the unnormalised 3D r2c / c2r transforms in FFTW layout that the
+// reference calls (fftw_plan_dft_r2c_3d / c2r_3d, evolution.cpp:1112,1146).
+struct Fft1D {
+    int n = 0;
+    std::vector<cd> tw;   // exp(-2 pi i k / n), k < n/2
+    std::vector<int> rev;
+    explicit Fft1D(int n_) : n(n_), tw(n_ / 2), rev(n_) {
+        int lg = 0;
+        while ((1 << lg) < n) ++lg;
+        for (int k = 0; k < n / 2; ++k) {
+            double ang = -2.0 * kPi * (double)k / (double)n;
+            tw[k] = cd(std::cos(ang), std::sin(ang));
+        }
+        for (int i = 0; i < n; ++i) {
+            int r = 0;
+            for (int b = 0; b < lg; ++b)
+                if (i & (1 << b)) r |= 1 << (lg - 1 - b);
+            rev[i] = r;
+        }
+    }
+    // sign = -1: forward (e^{-ikx}); +1: inverse, unnormalised.
+    void exec(cd* a, int sign) const {
+        for (int i = 0; i < n; ++i)
+            if (rev[i] > i) std::swap(a[i], a[rev[i]]);
+        for (int len = 2; len <= n; len <<= 1) {
+            int half = len / 2, step = n / len;
+            for (int i = 0; i < n; i += len) {
+                for (int k = 0; k < half; ++k) {
+                    cd w = tw[k * step];
+                    if (sign > 0) w = std::conj(w);
+                    cd u = a[i + k], v = a[i + k + half] * w;
+                    a[i + k] = u + v;
+                    a[i + k + half] = u - v;
+                }
+            }
+        }
+    }
+};
+
+struct Fft3D {
+    int N, Nh;
+    Fft1D f;
+    explicit Fft3D(int N_) : N(N_), Nh(N_ / 2 + 1), f(N_) {}
+
+    // Unnormalised forward real-to-complex transform; out in FFTW layout.
+    void r2c(const double* in, cd* out) const {
+        const int n = N, nh = Nh;
+#pragma omp parallel
+        {
+            std::vector<cd> buf(n);
+#pragma omp for schedule(static)
+            for (long xy = 0; xy < (long)n * n; ++xy) {
+                const double* src = in + xy * n;
+                for (int z = 0; z < n; ++z) buf[z] = cd(src[z], 0.0);
+                f.exec(buf.data(), -1);
+                cd* dst = out + xy * nh;
+                for (int k = 0; k < nh; ++k) dst[k] = buf[k];
+            }
+#pragma omp for schedule(static)
+            for (long xk = 0; xk < (long)n * nh; ++xk) {
+                long x = xk / nh, k = xk % nh;
+                for (int y = 0; y < n; ++y) buf[y] = out[(x * n + y) * nh + k];
+                f.exec(buf.data(), -1);
+                for (int y = 0; y < n; ++y) out[(x * n + y) * nh + k] = buf[y];
+            }
+#pragma omp for schedule(static)
+            for (long yk = 0; yk < (long)n * nh; ++yk) {
+                long y = yk / nh, k = yk % nh;
+                for (int x = 0; x < n; ++x) buf[x] = out[((long)x * n + y) * nh + k];
+                f.exec(buf.data(), -1);
+                for (int x = 0; x < n; ++x) out[((long)x * n + y) * nh + k] = buf[x];
+            }
+        }
+    }
+
+    // Unnormalised inverse complex-to-real transform of a Hermitian half
+    // spectrum; `in` is not modified (FFTW's c2r destroys its input, which is
+    // why the reference copies into Cache_Data_k first, evolution.cpp:1145).
+    void c2r(const cd* in, double* out, std::vector<cd>& tmp) const {
+        const int n = N, nh = Nh;
+        tmp.assign(in, in + (size_t)n * n * nh);
+#pragma omp parallel
+        {
+            std::vector<cd> buf(n);
+#pragma omp for schedule(static)
+            for (long yk = 0; yk < (long)n * nh; ++yk) {
+                long y = yk / nh, k = yk % nh;
+                for (int x = 0; x < n; ++x) buf[x] = tmp[((long)x * n + y) * nh + k];
+                f.exec(buf.data(), +1);
+                for (int x = 0; x < n; ++x) tmp[((long)x * n + y) * nh + k] = buf[x];
+            }
+#pragma omp for schedule(static)
+            for (long xk = 0; xk < (long)n * nh; ++xk) {
+                long x = xk / nh, k = xk % nh;
+                for (int y = 0; y < n; ++y) buf[y] = tmp[(x * n + y) * nh + k];
+                f.exec(buf.data(), +1);
+                for (int y = 0; y < n; ++y) tmp[(x * n + y) * nh + k] = buf[y];
+            }
+#pragma omp for schedule(static)
+            for (long xy = 0; xy < (long)n * n; ++xy) {
+                const cd* src = tmp.data() + xy * nh;
+                for (int k = 0; k < nh; ++k) buf[k] = src[k];
+                for (int k = 1; k < n / 2; ++k) buf[n - k] = std::conj(src[k]);
+                f.exec(buf.data(), +1);
+                double* dst = out + xy * n;
+                for (int z = 0; z < n; ++z) dst[z] = buf[z].real();
+            }
+        }
+    }
+};
+
+inline double pw2(double x) { return x * x; }
+inline int signed_index(int i, int N) { return i <= N / 2 ? i : i - N; }
+
+// ---------------------------------------------------------------------------
+// Model hooks: V = m2 phi^2/2 + g2 phi^2 chi^2/2, in the shape of model.h
+// (potential_energy model.h:35-73 per term, dvdf :75-82, d2Vdf2 :86-93 with the
+// Hessian term index fld + fld2, effective_mass :96-98).
+struct Model {
+    int nflds;
+    double m2, g2;
+    int num_terms() const { return nflds == 2 ? 2 : 1; }
+    double V_term(int term, const double* f) const {
+        if (term == 0) return 0.5 * m2 * pw2(f[0]);
+        return 0.5 * g2 * pw2(f[0]) * pw2(f[1]);
+    }
+    double dvdf(int fld, const double* f) const {
+        if (nflds == 1) return m2 * f[0];
+        if (fld == 0) return m2 * f[0] + g2 * f[0] * pw2(f[1]);
+        return g2 * pw2(f[0]) * f[1];
+    }
+    double d2Vdf2(int term, const double* f) const {
+        if (nflds == 1) return m2;
+        if (term == 0) return m2 + g2 * pw2(f[1]);
+        if (term == 1) return 2.0 * g2 * f[0] * f[1];
+        return g2 * pw2(f[0]);
+    }
+    void effective_mass(double* m, const double* f) const {
+        if (nflds == 1) { m[0] = m2; return; }
+        m[0] = m2 + g2 * pw2(f[1]);
+        m[1] = g2 * pw2(f[0]);
+    }
+};
+
+// Integrator weights (pspectcosmo.h:283-288).
+void integrator_weights(const psc_config& c, std::vector<double>& w) {
+    if (c.integrator == PSC_Y4) {
+        if (c.yoshida_exact) {
+            double cr = std::cbrt(2.0);
+            double w1 = 1.0 / (2.0 - cr), w0 = -cr / (2.0 - cr);
+            w = {w1, w0, w1};
+        } else {
+            w = {1.351207191959657771818, -1.702414403875838200264, 1.351207191959657771818};
+        }
+    } else {
+        w = {1.0};
+    }
+}
+
+}  // namespace
+
+// ===========================================================================
+struct orc_ctx {
+    psc_config cfg;
+    Model model;
+    int N, Nh;
+    size_t nr, nk;
+    double N3;
+    Fft3D fft;
+    std::vector<double> w;
+    std::vector<cd> f_k[2], pi_k[2], pi_k_half[2], kf[2], Vp_k[2];
+    std::vector<double> f[2], pif[2], Vp[2];
+    std::vector<cd> tmp;
+    double t = 0, a = 1, ad = 0, add = 0;
+    // GW (6 symmetric components 11,22,33,12,13,23)
+    std::vector<cd> u_k[6], pi_u_k[6], pi_u_half[6], src_k[6];
+    std::vector<double> grad[2][3], src[6];
+
+    orc_ctx(const psc_config& c, const psc_model& m)
+        : cfg(c), N(c.N), Nh(c.N / 2 + 1), fft(c.N) {
+        model.nflds = c.nflds;
+        model.m2 = m.m2;
+        model.g2 = m.g2;
+        nr = (size_t)N * N * N;
+        nk = (size_t)N * N * Nh;
+        N3 = (double)N * N * N;
+        integrator_weights(cfg, w);
+        for (int fl = 0; fl < cfg.nflds; ++fl) {
+            f_k[fl].assign(nk, 0.0);
+            pi_k[fl].assign(nk, 0.0);
+            pi_k_half[fl].assign(nk, 0.0);
+            kf[fl].assign(nk, 0.0);
+            Vp_k[fl].assign(nk, 0.0);
+            f[fl].assign(nr, 0.0);
+            Vp[fl].assign(nr, 0.0);
+        }
+        if (cfg.with_gw) {
+            for (int c6 = 0; c6 < 6; ++c6) {
+                u_k[c6].assign(nk, 0.0);
+                pi_u_k[c6].assign(nk, 0.0);
+                pi_u_half[c6].assign(nk, 0.0);
+                src_k[c6].assign(nk, 0.0);
+                src[c6].assign(nr, 0.0);
+            }
+            for (int fl = 0; fl < cfg.nflds; ++fl)
+                for (int d = 0; d < 3; ++d) grad[fl][d].assign(nr, 0.0);
+        }
+    }
+
+    // c2r then /N^3 (evolution.cpp:1144-1151).
+    void to_real(const std::vector<cd>& in_k, std::vector<double>& out) {
+        out.resize(nr);
+        fft.c2r(in_k.data(), out.data(), tmp);
+        const double inv = N3;
+#pragma omp parallel for schedule(static)
+        for (long i = 0; i < (long)nr; ++i) out[i] /= inv;
+    }
+    void refresh_real_f() {
+        for (int fl = 0; fl < cfg.nflds; ++fl) to_real(f_k[fl], f[fl]);
+    }
+
+    // kinetic_energy_k (evolution.cpp:923-970, non-inflation branch :949-966).
+    double kinetic_energy_k(int fl) const {
+        double kinetic = 0.0;
+        const cd* p = pi_k[fl].data();
+        for (int k = 1; k < N / 2; ++k)
+            for (int i = 0; i < N; ++i)
+                for (int j = 0; j < N; ++j) {
+                    const cd& v = p[((size_t)i * N + j) * Nh + k];
+                    kinetic += pw2(v.real()) + pw2(v.imag());
+                }
+        kinetic *= 2.0;
+        for (int k = 0; k <= N / 2; k += N / 2)
+            for (int i = 0; i < N; ++i)
+                for (int j = 0; j < N; ++j) {
+                    const cd& v = p[((size_t)i * N + j) * Nh + k];
+                    kinetic += pw2(v.real()) + pw2(v.imag());
+                }
+        return 0.5 * kinetic / std::pow(a, 6) / pw2(N3);
+    }
+
+    // gradient_energy_k (evolution.cpp:972-1035, non-inflation branch :1006-1029).
+    double gradient_energy_k(int fl) const {
+        double grad = 0.0;
+        const cd* p = f_k[fl].data();
+        const double k2ir = std::pow(cfg.kIR, 2);
+        for (int k = 1; k < N / 2; ++k) {
+            int pz = k;
+            for (int i = 0; i < N; ++i) {
+                int px = signed_index(i, N);
+                for (int j = 0; j < N; ++j) {
+                    int py = signed_index(j, N);
+                    const cd& v = p[((size_t)i * N + j) * Nh + k];
+                    grad += k2ir * (px * px + py * py + pz * pz) * (pw2(v.real()) + pw2(v.imag()));
+                }
+            }
+        }
+        grad *= 2.0;
+        for (int k = 0; k <= N / 2; k += N / 2) {
+            int pz = k;
+            for (int i = 0; i < N; ++i) {
+                int px = signed_index(i, N);
+                for (int j = 0; j < N; ++j) {
+                    int py = signed_index(j, N);
+                    const cd& v = p[((size_t)i * N + j) * Nh + k];
+                    grad += k2ir * (px * px + py * py + pz * pz) * (pw2(v.real()) + pw2(v.imag()));
+                }
+            }
+        }
+        grad /= N3;
+        grad /= 2.0;
+        return grad / pw2(a) / N3;
+    }
+
+    // Lattice average of one potential term (model.h:35-73 with NULL values).
+    double potential_term_avg(int term) const {
+        double s = 0.0;
+        double fp[2] = {0, 0};
+        for (size_t i = 0; i < nr; ++i) {
+            for (int fl = 0; fl < cfg.nflds; ++fl) fp[fl] = f[fl][i];
+            s += model.V_term(term, fp);
+        }
+        return s / N3;
+    }
+
+    // kernel_a_k_nopad (evolution.cpp:1061-1090): d2a/dt2 from K, G, <V>.
+    // Real f must be current.
+    double kernel_a() const {
+        if (cfg.expansion == PSC_EXPANSION_STATIC) return 0.0;
+        double K = 0, G = 0, V = 0;
+        for (int fl = 0; fl < cfg.nflds; ++fl) {
+            K += kinetic_energy_k(fl);
+            G += gradient_energy_k(fl);
+        }
+        double fp[2] = {0, 0};
+        for (size_t i = 0; i < nr; ++i) {
+            for (int fl = 0; fl < cfg.nflds; ++fl) fp[fl] = f[fl][i];
+            for (int term = 0; term < model.num_terms(); ++term) V += model.V_term(term, fp);
+        }
+        V /= N3;
+        const int al = cfg.alpha;
+        return 1.0 / 3.0 * pw2(cfg.fStar / cfg.MPl) * std::pow(a, 1 + 2 * al) *
+               ((al - 2) * K + al * G + (al + 1) * V);
+    }
+
+    // kernel_field_k_nopad (evolution.cpp:1092-1129).  Real f must be current.
+    void kernel_field(std::vector<cd>* out) {
+        const double norm1 = std::pow(a, 1 + cfg.alpha);
+        const double norm2 = std::pow(a, 3 + cfg.alpha);
+        const int nf = cfg.nflds;
+#pragma omp parallel for schedule(static)
+        for (long i = 0; i < (long)nr; ++i) {
+            double fp[2] = {0, 0};
+            for (int fl = 0; fl < nf; ++fl) fp[fl] = f[fl][i];
+            for (int fl = 0; fl < nf; ++fl) Vp[fl][i] = model.dvdf(fl, fp);
+        }
+        const double k2ir = pw2(cfg.kIR);
+        for (int fl = 0; fl < nf; ++fl) {
+            fft.r2c(Vp[fl].data(), Vp_k[fl].data());
+            cd* o = out[fl].data();
+            const cd* fk = f_k[fl].data();
+            const cd* vk = Vp_k[fl].data();
+#pragma omp parallel for schedule(static)
+            for (int i = 0; i < N; ++i) {
+                int px = signed_index(i, N);
+                for (int j = 0; j < N; ++j) {
+                    int py = signed_index(j, N);
+                    for (int k = 0; k < Nh; ++k) {
+                        int k2 = px * px + py * py + k * k;
+                        size_t idx = ((size_t)i * N + j) * Nh + k;
+                        o[idx] = cd(-norm1 * k2ir * k2 * fk[idx].real() - norm2 * vk[idx].real(),
+                                    -norm1 * k2ir * k2 * fk[idx].imag() - norm2 * vk[idx].imag());
+                    }
+                }
+            }
+        }
+    }
+
+    // Spectral first derivative d/dx_d of a half spectrum with the Nyquist
+    // component zeroed, then c2r and /N^3 -- the 3D form of the 1D-per-axis
+    // transforms of field_1st_derivative (evolution.cpp:466-578: multiply by
+    // i kIR k for k < N/2, zero at N/2, /N).
+    void derivative(const std::vector<cd>& in_k, int d, std::vector<double>& out) {
+        std::vector<cd> dk(nk);
+        const double kir = cfg.kIR;
+#pragma omp parallel for schedule(static)
+        for (int i = 0; i < N; ++i)
+            for (int j = 0; j < N; ++j)
+                for (int k = 0; k < Nh; ++k) {
+                    int n = d == 0 ? i : (d == 1 ? j : k);
+                    int pn = (n == N / 2) ? 0 : signed_index(n, N);
+                    size_t idx = ((size_t)i * N + j) * Nh + k;
+                    dk[idx] = cd(0.0, kir * pn) * in_k[idx];
+                }
+        to_real(dk, out);
+    }
+
+    // GW source r2c(sum_f d_i f d_j f) for the 6 components, as used by
+    // kernel_GW (evolution.cpp:581-601) whose Laplacian part lapl()
+    // (evolution.cpp:343-372) becomes -k^2 u_k in k space.
+    void gw_source() {
+        static const int ci[6] = {0, 1, 2, 0, 0, 1}, cj[6] = {0, 1, 2, 1, 2, 2};
+        for (int fl = 0; fl < cfg.nflds; ++fl)
+            for (int d = 0; d < 3; ++d) derivative(f_k[fl], d, grad[fl][d]);
+        for (int c6 = 0; c6 < 6; ++c6) {
+#pragma omp parallel for schedule(static)
+            for (long x = 0; x < (long)nr; ++x) {
+                double s = 0.0;
+                for (int fl = 0; fl < cfg.nflds; ++fl) s += grad[fl][ci[c6]][x] * grad[fl][cj[c6]][x];
+                src[c6][x] = s;
+            }
+            fft.r2c(src[c6].data(), src_k[c6].data());
+        }
+    }
+    // pi_u_out = pi_u_in + c * a^{1+alpha} (-kIR^2 n^2 u_k + 2 S_k)
+    void gw_kick(std::vector<cd>* pin, std::vector<cd>* pout, double c) {
+        const double norm = std::pow(a, 1 + cfg.alpha), k2ir = pw2(cfg.kIR);
+        for (int c6 = 0; c6 < 6; ++c6) {
+#pragma omp parallel for schedule(static)
+            for (int i = 0; i < N; ++i) {
+                int px = signed_index(i, N);
+                for (int j = 0; j < N; ++j) {
+                    int py = signed_index(j, N);
+                    for (int k = 0; k < Nh; ++k) {
+                        size_t idx = ((size_t)i * N + j) * Nh + k;
+                        double k2 = k2ir * (px * px + py * py + k * k);
+                        cd kern = norm * (-k2 * u_k[c6][idx] + 2.0 * src_k[c6][idx]);
+                        pout[c6][idx] = pin[c6][idx] + c * kern;
+                    }
+                }
+            }
+        }
+    }
+
+    // evolve_VV_k_nopad (evolution.cpp:1131-1198), plus the GW kick/drift of
+    // the real-space stepper (evolution.cpp:633-685) carried in k space.
+    void step(double dt) {
+        double b = ad, b_half, a_half, norm;
+        const int nf = cfg.nflds;
+        const bool gw = cfg.with_gw != 0;
+        const bool stat = cfg.expansion == PSC_EXPANSION_STATIC;
+        for (size_t p = 0; p < w.size(); ++p) {
+            const double wp = w[p];
+            b_half = b + wp * dt / 2.0 * add;                       // step 1
+            refresh_real_f();                                        // step 2
+            kernel_field(kf);
+            for (int fl = 0; fl < nf; ++fl) {
+                cd* ph = pi_k_half[fl].data();
+                const cd* pk = pi_k[fl].data();
+                const cd* k1 = kf[fl].data();
+#pragma omp parallel for schedule(static)
+                for (long i = 0; i < (long)nk; ++i) ph[i] = pk[i] + wp * dt / 2 * k1[i];
+            }
+            if (gw) {
+                gw_source();
+                gw_kick(pi_u_k, pi_u_half, wp * dt / 2);
+            }
+            a_half = stat ? 1.0 : a + b_half * wp * dt / 2.0;         // step 3
+            norm = std::pow(a_half, cfg.alpha - 3);                  // step 4
+            for (int fl = 0; fl < nf; ++fl) {
+                cd* fk = f_k[fl].data();
+                const cd* ph = pi_k_half[fl].data();
+#pragma omp parallel for schedule(static)
+                for (long i = 0; i < (long)nk; ++i) fk[i] += wp * dt * norm * ph[i];
+                to_real(f_k[fl], f[fl]);
+            }
+            if (gw) {
+                for (int c6 = 0; c6 < 6; ++c6) {
+                    cd* uk = u_k[c6].data();
+                    const cd* ph = pi_u_half[c6].data();
+#pragma omp parallel for schedule(static)
+                    for (long i = 0; i < (long)nk; ++i) uk[i] += wp * dt * norm * ph[i];
+                }
+            }
+            a = stat ? 1.0 : a_half + b_half * wp * dt / 2.0;         // step 5
+            kernel_field(kf);                                        // step 6
+            for (int fl = 0; fl < nf; ++fl) {
+                cd* pk = pi_k[fl].data();
+                const cd* ph = pi_k_half[fl].data();
+                const cd* k2 = kf[fl].data();
+#pragma omp parallel for schedule(static)
+                for (long i = 0; i < (long)nk; ++i) pk[i] = ph[i] + wp * dt / 2 * k2[i];
+            }
+            if (gw) {
+                gw_source();
+                gw_kick(pi_u_half, pi_u_k, wp * dt / 2);
+            }
+            add = kernel_a();                                        // step 7
+            b = b_half + wp * dt / 2.0 * add;
+        }
+        ad = stat ? 0.0 : b;
+        t += dt;
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Initial conditions (initialize.cpp:562-810), non-inflation Rayleigh branch.
+namespace {
+
+struct IcState {
+    const psc_config* cfg;
+    const Model* model;
+    ParkMiller* rng;
+    double a, hubble, norm;
+};
+
+// set_mode (initialize.cpp:128-220), SIMULATE_INFLATION == false branch.
+// norm = (omegaStar / fStar) (N / dx)^{3/2} / sqrt(2): the continuum vacuum
+// normalisation of the commented DeltaF_DeltaPi variant (parameters.h:132),
+// the same N^3 / L^{3/2} scaling as initialize.cpp:260-263.
+void set_mode(IcState& s, double p2, double m2, cd* fk, cd* pk, bool isReal) {
+    double omega;
+    if (p2 + m2 > 0.0)
+        omega = std::sqrt(p2 + pw2(s.a) * m2);
+    else
+        omega = std::sqrt(p2);
+    double rms = omega > 0.0 ? s.norm / std::sqrt(omega) : 0.0;
+    double amp_l = rms / std::sqrt(2.0) * std::sqrt(std::log(1.0 / s.rng->uniform())) / s.a;
+    double amp_r = rms / std::sqrt(2.0) * std::sqrt(std::log(1.0 / s.rng->uniform())) / s.a;
+    // g++ -O3 turns the reference's cos/sin pair into one sincos call.
+    double sn, cs;
+    ::sincos(2.0 * kPi * s.rng->uniform(), &sn, &cs);
+    double re_l = amp_l * cs, im_l = amp_l * sn;
+    ::sincos(2.0 * kPi * s.rng->uniform(), &sn, &cs);
+    double re_r = amp_r * cs, im_r = amp_r * sn;
+    double fre = re_l + re_r, fim = im_l + im_r;
+    const double ap = std::pow(s.a, s.cfg->alpha - 3);
+    double pre = ap * (-omega * (im_l - im_r) - s.hubble * fre);
+    double pim = ap * (omega * (re_l - re_r) - s.hubble * fim);
+    if (isReal) {
+        fim = 0.0;
+        pim = 0.0;
+    }
+    *fk = cd(fre, fim);
+    *pk = cd(pre, pim);
+}
+
+// Mode visiting order of initialize.cpp:685-749 (and initialize_GW,
+// initialize.cpp:355-428): callback(i, j, k, conj_index_or_minus1, isReal).
+template <class F>
+void visit_modes(int N, bool include_dc, F&& fn) {
+    const int Nh = N / 2 + 1;
+    auto idx = [&](int i, int j, int k) { return ((long)i * N + j) * Nh + k; };
+    for (int k = 0; k <= N / 2; k += N / 2) {
+        for (int i = 0; i <= N / 2; i += N / 2)
+            for (int j = 1; j < N / 2; j++) fn(i, j, k, idx(i, N - j, k), false);
+        for (int j = 0; j <= N / 2; j += N / 2)
+            for (int i = 1; i < N / 2; i++) fn(i, j, k, idx(N - i, j, k), false);
+        for (int i = 1; i < N / 2; i++) {
+            for (int j = 1; j < N / 2; j++) fn(i, j, k, idx(N - i, N - j, k), false);
+            for (int j = N / 2 + 1; j < N; j++) fn(i, j, k, idx(N - i, N - j, k), false);
+        }
+    }
+    for (int k = 1; k < N / 2; k++)
+        for (int i = 0; i < N; i++)
+            for (int j = 0; j < N; j++) fn(i, j, k, -1L, false);
+    for (int k = 0; k <= N / 2; k += N / 2)
+        for (int i = 0; i <= N / 2; i += N / 2)
+            for (int j = 0; j <= N / 2; j += N / 2) {
+                if (!include_dc && i == 0 && j == 0 && k == 0) continue;
+                fn(i, j, k, -1L, true);
+            }
+}
+
+// Hubble initialisation (initialize.cpp:624-668).
+void hubble_init(const psc_config& c, const Model& m, const psc_ic_params& ic, double* a,
+                 double* ad, double* add_print, double* H) {
+    double f0[2] = {ic.f0[0], c.nflds > 1 ? ic.f0[1] : 0.0};
+    double fdsq = 0.0, pot = 0.0;
+    for (int fl = 0; fl < c.nflds; ++fl) fdsq += pw2(ic.fd0[fl]);
+    for (int term = 0; term < m.num_terms(); ++term) pot += m.V_term(term, f0);
+    double h = fdsq / 2.0 + pot;
+    h *= pw2(c.fStar / c.MPl) / 3.0;
+    h = std::sqrt(h);
+    *H = h;
+    *a = ic.a0;
+    *ad = h * ic.a0;
+    *add_print = pw2(c.fStar / c.MPl) / 3.0 * ((c.alpha - 2) * fdsq + (c.alpha + 1) * pot) * ic.a0;
+    if (c.expansion == PSC_EXPANSION_STATIC) {
+        *H = 0.0;
+        *ad = 0.0;
+        *add_print = 0.0;
+    }
+}
+
+// orthogonal_unit_vectors (initialize.cpp:66-105) and TT_tensors (:109-122):
+// e = u u - v v + u v + v u.
+void tt_tensor(const double k[3], double e[3][3]) {
+    double kn = std::sqrt(k[0] * k[0] + k[1] * k[1] + k[2] * k[2]);
+    double kh[3] = {k[0] / kn, k[1] / kn, k[2] / kn};
+    double tmp[3];
+    bool along_z = std::fabs(kh[0]) < 1e-9 && std::fabs(kh[1]) < 1e-9 && std::fabs(kh[2] - 1) < 1e-9;
+    if (along_z) { tmp[0] = 0; tmp[1] = 1; tmp[2] = 0; }
+    else { tmp[0] = 0; tmp[1] = 0; tmp[2] = 1; }
+    auto cross = [](const double* x, const double* y, double* o) {
+        o[0] = x[1] * y[2] - x[2] * y[1];
+        o[1] = x[2] * y[0] - x[0] * y[2];
+        o[2] = x[0] * y[1] - x[1] * y[0];
+    };
+    auto normalize = [](double* v) {
+        double n = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+        v[0] /= n; v[1] /= n; v[2] /= n;
+    };
+    double u[3], v[3];
+    cross(kh, tmp, u);
+    normalize(u);
+    cross(kh, u, v);
+    normalize(v);
+    for (int i = 0; i < 3; ++i)
+        if (u[i] < 0) { for (int c = 0; c < 3; ++c) u[c] = -u[c]; break; }
+    for (int i = 0; i < 3; ++i)
+        if (v[i] < 0) { for (int c = 0; c < 3; ++c) v[c] = -v[c]; break; }
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) e[i][j] = u[i] * u[j] - v[i] * v[j] + u[i] * v[j] + v[i] * u[j];
+}
+
+const int kGwI[6] = {0, 1, 2, 0, 0, 1};
+const int kGwJ[6] = {0, 1, 2, 1, 2, 2};
+
+}  // namespace
+
+// ===========================================================================
+extern "C" {
+
+int orc_abi_version(void) { return PSC_ABI_VERSION; }
+
+int orc_set_threads(int n) {
+    if (n > 0) omp_set_num_threads(n);
+    return omp_get_max_threads();
+}
+
+// Raw Park-Miller state after `steps` iterations from `state` (textbook KAT).
+long long orc_park_miller_iterate(long long state, long long steps) {
+    ParkMiller pm(1);
+    pm.next = (int)state;
+    for (long long s = 0; s < steps; ++s) pm.advance();
+    return pm.next;
+}
+
+// First n values of the reference rand_uniform stream for `seed`.
+int orc_rng_draws(int seed, int n, double* out) {
+    ParkMiller pm(seed);
+    for (int i = 0; i < n; ++i) out[i] = pm.uniform();
+    return 0;
+}
+
+int orc_fft_r2c_3d(int N, const double* in, double* out) {
+    Fft3D f(N);
+    f.r2c(in, reinterpret_cast<cd*>(out));
+    return 0;
+}
+int orc_fft_c2r_3d(int N, const double* in, double* out) {
+    Fft3D f(N);
+    std::vector<cd> tmp;
+    f.c2r(reinterpret_cast<const cd*>(in), out, tmp);
+    return 0;
+}
+int orc_fft_1d(int n, double* data, int sign) {
+    Fft1D f(n);
+    f.exec(reinterpret_cast<cd*>(data), sign);
+    return 0;
+}
+
+static int check_cfg(const psc_config* c) {
+    if (!c || c->N < 4 || (c->N & (c->N - 1)) || c->nflds < 1 || c->nflds > 2) return PSC_ERR_ARG;
+    return 0;
+}
+
+// initialize() restated; scalars_out = a, ad, add (printed), H, add (kernel_a
+// on the generated state, initialize.cpp:797).
+int orc_generate_ic(const psc_config* cfg, const psc_model* model, const psc_ic_params* ic,
+                    double* f_k_out, double* pi_k_out, double* scalars_out) {
+    if (check_cfg(cfg)) return PSC_ERR_ARG;
+    orc_ctx ctx(*cfg, *model);
+    const int N = cfg->N;
+    const double L = 2.0 * kPi / cfg->kIR, dx = L / N;
+    double a, ad, addp, H;
+    hubble_init(*cfg, ctx.model, *ic, &a, &ad, &addp, &H);
+    ParkMiller rng(ic->seed);
+    IcState s{cfg, &ctx.model, &rng, a, H,
+              cfg->omegaStar / cfg->fStar * std::pow(N / dx, 1.5) / std::sqrt(2.0)};
+    double f0[2] = {ic->f0[0], cfg->nflds > 1 ? ic->f0[1] : 0.0};
+    double mass_sq[2];
+    ctx.model.effective_mass(mass_sq, f0);
+    const double dp2 = pw2(cfg->kIR);
+    const double N3 = (double)N * N * N;
+    for (int fl = 0; fl < cfg->nflds; ++fl) {
+        cd* fk = ctx.f_k[fl].data();
+        cd* pk = ctx.pi_k[fl].data();
+        visit_modes(N, true, [&](int i, int j, int k, long conj, bool isReal) {
+            int px = signed_index(i, N), py = signed_index(j, N);
+            // initialize.cpp uses i, j (not the signed indices) in the k = 0,
+            // N/2 planes where both are <= N/2 except for the j > N/2 loop,
+            // which uses j - N; signed_index reproduces each case.
+            double p2 = dp2 * (pw2(px) + pw2(py) + pw2(k));
+            long idx = ((long)i * N + j) * (N / 2 + 1) + k;
+            set_mode(s, p2, mass_sq[fl], fk + idx, pk + idx, isReal);
+            if (conj >= 0) {  // set_conjugate_mode (initialize.cpp:475-480)
+                fk[conj] = std::conj(fk[idx]);
+                pk[conj] = std::conj(pk[idx]);
+            }
+        });
+        fk[0] = cd(ic->f0[fl] * N3, 0.0);                                   // :752-753
+        pk[0] = cd(std::pow(a, cfg->alpha - 3) * ic->fd0[fl] * N3, 0.0);    // :755-756
+        std::memcpy(f_k_out + (size_t)fl * ctx.nk * 2, fk, ctx.nk * sizeof(cd));
+        std::memcpy(pi_k_out + (size_t)fl * ctx.nk * 2, pk, ctx.nk * sizeof(cd));
+    }
+    ctx.a = a;
+    ctx.refresh_real_f();
+    double add_kernel = ctx.kernel_a();
+    if (scalars_out) {
+        scalars_out[0] = a;
+        scalars_out[1] = ad;
+        scalars_out[2] = addp;
+        scalars_out[3] = H;
+        scalars_out[4] = add_kernel;
+    }
+    return 0;
+}
+
+// initialize_GW (initialize.cpp:341-470) for the 6 symmetric components.  The
+// RNG stream first replays the scalar-field draws of orc_generate_ic.
+int orc_generate_gw_ic(const psc_config* cfg, const psc_model* model, const psc_ic_params* ic,
+                       double* u_out, double* pi_u_out) {
+    if (check_cfg(cfg)) return PSC_ERR_ARG;
+    const int N = cfg->N, Nh = N / 2 + 1;
+    const size_t nk = (size_t)N * N * Nh;
+    const double L = 2.0 * kPi / cfg->kIR, dx = L / N;
+    Model m{cfg->nflds, model->m2, model->g2};
+    double a, ad, addp, H;
+    hubble_init(*cfg, m, *ic, &a, &ad, &addp, &H);
+    ParkMiller rng(ic->seed);
+    // 4 draws per visited scalar mode, DC corner included (initialize.cpp:181-190).
+    long scalar_modes = 0;
+    visit_modes(N, true, [&](int, int, int, long, bool) { ++scalar_modes; });
+    for (long d = 0; d < 4 * scalar_modes * cfg->nflds; ++d) rng.uniform();
+    const double norm = cfg->MPl * cfg->omegaStar / pw2(cfg->fStar) * std::pow(N / dx, 1.5);
+    cd* u = reinterpret_cast<cd*>(u_out);
+    cd* pu = reinterpret_cast<cd*>(pi_u_out);
+    std::memset(u_out, 0, 6 * nk * sizeof(cd));
+    std::memset(pi_u_out, 0, 6 * nk * sizeof(cd));
+    const double ap = std::pow(a, cfg->alpha - 3);
+    visit_modes(N, false, [&](int i, int j, int k, long conj, bool isReal) {
+        int px = signed_index(i, N), py = signed_index(j, N);
+        double kv[3] = {px * cfg->kIR, py * cfg->kIR, k * cfg->kIR};
+        double k2 = pw2(cfg->kIR) * (pw2(px) + pw2(py) + pw2(k));
+        double e[3][3];
+        tt_tensor(kv, e);
+        double kk = std::sqrt(k2);
+        double rms = norm / std::sqrt(kk);
+        double amp_l = rms / std::sqrt(2.0) * std::sqrt(std::log(1.0 / rng.uniform())) / a;
+        double amp_r = rms / std::sqrt(2.0) * std::sqrt(std::log(1.0 / rng.uniform())) / a;
+        double sn, cs;
+        ::sincos(2.0 * kPi * rng.uniform(), &sn, &cs);
+        double re_l = amp_l * cs, im_l = amp_l * sn;
+        ::sincos(2.0 * kPi * rng.uniform(), &sn, &cs);
+        double re_r = amp_r * cs, im_r = amp_r * sn;
+        long idx = ((long)i * N + j) * Nh + k;
+        for (int c = 0; c < 6; ++c) {
+            double ec = e[kGwI[c]][kGwJ[c]];
+            double ure = ec * (re_l + re_r), uim = ec * (im_l + im_r);
+            // set_mode_GW multiplies the Hubble term by e_ij once more
+            // (initialize.cpp:317-318): pi = a^{alpha-3} e (-k(...) - H u).
+            double pre = ap * (ec * (-kk * (im_l - im_r) - H * ure));
+            double pim = ap * (ec * (kk * (re_l - re_r) - H * uim));
+            if (isReal) { uim = 0; pim = 0; }
+            u[c * nk + idx] = cd(ure, uim);
+            pu[c * nk + idx] = cd(pre, pim);
+            if (conj >= 0) {
+                u[c * nk + conj] = cd(ure, -uim);
+                pu[c * nk + conj] = cd(pre, -pim);
+            }
+        }
+    });
+    return 0;
+}
+
+orc_ctx* orc_create(const psc_config* cfg, const psc_model* model) {
+    if (check_cfg(cfg)) return nullptr;
+    return new orc_ctx(*cfg, *model);
+}
+void orc_destroy(orc_ctx* c) { delete c; }
+
+int orc_load(orc_ctx* c, const double* f_k, const double* pi_k, double t, double a, double ad,
+             double add) {
+    for (int fl = 0; fl < c->cfg.nflds; ++fl) {
+        std::memcpy(c->f_k[fl].data(), f_k + (size_t)fl * c->nk * 2, c->nk * sizeof(cd));
+        std::memcpy(c->pi_k[fl].data(), pi_k + (size_t)fl * c->nk * 2, c->nk * sizeof(cd));
+    }
+    c->t = t;
+    c->a = a;
+    c->ad = ad;
+    c->add = add;
+    c->refresh_real_f();
+    return 0;
+}
+int orc_load_gw(orc_ctx* c, const double* u_k, const double* pi_u_k) {
+    if (!c->cfg.with_gw) return PSC_ERR_STATE;
+    for (int c6 = 0; c6 < 6; ++c6) {
+        std::memcpy(c->u_k[c6].data(), u_k + (size_t)c6 * c->nk * 2, c->nk * sizeof(cd));
+        std::memcpy(c->pi_u_k[c6].data(), pi_u_k + (size_t)c6 * c->nk * 2, c->nk * sizeof(cd));
+    }
+    return 0;
+}
+int orc_get_gw(orc_ctx* c, double* u_k, double* pi_u_k) {
+    if (!c->cfg.with_gw) return PSC_ERR_STATE;
+    for (int c6 = 0; c6 < 6; ++c6) {
+        if (u_k) std::memcpy(u_k + (size_t)c6 * c->nk * 2, c->u_k[c6].data(), c->nk * sizeof(cd));
+        if (pi_u_k) std::memcpy(pi_u_k + (size_t)c6 * c->nk * 2, c->pi_u_k[c6].data(), c->nk * sizeof(cd));
+    }
+    return 0;
+}
+
+// kernel_a on the current state (real f refreshed first).
+double orc_kernel_a(orc_ctx* c) {
+    c->refresh_real_f();
+    return c->kernel_a();
+}
+
+int orc_step(orc_ctx* c, int nsteps) {
+    for (int s = 0; s < nsteps; ++s) c->step(c->cfg.dt);
+    return 0;
+}
+
+int orc_get_k(orc_ctx* c, double* f_k, double* pi_k) {
+    for (int fl = 0; fl < c->cfg.nflds; ++fl) {
+        if (f_k) std::memcpy(f_k + (size_t)fl * c->nk * 2, c->f_k[fl].data(), c->nk * sizeof(cd));
+        if (pi_k) std::memcpy(pi_k + (size_t)fl * c->nk * 2, c->pi_k[fl].data(), c->nk * sizeof(cd));
+    }
+    return 0;
+}
+int orc_get_scalars(orc_ctx* c, double* out) {
+    out[0] = c->t;
+    out[1] = c->a;
+    out[2] = c->ad;
+    out[3] = c->add;
+    return 0;
+}
+int orc_get_fields(orc_ctx* c, double* f, double* pi) {
+    std::vector<double> r;
+    for (int fl = 0; fl < c->cfg.nflds; ++fl) {
+        if (f) {
+            c->to_real(c->f_k[fl], r);
+            std::memcpy(f + (size_t)fl * c->nr, r.data(), c->nr * sizeof(double));
+        }
+        if (pi) {
+            c->to_real(c->pi_k[fl], r);
+            std::memcpy(pi + (size_t)fl * c->nr, r.data(), c->nr * sizeof(double));
+        }
+    }
+    return 0;
+}
+
+// Real-space spectral derivative of field fl along axis d (test hook).
+int orc_field_derivative(orc_ctx* c, int fl, int d, double* out) {
+    std::vector<double> r;
+    c->derivative(c->f_k[fl], d, r);
+    std::memcpy(out, r.data(), c->nr * sizeof(double));
+    return 0;
+}
+
+// energy_density + energy + meansvars (output.cpp:57-115, 380-461, 1069-1187),
+// laid out as psc_energies documents.
+int orc_energies(orc_ctx* c, double* out) {
+    for (int i = 0; i < PSC_ENERGY_LEN; ++i) out[i] = 0.0;
+    c->refresh_real_f();
+    double total = 0.0;
+    for (int fl = 0; fl < c->cfg.nflds; ++fl) {
+        double K = c->kinetic_energy_k(fl), G = c->gradient_energy_k(fl);
+        out[2 * fl] = K;
+        out[2 * fl + 1] = G;
+        total += K + G;
+    }
+    for (int term = 0; term < c->model.num_terms(); ++term) {
+        double V = c->potential_term_avg(term);
+        out[4 + term] = V;
+        total += V;
+    }
+    out[6] = total;
+    const int al = c->cfg.alpha;
+    double LHS = pw2(c->ad);
+    double RHS = pw2(c->cfg.fStar / c->cfg.MPl) * std::pow(c->a, 2 * (al + 1)) * total / 3.0;
+    out[7] = std::fabs(LHS - RHS) / std::fabs(LHS + RHS);
+    out[8] = c->a;
+    out[9] = c->ad;
+    out[10] = c->add;
+    out[11] = c->t;
+    for (int fl = 0; fl < c->cfg.nflds; ++fl) {
+        double s = 0, s2 = 0;
+        for (size_t i = 0; i < c->nr; ++i) {
+            s += c->f[fl][i];
+            s2 += pw2(c->f[fl][i]);
+        }
+        s /= c->N3;
+        s2 /= c->N3;
+        out[12 + fl] = s;
+        out[14 + fl] = s2 - s * s > 0 ? s2 - s * s : 0.0;
+    }
+    return 0;
+}
+
+int orc_num_bins(int N) { return (int)(std::sqrt(3.0) * (N / 2)) + 2 - 1; }
+
+// Scalar spectra of field fl (output.cpp:626-705).  out: rows (bins 1..nb) of
+// p, S_f, S_pi, n_k, numpoints.
+int orc_spectra(orc_ctx* c, int fl, double* out, int nb) {
+    const int N = c->N, Nh = c->Nh;
+    const int maxnumbins = (int)(std::sqrt(3.0) * (N / 2)) + 2;
+    if (nb != maxnumbins - 1) return PSC_ERR_ARG;
+    c->refresh_real_f();
+    double mass_sq_av[2] = {0, 0};
+    double fp[2] = {0, 0}, ms[2];
+    for (size_t i = 0; i < c->nr; ++i) {
+        for (int q = 0; q < c->cfg.nflds; ++q) fp[q] = c->f[q][i];
+        c->model.effective_mass(ms, fp);
+        for (int q = 0; q < c->cfg.nflds; ++q) mass_sq_av[q] += ms[q];
+    }
+    for (int q = 0; q < c->cfg.nflds; ++q) mass_sq_av[q] /= c->N3;
+    std::vector<long> numpoints(maxnumbins, 0);
+    std::vector<double> f2(maxnumbins, 0), pi2(maxnumbins, 0), n2(maxnumbins, 0);
+    const double a = c->a, ad = c->ad;
+    const double aop = std::pow(a, c->cfg.alpha - 3);
+    const cd* fk = c->f_k[fl].data();
+    const cd* pk = c->pi_k[fl].data();
+    auto add_mode = [&](int i, int j, int k, int wgt) {
+        int px = signed_index(i, N), py = signed_index(j, N), pz = k;
+        size_t l = (size_t)std::floor(std::sqrt((double)(px * px + py * py + pz * pz)) + 0.5);
+        size_t idx = ((size_t)i * N + j) * Nh + k;
+        double fp2 = pw2(fk[idx].real()) + pw2(fk[idx].imag());
+        double pp2 = pw2(pk[idx].real()) + pw2(pk[idx].imag());
+        double np2 = pw2(aop * pk[idx].real() + ad / a * fk[idx].real()) +
+                     pw2(aop * pk[idx].imag() + ad / a * fk[idx].imag());
+        numpoints[l] += wgt;
+        f2[l] += wgt * fp2;
+        pi2[l] += wgt * pp2;
+        n2[l] += wgt * np2;
+    };
+    for (int i = 0; i < N; i++)
+        for (int j = 0; j < N; j++) {
+            for (int k = 1; k < N / 2; k++) add_mode(i, j, k, 2);
+            for (int k = 0; k <= N / 2; k += N / 2) add_mode(i, j, k, 1);
+        }
+    const double L = 2.0 * kPi / c->cfg.kIR;
+    for (int i = 1; i < maxnumbins; i++) {
+        double p = c->cfg.kIR * i;
+        double fa = f2[i], pa = pi2[i], na = n2[i];
+        if (numpoints[i] > 0) {
+            fa /= numpoints[i];
+            pa /= numpoints[i];
+            na /= numpoints[i];
+        }
+        double omegasq = std::fabs(pw2(p) + pw2(a) * mass_sq_av[fl]);
+        double nd = 0.0;
+        if (omegasq > 0.0) {
+            double omega = std::sqrt(omegasq);
+            nd = pw2(a) * std::pow(L, 3) / 2.0 / pw2(c->N3) * pw2(c->cfg.fStar / c->cfg.omegaStar) *
+                 (omega * fa + std::pow(a, 2 * (1 - c->cfg.alpha)) / omega * na);
+        }
+        double* row = out + (size_t)(i - 1) * PSC_SPECTRA_COLS;
+        row[0] = p;
+        row[1] = i / pw2(c->N3) * numpoints[i] * fa;
+        row[2] = i / pw2(c->N3) * numpoints[i] * pw2(aop) * pa;
+        row[3] = nd;
+        row[4] = (double)numpoints[i];
+    }
+    return 0;
+}
+
+// GW spectrum (output.cpp:780-1062).  projection PSC_GW_PROJ_REFERENCE sums
+// |P u|^2 - |tr P u|^2 / 2 with the one-sided P u (output.cpp:819-831);
+// PSC_GW_PROJ_TT sums |Lambda u|^2.  The k = 0 / N/2 planes follow the
+// reference's half-plane visiting (weight 2) since the one-sided projection of
+// a Nyquist-carrying mode differs from that of its stored conjugate index.
+int orc_gw_spectrum(orc_ctx* c, int projection, double* out, int nb, double* totals) {
+    if (!c->cfg.with_gw) return PSC_ERR_STATE;
+    const int N = c->N, Nh = c->Nh;
+    const int maxnumbins = (int)(std::sqrt(3.0) * (N / 2)) + 2;
+    if (nb != maxnumbins - 1) return PSC_ERR_ARG;
+    std::vector<long> numpoints(maxnumbins, 0);
+    std::vector<double> h2(maxnumbins, 0), hd2(maxnumbins, 0);
+    const double kir = c->cfg.kIR;
+    auto proj_norm = [&](const cd m[3][3], const double P[3][3]) {
+        if (projection == PSC_GW_PROJ_TT) {
+            cd A[3][3], Pu[3][3];
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 3; ++j) {
+                    Pu[i][j] = 0;
+                    for (int q = 0; q < 3; ++q) Pu[i][j] += P[i][q] * m[q][j];
+                }
+            cd tr = Pu[0][0] + Pu[1][1] + Pu[2][2];
+            double s = 0;
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 3; ++j) {
+                    A[i][j] = 0;
+                    for (int q = 0; q < 3; ++q) A[i][j] += Pu[i][q] * P[q][j];
+                    A[i][j] -= 0.5 * P[i][j] * tr;
+                    s += std::norm(A[i][j]);
+                }
+            return s;
+        }
+        cd v[3][3];
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) {
+                v[i][j] = 0;
+                for (int q = 0; q < 3; ++q) v[i][j] += P[i][q] * m[q][j];
+            }
+        double s = 0;
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) s += std::norm(v[i][j]);
+        return s - std::norm(v[0][0] + v[1][1] + v[2][2]) / 2.0;
+    };
+    auto add_mode = [&](int i, int j, int k, int wgt) {
+        int px = signed_index(i, N), py = signed_index(j, N);
+        double kv[3] = {px * kir, py * kir, k * kir};
+        double kn = std::sqrt(kv[0] * kv[0] + kv[1] * kv[1] + kv[2] * kv[2]);
+        double P[3][3];
+        for (int x = 0; x < 3; ++x)
+            for (int y = 0; y < 3; ++y) P[x][y] = (x == y ? 1.0 : 0.0) - kv[x] / kn * kv[y] / kn;
+        size_t idx = ((size_t)i * N + j) * Nh + k;
+        cd U[3][3], PU[3][3];
+        for (int c6 = 0; c6 < 6; ++c6) {
+            U[kGwI[c6]][kGwJ[c6]] = U[kGwJ[c6]][kGwI[c6]] = c->u_k[c6][idx];
+            PU[kGwI[c6]][kGwJ[c6]] = PU[kGwJ[c6]][kGwI[c6]] = c->pi_u_k[c6][idx];
+        }
+        size_t l = (size_t)std::floor(std::sqrt((double)(px * px + py * py + k * k)) + 0.5);
+        numpoints[l] += wgt;
+        h2[l] += wgt * proj_norm(U, P);
+        hd2[l] += wgt * proj_norm(PU, P);
+    };
+    visit_modes(N, false, [&](int i, int j, int k, long conj, bool isReal) {
+        add_mode(i, j, k, isReal ? 1 : 2);
+    });
+    double energy[PSC_ENERGY_LEN];
+    orc_energies(c, energy);
+    const double total = energy[6];
+    const double a = c->a, ad = c->ad;
+    const double fS = c->cfg.fStar, wS = c->cfg.omegaStar, MP = c->cfg.MPl;
+    double hd0 = 0.0;
+    for (int c6 = 0; c6 < 6; ++c6) hd0 += (c6 < 3 ? 1.0 : 2.0) * std::norm(c->pi_u_k[c6][0]);
+    double rho = hd0;
+    const double N3 = c->N3;
+    for (int i = 1; i < maxnumbins; ++i) {
+        rho += hd2[i];
+        double h = h2[i], hd = hd2[i];
+        if (numpoints[i] > 0) {
+            h /= numpoints[i];
+            hd /= numpoints[i];
+        }
+        double* row = out + (size_t)(i - 1) * PSC_GW_SPECTRA_COLS;
+        row[0] = kir * i;
+        row[1] = i / pw2(N3) * numpoints[i] * h;
+        row[2] = i / pw2(N3) / 4.0 / std::pow(a, 6) * pw2(fS / MP) / total * numpoints[i] * hd;
+        row[3] = pw2(MP * wS) / std::pow(fS, 4) * 2 / pw2(kPi) * pw2(ad) / std::pow(a, 2 * c->cfg.alpha + 2);
+        row[4] = (double)numpoints[i];
+    }
+    rho *= pw2(wS) * std::pow(fS, 4) / pw2(MP) / 4.0 / pw2(N3) / std::pow(a, 6);
+    rho /= pw2(fS * wS);
+    if (totals) {
+        totals[0] = rho / total;
+        totals[1] = rho;
+    }
+    return 0;
+}
+
+}  // extern "C"

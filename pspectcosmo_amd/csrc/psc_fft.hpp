@@ -1,0 +1,251 @@
+// psc_fft.hpp -- device-side Stockham FFT building blocks for gfx950.
+//
+// A line of N complex points is transformed cooperatively by TPL threads.
+// Thread t keeps elements x[t + TPL*m] (m < N/TPL) in registers before the
+// first stage and after the last stage, so global loads/stores, and the fused
+// pointwise work around a transform, never touch LDS.  Between stages the line
+// goes through LDS (in place, two barriers per stage).  Stage s with radix R
+// and Ns = product of earlier radices follows the Stockham autosort recursion:
+//   v[r] = x[j + r N/R] * W_{Ns R}^{r (j mod Ns)}, v = DFT_R(v),
+//   x'[(j / Ns) Ns R + (j mod Ns) + r Ns] = v[r],
+// for butterflies j < N/R; the result is in natural order.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace psc {
+
+template <typename T>
+struct alignas(2 * sizeof(T)) cpx {
+    T x, y;
+};
+
+template <typename T>
+__device__ __forceinline__ cpx<T> cadd(cpx<T> a, cpx<T> b) { return {a.x + b.x, a.y + b.y}; }
+template <typename T>
+__device__ __forceinline__ cpx<T> csub(cpx<T> a, cpx<T> b) { return {a.x - b.x, a.y - b.y}; }
+template <typename T>
+__device__ __forceinline__ cpx<T> cmul(cpx<T> a, cpx<T> b) {
+    return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+// a * conj(b)
+template <typename T>
+__device__ __forceinline__ cpx<T> cmulc(cpx<T> a, cpx<T> b) {
+    return {a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y};
+}
+
+// cos / sin of 2 pi m / 64, m < 64 (constant-folded after unrolling).
+struct Unit64 {
+    static constexpr double c[64] = {
+        1.0, 0.99518472667219688624, 0.98078528040323044913, 0.95694033573220886494,
+        0.92387953251128675613, 0.88192126434835502971, 0.83146961230254523708, 0.77301045336273696081,
+        0.70710678118654752440, 0.63439328416364549822, 0.55557023301960222474, 0.47139673682599764856,
+        0.38268343236508977173, 0.29028467725446236764, 0.19509032201612826785, 0.098017140329560601994,
+        0.0, -0.098017140329560601994, -0.19509032201612826785, -0.29028467725446236764,
+        -0.38268343236508977173, -0.47139673682599764856, -0.55557023301960222474, -0.63439328416364549822,
+        -0.70710678118654752440, -0.77301045336273696081, -0.83146961230254523708, -0.88192126434835502971,
+        -0.92387953251128675613, -0.95694033573220886494, -0.98078528040323044913, -0.99518472667219688624,
+        -1.0, -0.99518472667219688624, -0.98078528040323044913, -0.95694033573220886494,
+        -0.92387953251128675613, -0.88192126434835502971, -0.83146961230254523708, -0.77301045336273696081,
+        -0.70710678118654752440, -0.63439328416364549822, -0.55557023301960222474, -0.47139673682599764856,
+        -0.38268343236508977173, -0.29028467725446236764, -0.19509032201612826785, -0.098017140329560601994,
+        0.0, 0.098017140329560601994, 0.19509032201612826785, 0.29028467725446236764,
+        0.38268343236508977173, 0.47139673682599764856, 0.55557023301960222474, 0.63439328416364549822,
+        0.70710678118654752440, 0.77301045336273696081, 0.83146961230254523708, 0.88192126434835502971,
+        0.92387953251128675613, 0.95694033573220886494, 0.98078528040323044913, 0.99518472667219688624};
+    static constexpr double cosv(int m) { return c[m & 63]; }
+    static constexpr double sinv(int m) { return c[(m + 48) & 63]; }  // sin x = cos(x - pi/2)
+};
+
+// v * exp(SIGN 2 pi i m / 64) for a compile-time m.
+template <int SIGN, int M, typename T>
+__device__ __forceinline__ cpx<T> rot64(cpx<T> v) {
+    if constexpr (M == 0) {
+        return v;
+    } else if constexpr (M == 16) {
+        return SIGN > 0 ? cpx<T>{-v.y, v.x} : cpx<T>{v.y, -v.x};
+    } else {
+        constexpr T c = (T)Unit64::cosv(M);
+        constexpr T s = (T)(SIGN * Unit64::sinv(M));
+        return {v.x * c - v.y * s, v.x * s + v.y * c};
+    }
+}
+
+constexpr int ilog2(int n) { return n <= 1 ? 0 : 1 + ilog2(n / 2); }
+constexpr int bitrev(int i, int bits) {
+    int r = 0;
+    for (int b = 0; b < bits; ++b)
+        if (i & (1 << b)) r |= 1 << (bits - 1 - b);
+    return r;
+}
+
+// In-register DFT of length R (power of two <= 64) by a radix-2 DIT network:
+// out[k] = sum_r in[r] exp(SIGN 2 pi i r k / R).
+template <int R, int SIGN, typename T, int LEN = 2>
+struct DftLevel {
+    template <int I, int K>
+    static __device__ __forceinline__ void bfly(cpx<T>* v) {
+        if constexpr (I < R) {
+            if constexpr (K < LEN / 2) {
+                cpx<T> a = v[I + K];
+                cpx<T> b = rot64<SIGN, K * (64 / LEN)>(v[I + K + LEN / 2]);
+                v[I + K] = cadd(a, b);
+                v[I + K + LEN / 2] = csub(a, b);
+                bfly<I, K + 1>(v);
+            } else {
+                bfly<I + LEN, 0>(v);
+            }
+        }
+    }
+    static __device__ __forceinline__ void run(cpx<T>* v) {
+        bfly<0, 0>(v);
+        if constexpr (LEN < R) DftLevel<R, SIGN, T, LEN * 2>::run(v);
+    }
+};
+
+template <int R, int I = 0>
+struct BitRevPermute {
+    template <typename T>
+    static __device__ __forceinline__ void run(cpx<T>* v) {
+        if constexpr (I < R) {
+            constexpr int J = bitrev(I, ilog2(R));
+            if constexpr (J > I) {
+                cpx<T> tmp = v[I];
+                v[I] = v[J];
+                v[J] = tmp;
+            }
+            BitRevPermute<R, I + 1>::run(v);
+        }
+    }
+};
+
+template <int R, int SIGN, typename T>
+__device__ __forceinline__ void dft(cpx<T>* v) {
+    if constexpr (R == 1) {
+        return;
+    } else {
+        BitRevPermute<R>::run(v);
+        DftLevel<R, SIGN, T, 2>::run(v);
+    }
+}
+
+template <int... Rs>
+struct Radices {};
+
+// Radix plan per line length.  TPL = N / R1: one butterfly per thread in the
+// first stage; later stages have N / (Rs * TPL) butterflies per thread.
+template <int N>
+struct Plan;
+template <> struct Plan<16> { using R = Radices<4, 4>; static constexpr int TPL = 4; };
+template <> struct Plan<32> { using R = Radices<8, 4>; static constexpr int TPL = 4; };
+template <> struct Plan<64> { using R = Radices<8, 8>; static constexpr int TPL = 8; };
+template <> struct Plan<128> { using R = Radices<16, 8>; static constexpr int TPL = 8; };
+template <> struct Plan<256> { using R = Radices<16, 16>; static constexpr int TPL = 16; };
+template <> struct Plan<512> { using R = Radices<8, 8, 8>; static constexpr int TPL = 64; };
+
+// LDS layout for lines whose TPL threads are adjacent lanes (contiguous-axis
+// pass): line c, element n -> c (N + N/P) + n + n/P (one pad slot every P
+// elements breaks the stride-R1 bank pattern of the first stage's writes).
+template <int N, int P>
+struct LayoutRow {
+    static constexpr int stride = N + N / P;
+    __device__ __forceinline__ static int at(int c, int n) { return c * stride + n + n / P; }
+    static constexpr int size(int lines) { return lines * stride; }
+};
+// LDS layout for lines spread across lanes (strided-axis pass): consecutive
+// lanes hold consecutive columns c, element n -> n * TC + c.
+template <int N, int TC>
+struct LayoutCol {
+    __device__ __forceinline__ static int at(int c, int n) { return n * TC + c; }
+    static constexpr int size(int) { return N * TC; }
+};
+
+// One Stockham stage.
+template <typename T, int N, int TPL, int SIGN, int R, int NS, bool FIRST, bool LAST, class Lay>
+__device__ __forceinline__ void fft_stage(cpx<T>* e, int t, int c, cpx<T>* lds,
+                                          const cpx<T>* __restrict__ tw) {
+    constexpr int Q = N / (R * TPL);
+    static_assert(Q >= 1, "radix too large for the thread count");
+    cpx<T> v[Q][R];
+    if constexpr (FIRST) {
+        static_assert(Q == 1, "first stage must have one butterfly per thread");
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[0][r] = e[r];
+    } else {
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            const int j = t + TPL * q;
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[q][r] = lds[Lay::at(c, j + r * (N / R))];
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        const int j = t + TPL * q;
+        if constexpr (NS > 1) {
+            const int m = (j % NS) * (N / (NS * R));
+#pragma unroll
+            for (int r = 1; r < R; ++r) {
+                cpx<T> w = tw[r * m];
+                if (SIGN > 0) w.y = -w.y;
+                v[q][r] = cmul(v[q][r], w);
+            }
+        }
+        dft<R, SIGN>(v[q]);
+    }
+    if constexpr (LAST) {
+#pragma unroll
+        for (int q = 0; q < Q; ++q)
+#pragma unroll
+            for (int r = 0; r < R; ++r) e[q + r * Q] = v[q][r];
+    } else {
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            const int j = t + TPL * q;
+            const int base = (j / NS) * NS * R + (j % NS);
+#pragma unroll
+            for (int r = 0; r < R; ++r) lds[Lay::at(c, base + r * NS)] = v[q][r];
+        }
+    }
+}
+
+template <typename T, int N, int TPL, int SIGN, class Lay, int NS, bool FIRST, int... Rs>
+struct StageChain;
+template <typename T, int N, int TPL, int SIGN, class Lay, int NS, bool FIRST, int R>
+struct StageChain<T, N, TPL, SIGN, Lay, NS, FIRST, R> {
+    static __device__ __forceinline__ void run(cpx<T>* e, int t, int c, cpx<T>* lds,
+                                               const cpx<T>* tw) {
+        fft_stage<T, N, TPL, SIGN, R, NS, FIRST, true, Lay>(e, t, c, lds, tw);
+    }
+};
+template <typename T, int N, int TPL, int SIGN, class Lay, int NS, bool FIRST, int R, int R2,
+          int... Rest>
+struct StageChain<T, N, TPL, SIGN, Lay, NS, FIRST, R, R2, Rest...> {
+    static __device__ __forceinline__ void run(cpx<T>* e, int t, int c, cpx<T>* lds,
+                                               const cpx<T>* tw) {
+        fft_stage<T, N, TPL, SIGN, R, NS, FIRST, false, Lay>(e, t, c, lds, tw);
+        StageChain<T, N, TPL, SIGN, Lay, NS * R, false, R2, Rest...>::run(e, t, c, lds, tw);
+    }
+};
+
+template <typename T, int N, int SIGN, class Lay, class Rad>
+struct LineFftImpl;
+template <typename T, int N, int SIGN, class Lay, int... Rs>
+struct LineFftImpl<T, N, SIGN, Lay, Radices<Rs...>> {
+    static __device__ __forceinline__ void run(cpx<T>* e, int t, int c, cpx<T>* lds,
+                                               const cpx<T>* tw) {
+        StageChain<T, N, Plan<N>::TPL, SIGN, Lay, 1, true, Rs...>::run(e, t, c, lds, tw);
+    }
+};
+
+// Transform the line held as e[m] = x[t + TPL m]; result in the same layout.
+// SIGN = -1: forward (e^{-ikx}); +1: inverse, unnormalised.  tw[m] =
+// exp(-2 pi i m / N).
+template <typename T, int N, int SIGN, class Lay>
+__device__ __forceinline__ void line_fft(cpx<T>* e, int t, int c, cpx<T>* lds,
+                                         const cpx<T>* __restrict__ tw) {
+    LineFftImpl<T, N, SIGN, Lay, typename Plan<N>::R>::run(e, t, c, lds, tw);
+}
+
+}  // namespace psc

@@ -1,0 +1,241 @@
+// psc_ic.cpp -- host-side initial conditions of the product library.
+//
+// Restates initialize() (initialize.cpp:562-810) for the non-inflation
+// Rayleigh branch (set_mode, initialize.cpp:128-220) and initialize_GW
+// (initialize.cpp:341-470), writing FFTW-layout FP64 half spectra.  The output
+// is loaded onto the device with psc_load_ic_k / psc_load_gw_k.
+#include <cmath>
+#include <cstring>
+#include <vector>
+#include "../../include/psc.h"
+
+// Bit-exact agreement with other IEEE restatements of initialize(): no FMA
+// contraction in this file, and the phase's sine and cosine from one glibc
+// sincos call (what g++ -O3 emits for the reference's separate cos/sin calls at
+// initialize.cpp:185-191).
+#pragma clang fp contract(off)
+
+namespace {
+
+const double kPi = 3.14159265358979323846264338327950288;
+inline double pw2(double x) { return x * x; }
+inline int sidx(int i, int N) { return i <= N / 2 ? i : i - N; }
+
+// rand_uniform (initialize.cpp:13-28): Park-Miller minimal standard with
+// Schrage's factorisation; the first call advances 99 extra times.
+class Rng {
+  public:
+    explicit Rng(int seed) : seed_(seed), next_(seed > 0 ? seed : 1) {}
+    double uniform() {
+        if (seed_ < 1) return 0.33;
+        if (!warm_) {
+            for (int i = 1; i < 100; ++i) step();
+            warm_ = true;
+        }
+        step();
+        return (double)next_ / 2147483647.0;
+    }
+
+  private:
+    void step() {
+        next_ = 16807 * (next_ % 127773) - 2836 * (next_ / 127773);
+        if (next_ < 0) next_ += 2147483647;
+    }
+    int seed_;
+    int next_;
+    bool warm_ = false;
+};
+
+struct Pot {
+    int nflds;
+    double m2, g2;
+    int terms() const { return nflds == 2 ? 2 : 1; }
+    double term(int t, const double* f) const {
+        return t == 0 ? 0.5 * m2 * pw2(f[0]) : 0.5 * g2 * pw2(f[0]) * pw2(f[1]);
+    }
+    void meff(double* m, const double* f) const {
+        m[0] = m2 + (nflds == 2 ? g2 * pw2(f[1]) : 0.0);
+        if (nflds == 2) m[1] = g2 * pw2(f[0]);
+    }
+};
+
+// The mode order of initialize.cpp:685-749 (shared with initialize_GW):
+// fn(i, j, k, conjugate index or -1, isReal).
+template <class F>
+void for_each_mode(int N, bool dc, F&& fn) {
+    const int Nh = N / 2 + 1;
+    auto at = [&](int i, int j, int k) { return ((long)i * N + j) * Nh + k; };
+    for (int k = 0; k <= N / 2; k += N / 2) {
+        for (int i = 0; i <= N / 2; i += N / 2)
+            for (int j = 1; j < N / 2; ++j) fn(i, j, k, at(i, N - j, k), false);
+        for (int j = 0; j <= N / 2; j += N / 2)
+            for (int i = 1; i < N / 2; ++i) fn(i, j, k, at(N - i, j, k), false);
+        for (int i = 1; i < N / 2; ++i) {
+            for (int j = 1; j < N / 2; ++j) fn(i, j, k, at(N - i, N - j, k), false);
+            for (int j = N / 2 + 1; j < N; ++j) fn(i, j, k, at(N - i, N - j, k), false);
+        }
+    }
+    for (int k = 1; k < N / 2; ++k)
+        for (int i = 0; i < N; ++i)
+            for (int j = 0; j < N; ++j) fn(i, j, k, -1L, false);
+    for (int k = 0; k <= N / 2; k += N / 2)
+        for (int i = 0; i <= N / 2; i += N / 2)
+            for (int j = 0; j <= N / 2; j += N / 2) {
+                if (!dc && i == 0 && j == 0 && k == 0) continue;
+                fn(i, j, k, -1L, true);
+            }
+}
+
+bool valid(const psc_config* c) {
+    return c && c->N >= 16 && c->N <= 512 && (c->N & (c->N - 1)) == 0 && c->nflds >= 1 &&
+           c->nflds <= 2 && c->kIR > 0 && c->fStar > 0 && c->MPl > 0;
+}
+
+void background(const psc_config& c, const Pot& V, const psc_ic_params& ic, double* a, double* ad,
+                double* addp, double* H) {
+    double f0[2] = {ic.f0[0], c.nflds > 1 ? ic.f0[1] : 0.0};
+    double fdsq = 0.0, pot = 0.0;
+    for (int q = 0; q < c.nflds; ++q) fdsq += pw2(ic.fd0[q]);
+    for (int t = 0; t < V.terms(); ++t) pot += V.term(t, f0);
+    double h = (fdsq / 2.0 + pot) * pw2(c.fStar / c.MPl) / 3.0;
+    *H = std::sqrt(h);
+    *a = ic.a0;
+    *ad = *H * ic.a0;
+    *addp = pw2(c.fStar / c.MPl) / 3.0 * ((c.alpha - 2) * fdsq + (c.alpha + 1) * pot) * ic.a0;
+    if (c.expansion == PSC_EXPANSION_STATIC) *H = *ad = *addp = 0.0;
+}
+
+// orthogonal_unit_vectors / TT_tensors (initialize.cpp:66-122).
+void polarisation(const double k[3], double e[3][3]) {
+    const double kn = std::sqrt(k[0] * k[0] + k[1] * k[1] + k[2] * k[2]);
+    const double h[3] = {k[0] / kn, k[1] / kn, k[2] / kn};
+    const bool z = std::fabs(h[0]) < 1e-9 && std::fabs(h[1]) < 1e-9 && std::fabs(h[2] - 1.0) < 1e-9;
+    const double ref[3] = {0.0, z ? 1.0 : 0.0, z ? 0.0 : 1.0};
+    double u[3] = {h[1] * ref[2] - h[2] * ref[1], h[2] * ref[0] - h[0] * ref[2], h[0] * ref[1] - h[1] * ref[0]};
+    double un = std::sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+    for (double& x : u) x /= un;
+    double v[3] = {h[1] * u[2] - h[2] * u[1], h[2] * u[0] - h[0] * u[2], h[0] * u[1] - h[1] * u[0]};
+    double vn = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    for (double& x : v) x /= vn;
+    for (int i = 0; i < 3; ++i)
+        if (u[i] < 0) { for (double& x : u) x = -x; break; }
+    for (int i = 0; i < 3; ++i)
+        if (v[i] < 0) { for (double& x : v) x = -x; break; }
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) e[i][j] = u[i] * u[j] - v[i] * v[j] + u[i] * v[j] + v[i] * u[j];
+}
+
+}  // namespace
+
+extern "C" int psc_generate_ic(const psc_config* cfg, const psc_model* model, const psc_ic_params* ic,
+                               double* f_k, double* pi_k, double* scalars_out) {
+    if (!valid(cfg) || !model || !ic || !f_k || !pi_k || ic->a0 <= 0) return PSC_ERR_ARG;
+    const int N = cfg->N, Nh = N / 2 + 1;
+    const size_t nk = (size_t)N * N * Nh;
+    const Pot V{cfg->nflds, model->m2, model->g2};
+    double a, ad, addp, H;
+    background(*cfg, V, *ic, &a, &ad, &addp, &H);
+    if (!(H >= 0.0)) return PSC_ERR_ARG;  // initialize.cpp:661-664
+    const double L = 2.0 * kPi / cfg->kIR, dx = L / N;
+    // Vacuum normalisation (omegaStar/fStar) (N/dx)^{3/2} / sqrt(2): the
+    // commented DeltaF_DeltaPi of parameters.h:132.
+    const double norm = cfg->omegaStar / cfg->fStar * std::pow(N / dx, 1.5) / std::sqrt(2.0);
+    const double f0[2] = {ic->f0[0], cfg->nflds > 1 ? ic->f0[1] : 0.0};
+    double meff[2] = {0, 0};
+    V.meff(meff, f0);
+    const double dp2 = pw2(cfg->kIR), ap = std::pow(a, cfg->alpha - 3);
+    const double N3 = (double)N * N * N;
+    Rng rng(ic->seed);
+    for (int q = 0; q < cfg->nflds; ++q) {
+        double* fq = f_k + 2 * nk * q;
+        double* pq = pi_k + 2 * nk * q;
+        for_each_mode(N, true, [&](int i, int j, int k, long conj, bool real) {
+            const int px = sidx(i, N), py = sidx(j, N);
+            const double p2 = dp2 * (pw2(px) + pw2(py) + pw2(k));
+            const double om = (p2 + meff[q] > 0.0) ? std::sqrt(p2 + pw2(a) * meff[q]) : std::sqrt(p2);
+            const double rms = om > 0.0 ? norm / std::sqrt(om) : 0.0;
+            const double al = rms / std::sqrt(2.0) * std::sqrt(std::log(1.0 / rng.uniform())) / a;
+            const double ar = rms / std::sqrt(2.0) * std::sqrt(std::log(1.0 / rng.uniform())) / a;
+            double sn, cs;
+            ::sincos(2.0 * kPi * rng.uniform(), &sn, &cs);
+            const double rl = al * cs, il = al * sn;
+            ::sincos(2.0 * kPi * rng.uniform(), &sn, &cs);
+            const double rr = ar * cs, ir = ar * sn;
+            double fr = rl + rr, fi = il + ir;
+            double pr = ap * (-om * (il - ir) - H * fr);
+            double pim = ap * (om * (rl - rr) - H * fi);
+            if (real) fi = pim = 0.0;
+            const long at = ((long)i * N + j) * Nh + k;
+            fq[2 * at] = fr; fq[2 * at + 1] = fi;
+            pq[2 * at] = pr; pq[2 * at + 1] = pim;
+            if (conj >= 0) {
+                fq[2 * conj] = fr; fq[2 * conj + 1] = -fi;
+                pq[2 * conj] = pr; pq[2 * conj + 1] = -pim;
+            }
+        });
+        fq[0] = ic->f0[q] * N3;
+        fq[1] = 0.0;
+        pq[0] = ap * ic->fd0[q] * N3;
+        pq[1] = 0.0;
+    }
+    if (scalars_out) {
+        scalars_out[0] = a;
+        scalars_out[1] = ad;
+        scalars_out[2] = addp;
+        scalars_out[3] = H;
+    }
+    return PSC_OK;
+}
+
+extern "C" int psc_generate_gw_ic(const psc_config* cfg, const psc_model* model,
+                                  const psc_ic_params* ic, double* u_k, double* pi_u_k) {
+    if (!valid(cfg) || !model || !ic || !u_k || !pi_u_k || ic->a0 <= 0) return PSC_ERR_ARG;
+    static const int ci[6] = {0, 1, 2, 0, 0, 1}, cj[6] = {0, 1, 2, 1, 2, 2};
+    const int N = cfg->N, Nh = N / 2 + 1;
+    const size_t nk = (size_t)N * N * Nh;
+    const Pot V{cfg->nflds, model->m2, model->g2};
+    double a, ad, addp, H;
+    background(*cfg, V, *ic, &a, &ad, &addp, &H);
+    Rng rng(ic->seed);
+    long modes = 0;
+    for_each_mode(N, true, [&](int, int, int, long, bool) { ++modes; });
+    for (long d = 0; d < 4L * modes * cfg->nflds; ++d) rng.uniform();
+    const double L = 2.0 * kPi / cfg->kIR, dx = L / N;
+    const double norm = cfg->MPl * cfg->omegaStar / pw2(cfg->fStar) * std::pow(N / dx, 1.5);
+    const double ap = std::pow(a, cfg->alpha - 3);
+    std::memset(u_k, 0, sizeof(double) * 2 * nk * 6);
+    std::memset(pi_u_k, 0, sizeof(double) * 2 * nk * 6);
+    for_each_mode(N, false, [&](int i, int j, int k, long conj, bool real) {
+        const int px = sidx(i, N), py = sidx(j, N);
+        const double kv[3] = {px * cfg->kIR, py * cfg->kIR, k * cfg->kIR};
+        double e[3][3];
+        polarisation(kv, e);
+        const double kk = std::sqrt(pw2(cfg->kIR) * (pw2(px) + pw2(py) + pw2(k)));
+        const double rms = norm / std::sqrt(kk);
+        const double al = rms / std::sqrt(2.0) * std::sqrt(std::log(1.0 / rng.uniform())) / a;
+        const double ar = rms / std::sqrt(2.0) * std::sqrt(std::log(1.0 / rng.uniform())) / a;
+        double sn, cs;
+        ::sincos(2.0 * kPi * rng.uniform(), &sn, &cs);
+        const double rl = al * cs, il = al * sn;
+        ::sincos(2.0 * kPi * rng.uniform(), &sn, &cs);
+        const double rr = ar * cs, ir = ar * sn;
+        const long at = ((long)i * N + j) * Nh + k;
+        for (int c = 0; c < 6; ++c) {
+            const double ec = e[ci[c]][cj[c]];
+            const double ur = ec * (rl + rr), ui0 = ec * (il + ir);
+            // initialize.cpp:317-318 multiplies the Hubble term by e_ij again.
+            const double pr = ap * (ec * (-kk * (il - ir) - H * ur));
+            const double pi0 = ap * (ec * (kk * (rl - rr) - H * ui0));
+            const double ui = real ? 0.0 : ui0, pim = real ? 0.0 : pi0;
+            double* uc = u_k + 2 * nk * c;
+            double* pc = pi_u_k + 2 * nk * c;
+            uc[2 * at] = ur; uc[2 * at + 1] = ui;
+            pc[2 * at] = pr; pc[2 * at + 1] = pim;
+            if (conj >= 0) {
+                uc[2 * conj] = ur; uc[2 * conj + 1] = -ui;
+                pc[2 * conj] = pr; pc[2 * conj + 1] = -pim;
+            }
+        }
+    });
+    return PSC_OK;
+}

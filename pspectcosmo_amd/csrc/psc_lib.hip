@@ -1,0 +1,867 @@
+// psc_lib.hip -- host side of the HIP library: device state, kernel
+// sequencing of the time step (evolve_VV_k_nopad, evolution.cpp:1131-1198),
+// hipGraph capture, per-kernel timing and the C ABI of include/psc.h.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/psc.h"
+#include "psc_kernels.hpp"
+
+using namespace psc;
+
+namespace {
+
+enum Kind {
+    K_Z_INV_DRIFT = 0,
+    K_Y_INV,
+    K_X_FORCE,
+    K_Y_FWD,
+    K_Z_FWD_KICK,
+    K_SCALARS,
+    K_Z_INV,
+    K_Z_FWD_KICK_NR,
+    K_DIAG,
+    K_NKINDS
+};
+const char* kKindNames[K_NKINDS] = {"z_inverse_drift", "y_inverse", "x_force", "y_forward",
+                                    "z_forward_kick", "scalars", "z_inverse", "z_forward_kick_first",
+                                    "diagnostics"};
+
+struct Timing {
+    double ms = 0.0;
+    long long count = 0;
+    double bytes = 0.0;
+};
+
+struct Pending {
+    int kind;
+    hipEvent_t a, b;
+};
+
+struct EngineBase;
+
+}  // namespace
+
+struct psc_ctx {
+    psc_config cfg{};
+    int N = 0, B = 1;
+    size_t N3 = 0, es = 0;
+    hipStream_t stream = nullptr;
+    void* Pf = nullptr;
+    void* Pp = nullptr;
+    void* W = nullptr;
+    void* tw = nullptr;
+    MemberState* ms = nullptr;
+    double* partV = nullptr;
+    double* partKG = nullptr;
+    double* part = nullptr;   // diagnostics partials
+    double* sums = nullptr;   // diagnostics totals
+    size_t part_len = 0, sums_len = 0;
+    std::vector<double> w;
+    std::vector<psc_model> models;
+    std::vector<char> loaded;
+    std::string err;
+    EngineBase* eng = nullptr;
+    int timing = 0;
+    Timing kt[K_NKINDS];
+    std::vector<hipEvent_t> ev_free;
+    std::vector<Pending> pending;
+    hipGraphExec_t g_mid = nullptr, g_last = nullptr;
+    double dev_bytes = 0.0;
+
+    int fail(int code, const std::string& msg) {
+        err = msg;
+        return code;
+    }
+};
+
+#define HIPCHK(c, expr)                                                                  \
+    do {                                                                                 \
+        hipError_t e_ = (expr);                                                          \
+        if (e_ != hipSuccess)                                                            \
+            return (c)->fail(PSC_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+namespace {
+
+StepConsts consts_of(const psc_ctx* c) {
+    StepConsts s{};
+    s.dt = c->cfg.dt;
+    s.kIR = c->cfg.kIR;
+    s.f2m = (c->cfg.fStar / c->cfg.MPl) * (c->cfg.fStar / c->cfg.MPl);
+    s.N3 = (double)c->N3;
+    s.alpha = c->cfg.alpha;
+    s.expansion_static = c->cfg.expansion == PSC_EXPANSION_STATIC;
+    s.nflds = c->cfg.nflds;
+    return s;
+}
+
+// Event-bracketed launch when timing is on.
+template <class F>
+int timed(psc_ctx* c, int kind, double bytes, F&& launch) {
+    if (!c->timing) {
+        launch();
+        return PSC_OK;
+    }
+    hipEvent_t a, b;
+    for (int q = 0; q < 2; ++q) {
+        hipEvent_t e;
+        if (!c->ev_free.empty()) {
+            e = c->ev_free.back();
+            c->ev_free.pop_back();
+        } else {
+            HIPCHK(c, hipEventCreate(&e));
+        }
+        (q == 0 ? a : b) = e;
+    }
+    HIPCHK(c, hipEventRecord(a, c->stream));
+    launch();
+    HIPCHK(c, hipEventRecord(b, c->stream));
+    c->pending.push_back({kind, a, b});
+    c->kt[kind].bytes = bytes;
+    return PSC_OK;
+}
+
+int drain_timing(psc_ctx* c) {
+    if (c->pending.empty()) return PSC_OK;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (auto& p : c->pending) {
+        float ms = 0.0f;
+        HIPCHK(c, hipEventElapsedTime(&ms, p.a, p.b));
+        c->kt[p.kind].ms += ms;
+        c->kt[p.kind].count += 1;
+        c->ev_free.push_back(p.a);
+        c->ev_free.push_back(p.b);
+    }
+    c->pending.clear();
+    return PSC_OK;
+}
+
+struct EngineBase {
+    virtual ~EngineBase() = default;
+    virtual int step(psc_ctx* c, int nsteps) = 0;
+    // Real-space IFFT of src (Pf or Pp of one member) into W[member] as
+    // normalised (phi, chi), with diagnostic sums into c->sums[0..5].
+    virtual int real_space(psc_ctx* c, int member, const void* src) = 0;
+    // Per-field K, G raw sums into c->sums[0..3].
+    virtual int field_sums(psc_ctx* c, int member) = 0;
+    // Binned spectrum partial sums into c->sums[0 .. 4*maxnumbins).
+    virtual int spectra(psc_ctx* c, int member, int fld) = 0;
+};
+
+template <typename T, int N>
+struct Engine final : EngineBase {
+    using G = Geo<T, N>;
+    static cpx<T>* P(void* p) { return static_cast<cpx<T>*>(p); }
+
+    int launch_z_inverse(psc_ctx* c, bool drift, const cpx<T>* src, cpx<T>* Pf, const cpx<T>* Pp,
+                         cpx<T>* W, const MemberState* ms, int members) {
+        const double X = (double)members * G::N3 * sizeof(cpx<T>);
+        dim3 grid(G::BLOCKS_ROW, members), block(G::THREADS_ROW);
+        if (drift)
+            return timed(c, K_Z_INV_DRIFT, 4 * X, [&] {
+                hipLaunchKernelGGL((k_z_inverse<T, N, true>), grid, block, 0, c->stream, src, Pf, Pp, W,
+                                   ms, P(c->tw));
+            });
+        return timed(c, K_Z_INV, 2 * X, [&] {
+            hipLaunchKernelGGL((k_z_inverse<T, N, false>), grid, block, 0, c->stream, src, Pf, Pp, W, ms,
+                               P(c->tw));
+        });
+    }
+    template <int SIGN>
+    int launch_y(psc_ctx* c, cpx<T>* W, int members) {
+        const double X = (double)members * G::N3 * sizeof(cpx<T>);
+        dim3 grid(N / G::TC, N, members), block(G::THREADS_COL);
+        return timed(c, SIGN > 0 ? K_Y_INV : K_Y_FWD, 2 * X, [&] {
+            hipLaunchKernelGGL((k_strided<T, N, 1, SIGN>), grid, block, 0, c->stream, W, P(c->tw));
+        });
+    }
+    int launch_x_force(psc_ctx* c) {
+        const double X = (double)c->B * G::N3 * sizeof(cpx<T>);
+        dim3 grid(N / G::TC, N, c->B), block(G::THREADS_COL);
+        return timed(c, K_X_FORCE, 2 * X, [&] {
+            hipLaunchKernelGGL((k_x_force<T, N>), grid, block, 0, c->stream, P(c->W), c->ms, P(c->tw),
+                               c->partV);
+        });
+    }
+    int launch_z_kick(psc_ctx* c, bool reduce, double c1, double c2, int use_a_start) {
+        const double X = (double)c->B * G::N3 * sizeof(cpx<T>);
+        dim3 grid(G::BLOCKS_ROW, c->B), block(G::THREADS_ROW);
+        const StepConsts sc = consts_of(c);
+        if (reduce)
+            return timed(c, K_Z_FWD_KICK, 4 * X, [&] {
+                hipLaunchKernelGGL((k_z_kick<T, N, true>), grid, block, 0, c->stream, P(c->W), P(c->Pf),
+                                   P(c->Pp), c->ms, P(c->tw), c->partKG, c1, c2, use_a_start, sc);
+            });
+        return timed(c, K_Z_FWD_KICK_NR, 4 * X, [&] {
+            hipLaunchKernelGGL((k_z_kick<T, N, false>), grid, block, 0, c->stream, P(c->W), P(c->Pf),
+                               P(c->Pp), c->ms, P(c->tw), c->partKG, c1, c2, use_a_start, sc);
+        });
+    }
+    int launch_scalars(psc_ctx* c, const ScalarOp& op) {
+        const StepConsts sc = consts_of(c);
+        const double bytes = (double)c->B * ((double)(op.nV + 2 * op.nKG) * 8.0 + 2 * sizeof(MemberState));
+        return timed(c, K_SCALARS, bytes, [&] {
+            hipLaunchKernelGGL(k_scalars, dim3(c->B), dim3(256), 0, c->stream, c->ms, c->partV, c->partKG,
+                               op, sc);
+        });
+    }
+
+    // One force evaluation chain; DRIFT applies the substep's drift first.
+    int force_chain(psc_ctx* c, bool drift, bool reduce, double c1, double c2, int use_a_start) {
+        int rc;
+        if ((rc = launch_z_inverse(c, drift, P(c->Pf), P(c->Pf), P(c->Pp), P(c->W), c->ms, c->B))) return rc;
+        if ((rc = launch_y<+1>(c, P(c->W), c->B))) return rc;
+        if ((rc = launch_x_force(c))) return rc;
+        if ((rc = launch_y<-1>(c, P(c->W), c->B))) return rc;
+        return launch_z_kick(c, reduce, c1, c2, use_a_start);
+    }
+
+    // All substeps of one step; `final` drops the trailing first-kick of the
+    // next substep so the momenta are synchronised on return.
+    int record_step(psc_ctx* c, bool final) {
+        const int S = (int)c->w.size();
+        const double dt = c->cfg.dt;
+        for (int p = 0; p < S; ++p) {
+            const bool last = final && p == S - 1;
+            const double wn = c->w[(p + 1) % S];
+            int rc = force_chain(c, true, true, c->w[p] * dt / 2.0, last ? 0.0 : wn * dt / 2.0, 0);
+            if (rc) return rc;
+            ScalarOp op{};
+            op.finish = 1;
+            op.last_of_step = p == S - 1;
+            op.begin = last ? 0 : 1;
+            op.w_finish = c->w[p];
+            op.w_begin = wn;
+            op.nV = G::BLOCKS_COL;
+            op.nKG = G::BLOCKS_ROW;
+            if ((rc = launch_scalars(c, op))) return rc;
+        }
+        return PSC_OK;
+    }
+
+    int build_graph(psc_ctx* c, bool final, hipGraphExec_t* out) {
+        hipGraph_t g;
+        HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        int rc = record_step(c, final);
+        hipError_t e = hipStreamEndCapture(c->stream, &g);
+        if (rc) return rc;
+        if (e != hipSuccess) return c->fail(PSC_ERR_DEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+        HIPCHK(c, hipGraphInstantiate(out, g, nullptr, nullptr, 0));
+        HIPCHK(c, hipGraphDestroy(g));
+        return PSC_OK;
+    }
+
+    int step(psc_ctx* c, int nsteps) override {
+        if (nsteps <= 0) return PSC_OK;
+        ScalarOp op{};
+        op.begin = 1;
+        op.w_begin = c->w[0];
+        op.nV = G::BLOCKS_COL;
+        op.nKG = G::BLOCKS_ROW;
+        int rc = launch_scalars(c, op);
+        if (rc) return rc;
+        if ((rc = force_chain(c, false, false, c->w[0] * c->cfg.dt / 2.0, 0.0, 1))) return rc;
+        const bool graph = c->cfg.use_graph && !c->timing;
+        if (graph) {
+            if (!c->g_mid && (rc = build_graph(c, false, &c->g_mid))) return rc;
+            if (!c->g_last && (rc = build_graph(c, true, &c->g_last))) return rc;
+            for (int s = 0; s < nsteps - 1; ++s) HIPCHK(c, hipGraphLaunch(c->g_mid, c->stream));
+            HIPCHK(c, hipGraphLaunch(c->g_last, c->stream));
+        } else {
+            for (int s = 0; s < nsteps; ++s)
+                if ((rc = record_step(c, s == nsteps - 1))) return rc;
+        }
+        return c->timing ? drain_timing(c) : PSC_OK;
+    }
+
+    int real_space(psc_ctx* c, int member, const void* src) override {
+        const size_t off = (size_t)member * G::N3;
+        cpx<T>* W = P(c->W) + off;
+        int rc = launch_z_inverse(c, false, static_cast<const cpx<T>*>(src) + off, nullptr, nullptr, W,
+                                  c->ms + member, 1);
+        if (rc) return rc;
+        if ((rc = launch_y<+1>(c, W, 1))) return rc;
+        dim3 grid(N / G::TC, N, 1), block(G::THREADS_COL);
+        rc = timed(c, K_DIAG, 2.0 * G::N3 * sizeof(cpx<T>), [&] {
+            hipLaunchKernelGGL((k_x_inverse_diag<T, N>), grid, block, 0, c->stream, W, c->ms + member,
+                               P(c->tw), c->part);
+        });
+        if (rc) return rc;
+        hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(64), 0, c->stream, c->part, G::BLOCKS_COL, 6,
+                           c->sums);
+        HIPCHK(c, hipGetLastError());
+        return PSC_OK;
+    }
+
+    int field_sums(psc_ctx* c, int member) override {
+        const size_t off = (size_t)member * G::N3;
+        const int nb = 256;
+        hipLaunchKernelGGL((k_field_sums<T, N>), dim3(nb, 1), dim3(256), 0, c->stream, P(c->Pf) + off,
+                           P(c->Pp) + off, c->part, nb);
+        hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(64), 0, c->stream, c->part, nb, 4, c->sums);
+        HIPCHK(c, hipGetLastError());
+        return PSC_OK;
+    }
+
+    int spectra(psc_ctx* c, int member, int fld) override {
+        const size_t off = (size_t)member * G::N3;
+        const int nbt = (int)(std::sqrt(3.0) * (N / 2)) + 2;
+        const int nb = 128;
+        const size_t shmem = (size_t)nbt * 4 * sizeof(double);
+        hipLaunchKernelGGL((k_spectra<T, N>), dim3(nb, 1), dim3(256), shmem, c->stream, P(c->Pf) + off,
+                           P(c->Pp) + off, c->ms + member, fld, c->cfg.alpha, nbt, c->part, nb);
+        const int width = nbt * 4;
+        hipLaunchKernelGGL(k_sum_partials, dim3((width + 255) / 256), dim3(256), 0, c->stream, c->part,
+                           nb, width, c->sums);
+        HIPCHK(c, hipGetLastError());
+        return PSC_OK;
+    }
+};
+
+EngineBase* make_engine(int precision, int N) {
+#define PSC_CASE(NN)                                                 \
+    case NN:                                                         \
+        if (precision == PSC_FP32) return new Engine<float, NN>();   \
+        return new Engine<double, NN>();
+    switch (N) {
+        PSC_CASE(16)
+        PSC_CASE(32)
+        PSC_CASE(64)
+        PSC_CASE(128)
+        PSC_CASE(256)
+        PSC_CASE(512)
+        default:
+            return nullptr;
+    }
+#undef PSC_CASE
+}
+
+void integrator_weights(const psc_config& c, std::vector<double>& w) {
+    if (c.integrator == PSC_Y4) {
+        if (c.yoshida_exact) {
+            const double cr = std::cbrt(2.0);
+            w = {1.0 / (2.0 - cr), -cr / (2.0 - cr), 1.0 / (2.0 - cr)};
+        } else {
+            // pspectcosmo.h:288, digits as shipped
+            w = {1.351207191959657771818, -1.702414403875838200264, 1.351207191959657771818};
+        }
+    } else {
+        w = {1.0};
+    }
+}
+
+int maxnumbins(int N) { return (int)(std::sqrt(3.0) * (N / 2)) + 2; }
+
+int check_member(psc_ctx* c, int member) {
+    if (!c) return PSC_ERR_ARG;
+    if (member < 0 || member >= c->B) return c->fail(PSC_ERR_ARG, "member index out of range");
+    return PSC_OK;
+}
+
+// Copy one member's MemberState to the host.
+int read_state(psc_ctx* c, int member, MemberState* s) {
+    HIPCHK(c, hipMemcpyAsync(s, c->ms + member, sizeof(MemberState), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return PSC_OK;
+}
+int write_state(psc_ctx* c, int member, const MemberState* s) {
+    HIPCHK(c, hipMemcpyAsync(c->ms + member, s, sizeof(MemberState), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return PSC_OK;
+}
+
+// FP64 half spectra (nflds fields) -> packed full spectrum (phi + i chi).
+void pack_full(int N, int nflds, const double* hk, std::vector<double>& full) {
+    const int Nh = N / 2 + 1;
+    const size_t nk = (size_t)N * N * Nh, N3 = (size_t)N * N * N;
+    full.assign(2 * N3, 0.0);
+    for (int i = 0; i < N; ++i)
+        for (int j = 0; j < N; ++j)
+            for (int k = 0; k < N; ++k) {
+                double re[2] = {0, 0}, im[2] = {0, 0};
+                for (int q = 0; q < nflds; ++q) {
+                    const double* h = hk + 2 * nk * q;
+                    if (k <= N / 2) {
+                        const size_t at = ((size_t)i * N + j) * Nh + k;
+                        re[q] = h[2 * at];
+                        im[q] = h[2 * at + 1];
+                    } else {
+                        const size_t at = ((size_t)((N - i) % N) * N + (N - j) % N) * Nh + (N - k);
+                        re[q] = h[2 * at];
+                        im[q] = -h[2 * at + 1];
+                    }
+                }
+                const size_t o = ((size_t)i * N + j) * N + k;
+                full[2 * o] = re[0] - im[1];
+                full[2 * o + 1] = im[0] + re[1];
+            }
+}
+
+// Packed full spectrum -> FP64 half spectra of nflds fields.
+void unpack_half(int N, int nflds, const double* full, double* hk) {
+    const int Nh = N / 2 + 1;
+    const size_t nk = (size_t)N * N * Nh;
+    for (int i = 0; i < N; ++i)
+        for (int j = 0; j < N; ++j)
+            for (int k = 0; k < Nh; ++k) {
+                const size_t o = ((size_t)i * N + j) * N + k;
+                const size_t mo = ((size_t)((N - i) % N) * N + (N - j) % N) * N + (N - k) % N;
+                const double ar = full[2 * o], ai = full[2 * o + 1];
+                const double br = full[2 * mo], bi = -full[2 * mo + 1];
+                const size_t at = ((size_t)i * N + j) * Nh + k;
+                hk[2 * at] = 0.5 * (ar + br);
+                hk[2 * at + 1] = 0.5 * (ai + bi);
+                if (nflds > 1) {
+                    hk[2 * nk + 2 * at] = 0.5 * (ai - bi);
+                    hk[2 * nk + 2 * at + 1] = -0.5 * (ar - br);
+                }
+            }
+}
+
+int upload(psc_ctx* c, void* dst_base, int member, const std::vector<double>& full) {
+    const size_t n = c->N3;
+    char* dst = static_cast<char*>(dst_base) + (size_t)member * n * c->es;
+    if (c->cfg.precision == PSC_FP32) {
+        std::vector<float> tmp(2 * n);
+        for (size_t i = 0; i < 2 * n; ++i) tmp[i] = (float)full[i];
+        HIPCHK(c, hipMemcpy(dst, tmp.data(), tmp.size() * sizeof(float), hipMemcpyHostToDevice));
+    } else {
+        HIPCHK(c, hipMemcpy(dst, full.data(), full.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
+    return PSC_OK;
+}
+
+int download(psc_ctx* c, const void* src_base, int member, std::vector<double>& full) {
+    const size_t n = c->N3;
+    const char* src = static_cast<const char*>(src_base) + (size_t)member * n * c->es;
+    full.resize(2 * n);
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (c->cfg.precision == PSC_FP32) {
+        std::vector<float> tmp(2 * n);
+        HIPCHK(c, hipMemcpy(tmp.data(), src, tmp.size() * sizeof(float), hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < 2 * n; ++i) full[i] = tmp[i];
+    } else {
+        HIPCHK(c, hipMemcpy(full.data(), src, full.size() * sizeof(double), hipMemcpyDeviceToHost));
+    }
+    return PSC_OK;
+}
+
+struct Diag {
+    double K[2], G[2], Vt[2], mean[2], var[2], sumsq[2];
+};
+
+// Energies of one member from its device state (energy_density,
+// output.cpp:57-115, with kinetic/gradient from k space and V on the lattice).
+int diagnose(psc_ctx* c, int member, const MemberState& s, Diag* d) {
+    int rc = c->eng->field_sums(c, member);
+    if (rc) return rc;
+    double fs[4];
+    HIPCHK(c, hipMemcpyAsync(fs, c->sums, sizeof(fs), hipMemcpyDeviceToHost, c->stream));
+    if ((rc = c->eng->real_space(c, member, c->Pf))) return rc;
+    double rs[6];
+    HIPCHK(c, hipMemcpyAsync(rs, c->sums, sizeof(rs), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    const double N3 = (double)c->N3, a = s.a, kIR = c->cfg.kIR;
+    for (int q = 0; q < 2; ++q) {
+        d->K[q] = 0.5 * fs[q] / std::pow(a, 6) / (N3 * N3);
+        d->G[q] = kIR * kIR * fs[2 + q] / N3 / 2.0 / (a * a) / N3;
+        d->mean[q] = rs[q] / N3;
+        d->sumsq[q] = rs[2 + q] / N3;
+        const double v = d->sumsq[q] - d->mean[q] * d->mean[q];
+        d->var[q] = v > 0 ? v : 0.0;
+        d->Vt[q] = rs[4 + q] / N3;
+    }
+    if (c->cfg.nflds == 1) d->K[1] = d->G[1] = d->Vt[1] = d->mean[1] = d->var[1] = d->sumsq[1] = 0.0;
+    return PSC_OK;
+}
+
+double kernel_a_value(const psc_ctx* c, const Diag& d, double a) {
+    if (c->cfg.expansion == PSC_EXPANSION_STATIC) return 0.0;
+    const int al = c->cfg.alpha;
+    const double f2m = (c->cfg.fStar / c->cfg.MPl) * (c->cfg.fStar / c->cfg.MPl);
+    const double K = d.K[0] + d.K[1], G = d.G[0] + d.G[1], V = d.Vt[0] + d.Vt[1];
+    return 1.0 / 3.0 * f2m * std::pow(a, 1 + 2 * al) * ((al - 2) * K + al * G + (al + 1) * V);
+}
+
+}  // namespace
+
+// =============================================================================
+extern "C" {
+
+int psc_abi_version(void) { return PSC_ABI_VERSION; }
+
+const char* psc_status_string(int s) {
+    switch (s) {
+        case PSC_OK: return "ok";
+        case PSC_ERR_ARG: return "invalid argument";
+        case PSC_ERR_DEVICE: return "device error";
+        case PSC_ERR_NOMEM: return "out of memory";
+        case PSC_ERR_STATE: return "invalid state";
+        case PSC_ERR_NONFINITE: return "field not numerical (unstable solution)";
+        case PSC_ERR_NEGATIVE_A: return "scale factor is negative";
+        case PSC_ERR_UNSUPPORTED: return "unsupported configuration";
+        default: return "unknown status";
+    }
+}
+
+int psc_num_bins(int N) { return maxnumbins(N) - 1; }
+
+int psc_create(const psc_config* cfg, psc_ctx** out) {
+    if (!cfg || !out) return PSC_ERR_ARG;
+    *out = nullptr;
+    const int N = cfg->N;
+    if (N < 16 || N > 512 || (N & (N - 1)) || cfg->nflds < 1 || cfg->nflds > 2 || cfg->batch < 1 ||
+        (cfg->integrator != PSC_VV2 && cfg->integrator != PSC_Y4) ||
+        (cfg->precision != PSC_FP64 && cfg->precision != PSC_FP32) || !(cfg->kIR > 0) ||
+        !(cfg->fStar > 0) || !(cfg->MPl > 0) || !std::isfinite(cfg->dt) ||
+        (cfg->expansion != PSC_EXPANSION_SELF && cfg->expansion != PSC_EXPANSION_STATIC))
+        return PSC_ERR_ARG;
+    if (cfg->with_gw) return PSC_ERR_UNSUPPORTED;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return PSC_ERR_DEVICE;
+    std::unique_ptr<psc_ctx> c(new psc_ctx());
+    c->cfg = *cfg;
+    c->N = N;
+    c->B = cfg->batch;
+    c->N3 = (size_t)N * N * N;
+    c->es = cfg->precision == PSC_FP32 ? 2 * sizeof(float) : 2 * sizeof(double);
+    c->eng = make_engine(cfg->precision, N);
+    if (!c->eng) return PSC_ERR_UNSUPPORTED;
+    integrator_weights(*cfg, c->w);
+    c->models.assign(c->B, psc_model{1.0, 0.0});
+    c->loaded.assign(c->B, 0);
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return PSC_ERR_DEVICE;
+    const size_t arr = (size_t)c->B * c->N3 * c->es;
+    auto alloc = [&](void** p, size_t bytes) {
+        if (hipMalloc(p, bytes) != hipSuccess) return false;
+        c->dev_bytes += (double)bytes;
+        return true;
+    };
+    // partial-sum sizes: max blocks of any pass x widths
+    const size_t blocks_col = (size_t)(N / 8) * N, blocks_row = (size_t)N * N / 4;
+    c->part_len = std::max<size_t>(std::max(blocks_col * 6, (size_t)256 * 4), (size_t)128 * maxnumbins(N) * 4);
+    c->sums_len = (size_t)maxnumbins(N) * 4 + 64;
+    if (!alloc(&c->Pf, arr) || !alloc(&c->Pp, arr) || !alloc(&c->W, arr) ||
+        !alloc(&c->tw, (size_t)N * c->es) || !alloc((void**)&c->ms, sizeof(MemberState) * c->B) ||
+        !alloc((void**)&c->partV, sizeof(double) * blocks_col * c->B) ||
+        !alloc((void**)&c->partKG, sizeof(double) * 2 * blocks_row * c->B) ||
+        !alloc((void**)&c->part, sizeof(double) * c->part_len) ||
+        !alloc((void**)&c->sums, sizeof(double) * c->sums_len)) {
+        psc_destroy(c.release());
+        return PSC_ERR_NOMEM;
+    }
+    // Twiddles exp(-2 pi i m / N), m < N, rounded once from long double.
+    {
+        std::vector<double> twd(2 * N);
+        for (int m = 0; m < N; ++m) {
+            long double ang = -2.0L * 3.141592653589793238462643383279502884L * m / N;
+            twd[2 * m] = (double)cosl(ang);
+            twd[2 * m + 1] = (double)sinl(ang);
+        }
+        if (cfg->precision == PSC_FP32) {
+            std::vector<float> twf(2 * N);
+            for (int i = 0; i < 2 * N; ++i) twf[i] = (float)twd[i];
+            if (hipMemcpy(c->tw, twf.data(), twf.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess) {
+                psc_destroy(c.release());
+                return PSC_ERR_DEVICE;
+            }
+        } else if (hipMemcpy(c->tw, twd.data(), twd.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
+            psc_destroy(c.release());
+            return PSC_ERR_DEVICE;
+        }
+    }
+    std::vector<MemberState> init(c->B);
+    for (auto& s : init) {
+        std::memset(&s, 0, sizeof(s));
+        s.a = s.a_start = s.a_half = 1.0;
+        s.m2 = 1.0;
+    }
+    if (hipMemcpy(c->ms, init.data(), sizeof(MemberState) * c->B, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemset(c->Pf, 0, arr) != hipSuccess || hipMemset(c->Pp, 0, arr) != hipSuccess ||
+        hipMemset(c->W, 0, arr) != hipSuccess) {
+        psc_destroy(c.release());
+        return PSC_ERR_DEVICE;
+    }
+    *out = c.release();
+    return PSC_OK;
+}
+
+int psc_destroy(psc_ctx* c) {
+    if (!c) return PSC_OK;
+    if (c->stream) hipStreamSynchronize(c->stream);
+    if (c->g_mid) hipGraphExecDestroy(c->g_mid);
+    if (c->g_last) hipGraphExecDestroy(c->g_last);
+    for (auto& p : c->pending) {
+        hipEventDestroy(p.a);
+        hipEventDestroy(p.b);
+    }
+    for (auto e : c->ev_free) hipEventDestroy(e);
+    void* ptrs[] = {c->Pf, c->Pp, c->W, c->tw, c->ms, c->partV, c->partKG, c->part, c->sums};
+    for (void* p : ptrs)
+        if (p) hipFree(p);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c->eng;
+    delete c;
+    return PSC_OK;
+}
+
+const char* psc_last_error(const psc_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int psc_set_model(psc_ctx* c, int member, const psc_model* m) {
+    int rc = check_member(c, member);
+    if (rc) return rc;
+    if (!m) return c->fail(PSC_ERR_ARG, "null model");
+    c->models[member] = *m;
+    MemberState s;
+    if ((rc = read_state(c, member, &s))) return rc;
+    s.m2 = m->m2;
+    s.g2 = m->g2;
+    return write_state(c, member, &s);
+}
+
+int psc_load_ic_k(psc_ctx* c, int member, const double* f_k, const double* pi_k, double a, double ad) {
+    int rc = check_member(c, member);
+    if (rc) return rc;
+    if (!f_k || !pi_k || !(a > 0)) return c->fail(PSC_ERR_ARG, "bad initial data");
+    std::vector<double> full;
+    pack_full(c->N, c->cfg.nflds, f_k, full);
+    if ((rc = upload(c, c->Pf, member, full))) return rc;
+    pack_full(c->N, c->cfg.nflds, pi_k, full);
+    if ((rc = upload(c, c->Pp, member, full))) return rc;
+    MemberState s;
+    std::memset(&s, 0, sizeof(s));
+    s.t = 0.0;
+    s.a = s.a_start = s.a_half = a;
+    s.ad = s.b = (c->cfg.expansion == PSC_EXPANSION_STATIC) ? 0.0 : ad;
+    s.m2 = c->models[member].m2;
+    s.g2 = c->models[member].g2;
+    if ((rc = write_state(c, member, &s))) return rc;
+    Diag d;
+    if ((rc = diagnose(c, member, s, &d))) return rc;
+    s.add = kernel_a_value(c, d, a);  // initialize.cpp:797
+    if ((rc = write_state(c, member, &s))) return rc;
+    c->loaded[member] = 1;
+    return PSC_OK;
+}
+
+int psc_set_scalars(psc_ctx* c, int member, double t, double a, double ad, double add) {
+    int rc = check_member(c, member);
+    if (rc) return rc;
+    MemberState s;
+    if ((rc = read_state(c, member, &s))) return rc;
+    s.t = t;
+    s.a = s.a_start = s.a_half = a;
+    s.ad = s.b = ad;
+    s.add = add;
+    return write_state(c, member, &s);
+}
+
+int psc_get_scalars(psc_ctx* c, int member, double* out) {
+    int rc = check_member(c, member);
+    if (rc) return rc;
+    MemberState s;
+    if ((rc = read_state(c, member, &s))) return rc;
+    out[0] = s.t;
+    out[1] = s.a;
+    out[2] = s.ad;
+    out[3] = s.add;
+    if (s.a < 0) return c->fail(PSC_ERR_NEGATIVE_A, "scale factor is negative");
+    return PSC_OK;
+}
+
+int psc_step(psc_ctx* c, int nsteps) {
+    if (!c) return PSC_ERR_ARG;
+    if (nsteps < 0) return c->fail(PSC_ERR_ARG, "negative step count");
+    for (int b = 0; b < c->B; ++b)
+        if (!c->loaded[b]) return c->fail(PSC_ERR_STATE, "member " + std::to_string(b) + " has no initial data");
+    int rc = c->eng->step(c, nsteps);
+    if (rc) return rc;
+    HIPCHK(c, hipGetLastError());
+    return PSC_OK;
+}
+
+int psc_sync(psc_ctx* c) {
+    if (!c) return PSC_ERR_ARG;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return PSC_OK;
+}
+
+int psc_get_k(psc_ctx* c, int member, double* f_k, double* pi_k) {
+    int rc = check_member(c, member);
+    if (rc) return rc;
+    std::vector<double> full;
+    if (f_k) {
+        if ((rc = download(c, c->Pf, member, full))) return rc;
+        unpack_half(c->N, c->cfg.nflds, full.data(), f_k);
+    }
+    if (pi_k) {
+        if ((rc = download(c, c->Pp, member, full))) return rc;
+        unpack_half(c->N, c->cfg.nflds, full.data(), pi_k);
+    }
+    return PSC_OK;
+}
+
+int psc_get_fields(psc_ctx* c, int member, double* f, double* pi) {
+    int rc = check_member(c, member);
+    if (rc) return rc;
+    std::vector<double> full;
+    for (int which = 0; which < 2; ++which) {
+        double* out = which == 0 ? f : pi;
+        if (!out) continue;
+        if ((rc = c->eng->real_space(c, member, which == 0 ? c->Pf : c->Pp))) return rc;
+        if ((rc = download(c, c->W, member, full))) return rc;
+        for (size_t i = 0; i < c->N3; ++i) {
+            out[i] = full[2 * i];
+            if (c->cfg.nflds > 1) out[c->N3 + i] = full[2 * i + 1];
+        }
+    }
+    return PSC_OK;
+}
+
+int psc_energies(psc_ctx* c, int member, double* out) {
+    int rc = check_member(c, member);
+    if (rc) return rc;
+    MemberState s;
+    if ((rc = read_state(c, member, &s))) return rc;
+    Diag d;
+    if ((rc = diagnose(c, member, s, &d))) return rc;
+    for (int i = 0; i < PSC_ENERGY_LEN; ++i) out[i] = 0.0;
+    double total = 0.0;
+    for (int q = 0; q < c->cfg.nflds; ++q) {
+        out[2 * q] = d.K[q];
+        out[2 * q + 1] = d.G[q];
+        total += d.K[q] + d.G[q];
+    }
+    const int nterms = c->cfg.nflds == 2 ? 2 : 1;
+    for (int t = 0; t < nterms; ++t) {
+        out[4 + t] = d.Vt[t];
+        total += d.Vt[t];
+    }
+    out[6] = total;
+    const int al = c->cfg.alpha;
+    const double f2m = (c->cfg.fStar / c->cfg.MPl) * (c->cfg.fStar / c->cfg.MPl);
+    const double LHS = s.ad * s.ad, RHS = f2m * std::pow(s.a, 2 * (al + 1)) * total / 3.0;
+    out[7] = std::fabs(LHS - RHS) / std::fabs(LHS + RHS);  // output.cpp:1180-1184
+    out[8] = s.a;
+    out[9] = s.ad;
+    out[10] = s.add;
+    out[11] = s.t;
+    out[12] = d.mean[0];
+    out[13] = d.mean[1];
+    out[14] = d.var[0];
+    out[15] = d.var[1];
+    for (int q = 0; q < c->cfg.nflds; ++q)  // output.cpp:450-454
+        if (!std::isfinite(d.mean[q]))
+            return c->fail(PSC_ERR_NONFINITE, "field " + std::to_string(q) + " not numerical at t = " + std::to_string(s.t));
+    if (s.a < 0) return c->fail(PSC_ERR_NEGATIVE_A, "a is negative at t = " + std::to_string(s.t));  // output.cpp:480-483
+    return PSC_OK;
+}
+
+int psc_spectra(psc_ctx* c, int member, int fld, double* out, int nbins) {
+    int rc = check_member(c, member);
+    if (rc) return rc;
+    const int mb = maxnumbins(c->N);
+    if (fld < 0 || fld >= c->cfg.nflds || nbins != mb - 1 || !out) return c->fail(PSC_ERR_ARG, "bad spectra arguments");
+    MemberState s;
+    if ((rc = read_state(c, member, &s))) return rc;
+    // <m_eff^2> of the field (effective_mass, model.h:96-98, averaged as in
+    // output.cpp:597-608): m2 + g2 <chi^2> for phi, g2 <phi^2> for chi.
+    if ((rc = c->eng->real_space(c, member, c->Pf))) return rc;
+    double rs[6];
+    HIPCHK(c, hipMemcpyAsync(rs, c->sums, sizeof(rs), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    const double N3 = (double)c->N3;
+    const double msq = c->cfg.nflds == 1 ? s.m2
+                                         : (fld == 0 ? s.m2 + s.g2 * rs[3] / N3 : s.g2 * rs[2] / N3);
+    if ((rc = c->eng->spectra(c, member, fld))) return rc;
+    std::vector<double> h((size_t)mb * 4);
+    HIPCHK(c, hipMemcpyAsync(h.data(), c->sums, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    const double a = s.a, kIR = c->cfg.kIR, L = 2.0 * 3.14159265358979323846 / kIR;
+    const double aop = std::pow(a, c->cfg.alpha - 3);
+    for (int i = 1; i < mb; ++i) {
+        const double np = h[(size_t)i * 4];
+        double f2 = h[(size_t)i * 4 + 1], p2 = h[(size_t)i * 4 + 2], n2 = h[(size_t)i * 4 + 3];
+        if (np > 0) {
+            f2 /= np;
+            p2 /= np;
+            n2 /= np;
+        }
+        const double p = kIR * i;
+        const double om2 = std::fabs(p * p + a * a * msq);
+        double nd = 0.0;
+        if (om2 > 0) {
+            const double om = std::sqrt(om2);
+            nd = a * a * std::pow(L, 3) / 2.0 / (N3 * N3) *
+                 (c->cfg.fStar / c->cfg.omegaStar) * (c->cfg.fStar / c->cfg.omegaStar) *
+                 (om * f2 + std::pow(a, 2 * (1 - c->cfg.alpha)) / om * n2);
+        }
+        double* row = out + (size_t)(i - 1) * PSC_SPECTRA_COLS;
+        row[0] = p;
+        row[1] = i / (N3 * N3) * np * f2;
+        row[2] = i / (N3 * N3) * np * aop * aop * p2;
+        row[3] = nd;
+        row[4] = np;
+    }
+    return PSC_OK;
+}
+
+int psc_load_gw_k(psc_ctx* c, int, const double*, const double*) {
+    return c ? c->fail(PSC_ERR_UNSUPPORTED, "GW evolution not enabled in this context") : PSC_ERR_ARG;
+}
+int psc_get_gw_k(psc_ctx* c, int, double*, double*) {
+    return c ? c->fail(PSC_ERR_UNSUPPORTED, "GW evolution not enabled in this context") : PSC_ERR_ARG;
+}
+int psc_gw_spectrum(psc_ctx* c, int, int, double*, int, double*) {
+    return c ? c->fail(PSC_ERR_UNSUPPORTED, "GW evolution not enabled in this context") : PSC_ERR_ARG;
+}
+
+int psc_set_kernel_timing(psc_ctx* c, int on) {
+    if (!c) return PSC_ERR_ARG;
+    c->timing = on ? 1 : 0;
+    return PSC_OK;
+}
+
+int psc_reset_kernel_times(psc_ctx* c) {
+    if (!c) return PSC_ERR_ARG;
+    int rc = drain_timing(c);
+    for (auto& k : c->kt) k = Timing();
+    return rc;
+}
+
+int psc_get_kernel_times(psc_ctx* c, char* names, int names_len, double* total_ms, long long* counts,
+                         double* bytes, int cap, int* nkinds) {
+    if (!c) return PSC_ERR_ARG;
+    int rc = drain_timing(c);
+    if (rc) return rc;
+    std::string s;
+    int n = 0;
+    for (int k = 0; k < K_NKINDS && n < cap; ++k, ++n) {
+        if (k) s += ",";
+        s += kKindNames[k];
+        if (total_ms) total_ms[n] = c->kt[k].ms;
+        if (counts) counts[n] = c->kt[k].count;
+        if (bytes) bytes[n] = c->kt[k].bytes;
+    }
+    if (names && names_len > 0) {
+        std::strncpy(names, s.c_str(), names_len - 1);
+        names[names_len - 1] = 0;
+    }
+    if (nkinds) *nkinds = n;
+    return PSC_OK;
+}
+
+int psc_device_bytes(psc_ctx* c, double* bytes) {
+    if (!c || !bytes) return PSC_ERR_ARG;
+    *bytes = c->dev_bytes;
+    return PSC_OK;
+}
+
+}  // extern "C"

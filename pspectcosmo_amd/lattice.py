@@ -1,0 +1,220 @@
+"""Python binding of libpsc.so (include/psc.h) over ctypes.
+
+The HIP library is the only compute path: importing this module raises if the
+built library is missing, and there is no CPU fallback.  Arrays crossing the
+boundary are numpy FP64 in the reference's layouts (half spectra
+[nflds, N, N, N/2+1] complex; real fields [nflds, N, N, N])."""
+import ctypes as C
+import os
+
+import numpy as np
+
+from .abi import (PscConfig, PscModel, PscIcParams, PSC_ENERGY_LEN, PSC_SPECTRA_COLS,
+                  PSC_GW_SPECTRA_COLS, num_bins)
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libpsc.so")
+_lib = None
+_D = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+
+
+class PscError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"psc error {code}: {msg}")
+        self.code = code
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is not built; run __graft_entry__.build() "
+                              "or python pspectcosmo_amd/build.py")
+        L = C.CDLL(LIB_PATH)
+        P = C.POINTER
+        vp = C.c_void_p
+        L.psc_abi_version.restype = C.c_int
+        L.psc_status_string.restype = C.c_char_p
+        L.psc_status_string.argtypes = [C.c_int]
+        L.psc_num_bins.argtypes = [C.c_int]
+        L.psc_generate_ic.argtypes = [P(PscConfig), P(PscModel), P(PscIcParams), _D, _D, _D]
+        L.psc_generate_gw_ic.argtypes = [P(PscConfig), P(PscModel), P(PscIcParams), _D, _D]
+        L.psc_create.argtypes = [P(PscConfig), P(vp)]
+        L.psc_destroy.argtypes = [vp]
+        L.psc_last_error.restype = C.c_char_p
+        L.psc_last_error.argtypes = [vp]
+        L.psc_set_model.argtypes = [vp, C.c_int, P(PscModel)]
+        L.psc_load_ic_k.argtypes = [vp, C.c_int, _D, _D, C.c_double, C.c_double]
+        L.psc_set_scalars.argtypes = [vp, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double]
+        L.psc_get_scalars.argtypes = [vp, C.c_int, _D]
+        L.psc_step.argtypes = [vp, C.c_int]
+        L.psc_sync.argtypes = [vp]
+        L.psc_get_k.argtypes = [vp, C.c_int, _D, _D]
+        L.psc_get_fields.argtypes = [vp, C.c_int, _D, _D]
+        L.psc_energies.argtypes = [vp, C.c_int, _D]
+        L.psc_spectra.argtypes = [vp, C.c_int, C.c_int, _D, C.c_int]
+        L.psc_load_gw_k.argtypes = [vp, C.c_int, _D, _D]
+        L.psc_get_gw_k.argtypes = [vp, C.c_int, _D, _D]
+        L.psc_gw_spectrum.argtypes = [vp, C.c_int, C.c_int, _D, C.c_int, _D]
+        L.psc_set_kernel_timing.argtypes = [vp, C.c_int]
+        L.psc_reset_kernel_times.argtypes = [vp]
+        L.psc_get_kernel_times.argtypes = [vp, C.c_char_p, C.c_int, _D, P(C.c_longlong), _D,
+                                           C.c_int, P(C.c_int)]
+        L.psc_device_bytes.argtypes = [vp, _D]
+        _lib = L
+    return _lib
+
+
+def exported_symbols():
+    """Names declared in include/psc.h (checked against the built library)."""
+    return ["psc_abi_version", "psc_status_string", "psc_generate_ic", "psc_generate_gw_ic",
+            "psc_create", "psc_destroy", "psc_last_error", "psc_set_model", "psc_load_ic_k",
+            "psc_set_scalars", "psc_get_scalars", "psc_step", "psc_sync", "psc_get_k",
+            "psc_get_fields", "psc_energies", "psc_spectra", "psc_load_gw_k", "psc_get_gw_k",
+            "psc_gw_spectrum", "psc_num_bins", "psc_set_kernel_timing", "psc_get_kernel_times",
+            "psc_reset_kernel_times", "psc_device_bytes"]
+
+
+def generate_ic(cfg, model, ic):
+    """Host IC generator of the library (initialize.cpp:562-810): half spectra
+    and (a, ad, add_printed, H)."""
+    N, nf = cfg.N, cfg.nflds
+    fk = np.zeros((nf, N, N, N // 2 + 1), dtype=np.complex128)
+    pk = np.zeros_like(fk)
+    sc = np.zeros(4)
+    rc = lib().psc_generate_ic(C.byref(cfg), C.byref(model), C.byref(ic),
+                               fk.view(np.float64).ravel(), pk.view(np.float64).ravel(), sc)
+    if rc:
+        raise PscError(rc, lib().psc_status_string(rc).decode())
+    return fk, pk, sc
+
+
+def generate_gw_ic(cfg, model, ic):
+    N = cfg.N
+    u = np.zeros((6, N, N, N // 2 + 1), dtype=np.complex128)
+    pu = np.zeros_like(u)
+    rc = lib().psc_generate_gw_ic(C.byref(cfg), C.byref(model), C.byref(ic),
+                                  u.view(np.float64).ravel(), pu.view(np.float64).ravel())
+    if rc:
+        raise PscError(rc, lib().psc_status_string(rc).decode())
+    return u, pu
+
+
+class Lattice:
+    """A device context holding cfg.batch lattices (psc_ctx)."""
+
+    def __init__(self, cfg):
+        self.cfg = cfg
+        self.N, self.nf, self.B = cfg.N, cfg.nflds, cfg.batch
+        h = C.c_void_p()
+        rc = lib().psc_create(C.byref(cfg), C.byref(h))
+        if rc:
+            raise PscError(rc, "psc_create: " + lib().psc_status_string(rc).decode())
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().psc_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def _check(self, rc):
+        if rc:
+            raise PscError(rc, lib().psc_last_error(self._h).decode())
+
+    def set_model(self, member, model):
+        self._check(lib().psc_set_model(self._h, member, C.byref(model)))
+
+    def load_ic_k(self, member, fk, pk, a, ad):
+        fk = np.ascontiguousarray(fk, dtype=np.complex128)
+        pk = np.ascontiguousarray(pk, dtype=np.complex128)
+        self._check(lib().psc_load_ic_k(self._h, member, fk.view(np.float64).ravel(),
+                                        pk.view(np.float64).ravel(), a, ad))
+
+    def set_scalars(self, member, t, a, ad, add):
+        self._check(lib().psc_set_scalars(self._h, member, t, a, ad, add))
+
+    def scalars(self, member=0):
+        s = np.zeros(4)
+        self._check(lib().psc_get_scalars(self._h, member, s))
+        return s
+
+    def step(self, n=1):
+        self._check(lib().psc_step(self._h, int(n)))
+
+    def sync(self):
+        self._check(lib().psc_sync(self._h))
+
+    def get_k(self, member=0):
+        N = self.N
+        fk = np.zeros((self.nf, N, N, N // 2 + 1), dtype=np.complex128)
+        pk = np.zeros_like(fk)
+        self._check(lib().psc_get_k(self._h, member, fk.view(np.float64).ravel(),
+                                    pk.view(np.float64).ravel()))
+        return fk, pk
+
+    def fields(self, member=0):
+        N = self.N
+        f = np.zeros((self.nf, N, N, N))
+        p = np.zeros_like(f)
+        self._check(lib().psc_get_fields(self._h, member, f.ravel(), p.ravel()))
+        return f, p
+
+    def energies(self, member=0):
+        e = np.zeros(PSC_ENERGY_LEN)
+        self._check(lib().psc_energies(self._h, member, e))
+        return e
+
+    def spectra(self, fld, member=0):
+        nb = num_bins(self.N)
+        out = np.zeros((nb, PSC_SPECTRA_COLS))
+        self._check(lib().psc_spectra(self._h, member, fld, out.ravel(), nb))
+        return out
+
+    def load_gw_k(self, member, u, pu):
+        u = np.ascontiguousarray(u, dtype=np.complex128)
+        pu = np.ascontiguousarray(pu, dtype=np.complex128)
+        self._check(lib().psc_load_gw_k(self._h, member, u.view(np.float64).ravel(),
+                                        pu.view(np.float64).ravel()))
+
+    def get_gw_k(self, member=0):
+        N = self.N
+        u = np.zeros((6, N, N, N // 2 + 1), dtype=np.complex128)
+        pu = np.zeros_like(u)
+        self._check(lib().psc_get_gw_k(self._h, member, u.view(np.float64).ravel(),
+                                       pu.view(np.float64).ravel()))
+        return u, pu
+
+    def gw_spectrum(self, projection, member=0):
+        nb = num_bins(self.N)
+        out = np.zeros((nb, PSC_GW_SPECTRA_COLS))
+        tot = np.zeros(2)
+        self._check(lib().psc_gw_spectrum(self._h, member, projection, out.ravel(), nb, tot))
+        return out, tot
+
+    def set_kernel_timing(self, on):
+        self._check(lib().psc_set_kernel_timing(self._h, 1 if on else 0))
+
+    def reset_kernel_times(self):
+        self._check(lib().psc_reset_kernel_times(self._h))
+
+    def kernel_times(self):
+        cap = 32
+        names = C.create_string_buffer(2048)
+        ms = np.zeros(cap)
+        bytes_ = np.zeros(cap)
+        counts = (C.c_longlong * cap)()
+        n = C.c_int()
+        self._check(lib().psc_get_kernel_times(self._h, names, 2048, ms, counts, bytes_, cap,
+                                               C.byref(n)))
+        out = {}
+        for i, name in enumerate(names.value.decode().split(",")[: n.value]):
+            out[name] = {"ms": float(ms[i]), "count": int(counts[i]), "bytes": float(bytes_[i])}
+        return out
+
+    def device_bytes(self):
+        b = np.zeros(1)
+        self._check(lib().psc_device_bytes(self._h, b))
+        return float(b[0])

@@ -1,0 +1,67 @@
+"""C ABI checks that need no GPU: the library loads, exports every function
+include/psc.h declares, and its host-side initial conditions equal the
+oracle's restatement of initialize() bit for bit."""
+import os
+import re
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from pspectcosmo_amd import lattice as L
+from pspectcosmo_amd.abi import make_config, make_model, make_ic, PscConfig, PscModel, PscIcParams
+from tests._util import parity_setup
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_functions():
+    text = open(os.path.join(ROOT, "include", "psc.h")).read()
+    return sorted(set(re.findall(r"\b(psc_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_header_and_binding_agree():
+    assert header_functions() == sorted(L.exported_symbols())
+
+
+def test_library_exports_every_header_symbol():
+    lib = C.CDLL(L.LIB_PATH)
+    missing = [s for s in header_functions() if not hasattr(lib, s)]
+    assert not missing, missing
+
+
+def test_struct_sizes_match_header():
+    # psc_config: 16 ints + 8 doubles; psc_model: 2 doubles; psc_ic_params: 5 doubles + 2 ints + 4 doubles
+    assert C.sizeof(PscConfig) == 16 * 4 + 8 * 8
+    assert C.sizeof(PscModel) == 16
+    assert C.sizeof(PscIcParams) == 5 * 8 + 2 * 4 + 4 * 8
+
+
+def test_abi_version():
+    assert L.lib().psc_abi_version() == 1
+    assert L.lib().psc_num_bins(64) == 56
+
+
+@pytest.mark.parametrize("N,nflds,seed", [(16, 2, 1), (32, 1, 7), (16, 2, 0)])
+def test_host_ic_matches_oracle_bitwise(N, nflds, seed):
+    cfg, model, ic = parity_setup(N=N, nflds=nflds, seed=seed)
+    fk, pk, sc = L.generate_ic(cfg, model, ic)
+    ofk, opk, osc = O.generate_ic(cfg, model, ic)
+    np.testing.assert_array_equal(fk, ofk)
+    np.testing.assert_array_equal(pk, opk)
+    np.testing.assert_array_equal(sc, osc[:4])
+
+
+def test_host_gw_ic_matches_oracle_bitwise():
+    cfg, model, ic = parity_setup(N=16, nflds=2, with_gw=1)
+    u, pu = L.generate_gw_ic(cfg, model, ic)
+    ou, opu = O.generate_gw_ic(cfg, model, ic)
+    np.testing.assert_array_equal(u, ou)
+    np.testing.assert_array_equal(pu, opu)
+
+
+def test_create_rejects_bad_config():
+    cfg = make_config(48)  # not a power of two
+    h = C.c_void_p()
+    assert L.lib().psc_create(C.byref(cfg), C.byref(h)) == 1
