@@ -1,0 +1,160 @@
+"""Pin the oracle's building blocks against known answers that do not come
+from the oracle itself: the Park-Miller KAT, the reference README's example
+output, brute-force DFTs and numpy's pocketfft, and closed forms."""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from pspectcosmo_amd.abi import make_config, make_model, make_ic, PSC_EXPANSION_SELF
+from tests._util import parity_setup
+
+PM_M = 2147483647
+
+
+def test_park_miller_minimal_standard_kat():
+    # Park & Miller (1988): from state 1, the 10000th state is 1043618065.
+    assert O.park_miller_iterate(1, 10000) == 1043618065
+
+
+def test_park_miller_schrage_matches_direct_modmul():
+    # Schrage's factorisation (initialize.cpp:25-26) equals 16807 x mod (2^31-1).
+    x = 123456789
+    for _ in range(1000):
+        y = O.park_miller_iterate(x, 1)
+        assert y == (16807 * x) % PM_M
+        x = y
+
+
+def test_rand_uniform_warmup_offset():
+    # The first call advances 99 times before returning (initialize.cpp:22-24):
+    # the first value is the 100th iterate of the seed.
+    d = O.rng_draws(7, 3)
+    for k in range(3):
+        assert d[k] == O.park_miller_iterate(7, 100 + k) / PM_M
+
+
+def test_rand_uniform_debug_seed():
+    # seed < 1 -> 0.33 for every draw (initialize.cpp:15).
+    assert np.all(O.rng_draws(0, 5) == 0.33)
+
+
+def test_readme_hubble_initialisation():
+    """README.md:114-126: phi^2 inflation, phi0 = 11 MPl, dphi0 = -0.815284e-6
+    (omegaStar = 1e-6): kinetic 0.332344, potential 60.5, a = 1, ad = 4.50305,
+    add = 19.7235 (initialize.cpp:649-670)."""
+    cfg = make_config(16, nflds=1, kIR=25.0, dt=1e-3, omegaStar=1e-6, fStar=1.0, MPl=1.0)
+    model = make_model(1.0, 0.0)
+    fd0 = -0.815284e-6 / 1e-6  # initderivs / fStar / omegaStar
+    ic = make_ic(f0=(11.0, 0.0), fd0=(fd0, 0.0), seed=1)
+    fk, pk, sc = O.generate_ic(cfg, model, ic)
+    assert abs(fd0 ** 2 / 2 - 0.332344) < 5e-7
+    assert abs(0.5 * 11.0 ** 2 - 60.5) == 0
+    assert sc[0] == 1.0
+    assert abs(sc[1] - 4.50305) < 5e-6
+    assert abs(sc[2] - 19.7235) < 5e-5
+
+
+def brute_dft3(x):
+    N = x.shape[0]
+    n = np.arange(N)
+    W = np.exp(-2j * np.pi * np.outer(n, n) / N)
+    return np.einsum("ai,bj,ck,ijk->abc", W, W, W, x)
+
+
+@pytest.mark.parametrize("N", [4, 8])
+def test_fft_r2c_brute_force(N):
+    x = np.random.default_rng(N).standard_normal((N, N, N))
+    ref = brute_dft3(x)[:, :, : N // 2 + 1]
+    assert np.abs(O.fft_r2c_3d(x) - ref).max() < 1e-12 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("N", [16, 32])
+def test_fft_r2c_c2r_vs_pocketfft(N):
+    x = np.random.default_rng(N).standard_normal((N, N, N))
+    k = O.fft_r2c_3d(x)
+    ref = np.fft.rfftn(x)
+    assert np.abs(k - ref).max() < 1e-14 * np.abs(ref).max()
+    back = O.fft_c2r_3d(ref)
+    assert np.abs(back - np.fft.irfftn(ref) * N ** 3).max() < 1e-12 * N ** 3
+
+
+def test_fft_closed_forms():
+    N = 16
+    x = np.full((N, N, N), 2.5)
+    k = O.fft_r2c_3d(x)
+    assert abs(k[0, 0, 0] - 2.5 * N ** 3) < 1e-9
+    assert np.abs(k).sum() - abs(k[0, 0, 0]) < 1e-9
+    # cos(2 pi (3x + 5y + 2z) / N) -> N^3/2 at (3, 5, 2) (and its conjugate).
+    g = np.indices((N, N, N))
+    x = np.cos(2 * np.pi * (3 * g[0] + 5 * g[1] + 2 * g[2]) / N)
+    k = O.fft_r2c_3d(x)
+    assert abs(k[3, 5, 2] - N ** 3 / 2) < 1e-9
+    k[3, 5, 2] = 0
+    assert np.abs(k).max() < 1e-9
+
+
+def test_fft_1d_vs_numpy():
+    x = np.random.default_rng(3).standard_normal(64) + 1j * np.random.default_rng(4).standard_normal(64)
+    assert np.abs(O.fft_1d(x, -1) - np.fft.fft(x)).max() < 1e-12
+    assert np.abs(O.fft_1d(x, +1) - np.fft.ifft(x) * 64).max() < 1e-12
+
+
+@pytest.mark.parametrize("nflds", [1, 2])
+def test_ic_hermitian_and_dc(nflds):
+    """Initial half spectra describe real fields: f(-k) = conj f(k) in the
+    kz = 0, N/2 planes, real corners, and the DC closed form
+    (initialize.cpp:752-756)."""
+    N = 16
+    cfg, model, ic = parity_setup(N=N, nflds=nflds)
+    fk, pk, sc = O.generate_ic(cfg, model, ic)
+    for arr in (fk, pk):
+        for kz in (0, N // 2):
+            pl = arr[:, :, :, kz]
+            mi = (-np.arange(N)) % N
+            np.testing.assert_array_equal(pl, np.conj(pl[:, mi][:, :, mi]))
+    N3 = N ** 3
+    for q in range(nflds):
+        assert fk[q, 0, 0, 0] == ic.f0[q] * N3
+        assert pk[q, 0, 0, 0] == ic.fd0[q] * N3
+    # The c2r of a Hermitian half spectrum is the real field whose r2c gives it back.
+    x = O.fft_c2r_3d(fk[0]) / N3
+    np.testing.assert_allclose(O.fft_r2c_3d(x), fk[0], atol=1e-12 * np.abs(fk[0]).max())
+
+
+def test_ic_shell_statistics():
+    """<|f_k|^2> = norm^2 / (omega a^2) with norm = (omegaStar/fStar)
+    (N/dx)^{3/2} / sqrt 2 (Rayleigh amplitudes, initialize.cpp:181-182)."""
+    N, kIR = 32, 25.0
+    cfg, model, ic = parity_setup(N=N, nflds=2, kIR=kIR, omega_star=1e-3, seed=11)
+    fk, pk, sc = O.generate_ic(cfg, model, ic)
+    L = 2 * math.pi / kIR
+    norm2 = (1e-3 * (N / (L / N)) ** 1.5 / math.sqrt(2)) ** 2
+    i = np.arange(N)
+    n = np.where(i <= N // 2, i, i - N)
+    kz = np.arange(N // 2 + 1)
+    p2 = kIR ** 2 * (n[:, None, None] ** 2 + n[None, :, None] ** 2 + kz[None, None, :] ** 2)
+    meff = [1.0, 2.5e5 * 1.0 ** 2]
+    for q in range(2):
+        om = np.sqrt(p2 + meff[q])
+        ratio = (np.abs(fk[q]) ** 2) * om / norm2
+        ratio = ratio[:, :, 1: N // 2]  # interior planes (independent modes)
+        m = ratio.mean()
+        assert abs(m - 1.0) < 5.0 / math.sqrt(ratio.size), m
+
+
+def test_pi_k_relation_of_modes():
+    """pi_k = a^{alpha-3} (-omega Im(l - r) - H f)  (initialize.cpp:196-197):
+    with H = 0 (no homogeneous energy) |pi_k|^2 averages omega^2 |f_k|^2."""
+    N, kIR = 16, 25.0
+    cfg, model, ic = parity_setup(N=N, nflds=1, kIR=kIR, omega_star=1e-3, f0=(0.0, 0.0),
+                                  fd0=(0.0, 0.0), seed=5, g2=0.0)
+    fk, pk, sc = O.generate_ic(cfg, model, ic)
+    assert sc[3] == 0.0
+    i = np.arange(N)
+    n = np.where(i <= N // 2, i, i - N)
+    kz = np.arange(N // 2 + 1)
+    om2 = kIR ** 2 * (n[:, None, None] ** 2 + n[None, :, None] ** 2 + kz[None, None, :] ** 2) + 1.0
+    r = (np.abs(pk[0]) ** 2 / om2)[:, :, 1:N // 2].mean() / (np.abs(fk[0]) ** 2)[:, :, 1:N // 2].mean()
+    assert abs(r - 1.0) < 0.1
