@@ -49,9 +49,9 @@ def device_sync(lat):
         pass
 
 
-def run_gpu(precision, fk, pk, sc, model, steps, warmup):
+def run_gpu(precision, fk, pk, sc, model, steps, warmup, graph=True):
     cfg = make_config(N, nflds=2, integrator=PSC_Y4, precision=precision, kIR=KIR, dt=DT,
-                      use_graph=1)
+                      use_graph=1 if graph else 0)
     lat = Lattice(cfg)
     lat.set_model(0, model)
     lat.load_ic_k(0, fk, pk, sc[0], sc[1])
@@ -116,6 +116,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--fp64-steps", type=int, default=None)
+    ap.add_argument("--no-graph", action="store_true", help="launch kernels one by one")
+    ap.add_argument("--fp32-only", action="store_true", help="profiling runs: FP32 path only")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs the lattice on a single GPU; --gpus "
@@ -129,20 +131,26 @@ def main():
     fk, pk, sc = generate_ic(cfg0, model, ic)
     print(f"[bench] ICs ready (256^3, seed 1)", file=sys.stderr, flush=True)
 
-    lat32, el32 = run_gpu(PSC_FP32, fk, pk, sc, model, args.steps, args.warmup)
+    graph = not args.no_graph
+    lat32, el32 = run_gpu(PSC_FP32, fk, pk, sc, model, args.steps, args.warmup, graph)
     v32 = N ** 3 * args.steps / el32
     print(f"[bench] fp32: {el32:.3f} s for {args.steps} steps", file=sys.stderr, flush=True)
+    if args.fp32_only:
+        lat32.close()
+        print(json.dumps({"metric": METRIC, "value": v32, "unit": "site-updates/s",
+                          "steps": args.steps, "profiling_run": True}))
+        return
     dom, achieved, avg_s, bpl, per_kernel = kernel_roofline(lat32, min(args.steps, 50))
     lat32.close()
     s64 = args.fp64_steps or args.steps
-    lat64, el64 = run_gpu(PSC_FP64, fk, pk, sc, model, s64, args.warmup)
+    lat64, el64 = run_gpu(PSC_FP64, fk, pk, sc, model, s64, args.warmup, graph)
     v64 = N ** 3 * s64 / el64
     _, _, _, _, per_kernel64 = kernel_roofline(lat64, min(s64, 20))
     lat64.close()
     print(f"[bench] fp64: {el64:.3f} s for {s64} steps", file=sys.stderr, flush=True)
 
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", "traffic.json")
+    tpath = os.path.join(ROOT, "profiles", "traffic_256_fp32.json")
     if os.path.exists(tpath):
         with open(tpath) as fh:
             t = json.load(fh)

@@ -28,6 +28,15 @@ def build(verbose=False):
         sys.stderr.write(r.stdout)
         raise RuntimeError("hipcc failed building libpsc.so")
     os.replace(tmp, LIB)
+    probe = os.path.join(HERE, "copy_probe")
+    cmd = [hipcc, "--offload-arch=gfx950", "-O3", "-Xclang", "-target-feature", "-Xclang",
+           "-packed-fp32-ops", "-Wno-unused-command-line-argument", "-o", probe + ".tmp",
+           os.path.join(CSRC, "copy_probe.hip")]
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout)
+        raise RuntimeError("hipcc failed building copy_probe")
+    os.replace(probe + ".tmp", probe)
     return LIB
 
 
