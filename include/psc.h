@@ -75,7 +75,9 @@ typedef struct psc_config {
     int yoshida_exact; /* 0: the reference's Yoshida digits (pspectcosmo.h:288);
                           1: the exact triple-jump weights                               */
     int use_graph;     /* capture each step in a hipGraph and replay it                   */
-    int reserved_i[6];
+    int autotune;      /* 1: adopt the launch shape cached by psc_autotune for this
+                          device / N / precision / batch / GW, if any                   */
+    int reserved_i[5];
     double kIR;        /* lattice IR wavenumber, L = 2 pi / kIR (parameters.h:28,32)       */
     double dt;         /* time step in program units (parameters.h:36)                     */
     double fStar;      /* field rescaling (parameters.h:133)                               */
@@ -177,8 +179,32 @@ int psc_get_gw_k(psc_ctx* ctx, int member, double* u_k, double* pi_u_k);
 int psc_gw_spectrum(psc_ctx* ctx, int member, int projection, double* out, int nbins,
                     double* totals);
 
+/* GW energy spectrum per logarithmic interval, Omega_GW(k) = (1/rho)
+ * d rho_GW / d ln k, in nlog bins of ln|n| over [0, ln(sqrt(3) N / 2)]
+ * (|n| = k / kIR); rho_GW normalised as in output.cpp:1056-1058.  Has no
+ * counterpart in the reference beyond that normalisation.  k_center and omega
+ * hold nlog values (nlog <= 1024). */
+int psc_gw_omega_log(psc_ctx* ctx, int member, int projection, int nlog, double* k_center,
+                     double* omega);
+
 /* Number of FP64 bins of psc_spectra for a lattice of N points per edge. */
 int psc_num_bins(int N);
+
+/* ---- launch-shape autotuner (no counterpart in the reference) ----------- */
+
+/* Times every candidate FFT launch shape (radix plan, z columns per block of
+ * the strided passes, lines per block of the contiguous passes) for nsteps
+ * steps on the loaded state, keeps the fastest, restores the state, and
+ * appends "key shape milliseconds" to the cache file named by the environment
+ * variable PSC_TUNE_CACHE (default $HOME/.cache/pspectcosmo_amd/tune.txt).
+ * report (may be NULL) receives "shape=ms;..." for every candidate. */
+int psc_autotune(psc_ctx* ctx, int nsteps, char* report, int report_len);
+/* Name of the launch shape the context uses. */
+const char* psc_shape_name(const psc_ctx* ctx);
+/* Comma-separated names of the shapes available for the context's N and
+ * precision (the default first), and selection of one by name. */
+int psc_list_shapes(const psc_ctx* ctx, char* names, int names_len);
+int psc_set_shape(psc_ctx* ctx, const char* name);
 
 /* ---- instrumentation --------------------------------------------------- */
 

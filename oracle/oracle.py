@@ -56,6 +56,7 @@ def lib():
         L.orc_energies.argtypes = [C.c_void_p, _D]
         L.orc_spectra.argtypes = [C.c_void_p, C.c_int, _D, C.c_int]
         L.orc_gw_spectrum.argtypes = [C.c_void_p, C.c_int, _D, C.c_int, _D]
+        L.orc_gw_omega_log.argtypes = [C.c_void_p, C.c_int, C.c_int, _D, _D]
         _lib = L
     return _lib
 
@@ -205,6 +206,11 @@ class Oracle:
         assert lib().orc_gw_spectrum(self._h, projection, out.ravel(), nb, tot) == 0
         return out, tot
 
+    def gw_omega_log(self, projection, nlog):
+        kc = np.zeros(nlog)
+        om = np.zeros(nlog)
+        assert lib().orc_gw_omega_log(self._h, projection, nlog, kc, om) == 0
+        return kc, om
 
 __all__ = ["Oracle", "generate_ic", "generate_gw_ic", "fft_r2c_3d", "fft_c2r_3d", "fft_1d",
            "rng_draws", "park_miller_iterate", "set_threads", "half_len"]

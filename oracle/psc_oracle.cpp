@@ -1099,4 +1099,60 @@ int orc_gw_spectrum(orc_ctx* c, int projection, double* out, int nb, double* tot
     return 0;
 }
 
+// Omega_GW per logarithmic interval of |n| (nlog bins of ln|n| over
+// [0, ln(sqrt(3) N / 2)]) with rho_GW normalised as output.cpp:1056-1058.
+int orc_gw_omega_log(orc_ctx* c, int projection, int nlog, double* kc, double* om) {
+    if (!c->cfg.with_gw) return PSC_ERR_STATE;
+    const int N = c->N, Nh = c->Nh;
+    std::vector<double> acc(nlog, 0.0);
+    const double lmax = std::log(std::sqrt(3.0) * (N / 2));
+    const double kir = c->cfg.kIR;
+    visit_modes(N, false, [&](int i, int j, int k, long, bool isReal) {
+        const int wgt = isReal ? 1 : 2;
+        int px = signed_index(i, N), py = signed_index(j, N);
+        double kv[3] = {px * kir, py * kir, k * kir};
+        double kn = std::sqrt(kv[0] * kv[0] + kv[1] * kv[1] + kv[2] * kv[2]);
+        double P[3][3];
+        for (int x = 0; x < 3; ++x)
+            for (int y = 0; y < 3; ++y) P[x][y] = (x == y ? 1.0 : 0.0) - kv[x] / kn * kv[y] / kn;
+        size_t idx = ((size_t)i * N + j) * Nh + k;
+        cd M[3][3];
+        for (int c6 = 0; c6 < 6; ++c6) M[kGwI[c6]][kGwJ[c6]] = M[kGwJ[c6]][kGwI[c6]] = c->pi_u_k[c6][idx];
+        cd Pu[3][3];
+        for (int x = 0; x < 3; ++x)
+            for (int y = 0; y < 3; ++y) {
+                Pu[x][y] = 0;
+                for (int q = 0; q < 3; ++q) Pu[x][y] += P[x][q] * M[q][y];
+            }
+        cd tr = Pu[0][0] + Pu[1][1] + Pu[2][2];
+        double sv = 0;
+        if (projection == PSC_GW_PROJ_TT) {
+            for (int x = 0; x < 3; ++x)
+                for (int y = 0; y < 3; ++y) {
+                    cd a = 0;
+                    for (int q = 0; q < 3; ++q) a += Pu[x][q] * P[q][y];
+                    a -= 0.5 * P[x][y] * tr;
+                    sv += std::norm(a);
+                }
+        } else {
+            for (int x = 0; x < 3; ++x)
+                for (int y = 0; y < 3; ++y) sv += std::norm(Pu[x][y]);
+            sv -= std::norm(tr) / 2.0;
+        }
+        double nn = std::sqrt((double)(px * px + py * py + k * k));
+        int lb = (int)std::floor(std::log(nn) / lmax * nlog);
+        lb = lb < 0 ? 0 : (lb >= nlog ? nlog - 1 : lb);
+        acc[lb] += wgt * sv;
+    });
+    double energy[PSC_ENERGY_LEN];
+    orc_energies(c, energy);
+    const double total = energy[6], a = c->a;
+    const double f2m = pw2(c->cfg.fStar / c->cfg.MPl), dl = lmax / nlog;
+    for (int b = 0; b < nlog; ++b) {
+        kc[b] = kir * std::exp((b + 0.5) * dl);
+        om[b] = f2m / 4.0 / pw2(c->N3) / std::pow(a, 6) * acc[b] / total / dl;
+    }
+    return 0;
+}
+
 }  // extern "C"

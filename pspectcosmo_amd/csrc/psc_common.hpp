@@ -32,18 +32,19 @@ struct StepConsts {
     int pad;
 };
 
-// Geometry of the FFT passes for (T, N).
-template <typename T, int N>
-struct Geo {
-    static constexpr int TPL = Plan<N>::TPL;
-    static constexpr int E = N / TPL;            // elements per thread
-    static constexpr int R1 = N / TPL;           // first radix
-    // columns per block of the strided passes (lanes run along z)
-    static constexpr int TC = sizeof(T) == 4 ? (N >= 512 ? 16 : (N >= 32 ? 32 : N))
-                                             : (N >= 512 ? 8 : (N >= 128 ? 16 : (N >= 32 ? 32 : N)));
-    // lines per block of the contiguous (z) passes
-    static constexpr int L = sizeof(T) == 4 ? (N >= 512 ? 4 : (N >= 256 ? 16 : (N >= 64 ? 32 : 64)))
-                                            : (N >= 512 ? 4 : (N >= 256 ? 8 : (N >= 128 ? 16 : (N >= 64 ? 32 : 64))));
+// Launch shape of the FFT passes: precision T, lattice N, radix plan Rad
+// (first radix R1, TPL = N / R1 threads per line), TC consecutive z columns
+// per block in the strided passes, L lines per block in the contiguous passes.
+template <typename T_, int N_, class Rad_, int TC_, int L_>
+struct Shape {
+    using T = T_;
+    using Rad = Rad_;
+    static constexpr int N = N_;
+    static constexpr int R1 = first_radix(Rad{});
+    static constexpr int TPL = N / R1;
+    static constexpr int E = R1;                  // elements per thread
+    static constexpr int TC = TC_;
+    static constexpr int L = L_;
     static constexpr int THREADS_COL = TC * TPL;
     static constexpr int THREADS_ROW = L * TPL;
     static constexpr size_t N3 = (size_t)N * N * N;
@@ -51,6 +52,9 @@ struct Geo {
     static constexpr int BLOCKS_ROW = (N * N) / L;    // per member
     using LayRow = LayoutRow<N, R1>;
     using LayCol = LayoutCol<N, TC>;
+    static_assert(THREADS_COL <= 1024 && THREADS_ROW <= 1024, "block too large");
+    static_assert(LayCol::size(0) * sizeof(cpx<T>) <= 65536, "strided tile exceeds 64 KiB of LDS");
+    static_assert(LayRow::size(L) * sizeof(cpx<T>) <= 65536, "row tile exceeds 64 KiB of LDS");
 };
 
 __device__ __forceinline__ int signed_index(int i, int N) { return i <= N / 2 ? i : i - N; }

@@ -131,16 +131,9 @@ __device__ __forceinline__ void dft(cpx<T>* v) {
 template <int... Rs>
 struct Radices {};
 
-// Radix plan per line length.  TPL = N / R1: one butterfly per thread in the
-// first stage; later stages have N / (Rs * TPL) butterflies per thread.
-template <int N>
-struct Plan;
-template <> struct Plan<16> { using R = Radices<4, 4>; static constexpr int TPL = 4; };
-template <> struct Plan<32> { using R = Radices<8, 4>; static constexpr int TPL = 4; };
-template <> struct Plan<64> { using R = Radices<8, 8>; static constexpr int TPL = 8; };
-template <> struct Plan<128> { using R = Radices<16, 8>; static constexpr int TPL = 8; };
-template <> struct Plan<256> { using R = Radices<16, 16>; static constexpr int TPL = 16; };
-template <> struct Plan<512> { using R = Radices<8, 8, 8>; static constexpr int TPL = 64; };
+// Radix plans.  With first radix R1 a line is handled by TPL = N / R1 threads:
+// one butterfly per thread in the first stage; later stages (radix <= R1) give
+// each thread N / (R * TPL) butterflies.
 
 // LDS layout for lines whose TPL threads are adjacent lanes (contiguous-axis
 // pass): line c, element n -> c (N + N/P) + n + n/P (one pad slot every P
@@ -229,23 +222,27 @@ struct StageChain<T, N, TPL, SIGN, Lay, NS, FIRST, R, R2, Rest...> {
     }
 };
 
-template <typename T, int N, int SIGN, class Lay, class Rad>
+template <typename T, int N, int TPL, int SIGN, class Lay, class Rad>
 struct LineFftImpl;
-template <typename T, int N, int SIGN, class Lay, int... Rs>
-struct LineFftImpl<T, N, SIGN, Lay, Radices<Rs...>> {
+template <typename T, int N, int TPL, int SIGN, class Lay, int... Rs>
+struct LineFftImpl<T, N, TPL, SIGN, Lay, Radices<Rs...>> {
     static __device__ __forceinline__ void run(cpx<T>* e, int t, int c, cpx<T>* lds,
                                                const cpx<T>* tw) {
-        StageChain<T, N, Plan<N>::TPL, SIGN, Lay, 1, true, Rs...>::run(e, t, c, lds, tw);
+        StageChain<T, N, TPL, SIGN, Lay, 1, true, Rs...>::run(e, t, c, lds, tw);
     }
 };
 
-// Transform the line held as e[m] = x[t + TPL m]; result in the same layout.
-// SIGN = -1: forward (e^{-ikx}); +1: inverse, unnormalised.  tw[m] =
-// exp(-2 pi i m / N).
-template <typename T, int N, int SIGN, class Lay>
+template <int R, int... Rs>
+constexpr int first_radix(Radices<R, Rs...>) { return R; }
+
+// Transform the line held as e[m] = x[t + TPL m], TPL = N / first radix;
+// result in the same layout.  SIGN = -1: forward (e^{-ikx}); +1: inverse,
+// unnormalised.  tw[m] = exp(-2 pi i m / N).
+template <typename T, int N, class Rad, int SIGN, class Lay>
 __device__ __forceinline__ void line_fft(cpx<T>* e, int t, int c, cpx<T>* lds,
                                          const cpx<T>* __restrict__ tw) {
-    LineFftImpl<T, N, SIGN, Lay, typename Plan<N>::R>::run(e, t, c, lds, tw);
+    constexpr int TPL = N / first_radix(Rad{});
+    LineFftImpl<T, N, TPL, SIGN, Lay, Rad>::run(e, t, c, lds, tw);
 }
 
 }  // namespace psc

@@ -23,12 +23,13 @@ namespace psc {
 
 // ---------------------------------------------------------------------------
 // A: contiguous-axis inverse pass; DRIFT fuses f += c_drift * pi (evolution.cpp:1164-1169).
-template <typename T, int N, bool DRIFT>
-__global__ void __launch_bounds__((Geo<T, N>::THREADS_ROW))
-k_z_inverse(const cpx<T>* __restrict__ src, cpx<T>* __restrict__ Pf, const cpx<T>* __restrict__ Pp,
-            cpx<T>* __restrict__ W, const MemberState* __restrict__ ms,
-            const cpx<T>* __restrict__ tw) {
-    using G = Geo<T, N>;
+template <class G, bool DRIFT>
+__global__ void __launch_bounds__((G::THREADS_ROW))
+k_z_inverse(const cpx<typename G::T>* __restrict__ src, cpx<typename G::T>* __restrict__ Pf, const cpx<typename G::T>* __restrict__ Pp,
+            cpx<typename G::T>* __restrict__ W, const MemberState* __restrict__ ms,
+            const cpx<typename G::T>* __restrict__ tw) {
+    using T = typename G::T;
+    constexpr int N = G::N;
     __shared__ cpx<T> lds[G::LayRow::size(G::L)];
     const int t = threadIdx.x % G::TPL, c = threadIdx.x / G::TPL;
     const int member = blockIdx.y;
@@ -50,17 +51,18 @@ k_z_inverse(const cpx<T>* __restrict__ src, cpx<T>* __restrict__ Pf, const cpx<T
 #pragma unroll
         for (int m = 0; m < G::E; ++m) e[m] = src[base + G::TPL * m];
     }
-    line_fft<T, N, +1, typename G::LayRow>(e, t, c, lds, tw);
+    line_fft<T, N, typename G::Rad, +1, typename G::LayRow>(e, t, c, lds, tw);
 #pragma unroll
     for (int m = 0; m < G::E; ++m) W[base + G::TPL * m] = e[m];
 }
 
 // ---------------------------------------------------------------------------
 // B / D: strided-axis pass in place.  AXIS 1 = y (stride N), 0 = x (stride N^2).
-template <typename T, int N, int AXIS, int SIGN>
-__global__ void __launch_bounds__((Geo<T, N>::THREADS_COL))
-k_strided(cpx<T>* __restrict__ W, const cpx<T>* __restrict__ tw) {
-    using G = Geo<T, N>;
+template <class G, int AXIS, int SIGN>
+__global__ void __launch_bounds__((G::THREADS_COL))
+k_strided(cpx<typename G::T>* __restrict__ W, const cpx<typename G::T>* __restrict__ tw) {
+    using T = typename G::T;
+    constexpr int N = G::N;
     __shared__ cpx<T> lds[G::LayCol::size(0)];
     const int c = threadIdx.x % G::TC, t = threadIdx.x / G::TC;
     const int member = blockIdx.z;
@@ -71,7 +73,7 @@ k_strided(cpx<T>* __restrict__ W, const cpx<T>* __restrict__ tw) {
     cpx<T> e[G::E];
 #pragma unroll
     for (int m = 0; m < G::E; ++m) e[m] = W[base + (size_t)G::TPL * m * stride];
-    line_fft<T, N, SIGN, typename G::LayCol>(e, t, c, lds, tw);
+    line_fft<T, N, typename G::Rad, SIGN, typename G::LayCol>(e, t, c, lds, tw);
 #pragma unroll
     for (int m = 0; m < G::E; ++m) W[base + (size_t)G::TPL * m * stride] = e[m];
 }
@@ -80,11 +82,12 @@ k_strided(cpx<T>* __restrict__ W, const cpx<T>* __restrict__ tw) {
 // C: x-axis inverse, pointwise V'(phi, chi) of model.h (dvdf, model.h:75-82;
 // kernel_field_k_nopad evolution.cpp:1103-1109) and x-axis forward, with the
 // lattice sum of V (kernel_a_k_nopad evolution.cpp:1077-1085) in FP64.
-template <typename T, int N>
-__global__ void __launch_bounds__((Geo<T, N>::THREADS_COL))
-k_x_force(cpx<T>* __restrict__ W, const MemberState* __restrict__ ms, const cpx<T>* __restrict__ tw,
+template <class G>
+__global__ void __launch_bounds__((G::THREADS_COL))
+k_x_force(cpx<typename G::T>* __restrict__ W, const MemberState* __restrict__ ms, const cpx<typename G::T>* __restrict__ tw,
           double* __restrict__ partV) {
-    using G = Geo<T, N>;
+    using T = typename G::T;
+    constexpr int N = G::N;
     __shared__ cpx<T> lds[G::LayCol::size(0)];
     const int c = threadIdx.x % G::TC, t = threadIdx.x / G::TC;
     const int member = blockIdx.z;
@@ -95,7 +98,7 @@ k_x_force(cpx<T>* __restrict__ W, const MemberState* __restrict__ ms, const cpx<
     cpx<T> e[G::E];
 #pragma unroll
     for (int m = 0; m < G::E; ++m) e[m] = W[base + (size_t)G::TPL * m * stride];
-    line_fft<T, N, +1, typename G::LayCol>(e, t, c, lds, tw);
+    line_fft<T, N, typename G::Rad, +1, typename G::LayCol>(e, t, c, lds, tw);
     const T inv = (T)(1.0 / (double)G::N3);
     const T m2 = (T)ms[member].m2, g2 = (T)ms[member].g2;
     const double m2d = ms[member].m2, g2d = ms[member].g2;
@@ -109,7 +112,7 @@ k_x_force(cpx<T>* __restrict__ W, const MemberState* __restrict__ ms, const cpx<
         const double pd = phi, cdd = chi;
         v[0] += 0.5 * m2d * (pd * pd) + 0.5 * g2d * (pd * pd) * (cdd * cdd);
     }
-    line_fft<T, N, -1, typename G::LayCol>(e, t, c, lds, tw);
+    line_fft<T, N, typename G::Rad, -1, typename G::LayCol>(e, t, c, lds, tw);
 #pragma unroll
     for (int m = 0; m < G::E; ++m) W[base + (size_t)G::TPL * m * stride] = e[m];
     block_sum<1, G::THREADS_COL>(v, reinterpret_cast<double*>(lds));
@@ -123,12 +126,13 @@ k_x_force(cpx<T>* __restrict__ W, const MemberState* __restrict__ ms, const cpx<
 // unnormalised spectrum of V'.  c1/c2 are w dt / 2 of the two kicks applied
 // with this force; REDUCE accumulates sum |pi|^2 after the first kick and
 // sum n^2 |f|^2 (kinetic_energy_k / gradient_energy_k, evolution.cpp:923-1035).
-template <typename T, int N, bool REDUCE>
-__global__ void __launch_bounds__((Geo<T, N>::THREADS_ROW))
-k_z_kick(const cpx<T>* __restrict__ W, const cpx<T>* __restrict__ Pf, cpx<T>* __restrict__ Pp,
-         const MemberState* __restrict__ ms, const cpx<T>* __restrict__ tw,
+template <class G, bool REDUCE>
+__global__ void __launch_bounds__((G::THREADS_ROW))
+k_z_kick(const cpx<typename G::T>* __restrict__ W, const cpx<typename G::T>* __restrict__ Pf, cpx<typename G::T>* __restrict__ Pp,
+         const MemberState* __restrict__ ms, const cpx<typename G::T>* __restrict__ tw,
          double* __restrict__ partKG, double c1d, double c2d, int use_a_start, StepConsts sc) {
-    using G = Geo<T, N>;
+    using T = typename G::T;
+    constexpr int N = G::N;
     __shared__ cpx<T> lds[G::LayRow::size(G::L)];
     const int t = threadIdx.x % G::TPL, c = threadIdx.x / G::TPL;
     const int member = blockIdx.y;
@@ -137,7 +141,7 @@ k_z_kick(const cpx<T>* __restrict__ W, const cpx<T>* __restrict__ Pf, cpx<T>* __
     cpx<T> e[G::E];
 #pragma unroll
     for (int m = 0; m < G::E; ++m) e[m] = W[base + G::TPL * m];
-    line_fft<T, N, -1, typename G::LayRow>(e, t, c, lds, tw);
+    line_fft<T, N, typename G::Rad, -1, typename G::LayRow>(e, t, c, lds, tw);
     const double a = use_a_start ? ms[member].a_start : ms[member].a;
     const T lapc = (T)(pow(a, 1 + sc.alpha) * sc.kIR * sc.kIR);
     const T vc = (T)pow(a, 3 + sc.alpha);
@@ -190,35 +194,53 @@ struct ScalarOp {
     int nV, nKG;       // partial counts per member
 };
 
+// Merge of two compensated partial sums (Fast2Sum on the leading parts):
+// symmetric in its arguments, so both lanes of an xor-shuffle pair agree.
+__device__ __forceinline__ void comp_merge(double& s, double& c, double s2, double c2) {
+    const double t = s + s2;
+    const double err = fabs(s) >= fabs(s2) ? (s - t) + s2 : (s2 - t) + s;
+    s = t;
+    c = c + c2 + err;
+}
+
 __global__ void __launch_bounds__(256)
 k_scalars(MemberState* __restrict__ ms, const double* __restrict__ partV,
           const double* __restrict__ partKG, ScalarOp op, StepConsts sc) {
-    __shared__ double sh[3 * 2 * 256];
+    __shared__ double sh[3 * 2 * 4];
     const int member = blockIdx.x;
     MemberState& s = ms[member];
     if (op.finish) {
-        Comp cv, ck, cg;
-        for (int i = threadIdx.x; i < op.nV; i += blockDim.x) cv.add(partV[(size_t)member * op.nV + i]);
+        Comp acc[3];
+        for (int i = threadIdx.x; i < op.nV; i += blockDim.x) acc[0].add(partV[(size_t)member * op.nV + i]);
         for (int i = threadIdx.x; i < op.nKG; i += blockDim.x) {
-            ck.add(partKG[((size_t)member * op.nKG + i) * 2]);
-            cg.add(partKG[((size_t)member * op.nKG + i) * 2 + 1]);
+            acc[1].add(partKG[((size_t)member * op.nKG + i) * 2]);
+            acc[2].add(partKG[((size_t)member * op.nKG + i) * 2 + 1]);
         }
-        sh[threadIdx.x * 6 + 0] = cv.s; sh[threadIdx.x * 6 + 1] = cv.c;
-        sh[threadIdx.x * 6 + 2] = ck.s; sh[threadIdx.x * 6 + 3] = ck.c;
-        sh[threadIdx.x * 6 + 4] = cg.s; sh[threadIdx.x * 6 + 5] = cg.c;
+        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+#pragma unroll
+            for (int off = 32; off >= 1; off >>= 1) {
+                const double s2 = __shfl_xor(acc[q].s, off, 64), c2 = __shfl_xor(acc[q].c, off, 64);
+                comp_merge(acc[q].s, acc[q].c, s2, c2);
+            }
+            if (lane == 0) {
+                sh[(q * 4 + wave) * 2] = acc[q].s;
+                sh[(q * 4 + wave) * 2 + 1] = acc[q].c;
+            }
+        }
         __syncthreads();
         if (threadIdx.x == 0) {
-            Comp V, K, Gs;
-            for (int i = 0; i < (int)blockDim.x; ++i) {
-                Comp x;
-                x.s = sh[i * 6 + 0]; x.c = sh[i * 6 + 1]; V.merge(x);
-                x.s = sh[i * 6 + 2]; x.c = sh[i * 6 + 3]; K.merge(x);
-                x.s = sh[i * 6 + 4]; x.c = sh[i * 6 + 5]; Gs.merge(x);
+            double tot[3];
+            for (int q = 0; q < 3; ++q) {
+                double ss = sh[(q * 4) * 2], cc = sh[(q * 4) * 2 + 1];
+                for (int w = 1; w < 4; ++w) comp_merge(ss, cc, sh[(q * 4 + w) * 2], sh[(q * 4 + w) * 2 + 1]);
+                tot[q] = ss + cc;
             }
             const double a = s.a;
-            const double Kn = 0.5 * K.value() / pow(a, 6) / (sc.N3 * sc.N3);
-            const double Gn = sc.kIR * sc.kIR * Gs.value() / sc.N3 / 2.0 / (a * a) / sc.N3;
-            const double Vn = V.value() / sc.N3;
+            const double Kn = 0.5 * tot[1] / pow(a, 6) / (sc.N3 * sc.N3);
+            const double Gn = sc.kIR * sc.kIR * tot[2] / sc.N3 / 2.0 / (a * a) / sc.N3;
+            const double Vn = tot[0] / sc.N3;
             s.K = Kn;
             s.G = Gn;
             s.V = Vn;
@@ -248,11 +270,12 @@ k_scalars(MemberState* __restrict__ ms, const double* __restrict__ partV,
 // fields (phi, chi) back into W, and reduces sum phi, sum chi, sum phi^2,
 // sum chi^2, sum V_0, sum V_1 (meansvars output.cpp:427-447, potential terms
 // model.h:35-73).
-template <typename T, int N>
-__global__ void __launch_bounds__((Geo<T, N>::THREADS_COL))
-k_x_inverse_diag(cpx<T>* __restrict__ W, const MemberState* __restrict__ ms,
-                 const cpx<T>* __restrict__ tw, double* __restrict__ part) {
-    using G = Geo<T, N>;
+template <class G>
+__global__ void __launch_bounds__((G::THREADS_COL))
+k_x_inverse_diag(cpx<typename G::T>* __restrict__ W, const MemberState* __restrict__ ms,
+                 const cpx<typename G::T>* __restrict__ tw, double* __restrict__ part) {
+    using T = typename G::T;
+    constexpr int N = G::N;
     __shared__ cpx<T> lds[G::LayCol::size(0)];
     const int c = threadIdx.x % G::TC, t = threadIdx.x / G::TC;
     const int member = blockIdx.z;
@@ -263,7 +286,7 @@ k_x_inverse_diag(cpx<T>* __restrict__ W, const MemberState* __restrict__ ms,
     cpx<T> e[G::E];
 #pragma unroll
     for (int m = 0; m < G::E; ++m) e[m] = W[base + (size_t)G::TPL * m * stride];
-    line_fft<T, N, +1, typename G::LayCol>(e, t, c, lds, tw);
+    line_fft<T, N, typename G::Rad, +1, typename G::LayCol>(e, t, c, lds, tw);
     const double inv = 1.0 / (double)G::N3;
     const double m2 = ms[member].m2, g2 = ms[member].g2;
     double r[6] = {0, 0, 0, 0, 0, 0};
@@ -404,6 +427,332 @@ __global__ void k_convert_out(const cpx<T>* __restrict__ src, double* __restrict
         dst[2 * i] = (double)src[i].x;
         dst[2 * i + 1] = (double)src[i].y;
     }
+}
+
+}  // namespace psc
+
+// ===========================================================================
+// Gravitational waves.  The 6 symmetric components u_ij (11,22,33,12,13,23) are
+// packed in pairs U_0 = u11 + i u22, U_1 = u33 + i u12, U_2 = u13 + i u23 (and
+// likewise pi_u), so the GW kick of the real-space stepper (kernel_GW,
+// evolution.cpp:581-601, with lapl() evolution.cpp:343-372 as -k^2 u_k)
+//   pi_u += c a^{1+alpha} (-kIR^2 n^2 U + 2 S_k)
+// acts on packed spectra with real coefficients.  The source
+// S_ij = sum_f d_i f d_j f uses the spectral first derivative with the Nyquist
+// component zeroed (field_1st_derivative, evolution.cpp:466-578); the three
+// packed gradients d_d phi + i d_d chi ride through the same passes as the
+// fields in the work arrays W[1..3] (W[0] holds the fields / V').
+namespace psc {
+
+__device__ __forceinline__ int deriv_index(int i, int N) { return i == N / 2 ? 0 : signed_index(i, N); }
+
+// A (GW): drift of fields and u, then z-axis inverse of the fields and of the
+// three gradient spectra i kIR n_d P_f.
+template <class G, bool DRIFT>
+__global__ void __launch_bounds__((G::THREADS_ROW))
+k_z_inverse_gw(cpx<typename G::T>* __restrict__ Pf, const cpx<typename G::T>* __restrict__ Pp, cpx<typename G::T>* __restrict__ W,
+               cpx<typename G::T>* __restrict__ U, const cpx<typename G::T>* __restrict__ PU, const MemberState* __restrict__ ms,
+               const cpx<typename G::T>* __restrict__ tw, size_t astride, double kIR) {
+    using T = typename G::T;
+    constexpr int N = G::N;
+    __shared__ cpx<T> lds[G::LayRow::size(G::L)];
+    const int t = threadIdx.x % G::TPL, c = threadIdx.x / G::TPL;
+    const int member = blockIdx.y;
+    const size_t line = (size_t)blockIdx.x * G::L + c;
+    const size_t base = (size_t)member * G::N3 + line * N + t;
+    const int ix = (int)(line / N), iy = (int)(line % N);
+    const T kx = (T)(kIR * deriv_index(ix, N)), ky = (T)(kIR * deriv_index(iy, N));
+    const T cd = DRIFT ? (T)ms[member].c_drift : (T)0;
+    cpx<T> e[G::E];
+    cpx<T> f[G::E];
+#pragma unroll
+    for (int m = 0; m < G::E; ++m) {
+        cpx<T> v = Pf[base + G::TPL * m];
+        if constexpr (DRIFT) {
+            const cpx<T> p = Pp[base + G::TPL * m];
+            v.x += cd * p.x;
+            v.y += cd * p.y;
+            Pf[base + G::TPL * m] = v;
+        }
+        f[m] = v;
+        e[m] = v;
+    }
+    line_fft<T, N, typename G::Rad, +1, typename G::LayRow>(e, t, c, lds, tw);
+#pragma unroll
+    for (int m = 0; m < G::E; ++m) W[base + G::TPL * m] = e[m];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+#pragma unroll
+        for (int m = 0; m < G::E; ++m) {
+            const int kz = t + G::TPL * m;
+            const T kd = d == 0 ? kx : (d == 1 ? ky : (T)(kIR * deriv_index(kz, N)));
+            e[m] = cpx<T>{-kd * f[m].y, kd * f[m].x};
+        }
+        line_fft<T, N, typename G::Rad, +1, typename G::LayRow>(e, t, c, lds, tw);
+#pragma unroll
+        for (int m = 0; m < G::E; ++m) W[(size_t)(d + 1) * astride + base + G::TPL * m] = e[m];
+    }
+    if constexpr (DRIFT) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int m = 0; m < G::E; ++m) {
+                const size_t idx = (size_t)a * astride + base + G::TPL * m;
+                cpx<T> u = U[idx];
+                const cpx<T> p = PU[idx];
+                u.x += cd * p.x;
+                u.y += cd * p.y;
+                U[idx] = u;
+            }
+    }
+}
+
+// C (GW): x-axis inverse of fields and gradients, V' and the packed source
+// S = (S11 + i S22, S33 + i S12, S13 + i S23), x-axis forward of all four.
+template <class G>
+__global__ void __launch_bounds__((G::THREADS_COL))
+k_x_force_gw(cpx<typename G::T>* __restrict__ W, const MemberState* __restrict__ ms, const cpx<typename G::T>* __restrict__ tw,
+             double* __restrict__ partV, size_t astride) {
+    using T = typename G::T;
+    constexpr int N = G::N;
+    __shared__ cpx<T> lds[G::LayCol::size(0)];
+    const int c = threadIdx.x % G::TC, t = threadIdx.x / G::TC;
+    const int member = blockIdx.z;
+    const size_t z = (size_t)blockIdx.x * G::TC + c;
+    const size_t o = blockIdx.y;
+    constexpr size_t stride = (size_t)N * N;
+    const size_t base = (size_t)member * G::N3 + o * N + z + t * stride;
+    cpx<T> e[4][G::E];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+#pragma unroll
+        for (int m = 0; m < G::E; ++m) e[a][m] = W[(size_t)a * astride + base + (size_t)G::TPL * m * stride];
+        line_fft<T, N, typename G::Rad, +1, typename G::LayCol>(e[a], t, c, lds, tw);
+    }
+    const T inv = (T)(1.0 / (double)G::N3);
+    const T m2 = (T)ms[member].m2, g2 = (T)ms[member].g2;
+    const double m2d = ms[member].m2, g2d = ms[member].g2;
+    double v[1] = {0.0};
+#pragma unroll
+    for (int m = 0; m < G::E; ++m) {
+        const T phi = e[0][m].x * inv, chi = e[0][m].y * inv;
+        e[0][m].x = m2 * phi + g2 * phi * (chi * chi);
+        e[0][m].y = g2 * (phi * phi) * chi;
+        const double pd = phi, cdd = chi;
+        v[0] += 0.5 * m2d * (pd * pd) + 0.5 * g2d * (pd * pd) * (cdd * cdd);
+        const T ax = e[1][m].x * inv, bx = e[1][m].y * inv;
+        const T ay = e[2][m].x * inv, by = e[2][m].y * inv;
+        const T az = e[3][m].x * inv, bz = e[3][m].y * inv;
+        e[1][m] = cpx<T>{ax * ax + bx * bx, ay * ay + by * by};
+        e[2][m] = cpx<T>{az * az + bz * bz, ax * ay + bx * by};
+        e[3][m] = cpx<T>{ax * az + bx * bz, ay * az + by * bz};
+    }
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+        line_fft<T, N, typename G::Rad, -1, typename G::LayCol>(e[a], t, c, lds, tw);
+#pragma unroll
+        for (int m = 0; m < G::E; ++m) W[(size_t)a * astride + base + (size_t)G::TPL * m * stride] = e[a][m];
+    }
+    block_sum<1, G::THREADS_COL>(v, reinterpret_cast<double*>(lds));
+    if (threadIdx.x == 0)
+        partV[(size_t)member * G::BLOCKS_COL + blockIdx.y * gridDim.x + blockIdx.x] = v[0];
+}
+
+// E (GW): z-axis forward of V' and of the source, field kick as k_z_kick and
+// the GW kick of the three packed u arrays.
+template <class G, bool REDUCE>
+__global__ void __launch_bounds__((G::THREADS_ROW))
+k_z_kick_gw(const cpx<typename G::T>* __restrict__ W, const cpx<typename G::T>* __restrict__ Pf, cpx<typename G::T>* __restrict__ Pp,
+            const cpx<typename G::T>* __restrict__ U, cpx<typename G::T>* __restrict__ PU, const MemberState* __restrict__ ms,
+            const cpx<typename G::T>* __restrict__ tw, double* __restrict__ partKG, double c1d, double c2d,
+            int use_a_start, StepConsts sc, size_t astride) {
+    using T = typename G::T;
+    constexpr int N = G::N;
+    __shared__ cpx<T> lds[G::LayRow::size(G::L)];
+    const int t = threadIdx.x % G::TPL, c = threadIdx.x / G::TPL;
+    const int member = blockIdx.y;
+    const size_t line = (size_t)blockIdx.x * G::L + c;
+    const size_t base = (size_t)member * G::N3 + line * N + t;
+    const double a = use_a_start ? ms[member].a_start : ms[member].a;
+    const T norm1 = (T)pow(a, 1 + sc.alpha);
+    const T lapc = (T)(pow(a, 1 + sc.alpha) * sc.kIR * sc.kIR);
+    const T vc = (T)pow(a, 3 + sc.alpha);
+    const T k2c = (T)(sc.kIR * sc.kIR);
+    const T c1 = (T)c1d, c2 = (T)c2d;
+    const int ix = (int)(line / N), iy = (int)(line % N);
+    const int px = signed_index(ix, N), py = signed_index(iy, N);
+    const int pxy = px * px + py * py;
+    double red[2] = {0.0, 0.0};
+    cpx<T> e[G::E];
+#pragma unroll
+    for (int m = 0; m < G::E; ++m) e[m] = W[base + G::TPL * m];
+    line_fft<T, N, typename G::Rad, -1, typename G::LayRow>(e, t, c, lds, tw);
+#pragma unroll
+    for (int m = 0; m < G::E; ++m) {
+        const int pz = signed_index(t + G::TPL * m, N);
+        const T n2 = (T)(pxy + pz * pz);
+        const cpx<T> f = Pf[base + G::TPL * m];
+        cpx<T> p = Pp[base + G::TPL * m];
+        const T ln2 = lapc * n2;
+        const cpx<T> F = {-ln2 * f.x - vc * e[m].x, -ln2 * f.y - vc * e[m].y};
+        p.x += c1 * F.x;
+        p.y += c1 * F.y;
+        if constexpr (REDUCE) {
+            const double px_ = p.x, py_ = p.y, fx = f.x, fy = f.y;
+            red[0] += px_ * px_ + py_ * py_;
+            red[1] += (double)(pxy + pz * pz) * (fx * fx + fy * fy);
+        }
+        p.x += c2 * F.x;
+        p.y += c2 * F.y;
+        Pp[base + G::TPL * m] = p;
+    }
+#pragma unroll
+    for (int ga = 0; ga < 3; ++ga) {
+#pragma unroll
+        for (int m = 0; m < G::E; ++m) e[m] = W[(size_t)(ga + 1) * astride + base + G::TPL * m];
+        line_fft<T, N, typename G::Rad, -1, typename G::LayRow>(e, t, c, lds, tw);
+#pragma unroll
+        for (int m = 0; m < G::E; ++m) {
+            const int pz = signed_index(t + G::TPL * m, N);
+            const T k2 = k2c * (T)(pxy + pz * pz);
+            const size_t idx = (size_t)ga * astride + base + G::TPL * m;
+            const cpx<T> u = U[idx];
+            cpx<T> p = PU[idx];
+            const cpx<T> F = {norm1 * (-k2 * u.x + (T)2 * e[m].x), norm1 * (-k2 * u.y + (T)2 * e[m].y)};
+            p.x += c1 * F.x;
+            p.y += c1 * F.y;
+            p.x += c2 * F.x;
+            p.y += c2 * F.y;
+            PU[idx] = p;
+        }
+    }
+    if constexpr (REDUCE) {
+        block_sum<2, G::THREADS_ROW>(red, reinterpret_cast<double*>(lds));
+        if (threadIdx.x == 0) {
+            double* dst = partKG + ((size_t)member * G::BLOCKS_ROW + blockIdx.x) * 2;
+            dst[0] = red[0];
+            dst[1] = red[1];
+        }
+    }
+}
+
+// GW spectrum (output.cpp:780-1062): modes visited as the reference does (the
+// (+x) half of the kz = 0, N/2 planes and all interior kz with weight 2, real
+// corners with weight 1, no DC), components unpacked from the pairs, one-sided
+// P.u (projection 0) or Lambda u (projection 1).  Histograms: linear bins
+// l = floor(|n| + 0.5) of (numpoints, h2, hdot2) and nlog logarithmic bins of
+// hdot2 on ln|n| in [0, ln(sqrt(3) N / 2)].
+template <typename T, int N>
+__global__ void __launch_bounds__(256)
+k_gw_spectra(const cpx<T>* __restrict__ U, const cpx<T>* __restrict__ PU, size_t astride, int projection,
+             int nbins_total, int nlog, double* __restrict__ part, int blocks) {
+    extern __shared__ double hist[];  // 3 * nbins_total + nlog
+    constexpr int NH = N / 2 + 1;
+    const int width = 3 * nbins_total + nlog;
+    for (int i = threadIdx.x; i < width; i += blockDim.x) hist[i] = 0.0;
+    __syncthreads();
+    const size_t nk = (size_t)N * N * NH;
+    const double lmax = log(sqrt(3.0) * (N / 2));
+    for (size_t h = (size_t)blockIdx.x * blockDim.x + threadIdx.x; h < nk; h += (size_t)blocks * blockDim.x) {
+        const int i = (int)(h / ((size_t)N * NH)), j = (int)((h / NH) % N), k = (int)(h % NH);
+        double wgt;
+        if (k != 0 && k != N / 2) {
+            wgt = 2.0;
+        } else {
+            const bool iedge = (i == 0 || i == N / 2), jedge = (j == 0 || j == N / 2);
+            if (iedge && jedge) {
+                if (i == 0 && j == 0 && k == 0) continue;
+                wgt = 1.0;
+            } else if (iedge) {
+                if (!(j >= 1 && j < N / 2)) continue;
+                wgt = 2.0;
+            } else if (i >= 1 && i < N / 2) {
+                wgt = 2.0;
+            } else {
+                continue;
+            }
+        }
+        const size_t idx = ((size_t)i * N + j) * N + k;
+        const size_t midx = ((size_t)((N - i) % N) * N + (N - j) % N) * N + (N - k) % N;
+        double ur[6], ui[6], pr[6], pi_[6];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const cpx<T> A = U[a * astride + idx], B = U[a * astride + midx];
+            const double ar = A.x, ai = A.y, br = B.x, bi = -(double)B.y;
+            ur[2 * a] = 0.5 * (ar + br); ui[2 * a] = 0.5 * (ai + bi);
+            ur[2 * a + 1] = 0.5 * (ai - bi); ui[2 * a + 1] = -0.5 * (ar - br);
+            const cpx<T> C = PU[a * astride + idx], D = PU[a * astride + midx];
+            const double cr = C.x, ci = C.y, dr = D.x, di = -(double)D.y;
+            pr[2 * a] = 0.5 * (cr + dr); pi_[2 * a] = 0.5 * (ci + di);
+            pr[2 * a + 1] = 0.5 * (ci - di); pi_[2 * a + 1] = -0.5 * (cr - dr);
+        }
+        const int px = signed_index(i, N), py = signed_index(j, N), pz = k;
+        const double kn = sqrt((double)(px * px + py * py + pz * pz));
+        const double kh[3] = {px / kn, py / kn, pz / kn};
+        double P[3][3];
+#pragma unroll
+        for (int x = 0; x < 3; ++x)
+#pragma unroll
+            for (int y = 0; y < 3; ++y) P[x][y] = (x == y ? 1.0 : 0.0) - kh[x] * kh[y];
+        // component index of (x, y) in 11,22,33,12,13,23 order
+        const int cix[3][3] = {{0, 3, 4}, {3, 1, 5}, {4, 5, 2}};
+        double h2v[2];
+#pragma unroll
+        for (int which = 0; which < 2; ++which) {
+            const double* mr = which == 0 ? ur : pr;
+            const double* mi = which == 0 ? ui : pi_;
+            double vr[3][3], vi[3][3];
+#pragma unroll
+            for (int x = 0; x < 3; ++x)
+#pragma unroll
+                for (int y = 0; y < 3; ++y) {
+                    double sr = 0, si = 0;
+#pragma unroll
+                    for (int q = 0; q < 3; ++q) {
+                        sr += P[x][q] * mr[cix[q][y]];
+                        si += P[x][q] * mi[cix[q][y]];
+                    }
+                    vr[x][y] = sr;
+                    vi[x][y] = si;
+                }
+            double s = 0.0;
+            if (projection == 0) {
+#pragma unroll
+                for (int x = 0; x < 3; ++x)
+#pragma unroll
+                    for (int y = 0; y < 3; ++y) s += vr[x][y] * vr[x][y] + vi[x][y] * vi[x][y];
+                const double tr = vr[0][0] + vr[1][1] + vr[2][2], ti = vi[0][0] + vi[1][1] + vi[2][2];
+                s -= (tr * tr + ti * ti) / 2.0;
+            } else {
+                const double tr = vr[0][0] + vr[1][1] + vr[2][2], ti = vi[0][0] + vi[1][1] + vi[2][2];
+#pragma unroll
+                for (int x = 0; x < 3; ++x)
+#pragma unroll
+                    for (int y = 0; y < 3; ++y) {
+                        double ar = 0, ai = 0;
+#pragma unroll
+                        for (int q = 0; q < 3; ++q) {
+                            ar += vr[x][q] * P[q][y];
+                            ai += vi[x][q] * P[q][y];
+                        }
+                        ar -= 0.5 * P[x][y] * tr;
+                        ai -= 0.5 * P[x][y] * ti;
+                        s += ar * ar + ai * ai;
+                    }
+            }
+            h2v[which] = s;
+        }
+        const int l = (int)floor(kn + 0.5);
+        atomicAdd(&hist[l * 3 + 0], wgt);
+        atomicAdd(&hist[l * 3 + 1], wgt * h2v[0]);
+        atomicAdd(&hist[l * 3 + 2], wgt * h2v[1]);
+        int lb = (int)floor(log(kn) / lmax * nlog);
+        lb = lb < 0 ? 0 : (lb >= nlog ? nlog - 1 : lb);
+        atomicAdd(&hist[3 * nbins_total + lb], wgt * h2v[1]);
+    }
+    __syncthreads();
+    double* dst = part + (size_t)blockIdx.x * width;
+    for (int i = threadIdx.x; i < width; i += blockDim.x) dst[i] = hist[i];
 }
 
 }  // namespace psc

@@ -5,7 +5,11 @@
 
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <fstream>
+#include <sstream>
+#include <sys/stat.h>
 #include <memory>
 #include <string>
 #include <vector>
@@ -27,11 +31,14 @@ enum Kind {
     K_Z_INV,
     K_Z_FWD_KICK_NR,
     K_DIAG,
+    K_Z_INV_GW,
+    K_X_FORCE_GW,
+    K_Z_KICK_GW,
     K_NKINDS
 };
 const char* kKindNames[K_NKINDS] = {"z_inverse_drift", "y_inverse", "x_force", "y_forward",
                                     "z_forward_kick", "scalars", "z_inverse", "z_forward_kick_first",
-                                    "diagnostics"};
+                                    "diagnostics", "z_inverse_drift_gw", "x_force_gw", "z_forward_kick_gw"};
 
 struct Timing {
     double ms = 0.0;
@@ -55,8 +62,10 @@ struct psc_ctx {
     hipStream_t stream = nullptr;
     void* Pf = nullptr;
     void* Pp = nullptr;
-    void* W = nullptr;
+    void* W = nullptr;   // [1 or 4][B][N^3]: fields / V', then 3 GW gradients / sources
     void* tw = nullptr;
+    void* U = nullptr;   // GW: packed u pairs [3][B][N^3]
+    void* PU = nullptr;  // GW: packed pi_u pairs [3][B][N^3]
     MemberState* ms = nullptr;
     double* partV = nullptr;
     double* partKG = nullptr;
@@ -74,6 +83,8 @@ struct psc_ctx {
     std::vector<Pending> pending;
     hipGraphExec_t g_mid = nullptr, g_last = nullptr;
     double dev_bytes = 0.0;
+    int shape = 0;   // index into kShapes
+    std::string report;
 
     int fail(int code, const std::string& msg) {
         err = msg;
@@ -153,11 +164,14 @@ struct EngineBase {
     virtual int field_sums(psc_ctx* c, int member) = 0;
     // Binned spectrum partial sums into c->sums[0 .. 4*maxnumbins).
     virtual int spectra(psc_ctx* c, int member, int fld) = 0;
+    // GW spectrum partial sums into c->sums[0 .. 3*maxnumbins + nlog).
+    virtual int gw_spectra(psc_ctx* c, int member, int projection, int nlog) = 0;
 };
 
-template <typename T, int N>
+template <class G>
 struct Engine final : EngineBase {
-    using G = Geo<T, N>;
+    using T = typename G::T;
+    static constexpr int N = G::N;
     static cpx<T>* P(void* p) { return static_cast<cpx<T>*>(p); }
 
     int launch_z_inverse(psc_ctx* c, bool drift, const cpx<T>* src, cpx<T>* Pf, const cpx<T>* Pp,
@@ -166,11 +180,11 @@ struct Engine final : EngineBase {
         dim3 grid(G::BLOCKS_ROW, members), block(G::THREADS_ROW);
         if (drift)
             return timed(c, K_Z_INV_DRIFT, 4 * X, [&] {
-                hipLaunchKernelGGL((k_z_inverse<T, N, true>), grid, block, 0, c->stream, src, Pf, Pp, W,
+                hipLaunchKernelGGL((k_z_inverse<G, true>), grid, block, 0, c->stream, src, Pf, Pp, W,
                                    ms, P(c->tw));
             });
         return timed(c, K_Z_INV, 2 * X, [&] {
-            hipLaunchKernelGGL((k_z_inverse<T, N, false>), grid, block, 0, c->stream, src, Pf, Pp, W, ms,
+            hipLaunchKernelGGL((k_z_inverse<G, false>), grid, block, 0, c->stream, src, Pf, Pp, W, ms,
                                P(c->tw));
         });
     }
@@ -179,14 +193,14 @@ struct Engine final : EngineBase {
         const double X = (double)members * G::N3 * sizeof(cpx<T>);
         dim3 grid(N / G::TC, N, members), block(G::THREADS_COL);
         return timed(c, SIGN > 0 ? K_Y_INV : K_Y_FWD, 2 * X, [&] {
-            hipLaunchKernelGGL((k_strided<T, N, 1, SIGN>), grid, block, 0, c->stream, W, P(c->tw));
+            hipLaunchKernelGGL((k_strided<G, 1, SIGN>), grid, block, 0, c->stream, W, P(c->tw));
         });
     }
     int launch_x_force(psc_ctx* c) {
         const double X = (double)c->B * G::N3 * sizeof(cpx<T>);
         dim3 grid(N / G::TC, N, c->B), block(G::THREADS_COL);
         return timed(c, K_X_FORCE, 2 * X, [&] {
-            hipLaunchKernelGGL((k_x_force<T, N>), grid, block, 0, c->stream, P(c->W), c->ms, P(c->tw),
+            hipLaunchKernelGGL((k_x_force<G>), grid, block, 0, c->stream, P(c->W), c->ms, P(c->tw),
                                c->partV);
         });
     }
@@ -196,11 +210,11 @@ struct Engine final : EngineBase {
         const StepConsts sc = consts_of(c);
         if (reduce)
             return timed(c, K_Z_FWD_KICK, 4 * X, [&] {
-                hipLaunchKernelGGL((k_z_kick<T, N, true>), grid, block, 0, c->stream, P(c->W), P(c->Pf),
+                hipLaunchKernelGGL((k_z_kick<G, true>), grid, block, 0, c->stream, P(c->W), P(c->Pf),
                                    P(c->Pp), c->ms, P(c->tw), c->partKG, c1, c2, use_a_start, sc);
             });
         return timed(c, K_Z_FWD_KICK_NR, 4 * X, [&] {
-            hipLaunchKernelGGL((k_z_kick<T, N, false>), grid, block, 0, c->stream, P(c->W), P(c->Pf),
+            hipLaunchKernelGGL((k_z_kick<G, false>), grid, block, 0, c->stream, P(c->W), P(c->Pf),
                                P(c->Pp), c->ms, P(c->tw), c->partKG, c1, c2, use_a_start, sc);
         });
     }
@@ -213,8 +227,46 @@ struct Engine final : EngineBase {
         });
     }
 
+    int force_chain_gw(psc_ctx* c, bool drift, bool reduce, double c1, double c2, int use_a_start) {
+        const size_t astride = (size_t)c->B * G::N3;
+        const double X = (double)c->B * G::N3 * sizeof(cpx<T>);
+        const StepConsts sc = consts_of(c);
+        int rc;
+        dim3 rgrid(G::BLOCKS_ROW, c->B), rblock(G::THREADS_ROW);
+        if (drift)
+            rc = timed(c, K_Z_INV_GW, 16 * X, [&] {
+                hipLaunchKernelGGL((k_z_inverse_gw<G, true>), rgrid, rblock, 0, c->stream, P(c->Pf), P(c->Pp),
+                                   P(c->W), P(c->U), P(c->PU), c->ms, P(c->tw), astride, c->cfg.kIR);
+            });
+        else
+            rc = timed(c, K_Z_INV_GW, 5 * X, [&] {
+                hipLaunchKernelGGL((k_z_inverse_gw<G, false>), rgrid, rblock, 0, c->stream, P(c->Pf), P(c->Pp),
+                                   P(c->W), P(c->U), P(c->PU), c->ms, P(c->tw), astride, c->cfg.kIR);
+            });
+        if (rc) return rc;
+        if ((rc = launch_y<+1>(c, P(c->W), 4 * c->B))) return rc;
+        rc = timed(c, K_X_FORCE_GW, 8 * X, [&] {
+            hipLaunchKernelGGL((k_x_force_gw<G>), dim3(N / G::TC, N, c->B), dim3(G::THREADS_COL), 0, c->stream,
+                               P(c->W), c->ms, P(c->tw), c->partV, astride);
+        });
+        if (rc) return rc;
+        if ((rc = launch_y<-1>(c, P(c->W), 4 * c->B))) return rc;
+        if (reduce)
+            return timed(c, K_Z_KICK_GW, 16 * X, [&] {
+                hipLaunchKernelGGL((k_z_kick_gw<G, true>), rgrid, rblock, 0, c->stream, P(c->W), P(c->Pf),
+                                   P(c->Pp), P(c->U), P(c->PU), c->ms, P(c->tw), c->partKG, c1, c2, use_a_start,
+                                   sc, astride);
+            });
+        return timed(c, K_Z_KICK_GW, 16 * X, [&] {
+            hipLaunchKernelGGL((k_z_kick_gw<G, false>), rgrid, rblock, 0, c->stream, P(c->W), P(c->Pf),
+                               P(c->Pp), P(c->U), P(c->PU), c->ms, P(c->tw), c->partKG, c1, c2, use_a_start, sc,
+                               astride);
+        });
+    }
+
     // One force evaluation chain; DRIFT applies the substep's drift first.
     int force_chain(psc_ctx* c, bool drift, bool reduce, double c1, double c2, int use_a_start) {
+        if (c->cfg.with_gw) return force_chain_gw(c, drift, reduce, c1, c2, use_a_start);
         int rc;
         if ((rc = launch_z_inverse(c, drift, P(c->Pf), P(c->Pf), P(c->Pp), P(c->W), c->ms, c->B))) return rc;
         if ((rc = launch_y<+1>(c, P(c->W), c->B))) return rc;
@@ -290,7 +342,7 @@ struct Engine final : EngineBase {
         if ((rc = launch_y<+1>(c, W, 1))) return rc;
         dim3 grid(N / G::TC, N, 1), block(G::THREADS_COL);
         rc = timed(c, K_DIAG, 2.0 * G::N3 * sizeof(cpx<T>), [&] {
-            hipLaunchKernelGGL((k_x_inverse_diag<T, N>), grid, block, 0, c->stream, W, c->ms + member,
+            hipLaunchKernelGGL((k_x_inverse_diag<G>), grid, block, 0, c->stream, W, c->ms + member,
                                P(c->tw), c->part);
         });
         if (rc) return rc;
@@ -323,24 +375,81 @@ struct Engine final : EngineBase {
         HIPCHK(c, hipGetLastError());
         return PSC_OK;
     }
+
+    int gw_spectra(psc_ctx* c, int member, int projection, int nlog) override;
 };
 
-EngineBase* make_engine(int precision, int N) {
-#define PSC_CASE(NN)                                                 \
-    case NN:                                                         \
-        if (precision == PSC_FP32) return new Engine<float, NN>();   \
-        return new Engine<double, NN>();
-    switch (N) {
-        PSC_CASE(16)
-        PSC_CASE(32)
-        PSC_CASE(64)
-        PSC_CASE(128)
-        PSC_CASE(256)
-        PSC_CASE(512)
-        default:
-            return nullptr;
-    }
-#undef PSC_CASE
+template <typename T, int N>
+int gw_spectra_impl(psc_ctx* c, int member, int projection, int nlog) {
+    const size_t off = (size_t)member * (size_t)N * N * N;
+    const int nbt = (int)(std::sqrt(3.0) * (N / 2)) + 2;
+    const int nb = 128;
+    const int width = 3 * nbt + nlog;
+    const size_t shmem = (size_t)width * sizeof(double);
+    hipLaunchKernelGGL((k_gw_spectra<T, N>), dim3(nb), dim3(256), shmem, c->stream,
+                       static_cast<cpx<T>*>(c->U) + off, static_cast<cpx<T>*>(c->PU) + off,
+                       (size_t)c->B * N * N * N, projection, nbt, nlog, c->part, nb);
+    hipLaunchKernelGGL(k_sum_partials, dim3((width + 255) / 256), dim3(256), 0, c->stream, c->part, nb, width,
+                       c->sums);
+    HIPCHK(c, hipGetLastError());
+    return PSC_OK;
+}
+
+template <class G>
+int Engine<G>::gw_spectra(psc_ctx* c, int member, int projection, int nlog) {
+    return gw_spectra_impl<typename G::T, G::N>(c, member, projection, nlog);
+}
+
+// Launch shapes: index 0 is the default of each (precision, N); further
+// entries are the candidates the autotuner times (same results up to the
+// rounding of a different radix plan; identical when only TC / L differ).
+template <typename T, int N, class R, int TC, int L>
+using S = Shape<T, N, R, TC, L>;
+
+struct ShapeEntry {
+    int precision, N;
+    const char* name;
+    EngineBase* (*make)();
+};
+
+template <class G>
+EngineBase* make_one() { return new Engine<G>(); }
+
+#define PSC_SHAPE(P, T, N, NAME, ...) {P, N, NAME, &make_one<S<T, N, __VA_ARGS__>>}
+const ShapeEntry kShapes[] = {
+    PSC_SHAPE(PSC_FP32, float, 16, "r4.4_tc16_l64", Radices<4, 4>, 16, 64),
+    PSC_SHAPE(PSC_FP32, float, 32, "r8.4_tc32_l64", Radices<8, 4>, 32, 64),
+    PSC_SHAPE(PSC_FP32, float, 64, "r8.8_tc32_l32", Radices<8, 8>, 32, 32),
+    PSC_SHAPE(PSC_FP32, float, 128, "r16.8_tc32_l32", Radices<16, 8>, 32, 32),
+    PSC_SHAPE(PSC_FP32, float, 128, "r16.8_tc16_l16", Radices<16, 8>, 16, 16),
+    PSC_SHAPE(PSC_FP32, float, 128, "r8.4.4_tc32_l16", Radices<8, 4, 4>, 32, 16),
+    PSC_SHAPE(PSC_FP32, float, 256, "r16.16_tc32_l16", Radices<16, 16>, 32, 16),
+    PSC_SHAPE(PSC_FP32, float, 256, "r16.16_tc16_l16", Radices<16, 16>, 16, 16),
+    PSC_SHAPE(PSC_FP32, float, 256, "r16.16_tc32_l8", Radices<16, 16>, 32, 8),
+    PSC_SHAPE(PSC_FP32, float, 256, "r8.8.4_tc16_l8", Radices<8, 8, 4>, 16, 8),
+    PSC_SHAPE(PSC_FP32, float, 512, "r8.8.8_tc16_l4", Radices<8, 8, 8>, 16, 4),
+    PSC_SHAPE(PSC_FP32, float, 512, "r16.16.2_tc16_l8", Radices<16, 16, 2>, 16, 8),
+    PSC_SHAPE(PSC_FP32, float, 512, "r8.8.8_tc8_l8", Radices<8, 8, 8>, 8, 8),
+    PSC_SHAPE(PSC_FP64, double, 16, "r4.4_tc16_l64", Radices<4, 4>, 16, 64),
+    PSC_SHAPE(PSC_FP64, double, 32, "r8.4_tc32_l64", Radices<8, 4>, 32, 64),
+    PSC_SHAPE(PSC_FP64, double, 64, "r8.8_tc32_l32", Radices<8, 8>, 32, 32),
+    PSC_SHAPE(PSC_FP64, double, 128, "r16.8_tc16_l16", Radices<16, 8>, 16, 16),
+    PSC_SHAPE(PSC_FP64, double, 128, "r8.4.4_tc16_l16", Radices<8, 4, 4>, 16, 16),
+    PSC_SHAPE(PSC_FP64, double, 256, "r16.16_tc16_l8", Radices<16, 16>, 16, 8),
+    PSC_SHAPE(PSC_FP64, double, 256, "r8.8.4_tc16_l8", Radices<8, 8, 4>, 16, 8),
+    PSC_SHAPE(PSC_FP64, double, 256, "r16.16_tc8_l8", Radices<16, 16>, 8, 8),
+    PSC_SHAPE(PSC_FP64, double, 512, "r8.8.8_tc8_l4", Radices<8, 8, 8>, 8, 4),
+    PSC_SHAPE(PSC_FP64, double, 512, "r16.16.2_tc8_l4", Radices<16, 16, 2>, 8, 4),
+};
+#undef PSC_SHAPE
+constexpr int kNumShapes = sizeof(kShapes) / sizeof(kShapes[0]);
+
+// Indices of the shapes available for (precision, N), default first.
+std::vector<int> shapes_for(int precision, int N) {
+    std::vector<int> v;
+    for (int i = 0; i < kNumShapes; ++i)
+        if (kShapes[i].precision == precision && kShapes[i].N == N) v.push_back(i);
+    return v;
 }
 
 void integrator_weights(const psc_config& c, std::vector<double>& w) {
@@ -358,6 +467,7 @@ void integrator_weights(const psc_config& c, std::vector<double>& w) {
 }
 
 int maxnumbins(int N) { return (int)(std::sqrt(3.0) * (N / 2)) + 2; }
+constexpr int kMaxLogBins = 1024;
 
 int check_member(psc_ctx* c, int member) {
     if (!c) return PSC_ERR_ARG;
@@ -492,6 +602,56 @@ double kernel_a_value(const psc_ctx* c, const Diag& d, double a) {
 
 }  // namespace
 
+namespace {
+
+std::string tune_cache_path() {
+    const char* e = std::getenv("PSC_TUNE_CACHE");
+    if (e && *e) return e;
+    const char* h = std::getenv("HOME");
+    return std::string(h ? h : ".") + "/.cache/pspectcosmo_amd/tune.txt";
+}
+
+std::string tune_key(const psc_ctx* c) {
+    hipDeviceProp_t prop;
+    int dev = 0;
+    std::string arch = "unknown";
+    if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess) arch = prop.gcnArchName;
+    for (auto& ch : arch)
+        if (ch == ' ' || ch == ':') ch = '_';
+    std::ostringstream k;
+    k << arch << "|N" << c->N << "|" << (c->cfg.precision == PSC_FP32 ? "fp32" : "fp64") << "|B" << c->B
+      << "|gw" << c->cfg.with_gw;
+    return k.str();
+}
+
+// Last cached shape name for the key, or "".
+std::string tune_lookup(const std::string& key) {
+    std::ifstream in(tune_cache_path());
+    std::string line, best;
+    while (std::getline(in, line)) {
+        std::istringstream ls(line);
+        std::string k, name;
+        if (ls >> k >> name && k == key) best = name;
+    }
+    return best;
+}
+
+void tune_store(const std::string& key, const std::string& name, double ms) {
+    const std::string path = tune_cache_path();
+    for (size_t pos = path.find('/', 1); pos != std::string::npos; pos = path.find('/', pos + 1))
+        mkdir(path.substr(0, pos).c_str(), 0755);
+    std::ofstream out(path, std::ios::app);
+    if (out) out << key << " " << name << " " << ms << "\n";
+}
+
+void drop_graphs(psc_ctx* c) {
+    if (c->g_mid) hipGraphExecDestroy(c->g_mid);
+    if (c->g_last) hipGraphExecDestroy(c->g_last);
+    c->g_mid = c->g_last = nullptr;
+}
+
+}  // namespace
+
 // =============================================================================
 extern "C" {
 
@@ -523,7 +683,6 @@ int psc_create(const psc_config* cfg, psc_ctx** out) {
         !(cfg->fStar > 0) || !(cfg->MPl > 0) || !std::isfinite(cfg->dt) ||
         (cfg->expansion != PSC_EXPANSION_SELF && cfg->expansion != PSC_EXPANSION_STATIC))
         return PSC_ERR_ARG;
-    if (cfg->with_gw) return PSC_ERR_UNSUPPORTED;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return PSC_ERR_DEVICE;
     std::unique_ptr<psc_ctx> c(new psc_ctx());
@@ -532,8 +691,17 @@ int psc_create(const psc_config* cfg, psc_ctx** out) {
     c->B = cfg->batch;
     c->N3 = (size_t)N * N * N;
     c->es = cfg->precision == PSC_FP32 ? 2 * sizeof(float) : 2 * sizeof(double);
-    c->eng = make_engine(cfg->precision, N);
-    if (!c->eng) return PSC_ERR_UNSUPPORTED;
+    {
+        const std::vector<int> cand = shapes_for(cfg->precision, N);
+        if (cand.empty()) return PSC_ERR_UNSUPPORTED;
+        c->shape = cand[0];
+        if (cfg->autotune) {
+            const std::string name = tune_lookup(tune_key(c.get()));
+            for (int idx : cand)
+                if (name == kShapes[idx].name) c->shape = idx;
+        }
+        c->eng = kShapes[c->shape].make();
+    }
     integrator_weights(*cfg, c->w);
     c->models.assign(c->B, psc_model{1.0, 0.0});
     c->loaded.assign(c->B, 0);
@@ -546,9 +714,12 @@ int psc_create(const psc_config* cfg, psc_ctx** out) {
     };
     // partial-sum sizes: max blocks of any pass x widths
     const size_t blocks_col = (size_t)(N / 8) * N, blocks_row = (size_t)N * N / 4;
-    c->part_len = std::max<size_t>(std::max(blocks_col * 6, (size_t)256 * 4), (size_t)128 * maxnumbins(N) * 4);
-    c->sums_len = (size_t)maxnumbins(N) * 4 + 64;
-    if (!alloc(&c->Pf, arr) || !alloc(&c->Pp, arr) || !alloc(&c->W, arr) ||
+    c->part_len = std::max<size_t>(std::max(blocks_col * 6, (size_t)256 * 4),
+                                   (size_t)128 * (maxnumbins(N) * 4 + kMaxLogBins));
+    c->sums_len = (size_t)maxnumbins(N) * 4 + kMaxLogBins + 64;
+    const bool gw = cfg->with_gw != 0;
+    if ((gw && (!alloc(&c->U, 3 * arr) || !alloc(&c->PU, 3 * arr))) ||
+        !alloc(&c->Pf, arr) || !alloc(&c->Pp, arr) || !alloc(&c->W, gw ? 4 * arr : arr) ||
         !alloc(&c->tw, (size_t)N * c->es) || !alloc((void**)&c->ms, sizeof(MemberState) * c->B) ||
         !alloc((void**)&c->partV, sizeof(double) * blocks_col * c->B) ||
         !alloc((void**)&c->partKG, sizeof(double) * 2 * blocks_row * c->B) ||
@@ -585,7 +756,8 @@ int psc_create(const psc_config* cfg, psc_ctx** out) {
     }
     if (hipMemcpy(c->ms, init.data(), sizeof(MemberState) * c->B, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemset(c->Pf, 0, arr) != hipSuccess || hipMemset(c->Pp, 0, arr) != hipSuccess ||
-        hipMemset(c->W, 0, arr) != hipSuccess) {
+        hipMemset(c->W, 0, gw ? 4 * arr : arr) != hipSuccess ||
+        (gw && (hipMemset(c->U, 0, 3 * arr) != hipSuccess || hipMemset(c->PU, 0, 3 * arr) != hipSuccess))) {
         psc_destroy(c.release());
         return PSC_ERR_DEVICE;
     }
@@ -595,15 +767,14 @@ int psc_create(const psc_config* cfg, psc_ctx** out) {
 
 int psc_destroy(psc_ctx* c) {
     if (!c) return PSC_OK;
-    if (c->stream) hipStreamSynchronize(c->stream);
-    if (c->g_mid) hipGraphExecDestroy(c->g_mid);
-    if (c->g_last) hipGraphExecDestroy(c->g_last);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    drop_graphs(c);
     for (auto& p : c->pending) {
         hipEventDestroy(p.a);
         hipEventDestroy(p.b);
     }
     for (auto e : c->ev_free) hipEventDestroy(e);
-    void* ptrs[] = {c->Pf, c->Pp, c->W, c->tw, c->ms, c->partV, c->partKG, c->part, c->sums};
+    void* ptrs[] = {c->Pf, c->Pp, c->W, c->tw, c->U, c->PU, c->ms, c->partV, c->partKG, c->part, c->sums};
     for (void* p : ptrs)
         if (p) hipFree(p);
     if (c->stream) hipStreamDestroy(c->stream);
@@ -813,14 +984,255 @@ int psc_spectra(psc_ctx* c, int member, int fld, double* out, int nbins) {
     return PSC_OK;
 }
 
-int psc_load_gw_k(psc_ctx* c, int, const double*, const double*) {
-    return c ? c->fail(PSC_ERR_UNSUPPORTED, "GW evolution not enabled in this context") : PSC_ERR_ARG;
+int psc_load_gw_k(psc_ctx* c, int member, const double* u_k, const double* pi_u_k) {
+    int rc = check_member(c, member);
+    if (rc) return rc;
+    if (!c->cfg.with_gw) return c->fail(PSC_ERR_STATE, "context created without GW evolution");
+    if (!u_k || !pi_u_k) return c->fail(PSC_ERR_ARG, "null GW data");
+    const size_t nk = (size_t)c->N * c->N * (c->N / 2 + 1);
+    std::vector<double> full;
+    for (int a = 0; a < 3; ++a) {
+        const size_t arr_off = (size_t)a * c->B;  // pair a of member m sits at [a][m]
+        pack_full(c->N, 2, u_k + 2 * nk * (2 * a), full);
+        if ((rc = upload(c, c->U, (int)(arr_off + member), full))) return rc;
+        pack_full(c->N, 2, pi_u_k + 2 * nk * (2 * a), full);
+        if ((rc = upload(c, c->PU, (int)(arr_off + member), full))) return rc;
+    }
+    return PSC_OK;
 }
-int psc_get_gw_k(psc_ctx* c, int, double*, double*) {
-    return c ? c->fail(PSC_ERR_UNSUPPORTED, "GW evolution not enabled in this context") : PSC_ERR_ARG;
+
+int psc_get_gw_k(psc_ctx* c, int member, double* u_k, double* pi_u_k) {
+    int rc = check_member(c, member);
+    if (rc) return rc;
+    if (!c->cfg.with_gw) return c->fail(PSC_ERR_STATE, "context created without GW evolution");
+    const size_t nk = (size_t)c->N * c->N * (c->N / 2 + 1);
+    std::vector<double> full;
+    for (int a = 0; a < 3; ++a) {
+        const int slot = a * c->B + member;
+        if (u_k) {
+            if ((rc = download(c, c->U, slot, full))) return rc;
+            unpack_half(c->N, 2, full.data(), u_k + 2 * nk * (2 * a));
+        }
+        if (pi_u_k) {
+            if ((rc = download(c, c->PU, slot, full))) return rc;
+            unpack_half(c->N, 2, full.data(), pi_u_k + 2 * nk * (2 * a));
+        }
+    }
+    return PSC_OK;
 }
-int psc_gw_spectrum(psc_ctx* c, int, int, double*, int, double*) {
-    return c ? c->fail(PSC_ERR_UNSUPPORTED, "GW evolution not enabled in this context") : PSC_ERR_ARG;
+
+namespace {
+// Shared part of psc_gw_spectrum / psc_gw_omega_log: histograms, total energy
+// and the DC kinetic term sum_ij |pi_u_ij(0)|^2 (output.cpp:1038-1044).
+int gw_histograms(psc_ctx* c, int member, int projection, int nlog, std::vector<double>& h, double* total,
+                  double* hd0, MemberState* s) {
+    int rc = check_member(c, member);
+    if (rc) return rc;
+    if (!c->cfg.with_gw) return c->fail(PSC_ERR_STATE, "context created without GW evolution");
+    if (projection != PSC_GW_PROJ_REFERENCE && projection != PSC_GW_PROJ_TT)
+        return c->fail(PSC_ERR_ARG, "unknown projection");
+    if (nlog < 1 || nlog > kMaxLogBins) return c->fail(PSC_ERR_ARG, "log bin count out of range");
+    if ((rc = read_state(c, member, s))) return rc;
+    double e[PSC_ENERGY_LEN];
+    rc = psc_energies(c, member, e);
+    if (rc && rc != PSC_ERR_NONFINITE) return rc;
+    *total = e[6];
+    if ((rc = c->eng->gw_spectra(c, member, projection, nlog))) return rc;
+    const int mb = maxnumbins(c->N);
+    h.resize((size_t)3 * mb + nlog);
+    HIPCHK(c, hipMemcpyAsync(h.data(), c->sums, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    *hd0 = 0.0;
+    for (int a = 0; a < 3; ++a) {
+        double dc[2];
+        const char* src = static_cast<const char*>(c->PU) + ((size_t)a * c->B + member) * c->N3 * c->es;
+        if (c->cfg.precision == PSC_FP32) {
+            float f2[2];
+            HIPCHK(c, hipMemcpyAsync(f2, src, sizeof(f2), hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+            dc[0] = f2[0];
+            dc[1] = f2[1];
+        } else {
+            HIPCHK(c, hipMemcpyAsync(dc, src, sizeof(dc), hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+        }
+        // pair a = (component 2a, component 2a+1); at k = 0 both are real:
+        // u_{2a}(0) = Re P(0), u_{2a+1}(0) = Im P(0).  Off-diagonal components
+        // (index >= 3) count twice in the 9-component sum.
+        const double w0 = (2 * a) >= 3 ? 2.0 : 1.0, w1 = (2 * a + 1) >= 3 ? 2.0 : 1.0;
+        *hd0 += w0 * dc[0] * dc[0] + w1 * dc[1] * dc[1];
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return PSC_OK;
+}
+}  // namespace
+
+int psc_gw_spectrum(psc_ctx* c, int member, int projection, double* out, int nbins, double* totals) {
+    if (!c) return PSC_ERR_ARG;
+    const int mb = maxnumbins(c->N);
+    if (!out || nbins != mb - 1) return c->fail(PSC_ERR_ARG, "bad GW spectrum arguments");
+    std::vector<double> h;
+    double total, hd0;
+    MemberState s;
+    int rc = gw_histograms(c, member, projection, 1, h, &total, &hd0, &s);
+    if (rc) return rc;
+    const double N3 = (double)c->N3, a = s.a, ad = s.ad;
+    const double fS = c->cfg.fStar, wS = c->cfg.omegaStar, MP = c->cfg.MPl, kIR = c->cfg.kIR;
+    const double pi = 3.14159265358979323846;
+    double rho = hd0;
+    for (int i = 1; i < mb; ++i) {
+        const double np = h[(size_t)i * 3];
+        double h2 = h[(size_t)i * 3 + 1], hd2 = h[(size_t)i * 3 + 2];
+        rho += hd2;
+        if (np > 0) {
+            h2 /= np;
+            hd2 /= np;
+        }
+        double* row = out + (size_t)(i - 1) * PSC_GW_SPECTRA_COLS;
+        row[0] = kIR * i;
+        row[1] = i / (N3 * N3) * np * h2;
+        row[2] = i / (N3 * N3) / 4.0 / std::pow(a, 6) * (fS / MP) * (fS / MP) / total * np * hd2;
+        row[3] = (MP * wS) * (MP * wS) / std::pow(fS, 4) * 2 / (pi * pi) * ad * ad / std::pow(a, 2 * c->cfg.alpha + 2);
+        row[4] = np;
+    }
+    rho *= wS * wS * std::pow(fS, 4) / (MP * MP) / 4.0 / (N3 * N3) / std::pow(a, 6);
+    rho /= (fS * wS) * (fS * wS);
+    if (totals) {
+        totals[0] = rho / total;
+        totals[1] = rho;
+    }
+    return PSC_OK;
+}
+
+int psc_gw_omega_log(psc_ctx* c, int member, int projection, int nlog, double* k_center, double* omega) {
+    if (!c) return PSC_ERR_ARG;
+    if (!k_center || !omega) return c->fail(PSC_ERR_ARG, "null output");
+    std::vector<double> h;
+    double total, hd0;
+    MemberState s;
+    int rc = gw_histograms(c, member, projection, nlog, h, &total, &hd0, &s);
+    if (rc) return rc;
+    const int mb = maxnumbins(c->N);
+    const double N3 = (double)c->N3, a = s.a;
+    const double f2m = (c->cfg.fStar / c->cfg.MPl) * (c->cfg.fStar / c->cfg.MPl);
+    const double lmax = std::log(std::sqrt(3.0) * (c->N / 2)), dl = lmax / nlog;
+    for (int b = 0; b < nlog; ++b) {
+        k_center[b] = c->cfg.kIR * std::exp((b + 0.5) * dl);
+        const double rho_b = f2m / 4.0 / (N3 * N3) / std::pow(a, 6) * h[(size_t)3 * mb + b];
+        omega[b] = rho_b / total / dl;
+    }
+    return PSC_OK;
+}
+
+
+int psc_autotune(psc_ctx* c, int nsteps, char* report, int report_len) {
+    if (!c) return PSC_ERR_ARG;
+    if (nsteps < 1) return c->fail(PSC_ERR_ARG, "nsteps must be >= 1");
+    for (int b = 0; b < c->B; ++b)
+        if (!c->loaded[b]) return c->fail(PSC_ERR_STATE, "autotune needs loaded initial data");
+    const size_t arr = (size_t)c->B * c->N3 * c->es;
+    const bool gw = c->cfg.with_gw != 0;
+    void *sf = nullptr, *sp = nullptr, *su = nullptr, *spu = nullptr;
+    std::vector<MemberState> ms(c->B);
+    auto cleanup = [&]() {
+        for (void* q : {sf, sp, su, spu})
+            if (q) hipFree(q);
+    };
+    if (hipMalloc(&sf, arr) != hipSuccess || hipMalloc(&sp, arr) != hipSuccess ||
+        (gw && (hipMalloc(&su, 3 * arr) != hipSuccess || hipMalloc(&spu, 3 * arr) != hipSuccess))) {
+        cleanup();
+        return c->fail(PSC_ERR_NOMEM, "autotune snapshot allocation failed");
+    }
+    auto copy = [&](bool save) -> int {
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        HIPCHK(c, hipMemcpy(save ? sf : c->Pf, save ? c->Pf : sf, arr, hipMemcpyDeviceToDevice));
+        HIPCHK(c, hipMemcpy(save ? sp : c->Pp, save ? c->Pp : sp, arr, hipMemcpyDeviceToDevice));
+        if (gw) {
+            HIPCHK(c, hipMemcpy(save ? su : c->U, save ? c->U : su, 3 * arr, hipMemcpyDeviceToDevice));
+            HIPCHK(c, hipMemcpy(save ? spu : c->PU, save ? c->PU : spu, 3 * arr, hipMemcpyDeviceToDevice));
+        }
+        if (save)
+            HIPCHK(c, hipMemcpy(ms.data(), c->ms, sizeof(MemberState) * c->B, hipMemcpyDeviceToHost));
+        else
+            HIPCHK(c, hipMemcpy(c->ms, ms.data(), sizeof(MemberState) * c->B, hipMemcpyHostToDevice));
+        return PSC_OK;
+    };
+    int rc = copy(true);
+    if (rc) { cleanup(); return rc; }
+    const std::vector<int> cand = shapes_for(c->cfg.precision, c->N);
+    const int saved_timing = c->timing;
+    c->timing = 0;
+    double best_ms = 1e300;
+    int best = c->shape;
+    std::ostringstream rep;
+    hipEvent_t e0, e1;
+    hipEventCreate(&e0);
+    hipEventCreate(&e1);
+    for (int idx : cand) {
+        drop_graphs(c);
+        delete c->eng;
+        c->eng = kShapes[idx].make();
+        c->shape = idx;
+        if ((rc = copy(false))) break;
+        if ((rc = c->eng->step(c, 1))) break;   // warm-up (and graph capture)
+        if ((rc = copy(false))) break;
+        hipEventRecord(e0, c->stream);
+        if ((rc = c->eng->step(c, nsteps))) break;
+        hipEventRecord(e1, c->stream);
+        hipEventSynchronize(e1);
+        float ms_ = 0.0f;
+        hipEventElapsedTime(&ms_, e0, e1);
+        rep << kShapes[idx].name << "=" << ms_ << ";";
+        if (ms_ < best_ms) {
+            best_ms = ms_;
+            best = idx;
+        }
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    c->timing = saved_timing;
+    drop_graphs(c);
+    delete c->eng;
+    c->shape = best;
+    c->eng = kShapes[best].make();
+    int rc2 = copy(false);
+    cleanup();
+    if (rc) return rc;
+    if (rc2) return rc2;
+    tune_store(tune_key(c), kShapes[best].name, best_ms);
+    c->report = rep.str();
+    if (report && report_len > 0) {
+        std::strncpy(report, c->report.c_str(), report_len - 1);
+        report[report_len - 1] = 0;
+    }
+    return PSC_OK;
+}
+
+const char* psc_shape_name(const psc_ctx* c) { return c ? kShapes[c->shape].name : "null context"; }
+
+int psc_list_shapes(const psc_ctx* c, char* names, int names_len) {
+    if (!c || !names || names_len < 1) return PSC_ERR_ARG;
+    std::string s;
+    for (int idx : shapes_for(c->cfg.precision, c->N)) {
+        if (!s.empty()) s += ",";
+        s += kShapes[idx].name;
+    }
+    std::strncpy(names, s.c_str(), names_len - 1);
+    names[names_len - 1] = 0;
+    return PSC_OK;
+}
+
+int psc_set_shape(psc_ctx* c, const char* name) {
+    if (!c || !name) return PSC_ERR_ARG;
+    for (int idx : shapes_for(c->cfg.precision, c->N))
+        if (std::strcmp(name, kShapes[idx].name) == 0) {
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+            drop_graphs(c);
+            delete c->eng;
+            c->shape = idx;
+            c->eng = kShapes[idx].make();
+            return PSC_OK;
+        }
+    return c->fail(PSC_ERR_ARG, std::string("no launch shape named ") + name);
 }
 
 int psc_set_kernel_timing(psc_ctx* c, int on) {

@@ -56,11 +56,17 @@ def lib():
         L.psc_load_gw_k.argtypes = [vp, C.c_int, _D, _D]
         L.psc_get_gw_k.argtypes = [vp, C.c_int, _D, _D]
         L.psc_gw_spectrum.argtypes = [vp, C.c_int, C.c_int, _D, C.c_int, _D]
+        L.psc_gw_omega_log.argtypes = [vp, C.c_int, C.c_int, C.c_int, _D, _D]
         L.psc_set_kernel_timing.argtypes = [vp, C.c_int]
         L.psc_reset_kernel_times.argtypes = [vp]
         L.psc_get_kernel_times.argtypes = [vp, C.c_char_p, C.c_int, _D, P(C.c_longlong), _D,
                                            C.c_int, P(C.c_int)]
         L.psc_device_bytes.argtypes = [vp, _D]
+        L.psc_autotune.argtypes = [vp, C.c_int, C.c_char_p, C.c_int]
+        L.psc_shape_name.restype = C.c_char_p
+        L.psc_shape_name.argtypes = [vp]
+        L.psc_list_shapes.argtypes = [vp, C.c_char_p, C.c_int]
+        L.psc_set_shape.argtypes = [vp, C.c_char_p]
         _lib = L
     return _lib
 
@@ -71,8 +77,9 @@ def exported_symbols():
             "psc_create", "psc_destroy", "psc_last_error", "psc_set_model", "psc_load_ic_k",
             "psc_set_scalars", "psc_get_scalars", "psc_step", "psc_sync", "psc_get_k",
             "psc_get_fields", "psc_energies", "psc_spectra", "psc_load_gw_k", "psc_get_gw_k",
-            "psc_gw_spectrum", "psc_num_bins", "psc_set_kernel_timing", "psc_get_kernel_times",
-            "psc_reset_kernel_times", "psc_device_bytes"]
+            "psc_gw_spectrum", "psc_gw_omega_log", "psc_num_bins", "psc_set_kernel_timing", "psc_get_kernel_times",
+            "psc_reset_kernel_times", "psc_device_bytes", "psc_autotune", "psc_shape_name",
+            "psc_list_shapes", "psc_set_shape"]
 
 
 def generate_ic(cfg, model, ic):
@@ -194,6 +201,12 @@ class Lattice:
         self._check(lib().psc_gw_spectrum(self._h, member, projection, out.ravel(), nb, tot))
         return out, tot
 
+    def gw_omega_log(self, projection, nlog=256, member=0):
+        kc = np.zeros(nlog)
+        om = np.zeros(nlog)
+        self._check(lib().psc_gw_omega_log(self._h, member, projection, nlog, kc, om))
+        return kc, om
+
     def set_kernel_timing(self, on):
         self._check(lib().psc_set_kernel_timing(self._h, 1 if on else 0))
 
@@ -213,6 +226,27 @@ class Lattice:
         for i, name in enumerate(names.value.decode().split(",")[: n.value]):
             out[name] = {"ms": float(ms[i]), "count": int(counts[i]), "bytes": float(bytes_[i])}
         return out
+
+    def autotune(self, nsteps=3):
+        buf = C.create_string_buffer(4096)
+        self._check(lib().psc_autotune(self._h, int(nsteps), buf, 4096))
+        out = {}
+        for item in buf.value.decode().split(";"):
+            if "=" in item:
+                k, v = item.split("=")
+                out[k] = float(v)
+        return out
+
+    def shape_name(self):
+        return lib().psc_shape_name(self._h).decode()
+
+    def shapes(self):
+        buf = C.create_string_buffer(4096)
+        self._check(lib().psc_list_shapes(self._h, buf, 4096))
+        return buf.value.decode().split(",")
+
+    def set_shape(self, name):
+        self._check(lib().psc_set_shape(self._h, name.encode()))
 
     def device_bytes(self):
         b = np.zeros(1)

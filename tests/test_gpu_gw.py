@@ -1,0 +1,93 @@
+"""GPU parity of the gravitational-wave evolution and spectra against the FP64
+oracle (same seeded fields and GW initial modes)."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from pspectcosmo_amd.abi import (PSC_VV2, PSC_Y4, PSC_FP64, PSC_FP32, PSC_GW_PROJ_REFERENCE,
+                                 PSC_GW_PROJ_TT)
+from pspectcosmo_amd.lattice import Lattice
+from tests._util import parity_setup, rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def _pair(**kw):
+    cfg, model, ic = parity_setup(with_gw=1, **kw)
+    fk, pk, sc = O.generate_ic(cfg, model, ic)
+    u, pu = O.generate_gw_ic(cfg, model, ic)
+    lat = Lattice(cfg)
+    lat.set_model(0, model)
+    lat.load_ic_k(0, fk, pk, sc[0], sc[1])
+    lat.load_gw_k(0, u, pu)
+    orc = O.Oracle(cfg, model)
+    orc.load(fk, pk, 0.0, sc[0], sc[1], sc[4])
+    orc.load_gw(u, pu)
+    return cfg, lat, orc, u, pu
+
+
+def test_gw_load_roundtrip():
+    cfg, lat, orc, u, pu = _pair(N=16)
+    gu, gpu_ = lat.get_gw_k(0)
+    assert rel_err(gu, u) < 1e-15 and rel_err(gpu_, pu) < 1e-15
+
+
+@pytest.mark.parametrize("integrator", [PSC_VV2, PSC_Y4])
+@pytest.mark.parametrize("N", [16, 32])
+def test_gw_step_parity_fp64(N, integrator):
+    cfg, lat, orc, u, pu = _pair(N=N, integrator=integrator)
+    lat.step(10)
+    orc.step(10)
+    gk, gp = lat.get_k(0)
+    ok, op = orc.get_k()
+    gu, gpu_ = lat.get_gw_k(0)
+    ou, opu = orc.get_gw()
+    errs = [rel_err(gk, ok), rel_err(gp, op), rel_err(gu, ou), rel_err(gpu_, opu)]
+    print(f"N={N} int={integrator} field/mom/u/pi_u errors {errs}")
+    assert max(errs) <= 1e-10
+    np.testing.assert_allclose(lat.scalars(0), orc.scalars(), rtol=1e-10)
+
+
+@pytest.mark.parametrize("projection", [PSC_GW_PROJ_REFERENCE, PSC_GW_PROJ_TT])
+def test_gw_spectrum_parity(projection):
+    cfg, lat, orc, u, pu = _pair(N=16, integrator=PSC_Y4)
+    lat.step(4)
+    orc.step(4)
+    s, tot = lat.gw_spectrum(projection, 0)
+    so, toto = orc.gw_spectrum(projection)
+    np.testing.assert_array_equal(s[:, 4], so[:, 4])
+    np.testing.assert_allclose(s, so, rtol=1e-10, atol=0)
+    np.testing.assert_allclose(tot, toto, rtol=1e-10)
+    kc, om = lat.gw_omega_log(projection, 64, 0)
+    kco, omo = orc.gw_omega_log(projection, 64)
+    np.testing.assert_allclose(kc, kco, rtol=1e-14)
+    np.testing.assert_allclose(om, omo, rtol=1e-10, atol=1e-300)
+
+
+def test_gw_step_fp32_error():
+    cfg, lat, orc, u, pu = _pair(N=32, integrator=PSC_Y4, precision=PSC_FP32)
+    lat.step(10)
+    orc.step(10)
+    gu, gpu_ = lat.get_gw_k(0)
+    ou, opu = orc.get_gw()
+    gk, _ = lat.get_k(0)
+    ok, _ = orc.get_k()
+    errs = [rel_err(gk, ok), rel_err(gu, ou), rel_err(gpu_, opu)]
+    print("fp32 gw errors", errs)
+    assert max(errs) <= 1e-5
+
+
+def test_gw_graph_replay_matches_eager():
+    cfg, lat, orc, u, pu = _pair(N=16, integrator=PSC_Y4)
+    cfg_g, model, ic = parity_setup(N=16, integrator=PSC_Y4, with_gw=1, use_graph=1)
+    fk, pk, sc = O.generate_ic(cfg_g, model, ic)
+    g = Lattice(cfg_g)
+    g.set_model(0, model)
+    g.load_ic_k(0, fk, pk, sc[0], sc[1])
+    g.load_gw_k(0, u, pu)
+    for _ in range(2):
+        lat.step(3)
+        g.step(3)
+        a, b = lat.get_gw_k(0), g.get_gw_k(0)
+        np.testing.assert_array_equal(a[0], b[0])
+        np.testing.assert_array_equal(a[1], b[1])
