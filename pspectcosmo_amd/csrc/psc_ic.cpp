@@ -4,8 +4,11 @@
 // Rayleigh branch (set_mode, initialize.cpp:128-220) and initialize_GW
 // (initialize.cpp:341-470), writing FFTW-layout FP64 half spectra.  The output
 // is loaded onto the device with psc_load_ic_k / psc_load_gw_k.
+#include <algorithm>
 #include <cmath>
+#include <cstdint>
 #include <cstring>
+#include <thread>
 #include <vector>
 #include "../../include/psc.h"
 
@@ -26,6 +29,22 @@ inline int sidx(int i, int N) { return i <= N / 2 ? i : i - N; }
 class Rng {
   public:
     explicit Rng(int seed) : seed_(seed), next_(seed > 0 ? seed : 1) {}
+    // Position the stream so the next uniform() returns draw number d
+    // (0-based): state = 16807^(99 + d + 1 - 1) x seed mod (2^31 - 1) before
+    // the advance inside uniform(), i.e. skip-ahead by modular exponentiation.
+    static Rng at(int seed, uint64_t d) {
+        Rng r(seed);
+        r.warm_ = true;
+        uint64_t e = 99 + d, base = 16807, acc = 1;
+        const uint64_t m = 2147483647ull;
+        while (e) {
+            if (e & 1) acc = acc * base % m;
+            base = base * base % m;
+            e >>= 1;
+        }
+        r.next_ = (int)(acc * (uint64_t)r.next_ % m);
+        return r;
+    }
     double uniform() {
         if (seed_ < 1) return 0.33;
         if (!warm_) {
@@ -59,30 +78,56 @@ struct Pot {
     }
 };
 
-// The mode order of initialize.cpp:685-749 (shared with initialize_GW):
-// fn(i, j, k, conjugate index or -1, isReal).
+// Number of modes visited per field in the order of initialize.cpp:685-749
+// (shared with initialize_GW, initialize.cpp:355-428): the kz = 0 and N/2
+// planes (half of each plane, conjugates filled alongside), the interior
+// planes 0 < kz < N/2, then the 8 real corners (7 without the DC mode).
+uint64_t mode_count(int N, bool dc) {
+    const uint64_t h = N / 2 - 1;
+    return 2 * (4 * h + 2 * h * h) + h * (uint64_t)N * N + (dc ? 8 : 7);
+}
+
+// Visit the modes in that order; each mode consumes 4 draws from the stream
+// `rng` handed to fn(rng, i, j, k, conjugate index or -1, isReal).  The
+// stream starts at draw `first`.  The interior planes (the bulk) run on worker
+// threads, each (kz, i) row starting from its own skip-ahead position, which
+// leaves every mode with exactly the draws of the sequential order.
 template <class F>
-void for_each_mode(int N, bool dc, F&& fn) {
+void for_each_mode(int N, bool dc, int seed, uint64_t first, F&& fn) {
     const int Nh = N / 2 + 1;
     auto at = [&](int i, int j, int k) { return ((long)i * N + j) * Nh + k; };
+    Rng rng = Rng::at(seed, first);
+    uint64_t v = 0;
     for (int k = 0; k <= N / 2; k += N / 2) {
         for (int i = 0; i <= N / 2; i += N / 2)
-            for (int j = 1; j < N / 2; ++j) fn(i, j, k, at(i, N - j, k), false);
+            for (int j = 1; j < N / 2; ++j, ++v) fn(rng, i, j, k, at(i, N - j, k), false);
         for (int j = 0; j <= N / 2; j += N / 2)
-            for (int i = 1; i < N / 2; ++i) fn(i, j, k, at(N - i, j, k), false);
+            for (int i = 1; i < N / 2; ++i, ++v) fn(rng, i, j, k, at(N - i, j, k), false);
         for (int i = 1; i < N / 2; ++i) {
-            for (int j = 1; j < N / 2; ++j) fn(i, j, k, at(N - i, N - j, k), false);
-            for (int j = N / 2 + 1; j < N; ++j) fn(i, j, k, at(N - i, N - j, k), false);
+            for (int j = 1; j < N / 2; ++j, ++v) fn(rng, i, j, k, at(N - i, N - j, k), false);
+            for (int j = N / 2 + 1; j < N; ++j, ++v) fn(rng, i, j, k, at(N - i, N - j, k), false);
         }
     }
-    for (int k = 1; k < N / 2; ++k)
-        for (int i = 0; i < N; ++i)
-            for (int j = 0; j < N; ++j) fn(i, j, k, -1L, false);
+    const uint64_t v0 = v;
+    const long rows = (long)(N / 2 - 1) * N;  // (kz, i) rows of N modes each
+    unsigned nt = std::max(1u, std::min(32u, std::thread::hardware_concurrency()));
+    if (rows < 256) nt = 1;
+    std::vector<std::thread> pool;
+    for (unsigned w = 0; w < nt; ++w)
+        pool.emplace_back([&, w] {
+            for (long r = w; r < rows; r += nt) {
+                const int k = 1 + (int)(r / N), i = (int)(r % N);
+                Rng rr = Rng::at(seed, first + 4 * (v0 + (uint64_t)r * N));
+                for (int j = 0; j < N; ++j) fn(rr, i, j, k, -1L, false);
+            }
+        });
+    for (auto& t : pool) t.join();
+    Rng rc = Rng::at(seed, first + 4 * (v0 + (uint64_t)rows * N));
     for (int k = 0; k <= N / 2; k += N / 2)
         for (int i = 0; i <= N / 2; i += N / 2)
             for (int j = 0; j <= N / 2; j += N / 2) {
                 if (!dc && i == 0 && j == 0 && k == 0) continue;
-                fn(i, j, k, -1L, true);
+                fn(rc, i, j, k, -1L, true);
             }
 }
 
@@ -145,11 +190,12 @@ extern "C" int psc_generate_ic(const psc_config* cfg, const psc_model* model, co
     V.meff(meff, f0);
     const double dp2 = pw2(cfg->kIR), ap = std::pow(a, cfg->alpha - 3);
     const double N3 = (double)N * N * N;
-    Rng rng(ic->seed);
+    const uint64_t M = mode_count(N, true);
     for (int q = 0; q < cfg->nflds; ++q) {
         double* fq = f_k + 2 * nk * q;
         double* pq = pi_k + 2 * nk * q;
-        for_each_mode(N, true, [&](int i, int j, int k, long conj, bool real) {
+        // four draws per mode (initialize.cpp:181-190), fields in sequence
+        for_each_mode(N, true, ic->seed, 4 * (uint64_t)q * M, [&](Rng& rng, int i, int j, int k, long conj, bool real) {
             const int px = sidx(i, N), py = sidx(j, N);
             const double p2 = dp2 * (pw2(px) + pw2(py) + pw2(k));
             const double om = (p2 + meff[q] > 0.0) ? std::sqrt(p2 + pw2(a) * meff[q]) : std::sqrt(p2);
@@ -196,16 +242,14 @@ extern "C" int psc_generate_gw_ic(const psc_config* cfg, const psc_model* model,
     const Pot V{cfg->nflds, model->m2, model->g2};
     double a, ad, addp, H;
     background(*cfg, V, *ic, &a, &ad, &addp, &H);
-    Rng rng(ic->seed);
-    long modes = 0;
-    for_each_mode(N, true, [&](int, int, int, long, bool) { ++modes; });
-    for (long d = 0; d < 4L * modes * cfg->nflds; ++d) rng.uniform();
+    // The GW draws follow the 4 draws of every scalar mode of every field.
+    const uint64_t skip = 4 * mode_count(N, true) * (uint64_t)cfg->nflds;
     const double L = 2.0 * kPi / cfg->kIR, dx = L / N;
     const double norm = cfg->MPl * cfg->omegaStar / pw2(cfg->fStar) * std::pow(N / dx, 1.5);
     const double ap = std::pow(a, cfg->alpha - 3);
     std::memset(u_k, 0, sizeof(double) * 2 * nk * 6);
     std::memset(pi_u_k, 0, sizeof(double) * 2 * nk * 6);
-    for_each_mode(N, false, [&](int i, int j, int k, long conj, bool real) {
+    for_each_mode(N, false, ic->seed, skip, [&](Rng& rng, int i, int j, int k, long conj, bool real) {
         const int px = sidx(i, N), py = sidx(j, N);
         const double kv[3] = {px * cfg->kIR, py * cfg->kIR, k * cfg->kIR};
         double e[3][3];

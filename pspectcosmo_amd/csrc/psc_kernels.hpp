@@ -101,7 +101,7 @@ k_x_force(cpx<typename G::T>* __restrict__ W, const MemberState* __restrict__ ms
     line_fft<T, N, typename G::Rad, +1, typename G::LayCol>(e, t, c, lds, tw);
     const T inv = (T)(1.0 / (double)G::N3);
     const T m2 = (T)ms[member].m2, g2 = (T)ms[member].g2;
-    const double m2d = ms[member].m2, g2d = ms[member].g2;
+    const T half = (T)0.5;
     double v[1] = {0.0};
 #pragma unroll
     for (int m = 0; m < G::E; ++m) {
@@ -109,8 +109,8 @@ k_x_force(cpx<typename G::T>* __restrict__ W, const MemberState* __restrict__ ms
         const T chi2 = chi * chi, phi2 = phi * phi;
         e[m].x = m2 * phi + g2 * phi * chi2;
         e[m].y = g2 * phi2 * chi;
-        const double pd = phi, cdd = chi;
-        v[0] += 0.5 * m2d * (pd * pd) + 0.5 * g2d * (pd * pd) * (cdd * cdd);
+        // V per site in the field precision, summed in FP64
+        v[0] += (double)(half * m2 * phi2 + half * g2 * phi2 * chi2);
     }
     line_fft<T, N, typename G::Rad, -1, typename G::LayCol>(e, t, c, lds, tw);
 #pragma unroll
@@ -405,6 +405,41 @@ k_spectra(const cpx<T>* __restrict__ Pf, const cpx<T>* __restrict__ Pp,
     for (int i = threadIdx.x; i < nbins_total * 4; i += blockDim.x) dst[i] = hist[i];
 }
 
+// First level of the substep reduction when a pass has many blocks: per
+// member, the n partial vectors of width W are cut into `chunks` contiguous
+// ranges, each summed with compensation in a fixed order.
+template <int W>
+__global__ void __launch_bounds__(256)
+k_prereduce(const double* __restrict__ in, int n, double* __restrict__ out, int chunks) {
+    __shared__ double sh[W * 2 * 4];
+    const int member = blockIdx.y, chunk = blockIdx.x;
+    const long lo = (long)n * chunk / chunks, hi = (long)n * (chunk + 1) / chunks;
+    Comp acc[W];
+    for (long i = lo + threadIdx.x; i < hi; i += blockDim.x)
+#pragma unroll
+        for (int q = 0; q < W; ++q) acc[q].add(in[((size_t)member * n + i) * W + q]);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int q = 0; q < W; ++q) {
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) {
+            const double s2 = __shfl_xor(acc[q].s, off, 64), c2 = __shfl_xor(acc[q].c, off, 64);
+            comp_merge(acc[q].s, acc[q].c, s2, c2);
+        }
+        if (lane == 0) {
+            sh[(q * 4 + wave) * 2] = acc[q].s;
+            sh[(q * 4 + wave) * 2 + 1] = acc[q].c;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0)
+        for (int q = 0; q < W; ++q) {
+            double ss = sh[(q * 4) * 2], cc = sh[(q * 4) * 2 + 1];
+            for (int w = 1; w < 4; ++w) comp_merge(ss, cc, sh[(q * 4 + w) * 2], sh[(q * 4 + w) * 2 + 1]);
+            out[((size_t)member * chunks + chunk) * W + q] = ss + cc;
+        }
+}
+
 // Fixed-order sum of per-block partial vectors: out[q] = sum_b part[b][q].
 __global__ void k_sum_partials(const double* __restrict__ part, int nblocks, int width,
                                double* __restrict__ out) {
@@ -507,12 +542,15 @@ k_z_inverse_gw(cpx<typename G::T>* __restrict__ Pf, const cpx<typename G::T>* __
     }
 }
 
-// C (GW): x-axis inverse of fields and gradients, V' and the packed source
-// S = (S11 + i S22, S33 + i S12, S13 + i S23), x-axis forward of all four.
+// C (GW): the field array as in k_x_force (V' and sum V), then the three
+// gradient arrays: x-axis inverse, the packed source
+// S = (S11 + i S22, S33 + i S12, S13 + i S23), x-axis forward.  The field array
+// is finished before the gradients are loaded, so at most three arrays are
+// live in registers.
 template <class G>
 __global__ void __launch_bounds__((G::THREADS_COL))
-k_x_force_gw(cpx<typename G::T>* __restrict__ W, const MemberState* __restrict__ ms, const cpx<typename G::T>* __restrict__ tw,
-             double* __restrict__ partV, size_t astride) {
+k_x_force_gw(cpx<typename G::T>* __restrict__ W, const MemberState* __restrict__ ms,
+             const cpx<typename G::T>* __restrict__ tw, double* __restrict__ partV, size_t astride) {
     using T = typename G::T;
     constexpr int N = G::N;
     __shared__ cpx<T> lds[G::LayCol::size(0)];
@@ -522,36 +560,48 @@ k_x_force_gw(cpx<typename G::T>* __restrict__ W, const MemberState* __restrict__
     const size_t o = blockIdx.y;
     constexpr size_t stride = (size_t)N * N;
     const size_t base = (size_t)member * G::N3 + o * N + z + t * stride;
-    cpx<T> e[4][G::E];
-#pragma unroll
-    for (int a = 0; a < 4; ++a) {
-#pragma unroll
-        for (int m = 0; m < G::E; ++m) e[a][m] = W[(size_t)a * astride + base + (size_t)G::TPL * m * stride];
-        line_fft<T, N, typename G::Rad, +1, typename G::LayCol>(e[a], t, c, lds, tw);
-    }
     const T inv = (T)(1.0 / (double)G::N3);
-    const T m2 = (T)ms[member].m2, g2 = (T)ms[member].g2;
-    const double m2d = ms[member].m2, g2d = ms[member].g2;
     double v[1] = {0.0};
+    {
+        cpx<T> e[G::E];
+#pragma unroll
+        for (int m = 0; m < G::E; ++m) e[m] = W[base + (size_t)G::TPL * m * stride];
+        line_fft<T, N, typename G::Rad, +1, typename G::LayCol>(e, t, c, lds, tw);
+        const T m2 = (T)ms[member].m2, g2 = (T)ms[member].g2;
+        const T half = (T)0.5;
+#pragma unroll
+        for (int m = 0; m < G::E; ++m) {
+            const T phi = e[m].x * inv, chi = e[m].y * inv;
+            const T chi2 = chi * chi, phi2 = phi * phi;
+            e[m].x = m2 * phi + g2 * phi * chi2;
+            e[m].y = g2 * phi2 * chi;
+            v[0] += (double)(half * m2 * phi2 + half * g2 * phi2 * chi2);
+        }
+        line_fft<T, N, typename G::Rad, -1, typename G::LayCol>(e, t, c, lds, tw);
+#pragma unroll
+        for (int m = 0; m < G::E; ++m) W[base + (size_t)G::TPL * m * stride] = e[m];
+    }
+    cpx<T> g[3][G::E];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+#pragma unroll
+        for (int m = 0; m < G::E; ++m) g[a][m] = W[(size_t)(a + 1) * astride + base + (size_t)G::TPL * m * stride];
+        line_fft<T, N, typename G::Rad, +1, typename G::LayCol>(g[a], t, c, lds, tw);
+    }
 #pragma unroll
     for (int m = 0; m < G::E; ++m) {
-        const T phi = e[0][m].x * inv, chi = e[0][m].y * inv;
-        e[0][m].x = m2 * phi + g2 * phi * (chi * chi);
-        e[0][m].y = g2 * (phi * phi) * chi;
-        const double pd = phi, cdd = chi;
-        v[0] += 0.5 * m2d * (pd * pd) + 0.5 * g2d * (pd * pd) * (cdd * cdd);
-        const T ax = e[1][m].x * inv, bx = e[1][m].y * inv;
-        const T ay = e[2][m].x * inv, by = e[2][m].y * inv;
-        const T az = e[3][m].x * inv, bz = e[3][m].y * inv;
-        e[1][m] = cpx<T>{ax * ax + bx * bx, ay * ay + by * by};
-        e[2][m] = cpx<T>{az * az + bz * bz, ax * ay + bx * by};
-        e[3][m] = cpx<T>{ax * az + bx * bz, ay * az + by * bz};
+        const T ax = g[0][m].x * inv, bx = g[0][m].y * inv;
+        const T ay = g[1][m].x * inv, by = g[1][m].y * inv;
+        const T az = g[2][m].x * inv, bz = g[2][m].y * inv;
+        g[0][m] = cpx<T>{ax * ax + bx * bx, ay * ay + by * by};
+        g[1][m] = cpx<T>{az * az + bz * bz, ax * ay + bx * by};
+        g[2][m] = cpx<T>{ax * az + bx * bz, ay * az + by * bz};
     }
 #pragma unroll
-    for (int a = 0; a < 4; ++a) {
-        line_fft<T, N, typename G::Rad, -1, typename G::LayCol>(e[a], t, c, lds, tw);
+    for (int a = 0; a < 3; ++a) {
+        line_fft<T, N, typename G::Rad, -1, typename G::LayCol>(g[a], t, c, lds, tw);
 #pragma unroll
-        for (int m = 0; m < G::E; ++m) W[(size_t)a * astride + base + (size_t)G::TPL * m * stride] = e[a][m];
+        for (int m = 0; m < G::E; ++m) W[(size_t)(a + 1) * astride + base + (size_t)G::TPL * m * stride] = g[a][m];
     }
     block_sum<1, G::THREADS_COL>(v, reinterpret_cast<double*>(lds));
     if (threadIdx.x == 0)

@@ -12,6 +12,7 @@
 #include <sys/stat.h>
 #include <memory>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/psc.h"
@@ -39,6 +40,9 @@ enum Kind {
 const char* kKindNames[K_NKINDS] = {"z_inverse_drift", "y_inverse", "x_force", "y_forward",
                                     "z_forward_kick", "scalars", "z_inverse", "z_forward_kick_first",
                                     "diagnostics", "z_inverse_drift_gw", "x_force_gw", "z_forward_kick_gw"};
+
+// Chunks of the first reduction level for passes with many blocks.
+constexpr int kPreChunks = 512;
 
 struct Timing {
     double ms = 0.0;
@@ -69,6 +73,8 @@ struct psc_ctx {
     MemberState* ms = nullptr;
     double* partV = nullptr;
     double* partKG = nullptr;
+    double* partV2 = nullptr;   // first-level reduction of partV (large N)
+    double* partKG2 = nullptr;  // first-level reduction of partKG (large N)
     double* part = nullptr;   // diagnostics partials
     double* sums = nullptr;   // diagnostics totals
     size_t part_len = 0, sums_len = 0;
@@ -172,6 +178,11 @@ template <class G>
 struct Engine final : EngineBase {
     using T = typename G::T;
     static constexpr int N = G::N;
+    // The GW x-pass keeps three arrays live in registers: above 512 threads per
+    // block (a 128-VGPR cap) it runs with half the columns per block.
+    using GX = std::conditional_t<(G::THREADS_COL > 512 && G::TC >= 16),
+                                  Shape<T, N, typename G::Rad, G::TC / 2, G::L>, G>;
+    static int nV_blocks(const psc_ctx* c) { return c->cfg.with_gw ? GX::BLOCKS_COL : G::BLOCKS_COL; }
     static cpx<T>* P(void* p) { return static_cast<cpx<T>*>(p); }
 
     int launch_z_inverse(psc_ctx* c, bool drift, const cpx<T>* src, cpx<T>* Pf, const cpx<T>* Pp,
@@ -218,12 +229,28 @@ struct Engine final : EngineBase {
                                P(c->Pp), c->ms, P(c->tw), c->partKG, c1, c2, use_a_start, sc);
         });
     }
-    int launch_scalars(psc_ctx* c, const ScalarOp& op) {
+    // Reduce the substep partials to the scalar update.  Passes with more than
+    // kPreChunks blocks per member are first cut to kPreChunks compensated
+    // chunk sums by many blocks, so the single-block update stays short.
+    int launch_scalars(psc_ctx* c, ScalarOp op) {
         const StepConsts sc = consts_of(c);
+        const double* pv = c->partV;
+        const double* pkg = c->partKG;
+        if (op.finish && op.nV > kPreChunks) {
+            hipLaunchKernelGGL(k_prereduce<1>, dim3(kPreChunks, c->B), dim3(256), 0, c->stream, c->partV, op.nV,
+                               c->partV2, kPreChunks);
+            pv = c->partV2;
+            op.nV = kPreChunks;
+        }
+        if (op.finish && op.nKG > kPreChunks) {
+            hipLaunchKernelGGL(k_prereduce<2>, dim3(kPreChunks, c->B), dim3(256), 0, c->stream, c->partKG, op.nKG,
+                               c->partKG2, kPreChunks);
+            pkg = c->partKG2;
+            op.nKG = kPreChunks;
+        }
         const double bytes = (double)c->B * ((double)(op.nV + 2 * op.nKG) * 8.0 + 2 * sizeof(MemberState));
         return timed(c, K_SCALARS, bytes, [&] {
-            hipLaunchKernelGGL(k_scalars, dim3(c->B), dim3(256), 0, c->stream, c->ms, c->partV, c->partKG,
-                               op, sc);
+            hipLaunchKernelGGL(k_scalars, dim3(c->B), dim3(256), 0, c->stream, c->ms, pv, pkg, op, sc);
         });
     }
 
@@ -246,7 +273,7 @@ struct Engine final : EngineBase {
         if (rc) return rc;
         if ((rc = launch_y<+1>(c, P(c->W), 4 * c->B))) return rc;
         rc = timed(c, K_X_FORCE_GW, 8 * X, [&] {
-            hipLaunchKernelGGL((k_x_force_gw<G>), dim3(N / G::TC, N, c->B), dim3(G::THREADS_COL), 0, c->stream,
+            hipLaunchKernelGGL((k_x_force_gw<GX>), dim3(N / GX::TC, N, c->B), dim3(GX::THREADS_COL), 0, c->stream,
                                P(c->W), c->ms, P(c->tw), c->partV, astride);
         });
         if (rc) return rc;
@@ -291,7 +318,7 @@ struct Engine final : EngineBase {
             op.begin = last ? 0 : 1;
             op.w_finish = c->w[p];
             op.w_begin = wn;
-            op.nV = G::BLOCKS_COL;
+            op.nV = nV_blocks(c);
             op.nKG = G::BLOCKS_ROW;
             if ((rc = launch_scalars(c, op))) return rc;
         }
@@ -315,7 +342,7 @@ struct Engine final : EngineBase {
         ScalarOp op{};
         op.begin = 1;
         op.w_begin = c->w[0];
-        op.nV = G::BLOCKS_COL;
+        op.nV = nV_blocks(c);
         op.nKG = G::BLOCKS_ROW;
         int rc = launch_scalars(c, op);
         if (rc) return rc;
@@ -723,6 +750,8 @@ int psc_create(const psc_config* cfg, psc_ctx** out) {
         !alloc(&c->tw, (size_t)N * c->es) || !alloc((void**)&c->ms, sizeof(MemberState) * c->B) ||
         !alloc((void**)&c->partV, sizeof(double) * blocks_col * c->B) ||
         !alloc((void**)&c->partKG, sizeof(double) * 2 * blocks_row * c->B) ||
+        !alloc((void**)&c->partV2, sizeof(double) * kPreChunks * c->B) ||
+        !alloc((void**)&c->partKG2, sizeof(double) * 2 * kPreChunks * c->B) ||
         !alloc((void**)&c->part, sizeof(double) * c->part_len) ||
         !alloc((void**)&c->sums, sizeof(double) * c->sums_len)) {
         psc_destroy(c.release());
@@ -774,7 +803,8 @@ int psc_destroy(psc_ctx* c) {
         hipEventDestroy(p.b);
     }
     for (auto e : c->ev_free) hipEventDestroy(e);
-    void* ptrs[] = {c->Pf, c->Pp, c->W, c->tw, c->U, c->PU, c->ms, c->partV, c->partKG, c->part, c->sums};
+    void* ptrs[] = {c->Pf, c->Pp, c->W, c->tw, c->U, c->PU, c->ms, c->partV, c->partKG, c->partV2, c->partKG2,
+                    c->part, c->sums};
     for (void* p : ptrs)
         if (p) hipFree(p);
     if (c->stream) hipStreamDestroy(c->stream);

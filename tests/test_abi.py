@@ -43,7 +43,7 @@ def test_abi_version():
     assert L.lib().psc_num_bins(64) == 56
 
 
-@pytest.mark.parametrize("N,nflds,seed", [(16, 2, 1), (32, 1, 7), (16, 2, 0)])
+@pytest.mark.parametrize("N,nflds,seed", [(16, 2, 1), (32, 1, 7), (16, 2, 0), (64, 2, 3)])
 def test_host_ic_matches_oracle_bitwise(N, nflds, seed):
     cfg, model, ic = parity_setup(N=N, nflds=nflds, seed=seed)
     fk, pk, sc = L.generate_ic(cfg, model, ic)
@@ -53,8 +53,9 @@ def test_host_ic_matches_oracle_bitwise(N, nflds, seed):
     np.testing.assert_array_equal(sc, osc[:4])
 
 
-def test_host_gw_ic_matches_oracle_bitwise():
-    cfg, model, ic = parity_setup(N=16, nflds=2, with_gw=1)
+@pytest.mark.parametrize("N", [16, 64])
+def test_host_gw_ic_matches_oracle_bitwise(N):
+    cfg, model, ic = parity_setup(N=N, nflds=2, with_gw=1)
     u, pu = L.generate_gw_ic(cfg, model, ic)
     ou, opu = O.generate_gw_ic(cfg, model, ic)
     np.testing.assert_array_equal(u, ou)
