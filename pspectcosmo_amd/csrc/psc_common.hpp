@@ -48,6 +48,11 @@ struct Shape {
     static constexpr int THREADS_COL = TC * TPL;
     static constexpr int THREADS_ROW = L * TPL;
     static constexpr size_t N3 = (size_t)N * N * N;
+    // Device arrays pad every x-plane with one unused row: plane stride
+    // N (N + 1) elements.  A power-of-two plane stride (2 MiB at 512^3 FP32)
+    // concentrates the x-axis passes' accesses on few DRAM channels.
+    static constexpr size_t PS = (size_t)N * (N + 1);
+    static constexpr size_t MS = (size_t)N * PS;      // member stride
     static constexpr int BLOCKS_COL = (N / TC) * N;   // per member
     static constexpr int BLOCKS_ROW = (N * N) / L;    // per member
     using LayRow = LayoutRow<N, R1>;
@@ -58,6 +63,10 @@ struct Shape {
 };
 
 __device__ __forceinline__ int signed_index(int i, int N) { return i <= N / 2 ? i : i - N; }
+
+// Padded layout of one member: element (x, y, z) at x * N (N + 1) + y * N + z.
+__host__ __device__ constexpr size_t plane_stride(int N) { return (size_t)N * (N + 1); }
+__host__ __device__ constexpr size_t member_stride(int N) { return (size_t)N * plane_stride(N); }
 
 // Block-wide sum of NV doubles, deterministic (fixed shuffle and LDS order).
 // Result valid in thread 0.

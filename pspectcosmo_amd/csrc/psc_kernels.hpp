@@ -34,7 +34,7 @@ k_z_inverse(const cpx<typename G::T>* __restrict__ src, cpx<typename G::T>* __re
     const int t = threadIdx.x % G::TPL, c = threadIdx.x / G::TPL;
     const int member = blockIdx.y;
     const size_t line = (size_t)blockIdx.x * G::L + c;
-    const size_t base = (size_t)member * G::N3 + line * N + t;
+    const size_t base = (size_t)member * G::MS + (line / N) * G::PS + (line % N) * N + t;
     cpx<T> e[G::E];
     if constexpr (DRIFT) {
         const T cd = (T)ms[member].c_drift;
@@ -68,8 +68,8 @@ k_strided(cpx<typename G::T>* __restrict__ W, const cpx<typename G::T>* __restri
     const int member = blockIdx.z;
     const size_t z = (size_t)blockIdx.x * G::TC + c;
     const size_t o = blockIdx.y;
-    constexpr size_t stride = AXIS == 1 ? (size_t)N : (size_t)N * N;
-    const size_t base = (size_t)member * G::N3 + (AXIS == 1 ? o * N * N : o * N) + z + t * stride;
+    constexpr size_t stride = AXIS == 1 ? (size_t)N : G::PS;
+    const size_t base = (size_t)member * G::MS + (AXIS == 1 ? o * G::PS : o * N) + z + t * stride;
     cpx<T> e[G::E];
 #pragma unroll
     for (int m = 0; m < G::E; ++m) e[m] = W[base + (size_t)G::TPL * m * stride];
@@ -93,8 +93,8 @@ k_x_force(cpx<typename G::T>* __restrict__ W, const MemberState* __restrict__ ms
     const int member = blockIdx.z;
     const size_t z = (size_t)blockIdx.x * G::TC + c;
     const size_t o = blockIdx.y;
-    constexpr size_t stride = (size_t)N * N;
-    const size_t base = (size_t)member * G::N3 + o * N + z + t * stride;
+    constexpr size_t stride = G::PS;
+    const size_t base = (size_t)member * G::MS + o * N + z + t * stride;
     cpx<T> e[G::E];
 #pragma unroll
     for (int m = 0; m < G::E; ++m) e[m] = W[base + (size_t)G::TPL * m * stride];
@@ -137,7 +137,7 @@ k_z_kick(const cpx<typename G::T>* __restrict__ W, const cpx<typename G::T>* __r
     const int t = threadIdx.x % G::TPL, c = threadIdx.x / G::TPL;
     const int member = blockIdx.y;
     const size_t line = (size_t)blockIdx.x * G::L + c;
-    const size_t base = (size_t)member * G::N3 + line * N + t;
+    const size_t base = (size_t)member * G::MS + (line / N) * G::PS + (line % N) * N + t;
     cpx<T> e[G::E];
 #pragma unroll
     for (int m = 0; m < G::E; ++m) e[m] = W[base + G::TPL * m];
@@ -281,8 +281,8 @@ k_x_inverse_diag(cpx<typename G::T>* __restrict__ W, const MemberState* __restri
     const int member = blockIdx.z;
     const size_t z = (size_t)blockIdx.x * G::TC + c;
     const size_t o = blockIdx.y;
-    constexpr size_t stride = (size_t)N * N;
-    const size_t base = (size_t)member * G::N3 + o * N + z + t * stride;
+    constexpr size_t stride = G::PS;
+    const size_t base = (size_t)member * G::MS + o * N + z + t * stride;
     cpx<T> e[G::E];
 #pragma unroll
     for (int m = 0; m < G::E; ++m) e[m] = W[base + (size_t)G::TPL * m * stride];
@@ -322,13 +322,15 @@ __global__ void __launch_bounds__(256)
 k_field_sums(const cpx<T>* __restrict__ Pf, const cpx<T>* __restrict__ Pp, double* __restrict__ part,
              int blocks_per_member) {
     constexpr size_t N3 = (size_t)N * N * N;
+    constexpr size_t PS = plane_stride(N);
     const int member = blockIdx.y;
-    const cpx<T>* f = Pf + (size_t)member * N3;
-    const cpx<T>* p = Pp + (size_t)member * N3;
+    const cpx<T>* f = Pf + (size_t)member * member_stride(N);
+    const cpx<T>* p = Pp + (size_t)member * member_stride(N);
     double r[4] = {0, 0, 0, 0};
-    for (size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x; idx < N3; idx += (size_t)blocks_per_member * 256) {
-        const int i = (int)(idx / ((size_t)N * N)), j = (int)((idx / N) % N), k = (int)(idx % N);
-        const size_t midx = ((size_t)((N - i) % N) * N + (N - j) % N) * N + (N - k) % N;
+    for (size_t li = (size_t)blockIdx.x * 256 + threadIdx.x; li < N3; li += (size_t)blocks_per_member * 256) {
+        const int i = (int)(li / ((size_t)N * N)), j = (int)((li / N) % N), k = (int)(li % N);
+        const size_t idx = (size_t)i * PS + (size_t)j * N + k;
+        const size_t midx = (size_t)((N - i) % N) * PS + (size_t)((N - j) % N) * N + (N - k) % N;
         const int px = signed_index(i, N), py = signed_index(j, N), pz = signed_index(k, N);
         const double n2 = (double)(px * px + py * py + pz * pz);
         {
@@ -364,20 +366,20 @@ k_spectra(const cpx<T>* __restrict__ Pf, const cpx<T>* __restrict__ Pp,
           const MemberState* __restrict__ ms, int fld, int alpha, int nbins_total,
           double* __restrict__ part, int blocks_per_member) {
     extern __shared__ double hist[];  // nbins_total * 4
-    constexpr size_t N3 = (size_t)N * N * N;
     constexpr int NH = N / 2 + 1;
     const int member = blockIdx.y;
     for (int i = threadIdx.x; i < nbins_total * 4; i += blockDim.x) hist[i] = 0.0;
     __syncthreads();
-    const cpx<T>* f = Pf + (size_t)member * N3;
-    const cpx<T>* p = Pp + (size_t)member * N3;
+    constexpr size_t PS = plane_stride(N);
+    const cpx<T>* f = Pf + (size_t)member * member_stride(N);
+    const cpx<T>* p = Pp + (size_t)member * member_stride(N);
     const double a = ms[member].a, ad = ms[member].ad;
     const double aop = pow(a, alpha - 3), hub = ad / a;
     const size_t nk = (size_t)N * N * NH;
     for (size_t h = (size_t)blockIdx.x * blockDim.x + threadIdx.x; h < nk; h += (size_t)blocks_per_member * blockDim.x) {
         const int i = (int)(h / ((size_t)N * NH)), j = (int)((h / NH) % N), k = (int)(h % NH);
-        const size_t idx = ((size_t)i * N + j) * N + k;
-        const size_t midx = ((size_t)((N - i) % N) * N + (N - j) % N) * N + (N - k) % N;
+        const size_t idx = (size_t)i * PS + (size_t)j * N + k;
+        const size_t midx = (size_t)((N - i) % N) * PS + (size_t)((N - j) % N) * N + (N - k) % N;
         double fr, fi, pr, pi_;
         {
             const double ar = f[idx].x, ai = f[idx].y, br = f[midx].x, bi = -(double)f[midx].y;
@@ -450,20 +452,6 @@ __global__ void k_sum_partials(const double* __restrict__ part, int nblocks, int
     out[q] = acc.value();
 }
 
-// Packed-pair conversion FP64 -> T on the device (load path).
-template <typename T>
-__global__ void k_convert_in(const double* __restrict__ src, cpx<T>* __restrict__ dst, size_t n) {
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
-        dst[i] = cpx<T>{(T)src[2 * i], (T)src[2 * i + 1]};
-}
-template <typename T>
-__global__ void k_convert_out(const cpx<T>* __restrict__ src, double* __restrict__ dst, size_t n) {
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-        dst[2 * i] = (double)src[i].x;
-        dst[2 * i + 1] = (double)src[i].y;
-    }
-}
-
 }  // namespace psc
 
 // ===========================================================================
@@ -494,7 +482,7 @@ k_z_inverse_gw(cpx<typename G::T>* __restrict__ Pf, const cpx<typename G::T>* __
     const int t = threadIdx.x % G::TPL, c = threadIdx.x / G::TPL;
     const int member = blockIdx.y;
     const size_t line = (size_t)blockIdx.x * G::L + c;
-    const size_t base = (size_t)member * G::N3 + line * N + t;
+    const size_t base = (size_t)member * G::MS + (line / N) * G::PS + (line % N) * N + t;
     const int ix = (int)(line / N), iy = (int)(line % N);
     const T kx = (T)(kIR * deriv_index(ix, N)), ky = (T)(kIR * deriv_index(iy, N));
     const T cd = DRIFT ? (T)ms[member].c_drift : (T)0;
@@ -542,15 +530,13 @@ k_z_inverse_gw(cpx<typename G::T>* __restrict__ Pf, const cpx<typename G::T>* __
     }
 }
 
-// C (GW): the field array as in k_x_force (V' and sum V), then the three
-// gradient arrays: x-axis inverse, the packed source
-// S = (S11 + i S22, S33 + i S12, S13 + i S23), x-axis forward.  The field array
-// is finished before the gradients are loaded, so at most three arrays are
-// live in registers.
+// C (GW), last part: after the fields went through k_x_force and the three
+// gradient arrays through a plain x-axis inverse pass, form the packed source
+// S = (S11 + i S22, S33 + i S12, S13 + i S23) from the real-space gradients
+// and run the three x-axis forward transforms.
 template <class G>
 __global__ void __launch_bounds__((G::THREADS_COL))
-k_x_force_gw(cpx<typename G::T>* __restrict__ W, const MemberState* __restrict__ ms,
-             const cpx<typename G::T>* __restrict__ tw, double* __restrict__ partV, size_t astride) {
+k_x_source(cpx<typename G::T>* __restrict__ W, const cpx<typename G::T>* __restrict__ tw, size_t astride) {
     using T = typename G::T;
     constexpr int N = G::N;
     __shared__ cpx<T> lds[G::LayCol::size(0)];
@@ -558,36 +544,14 @@ k_x_force_gw(cpx<typename G::T>* __restrict__ W, const MemberState* __restrict__
     const int member = blockIdx.z;
     const size_t z = (size_t)blockIdx.x * G::TC + c;
     const size_t o = blockIdx.y;
-    constexpr size_t stride = (size_t)N * N;
-    const size_t base = (size_t)member * G::N3 + o * N + z + t * stride;
+    constexpr size_t stride = G::PS;
+    const size_t base = (size_t)member * G::MS + o * N + z + t * stride;
     const T inv = (T)(1.0 / (double)G::N3);
-    double v[1] = {0.0};
-    {
-        cpx<T> e[G::E];
-#pragma unroll
-        for (int m = 0; m < G::E; ++m) e[m] = W[base + (size_t)G::TPL * m * stride];
-        line_fft<T, N, typename G::Rad, +1, typename G::LayCol>(e, t, c, lds, tw);
-        const T m2 = (T)ms[member].m2, g2 = (T)ms[member].g2;
-        const T half = (T)0.5;
-#pragma unroll
-        for (int m = 0; m < G::E; ++m) {
-            const T phi = e[m].x * inv, chi = e[m].y * inv;
-            const T chi2 = chi * chi, phi2 = phi * phi;
-            e[m].x = m2 * phi + g2 * phi * chi2;
-            e[m].y = g2 * phi2 * chi;
-            v[0] += (double)(half * m2 * phi2 + half * g2 * phi2 * chi2);
-        }
-        line_fft<T, N, typename G::Rad, -1, typename G::LayCol>(e, t, c, lds, tw);
-#pragma unroll
-        for (int m = 0; m < G::E; ++m) W[base + (size_t)G::TPL * m * stride] = e[m];
-    }
     cpx<T> g[3][G::E];
 #pragma unroll
-    for (int a = 0; a < 3; ++a) {
+    for (int a = 0; a < 3; ++a)
 #pragma unroll
         for (int m = 0; m < G::E; ++m) g[a][m] = W[(size_t)(a + 1) * astride + base + (size_t)G::TPL * m * stride];
-        line_fft<T, N, typename G::Rad, +1, typename G::LayCol>(g[a], t, c, lds, tw);
-    }
 #pragma unroll
     for (int m = 0; m < G::E; ++m) {
         const T ax = g[0][m].x * inv, bx = g[0][m].y * inv;
@@ -603,9 +567,6 @@ k_x_force_gw(cpx<typename G::T>* __restrict__ W, const MemberState* __restrict__
 #pragma unroll
         for (int m = 0; m < G::E; ++m) W[(size_t)(a + 1) * astride + base + (size_t)G::TPL * m * stride] = g[a][m];
     }
-    block_sum<1, G::THREADS_COL>(v, reinterpret_cast<double*>(lds));
-    if (threadIdx.x == 0)
-        partV[(size_t)member * G::BLOCKS_COL + blockIdx.y * gridDim.x + blockIdx.x] = v[0];
 }
 
 // E (GW): z-axis forward of V' and of the source, field kick as k_z_kick and
@@ -622,7 +583,7 @@ k_z_kick_gw(const cpx<typename G::T>* __restrict__ W, const cpx<typename G::T>* 
     const int t = threadIdx.x % G::TPL, c = threadIdx.x / G::TPL;
     const int member = blockIdx.y;
     const size_t line = (size_t)blockIdx.x * G::L + c;
-    const size_t base = (size_t)member * G::N3 + line * N + t;
+    const size_t base = (size_t)member * G::MS + (line / N) * G::PS + (line % N) * N + t;
     const double a = use_a_start ? ms[member].a_start : ms[member].a;
     const T norm1 = (T)pow(a, 1 + sc.alpha);
     const T lapc = (T)(pow(a, 1 + sc.alpha) * sc.kIR * sc.kIR);
@@ -702,6 +663,7 @@ k_gw_spectra(const cpx<T>* __restrict__ U, const cpx<T>* __restrict__ PU, size_t
     for (int i = threadIdx.x; i < width; i += blockDim.x) hist[i] = 0.0;
     __syncthreads();
     const size_t nk = (size_t)N * N * NH;
+    constexpr size_t PS = plane_stride(N);
     const double lmax = log(sqrt(3.0) * (N / 2));
     for (size_t h = (size_t)blockIdx.x * blockDim.x + threadIdx.x; h < nk; h += (size_t)blocks * blockDim.x) {
         const int i = (int)(h / ((size_t)N * NH)), j = (int)((h / NH) % N), k = (int)(h % NH);
@@ -722,8 +684,8 @@ k_gw_spectra(const cpx<T>* __restrict__ U, const cpx<T>* __restrict__ PU, size_t
                 continue;
             }
         }
-        const size_t idx = ((size_t)i * N + j) * N + k;
-        const size_t midx = ((size_t)((N - i) % N) * N + (N - j) % N) * N + (N - k) % N;
+        const size_t idx = (size_t)i * PS + (size_t)j * N + k;
+        const size_t midx = (size_t)((N - i) % N) * PS + (size_t)((N - j) % N) * N + (N - k) % N;
         double ur[6], ui[6], pr[6], pi_[6];
 #pragma unroll
         for (int a = 0; a < 3; ++a) {

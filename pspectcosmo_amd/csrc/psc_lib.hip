@@ -35,11 +35,13 @@ enum Kind {
     K_Z_INV_GW,
     K_X_FORCE_GW,
     K_Z_KICK_GW,
+    K_X_INV_GW,
     K_NKINDS
 };
 const char* kKindNames[K_NKINDS] = {"z_inverse_drift", "y_inverse", "x_force", "y_forward",
                                     "z_forward_kick", "scalars", "z_inverse", "z_forward_kick_first",
-                                    "diagnostics", "z_inverse_drift_gw", "x_force_gw", "z_forward_kick_gw"};
+                                    "diagnostics", "z_inverse_drift_gw", "x_source_gw", "z_forward_kick_gw",
+                                    "x_inverse_gw"};
 
 // Chunks of the first reduction level for passes with many blocks.
 constexpr int kPreChunks = 512;
@@ -63,6 +65,7 @@ struct psc_ctx {
     psc_config cfg{};
     int N = 0, B = 1;
     size_t N3 = 0, es = 0;
+    size_t MS = 0;   // member stride of the padded device arrays (elements)
     hipStream_t stream = nullptr;
     void* Pf = nullptr;
     void* Pp = nullptr;
@@ -178,11 +181,7 @@ template <class G>
 struct Engine final : EngineBase {
     using T = typename G::T;
     static constexpr int N = G::N;
-    // The GW x-pass keeps three arrays live in registers: above 512 threads per
-    // block (a 128-VGPR cap) it runs with half the columns per block.
-    using GX = std::conditional_t<(G::THREADS_COL > 512 && G::TC >= 16),
-                                  Shape<T, N, typename G::Rad, G::TC / 2, G::L>, G>;
-    static int nV_blocks(const psc_ctx* c) { return c->cfg.with_gw ? GX::BLOCKS_COL : G::BLOCKS_COL; }
+    static int nV_blocks(const psc_ctx*) { return G::BLOCKS_COL; }
     static cpx<T>* P(void* p) { return static_cast<cpx<T>*>(p); }
 
     int launch_z_inverse(psc_ctx* c, bool drift, const cpx<T>* src, cpx<T>* Pf, const cpx<T>* Pp,
@@ -255,7 +254,7 @@ struct Engine final : EngineBase {
     }
 
     int force_chain_gw(psc_ctx* c, bool drift, bool reduce, double c1, double c2, int use_a_start) {
-        const size_t astride = (size_t)c->B * G::N3;
+        const size_t astride = (size_t)c->B * G::MS;
         const double X = (double)c->B * G::N3 * sizeof(cpx<T>);
         const StepConsts sc = consts_of(c);
         int rc;
@@ -272,9 +271,17 @@ struct Engine final : EngineBase {
             });
         if (rc) return rc;
         if ((rc = launch_y<+1>(c, P(c->W), 4 * c->B))) return rc;
-        rc = timed(c, K_X_FORCE_GW, 8 * X, [&] {
-            hipLaunchKernelGGL((k_x_force_gw<GX>), dim3(N / GX::TC, N, c->B), dim3(GX::THREADS_COL), 0, c->stream,
-                               P(c->W), c->ms, P(c->tw), c->partV, astride);
+        // fields: x inverse, V', x forward; gradients: x inverse, then the
+        // source and its x forward transforms
+        if ((rc = launch_x_force(c))) return rc;
+        rc = timed(c, K_X_INV_GW, 6 * X, [&] {
+            hipLaunchKernelGGL((k_strided<G, 0, +1>), dim3(N / G::TC, N, 3 * c->B), dim3(G::THREADS_COL), 0,
+                               c->stream, P(c->W) + astride, P(c->tw));
+        });
+        if (rc) return rc;
+        rc = timed(c, K_X_FORCE_GW, 6 * X, [&] {
+            hipLaunchKernelGGL((k_x_source<G>), dim3(N / G::TC, N, c->B), dim3(G::THREADS_COL), 0, c->stream,
+                               P(c->W), P(c->tw), astride);
         });
         if (rc) return rc;
         if ((rc = launch_y<-1>(c, P(c->W), 4 * c->B))) return rc;
@@ -361,7 +368,7 @@ struct Engine final : EngineBase {
     }
 
     int real_space(psc_ctx* c, int member, const void* src) override {
-        const size_t off = (size_t)member * G::N3;
+        const size_t off = (size_t)member * G::MS;
         cpx<T>* W = P(c->W) + off;
         int rc = launch_z_inverse(c, false, static_cast<const cpx<T>*>(src) + off, nullptr, nullptr, W,
                                   c->ms + member, 1);
@@ -380,7 +387,7 @@ struct Engine final : EngineBase {
     }
 
     int field_sums(psc_ctx* c, int member) override {
-        const size_t off = (size_t)member * G::N3;
+        const size_t off = (size_t)member * G::MS;
         const int nb = 256;
         hipLaunchKernelGGL((k_field_sums<T, N>), dim3(nb, 1), dim3(256), 0, c->stream, P(c->Pf) + off,
                            P(c->Pp) + off, c->part, nb);
@@ -390,7 +397,7 @@ struct Engine final : EngineBase {
     }
 
     int spectra(psc_ctx* c, int member, int fld) override {
-        const size_t off = (size_t)member * G::N3;
+        const size_t off = (size_t)member * G::MS;
         const int nbt = (int)(std::sqrt(3.0) * (N / 2)) + 2;
         const int nb = 128;
         const size_t shmem = (size_t)nbt * 4 * sizeof(double);
@@ -408,14 +415,14 @@ struct Engine final : EngineBase {
 
 template <typename T, int N>
 int gw_spectra_impl(psc_ctx* c, int member, int projection, int nlog) {
-    const size_t off = (size_t)member * (size_t)N * N * N;
+    const size_t off = (size_t)member * member_stride(N);
     const int nbt = (int)(std::sqrt(3.0) * (N / 2)) + 2;
     const int nb = 128;
     const int width = 3 * nbt + nlog;
     const size_t shmem = (size_t)width * sizeof(double);
     hipLaunchKernelGGL((k_gw_spectra<T, N>), dim3(nb), dim3(256), shmem, c->stream,
                        static_cast<cpx<T>*>(c->U) + off, static_cast<cpx<T>*>(c->PU) + off,
-                       (size_t)c->B * N * N * N, projection, nbt, nlog, c->part, nb);
+                       (size_t)c->B * member_stride(N), projection, nbt, nlog, c->part, nb);
     hipLaunchKernelGGL(k_sum_partials, dim3((width + 255) / 256), dim3(256), 0, c->stream, c->part, nb, width,
                        c->sums);
     HIPCHK(c, hipGetLastError());
@@ -562,30 +569,34 @@ void unpack_half(int N, int nflds, const double* full, double* hk) {
             }
 }
 
+// Copy one unpadded N^3 complex FP64 array into slot `member` of a padded
+// device array (x-planes N (N + 1) apart), converting to the field precision.
 int upload(psc_ctx* c, void* dst_base, int member, const std::vector<double>& full) {
-    const size_t n = c->N3;
-    char* dst = static_cast<char*>(dst_base) + (size_t)member * n * c->es;
+    const size_t n = c->N3, plane = (size_t)c->N * c->N;
+    char* dst = static_cast<char*>(dst_base) + (size_t)member * c->MS * c->es;
+    const size_t dpitch = plane_stride(c->N) * c->es, width = plane * c->es;
     if (c->cfg.precision == PSC_FP32) {
         std::vector<float> tmp(2 * n);
         for (size_t i = 0; i < 2 * n; ++i) tmp[i] = (float)full[i];
-        HIPCHK(c, hipMemcpy(dst, tmp.data(), tmp.size() * sizeof(float), hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy2D(dst, dpitch, tmp.data(), width, width, c->N, hipMemcpyHostToDevice));
     } else {
-        HIPCHK(c, hipMemcpy(dst, full.data(), full.size() * sizeof(double), hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy2D(dst, dpitch, full.data(), width, width, c->N, hipMemcpyHostToDevice));
     }
     return PSC_OK;
 }
 
 int download(psc_ctx* c, const void* src_base, int member, std::vector<double>& full) {
-    const size_t n = c->N3;
-    const char* src = static_cast<const char*>(src_base) + (size_t)member * n * c->es;
+    const size_t n = c->N3, plane = (size_t)c->N * c->N;
+    const char* src = static_cast<const char*>(src_base) + (size_t)member * c->MS * c->es;
+    const size_t spitch = plane_stride(c->N) * c->es, width = plane * c->es;
     full.resize(2 * n);
     HIPCHK(c, hipStreamSynchronize(c->stream));
     if (c->cfg.precision == PSC_FP32) {
         std::vector<float> tmp(2 * n);
-        HIPCHK(c, hipMemcpy(tmp.data(), src, tmp.size() * sizeof(float), hipMemcpyDeviceToHost));
+        HIPCHK(c, hipMemcpy2D(tmp.data(), width, src, spitch, width, c->N, hipMemcpyDeviceToHost));
         for (size_t i = 0; i < 2 * n; ++i) full[i] = tmp[i];
     } else {
-        HIPCHK(c, hipMemcpy(full.data(), src, full.size() * sizeof(double), hipMemcpyDeviceToHost));
+        HIPCHK(c, hipMemcpy2D(full.data(), width, src, spitch, width, c->N, hipMemcpyDeviceToHost));
     }
     return PSC_OK;
 }
@@ -717,6 +728,7 @@ int psc_create(const psc_config* cfg, psc_ctx** out) {
     c->N = N;
     c->B = cfg->batch;
     c->N3 = (size_t)N * N * N;
+    c->MS = member_stride(N);
     c->es = cfg->precision == PSC_FP32 ? 2 * sizeof(float) : 2 * sizeof(double);
     {
         const std::vector<int> cand = shapes_for(cfg->precision, N);
@@ -733,7 +745,7 @@ int psc_create(const psc_config* cfg, psc_ctx** out) {
     c->models.assign(c->B, psc_model{1.0, 0.0});
     c->loaded.assign(c->B, 0);
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return PSC_ERR_DEVICE;
-    const size_t arr = (size_t)c->B * c->N3 * c->es;
+    const size_t arr = (size_t)c->B * c->MS * c->es;
     auto alloc = [&](void** p, size_t bytes) {
         if (hipMalloc(p, bytes) != hipSuccess) return false;
         c->dev_bytes += (double)bytes;
@@ -1074,7 +1086,7 @@ int gw_histograms(psc_ctx* c, int member, int projection, int nlog, std::vector<
     *hd0 = 0.0;
     for (int a = 0; a < 3; ++a) {
         double dc[2];
-        const char* src = static_cast<const char*>(c->PU) + ((size_t)a * c->B + member) * c->N3 * c->es;
+        const char* src = static_cast<const char*>(c->PU) + ((size_t)a * c->B + member) * c->MS * c->es;
         if (c->cfg.precision == PSC_FP32) {
             float f2[2];
             HIPCHK(c, hipMemcpyAsync(f2, src, sizeof(f2), hipMemcpyDeviceToHost, c->stream));
@@ -1159,7 +1171,7 @@ int psc_autotune(psc_ctx* c, int nsteps, char* report, int report_len) {
     if (nsteps < 1) return c->fail(PSC_ERR_ARG, "nsteps must be >= 1");
     for (int b = 0; b < c->B; ++b)
         if (!c->loaded[b]) return c->fail(PSC_ERR_STATE, "autotune needs loaded initial data");
-    const size_t arr = (size_t)c->B * c->N3 * c->es;
+    const size_t arr = (size_t)c->B * c->MS * c->es;
     const bool gw = c->cfg.with_gw != 0;
     void *sf = nullptr, *sp = nullptr, *su = nullptr, *spu = nullptr;
     std::vector<MemberState> ms(c->B);
