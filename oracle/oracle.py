@@ -57,6 +57,9 @@ def lib():
         L.orc_spectra.argtypes = [C.c_void_p, C.c_int, _D, C.c_int]
         L.orc_gw_spectrum.argtypes = [C.c_void_p, C.c_int, _D, C.c_int, _D]
         L.orc_gw_omega_log.argtypes = [C.c_void_p, C.c_int, C.c_int, _D, _D]
+        L.orc_adjoint_chi2.argtypes = [P(PscConfig), P(PscModel), _D, _D, C.c_double, C.c_double,
+                                       C.c_double, C.c_double, C.c_int, P(C.c_double), P(C.c_double),
+                                       P(C.c_double), C.c_void_p]
         _lib = L
     return _lib
 
@@ -119,6 +122,22 @@ def generate_gw_ic(cfg, model, ic):
                                   u.view(np.float64).ravel(), pu.view(np.float64).ravel())
     assert rc == 0, rc
     return u, pu
+
+
+def adjoint_chi2(cfg, model, fk, pk, a0, phi0, dphi0, chi0, nsteps, want_field=True):
+    """FP64 reference gradient of the final <chi^2>: (J, dJ/dg2, dJ/dphi0,
+    dJ/dchi(x) as an (N, N, N) array or None)."""
+    fk = np.ascontiguousarray(fk, dtype=np.complex128)
+    pk = np.ascontiguousarray(pk, dtype=np.complex128)
+    J, g2b, p0b = C.c_double(), C.c_double(), C.c_double()
+    N = cfg.N
+    chib = np.zeros((N, N, N)) if want_field else None
+    rc = lib().orc_adjoint_chi2(C.byref(cfg), C.byref(model), fk.view(np.float64).ravel(),
+                                pk.view(np.float64).ravel(), a0, phi0, dphi0, chi0, int(nsteps),
+                                C.byref(J), C.byref(g2b), C.byref(p0b),
+                                chib.ctypes.data_as(C.c_void_p) if want_field else None)
+    assert rc == 0, rc
+    return J.value, g2b.value, p0b.value, chib
 
 
 class Oracle:
