@@ -17,9 +17,13 @@ Other workloads (own JSON line each):
                          g^2 and phi0 from pspectcosmo_amd/workloads/
                          ensemble_128_64.json), velocity Verlet, dt = 0.004,
                          FP32-mixed, against the same members run one by one
+  --workload adjoint128  gradient of the final <chi^2> w.r.t. g^2, phi0 and the
+                         initial chi field at 128^3, FP32-mixed velocity Verlet,
+                         500 steps, a checkpoint every 25 steps: adjoint/forward
+                         time, extra memory, central differences (eps = 1e-3)
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
-                       [--workload preheat256|gw512|ensemble128] [--autotune]
+                       [--workload preheat256|gw512|ensemble128|adjoint128] [--autotune]
 """
 import argparse
 import json
@@ -231,6 +235,127 @@ def bench_ensemble128(args):
     print(json.dumps(out))
 
 
+ADJ_N, ADJ_STEPS, ADJ_EVERY, ADJ_KIR, ADJ_G2, ADJ_DT, ADJ_OMEGA = 128, 500, 25, 2.0, 1e4, 0.004, 1e-4
+PHI0, DPHI0 = 1.0, -0.7
+
+
+_ADJ_IC = {}
+
+
+def adjoint_case(precision, phi0=PHI0, g2=ADJ_G2, dchi=None, tuned=False):
+    """Context loaded with the adjoint workload's initial state.  The
+    fluctuations are generated once (with g^2 = 1e4) and held fixed, as the
+    gradient holds them: g2 changes only the coupling, phi0 sets the DC mode
+    and da/dt = a H(phi0), dchi (real space) is added to chi."""
+    n = ADJ_N
+    cfg = make_config(n, nflds=2, integrator=2, precision=precision, kIR=ADJ_KIR, dt=ADJ_DT,
+                      omegaStar=ADJ_OMEGA, use_graph=1, autotune=1 if tuned else 0)
+    if not _ADJ_IC:
+        _ADJ_IC["ic"] = generate_ic(cfg, make_model(1.0, ADJ_G2), make_ic(f0=(PHI0, 0.0), fd0=(DPHI0, 0.0), seed=1))
+    fk, pk, _ = _ADJ_IC["ic"]
+    fk = fk.copy()
+    fk[0, 0, 0, 0] = phi0 * n ** 3
+    if dchi is not None:
+        fk[1] += np.fft.rfftn(dchi)
+    H = math.sqrt((DPHI0 ** 2 / 2 + 0.5 * phi0 ** 2) / 3.0)
+    lat = Lattice(cfg)
+    lat.set_model(0, make_model(1.0, g2))
+    lat.load_ic_k(0, fk, pk, 1.0, H)
+    return lat
+
+
+def chi2_after(lat, steps):
+    lat.step(steps)
+    f, _ = lat.fields()
+    return float((f[1] ** 2).mean())
+
+
+def bench_adjoint128(args):
+    steps, every = ADJ_STEPS, ADJ_EVERY
+    tuned = not args.no_autotune
+    lat = adjoint_case(PSC_FP32, tuned=tuned)
+    if tuned:
+        print("[bench] autotune", lat.autotune(3), file=sys.stderr, flush=True)
+    # plain forward of the same run (graph-captured steps)
+    lat.step(args.warmup)
+    device_sync(lat)
+    lat.close()
+    lat = adjoint_case(PSC_FP32, tuned=tuned)
+    shape = lat.shape_name()
+    device_sync(lat)
+    t0 = time.perf_counter()
+    lat.step(steps)
+    device_sync(lat)
+    fwd = time.perf_counter() - t0
+    lat.close()
+    lat = adjoint_case(PSC_FP32, tuned=tuned)
+    lat.adjoint(steps, every, PHI0, DPHI0, want_field=False)   # warm-up: workspace, captured segments
+    t0 = time.perf_counter()
+    g = lat.adjoint(steps, every, PHI0, DPHI0)
+    wall = time.perf_counter() - t0
+    dev = lat.device_bytes()
+    lat.close()
+    print(f"[bench] adjoint fp32: J {g['J']:.6e} dg2 {g['dg2']:.6e} dphi0 {g['dphi0']:.6e} "
+          f"fwd {fwd * 1e3:.1f} ms, adjoint fwd {g['forward_ms']:.1f} + bwd {g['backward_ms']:.1f} ms",
+          file=sys.stderr, flush=True)
+    if args.no_fd:
+        print(json.dumps({"metric": "adjoint/forward time at 128^3, FP32-mixed", "value": g["backward_ms"] / (fwd * 1e3),
+                          "unit": "x", "profiling_run": True}))
+        return
+    # FP64 GPU gradient of the same run
+    l64 = adjoint_case(PSC_FP64)
+    g64 = l64.adjoint(steps, every, PHI0, DPHI0)
+    l64.close()
+    # central differences of the FP32 forward (eps = 1e-3 relative) and of the FP64 forward
+    fd = {}
+    for name, prec in (("fp32", PSC_FP32), ("fp64", PSC_FP64)):
+        e = 1e-3 * ADJ_G2
+        Jp = chi2_after(adjoint_case(prec, g2=ADJ_G2 + e), steps)
+        Jm = chi2_after(adjoint_case(prec, g2=ADJ_G2 - e), steps)
+        dg2 = (Jp - Jm) / (2 * e)
+        e = 1e-3 * PHI0
+        Jp = chi2_after(adjoint_case(prec, phi0=PHI0 + e), steps)
+        Jm = chi2_after(adjoint_case(prec, phi0=PHI0 - e), steps)
+        dphi0 = (Jp - Jm) / (2 * e)
+        v = np.random.default_rng(7).standard_normal((ADJ_N,) * 3)
+        e = 1e-3 * math.sqrt(g["J"])
+        Jp = chi2_after(adjoint_case(prec, dchi=e * v), steps)
+        Jm = chi2_after(adjoint_case(prec, dchi=-e * v), steps)
+        fd[name] = {"dg2": dg2, "dphi0": dphi0, "dchi_dir": (Jp - Jm) / (2 * e)}
+    v = np.random.default_rng(7).standard_normal((ADJ_N,) * 3)
+    ad_dir = float((g["dchi"] * v).sum())
+    ad64_dir = float((g64["dchi"] * v).sum())
+
+    def rel(a, b):
+        return abs(a - b) / abs(b)
+
+    ratio = g["backward_ms"] / (fwd * 1e3)
+    out = {"metric": "adjoint/forward time at 128^3, 500 VV steps, checkpoint every 25, FP32-mixed",
+           "value": ratio, "unit": "x", "n_gpus": 1, "steps": steps, "warmup": args.warmup,
+           "ms_per_step": fwd / steps * 1e3, "higher_is_better": False, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+           "config": {"workload": "128^3 two-field lattice, g^2 = 1e4, kIR = 2, omegaStar = 1e-4, "
+                                  "velocity Verlet dt = 0.004, 500 steps, checkpoint every 25; "
+                                  "objective <chi^2> at the end", "shape": shape},
+           "forward_ms": fwd * 1e3,
+           "adjoint_forward_ms": g["forward_ms"], "adjoint_backward_ms": g["backward_ms"],
+           "adjoint_total_over_forward": (g["forward_ms"] + g["backward_ms"]) / (fwd * 1e3),
+           "adjoint_wall_s": wall,
+           "extra_device_bytes": g["extra_bytes"], "context_device_bytes": dev,
+           "J": g["J"],
+           "grad_fp32": {"dg2": g["dg2"], "dphi0": g["dphi0"], "dchi_dir": ad_dir},
+           "grad_fp64": {"dg2": g64["dg2"], "dphi0": g64["dphi0"], "dchi_dir": ad64_dir},
+           "fd": fd,
+           "rel_fp32_vs_fp64": {"dg2": rel(g["dg2"], g64["dg2"]), "dphi0": rel(g["dphi0"], g64["dphi0"]),
+                                "dchi_field": float(np.linalg.norm(g["dchi"] - g64["dchi"]) /
+                                                    np.linalg.norm(g64["dchi"]))},
+           "rel_fp32_vs_fd32": {k: rel(a, fd["fp32"][k]) for k, a in
+                                (("dg2", g["dg2"]), ("dphi0", g["dphi0"]), ("dchi_dir", ad_dir))},
+           "rel_fp64_vs_fd64": {k: rel(a, fd["fp64"][k]) for k, a in
+                                (("dg2", g64["dg2"]), ("dphi0", g64["dphi0"]), ("dchi_dir", ad64_dir))}}
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -240,10 +365,11 @@ def main():
     ap.add_argument("--fp64-steps", type=int, default=None)
     ap.add_argument("--no-graph", action="store_true", help="launch kernels one by one")
     ap.add_argument("--fp32-only", action="store_true", help="profiling runs: FP32 path only")
-    ap.add_argument("--workload", default="preheat256", choices=["preheat256", "gw512", "ensemble128"])
+    ap.add_argument("--workload", default="preheat256", choices=["preheat256", "gw512", "ensemble128", "adjoint128"])
     ap.add_argument("--autotune", action="store_true", help="time the launch shapes first")
     ap.add_argument("--no-autotune", action="store_true", help="headline: keep the default shapes")
     ap.add_argument("--seq-steps", type=int, default=150, help="ensemble: steps per member run alone")
+    ap.add_argument("--no-fd", action="store_true", help="adjoint: skip the finite-difference and FP64 runs")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs the lattice on a single GPU; --gpus "
@@ -252,6 +378,8 @@ def main():
         sys.exit(2)
     if args.workload == "gw512":
         return bench_gw512(args)
+    if args.workload == "adjoint128":
+        return bench_adjoint128(args)
     if args.workload == "ensemble128":
         return bench_ensemble128(args)
 

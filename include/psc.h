@@ -190,6 +190,36 @@ int psc_gw_omega_log(psc_ctx* ctx, int member, int projection, int nlog, double*
 /* Number of FP64 bins of psc_spectra for a lattice of N points per edge. */
 int psc_num_bins(int N);
 
+/* ---- adjoint (no counterpart in the reference) -------------------------- */
+
+#define PSC_OBJ_CHI2 0          /* <chi^2> over the lattice at the end of the run */
+
+typedef struct psc_adjoint_spec {
+    int objective;          /* PSC_OBJ_CHI2                                          */
+    int nsteps;             /* forward steps of cfg->dt from the loaded state        */
+    int checkpoint_every;   /* steps between stored states; each segment is replayed
+                               once with recording before it is swept backwards     */
+    int member;             /* 0 (single-lattice contexts only)                      */
+    double phi0, dphi0, chi0;  /* homogeneous initial values; da/dt(0) = a0 H(phi0)
+                                  as initialize.cpp:649-668                         */
+    double reserved[5];
+} psc_adjoint_spec;
+
+/* Gradient of the objective by the discrete adjoint of the step (reverse
+ * kick-drift over the recorded substeps, Friedmann coupling included), with
+ * respect to g2, phi0 (the homogeneous mode and H(phi0), fluctuation modes
+ * held fixed) and the initial real-space chi field.  The run is split into
+ * segments of checkpoint_every steps: a forward pass stores each segment's
+ * start state, then from the last segment backwards the segment is replayed
+ * with recording and swept back.  The loaded state is left unchanged.
+ * out[0..5] = J, dJ/dg2, dJ/dphi0, forward milliseconds, backward milliseconds
+ * (replays and sweeps), device bytes of the adjoint workspace.  dchi receives
+ * N^3 values dJ/dchi(x) when not NULL.  The workspace (checkpoints, one
+ * recorded segment, adjoint arrays) stays allocated for further calls until
+ * psc_destroy, psc_set_shape or psc_autotune.  Requires nflds == 2, batch == 1
+ * and no GW evolution. */
+int psc_adjoint(psc_ctx* ctx, const psc_adjoint_spec* spec, double* out, double* dchi);
+
 /* ---- launch-shape autotuner (no counterpart in the reference) ----------- */
 
 /* Times every candidate FFT launch shape (radix plan, z columns per block of

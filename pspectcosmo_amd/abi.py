@@ -22,6 +22,7 @@ PSC_GW_PROJ_TT = 1
 PSC_ENERGY_LEN = 16
 PSC_SPECTRA_COLS = 5
 PSC_GW_SPECTRA_COLS = 5
+PSC_OBJ_CHI2 = 0
 
 
 class PscConfig(C.Structure):
@@ -43,6 +44,13 @@ class PscIcParams(C.Structure):
     _fields_ = [
         ("f0", C.c_double * 2), ("fd0", C.c_double * 2), ("a0", C.c_double),
         ("seed", C.c_int), ("reserved_i", C.c_int), ("reserved_d", C.c_double * 4),
+    ]
+
+
+class PscAdjointSpec(C.Structure):
+    _fields_ = [
+        ("objective", C.c_int), ("nsteps", C.c_int), ("checkpoint_every", C.c_int), ("member", C.c_int),
+        ("phi0", C.c_double), ("dphi0", C.c_double), ("chi0", C.c_double), ("reserved", C.c_double * 5),
     ]
 
 

@@ -1,0 +1,310 @@
+// psc_adjoint.hpp -- discrete adjoint of the spectral lattice step.
+//
+// Reverse-mode derivative of evolve_VV_k_nopad (evolution.cpp:1131-1198) with
+// the Friedmann coupling of kernel_a_k_nopad (evolution.cpp:1061-1090), on the
+// packed spectra.  Conventions: for a real objective J and a complex array z,
+// zbar = dJ/dRe z + i dJ/dIm z.  With r = F^H f / N^3 the packed real-space
+// fields, the force F(f, a) = -a^{1+al} kIR^2 n^2 f - a^{3+al} F(V'(r)) has the
+// adjoint  fbar += -a^{1+al} kIR^2 n^2 Fbar + F(Hess V(r) . F^H(-a^{3+al} Fbar)) / N^3.
+//
+// One backward substep s (end point e = s + 1) runs five passes:
+//   P1 z-axis row pass: the add-adjoint and kick-adjoint updates of pibar,
+//      fbar; Fbar = h pibar + h_next pibar_old (the first kick of substep s + 1
+//      evaluates the force at the same point, so its adjoint h_next pibar_old,
+//      pibar as that substep's sweep left it, is merged here);
+//      ybar = -a'^{3+al} Fbar, z inverse.
+//   P2 y-axis inverse (k_strided).
+//   P3 x-axis inverse -> vbar = F^H ybar; q = Hess V(r') vbar + cV V'(r')
+//      (the V part of the add adjoint); x forward.  Reductions for abar, g2bar.
+//   P4 y-axis forward (k_strided).
+//   P5 z-axis forward -> fbar += q_k / N^3; the drift adjoint
+//      pibar_h = pibar + 2 h a_h^{al-3} fbar.
+// The scalar recursion (abar, bbar, addbar, g2bar) runs in k_adj_scalars.
+#pragma once
+#include "psc_kernels.hpp"
+
+namespace psc {
+
+// Coefficients of one backward substep, written by k_adj_scalars.
+struct AdjCoef {
+    double h;    // w dt / 2
+    double hn;   // w dt / 2 of the substep undone before (0 at the end of the run)
+    double ck;   // addbar cA (al - 2) / (a'^6 N^6)    (K part of the add adjoint)
+    double cg;   // addbar cA al kIR^2 / (a'^2 N^6)     (G part)
+    double c1;   // a'^{1+al} kIR^2
+    double c3;   // a'^{3+al}
+    double cV;   // addbar cA (al + 1) / N^3            (V part)
+    double cd;   // 2 h a_h^{al-3}                      (drift)
+    int final_pass;  // t = 0: no substep to undo, only the first-kick force and add0
+    int pad;
+};
+
+// Scalar adjoint state.
+struct AdjState {
+    double abar, bbar, addbar, g2bar;
+    double addbar_used;   // addbar of the substep being undone (V part of its add adjoint)
+    double a_e;           // forward a at that substep's end point
+    double h_undo;        // w dt / 2 of that substep
+    double ah_undo;       // its a_half
+    int pending;          // a substep has been prepared and awaits its POST update
+    int pad_i;
+    double pad[7];
+};
+
+// P1
+template <class G>
+__global__ void __launch_bounds__((G::THREADS_ROW))
+k_adj_p1(const cpx<typename G::T>* __restrict__ fe, const cpx<typename G::T>* __restrict__ pe,
+         cpx<typename G::T>* __restrict__ fbar, cpx<typename G::T>* __restrict__ pbar,
+         cpx<typename G::T>* __restrict__ W, const AdjCoef* __restrict__ co,
+         const cpx<typename G::T>* __restrict__ tw, double* __restrict__ part) {
+    using T = typename G::T;
+    constexpr int N = G::N;
+    __shared__ cpx<T> lds[G::LayRow::size(G::L)];
+    const int t = threadIdx.x % G::TPL, c = threadIdx.x / G::TPL;
+    const size_t line = (size_t)blockIdx.x * G::L + c;
+    const size_t base = (line / N) * G::PS + (line % N) * N + t;
+    const T h = (T)co->h, hn = (T)co->hn, ck = (T)co->ck, cg = (T)co->cg, c1 = (T)co->c1, c3 = (T)co->c3;
+    const int px = signed_index((int)(line / N), N), py = signed_index((int)(line % N), N);
+    const int pxy = px * px + py * py;
+    double red[1] = {0.0};
+    cpx<T> e[G::E];
+#pragma unroll
+    for (int m = 0; m < G::E; ++m) {
+        const size_t idx = base + G::TPL * m;
+        const int pz = signed_index(t + G::TPL * m, N);
+        const T n2 = (T)(pxy + pz * pz);
+        const cpx<T> f = fe[idx], p = pe[idx], po = pbar[idx];
+        cpx<T> fb = fbar[idx];
+        const cpx<T> pb = {po.x + ck * p.x, po.y + ck * p.y};
+        pbar[idx] = pb;
+        const cpx<T> Fb = {h * pb.x + hn * po.x, h * pb.y + hn * po.y};
+        fb.x += cg * n2 * f.x - c1 * n2 * Fb.x;
+        fb.y += cg * n2 * f.y - c1 * n2 * Fb.y;
+        fbar[idx] = fb;
+        red[0] += (double)n2 * ((double)Fb.x * f.x + (double)Fb.y * f.y);
+        e[m] = cpx<T>{-c3 * Fb.x, -c3 * Fb.y};
+    }
+    line_fft<T, N, typename G::Rad, +1, typename G::LayRow>(e, t, c, lds, tw);
+#pragma unroll
+    for (int m = 0; m < G::E; ++m) W[base + G::TPL * m] = e[m];
+    block_sum<1, G::THREADS_ROW>(red, reinterpret_cast<double*>(lds));
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+// P3
+template <class G>
+__global__ void __launch_bounds__((G::THREADS_COL))
+k_adj_p3(cpx<typename G::T>* __restrict__ W, const cpx<typename G::T>* __restrict__ re,
+         const MemberState* __restrict__ ms, const AdjCoef* __restrict__ co,
+         const cpx<typename G::T>* __restrict__ tw, double* __restrict__ part) {
+    using T = typename G::T;
+    constexpr int N = G::N;
+    __shared__ cpx<T> lds[G::LayCol::size(0)];
+    const int c = threadIdx.x % G::TC, t = threadIdx.x / G::TC;
+    const size_t z = (size_t)blockIdx.x * G::TC + c;
+    const size_t o = blockIdx.y;
+    constexpr size_t stride = G::PS;
+    const size_t base = o * N + z + t * stride;
+    cpx<T> e[G::E];
+#pragma unroll
+    for (int m = 0; m < G::E; ++m) e[m] = W[base + (size_t)G::TPL * m * stride];
+    line_fft<T, N, typename G::Rad, +1, typename G::LayCol>(e, t, c, lds, tw);
+    const T m2 = (T)ms->m2, g2 = (T)ms->g2, cV = (T)co->cV;
+    double red[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int m = 0; m < G::E; ++m) {
+        const cpx<T> r = re[base + (size_t)G::TPL * m * stride];
+        const T ph = r.x, ch = r.y;
+        const cpx<T> vb = e[m];
+        const T vpx = m2 * ph + g2 * ph * (ch * ch), vpy = g2 * (ph * ph) * ch;
+        red[0] += (double)vb.x * vpx + (double)vb.y * vpy;
+        red[1] += (double)vb.x * (ph * ch * ch) + (double)vb.y * (ph * ph * ch);
+        red[2] += 0.5 * (double)(ph * ph) * (double)(ch * ch);
+        const T hpp = m2 + g2 * ch * ch, hpc = (T)2 * g2 * ph * ch, hcc = g2 * ph * ph;
+        e[m] = cpx<T>{hpp * vb.x + hpc * vb.y + cV * vpx, hpc * vb.x + hcc * vb.y + cV * vpy};
+    }
+    line_fft<T, N, typename G::Rad, -1, typename G::LayCol>(e, t, c, lds, tw);
+#pragma unroll
+    for (int m = 0; m < G::E; ++m) W[base + (size_t)G::TPL * m * stride] = e[m];
+    block_sum<3, G::THREADS_COL>(red, reinterpret_cast<double*>(lds));
+    if (threadIdx.x == 0) {
+        double* dst = part + ((size_t)blockIdx.y * gridDim.x + blockIdx.x) * 3;
+        dst[0] = red[0];
+        dst[1] = red[1];
+        dst[2] = red[2];
+    }
+}
+
+// P5
+template <class G>
+__global__ void __launch_bounds__((G::THREADS_ROW))
+k_adj_p5(const cpx<typename G::T>* __restrict__ W, cpx<typename G::T>* __restrict__ fbar,
+         cpx<typename G::T>* __restrict__ pbar, const cpx<typename G::T>* __restrict__ ph,
+         const AdjCoef* __restrict__ co,
+         const cpx<typename G::T>* __restrict__ tw, double* __restrict__ part) {
+    using T = typename G::T;
+    constexpr int N = G::N;
+    __shared__ cpx<T> lds[G::LayRow::size(G::L)];
+    const int t = threadIdx.x % G::TPL, c = threadIdx.x / G::TPL;
+    const size_t line = (size_t)blockIdx.x * G::L + c;
+    const size_t base = (line / N) * G::PS + (line % N) * N + t;
+    cpx<T> e[G::E];
+#pragma unroll
+    for (int m = 0; m < G::E; ++m) e[m] = W[base + G::TPL * m];
+    line_fft<T, N, typename G::Rad, -1, typename G::LayRow>(e, t, c, lds, tw);
+    const T inv = (T)(1.0 / (double)G::N3);
+    const T cd = (T)co->cd;
+    const bool fin = co->final_pass != 0;
+    double red[1] = {0.0};
+#pragma unroll
+    for (int m = 0; m < G::E; ++m) {
+        const size_t idx = base + G::TPL * m;
+        cpx<T> fb = fbar[idx];
+        fb.x += inv * e[m].x;
+        fb.y += inv * e[m].y;
+        fbar[idx] = fb;
+        if (!fin) {
+            const cpx<T> p = ph[idx];
+            red[0] += (double)fb.x * p.x + (double)fb.y * p.y;
+            cpx<T> pb = pbar[idx];
+            pb.x += cd * fb.x;
+            pb.y += cd * fb.y;
+            pbar[idx] = pb;
+        }
+    }
+    block_sum<1, G::THREADS_ROW>(red, reinterpret_cast<double*>(lds));
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+// Compensated single-block sums of the five adjoint reductions: R1 (P1
+// partials), R2..R4 (P3 partials, width 3), R5 (P5 partials, when r5).
+__device__ void adj_block_reduce(const double* __restrict__ p1, int n1, const double* __restrict__ p3, int n3,
+                                 const double* __restrict__ p5, int n5, bool r5, double* sh, double* R) {
+    Comp acc[5];
+    for (int i = threadIdx.x; i < n1; i += blockDim.x) acc[0].add(p1[i]);
+    for (int i = threadIdx.x; i < n3; i += blockDim.x) {
+        acc[1].add(p3[3 * (size_t)i]);
+        acc[2].add(p3[3 * (size_t)i + 1]);
+        acc[3].add(p3[3 * (size_t)i + 2]);
+    }
+    if (r5)
+        for (int i = threadIdx.x; i < n5; i += blockDim.x) acc[4].add(p5[i]);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int q = 0; q < 5; ++q) {
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) {
+            const double s2 = __shfl_xor(acc[q].s, off, 64), c2 = __shfl_xor(acc[q].c, off, 64);
+            comp_merge(acc[q].s, acc[q].c, s2, c2);
+        }
+        if (lane == 0) {
+            sh[(q * 4 + wave) * 2] = acc[q].s;
+            sh[(q * 4 + wave) * 2 + 1] = acc[q].c;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0)
+        for (int q = 0; q < 5; ++q) {
+            double ss = sh[(q * 4) * 2], cc = sh[(q * 4) * 2 + 1];
+            for (int w = 1; w < 4; ++w) comp_merge(ss, cc, sh[(q * 4 + w) * 2], sh[(q * 4 + w) * 2 + 1]);
+            R[q] = ss + cc;
+        }
+}
+
+// Scalar adjoint recursion.  mode bit 1: POST for the substep just undone
+// (uses the reductions of P1 / P3 / P5; skipped while nothing is pending);
+// bit 2: PRE for the next substep to undo; bit 4: the t = 0 pass (PRE without
+// a substep, POST without the drift and kick parts).  PRE takes the forward
+// values at the substep's end point from the recorded scalar states: me is the
+// state recorded after the substep (a at its end is me->a_start, or me->a for
+// the last substep of a segment, where no next substep has begun), mh the one
+// that began it (a_half); the t = 0 pass takes them from the op.
+struct AdjScalarOp {
+    int mode;
+    int static_bg;
+    int alpha;
+    int end_of_segment;
+    double h_pre;                // w dt / 2 of the substep to undo next
+    double a_e, K_e, G_e, V_e;   // t = 0 pass: forward values at t = 0
+    double kIR, f2m, N3;
+    int n1, n3, n5, pad2;        // partial counts of P1, P3, P5
+};
+
+__global__ void __launch_bounds__(256)
+k_adj_scalars(AdjState* __restrict__ st, AdjCoef* __restrict__ co, const double* __restrict__ p1,
+              const double* __restrict__ p3, const double* __restrict__ p5, const MemberState* __restrict__ me,
+              const MemberState* __restrict__ mh, AdjScalarOp op) {
+    __shared__ double sh[5 * 4 * 2];
+    const int al = op.alpha;
+    double R[5] = {0, 0, 0, 0, 0};
+    // states loaded up front (in flight during the reduction), written back once
+    AdjState S = *st;
+    const MemberState E = *me;
+    const double ah_rec = mh->a_half;
+    __syncthreads();   // every thread has read the state before thread 0 rewrites it
+    if (!S.pending) op.mode &= ~1;
+    if (op.mode & 1) adj_block_reduce(p1, op.n1, p3, op.n3, p5, op.n5, !(op.mode & 4), sh, R);
+    if (threadIdx.x != 0) return;
+    const double R1 = R[0], R2 = R[1], R3 = R[2], R4 = R[3], R5 = R[4];
+    if (op.mode & 1) {
+        const double a = S.a_e;
+        const double cA = 1.0 / 3.0 * op.f2m * ipow(a, 1 + 2 * al);
+        // force adjoint: d/da of -a^{1+al} kIR^2 n^2 f and -a^{3+al} F(V'); g^2 via V' and add
+        S.g2bar += R3 + S.addbar_used * cA * (al + 1) * R4 / op.N3;
+        if (op.static_bg) {
+            S.abar = S.bbar = S.addbar = 0.0;
+        } else {
+            S.abar += R1 * (-(1 + al) * ipow(a, al) * op.kIR * op.kIR) + (3 + al) / a * R2;
+            if (!(op.mode & 4)) {
+                const double h = S.h_undo, ah = S.ah_undo;
+                double ahbar = S.abar;                            // a' = ah + bh h
+                double bhbar = S.bbar + h * S.abar;
+                ahbar += 2.0 * h * (al - 3) * ipow(ah, al - 4) * R5;  // f' = f + 2h ah^{al-3} pi_h
+                bhbar += h * ahbar;                               // ah = a + bh h
+                S.abar = ahbar;
+                S.bbar = bhbar;                                   // bh = b + h add
+                S.addbar = h * bhbar;
+            }
+        }
+    }
+    if (op.mode & 2) {
+        const bool fin = (op.mode & 4) != 0;
+        if (!fin) {
+            op.a_e = op.end_of_segment ? E.a : E.a_start;
+            op.K_e = E.K;
+            op.G_e = E.G;
+            op.V_e = E.V;
+        }
+        const double a_h_pre = fin ? op.a_e : ah_rec;
+        const double a = op.a_e;
+        const double cA = 1.0 / 3.0 * op.f2m * ipow(a, 1 + 2 * al);
+        const double h = fin ? 0.0 : op.h_pre;
+        if (!fin) S.addbar += h * S.bbar;                        // b' = bh + h add'
+        if (op.static_bg) S.addbar = 0.0;
+        const double addbar = S.addbar;
+        const double Q = (al - 2) * op.K_e + al * op.G_e + (al + 1) * op.V_e;
+        if (!op.static_bg)                                        // add' = A(f', pi', a')
+            S.abar += addbar * cA * ((1 + 2 * al) * Q / a - 6.0 * (al - 2) * op.K_e / a - 2.0 * al * op.G_e / a);
+        S.addbar_used = addbar;
+        S.a_e = a;
+        AdjCoef C;
+        C.hn = S.h_undo;   // the substep undone before this one (0 at the start of the sweep)
+        S.h_undo = h;
+        S.ah_undo = a_h_pre;
+        C.h = h;
+        C.ck = addbar * cA * (al - 2) / ipow(a, 6) / (op.N3 * op.N3);
+        C.cg = addbar * cA * al * op.kIR * op.kIR / (a * a) / (op.N3 * op.N3);
+        C.c1 = ipow(a, 1 + al) * op.kIR * op.kIR;
+        C.c3 = ipow(a, 3 + al);
+        C.cV = addbar * cA * (al + 1) / op.N3;
+        C.cd = fin ? 0.0 : 2.0 * h * ipow(a_h_pre, al - 3);
+        C.final_pass = fin ? 1 : 0;
+        C.pad = 0;
+        *co = C;
+    }
+    S.pending = (op.mode & 2) ? 1 : 0;
+    *st = S;
+}
+
+}  // namespace psc

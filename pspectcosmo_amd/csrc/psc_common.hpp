@@ -64,6 +64,18 @@ struct Shape {
 
 __device__ __forceinline__ int signed_index(int i, int N) { return i <= N / 2 ? i : i - N; }
 
+// a^n for a small integer exponent (powers of the scale factor) by repeated
+// squaring: a few multiplications instead of a general pow on the single
+// thread that runs each scalar update.
+__device__ __forceinline__ double ipow(double a, int n) {
+    double r = 1.0, b = n < 0 ? 1.0 / a : a;
+    for (unsigned k = n < 0 ? -n : n; k; k >>= 1) {
+        if (k & 1) r *= b;
+        b *= b;
+    }
+    return r;
+}
+
 // Padded layout of one member: element (x, y, z) at x * N (N + 1) + y * N + z.
 __host__ __device__ constexpr size_t plane_stride(int N) { return (size_t)N * (N + 1); }
 __host__ __device__ constexpr size_t member_stride(int N) { return (size_t)N * plane_stride(N); }

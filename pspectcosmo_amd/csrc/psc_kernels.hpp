@@ -27,7 +27,8 @@ template <class G, bool DRIFT>
 __global__ void __launch_bounds__((G::THREADS_ROW))
 k_z_inverse(const cpx<typename G::T>* __restrict__ src, cpx<typename G::T>* __restrict__ Pf, const cpx<typename G::T>* __restrict__ Pp,
             cpx<typename G::T>* __restrict__ W, const MemberState* __restrict__ ms,
-            const cpx<typename G::T>* __restrict__ tw) {
+            const cpx<typename G::T>* __restrict__ tw, cpx<typename G::T>* __restrict__ rec_f = nullptr,
+            cpx<typename G::T>* __restrict__ rec_ph = nullptr) {
     using T = typename G::T;
     constexpr int N = G::N;
     __shared__ cpx<T> lds[G::LayRow::size(G::L)];
@@ -42,6 +43,10 @@ k_z_inverse(const cpx<typename G::T>* __restrict__ src, cpx<typename G::T>* __re
         for (int m = 0; m < G::E; ++m) {
             cpx<T> f = Pf[base + G::TPL * m];
             const cpx<T> p = Pp[base + G::TPL * m];
+            if (rec_f) {  // adjoint recording: substep start f and pi_half
+                rec_f[base + G::TPL * m] = f;
+                rec_ph[base + G::TPL * m] = p;
+            }
             f.x += cd * p.x;
             f.y += cd * p.y;
             Pf[base + G::TPL * m] = f;
@@ -85,7 +90,7 @@ k_strided(cpx<typename G::T>* __restrict__ W, const cpx<typename G::T>* __restri
 template <class G>
 __global__ void __launch_bounds__((G::THREADS_COL))
 k_x_force(cpx<typename G::T>* __restrict__ W, const MemberState* __restrict__ ms, const cpx<typename G::T>* __restrict__ tw,
-          double* __restrict__ partV) {
+          double* __restrict__ partV, cpx<typename G::T>* __restrict__ rec_r = nullptr) {
     using T = typename G::T;
     constexpr int N = G::N;
     __shared__ cpx<T> lds[G::LayCol::size(0)];
@@ -106,6 +111,7 @@ k_x_force(cpx<typename G::T>* __restrict__ W, const MemberState* __restrict__ ms
 #pragma unroll
     for (int m = 0; m < G::E; ++m) {
         const T phi = e[m].x * inv, chi = e[m].y * inv;
+        if (rec_r) rec_r[base + (size_t)G::TPL * m * stride] = cpx<T>{phi, chi};  // adjoint recording
         const T chi2 = chi * chi, phi2 = phi * phi;
         e[m].x = m2 * phi + g2 * phi * chi2;
         e[m].y = g2 * phi2 * chi;
@@ -130,7 +136,8 @@ template <class G, bool REDUCE>
 __global__ void __launch_bounds__((G::THREADS_ROW))
 k_z_kick(const cpx<typename G::T>* __restrict__ W, const cpx<typename G::T>* __restrict__ Pf, cpx<typename G::T>* __restrict__ Pp,
          const MemberState* __restrict__ ms, const cpx<typename G::T>* __restrict__ tw,
-         double* __restrict__ partKG, double c1d, double c2d, int use_a_start, StepConsts sc) {
+         double* __restrict__ partKG, double c1d, double c2d, int use_a_start, StepConsts sc,
+         cpx<typename G::T>* __restrict__ rec_p = nullptr) {
     using T = typename G::T;
     constexpr int N = G::N;
     __shared__ cpx<T> lds[G::LayRow::size(G::L)];
@@ -143,8 +150,8 @@ k_z_kick(const cpx<typename G::T>* __restrict__ W, const cpx<typename G::T>* __r
     for (int m = 0; m < G::E; ++m) e[m] = W[base + G::TPL * m];
     line_fft<T, N, typename G::Rad, -1, typename G::LayRow>(e, t, c, lds, tw);
     const double a = use_a_start ? ms[member].a_start : ms[member].a;
-    const T lapc = (T)(pow(a, 1 + sc.alpha) * sc.kIR * sc.kIR);
-    const T vc = (T)pow(a, 3 + sc.alpha);
+    const T lapc = (T)(ipow(a, 1 + sc.alpha) * sc.kIR * sc.kIR);
+    const T vc = (T)ipow(a, 3 + sc.alpha);
     const T c1 = (T)c1d, c2 = (T)c2d;
     const int ix = (int)(line / N), iy = (int)(line % N);
     const int px = signed_index(ix, N), py = signed_index(iy, N);
@@ -161,6 +168,7 @@ k_z_kick(const cpx<typename G::T>* __restrict__ W, const cpx<typename G::T>* __r
         const cpx<T> F = {-ln2 * f.x - vc * e[m].x, -ln2 * f.y - vc * e[m].y};
         p.x += c1 * F.x;
         p.y += c1 * F.y;
+        if (rec_p) rec_p[base + G::TPL * m] = p;  // adjoint recording: synchronised pi
         if constexpr (REDUCE) {
             const double px_ = p.x, py_ = p.y, fx = f.x, fy = f.y;
             red[0] += px_ * px_ + py_ * py_;
@@ -205,10 +213,12 @@ __device__ __forceinline__ void comp_merge(double& s, double& c, double s2, doub
 
 __global__ void __launch_bounds__(256)
 k_scalars(MemberState* __restrict__ ms, const double* __restrict__ partV,
-          const double* __restrict__ partKG, ScalarOp op, StepConsts sc) {
+          const double* __restrict__ partKG, ScalarOp op, StepConsts sc, MemberState* __restrict__ rec = nullptr) {
     __shared__ double sh[3 * 2 * 4];
     const int member = blockIdx.x;
-    MemberState& s = ms[member];
+    // the state is loaded up front (in flight during the reduction) and
+    // written back once at the end
+    MemberState s = ms[member];
     if (op.finish) {
         Comp acc[3];
         for (int i = threadIdx.x; i < op.nV; i += blockDim.x) acc[0].add(partV[(size_t)member * op.nV + i]);
@@ -238,14 +248,14 @@ k_scalars(MemberState* __restrict__ ms, const double* __restrict__ partV,
                 tot[q] = ss + cc;
             }
             const double a = s.a;
-            const double Kn = 0.5 * tot[1] / pow(a, 6) / (sc.N3 * sc.N3);
+            const double Kn = 0.5 * tot[1] / ipow(a, 6) / (sc.N3 * sc.N3);
             const double Gn = sc.kIR * sc.kIR * tot[2] / sc.N3 / 2.0 / (a * a) / sc.N3;
             const double Vn = tot[0] / sc.N3;
             s.K = Kn;
             s.G = Gn;
             s.V = Vn;
             const int al = sc.alpha;
-            double add = 1.0 / 3.0 * sc.f2m * pow(a, 1 + 2 * al) * ((al - 2) * Kn + al * Gn + (al + 1) * Vn);
+            double add = 1.0 / 3.0 * sc.f2m * ipow(a, 1 + 2 * al) * ((al - 2) * Kn + al * Gn + (al + 1) * Vn);
             if (sc.expansion_static) add = 0.0;
             s.add = add;
             s.b = s.b_half + op.w_finish * sc.dt / 2.0 * add;
@@ -260,8 +270,12 @@ k_scalars(MemberState* __restrict__ ms, const double* __restrict__ partV,
         s.b_half = s.b + w * dt / 2.0 * s.add;
         s.a_start = s.a;
         s.a_half = sc.expansion_static ? 1.0 : s.a + s.b_half * w * dt / 2.0;
-        s.c_drift = w * dt * pow(s.a_half, sc.alpha - 3);
+        s.c_drift = w * dt * ipow(s.a_half, sc.alpha - 3);
         s.a = sc.expansion_static ? 1.0 : s.a_half + s.b_half * w * dt / 2.0;
+    }
+    if (threadIdx.x == 0) {
+        ms[member] = s;
+        if (rec) rec[member] = s;   // adjoint recording
     }
 }
 
@@ -374,7 +388,7 @@ k_spectra(const cpx<T>* __restrict__ Pf, const cpx<T>* __restrict__ Pp,
     const cpx<T>* f = Pf + (size_t)member * member_stride(N);
     const cpx<T>* p = Pp + (size_t)member * member_stride(N);
     const double a = ms[member].a, ad = ms[member].ad;
-    const double aop = pow(a, alpha - 3), hub = ad / a;
+    const double aop = ipow(a, alpha - 3), hub = ad / a;
     const size_t nk = (size_t)N * N * NH;
     for (size_t h = (size_t)blockIdx.x * blockDim.x + threadIdx.x; h < nk; h += (size_t)blocks_per_member * blockDim.x) {
         const int i = (int)(h / ((size_t)N * NH)), j = (int)((h / NH) % N), k = (int)(h % NH);
@@ -585,9 +599,9 @@ k_z_kick_gw(const cpx<typename G::T>* __restrict__ W, const cpx<typename G::T>* 
     const size_t line = (size_t)blockIdx.x * G::L + c;
     const size_t base = (size_t)member * G::MS + (line / N) * G::PS + (line % N) * N + t;
     const double a = use_a_start ? ms[member].a_start : ms[member].a;
-    const T norm1 = (T)pow(a, 1 + sc.alpha);
-    const T lapc = (T)(pow(a, 1 + sc.alpha) * sc.kIR * sc.kIR);
-    const T vc = (T)pow(a, 3 + sc.alpha);
+    const T norm1 = (T)ipow(a, 1 + sc.alpha);
+    const T lapc = (T)(ipow(a, 1 + sc.alpha) * sc.kIR * sc.kIR);
+    const T vc = (T)ipow(a, 3 + sc.alpha);
     const T k2c = (T)(sc.kIR * sc.kIR);
     const T c1 = (T)c1d, c2 = (T)c2d;
     const int ix = (int)(line / N), iy = (int)(line % N);

@@ -17,6 +17,7 @@
 
 #include "../../include/psc.h"
 #include "psc_kernels.hpp"
+#include "psc_adjoint.hpp"
 
 using namespace psc;
 
@@ -59,6 +60,34 @@ struct Pending {
 
 struct EngineBase;
 
+// Device workspace of the adjoint, kept between calls with the same checkpoint
+// spacing: checkpoints (f, pi, scalar state) at every segment start, one
+// segment's recording (f, pi and the real-space fields at each substep
+// boundary, pi_half per substep, the scalar state after each scalar update),
+// the adjoint arrays and, when steps are graph-captured, the captured segment.
+struct AdjWork {
+    int K = 0, S = 0, nseg = 0;
+    std::vector<void*> ck;            // f, pi per segment
+    MemberState* ck_ms = nullptr;
+    std::vector<void*> f, p, r, ph;
+    MemberState* rms = nullptr;
+    void* fbar = nullptr;
+    void* pbar = nullptr;
+    psc::AdjState* st = nullptr;
+    psc::AdjCoef* co = nullptr;
+    double* p1 = nullptr;
+    double* p3 = nullptr;
+    double* p5 = nullptr;
+    hipGraphExec_t g_rec = nullptr, g_bwd = nullptr;   // full segment: recording replay, sweep
+    double bytes = 0.0;
+    std::vector<void*> owned;
+    ~AdjWork() {
+        if (g_rec) (void)hipGraphExecDestroy(g_rec);
+        if (g_bwd) (void)hipGraphExecDestroy(g_bwd);
+        for (void* q : owned) (void)hipFree(q);
+    }
+};
+
 }  // namespace
 
 struct psc_ctx {
@@ -91,6 +120,7 @@ struct psc_ctx {
     std::vector<hipEvent_t> ev_free;
     std::vector<Pending> pending;
     hipGraphExec_t g_mid = nullptr, g_last = nullptr;
+    std::unique_ptr<AdjWork> adj;
     double dev_bytes = 0.0;
     int shape = 0;   // index into kShapes
     std::string report;
@@ -175,6 +205,8 @@ struct EngineBase {
     virtual int spectra(psc_ctx* c, int member, int fld) = 0;
     // GW spectrum partial sums into c->sums[0 .. 3*maxnumbins + nlog).
     virtual int gw_spectra(psc_ctx* c, int member, int projection, int nlog) = 0;
+    // Discrete-adjoint gradient (see psc_adjoint).
+    virtual int adjoint(psc_ctx* c, const psc_adjoint_spec* sp, double* out, double* dchi) = 0;
 };
 
 template <class G>
@@ -231,7 +263,7 @@ struct Engine final : EngineBase {
     // Reduce the substep partials to the scalar update.  Passes with more than
     // kPreChunks blocks per member are first cut to kPreChunks compensated
     // chunk sums by many blocks, so the single-block update stays short.
-    int launch_scalars(psc_ctx* c, ScalarOp op) {
+    int launch_scalars(psc_ctx* c, ScalarOp op, MemberState* rec = nullptr) {
         const StepConsts sc = consts_of(c);
         const double* pv = c->partV;
         const double* pkg = c->partKG;
@@ -249,7 +281,7 @@ struct Engine final : EngineBase {
         }
         const double bytes = (double)c->B * ((double)(op.nV + 2 * op.nKG) * 8.0 + 2 * sizeof(MemberState));
         return timed(c, K_SCALARS, bytes, [&] {
-            hipLaunchKernelGGL(k_scalars, dim3(c->B), dim3(256), 0, c->stream, c->ms, pv, pkg, op, sc);
+            hipLaunchKernelGGL(k_scalars, dim3(c->B), dim3(256), 0, c->stream, c->ms, pv, pkg, op, sc, rec);
         });
     }
 
@@ -411,7 +443,184 @@ struct Engine final : EngineBase {
     }
 
     int gw_spectra(psc_ctx* c, int member, int projection, int nlog) override;
+    int adjoint(psc_ctx* c, const psc_adjoint_spec* sp, double* out, double* dchi) override;
+
+    // ---- adjoint helpers --------------------------------------------------
+    int adj_workspace(psc_ctx* c, int K, int nseg, AdjWork** out) {
+        const int S = (int)c->w.size();
+        AdjWork* cur = c->adj.get();
+        if (cur && cur->K == K && cur->S == S && cur->nseg >= nseg) {
+            *out = cur;
+            return PSC_OK;
+        }
+        c->adj.reset();
+        auto w = std::make_unique<AdjWork>();
+        const size_t arr = G::MS * sizeof(cpx<T>);
+        const int SK = K * S;
+        bool ok = true;
+        auto dalloc = [&](size_t bytes) -> void* {
+            void* q = nullptr;
+            if (!ok || hipMalloc(&q, bytes) != hipSuccess) {
+                ok = false;
+                return nullptr;
+            }
+            w->owned.push_back(q);
+            w->bytes += (double)bytes;
+            return q;
+        };
+        w->K = K;
+        w->S = S;
+        w->nseg = nseg;
+        w->ck.resize(2 * (size_t)nseg);
+        for (auto& q : w->ck) q = dalloc(arr);
+        w->ck_ms = static_cast<MemberState*>(dalloc(sizeof(MemberState) * nseg));
+        w->f.resize(SK + 1);
+        w->p.resize(SK + 1);
+        w->r.resize(SK + 1);
+        w->ph.resize(SK);
+        for (int j = 0; j <= SK; ++j) {
+            w->f[j] = dalloc(arr);
+            w->p[j] = dalloc(arr);
+            w->r[j] = dalloc(arr);
+            if (j < SK) w->ph[j] = dalloc(arr);
+        }
+        w->rms = static_cast<MemberState*>(dalloc(sizeof(MemberState) * (SK + 1)));
+        w->fbar = dalloc(arr);
+        w->pbar = dalloc(arr);
+        w->st = static_cast<AdjState*>(dalloc(sizeof(AdjState)));
+        w->co = static_cast<AdjCoef*>(dalloc(sizeof(AdjCoef)));
+        w->p1 = static_cast<double*>(dalloc(sizeof(double) * G::BLOCKS_ROW));
+        w->p3 = static_cast<double*>(dalloc(sizeof(double) * 3 * G::BLOCKS_COL));
+        w->p5 = static_cast<double*>(dalloc(sizeof(double) * G::BLOCKS_ROW));
+        if (!ok) return c->fail(PSC_ERR_NOMEM, "adjoint: device allocation failed");
+        c->adj = std::move(w);
+        *out = c->adj.get();
+        return PSC_OK;
+    }
+
+    // Replays `steps` steps from the current state, recording f, pi and the
+    // real-space fields at every substep boundary, pi_half per substep and the
+    // scalar state after each scalar update.
+    int record_segment(psc_ctx* c, int steps, AdjWork& w) {
+        const int S = (int)c->w.size();
+        const double dt = c->cfg.dt;
+        const StepConsts sc = consts_of(c);
+        HIPCHK(c, hipMemcpyAsync(w.p[0], c->Pp, G::MS * sizeof(cpx<T>), hipMemcpyDeviceToDevice, c->stream));
+        ScalarOp op{};
+        op.begin = 1;
+        op.w_begin = c->w[0];
+        op.nV = G::BLOCKS_COL;
+        op.nKG = G::BLOCKS_ROW;
+        int rc = launch_scalars(c, op, w.rms);
+        if (rc) return rc;
+        dim3 rgrid(G::BLOCKS_ROW, 1), rblock(G::THREADS_ROW), cgrid(N / G::TC, N, 1), cblock(G::THREADS_COL);
+        // prologue force at (f_0, a_0): first kick, real-space fields of f_0
+        hipLaunchKernelGGL((k_z_inverse<G, false>), rgrid, rblock, 0, c->stream, P(c->Pf), P(c->Pf), P(c->Pp),
+                           P(c->W), c->ms, P(c->tw), nullptr, nullptr);
+        if ((rc = launch_y<+1>(c, P(c->W), 1))) return rc;
+        hipLaunchKernelGGL((k_x_force<G>), cgrid, cblock, 0, c->stream, P(c->W), c->ms, P(c->tw), c->partV,
+                           P(w.r[0]));
+        if ((rc = launch_y<-1>(c, P(c->W), 1))) return rc;
+        hipLaunchKernelGGL((k_z_kick<G, false>), rgrid, rblock, 0, c->stream, P(c->W), P(c->Pf), P(c->Pp), c->ms,
+                           P(c->tw), c->partKG, c->w[0] * dt / 2.0, 0.0, 1, sc, nullptr);
+        for (int j = 0; j < steps * S; ++j) {
+            const int p = j % S;
+            const bool last = j == steps * S - 1;
+            const double wn = c->w[(p + 1) % S];
+            hipLaunchKernelGGL((k_z_inverse<G, true>), rgrid, rblock, 0, c->stream, P(c->Pf), P(c->Pf), P(c->Pp),
+                               P(c->W), c->ms, P(c->tw), P(w.f[j]), P(w.ph[j]));
+            if ((rc = launch_y<+1>(c, P(c->W), 1))) return rc;
+            hipLaunchKernelGGL((k_x_force<G>), cgrid, cblock, 0, c->stream, P(c->W), c->ms, P(c->tw), c->partV,
+                               P(w.r[j + 1]));
+            if ((rc = launch_y<-1>(c, P(c->W), 1))) return rc;
+            hipLaunchKernelGGL((k_z_kick<G, true>), rgrid, rblock, 0, c->stream, P(c->W), P(c->Pf), P(c->Pp),
+                               c->ms, P(c->tw), c->partKG, c->w[p] * dt / 2.0, last ? 0.0 : wn * dt / 2.0, 0, sc,
+                               P(w.p[j + 1]));
+            ScalarOp o{};
+            o.finish = 1;
+            o.last_of_step = p == S - 1;
+            o.begin = last ? 0 : 1;
+            o.w_finish = c->w[p];
+            o.w_begin = wn;
+            o.nV = G::BLOCKS_COL;
+            o.nKG = G::BLOCKS_ROW;
+            if ((rc = launch_scalars(c, o, w.rms + j + 1))) return rc;
+        }
+        HIPCHK(c, hipMemcpyAsync(w.f[steps * S], c->Pf, G::MS * sizeof(cpx<T>), hipMemcpyDeviceToDevice, c->stream));
+        HIPCHK(c, hipGetLastError());
+        return PSC_OK;
+    }
+
+    // One backward substep (or the t = 0 pass): P1, y inverse, P3, y forward, P5.
+    int backward_passes(psc_ctx* c, AdjWork& w, int jf, int jph) {
+        dim3 rgrid(G::BLOCKS_ROW, 1), rblock(G::THREADS_ROW), cgrid(N / G::TC, N, 1), cblock(G::THREADS_COL);
+        hipLaunchKernelGGL((k_adj_p1<G>), rgrid, rblock, 0, c->stream, P(w.f[jf]), P(w.p[jf]), P(w.fbar),
+                           P(w.pbar), P(c->W), w.co, P(c->tw), w.p1);
+        int rc = launch_y<+1>(c, P(c->W), 1);
+        if (rc) return rc;
+        hipLaunchKernelGGL((k_adj_p3<G>), cgrid, cblock, 0, c->stream, P(c->W), P(w.r[jf]), c->ms, w.co, P(c->tw),
+                           w.p3);
+        if ((rc = launch_y<-1>(c, P(c->W), 1))) return rc;
+        hipLaunchKernelGGL((k_adj_p5<G>), rgrid, rblock, 0, c->stream, P(c->W), P(w.fbar), P(w.pbar),
+                           P(w.ph[jph]), w.co, P(c->tw), w.p5);
+        HIPCHK(c, hipGetLastError());
+        return PSC_OK;
+    }
+
+    AdjScalarOp adj_op(const psc_ctx* c) {
+        const StepConsts sc = consts_of(c);
+        AdjScalarOp op{};
+        op.static_bg = sc.expansion_static;
+        op.alpha = sc.alpha;
+        op.kIR = sc.kIR;
+        op.f2m = sc.f2m;
+        op.N3 = sc.N3;
+        op.n1 = G::BLOCKS_ROW;
+        op.n3 = G::BLOCKS_COL;
+        op.n5 = G::BLOCKS_ROW;
+        return op;
+    }
+
+    // Sweeps the recorded segment of `steps` steps backwards.
+    int backward_segment(psc_ctx* c, int steps, AdjWork& w) {
+        const int S = (int)c->w.size();
+        const int nsub = steps * S;
+        AdjScalarOp op = adj_op(c);
+        for (int j = nsub - 1; j >= 0; --j) {
+            op.mode = 1 | 2;
+            op.h_pre = c->w[j % S] * c->cfg.dt / 2.0;
+            op.end_of_segment = j == nsub - 1;
+            hipLaunchKernelGGL(k_adj_scalars, dim3(1), dim3(256), 0, c->stream, w.st, w.co, w.p1, w.p3, w.p5,
+                               w.rms + j + 1, w.rms + j, op);
+            int rc = backward_passes(c, w, j + 1, j);
+            if (rc) return rc;
+        }
+        return PSC_OK;
+    }
+
+    // Records (bwd = false) or sweeps (bwd = true) a segment of `steps` steps;
+    // full segments go through a graph captured on first use when steps are
+    // graph-captured.
+    int run_segment(psc_ctx* c, int steps, AdjWork& w, bool bwd) {
+        if (steps != w.K || !c->cfg.use_graph)
+            return bwd ? backward_segment(c, steps, w) : record_segment(c, steps, w);
+        hipGraphExec_t& ge = bwd ? w.g_bwd : w.g_rec;
+        if (!ge) {
+            hipGraph_t g;
+            HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            int rc = bwd ? backward_segment(c, steps, w) : record_segment(c, steps, w);
+            hipError_t e = hipStreamEndCapture(c->stream, &g);
+            if (rc) return rc;
+            if (e != hipSuccess)
+                return c->fail(PSC_ERR_DEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+            HIPCHK(c, hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+            HIPCHK(c, hipGraphDestroy(g));
+        }
+        HIPCHK(c, hipGraphLaunch(ge, c->stream));
+        return PSC_OK;
+    }
 };
+
 
 template <typename T, int N>
 int gw_spectra_impl(psc_ctx* c, int member, int projection, int nlog) {
@@ -638,6 +847,129 @@ double kernel_a_value(const psc_ctx* c, const Diag& d, double a) {
     return 1.0 / 3.0 * f2m * std::pow(a, 1 + 2 * al) * ((al - 2) * K + al * G + (al + 1) * V);
 }
 
+// <chi^2> objective: fbar = (P(k) - conj P(-k)) / N^6 on the padded layout.
+template <class G>
+__global__ void k_adj_init_chi2(const cpx<typename G::T>* __restrict__ Pf, cpx<typename G::T>* __restrict__ fbar) {
+    constexpr int N = G::N;
+    const size_t li = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (li >= G::N3) return;
+    const int i = (int)(li / ((size_t)N * N)), j = (int)((li / N) % N), k = (int)(li % N);
+    const size_t idx = (size_t)i * G::PS + (size_t)j * N + k;
+    const size_t midx = (size_t)((N - i) % N) * G::PS + (size_t)((N - j) % N) * N + (N - k) % N;
+    const double inv = 1.0 / ((double)G::N3 * (double)G::N3);
+    const cpx<typename G::T> a = Pf[idx], b = Pf[midx];
+    fbar[idx] = cpx<typename G::T>{(typename G::T)(((double)a.x - b.x) * inv), (typename G::T)(((double)a.y + b.y) * inv)};
+}
+
+template <class G>
+int Engine<G>::adjoint(psc_ctx* c, const psc_adjoint_spec* sp, double* out, double* dchi) {
+    const int M = sp->nsteps;
+    const int K = std::min(sp->checkpoint_every, M);
+    const int nseg = (M + K - 1) / K;
+    const size_t arr = G::MS * sizeof(cpx<T>);
+    AdjWork* w = nullptr;
+    int rc = adj_workspace(c, K, nseg, &w);
+    if (rc) return rc;
+    MemberState s0;
+    if ((rc = read_state(c, 0, &s0))) return rc;
+    Diag d0;   // K, G, V at t = 0 for the adjoint of the initial d2a/dt2
+    if ((rc = diagnose(c, 0, s0, &d0))) return rc;
+    hipEvent_t ev[3];
+    for (auto& e : ev) HIPCHK(c, hipEventCreate(&e));
+    struct Events {
+        hipEvent_t* e;
+        ~Events() {
+            for (int i = 0; i < 3; ++i) (void)hipEventDestroy(e[i]);
+        }
+    } guard{ev};
+    // forward run with a checkpoint at every segment start; the last segment
+    // is recorded on the way
+    HIPCHK(c, hipEventRecord(ev[0], c->stream));
+    for (int sg = 0; sg < nseg; ++sg) {
+        HIPCHK(c, hipMemcpyAsync(w->ck[2 * sg], c->Pf, arr, hipMemcpyDeviceToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(w->ck[2 * sg + 1], c->Pp, arr, hipMemcpyDeviceToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(w->ck_ms + sg, c->ms, sizeof(MemberState), hipMemcpyDeviceToDevice, c->stream));
+        const int steps = std::min(K, M - sg * K);
+        if ((rc = sg == nseg - 1 ? run_segment(c, steps, *w, false) : step(c, steps))) return rc;
+    }
+    HIPCHK(c, hipEventRecord(ev[1], c->stream));
+    // the objective and its adjoint at the end
+    if ((rc = real_space(c, 0, c->Pf))) return rc;
+    double rs[6];
+    HIPCHK(c, hipMemcpyAsync(rs, c->sums, sizeof(rs), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    const double J = rs[3] / (double)G::N3;
+    HIPCHK(c, hipMemsetAsync(w->pbar, 0, arr, c->stream));
+    HIPCHK(c, hipMemsetAsync(w->st, 0, sizeof(AdjState), c->stream));
+    hipLaunchKernelGGL((k_adj_init_chi2<G>), dim3((unsigned)((G::N3 + 255) / 256)), dim3(256), 0, c->stream,
+                       P(c->Pf), P(w->fbar));
+    // segments from the last: restore the checkpoint, replay with recording
+    // (the last segment is still recorded), sweep back
+    for (int sg = nseg - 1; sg >= 0; --sg) {
+        const int steps = std::min(K, M - sg * K);
+        if (sg < nseg - 1) {
+            HIPCHK(c, hipMemcpyAsync(c->Pf, w->ck[2 * sg], arr, hipMemcpyDeviceToDevice, c->stream));
+            HIPCHK(c, hipMemcpyAsync(c->Pp, w->ck[2 * sg + 1], arr, hipMemcpyDeviceToDevice, c->stream));
+            HIPCHK(c, hipMemcpyAsync(c->ms, w->ck_ms + sg, sizeof(MemberState), hipMemcpyDeviceToDevice, c->stream));
+            if ((rc = run_segment(c, steps, *w, false))) return rc;
+        }
+        if ((rc = run_segment(c, steps, *w, true))) return rc;
+    }
+    // close substep 0, then the t = 0 pass: the pending force at (f_0, a_0)
+    // and the initial d2a/dt2
+    AdjScalarOp op = adj_op(c);
+    op.mode = 1;
+    hipLaunchKernelGGL(k_adj_scalars, dim3(1), dim3(256), 0, c->stream, w->st, w->co, w->p1, w->p3, w->p5, w->rms,
+                       w->rms, op);
+    op.mode = 2 | 4;
+    op.a_e = s0.a;
+    op.K_e = d0.K[0] + d0.K[1];
+    op.G_e = d0.G[0] + d0.G[1];
+    op.V_e = d0.Vt[0] + d0.Vt[1];
+    hipLaunchKernelGGL(k_adj_scalars, dim3(1), dim3(256), 0, c->stream, w->st, w->co, w->p1, w->p3, w->p5, w->rms,
+                       w->rms, op);
+    if ((rc = backward_passes(c, *w, 0, 0))) return rc;
+    op.mode = 1 | 4;
+    hipLaunchKernelGGL(k_adj_scalars, dim3(1), dim3(256), 0, c->stream, w->st, w->co, w->p1, w->p3, w->p5, w->rms,
+                       w->rms, op);
+    HIPCHK(c, hipEventRecord(ev[2], c->stream));
+    HIPCHK(c, hipEventSynchronize(ev[2]));
+    float fwd_ms = 0, bwd_ms = 0;
+    HIPCHK(c, hipEventElapsedTime(&fwd_ms, ev[0], ev[1]));
+    HIPCHK(c, hipEventElapsedTime(&bwd_ms, ev[1], ev[2]));
+    AdjState hst;
+    HIPCHK(c, hipMemcpy(&hst, w->st, sizeof(AdjState), hipMemcpyDeviceToHost));
+    cpx<T> f00;
+    HIPCHK(c, hipMemcpy(&f00, w->fbar, sizeof(f00), hipMemcpyDeviceToHost));
+    // phi0 enters through the DC mode phi0 N^3 and da/dt(0) = a0 H(phi0),
+    // dH/dphi0 = f2m V'(phi0) / (6 H)  (initialize.cpp:649-668)
+    const StepConsts sc = consts_of(c);
+    const double V0 = 0.5 * s0.m2 * sp->phi0 * sp->phi0 + 0.5 * s0.g2 * sp->phi0 * sp->phi0 * sp->chi0 * sp->chi0;
+    const double H = std::sqrt((sp->dphi0 * sp->dphi0 / 2.0 + V0) * sc.f2m / 3.0);
+    const double vphi = s0.m2 * sp->phi0 + s0.g2 * sp->phi0 * sp->chi0 * sp->chi0;
+    const double phi0bar =
+        (double)G::N3 * (double)f00.x + (sc.expansion_static ? 0.0 : hst.bbar * s0.a * sc.f2m * vphi / (6.0 * H));
+    // restore the loaded state
+    HIPCHK(c, hipMemcpyAsync(c->Pf, w->ck[0], arr, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->Pp, w->ck[1], arr, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->ms, w->ck_ms, sizeof(MemberState), hipMemcpyDeviceToDevice, c->stream));
+    if (dchi) {
+        // dJ/dchi(x) = Im(F^H fbar)(x); real_space() leaves F^H src / N^3 in W
+        if ((rc = real_space(c, 0, w->fbar))) return rc;
+        std::vector<double> full;
+        if ((rc = download(c, c->W, 0, full))) return rc;
+        for (size_t i = 0; i < G::N3; ++i) dchi[i] = full[2 * i + 1] * (double)G::N3;
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    out[0] = J;
+    out[1] = hst.g2bar;
+    out[2] = phi0bar;
+    out[3] = fwd_ms;
+    out[4] = bwd_ms;
+    out[5] = w->bytes;
+    return PSC_OK;
+}
+
 }  // namespace
 
 namespace {
@@ -683,9 +1015,10 @@ void tune_store(const std::string& key, const std::string& name, double ms) {
 }
 
 void drop_graphs(psc_ctx* c) {
-    if (c->g_mid) hipGraphExecDestroy(c->g_mid);
-    if (c->g_last) hipGraphExecDestroy(c->g_last);
+    if (c->g_mid) (void)hipGraphExecDestroy(c->g_mid);
+    if (c->g_last) (void)hipGraphExecDestroy(c->g_last);
     c->g_mid = c->g_last = nullptr;
+    c->adj.reset();
 }
 
 }  // namespace
@@ -1275,6 +1608,21 @@ int psc_set_shape(psc_ctx* c, const char* name) {
             return PSC_OK;
         }
     return c->fail(PSC_ERR_ARG, std::string("no launch shape named ") + name);
+}
+
+int psc_adjoint(psc_ctx* c, const psc_adjoint_spec* sp, double* out, double* dchi) {
+    if (!c) return PSC_ERR_ARG;
+    if (!sp || !out) return c->fail(PSC_ERR_ARG, "null adjoint arguments");
+    if (sp->objective != PSC_OBJ_CHI2) return c->fail(PSC_ERR_UNSUPPORTED, "unknown objective");
+    if (c->cfg.nflds != 2 || c->B != 1 || c->cfg.with_gw || sp->member != 0)
+        return c->fail(PSC_ERR_UNSUPPORTED, "the adjoint needs two fields, one member and no GW");
+    if (sp->nsteps < 1 || sp->checkpoint_every < 1) return c->fail(PSC_ERR_ARG, "bad step counts");
+    if (!c->loaded[0]) return c->fail(PSC_ERR_STATE, "no initial data");
+    const int saved = c->timing;
+    c->timing = 0;
+    int rc = c->eng->adjoint(c, sp, out, dchi);
+    c->timing = saved;
+    return rc;
 }
 
 int psc_set_kernel_timing(psc_ctx* c, int on) {

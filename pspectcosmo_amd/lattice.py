@@ -9,8 +9,8 @@ import os
 
 import numpy as np
 
-from .abi import (PscConfig, PscModel, PscIcParams, PSC_ENERGY_LEN, PSC_SPECTRA_COLS,
-                  PSC_GW_SPECTRA_COLS, num_bins)
+from .abi import (PscConfig, PscModel, PscIcParams, PscAdjointSpec, PSC_ENERGY_LEN, PSC_SPECTRA_COLS,
+                  PSC_GW_SPECTRA_COLS, PSC_OBJ_CHI2, num_bins)
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libpsc.so")
@@ -67,6 +67,7 @@ def lib():
         L.psc_shape_name.argtypes = [vp]
         L.psc_list_shapes.argtypes = [vp, C.c_char_p, C.c_int]
         L.psc_set_shape.argtypes = [vp, C.c_char_p]
+        L.psc_adjoint.argtypes = [vp, P(PscAdjointSpec), _D, vp]
         _lib = L
     return _lib
 
@@ -79,7 +80,7 @@ def exported_symbols():
             "psc_get_fields", "psc_energies", "psc_spectra", "psc_load_gw_k", "psc_get_gw_k",
             "psc_gw_spectrum", "psc_gw_omega_log", "psc_num_bins", "psc_set_kernel_timing", "psc_get_kernel_times",
             "psc_reset_kernel_times", "psc_device_bytes", "psc_autotune", "psc_shape_name",
-            "psc_list_shapes", "psc_set_shape"]
+            "psc_list_shapes", "psc_set_shape", "psc_adjoint"]
 
 
 def generate_ic(cfg, model, ic):
@@ -247,6 +248,21 @@ class Lattice:
 
     def set_shape(self, name):
         self._check(lib().psc_set_shape(self._h, name.encode()))
+
+    def adjoint(self, nsteps, checkpoint_every, phi0, dphi0, chi0=0.0, want_field=True):
+        """Gradient of the final <chi^2> (psc_adjoint): returns a dict with J,
+        dJ/dg2, dJ/dphi0, dJ/dchi(x) (N^3 array or None), forward and backward
+        milliseconds and the extra device bytes of the sweep."""
+        spec = PscAdjointSpec()
+        spec.objective, spec.nsteps, spec.checkpoint_every, spec.member = PSC_OBJ_CHI2, int(nsteps), \
+            int(checkpoint_every), 0
+        spec.phi0, spec.dphi0, spec.chi0 = phi0, dphi0, chi0
+        out = np.zeros(6)
+        dchi = np.zeros((self.N,) * 3) if want_field else None
+        ptr = dchi.ctypes.data_as(C.c_void_p) if want_field else None
+        self._check(lib().psc_adjoint(self._h, C.byref(spec), out, ptr))
+        return {"J": out[0], "dg2": out[1], "dphi0": out[2], "dchi": dchi, "forward_ms": out[3],
+                "backward_ms": out[4], "extra_bytes": out[5]}
 
     def device_bytes(self):
         b = np.zeros(1)

@@ -1,0 +1,189 @@
+"""GPU discrete adjoint (psc_adjoint) against the oracle's FP64 adjoint and
+against central finite differences of the GPU forward step.
+
+The oracle adjoint is itself pinned by central differences of the oracle's
+half-spectrum forward restatement (test_oracle_adjoint.py)."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import oracle as O
+from pspectcosmo_amd.abi import (PSC_VV2, PSC_Y4, PSC_FP32, PSC_FP64, PSC_EXPANSION_SELF,
+                                 PSC_EXPANSION_STATIC, make_model)
+from pspectcosmo_amd.lattice import Lattice
+from tests._util import parity_setup, rel_err
+
+pytestmark = pytest.mark.gpu
+
+PHI0, DPHI0 = 1.0, -0.7
+
+
+def setup(integrator, expansion, N=16, precision=PSC_FP64, g2=2.0e3, kIR=4.0, dt=0.01, omega_star=2e-2):
+    """Small lattices with fluctuations large enough for the coupling and the
+    Friedmann feedback to matter while a(t) stays regular (at N = 32,
+    omega_star = 2e-2 makes the background collapse within 13 steps, so the
+    32^3 case uses 5e-3)."""
+    cfg, model, ic = parity_setup(N=N, nflds=2, integrator=integrator, expansion=expansion,
+                                  precision=precision, kIR=kIR, dt=dt, g2=g2, omega_star=omega_star,
+                                  f0=(PHI0, 0.0), fd0=(DPHI0, 0.0), yoshida_exact=1)
+    fk, pk, sc = O.generate_ic(cfg, model, ic)
+    return cfg, model, fk, pk, sc
+
+
+def lattice(cfg, model, fk, pk, sc):
+    lat = Lattice(cfg)
+    lat.set_model(0, model)
+    lat.load_ic_k(0, fk, pk, sc[0], sc[1])
+    return lat
+
+
+@pytest.mark.parametrize("integrator,expansion,steps,every", [
+    (PSC_VV2, PSC_EXPANSION_SELF, 12, 5),
+    (PSC_Y4, PSC_EXPANSION_SELF, 7, 3),
+    (PSC_VV2, PSC_EXPANSION_STATIC, 9, 9),
+])
+def test_fp64_adjoint_matches_oracle(integrator, expansion, steps, every):
+    cfg, model, fk, pk, sc = setup(integrator, expansion)
+    print("case", integrator, expansion, steps, every, flush=True)
+    lat = lattice(cfg, model, fk, pk, sc)
+    k0 = lat.get_k()
+    g = lat.adjoint(steps, every, PHI0, DPHI0)
+    J, dg2, dphi0, dchi = O.adjoint_chi2(cfg, model, fk, pk, 1.0, PHI0, DPHI0, 0.0, steps)
+    print("J", g["J"], J, "g2", g["dg2"], dg2, "phi0", g["dphi0"], dphi0)
+    assert abs(g["J"] - J) <= 1e-12 * abs(J)
+    assert abs(g["dg2"] - dg2) <= 1e-9 * abs(dg2)
+    assert abs(g["dphi0"] - dphi0) <= 1e-9 * abs(dphi0)
+    assert rel_err(g["dchi"], dchi) <= 1e-9
+    # the loaded state is left unchanged
+    k1 = lat.get_k()
+    assert rel_err(k1[0], k0[0]) == 0.0 and rel_err(k1[1], k0[1]) == 0.0
+    lat.close()
+
+
+def test_checkpoint_spacing_does_not_change_the_gradient():
+    cfg, model, fk, pk, sc = setup(PSC_VV2, PSC_EXPANSION_SELF)
+    lat = lattice(cfg, model, fk, pk, sc)
+    ref = lat.adjoint(10, 10, PHI0, DPHI0)
+    for every in (1, 3, 4):
+        g = lat.adjoint(10, every, PHI0, DPHI0)
+        assert abs(g["dg2"] - ref["dg2"]) <= 1e-12 * abs(ref["dg2"])
+        assert abs(g["dphi0"] - ref["dphi0"]) <= 1e-12 * abs(ref["dphi0"])
+        assert rel_err(g["dchi"], ref["dchi"]) <= 1e-12
+    lat.close()
+
+
+def test_fp32_adjoint_tracks_the_fp64_oracle():
+    cfg, model, fk, pk, sc = setup(PSC_VV2, PSC_EXPANSION_SELF, N=32, precision=PSC_FP32, omega_star=5e-3)
+    lat = lattice(cfg, model, fk, pk, sc)
+    g = lat.adjoint(20, 5, PHI0, DPHI0)
+    J, dg2, dphi0, dchi = O.adjoint_chi2(cfg, model, fk, pk, 1.0, PHI0, DPHI0, 0.0, 20)
+    print("fp32 g2", g["dg2"], dg2, "phi0", g["dphi0"], dphi0, "chi", rel_err(g["dchi"], dchi))
+    # FP32 fields and FFTs; FP64 scalar recursion and reductions
+    assert abs(g["J"] - J) <= 1e-5 * abs(J)
+    assert abs(g["dg2"] - dg2) <= 1e-4 * abs(dg2)
+    assert abs(g["dphi0"] - dphi0) <= 1e-4 * abs(dphi0)
+    assert rel_err(g["dchi"], dchi) <= 1e-4
+    lat.close()
+
+
+def test_unsupported_configurations_are_rejected():
+    cfg, model, fk, pk, sc = setup(PSC_VV2, PSC_EXPANSION_SELF)
+    lat = lattice(cfg, model, fk, pk, sc)
+    with pytest.raises(RuntimeError):
+        lat.adjoint(0, 1, PHI0, DPHI0)
+    lat.close()
+
+
+class Chi2(torch.autograd.Function):
+    """<chi^2> after `steps` steps as a function of (g2, phi0, delta chi(x)),
+    evaluated and differentiated by the library."""
+
+    @staticmethod
+    def forward(ctx, g2, phi0, dchi0, cfg, base, steps, every):
+        fk, pk, sc = base
+        fk = fk.copy()
+        N = cfg.N
+        fk[0, 0, 0, 0] = float(phi0) * N ** 3
+        fk[1] += np.fft.rfftn(dchi0.detach().cpu().numpy())
+        V0 = 0.5 * float(phi0) ** 2
+        H = np.sqrt((DPHI0 ** 2 / 2 + V0) / 3.0)
+        lat = Lattice(cfg)
+        lat.set_model(0, make_model(1.0, float(g2)))
+        lat.load_ic_k(0, fk, pk, 1.0, H)
+        g = lat.adjoint(steps, every, float(phi0), DPHI0)
+        lat.close()
+        ctx.save_for_backward(torch.tensor(g["dg2"], dtype=torch.float64),
+                              torch.tensor(g["dphi0"], dtype=torch.float64),
+                              torch.from_numpy(g["dchi"]))
+        return torch.tensor(g["J"], dtype=torch.float64)
+
+    @staticmethod
+    def backward(ctx, gout):
+        dg2, dphi0, dchi = ctx.saved_tensors
+        return gout * dg2, gout * dphi0, gout * dchi, None, None, None, None
+
+
+def test_autograd_function_against_central_differences():
+    """Gradient through torch.autograd of the library adjoint vs central
+    differences (eps = 1e-3 relative for the parameters, a random direction
+    for the field) of the FP64 GPU forward run."""
+    cfg, model, fk, pk, sc = setup(PSC_VV2, PSC_EXPANSION_SELF)
+    steps, every = 10, 4
+    base = (fk, pk, sc)
+    N = cfg.N
+    g2 = torch.tensor(model.g2, dtype=torch.float64, requires_grad=True)
+    phi0 = torch.tensor(PHI0, dtype=torch.float64, requires_grad=True)
+    dchi0 = torch.zeros((N, N, N), dtype=torch.float64, requires_grad=True)
+    J = Chi2.apply(g2, phi0, dchi0, cfg, base, steps, every)
+    gg2, gphi0, gchi = torch.autograd.grad(J, (g2, phi0, dchi0))
+
+    def f(dg=0.0, dp=0.0, dc=None):
+        x = torch.zeros((N, N, N), dtype=torch.float64) if dc is None else dc
+        return float(Chi2.apply(g2.detach() + dg, phi0.detach() + dp, x, cfg, base, steps, every))
+
+    eps = 1e-3 * model.g2
+    fd = (f(dg=eps) - f(dg=-eps)) / (2 * eps)
+    print("g2", float(gg2), fd)
+    assert abs(float(gg2) - fd) <= 1e-4 * abs(fd)
+    eps = 1e-3 * PHI0
+    fd = (f(dp=eps) - f(dp=-eps)) / (2 * eps)
+    print("phi0", float(gphi0), fd)
+    assert abs(float(gphi0) - fd) <= 1e-4 * abs(fd)
+    v = torch.from_numpy(np.random.default_rng(5).standard_normal((N, N, N)) * 1e-3)
+    eps = 1e-3
+    fd = (f(dc=eps * v) - f(dc=-eps * v)) / (2 * eps)
+    ad = float((gchi * v).sum())
+    print("chi dir", ad, fd)
+    assert abs(ad - fd) <= 1e-4 * abs(fd)
+
+
+@pytest.mark.parametrize("precision,tol", [(PSC_FP64, 1e-9), (PSC_FP32, 1e-2)])
+def test_adjoint_at_128_matches_oracle(precision, tol):
+    """The 128^3 adjoint workload (g^2 = 1e4, kIR = 2, dt = 0.004), 20 steps
+    with a checkpoint every 5, against the FP64 oracle adjoint."""
+    cfg, model, fk, pk, sc = setup(PSC_VV2, PSC_EXPANSION_SELF, N=128, precision=precision, g2=1e4,
+                                   kIR=2.0, dt=0.004, omega_star=1e-4)
+    print("case 128", precision, flush=True)
+    lat = lattice(cfg, model, fk, pk, sc)
+    g = lat.adjoint(20, 5, PHI0, DPHI0)
+    lat.close()
+    J, dg2, dphi0, dchi = O.adjoint_chi2(cfg, model, fk, pk, 1.0, PHI0, DPHI0, 0.0, 20)
+    errs = (abs(g["J"] - J) / abs(J), abs(g["dg2"] - dg2) / abs(dg2), abs(g["dphi0"] - dphi0) / abs(dphi0),
+            rel_err(g["dchi"], dchi))
+    print("128 errors J, g2, phi0, chi:", errs)
+    assert max(errs) <= tol
+
+
+def test_graph_captured_segments_and_workspace_reuse_agree_with_eager():
+    cfg, model, fk, pk, sc = setup(PSC_Y4, PSC_EXPANSION_SELF)
+    eager = lattice(cfg, model, fk, pk, sc)
+    ref = eager.adjoint(8, 3, PHI0, DPHI0)
+    eager.close()
+    cfg.use_graph = 1
+    lat = lattice(cfg, model, fk, pk, sc)
+    for _ in range(2):   # the second call reuses the workspace and the captured segment
+        g = lat.adjoint(8, 3, PHI0, DPHI0)
+        for k in ("J", "dg2", "dphi0"):
+            assert g[k] == ref[k], k
+        assert np.array_equal(g["dchi"], ref["dchi"])
+    lat.close()
