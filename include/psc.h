@@ -150,6 +150,10 @@ int psc_get_scalars(psc_ctx* ctx, int member, double* out);
 /* Advance every member by nsteps fixed steps of cfg->dt; replaces
  * `evolve_VV_k_nopad(dt)` called from the main loop (pspectcosmo.cpp:636). */
 int psc_step(psc_ctx* ctx, int nsteps);
+/* Change the time step used by later psc_step calls, for the reference's
+ * adaptive rule dt = 2 / (k_max / a) / 10 clamped to output times
+ * (calculate_dt, pspectcosmo.cpp:213-233, 622-627).  Drops captured steps. */
+int psc_set_dt(psc_ctx* ctx, double dt);
 /* Block until all device work of the context is done. */
 int psc_sync(psc_ctx* ctx);
 
@@ -163,6 +167,11 @@ int psc_get_fields(psc_ctx* ctx, int member, double* f, double* pi);
  * constraint violation |H^2 - rho/3| / |H^2 + rho/3| (output.cpp:1180-1184),
  * a, da/dt, d2a/dt2, t, then <f_0>, <f_1>, var f_0, var f_1 (output.cpp:380-461). */
 int psc_energies(psc_ctx* ctx, int member, double* out);
+
+/* Means and variances as written by meansvars() (output.cpp:380-461), per
+ * field q: out[6q .. 6q+5] = <f>, a^{alpha-3} <pi>, <f^2>, a^{2(alpha-3)} <pi^2>,
+ * sigma_f, a^{alpha-3} sigma_pi (<pi>, <pi^2> from the k = 0 mode and Parseval). */
+int psc_means(psc_ctx* ctx, int member, double* out);
 
 /* Binned scalar spectra of field `fld`, replacing spectra() (output.cpp:490-705).
  * out holds nbins rows of PSC_SPECTRA_COLS doubles for bins 1..nbins, where

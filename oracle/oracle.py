@@ -49,6 +49,7 @@ def lib():
         L.orc_kernel_a.restype = C.c_double
         L.orc_kernel_a.argtypes = [C.c_void_p]
         L.orc_step.argtypes = [C.c_void_p, C.c_int]
+        L.orc_set_dt.argtypes = [C.c_void_p, C.c_double]
         L.orc_get_k.argtypes = [C.c_void_p, _D, _D]
         L.orc_get_scalars.argtypes = [C.c_void_p, _D]
         L.orc_get_fields.argtypes = [C.c_void_p, _D, _D]
@@ -181,6 +182,9 @@ class Oracle:
 
     def step(self, n=1):
         lib().orc_step(self._h, int(n))
+
+    def set_dt(self, dt):
+        lib().orc_set_dt(self._h, float(dt))
 
     def get_k(self):
         N = self.N

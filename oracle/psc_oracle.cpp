@@ -859,6 +859,13 @@ int orc_step(orc_ctx* c, int nsteps) {
     return 0;
 }
 
+// Time step of later orc_step calls (the main loop recomputes dt every step,
+// pspectcosmo.cpp:622-627).
+int orc_set_dt(orc_ctx* c, double dt) {
+    c->cfg.dt = dt;
+    return 0;
+}
+
 int orc_get_k(orc_ctx* c, double* f_k, double* pi_k) {
     for (int fl = 0; fl < c->cfg.nflds; ++fl) {
         if (f_k) std::memcpy(f_k + (size_t)fl * c->nk * 2, c->f_k[fl].data(), c->nk * sizeof(cd));

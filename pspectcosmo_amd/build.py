@@ -1,4 +1,5 @@
-"""Build the HIP library libpsc.so in-tree with hipcc for gfx950."""
+"""Build the HIP library libpsc.so in-tree with hipcc for gfx950, the
+command-line driver psc_run (host C++ over the C ABI) and the copy probe."""
 import os
 import subprocess
 import sys
@@ -37,6 +38,17 @@ def build(verbose=False):
         sys.stderr.write(r.stdout)
         raise RuntimeError("hipcc failed building copy_probe")
     os.replace(probe + ".tmp", probe)
+    run = os.path.join(HERE, "psc_run")
+    cxx = os.environ.get("CXX", "g++")
+    cmd = [cxx, "-O2", "-std=c++17", "-Wall", "-o", run + ".tmp", os.path.join(CSRC, "psc_run.cpp"),
+           "-L" + HERE, "-lpsc", "-Wl,-rpath,$ORIGIN"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout)
+        raise RuntimeError("building psc_run failed")
+    os.replace(run + ".tmp", run)
     return LIB
 
 

@@ -1271,6 +1271,57 @@ int psc_get_fields(psc_ctx* c, int member, double* f, double* pi) {
     return PSC_OK;
 }
 
+int psc_set_dt(psc_ctx* c, double dt) {
+    if (!c) return PSC_ERR_ARG;
+    if (!(dt > 0.0) || !std::isfinite(dt)) return c->fail(PSC_ERR_ARG, "dt must be positive");
+    if (dt != c->cfg.dt) {
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        c->cfg.dt = dt;
+        drop_graphs(c);
+    }
+    return PSC_OK;
+}
+
+int psc_means(psc_ctx* c, int member, double* out) {
+    int rc = check_member(c, member);
+    if (rc) return rc;
+    MemberState s;
+    if ((rc = read_state(c, member, &s))) return rc;
+    Diag d;
+    if ((rc = diagnose(c, member, s, &d))) return rc;
+    // Sum |pi_k|^2 per field (the kinetic sums) and the k = 0 mode of P_pi
+    if ((rc = c->eng->field_sums(c, member))) return rc;
+    double fs[4];
+    HIPCHK(c, hipMemcpyAsync(fs, c->sums, sizeof(fs), hipMemcpyDeviceToHost, c->stream));
+    double p0[2] = {0.0, 0.0};
+    const char* src = static_cast<const char*>(c->Pp) + (size_t)member * c->MS * c->es;
+    if (c->cfg.precision == PSC_FP32) {
+        float v[2];
+        HIPCHK(c, hipMemcpyAsync(v, src, sizeof(v), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        p0[0] = v[0];
+        p0[1] = v[1];
+    } else {
+        HIPCHK(c, hipMemcpyAsync(p0, src, sizeof(p0), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
+    const double N3 = (double)c->N3, aop = std::pow(s.a, c->cfg.alpha - 3);
+    for (int q = 0; q < c->cfg.nflds; ++q) {
+        const double pav = p0[q] / N3, psq = fs[q] / (N3 * N3);
+        const double vf = d.sumsq[q] - d.mean[q] * d.mean[q], vp = psq - pav * pav;
+        double* o = out + 6 * q;
+        o[0] = d.mean[q];
+        o[1] = aop * pav;
+        o[2] = d.sumsq[q];
+        o[3] = aop * aop * psq;
+        o[4] = vf > 0 ? std::sqrt(vf) : 0.0;
+        o[5] = vp > 0 ? aop * std::sqrt(vp) : 0.0;
+        if (!std::isfinite(d.mean[q]))  // output.cpp:450-454
+            return c->fail(PSC_ERR_NONFINITE, "field " + std::to_string(q) + " not numerical at t = " + std::to_string(s.t));
+    }
+    return PSC_OK;
+}
+
 int psc_energies(psc_ctx* c, int member, double* out) {
     int rc = check_member(c, member);
     if (rc) return rc;

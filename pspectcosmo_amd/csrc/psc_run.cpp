@@ -1,0 +1,479 @@
+// psc_run.cpp -- command-line driver over the C ABI: the main loop of the
+// reference program (main, pspectcosmo.cpp:237-705) for the spectral stepper
+// (evolve_VV_k_nopad), writing the reference's text outputs (output.cpp):
+//   average_scalar_<f>.txt, average_scale_factor.txt, average_energies.txt,
+//   average_energy_conservation.txt          every `freq` (save_Freq, output.cpp:1831-1840)
+//   spectra_scalar_<f>.txt, spectra_scalar_<f>_evolution.txt,
+//   average_spectra_times.txt, spectra_gws.txt, energy_gws.txt
+//                                            every `infreq` (save_Infreq, output.cpp:1842-1848)
+//   info.txt                                 at the end (output_parameters, output.cpp:1750)
+// and snapshots of the real-space fields every `snap` (save_Snap, output.cpp:1850-1853).
+//
+// Parameters (names follow parameters.h) come from an optional file of
+// `key = value` lines and from `key=value` arguments; `psc_run --help` lists
+// them.  Two time-step modes:
+//   dt = <value>  fixed step; output every freq_steps / infreq_steps steps;
+//   dt = auto     the reference's rule dt = 2 / (kIR sqrt(3) N / 2 / a) / 10,
+//                 capped at tOutputFreq and clamped to the next output time
+//                 (pspectcosmo.cpp:619-627), outputs at t within 1e-10 of the
+//                 output times (pspectcosmo.cpp:652-697).
+#include <cerrno>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <iomanip>
+#include <iostream>
+#include <map>
+#include <sstream>
+#include <string>
+#include <sys/stat.h>
+#include <vector>
+
+#include "../../include/psc.h"
+
+namespace {
+
+const int kFreqPrec = 3, kInfreqPrec = 2, kSetw = 15, kKirPrec = 1;   // parameters.h:29,51-55
+
+struct Params {
+    std::map<std::string, std::string> kv;
+    std::string get(const std::string& k, const std::string& def) const {
+        auto it = kv.find(k);
+        return it == kv.end() ? def : it->second;
+    }
+    double num(const std::string& k, double def) const {
+        auto it = kv.find(k);
+        return it == kv.end() ? def : std::strtod(it->second.c_str(), nullptr);
+    }
+    std::vector<double> list(const std::string& k, std::vector<double> def) const {
+        auto it = kv.find(k);
+        if (it == kv.end()) return def;
+        std::vector<double> v;
+        std::string s = it->second;
+        for (char& ch : s)
+            if (ch == ',') ch = ' ';
+        std::istringstream in(s);
+        double x;
+        while (in >> x) v.push_back(x);
+        return v;
+    }
+};
+
+const char* kKeys[][2] = {
+    {"N", "points per edge (16..512, power of two)"},
+    {"nflds", "1 or 2"},
+    {"kIR", "infrared momentum; L = 2 pi / kIR"},
+    {"dt", "time step, or `auto` for the reference's adaptive rule"},
+    {"tf", "final time (default: nsteps * dt)"},
+    {"nsteps", "fixed-dt runs: number of steps"},
+    {"tOutputFreq", "auto dt: interval of the frequent outputs"},
+    {"tOutputInfreq", "auto dt: interval of the spectra"},
+    {"tSnapFreq", "auto dt: interval of the snapshots (0: none)"},
+    {"freq_steps", "fixed dt: steps between frequent outputs"},
+    {"infreq_steps", "fixed dt: steps between spectra"},
+    {"snap_steps", "fixed dt: steps between snapshots (0: none)"},
+    {"integrator", "VV (2) or Y4 (4)"},
+    {"yoshida_exact", "1: Yoshida weights from 2^(1/3); 0: the reference's digits"},
+    {"precision", "fp64 or fp32"},
+    {"expansion", "0 no expansion, 2 self-consistent"},
+    {"alpha", "time variable exponent (0: cosmic time)"},
+    {"fStar", "field unit"},
+    {"omegaStar", "time unit"},
+    {"MPl", "Planck mass"},
+    {"m2", "V = m2 phi^2 / 2 + g2 phi^2 chi^2 / 2"},
+    {"g2", "coupling"},
+    {"initfield", "homogeneous field values, comma separated"},
+    {"initderivs", "homogeneous field derivatives, comma separated"},
+    {"seed", "Park-Miller seed"},
+    {"with_gw", "1: evolve the 6 GW components"},
+    {"gw_projection", "ref (the reference's one-sided projection) or tt"},
+    {"use_graph", "1: replay captured steps"},
+    {"autotune", "1: time the launch shapes before the run"},
+    {"dir", "output directory (default output)"},
+};
+
+void usage() {
+    std::printf("usage: psc_run [param-file] [key=value ...] [--dry-run]\n\nkeys:\n");
+    for (auto& k : kKeys) std::printf("  %-15s %s\n", k[0], k[1]);
+}
+
+bool parse_line(const std::string& line, Params& p) {
+    std::string s = line.substr(0, line.find('#'));
+    const size_t eq = s.find('=');
+    if (eq == std::string::npos) return false;
+    auto trim = [](std::string x) {
+        const size_t a = x.find_first_not_of(" \t\r\n"), b = x.find_last_not_of(" \t\r\n");
+        return a == std::string::npos ? std::string() : x.substr(a, b - a + 1);
+    };
+    const std::string k = trim(s.substr(0, eq)), v = trim(s.substr(eq + 1));
+    if (k.empty()) return false;
+    p.kv[k] = v;
+    return true;
+}
+
+bool known_key(const std::string& k) {
+    for (auto& e : kKeys)
+        if (k == e[0]) return true;
+    return false;
+}
+
+struct Run {
+    psc_config cfg{};
+    psc_model model{};
+    psc_ic_params ic{};
+    psc_ctx* ctx = nullptr;
+    std::string dir = "output";
+    bool auto_dt = false;
+    int gw_proj = PSC_GW_PROJ_REFERENCE;
+    int nbins = 0;
+    std::ofstream fields[2], sf, energy, conservation, spectra[2], spectra_ev[2], spectra_times, gw_spec, gw_energy;
+    double total_initial = 0.0;
+    bool first_energy = true;
+
+    int check(int rc, const char* what) {
+        if (rc != PSC_OK) {
+            std::fprintf(stderr, "psc_run: %s failed (%s): %s\n", what, psc_status_string(rc),
+                         ctx ? psc_last_error(ctx) : "");
+            std::exit(rc == PSC_ERR_NONFINITE || rc == PSC_ERR_NEGATIVE_A ? 1 : 2);
+        }
+        return rc;
+    }
+
+    std::string path(const std::string& name) const { return dir + "/" + name + ".txt"; }
+
+    void open_outputs() {
+        for (int f = 0; f < cfg.nflds; ++f) {
+            fields[f].open(path("average_scalar_" + std::to_string(f)), std::ios::out | std::ios::app);
+            spectra[f].open(path("spectra_scalar_" + std::to_string(f)), std::ios::out | std::ios::app);
+            spectra_ev[f].open(path("spectra_scalar_" + std::to_string(f) + "_evolution"),
+                               std::ios::out | std::ios::app);
+            // header row of bin momenta (output.cpp:527-533)
+            for (int i = 0; i < nbins + 1; ++i)
+                spectra_ev[f] << std::fixed << std::setprecision(kKirPrec) << cfg.kIR * i << " ";
+            spectra_ev[f] << std::endl;
+        }
+        if (cfg.expansion == PSC_EXPANSION_SELF)
+            sf.open(path("average_scale_factor"), std::ios::out | std::ios::app);
+        energy.open(path("average_energies"), std::ios::out | std::ios::app);
+        conservation.open(path("average_energy_conservation"), std::ios::out | std::ios::app);
+        spectra_times.open(path("average_spectra_times"), std::ios::out | std::ios::app);
+        if (cfg.with_gw) {
+            gw_spec.open(path("spectra_gws"), std::ios::out | std::ios::app);
+            gw_energy.open(path("energy_gws"), std::ios::out | std::ios::app);
+        }
+    }
+
+    // save_Freq: meansvars, scale, energy (output.cpp:380-486,1069-1187)
+    void save_freq() {
+        double sc[4];
+        check(psc_get_scalars(ctx, 0, sc), "psc_get_scalars");
+        const double t = sc[0], a = sc[1], ad = sc[2], add = sc[3];
+        double m[12];
+        check(psc_means(ctx, 0, m), "psc_means");
+        for (int f = 0; f < cfg.nflds; ++f) {
+            const double* o = m + 6 * f;
+            fields[f] << std::fixed << std::setprecision(kFreqPrec) << t << std::scientific << std::setprecision(6)
+                      << std::setw(kSetw) << o[0] << std::setw(kSetw) << o[1] << std::setw(kSetw) << o[2]
+                      << std::setw(15) << o[3] << std::setw(kSetw) << o[4] << std::setw(kSetw) << o[5]
+                      << std::endl;
+        }
+        if (cfg.expansion == PSC_EXPANSION_SELF) {
+            sf << std::fixed << std::setprecision(6) << t << " " << a << " " << ad << " " << ad / a << " " << add
+               << " " << std::scientific << std::setprecision(6) << std::setw(kSetw)
+               << 1 / (std::pow(a, -cfg.alpha) * ad) << " " << std::fixed << std::setprecision(6)
+               << 1 + cfg.alpha - a * add / (ad * ad) << std::endl;
+        }
+        double e[PSC_ENERGY_LEN];
+        check(psc_energies(ctx, 0, e), "psc_energies");
+        const int nterms = cfg.nflds == 2 ? 2 : 1;
+        std::vector<double> vals;
+        for (int f = 0; f < cfg.nflds; ++f) {
+            vals.push_back(e[2 * f]);
+            vals.push_back(e[2 * f + 1]);
+        }
+        for (int k = 0; k < nterms; ++k) vals.push_back(e[4 + k]);
+        const double total = e[6];
+        vals.push_back(total);
+        energy << std::fixed << std::setprecision(kFreqPrec) << t;
+        for (double v : vals) energy << std::scientific << std::setprecision(6) << " " << v;
+        energy << std::endl;
+        if (first_energy) {
+            total_initial = total;
+            first_energy = false;
+        }
+        if (cfg.expansion == PSC_EXPANSION_STATIC) {
+            conservation << std::fixed << std::setprecision(kFreqPrec) << t << " " << std::scientific
+                         << std::setprecision(6) << total / total_initial << std::endl;
+        } else {
+            const double f2m = (cfg.fStar / cfg.MPl) * (cfg.fStar / cfg.MPl);
+            const double LHS = ad * ad, RHS = f2m * std::pow(a, 2 * (cfg.alpha + 1)) * total / 3.;
+            conservation << std::fixed << std::setprecision(kFreqPrec) << t << " " << std::scientific
+                         << std::setprecision(8) << std::fabs(LHS - RHS) / std::fabs(LHS + RHS) << " " << std::fixed
+                         << std::setprecision(6) << LHS << " " << RHS << std::endl;
+        }
+    }
+
+    // save_Infreq: spectra (output.cpp:490-705, GW 780-1062)
+    void save_infreq() {
+        double sc[4];
+        check(psc_get_scalars(ctx, 0, sc), "psc_get_scalars");
+        const double t = sc[0];
+        const double N = cfg.N, N3 = N * N * N, L = 2 * M_PI / cfg.kIR, dx = L / N;
+        std::vector<double> rows((size_t)nbins * PSC_SPECTRA_COLS);
+        for (int f = 0; f < cfg.nflds; ++f) {
+            check(psc_spectra(ctx, 0, f, rows.data(), nbins), "psc_spectra");
+            for (int b = 0; b < nbins; ++b) {
+                const double* r = &rows[(size_t)b * PSC_SPECTRA_COLS];
+                spectra[f] << std::fixed << std::setprecision(kKirPrec) << r[0] << " " << std::scientific
+                           << std::setprecision(6) << std::setw(kSetw) << r[1] << " " << std::setw(kSetw) << r[2]
+                           << " " << std::setw(kSetw) << r[3] << " " << (long long)r[4] << std::endl;
+            }
+            spectra[f] << std::endl;
+            // amplitude per bin (N/dx) omegaStar L^{3/2} / N^3 sqrt(<|f_k|^2>) (output.cpp:693-700)
+            spectra_ev[f] << std::fixed << std::setprecision(6) << t << " ";
+            for (int b = 0; b < nbins; ++b) {
+                const double* r = &rows[(size_t)b * PSC_SPECTRA_COLS];
+                const int i = b + 1;
+                const double f2 = r[4] > 0 ? r[1] * N3 * N3 / (i * r[4]) : 0.0;
+                spectra_ev[f] << std::scientific << std::setprecision(6) << std::setw(kSetw)
+                              << (N / dx) * cfg.omegaStar * std::pow(L, 1.5) / N3 * std::sqrt(f2) << " ";
+            }
+            spectra_ev[f] << std::endl;
+        }
+        spectra_times << std::fixed << std::setprecision(kInfreqPrec) << t << std::endl;
+        if (cfg.with_gw) {
+            std::vector<double> g((size_t)nbins * PSC_GW_SPECTRA_COLS);
+            double tot[2];
+            check(psc_gw_spectrum(ctx, 0, gw_proj, g.data(), nbins, tot), "psc_gw_spectrum");
+            for (int b = 0; b < nbins; ++b) {
+                const double* r = &g[(size_t)b * PSC_GW_SPECTRA_COLS];
+                gw_spec << std::fixed << std::setprecision(kKirPrec) << r[0] << " " << std::scientific
+                        << std::setprecision(6) << std::setw(kSetw) << r[1] << " " << std::scientific
+                        << std::setprecision(6) << std::setw(kSetw) << r[2] << " " << std::scientific
+                        << std::setprecision(6) << std::setw(kSetw) << r[3] << " " << (long long)r[4] << std::endl;
+            }
+            gw_spec << std::endl;
+            gw_energy << std::fixed << std::setprecision(kInfreqPrec) << t << " " << std::scientific
+                      << std::setprecision(6) << tot[0] << " " << tot[1] << std::endl;
+        }
+    }
+
+    // Snapshot of the real-space fields.  The reference writes HDF5
+    // (snapshots, output.cpp:1541-1648); with no HDF5 library available this
+    // writes each dataset the reference names as raw little-endian float64
+    // N^3 arrays in C order, plus a text index.
+    void save_snap(int number) {
+        const std::string sdir = dir + "/Snapshots";
+        mkdir(sdir.c_str(), 0755);
+        const size_t n3 = (size_t)cfg.N * cfg.N * cfg.N;
+        std::vector<double> f(n3 * cfg.nflds), p(n3 * cfg.nflds);
+        check(psc_get_fields(ctx, 0, f.data(), p.data()), "psc_get_fields");
+        double sc[4];
+        check(psc_get_scalars(ctx, 0, sc), "psc_get_scalars");
+        const std::string base = sdir + "/data_" + std::to_string(number) + "_snapshot";
+        std::ofstream idx(base + ".txt");
+        idx << "t " << std::setprecision(17) << sc[0] << "\na " << sc[1] << "\nN " << cfg.N << "\n";
+        for (int q = 0; q < cfg.nflds; ++q) {
+            const std::string name = "ScalarField_" + std::to_string(q);
+            std::ofstream out(base + "." + name + ".f64", std::ios::binary);
+            out.write(reinterpret_cast<const char*>(f.data() + q * n3), n3 * sizeof(double));
+            idx << "/" << name << " float64 " << cfg.N << "x" << cfg.N << "x" << cfg.N << " " << name << ".f64\n";
+        }
+    }
+
+    void write_info(double dt_last, long long steps) {
+        std::ofstream info(path("info"), std::ios::out);
+        const double L = 2 * M_PI / cfg.kIR;
+        info << "--------------------------\nModel Specific Information\n--------------------------\n";
+        info << "V = m2 phi^2 / 2 + g2 phi^2 chi^2 / 2, m2 = " << model.m2 << ", g2 = " << model.g2 << "\n";
+        info << "\n--------------------------\nGeneral Program Information\n-----------------------------\n";
+        info << "Grid size=" << cfg.N << "^3\n";
+        info << "Number of fields=" << cfg.nflds << "\n";
+        info << "L=" << L << "\n";
+        info << "kIR=" << cfg.kIR << "\n";
+        info << std::setprecision(6) << "dt=" << dt_last << ", dt/dx=" << dt_last / (L / cfg.N) << "\n";
+        info << (cfg.expansion == PSC_EXPANSION_STATIC ? "No expansion\n" : "Self-consistent expansion\n");
+        info << "Integrator=" << (cfg.integrator == PSC_Y4 ? "Yoshida-4" : "velocity Verlet") << "\n";
+        info << "Precision=" << (cfg.precision == PSC_FP32 ? "FP32 fields, FP64 background" : "FP64") << "\n";
+        info << "Steps=" << steps << "\n";
+        info << "Launch shape=" << psc_shape_name(ctx) << "\n";
+    }
+};
+
+}  // namespace
+
+int main(int argc, char** argv) {
+    Params p;
+    bool dry = false;
+    for (int i = 1; i < argc; ++i) {
+        const std::string a = argv[i];
+        if (a == "--help" || a == "-h") {
+            usage();
+            return 0;
+        }
+        if (a == "--dry-run") {
+            dry = true;
+            continue;
+        }
+        if (a.find('=') != std::string::npos) {
+            parse_line(a, p);
+            continue;
+        }
+        std::ifstream in(a);
+        if (!in) {
+            std::fprintf(stderr, "psc_run: cannot read %s\n", a.c_str());
+            return 2;
+        }
+        std::string line;
+        while (std::getline(in, line)) parse_line(line, p);
+    }
+    for (auto& kv : p.kv)
+        if (!known_key(kv.first)) {
+            std::fprintf(stderr, "psc_run: unknown parameter %s\n", kv.first.c_str());
+            return 2;
+        }
+
+    Run r;
+    psc_config& c = r.cfg;
+    c.N = (int)p.num("N", 64);
+    c.nflds = (int)p.num("nflds", 2);
+    const std::string integ = p.get("integrator", "VV");
+    c.integrator = (integ == "Y4" || integ == "4") ? PSC_Y4 : PSC_VV2;
+    c.yoshida_exact = (int)p.num("yoshida_exact", 0);
+    c.precision = p.get("precision", "fp64") == "fp32" ? PSC_FP32 : PSC_FP64;
+    c.batch = 1;
+    c.with_gw = (int)p.num("with_gw", 0);
+    c.alpha = (int)p.num("alpha", 0);
+    c.expansion = (int)p.num("expansion", 2) == 0 ? PSC_EXPANSION_STATIC : PSC_EXPANSION_SELF;
+    c.use_graph = (int)p.num("use_graph", 1);
+    c.autotune = (int)p.num("autotune", 0);
+    c.kIR = p.num("kIR", 25.0);
+    c.fStar = p.num("fStar", 1.0);
+    c.omegaStar = p.num("omegaStar", 1.0);
+    c.MPl = p.num("MPl", 1.0);
+    r.model.m2 = p.num("m2", 1.0);
+    r.model.g2 = p.num("g2", 2.5e5);
+    const std::vector<double> f0 = p.list("initfield", {1.0, 0.0}), fd0 = p.list("initderivs", {-0.7, 0.0});
+    for (int q = 0; q < 2; ++q) {
+        r.ic.f0[q] = q < (int)f0.size() ? f0[q] : 0.0;
+        r.ic.fd0[q] = q < (int)fd0.size() ? fd0[q] : 0.0;
+    }
+    r.ic.a0 = 1.0;
+    r.ic.seed = (int)p.num("seed", 1);
+    r.dir = p.get("dir", "output");
+    r.gw_proj = p.get("gw_projection", "ref") == "tt" ? PSC_GW_PROJ_TT : PSC_GW_PROJ_REFERENCE;
+    const double kmax = c.kIR * c.N * std::sqrt(3.0) / 2.0;
+    auto calculate_dt = [&](double a) { return 2.0 / (kmax / a) / 10; };   // pspectcosmo.cpp:213-233
+    r.auto_dt = p.get("dt", "auto") == "auto";
+    c.dt = r.auto_dt ? calculate_dt(r.ic.a0) : p.num("dt", 0.005);
+    if (r.auto_dt) c.use_graph = 0;   // dt changes every step
+    const long long nsteps = (long long)p.num("nsteps", 200);
+    const double tf = p.num("tf", r.auto_dt ? 1.0 : nsteps * c.dt);
+    const double tOutFreq = p.num("tOutputFreq", 0.01), tOutInfreq = p.num("tOutputInfreq", 0.1);
+    const double tSnap = p.num("tSnapFreq", 0.0);
+    const long long freq_steps = (long long)p.num("freq_steps", 20), infreq_steps = (long long)p.num("infreq_steps", 20);
+    const long long snap_steps = (long long)p.num("snap_steps", 0);
+    r.nbins = psc_num_bins(c.N);
+
+    if (dry) {
+        std::printf("N=%d nflds=%d integrator=%s precision=%s expansion=%d alpha=%d kIR=%g dt=%s%g "
+                    "m2=%g g2=%g initfield=%g,%g initderivs=%g,%g seed=%d with_gw=%d dir=%s\n",
+                    c.N, c.nflds, c.integrator == PSC_Y4 ? "Y4" : "VV", c.precision == PSC_FP32 ? "fp32" : "fp64",
+                    c.expansion, c.alpha, c.kIR, r.auto_dt ? "auto:" : "", c.dt, r.model.m2, r.model.g2,
+                    r.ic.f0[0], r.ic.f0[1], r.ic.fd0[0], r.ic.fd0[1], r.ic.seed, c.with_gw, r.dir.c_str());
+        if (r.auto_dt)
+            std::printf("tf=%g tOutputFreq=%g tOutputInfreq=%g tSnapFreq=%g\n", tf, tOutFreq, tOutInfreq, tSnap);
+        else
+            std::printf("nsteps=%lld freq_steps=%lld infreq_steps=%lld snap_steps=%lld\n", nsteps, freq_steps,
+                        infreq_steps, snap_steps);
+        return 0;
+    }
+
+    if (mkdir(r.dir.c_str(), 0755) != 0 && errno != EEXIST) {
+        std::fprintf(stderr, "psc_run: cannot create %s\n", r.dir.c_str());
+        return 2;
+    }
+    // initial conditions (initialize(), initialize.cpp:624-800)
+    const size_t nk = (size_t)c.nflds * c.N * c.N * (c.N / 2 + 1) * 2;
+    std::vector<double> fk(nk), pk(nk);
+    double sc0[4];
+    r.check(psc_generate_ic(&c, &r.model, &r.ic, fk.data(), pk.data(), sc0), "psc_generate_ic");
+    r.check(psc_create(&c, &r.ctx), "psc_create");
+    r.check(psc_set_model(r.ctx, 0, &r.model), "psc_set_model");
+    r.check(psc_load_ic_k(r.ctx, 0, fk.data(), pk.data(), sc0[0], sc0[1]), "psc_load_ic_k");
+    if (c.with_gw) {
+        const size_t gk = (size_t)6 * c.N * c.N * (c.N / 2 + 1) * 2;
+        std::vector<double> u(gk), pu(gk);
+        r.check(psc_generate_gw_ic(&c, &r.model, &r.ic, u.data(), pu.data()), "psc_generate_gw_ic");
+        r.check(psc_load_gw_k(r.ctx, 0, u.data(), pu.data()), "psc_load_gw_k");
+    }
+    if (c.autotune) {
+        char rep[1024];
+        r.check(psc_autotune(r.ctx, 2, rep, sizeof(rep)), "psc_autotune");
+        std::printf("autotune: %s -> %s\n", rep, psc_shape_name(r.ctx));
+    }
+    r.open_outputs();
+
+    long long steps = 0;
+    int snaps = 0;
+    double dt_last = c.dt;
+    if (!r.auto_dt) {
+        // fixed dt with a step-count cadence
+        long long done = 0;
+        while (done < nsteps) {
+            long long next = nsteps;
+            auto upto = [&](long long every) {
+                if (every > 0) next = std::min(next, (done / every + 1) * every);
+            };
+            upto(freq_steps);
+            upto(infreq_steps);
+            upto(snap_steps);
+            r.check(psc_step(r.ctx, (int)(next - done)), "psc_step");
+            done = next;
+            if (freq_steps > 0 && done % freq_steps == 0) r.save_freq();
+            if (infreq_steps > 0 && done % infreq_steps == 0) {
+                r.save_infreq();
+                std::printf("Lattice Step %lld done. Current time: %g\n", done, done * c.dt);
+                std::fflush(stdout);
+            }
+            if (snap_steps > 0 && done % snap_steps == 0) r.save_snap(++snaps);
+        }
+        steps = done;
+    } else {
+        // the reference's loop (pspectcosmo.cpp:601-697)
+        double t = 0.0, a = r.ic.a0;
+        double next_freq = tOutFreq, next_infreq = tOutInfreq, next_snap = tSnap > 0 ? tSnap : 1e300;
+        while (t < tf) {
+            double dt = std::min(calculate_dt(a), tOutFreq);
+            if (t + dt > next_freq || t + dt > next_infreq) dt = std::min(next_freq - t, next_infreq - t);
+            r.check(psc_set_dt(r.ctx, dt), "psc_set_dt");
+            r.check(psc_step(r.ctx, 1), "psc_step");
+            dt_last = dt;
+            ++steps;
+            double s[4];
+            r.check(psc_get_scalars(r.ctx, 0, s), "psc_get_scalars");
+            t = s[0];
+            a = s[1];
+            if (std::fabs(t - next_freq) < 1e-10) {
+                next_freq += tOutFreq;
+                r.save_freq();
+            }
+            if (std::fabs(t - next_infreq) < 1e-10) {
+                next_infreq += tOutInfreq;
+                r.save_infreq();
+                std::printf("Lattice Step %lld done. Current time: %g\n", steps, t);
+                std::fflush(stdout);
+            }
+            if (std::fabs(t - next_snap) < 1e-10) {
+                next_snap += tSnap;
+                r.save_snap(++snaps);
+            }
+        }
+    }
+    r.write_info(dt_last, steps);
+    psc_destroy(r.ctx);
+    std::printf("PSpectCosmo MI355X run finished: %lld steps\n", steps);
+    return 0;
+}

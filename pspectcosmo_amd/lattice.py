@@ -48,6 +48,8 @@ def lib():
         L.psc_set_scalars.argtypes = [vp, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double]
         L.psc_get_scalars.argtypes = [vp, C.c_int, _D]
         L.psc_step.argtypes = [vp, C.c_int]
+        L.psc_set_dt.argtypes = [vp, C.c_double]
+        L.psc_means.argtypes = [vp, C.c_int, _D]
         L.psc_sync.argtypes = [vp]
         L.psc_get_k.argtypes = [vp, C.c_int, _D, _D]
         L.psc_get_fields.argtypes = [vp, C.c_int, _D, _D]
@@ -80,7 +82,7 @@ def exported_symbols():
             "psc_get_fields", "psc_energies", "psc_spectra", "psc_load_gw_k", "psc_get_gw_k",
             "psc_gw_spectrum", "psc_gw_omega_log", "psc_num_bins", "psc_set_kernel_timing", "psc_get_kernel_times",
             "psc_reset_kernel_times", "psc_device_bytes", "psc_autotune", "psc_shape_name",
-            "psc_list_shapes", "psc_set_shape", "psc_adjoint"]
+            "psc_list_shapes", "psc_set_shape", "psc_adjoint", "psc_set_dt", "psc_means"]
 
 
 def generate_ic(cfg, model, ic):
@@ -151,6 +153,15 @@ class Lattice:
 
     def step(self, n=1):
         self._check(lib().psc_step(self._h, int(n)))
+
+    def set_dt(self, dt):
+        self._check(lib().psc_set_dt(self._h, float(dt)))
+
+    def means(self, member=0):
+        """Per field: <f>, a^{alpha-3}<pi>, <f^2>, a^{2(alpha-3)}<pi^2>, sigma_f, sigma_fd."""
+        out = np.zeros(12)
+        self._check(lib().psc_means(self._h, member, out))
+        return out.reshape(2, 6)[: self.nf]
 
     def sync(self):
         self._check(lib().psc_sync(self._h))
