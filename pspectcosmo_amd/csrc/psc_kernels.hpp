@@ -23,7 +23,7 @@ namespace psc {
 
 // ---------------------------------------------------------------------------
 // A: contiguous-axis inverse pass; DRIFT fuses f += c_drift * pi (evolution.cpp:1164-1169).
-template <class G, bool DRIFT>
+template <class G, bool DRIFT, bool REC = false>
 __global__ void __launch_bounds__((G::THREADS_ROW))
 k_z_inverse(const cpx<typename G::T>* __restrict__ src, cpx<typename G::T>* __restrict__ Pf, const cpx<typename G::T>* __restrict__ Pp,
             cpx<typename G::T>* __restrict__ W, const MemberState* __restrict__ ms,
@@ -43,7 +43,7 @@ k_z_inverse(const cpx<typename G::T>* __restrict__ src, cpx<typename G::T>* __re
         for (int m = 0; m < G::E; ++m) {
             cpx<T> f = Pf[base + G::TPL * m];
             const cpx<T> p = Pp[base + G::TPL * m];
-            if (rec_f) {  // adjoint recording: substep start f and pi_half
+            if constexpr (REC) {  // adjoint recording: substep start f and pi_half
                 rec_f[base + G::TPL * m] = f;
                 rec_ph[base + G::TPL * m] = p;
             }
@@ -87,7 +87,7 @@ k_strided(cpx<typename G::T>* __restrict__ W, const cpx<typename G::T>* __restri
 // C: x-axis inverse, pointwise V'(phi, chi) of model.h (dvdf, model.h:75-82;
 // kernel_field_k_nopad evolution.cpp:1103-1109) and x-axis forward, with the
 // lattice sum of V (kernel_a_k_nopad evolution.cpp:1077-1085) in FP64.
-template <class G>
+template <class G, bool REC = false>
 __global__ void __launch_bounds__((G::THREADS_COL))
 k_x_force(cpx<typename G::T>* __restrict__ W, const MemberState* __restrict__ ms, const cpx<typename G::T>* __restrict__ tw,
           double* __restrict__ partV, cpx<typename G::T>* __restrict__ rec_r = nullptr) {
@@ -111,7 +111,7 @@ k_x_force(cpx<typename G::T>* __restrict__ W, const MemberState* __restrict__ ms
 #pragma unroll
     for (int m = 0; m < G::E; ++m) {
         const T phi = e[m].x * inv, chi = e[m].y * inv;
-        if (rec_r) rec_r[base + (size_t)G::TPL * m * stride] = cpx<T>{phi, chi};  // adjoint recording
+        if constexpr (REC) rec_r[base + (size_t)G::TPL * m * stride] = cpx<T>{phi, chi};  // adjoint recording
         const T chi2 = chi * chi, phi2 = phi * phi;
         e[m].x = m2 * phi + g2 * phi * chi2;
         e[m].y = g2 * phi2 * chi;
@@ -132,7 +132,7 @@ k_x_force(cpx<typename G::T>* __restrict__ W, const MemberState* __restrict__ ms
 // unnormalised spectrum of V'.  c1/c2 are w dt / 2 of the two kicks applied
 // with this force; REDUCE accumulates sum |pi|^2 after the first kick and
 // sum n^2 |f|^2 (kinetic_energy_k / gradient_energy_k, evolution.cpp:923-1035).
-template <class G, bool REDUCE>
+template <class G, bool REDUCE, bool REC = false>
 __global__ void __launch_bounds__((G::THREADS_ROW))
 k_z_kick(const cpx<typename G::T>* __restrict__ W, const cpx<typename G::T>* __restrict__ Pf, cpx<typename G::T>* __restrict__ Pp,
          const MemberState* __restrict__ ms, const cpx<typename G::T>* __restrict__ tw,
@@ -150,8 +150,8 @@ k_z_kick(const cpx<typename G::T>* __restrict__ W, const cpx<typename G::T>* __r
     for (int m = 0; m < G::E; ++m) e[m] = W[base + G::TPL * m];
     line_fft<T, N, typename G::Rad, -1, typename G::LayRow>(e, t, c, lds, tw);
     const double a = use_a_start ? ms[member].a_start : ms[member].a;
-    const T lapc = (T)(ipow(a, 1 + sc.alpha) * sc.kIR * sc.kIR);
-    const T vc = (T)ipow(a, 3 + sc.alpha);
+    const T lapc = (T)(pow(a, 1 + sc.alpha) * sc.kIR * sc.kIR);
+    const T vc = (T)pow(a, 3 + sc.alpha);
     const T c1 = (T)c1d, c2 = (T)c2d;
     const int ix = (int)(line / N), iy = (int)(line % N);
     const int px = signed_index(ix, N), py = signed_index(iy, N);
@@ -168,7 +168,7 @@ k_z_kick(const cpx<typename G::T>* __restrict__ W, const cpx<typename G::T>* __r
         const cpx<T> F = {-ln2 * f.x - vc * e[m].x, -ln2 * f.y - vc * e[m].y};
         p.x += c1 * F.x;
         p.y += c1 * F.y;
-        if (rec_p) rec_p[base + G::TPL * m] = p;  // adjoint recording: synchronised pi
+        if constexpr (REC) rec_p[base + G::TPL * m] = p;  // adjoint recording: synchronised pi
         if constexpr (REDUCE) {
             const double px_ = p.x, py_ = p.y, fx = f.x, fy = f.y;
             red[0] += px_ * px_ + py_ * py_;
@@ -388,7 +388,7 @@ k_spectra(const cpx<T>* __restrict__ Pf, const cpx<T>* __restrict__ Pp,
     const cpx<T>* f = Pf + (size_t)member * member_stride(N);
     const cpx<T>* p = Pp + (size_t)member * member_stride(N);
     const double a = ms[member].a, ad = ms[member].ad;
-    const double aop = ipow(a, alpha - 3), hub = ad / a;
+    const double aop = pow(a, alpha - 3), hub = ad / a;
     const size_t nk = (size_t)N * N * NH;
     for (size_t h = (size_t)blockIdx.x * blockDim.x + threadIdx.x; h < nk; h += (size_t)blocks_per_member * blockDim.x) {
         const int i = (int)(h / ((size_t)N * NH)), j = (int)((h / NH) % N), k = (int)(h % NH);
@@ -599,9 +599,9 @@ k_z_kick_gw(const cpx<typename G::T>* __restrict__ W, const cpx<typename G::T>* 
     const size_t line = (size_t)blockIdx.x * G::L + c;
     const size_t base = (size_t)member * G::MS + (line / N) * G::PS + (line % N) * N + t;
     const double a = use_a_start ? ms[member].a_start : ms[member].a;
-    const T norm1 = (T)ipow(a, 1 + sc.alpha);
-    const T lapc = (T)(ipow(a, 1 + sc.alpha) * sc.kIR * sc.kIR);
-    const T vc = (T)ipow(a, 3 + sc.alpha);
+    const T norm1 = (T)pow(a, 1 + sc.alpha);
+    const T lapc = (T)(pow(a, 1 + sc.alpha) * sc.kIR * sc.kIR);
+    const T vc = (T)pow(a, 3 + sc.alpha);
     const T k2c = (T)(sc.kIR * sc.kIR);
     const T c1 = (T)c1d, c2 = (T)c2d;
     const int ix = (int)(line / N), iy = (int)(line % N);

@@ -518,7 +518,7 @@ struct Engine final : EngineBase {
         hipLaunchKernelGGL((k_z_inverse<G, false>), rgrid, rblock, 0, c->stream, P(c->Pf), P(c->Pf), P(c->Pp),
                            P(c->W), c->ms, P(c->tw), nullptr, nullptr);
         if ((rc = launch_y<+1>(c, P(c->W), 1))) return rc;
-        hipLaunchKernelGGL((k_x_force<G>), cgrid, cblock, 0, c->stream, P(c->W), c->ms, P(c->tw), c->partV,
+        hipLaunchKernelGGL((k_x_force<G, true>), cgrid, cblock, 0, c->stream, P(c->W), c->ms, P(c->tw), c->partV,
                            P(w.r[0]));
         if ((rc = launch_y<-1>(c, P(c->W), 1))) return rc;
         hipLaunchKernelGGL((k_z_kick<G, false>), rgrid, rblock, 0, c->stream, P(c->W), P(c->Pf), P(c->Pp), c->ms,
@@ -527,15 +527,15 @@ struct Engine final : EngineBase {
             const int p = j % S;
             const bool last = j == steps * S - 1;
             const double wn = c->w[(p + 1) % S];
-            hipLaunchKernelGGL((k_z_inverse<G, true>), rgrid, rblock, 0, c->stream, P(c->Pf), P(c->Pf), P(c->Pp),
-                               P(c->W), c->ms, P(c->tw), P(w.f[j]), P(w.ph[j]));
+            hipLaunchKernelGGL((k_z_inverse<G, true, true>), rgrid, rblock, 0, c->stream, P(c->Pf), P(c->Pf),
+                               P(c->Pp), P(c->W), c->ms, P(c->tw), P(w.f[j]), P(w.ph[j]));
             if ((rc = launch_y<+1>(c, P(c->W), 1))) return rc;
-            hipLaunchKernelGGL((k_x_force<G>), cgrid, cblock, 0, c->stream, P(c->W), c->ms, P(c->tw), c->partV,
-                               P(w.r[j + 1]));
+            hipLaunchKernelGGL((k_x_force<G, true>), cgrid, cblock, 0, c->stream, P(c->W), c->ms, P(c->tw),
+                               c->partV, P(w.r[j + 1]));
             if ((rc = launch_y<-1>(c, P(c->W), 1))) return rc;
-            hipLaunchKernelGGL((k_z_kick<G, true>), rgrid, rblock, 0, c->stream, P(c->W), P(c->Pf), P(c->Pp),
-                               c->ms, P(c->tw), c->partKG, c->w[p] * dt / 2.0, last ? 0.0 : wn * dt / 2.0, 0, sc,
-                               P(w.p[j + 1]));
+            hipLaunchKernelGGL((k_z_kick<G, true, true>), rgrid, rblock, 0, c->stream, P(c->W), P(c->Pf),
+                               P(c->Pp), c->ms, P(c->tw), c->partKG, c->w[p] * dt / 2.0, last ? 0.0 : wn * dt / 2.0,
+                               0, sc, P(w.p[j + 1]));
             ScalarOp o{};
             o.finish = 1;
             o.last_of_step = p == S - 1;
