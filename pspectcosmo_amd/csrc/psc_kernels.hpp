@@ -211,20 +211,44 @@ __device__ __forceinline__ void comp_merge(double& s, double& c, double s2, doub
     c = c + c2 + err;
 }
 
-__global__ void __launch_bounds__(256)
+// Threads per block of k_scalars: the partial sums of up to kScalarDirect
+// blocks per pass are reduced directly, larger counts are first cut to
+// chunk sums by k_prereduce.
+constexpr int kScalarThreads = 256;
+constexpr int kScalarDirect = 512;
+
+__global__ void __launch_bounds__(kScalarThreads)
 k_scalars(MemberState* __restrict__ ms, const double* __restrict__ partV,
           const double* __restrict__ partKG, ScalarOp op, StepConsts sc, MemberState* __restrict__ rec = nullptr) {
-    __shared__ double sh[3 * 2 * 4];
+    constexpr int WAVES = kScalarThreads / 64;
+    __shared__ double sh[3 * 2 * WAVES];
     const int member = blockIdx.x;
     // the state is loaded up front (in flight during the reduction) and
     // written back once at the end
     MemberState s = ms[member];
     if (op.finish) {
         Comp acc[3];
-        for (int i = threadIdx.x; i < op.nV; i += blockDim.x) acc[0].add(partV[(size_t)member * op.nV + i]);
-        for (int i = threadIdx.x; i < op.nKG; i += blockDim.x) {
-            acc[1].add(partKG[((size_t)member * op.nKG + i) * 2]);
-            acc[2].add(partKG[((size_t)member * op.nKG + i) * 2 + 1]);
+        const double* pv = partV + (size_t)member * op.nV;
+        const double* pk = partKG + (size_t)member * op.nKG * 2;
+        // loads of a batch are all issued before the compensated adds, so a
+        // batch costs one memory round trip
+        constexpr int U = 4;
+        const int n = op.nV > op.nKG ? op.nV : op.nKG;
+        for (int i0 = threadIdx.x; i0 < n; i0 += U * kScalarThreads) {
+            double v[U];
+            double2 kg[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int i = i0 + u * kScalarThreads;
+                v[u] = i < op.nV ? pv[i] : 0.0;
+                kg[u] = i < op.nKG ? reinterpret_cast<const double2*>(pk)[i] : double2{0.0, 0.0};
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                acc[0].add(v[u]);
+                acc[1].add(kg[u].x);
+                acc[2].add(kg[u].y);
+            }
         }
         const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
@@ -235,16 +259,16 @@ k_scalars(MemberState* __restrict__ ms, const double* __restrict__ partV,
                 comp_merge(acc[q].s, acc[q].c, s2, c2);
             }
             if (lane == 0) {
-                sh[(q * 4 + wave) * 2] = acc[q].s;
-                sh[(q * 4 + wave) * 2 + 1] = acc[q].c;
+                sh[(q * WAVES + wave) * 2] = acc[q].s;
+                sh[(q * WAVES + wave) * 2 + 1] = acc[q].c;
             }
         }
         __syncthreads();
         if (threadIdx.x == 0) {
             double tot[3];
             for (int q = 0; q < 3; ++q) {
-                double ss = sh[(q * 4) * 2], cc = sh[(q * 4) * 2 + 1];
-                for (int w = 1; w < 4; ++w) comp_merge(ss, cc, sh[(q * 4 + w) * 2], sh[(q * 4 + w) * 2 + 1]);
+                double ss = sh[(q * WAVES) * 2], cc = sh[(q * WAVES) * 2 + 1];
+                for (int w = 1; w < WAVES; ++w) comp_merge(ss, cc, sh[(q * WAVES + w) * 2], sh[(q * WAVES + w) * 2 + 1]);
                 tot[q] = ss + cc;
             }
             const double a = s.a;
@@ -425,10 +449,9 @@ k_spectra(const cpx<T>* __restrict__ Pf, const cpx<T>* __restrict__ Pp,
 // member, the n partial vectors of width W are cut into `chunks` contiguous
 // ranges, each summed with compensation in a fixed order.
 template <int W>
-__global__ void __launch_bounds__(256)
-k_prereduce(const double* __restrict__ in, int n, double* __restrict__ out, int chunks) {
+__device__ void prereduce_chunk(const double* __restrict__ in, int n, double* __restrict__ out, int chunks,
+                                int member, int chunk) {
     __shared__ double sh[W * 2 * 4];
-    const int member = blockIdx.y, chunk = blockIdx.x;
     const long lo = (long)n * chunk / chunks, hi = (long)n * (chunk + 1) / chunks;
     Comp acc[W];
     for (long i = lo + threadIdx.x; i < hi; i += blockDim.x)
@@ -454,6 +477,20 @@ k_prereduce(const double* __restrict__ in, int n, double* __restrict__ out, int 
             for (int w = 1; w < 4; ++w) comp_merge(ss, cc, sh[(q * 4 + w) * 2], sh[(q * 4 + w) * 2 + 1]);
             out[((size_t)member * chunks + chunk) * W + q] = ss + cc;
         }
+}
+
+// Both first-level reductions of a substep in one launch: blocks
+// [0, chunks) cut the V partials (width 1), blocks [chunks, 2 chunks) the
+// K, G partials (width 2); a pass with few blocks (n <= chunks) is skipped.
+__global__ void __launch_bounds__(256)
+k_prereduce(const double* __restrict__ inV, int nV, double* __restrict__ outV, const double* __restrict__ inKG,
+            int nKG, double* __restrict__ outKG, int chunks) {
+    const int member = blockIdx.y;
+    if ((int)blockIdx.x < chunks) {
+        if (nV > chunks) prereduce_chunk<1>(inV, nV, outV, chunks, member, blockIdx.x);
+    } else if (nKG > chunks) {
+        prereduce_chunk<2>(inKG, nKG, outKG, chunks, member, blockIdx.x - chunks);
+    }
 }
 
 // Fixed-order sum of per-block partial vectors: out[q] = sum_b part[b][q].

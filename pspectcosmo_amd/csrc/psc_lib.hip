@@ -261,27 +261,28 @@ struct Engine final : EngineBase {
         });
     }
     // Reduce the substep partials to the scalar update.  Passes with more than
-    // kPreChunks blocks per member are first cut to kPreChunks compensated
+    // kScalarDirect blocks per member are first cut to kPreChunks compensated
     // chunk sums by many blocks, so the single-block update stays short.
     int launch_scalars(psc_ctx* c, ScalarOp op, MemberState* rec = nullptr) {
         const StepConsts sc = consts_of(c);
         const double* pv = c->partV;
         const double* pkg = c->partKG;
-        if (op.finish && op.nV > kPreChunks) {
-            hipLaunchKernelGGL(k_prereduce<1>, dim3(kPreChunks, c->B), dim3(256), 0, c->stream, c->partV, op.nV,
-                               c->partV2, kPreChunks);
-            pv = c->partV2;
-            op.nV = kPreChunks;
-        }
-        if (op.finish && op.nKG > kPreChunks) {
-            hipLaunchKernelGGL(k_prereduce<2>, dim3(kPreChunks, c->B), dim3(256), 0, c->stream, c->partKG, op.nKG,
-                               c->partKG2, kPreChunks);
-            pkg = c->partKG2;
-            op.nKG = kPreChunks;
+        if (op.finish && (op.nV > kScalarDirect || op.nKG > kScalarDirect)) {
+            hipLaunchKernelGGL(k_prereduce, dim3(2 * kPreChunks, c->B), dim3(256), 0, c->stream, c->partV, op.nV,
+                               c->partV2, c->partKG, op.nKG, c->partKG2, kPreChunks);
+            if (op.nV > kPreChunks) {
+                pv = c->partV2;
+                op.nV = kPreChunks;
+            }
+            if (op.nKG > kPreChunks) {
+                pkg = c->partKG2;
+                op.nKG = kPreChunks;
+            }
         }
         const double bytes = (double)c->B * ((double)(op.nV + 2 * op.nKG) * 8.0 + 2 * sizeof(MemberState));
         return timed(c, K_SCALARS, bytes, [&] {
-            hipLaunchKernelGGL(k_scalars, dim3(c->B), dim3(256), 0, c->stream, c->ms, pv, pkg, op, sc, rec);
+            hipLaunchKernelGGL(k_scalars, dim3(c->B), dim3(kScalarThreads), 0, c->stream, c->ms, pv, pkg, op, sc,
+                               rec);
         });
     }
 
