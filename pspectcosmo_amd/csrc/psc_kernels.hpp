@@ -56,6 +56,11 @@ k_z_inverse(const cpx<typename G::T>* __restrict__ src, cpx<typename G::T>* __re
 #pragma unroll
         for (int m = 0; m < G::E; ++m) e[m] = src[base + G::TPL * m];
     }
+    // The k = 0 mode (the homogeneous field, N^3 times larger than the
+    // fluctuations) stays out of the transforms: it is added back as an exact
+    // constant in the x pass, so FP32 rounding of the transform scales with the
+    // fluctuations, not with the mean.
+    if (line == 0 && t == 0) e[0] = cpx<T>{(T)0, (T)0};
     line_fft<T, N, typename G::Rad, +1, typename G::LayRow>(e, t, c, lds, tw);
 #pragma unroll
     for (int m = 0; m < G::E; ++m) W[base + G::TPL * m] = e[m];
@@ -90,7 +95,8 @@ k_strided(cpx<typename G::T>* __restrict__ W, const cpx<typename G::T>* __restri
 template <class G, bool REC = false>
 __global__ void __launch_bounds__((G::THREADS_COL))
 k_x_force(cpx<typename G::T>* __restrict__ W, const MemberState* __restrict__ ms, const cpx<typename G::T>* __restrict__ tw,
-          double* __restrict__ partV, cpx<typename G::T>* __restrict__ rec_r = nullptr) {
+          double* __restrict__ partV, const cpx<typename G::T>* __restrict__ Pf,
+          cpx<typename G::T>* __restrict__ rec_r = nullptr) {
     using T = typename G::T;
     constexpr int N = G::N;
     __shared__ cpx<T> lds[G::LayCol::size(0)];
@@ -107,14 +113,21 @@ k_x_force(cpx<typename G::T>* __restrict__ W, const MemberState* __restrict__ ms
     const T inv = (T)(1.0 / (double)G::N3);
     const T m2 = (T)ms[member].m2, g2 = (T)ms[member].g2;
     const T half = (T)0.5;
+    // homogeneous values from the k = 0 mode left out of the inverse transform
+    const cpx<T> dc = Pf[(size_t)member * G::MS];
+    const T pb = (T)((double)dc.x / (double)G::N3), qb = (T)((double)dc.y / (double)G::N3);
     double v[1] = {0.0};
 #pragma unroll
     for (int m = 0; m < G::E; ++m) {
-        const T phi = e[m].x * inv, chi = e[m].y * inv;
+        const T p = e[m].x * inv, q = e[m].y * inv;   // fluctuations
+        const T phi = pb + p, chi = qb + q;
         if constexpr (REC) rec_r[base + (size_t)G::TPL * m * stride] = cpx<T>{phi, chi};  // adjoint recording
         const T chi2 = chi * chi, phi2 = phi * phi;
-        e[m].x = m2 * phi + g2 * phi * chi2;
-        e[m].y = g2 * phi2 * chi;
+        // V'(phi, chi) - V'(pb, qb), written so that every term carries a
+        // fluctuation factor (no cancellation against the mean); the k = 0
+        // mode of V'(pb, qb) is added back in the z pass
+        e[m].x = m2 * p + g2 * (p * chi2 + pb * q * ((T)2 * qb + q));
+        e[m].y = g2 * (p * ((T)2 * pb + p) * chi + pb * pb * q);
         // V per site in the field precision, summed in FP64
         v[0] += (double)(half * m2 * phi2 + half * g2 * phi2 * chi2);
     }
@@ -124,6 +137,15 @@ k_x_force(cpx<typename G::T>* __restrict__ W, const MemberState* __restrict__ ms
     block_sum<1, G::THREADS_COL>(v, reinterpret_cast<double*>(lds));
     if (threadIdx.x == 0)
         partV[(size_t)member * G::BLOCKS_COL + blockIdx.y * gridDim.x + blockIdx.x] = v[0];
+}
+
+// The k = 0 mode of V' that the x pass leaves out: N^3 V'(pb, qb) with the
+// homogeneous values pb + i qb = f(k = 0) / N^3, computed in FP64.
+template <typename T>
+__device__ __forceinline__ void add_mean_force(cpx<T>& vk, const cpx<T>& f0, const MemberState& s, double N3) {
+    const double pb = (double)f0.x / N3, qb = (double)f0.y / N3;
+    vk.x += (T)(N3 * (s.m2 * pb + s.g2 * pb * qb * qb));
+    vk.y += (T)(N3 * (s.g2 * pb * pb * qb));
 }
 
 // ---------------------------------------------------------------------------
@@ -164,6 +186,7 @@ k_z_kick(const cpx<typename G::T>* __restrict__ W, const cpx<typename G::T>* __r
         const T n2 = (T)(pxy + pz * pz);
         const cpx<T> f = Pf[base + G::TPL * m];
         cpx<T> p = Pp[base + G::TPL * m];
+        if (m == 0 && line == 0 && t == 0) add_mean_force(e[0], f, ms[member], (double)G::N3);
         const T ln2 = lapc * n2;
         const cpx<T> F = {-ln2 * f.x - vc * e[m].x, -ln2 * f.y - vc * e[m].y};
         p.x += c1 * F.x;
@@ -311,7 +334,8 @@ k_scalars(MemberState* __restrict__ ms, const double* __restrict__ partV,
 template <class G>
 __global__ void __launch_bounds__((G::THREADS_COL))
 k_x_inverse_diag(cpx<typename G::T>* __restrict__ W, const MemberState* __restrict__ ms,
-                 const cpx<typename G::T>* __restrict__ tw, double* __restrict__ part) {
+                 const cpx<typename G::T>* __restrict__ tw, double* __restrict__ part,
+                 const cpx<typename G::T>* __restrict__ src) {
     using T = typename G::T;
     constexpr int N = G::N;
     __shared__ cpx<T> lds[G::LayCol::size(0)];
@@ -327,10 +351,12 @@ k_x_inverse_diag(cpx<typename G::T>* __restrict__ W, const MemberState* __restri
     line_fft<T, N, typename G::Rad, +1, typename G::LayCol>(e, t, c, lds, tw);
     const double inv = 1.0 / (double)G::N3;
     const double m2 = ms[member].m2, g2 = ms[member].g2;
+    const cpx<T> dc = src[(size_t)member * G::MS];   // k = 0 mode, left out of the z pass
+    const double pb = (double)dc.x * inv, qb = (double)dc.y * inv;
     double r[6] = {0, 0, 0, 0, 0, 0};
 #pragma unroll
     for (int m = 0; m < G::E; ++m) {
-        const double phi = (double)e[m].x * inv, chi = (double)e[m].y * inv;
+        const double phi = pb + (double)e[m].x * inv, chi = qb + (double)e[m].y * inv;
         e[m].x = (T)phi;
         e[m].y = (T)chi;
         r[0] += phi;
@@ -551,6 +577,7 @@ k_z_inverse_gw(cpx<typename G::T>* __restrict__ Pf, const cpx<typename G::T>* __
         f[m] = v;
         e[m] = v;
     }
+    if (line == 0 && t == 0) e[0] = cpx<T>{(T)0, (T)0};   // k = 0 mode added back in k_x_force
     line_fft<T, N, typename G::Rad, +1, typename G::LayRow>(e, t, c, lds, tw);
 #pragma unroll
     for (int m = 0; m < G::E; ++m) W[base + G::TPL * m] = e[m];
@@ -655,6 +682,7 @@ k_z_kick_gw(const cpx<typename G::T>* __restrict__ W, const cpx<typename G::T>* 
         const T n2 = (T)(pxy + pz * pz);
         const cpx<T> f = Pf[base + G::TPL * m];
         cpx<T> p = Pp[base + G::TPL * m];
+        if (m == 0 && line == 0 && t == 0) add_mean_force(e[0], f, ms[member], (double)G::N3);
         const T ln2 = lapc * n2;
         const cpx<T> F = {-ln2 * f.x - vc * e[m].x, -ln2 * f.y - vc * e[m].y};
         p.x += c1 * F.x;

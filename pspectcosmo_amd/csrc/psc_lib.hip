@@ -243,7 +243,7 @@ struct Engine final : EngineBase {
         dim3 grid(N / G::TC, N, c->B), block(G::THREADS_COL);
         return timed(c, K_X_FORCE, 2 * X, [&] {
             hipLaunchKernelGGL((k_x_force<G>), grid, block, 0, c->stream, P(c->W), c->ms, P(c->tw),
-                               c->partV);
+                               c->partV, P(c->Pf));
         });
     }
     int launch_z_kick(psc_ctx* c, bool reduce, double c1, double c2, int use_a_start) {
@@ -410,7 +410,7 @@ struct Engine final : EngineBase {
         dim3 grid(N / G::TC, N, 1), block(G::THREADS_COL);
         rc = timed(c, K_DIAG, 2.0 * G::N3 * sizeof(cpx<T>), [&] {
             hipLaunchKernelGGL((k_x_inverse_diag<G>), grid, block, 0, c->stream, W, c->ms + member,
-                               P(c->tw), c->part);
+                               P(c->tw), c->part, static_cast<const cpx<T>*>(src) + off);
         });
         if (rc) return rc;
         hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(64), 0, c->stream, c->part, G::BLOCKS_COL, 6,
@@ -520,7 +520,7 @@ struct Engine final : EngineBase {
                            P(c->W), c->ms, P(c->tw), nullptr, nullptr);
         if ((rc = launch_y<+1>(c, P(c->W), 1))) return rc;
         hipLaunchKernelGGL((k_x_force<G, true>), cgrid, cblock, 0, c->stream, P(c->W), c->ms, P(c->tw), c->partV,
-                           P(w.r[0]));
+                           P(c->Pf), P(w.r[0]));
         if ((rc = launch_y<-1>(c, P(c->W), 1))) return rc;
         hipLaunchKernelGGL((k_z_kick<G, false>), rgrid, rblock, 0, c->stream, P(c->W), P(c->Pf), P(c->Pp), c->ms,
                            P(c->tw), c->partKG, c->w[0] * dt / 2.0, 0.0, 1, sc, nullptr);
@@ -532,7 +532,7 @@ struct Engine final : EngineBase {
                                P(c->Pp), P(c->W), c->ms, P(c->tw), P(w.f[j]), P(w.ph[j]));
             if ((rc = launch_y<+1>(c, P(c->W), 1))) return rc;
             hipLaunchKernelGGL((k_x_force<G, true>), cgrid, cblock, 0, c->stream, P(c->W), c->ms, P(c->tw),
-                               c->partV, P(w.r[j + 1]));
+                               c->partV, P(c->Pf), P(w.r[j + 1]));
             if ((rc = launch_y<-1>(c, P(c->W), 1))) return rc;
             hipLaunchKernelGGL((k_z_kick<G, true, true>), rgrid, rblock, 0, c->stream, P(c->W), P(c->Pf),
                                P(c->Pp), c->ms, P(c->tw), c->partKG, c->w[p] * dt / 2.0, last ? 0.0 : wn * dt / 2.0,

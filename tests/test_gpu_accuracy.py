@@ -1,0 +1,100 @@
+"""Accuracy at the headline size (256^3, two fields, Yoshida-4) on the stable
+variant of the preheating lattice (the reference's dt rule; vacuum amplitude
+omegaStar/fStar = 1e-4 for the 10-step check, 1e-7 for the 1000-step run: at
+256^3 with kIR = 25 the vacuum energy of the modes up to k_max scales as
+(omegaStar k_max^2)^2 and exceeds the background's at 1e-4, so the
+background collapses; at 1e-7 it is 0.5 % of it and the run stays in the
+linear window):
+
+* the FP64 GPU path against the FP64 oracle after 10 steps;
+* the FP32-mixed path against the FP64 GPU path (the proxy oracle, pinned by
+  the first test) after 1000 steps: fields, momenta, per-bin spectra,
+  energies and the Friedmann-constraint violation of both paths.
+
+The measured errors are printed; the bounds are the stated targets."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from pspectcosmo_amd.abi import PSC_Y4, PSC_FP32, PSC_FP64
+from pspectcosmo_amd.lattice import Lattice, generate_ic
+from tests._util import parity_setup, rel_err, fluct_rel_err
+
+pytestmark = pytest.mark.gpu
+
+N = 256
+
+
+def setup(precision, omega_star=1e-4):
+    cfg, model, ic = parity_setup(N=N, integrator=PSC_Y4, precision=precision, use_graph=1,
+                                  omega_star=omega_star)
+    return cfg, model, ic
+
+
+def load(cfg, model, fk, pk, sc):
+    lat = Lattice(cfg)
+    lat.set_model(0, model)
+    lat.load_ic_k(0, fk, pk, sc[0], sc[1])
+    return lat
+
+
+def test_fp64_matches_oracle_at_256_after_10_steps():
+    cfg, model, ic = setup(PSC_FP64)
+    print("case fp64 vs oracle 256^3, 10 Y4 steps, dt", cfg.dt, flush=True)
+    fk, pk, sc = O.generate_ic(cfg, model, ic)
+    lat = load(cfg, model, fk, pk, sc)
+    lat.step(10)
+    gk, gp = lat.get_k()
+    ge = lat.energies()
+    gs = [lat.spectra(f) for f in range(2)]
+    gsc = lat.scalars()
+    lat.close()
+    o = O.Oracle(cfg, model)
+    o.load(fk, pk, 0.0, sc[0], sc[1], sc[4])
+    o.step(10)
+    ok, op = o.get_k()
+    ef, ep = rel_err(gk, ok), rel_err(gp, op)
+    efl = fluct_rel_err(gk, ok)
+    print(f"fp64 vs oracle: field {ef:.2e} (fluctuations {efl:.2e}), momentum {ep:.2e}")
+    assert ef <= 1e-10 and ep <= 1e-10 and efl <= 1e-10
+    np.testing.assert_allclose(gsc, o.scalars(), rtol=1e-12)
+    np.testing.assert_allclose(ge[:8], o.energies()[:8], rtol=1e-10)
+    for f in range(2):
+        np.testing.assert_allclose(gs[f], o.spectra(f), rtol=1e-9, atol=0)
+
+
+def test_fp32_mixed_against_fp64_over_1000_steps():
+    steps = 1000
+    cfg64, model, ic = setup(PSC_FP64, 1e-7)
+    cfg32, _, _ = setup(PSC_FP32, 1e-7)
+    print("case fp32 vs fp64 256^3, 1000 Y4 steps, dt", cfg64.dt, flush=True)
+    fk, pk, sc = generate_ic(cfg64, model, ic)
+    out = {}
+    for name, cfg in (("fp64", cfg64), ("fp32", cfg32)):
+        lat = load(cfg, model, fk, pk, sc)
+        e0 = lat.energies()
+        lat.step(steps)
+        out[name] = dict(k=lat.get_k(), e=lat.energies(), e0=e0, s=[lat.spectra(f) for f in range(2)],
+                         sc=lat.scalars())
+        lat.close()
+    a, b = out["fp32"], out["fp64"]
+    ef, ep = rel_err(a["k"][0], b["k"][0]), rel_err(a["k"][1], b["k"][1])
+    efl = fluct_rel_err(a["k"][0], b["k"][0])
+    per_field = [rel_err(a["k"][0][q], b["k"][0][q]) for q in range(2)]
+    per_field_fl = [fluct_rel_err(a["k"][0][q:q + 1], b["k"][0][q:q + 1]) for q in range(2)]
+    print(f"fp32 vs fp64 after {steps} steps: field {ef:.2e} (fluctuations {efl:.2e}; phi {per_field[0]:.2e}, "
+          f"chi {per_field[1]:.2e}; fluctuations phi {per_field_fl[0]:.2e}, chi {per_field_fl[1]:.2e}), "
+          f"momentum {ep:.2e}")
+    for f in range(2):
+        sa, sb = a["s"][f], b["s"][f]
+        w = sb[:, 1] > 0
+        rel = np.abs(sa[w, 1] - sb[w, 1]) / sb[w, 1]
+        relp = np.abs(sa[w, 2] - sb[w, 2]) / sb[w, 2]
+        print(f"spectrum field {f}: per-bin |f_k|^2 rel error max {rel.max():.2e} median {np.median(rel):.2e}; "
+              f"|pi_k|^2 max {relp.max():.2e}")
+        assert rel.max() <= 1e-3
+    print("energies fp32", a["e"][:8], "fp64", b["e"][:8])
+    print(f"constraint violation: fp64 {b['e'][7]:.3e} (t = 0: {b['e0'][7]:.3e}), fp32 {a['e'][7]:.3e}")
+    print(f"scale factor: fp64 a = {b['sc'][1]:.15f}, fp32 a = {a['sc'][1]:.15f}")
+    assert ef <= 1e-5 and ep <= 1e-5
+    np.testing.assert_allclose(a["e"][:7], b["e"][:7], rtol=1e-4)
