@@ -142,6 +142,12 @@ int psc_set_model(psc_ctx* ctx, int member, const psc_model* model);
  * initialize.cpp:797 does with kernel_a_k_nopad (evolution.cpp:1061-1090). */
 int psc_load_ic_k(psc_ctx* ctx, int member, const double* f_k, const double* pi_k,
                   double a, double ad);
+/* Load a member from real-space fields and momenta (nflds x N^3 each, the
+ * layout of pspectcosmo.h:63) and t, a, da/dt, as the continue_run branch
+ * of the reference reads its lattice image (initialize.cpp:590-616); the
+ * spectra are formed on the device and d2a/dt2 is recomputed. */
+int psc_load_fields(psc_ctx* ctx, int member, const double* f, const double* pi, double t, double a,
+                    double ad);
 /* Overwrite the background scalars t, a, da/dt, d2a/dt2 of one member. */
 int psc_set_scalars(psc_ctx* ctx, int member, double t, double a, double ad, double add);
 /* out[0..3] = t, a, da/dt, d2a/dt2 (pspectcosmo.h:44-45). */

@@ -66,6 +66,27 @@ k_z_inverse(const cpx<typename G::T>* __restrict__ src, cpx<typename G::T>* __re
     for (int m = 0; m < G::E; ++m) W[base + G::TPL * m] = e[m];
 }
 
+// Plain z-axis forward pass W -> dst (the last axis of a forward 3D transform
+// of real-space data, used when a state is loaded from real space).
+template <class G>
+__global__ void __launch_bounds__((G::THREADS_ROW))
+k_z_forward(const cpx<typename G::T>* __restrict__ W, cpx<typename G::T>* __restrict__ dst,
+            const cpx<typename G::T>* __restrict__ tw) {
+    using T = typename G::T;
+    constexpr int N = G::N;
+    __shared__ cpx<T> lds[G::LayRow::size(G::L)];
+    const int t = threadIdx.x % G::TPL, c = threadIdx.x / G::TPL;
+    const int member = blockIdx.y;
+    const size_t line = (size_t)blockIdx.x * G::L + c;
+    const size_t base = (size_t)member * G::MS + (line / N) * G::PS + (line % N) * N + t;
+    cpx<T> e[G::E];
+#pragma unroll
+    for (int m = 0; m < G::E; ++m) e[m] = W[base + G::TPL * m];
+    line_fft<T, N, typename G::Rad, -1, typename G::LayRow>(e, t, c, lds, tw);
+#pragma unroll
+    for (int m = 0; m < G::E; ++m) dst[base + G::TPL * m] = e[m];
+}
+
 // ---------------------------------------------------------------------------
 // B / D: strided-axis pass in place.  AXIS 1 = y (stride N), 0 = x (stride N^2).
 template <class G, int AXIS, int SIGN>

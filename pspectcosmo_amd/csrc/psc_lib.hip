@@ -207,6 +207,8 @@ struct EngineBase {
     virtual int gw_spectra(psc_ctx* c, int member, int projection, int nlog) = 0;
     // Discrete-adjoint gradient (see psc_adjoint).
     virtual int adjoint(psc_ctx* c, const psc_adjoint_spec* sp, double* out, double* dchi) = 0;
+    // Forward 3D transform of the packed real-space lattice in W[member] into dst[member].
+    virtual int to_spectrum(psc_ctx* c, int member, void* dst) = 0;
 };
 
 template <class G>
@@ -445,6 +447,19 @@ struct Engine final : EngineBase {
 
     int gw_spectra(psc_ctx* c, int member, int projection, int nlog) override;
     int adjoint(psc_ctx* c, const psc_adjoint_spec* sp, double* out, double* dchi) override;
+
+    int to_spectrum(psc_ctx* c, int member, void* dst) override {
+        const size_t off = (size_t)member * G::MS;
+        cpx<T>* W = P(c->W) + off;
+        dim3 cgrid(N / G::TC, N, 1), cblock(G::THREADS_COL);
+        hipLaunchKernelGGL((k_strided<G, 0, -1>), cgrid, cblock, 0, c->stream, W, P(c->tw));
+        int rc = launch_y<-1>(c, W, 1);
+        if (rc) return rc;
+        hipLaunchKernelGGL((k_z_forward<G>), dim3(G::BLOCKS_ROW, 1), dim3(G::THREADS_ROW), 0, c->stream, W,
+                           static_cast<cpx<T>*>(dst) + off, P(c->tw));
+        HIPCHK(c, hipGetLastError());
+        return PSC_OK;
+    }
 
     // ---- adjoint helpers --------------------------------------------------
     int adj_workspace(psc_ctx* c, int K, int nseg, AdjWork** out) {
@@ -1193,6 +1208,37 @@ int psc_load_ic_k(psc_ctx* c, int member, const double* f_k, const double* pi_k,
     Diag d;
     if ((rc = diagnose(c, member, s, &d))) return rc;
     s.add = kernel_a_value(c, d, a);  // initialize.cpp:797
+    if ((rc = write_state(c, member, &s))) return rc;
+    c->loaded[member] = 1;
+    return PSC_OK;
+}
+
+int psc_load_fields(psc_ctx* c, int member, const double* f, const double* pi, double t, double a, double ad) {
+    int rc = check_member(c, member);
+    if (rc) return rc;
+    if (!f || !pi || !(a > 0)) return c->fail(PSC_ERR_ARG, "bad lattice image");
+    const size_t n3 = c->N3;
+    std::vector<double> full(2 * n3);
+    for (int which = 0; which < 2; ++which) {
+        const double* src = which == 0 ? f : pi;
+        for (size_t i = 0; i < n3; ++i) {
+            full[2 * i] = src[i];
+            full[2 * i + 1] = c->cfg.nflds > 1 ? src[n3 + i] : 0.0;
+        }
+        if ((rc = upload(c, c->W, member, full))) return rc;
+        if ((rc = c->eng->to_spectrum(c, member, which == 0 ? c->Pf : c->Pp))) return rc;
+    }
+    MemberState s;
+    std::memset(&s, 0, sizeof(s));
+    s.t = t;
+    s.a = s.a_start = s.a_half = a;
+    s.ad = s.b = (c->cfg.expansion == PSC_EXPANSION_STATIC) ? 0.0 : ad;
+    s.m2 = c->models[member].m2;
+    s.g2 = c->models[member].g2;
+    if ((rc = write_state(c, member, &s))) return rc;
+    Diag d;
+    if ((rc = diagnose(c, member, s, &d))) return rc;
+    s.add = kernel_a_value(c, d, a);  // initialize.cpp:797, as for new initial data
     if ((rc = write_state(c, member, &s))) return rc;
     c->loaded[member] = 1;
     return PSC_OK;

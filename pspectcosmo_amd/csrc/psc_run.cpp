@@ -7,7 +7,10 @@
 //   average_spectra_times.txt, spectra_gws.txt, energy_gws.txt
 //                                            every `infreq` (save_Infreq, output.cpp:1842-1848)
 //   info.txt                                 at the end (output_parameters, output.cpp:1750)
-// and snapshots of the real-space fields every `snap` (save_Snap, output.cpp:1850-1853).
+// snapshots of the real-space fields every `snap` (save_Snap, output.cpp:1850-1853),
+// and the lattice image grid.img (checkpoint, output.cpp:1652-1703) with
+// every spectra output and at the end when checkpoint = 1; continue_run = 1
+// resumes from it (initialize.cpp:590-616).
 //
 // Parameters (names follow parameters.h) come from an optional file of
 // `key = value` lines and from `key=value` arguments; `psc_run --help` lists
@@ -92,6 +95,8 @@ const char* kKeys[][2] = {
     {"use_graph", "1: replay captured steps"},
     {"autotune", "1: time the launch shapes before the run"},
     {"dir", "output directory (default output)"},
+    {"checkpoint", "1: write <dir>/grid.img with the spectra and at the end"},
+    {"continue_run", "1: resume from <dir>/grid.img when it exists"},
 };
 
 void usage() {
@@ -283,6 +288,47 @@ struct Run {
         }
     }
 
+    // Lattice image in the reference's layout (checkpoint, output.cpp:1690-1697):
+    // int run_number, double t, a, da/dt, then f and pi, nflds x N^3 doubles each.
+    void write_grid(int run_number) {
+        const size_t n3 = (size_t)cfg.N * cfg.N * cfg.N;
+        std::vector<double> f(n3 * cfg.nflds), p(n3 * cfg.nflds);
+        check(psc_get_fields(ctx, 0, f.data(), p.data()), "psc_get_fields");
+        double sc[4];
+        check(psc_get_scalars(ctx, 0, sc), "psc_get_scalars");
+        const std::string name = dir + "/grid.img";
+        std::ofstream g(name + ".tmp", std::ios::binary);
+        g.write(reinterpret_cast<const char*>(&run_number), sizeof(run_number));
+        g.write(reinterpret_cast<const char*>(&sc[0]), sizeof(double));
+        g.write(reinterpret_cast<const char*>(&sc[1]), sizeof(double));
+        g.write(reinterpret_cast<const char*>(&sc[2]), sizeof(double));
+        g.write(reinterpret_cast<const char*>(f.data()), f.size() * sizeof(double));
+        g.write(reinterpret_cast<const char*>(p.data()), p.size() * sizeof(double));
+        g.close();
+        std::rename((name + ".tmp").c_str(), name.c_str());
+    }
+
+    // Reads grid.img into the context; returns false when there is none.
+    bool read_grid(int& run_number, double& t0) {
+        std::ifstream g(dir + "/grid.img", std::ios::binary);
+        if (!g) return false;
+        const size_t n3 = (size_t)cfg.N * cfg.N * cfg.N;
+        std::vector<double> f(n3 * cfg.nflds), p(n3 * cfg.nflds);
+        double a = 0, ad = 0;
+        g.read(reinterpret_cast<char*>(&run_number), sizeof(run_number));
+        g.read(reinterpret_cast<char*>(&t0), sizeof(double));
+        g.read(reinterpret_cast<char*>(&a), sizeof(double));
+        g.read(reinterpret_cast<char*>(&ad), sizeof(double));
+        g.read(reinterpret_cast<char*>(f.data()), f.size() * sizeof(double));
+        g.read(reinterpret_cast<char*>(p.data()), p.size() * sizeof(double));
+        if (!g) {
+            std::fprintf(stderr, "psc_run: %s/grid.img is truncated or from another lattice size\n", dir.c_str());
+            std::exit(2);
+        }
+        check(psc_load_fields(ctx, 0, f.data(), p.data(), t0, a, ad), "psc_load_fields");
+        return true;
+    }
+
     void write_info(double dt_last, long long steps) {
         std::ofstream info(path("info"), std::ios::out);
         const double L = 2 * M_PI / cfg.kIR;
@@ -396,13 +442,24 @@ int main(int argc, char** argv) {
         return 2;
     }
     // initial conditions (initialize(), initialize.cpp:624-800)
-    const size_t nk = (size_t)c.nflds * c.N * c.N * (c.N / 2 + 1) * 2;
+    const size_t nk = (size_t)c.nflds * c.N * c.N * (c.N / 2 + 1) * 2;   // FFTW half spectra
     std::vector<double> fk(nk), pk(nk);
     double sc0[4];
     r.check(psc_generate_ic(&c, &r.model, &r.ic, fk.data(), pk.data(), sc0), "psc_generate_ic");
     r.check(psc_create(&c, &r.ctx), "psc_create");
     r.check(psc_set_model(r.ctx, 0, &r.model), "psc_set_model");
-    r.check(psc_load_ic_k(r.ctx, 0, fk.data(), pk.data(), sc0[0], sc0[1]), "psc_load_ic_k");
+    const bool checkpointing = (int)p.num("checkpoint", 0) != 0;
+    int run_number = 0;
+    double t_start = 0.0;
+    bool resumed = false;
+    if ((int)p.num("continue_run", 0) != 0 && r.read_grid(run_number, t_start)) {
+        ++run_number;
+        resumed = true;
+        std::printf("Data read. Resuming run at t = %g\n", t_start);
+        if (c.with_gw) std::fprintf(stderr, "psc_run: grid.img holds no GW state; GW restart from vacuum\n");
+    } else {
+        r.check(psc_load_ic_k(r.ctx, 0, fk.data(), pk.data(), sc0[0], sc0[1]), "psc_load_ic_k");
+    }
     if (c.with_gw) {
         const size_t gk = (size_t)6 * c.N * c.N * (c.N / 2 + 1) * 2;
         std::vector<double> u(gk), pu(gk);
@@ -421,7 +478,12 @@ int main(int argc, char** argv) {
     double dt_last = c.dt;
     if (!r.auto_dt) {
         // fixed dt with a step-count cadence
-        long long done = 0;
+        long long done = resumed ? std::llround(t_start / c.dt) : 0;
+        if (done >= nsteps) {
+            std::fprintf(stderr, "psc_run: the lattice image is at step %lld of %lld; raise nsteps to continue\n",
+                         done, nsteps);
+            return 1;
+        }
         while (done < nsteps) {
             long long next = nsteps;
             auto upto = [&](long long every) {
@@ -435,6 +497,7 @@ int main(int argc, char** argv) {
             if (freq_steps > 0 && done % freq_steps == 0) r.save_freq();
             if (infreq_steps > 0 && done % infreq_steps == 0) {
                 r.save_infreq();
+                if (checkpointing) r.write_grid(run_number);
                 std::printf("Lattice Step %lld done. Current time: %g\n", done, done * c.dt);
                 std::fflush(stdout);
             }
@@ -443,8 +506,20 @@ int main(int argc, char** argv) {
         steps = done;
     } else {
         // the reference's loop (pspectcosmo.cpp:601-697)
-        double t = 0.0, a = r.ic.a0;
-        double next_freq = tOutFreq, next_infreq = tOutInfreq, next_snap = tSnap > 0 ? tSnap : 1e300;
+        double t = t_start, a = r.ic.a0;
+        if (resumed) {
+            double s0[4];
+            r.check(psc_get_scalars(r.ctx, 0, s0), "psc_get_scalars");
+            a = s0[1];
+            if (t >= tf) {
+                std::fprintf(stderr, "psc_run: the lattice image is at t = %g >= tf; raise tf to continue\n", t);
+                return 1;
+            }
+        }
+        // next output times after t (pspectcosmo.cpp:366-367 for a new run)
+        auto next_after = [&](double every) { return every * (std::floor(t / every + 1e-9) + 1); };
+        double next_freq = next_after(tOutFreq), next_infreq = next_after(tOutInfreq);
+        double next_snap = tSnap > 0 ? next_after(tSnap) : 1e300;
         while (t < tf) {
             double dt = std::min(calculate_dt(a), tOutFreq);
             if (t + dt > next_freq || t + dt > next_infreq) dt = std::min(next_freq - t, next_infreq - t);
@@ -463,6 +538,7 @@ int main(int argc, char** argv) {
             if (std::fabs(t - next_infreq) < 1e-10) {
                 next_infreq += tOutInfreq;
                 r.save_infreq();
+                if (checkpointing) r.write_grid(run_number);
                 std::printf("Lattice Step %lld done. Current time: %g\n", steps, t);
                 std::fflush(stdout);
             }
@@ -472,6 +548,7 @@ int main(int argc, char** argv) {
             }
         }
     }
+    if (checkpointing) r.write_grid(run_number);
     r.write_info(dt_last, steps);
     psc_destroy(r.ctx);
     std::printf("PSpectCosmo MI355X run finished: %lld steps\n", steps);

@@ -45,6 +45,7 @@ def lib():
         L.psc_last_error.argtypes = [vp]
         L.psc_set_model.argtypes = [vp, C.c_int, P(PscModel)]
         L.psc_load_ic_k.argtypes = [vp, C.c_int, _D, _D, C.c_double, C.c_double]
+        L.psc_load_fields.argtypes = [vp, C.c_int, _D, _D, C.c_double, C.c_double, C.c_double]
         L.psc_set_scalars.argtypes = [vp, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double]
         L.psc_get_scalars.argtypes = [vp, C.c_int, _D]
         L.psc_step.argtypes = [vp, C.c_int]
@@ -82,7 +83,7 @@ def exported_symbols():
             "psc_get_fields", "psc_energies", "psc_spectra", "psc_load_gw_k", "psc_get_gw_k",
             "psc_gw_spectrum", "psc_gw_omega_log", "psc_num_bins", "psc_set_kernel_timing", "psc_get_kernel_times",
             "psc_reset_kernel_times", "psc_device_bytes", "psc_autotune", "psc_shape_name",
-            "psc_list_shapes", "psc_set_shape", "psc_adjoint", "psc_set_dt", "psc_means"]
+            "psc_list_shapes", "psc_set_shape", "psc_adjoint", "psc_set_dt", "psc_means", "psc_load_fields"]
 
 
 def generate_ic(cfg, model, ic):
@@ -142,6 +143,11 @@ class Lattice:
         pk = np.ascontiguousarray(pk, dtype=np.complex128)
         self._check(lib().psc_load_ic_k(self._h, member, fk.view(np.float64).ravel(),
                                         pk.view(np.float64).ravel(), a, ad))
+
+    def load_fields(self, member, f, pi, t, a, ad):
+        f = np.ascontiguousarray(f, dtype=np.float64)
+        pi = np.ascontiguousarray(pi, dtype=np.float64)
+        self._check(lib().psc_load_fields(self._h, member, f.ravel(), pi.ravel(), t, a, ad))
 
     def set_scalars(self, member, t, a, ad, add):
         self._check(lib().psc_set_scalars(self._h, member, t, a, ad, add))

@@ -122,3 +122,21 @@ def test_adaptive_step_loop_matches_oracle(tmp_path):
         assert abs(row[0] - t) < 1e-3
         close(row[1:], e)
     assert f"Steps={steps}" in open(tmp_path / "info.txt").read()
+
+
+def test_checkpoint_and_continue_match_an_uninterrupted_run(tmp_path):
+    a_dir, b_dir = tmp_path / "a", tmp_path / "b"
+    a_dir.mkdir()
+    b_dir.mkdir()
+    common = ["dt=0.01", "freq_steps=5", "infreq_steps=5"]
+    run_driver(a_dir, *common, "nsteps=10")
+    run_driver(b_dir, *common, "nsteps=5", "checkpoint=1")
+    assert (b_dir / "grid.img").stat().st_size == 4 + 3 * 8 + 2 * 2 * 16 ** 3 * 8
+    r = run_driver(b_dir, *common, "nsteps=10", "checkpoint=1", "continue_run=1")
+    assert "Resuming run at t = 0.05" in r.stdout
+    ea, eb = rows(a_dir / "average_energies.txt"), rows(b_dir / "average_energies.txt")
+    assert len(ea) == len(eb) == 2
+    close(eb[1], ea[1], rtol=2e-6)
+    sa = [x for x in rows(a_dir / "spectra_scalar_1.txt") if x is not None]
+    sb = [x for x in rows(b_dir / "spectra_scalar_1.txt") if x is not None]
+    close(sb, sa, rtol=2e-6)
