@@ -61,6 +61,9 @@ def lib():
         L.orc_adjoint_chi2.argtypes = [P(PscConfig), P(PscModel), _D, _D, C.c_double, C.c_double,
                                        C.c_double, C.c_double, C.c_int, P(C.c_double), P(C.c_double),
                                        P(C.c_double), C.c_void_p]
+        L.orc_adjoint_gw.argtypes = [P(PscConfig), P(PscModel), _D, _D, _D, _D, C.c_double, C.c_double,
+                                     C.c_double, C.c_double, C.c_int, C.c_int, P(C.c_double), P(C.c_double),
+                                     P(C.c_double), C.c_void_p]
         _lib = L
     return _lib
 
@@ -137,6 +140,20 @@ def adjoint_chi2(cfg, model, fk, pk, a0, phi0, dphi0, chi0, nsteps, want_field=T
                                 pk.view(np.float64).ravel(), a0, phi0, dphi0, chi0, int(nsteps),
                                 C.byref(J), C.byref(g2b), C.byref(p0b),
                                 chib.ctypes.data_as(C.c_void_p) if want_field else None)
+    assert rc == 0, rc
+    return J.value, g2b.value, p0b.value, chib
+
+
+def adjoint_gw(cfg, model, fk, pk, u, pu, a0, phi0, dphi0, chi0, nsteps, projection, want_field=True):
+    """FP64 reference gradient of the final GW energy fraction rho_GW / rho:
+    (J, dJ/dg2, dJ/dphi0, dJ/dchi(x) or None)."""
+    arrs = [np.ascontiguousarray(x, dtype=np.complex128).view(np.float64).ravel() for x in (fk, pk, u, pu)]
+    J, g2b, p0b = C.c_double(), C.c_double(), C.c_double()
+    N = cfg.N
+    chib = np.zeros((N, N, N)) if want_field else None
+    rc = lib().orc_adjoint_gw(C.byref(cfg), C.byref(model), *arrs, a0, phi0, dphi0, chi0, int(nsteps),
+                              int(projection), C.byref(J), C.byref(g2b), C.byref(p0b),
+                              chib.ctypes.data_as(C.c_void_p) if want_field else None)
     assert rc == 0, rc
     return J.value, g2b.value, p0b.value, chib
 

@@ -9,6 +9,11 @@
 // sweeps back.  Pinned by central finite differences of the half-spectrum
 // forward restatement in psc_oracle.cpp (tests/test_oracle_adjoint.py).
 //
+// Objectives: <chi^2> at the end, and the GW energy fraction rho_GW / rho at
+// the end (output.cpp:1027-1060, either projection), with the six u_ij packed
+// as three full-spectrum pairs (u11 + i u22, u33 + i u12, u13 + i u23) and the
+// GW kick/drift of the spectral restatement (evolution.cpp:581-601, 633-685).
+//
 // Parameters: g^2; phi0 = the homogeneous inflaton value with the fluctuation
 // modes held fixed (it sets the DC mode phi0 N^3 and the initial expansion rate
 // da/dt = a0 H(phi0), initialize.cpp:649-668); the initial real-space chi
@@ -169,13 +174,91 @@ struct Adj {
         return cA(a) * ((al - 2) * e.K + al * e.G + (al + 1) * e.V);
     }
 
+    // ---- gravitational waves (packed pairs) --------------------------------
+    bool gw = false;
+    // derivative index with the Nyquist component zeroed (evolution.cpp:494-495)
+    int didx(int i) const { return i == N / 2 ? 0 : sidx(i, N); }
+    double kd(long idx, int d) const {
+        const int i = (int)(idx / ((long)N * N)), j = (int)((idx / N) % N), k = (int)(idx % N);
+        return cfg.kIR * didx(d == 0 ? i : (d == 1 ? j : k));
+    }
+    // packed real-space gradients G_d = F^H(i k_d f) / N^3 = d_d phi + i d_d chi
+    void gradients(const std::vector<cd>& f, std::vector<cd> G[3]) const {
+        for (int d = 0; d < 3; ++d) {
+            G[d].resize(n3);
+            for (long i = 0; i < n3; ++i) G[d][i] = cd(0, kd(i, d)) * f[i];
+            fft3(G[d], N, +1, f1);
+            for (auto& v : G[d]) v /= N3;
+        }
+    }
+    // s_ij = sum_f d_i f d_j f for a packed pair of gradients
+    static double sprod(cd gi, cd gj) { return gi.real() * gj.real() + gi.imag() * gj.imag(); }
+    // packed source S = (F(s11 + i s22), F(s33 + i s12), F(s13 + i s23))
+    void source(const std::vector<cd> G[3], std::vector<cd> S[3]) const {
+        for (int q = 0; q < 3; ++q) S[q].assign(n3, 0.0);
+        for (long x = 0; x < n3; ++x) {
+            const cd g0 = G[0][x], g1 = G[1][x], g2v = G[2][x];
+            S[0][x] = cd(sprod(g0, g0), sprod(g1, g1));
+            S[1][x] = cd(sprod(g2v, g2v), sprod(g0, g1));
+            S[2][x] = cd(sprod(g0, g2v), sprod(g1, g2v));
+        }
+        for (int q = 0; q < 3; ++q) fft3(S[q], N, -1, f1);
+    }
+    // PU += c a^{1+alpha} (-kIR^2 n^2 U + 2 S)
+    void gw_kick(std::vector<cd> PU[3], const std::vector<cd> U[3], const std::vector<cd> S[3], double a,
+                 double c) const {
+        const double norm = std::pow(a, 1 + cfg.alpha), k2 = pw2(cfg.kIR);
+        for (int q = 0; q < 3; ++q)
+            for (long i = 0; i < n3; ++i) PU[q][i] += c * norm * (-k2 * n2[i] * U[q][i] + 2.0 * S[q][i]);
+    }
+    // Adjoint of the GW kick force a^{1+alpha} (-kIR^2 n^2 u + 2 S(f)) at (f, a, u)
+    // applied to Pib: adds to ubar, fbar; returns the abar contribution.
+    double gw_force_adjoint(const std::vector<cd>& f, double a, const std::vector<cd> U[3],
+                            const std::vector<cd> Pib[3], std::vector<cd> ubar[3], std::vector<cd>& fbar) const {
+        const int al = cfg.alpha;
+        const double norm = std::pow(a, 1 + al), k2 = pw2(cfg.kIR);
+        double ru = 0.0;
+        std::vector<cd> sb[3];
+        for (int q = 0; q < 3; ++q) {
+            sb[q].resize(n3);
+            for (long i = 0; i < n3; ++i) {
+                ubar[q][i] += -norm * k2 * n2[i] * Pib[q][i];
+                ru += (std::conj(Pib[q][i]) * (-k2 * n2[i] * U[q][i])).real();
+                sb[q][i] = 2.0 * norm * Pib[q][i];
+            }
+            fft3(sb[q], N, +1, f1);  // sbar_p = F^H Sbar (adjoint of the forward transform)
+        }
+        std::vector<cd> G[3];
+        gradients(f, G);
+        double rs = 0.0;
+        std::vector<cd> gb[3];
+        for (int d = 0; d < 3; ++d) gb[d].assign(n3, 0.0);
+        for (long x = 0; x < n3; ++x) {
+            const double s11 = sb[0][x].real(), s22 = sb[0][x].imag(), s33 = sb[1][x].real();
+            const double s12 = sb[1][x].imag(), s13 = sb[2][x].real(), s23 = sb[2][x].imag();
+            const cd g0 = G[0][x], g1 = G[1][x], g2v = G[2][x];
+            rs += s11 * sprod(g0, g0) + s22 * sprod(g1, g1) + s33 * sprod(g2v, g2v) + s12 * sprod(g0, g1) +
+                  s13 * sprod(g0, g2v) + s23 * sprod(g1, g2v);
+            gb[0][x] = 2.0 * s11 * g0 + s12 * g1 + s13 * g2v;
+            gb[1][x] = 2.0 * s22 * g1 + s12 * g0 + s23 * g2v;
+            gb[2][x] = 2.0 * s33 * g2v + s13 * g0 + s23 * g1;
+        }
+        for (int d = 0; d < 3; ++d) {
+            fft3(gb[d], N, -1, f1);
+            for (long i = 0; i < n3; ++i) fbar[i] += cd(0, -kd(i, d)) * gb[d][i] / N3;
+        }
+        // d/da of a^{1+alpha} (...): (1+alpha)/a times the two inner products
+        return (1 + al) / a * (norm * ru + rs);
+    }
+
     struct Rec {
         std::vector<cd> f, p, ph;
+        std::vector<cd> u[3], puh[3];
         double a, b, add, ah, bh;
     };
     // One substep with weight wp; fills rec with its start state and pi_h.
     void substep(double wp, std::vector<cd>& f, std::vector<cd>& p, double& a, double& b, double& add,
-                 Rec& rec) const {
+                 Rec& rec, std::vector<cd>* U = nullptr, std::vector<cd>* PU = nullptr) const {
         const double h = wp * cfg.dt / 2.0;
         rec.f = f;
         rec.p = p;
@@ -186,12 +269,30 @@ struct Adj {
         std::vector<cd> F1 = force(f, a);
         std::vector<cd> ph(n3);
         for (long i = 0; i < n3; ++i) ph[i] = p[i] + h * F1[i];
+        if (U) {  // first GW kick at (f, a, u)
+            std::vector<cd> G[3], S[3];
+            gradients(f, G);
+            source(G, S);
+            for (int q = 0; q < 3; ++q) rec.u[q] = U[q];
+            gw_kick(PU, U, S, a, h);
+        }
         const double ah = stat ? 1.0 : a + bh * h;
         const double c = 2 * h * std::pow(ah, cfg.alpha - 3);
         for (long i = 0; i < n3; ++i) f[i] += c * ph[i];
+        if (U)
+            for (int q = 0; q < 3; ++q) {
+                rec.puh[q] = PU[q];
+                for (long i = 0; i < n3; ++i) U[q][i] += c * PU[q][i];
+            }
         a = stat ? 1.0 : ah + bh * h;
         std::vector<cd> F2 = force(f, a);
         for (long i = 0; i < n3; ++i) p[i] = ph[i] + h * F2[i];
+        if (U) {  // second GW kick at (f', a', u')
+            std::vector<cd> G[3], S[3];
+            gradients(f, G);
+            source(G, S);
+            gw_kick(PU, U, S, a, h);
+        }
         add = A(kgv(f, p, a), a);
         b = stat ? 0.0 : bh + h * add;
         rec.ph = std::move(ph);
@@ -275,47 +376,215 @@ void pack(int N, int nflds, const double* hk, std::vector<cd>& P) {
             }
 }
 
-}  // namespace
+// Mode visiting of the GW spectrum (output.cpp:800-1040): the half spectrum
+// kz <= N/2 without k = 0, isReal for the self-conjugate corners (weight 1).
+template <class Fn>
+void visit_modes_adj(int N, Fn&& fn) {
+    for (int k = 0; k <= N / 2; k += N / 2) {
+        for (int i = 0; i <= N / 2; i += N / 2)
+            for (int j = 1; j < N / 2; j++) fn(i, j, k, false);
+        for (int j = 0; j <= N / 2; j += N / 2)
+            for (int i = 1; i < N / 2; i++) fn(i, j, k, false);
+        for (int i = 1; i < N / 2; i++) {
+            for (int j = 1; j < N / 2; j++) fn(i, j, k, false);
+            for (int j = N / 2 + 1; j < N; j++) fn(i, j, k, false);
+        }
+    }
+    for (int k = 1; k < N / 2; k++)
+        for (int i = 0; i < N; i++)
+            for (int j = 0; j < N; j++) fn(i, j, k, false);
+    for (int k = 0; k <= N / 2; k += N / 2)
+        for (int i = 0; i <= N / 2; i += N / 2)
+            for (int j = 0; j <= N / 2; j += N / 2) {
+                if (i == 0 && j == 0 && k == 0) continue;
+                fn(i, j, k, true);
+            }
+}
 
-extern "C" {
+// GW energy fraction at the end and its gradient with respect to the final
+// pi_u pairs (packed full spectra), following orc_gw_spectrum: the reference's
+// half-spectrum visiting with weights (output.cpp:800-1040), the k = 0 term
+// without projection (output.cpp:1032-1038) and
+// rho_GW = (f*/M_P)^2 / (4 N^6 a^6) * sum (output.cpp:1056-1058).
+struct GwObjective {
+    double Q = 0.0;                 // weighted sum of the projected |pi_u|^2
+    std::vector<cd> grad[3];        // dQ / d(pi_u pairs)
+};
 
-// Gradient of J = <chi^2> at the end of nsteps steps (objective 0).  Inputs:
-// initial half spectra, a0, the homogeneous phi0, dphi0, chi0 used for
-// H(phi0).  Outputs: J, dJ/dg2, dJ/dphi0, dJ/dchi(x) (N^3, may be NULL).
-int orc_adjoint_chi2(const psc_config* cfg, const psc_model* model, const double* f_k, const double* pi_k,
-                     double a0, double phi0, double dphi0, double chi0, int nsteps, double* J,
-                     double* g2bar_out, double* phi0bar_out, double* chibar_out) {
+GwObjective gw_objective(const Adj& s, const std::vector<cd> PU[3], int projection) {
+    const int N = s.N;
+    const long n3 = s.n3;
+    GwObjective o;
+    for (int q = 0; q < 3; ++q) o.grad[q].assign(n3, 0.0);
+    auto full = [&](int i, int j, int k) { return ((long)((i % N + N) % N) * N + (j % N + N) % N) * N + (k % N + N) % N; };
+    // component c of the pair layout: pair c / 2, real part (even c) or imaginary part (odd c)
+    static const int pairI[6] = {0, 0, 1, 1, 2, 2};
+    static const int compOf[6] = {0, 1, 2, 3, 4, 5};   // 11, 22, 33, 12, 13, 23 in pair order
+    (void)compOf;
+    static const int ci[6] = {0, 1, 2, 0, 0, 1}, cj[6] = {0, 1, 2, 1, 2, 2};
+    auto unpack = [&](int c, long k, long mk) {
+        const cd a = PU[pairI[c]][k], b = std::conj(PU[pairI[c]][mk]);
+        return (c % 2 == 0) ? 0.5 * (a + b) : cd(0, -0.5) * (a - b);
+    };
+    auto scatter = [&](int c, long k, long mk, cd g) {   // chain rule of unpack
+        if (c % 2 == 0) {
+            o.grad[pairI[c]][k] += 0.5 * g;
+            o.grad[pairI[c]][mk] += 0.5 * std::conj(g);
+        } else {
+            o.grad[pairI[c]][k] += cd(0, 0.5) * g;
+            o.grad[pairI[c]][mk] += cd(0, 0.5) * std::conj(g);
+        }
+    };
+    // pair order of the components: (11, 22), (33, 12), (13, 23)
+    static const int pc_i[6] = {0, 1, 2, 0, 0, 1}, pc_j[6] = {0, 1, 2, 1, 2, 2};
+    (void)ci;
+    (void)cj;
+    auto mode = [&](int i, int j, int k, double wgt) {
+        const long kk = full(i, j, k), mk = full(-i, -j, -k);
+        const int px = sidx(i, N), py = sidx(j, N);
+        const double kv[3] = {(double)px, (double)py, (double)k};
+        const double kn = std::sqrt(kv[0] * kv[0] + kv[1] * kv[1] + kv[2] * kv[2]);
+        double P[3][3];
+        for (int x = 0; x < 3; ++x)
+            for (int y = 0; y < 3; ++y) P[x][y] = (x == y ? 1.0 : 0.0) - kv[x] / kn * kv[y] / kn;
+        cd M[3][3];
+        for (int c = 0; c < 6; ++c) M[pc_i[c]][pc_j[c]] = M[pc_j[c]][pc_i[c]] = unpack(c, kk, mk);
+        cd PM[3][3];
+        for (int x = 0; x < 3; ++x)
+            for (int y = 0; y < 3; ++y) {
+                PM[x][y] = 0;
+                for (int z = 0; z < 3; ++z) PM[x][y] += P[x][z] * M[z][y];
+            }
+        const cd tr = PM[0][0] + PM[1][1] + PM[2][2];
+        cd Gm[3][3];   // dq/dM (all nine entries independent)
+        double q = 0.0;
+        if (projection == PSC_GW_PROJ_TT) {
+            cd A[3][3];
+            for (int x = 0; x < 3; ++x)
+                for (int y = 0; y < 3; ++y) {
+                    A[x][y] = 0;
+                    for (int z = 0; z < 3; ++z) A[x][y] += PM[x][z] * P[z][y];
+                    A[x][y] -= 0.5 * P[x][y] * tr;
+                    q += std::norm(A[x][y]);
+                }
+            for (int x = 0; x < 3; ++x)   // 2 L^H(A) = 2 (P A P - tr(P A) P / 2)
+                for (int y = 0; y < 3; ++y) {
+                    cd v = 0, trPA = 0;
+                    for (int z = 0; z < 3; ++z)
+                        for (int w = 0; w < 3; ++w) v += P[x][z] * A[z][w] * P[w][y];
+                    for (int z = 0; z < 3; ++z)
+                        for (int w = 0; w < 3; ++w) trPA += P[z][w] * A[w][z];
+                    Gm[x][y] = 2.0 * (v - 0.5 * trPA * P[x][y]);
+                }
+        } else {
+            for (int x = 0; x < 3; ++x)
+                for (int y = 0; y < 3; ++y) q += std::norm(PM[x][y]);
+            q -= 0.5 * std::norm(tr);
+            for (int x = 0; x < 3; ++x)   // 2 P (P M) - tr(P M) P
+                for (int y = 0; y < 3; ++y) {
+                    cd v = 0;
+                    for (int z = 0; z < 3; ++z) v += P[x][z] * PM[z][y];
+                    Gm[x][y] = 2.0 * v - tr * P[x][y];
+                }
+        }
+        o.Q += wgt * q;
+        for (int c = 0; c < 6; ++c) {
+            const int x = pc_i[c], y = pc_j[c];
+            const cd g = (x == y) ? Gm[x][x] : Gm[x][y] + Gm[y][x];
+            scatter(c, kk, mk, wgt * g);
+        }
+    };
+    visit_modes_adj(N, [&](int i, int j, int k, bool isReal) { mode(i, j, k, isReal ? 1.0 : 2.0); });
+    // k = 0 term: sum over the nine entries of |pi_u(0)|^2 without projection
+    for (int c = 0; c < 6; ++c) {
+        const double wgt = (pc_i[c] == pc_j[c]) ? 1.0 : 2.0;
+        const cd v = unpack(c, 0, 0);
+        o.Q += wgt * std::norm(v);
+        scatter(c, 0, 0, wgt * 2.0 * v);
+    }
+    return o;
+}
+
+// Shared forward/backward of the adjoint.  objective 0: <chi^2>; 1: rho_GW / rho.
+int run_adjoint(const psc_config* cfg, const psc_model* model, const double* f_k, const double* pi_k,
+                const double* u_k, const double* pi_u_k, double a0, double phi0, double dphi0, double chi0,
+                int nsteps, int objective, int projection, double* J, double* g2bar_out, double* phi0bar_out,
+                double* chibar_out) {
     if (cfg->nflds != 2) return PSC_ERR_ARG;
     Adj s(*cfg, model->m2, model->g2);
+    s.gw = objective == 1;
     const long n3 = s.n3;
     const double N3 = s.N3;
+    const int N = cfg->N;
+    const long nkh = (long)N * N * (N / 2 + 1);
     std::vector<cd> f, p;
     pack(cfg->N, 2, f_k, f);
     pack(cfg->N, 2, pi_k, p);
+    std::vector<cd> U[3], PU[3];
+    if (s.gw)
+        for (int q = 0; q < 3; ++q) {
+            pack(N, 2, u_k + (size_t)2 * q * nkh * 2, U[q]);
+            pack(N, 2, pi_u_k + (size_t)2 * q * nkh * 2, PU[q]);
+        }
     // background at t = 0 (initialize.cpp:649-668) and kernel_a on the state
     const double V0 = 0.5 * s.m2 * phi0 * phi0 + 0.5 * s.g2 * phi0 * phi0 * chi0 * chi0;
     const double f2m = pw2(cfg->fStar / cfg->MPl);
     const double H = std::sqrt((dphi0 * dphi0 / 2.0 + V0) * f2m / 3.0);
     double a = a0, b = s.stat ? 0.0 : H * a0;
     const std::vector<cd> f0 = f, p0 = p;
+    std::vector<cd> U0[3];
+    for (int q = 0; q < 3; ++q) U0[q] = U[q];
     double add = s.A(s.kgv(f, p, a), a);
     std::vector<Adj::Rec> recs;
     const int S = (int)s.w.size();
     recs.resize((size_t)nsteps * S);
     for (int st = 0; st < nsteps; ++st)
-        for (int q = 0; q < S; ++q) s.substep(s.w[q], f, p, a, b, add, recs[(size_t)st * S + q]);
-    // objective and its adjoint at the end
-    std::vector<cd> r = s.real_space(f);
-    double jj = 0;
-    for (long i = 0; i < n3; ++i) jj += pw2(r[i].imag());
-    jj /= N3;
-    std::vector<cd> fbar(n3), pbar(n3, 0.0), Fpend(n3, 0.0);
-    for (long i = 0; i < n3; ++i) fbar[i] = cd(0, 2.0 * r[i].imag() / N3);
-    fft3(fbar, cfg->N, -1, s.f1);
-    for (auto& v : fbar) v /= N3;
+        for (int q = 0; q < S; ++q)
+            s.substep(s.w[q], f, p, a, b, add, recs[(size_t)st * S + q], s.gw ? U : nullptr, s.gw ? PU : nullptr);
+    std::vector<cd> fbar(n3, 0.0), pbar(n3, 0.0), Fpend(n3, 0.0);
+    std::vector<cd> ubar[3], pubar[3], Fpend_u[3];
+    for (int q = 0; q < 3; ++q) {
+        ubar[q].assign(n3, 0.0);
+        pubar[q].assign(n3, 0.0);
+        Fpend_u[q].assign(n3, 0.0);
+    }
     double abar = 0, bbar = 0, addbar = 0, g2bar = 0;
+    std::vector<cd> r = s.real_space(f);
+    if (objective == 0) {
+        double jj = 0;
+        for (long i = 0; i < n3; ++i) jj += pw2(r[i].imag());
+        *J = jj / N3;
+        for (long i = 0; i < n3; ++i) fbar[i] = cd(0, 2.0 * r[i].imag() / N3);
+        fft3(fbar, cfg->N, -1, s.f1);
+        for (auto& v : fbar) v /= N3;
+    } else {
+        // J = rho_GW / rho with rho = K + G + V (energy_density, output.cpp:57-115)
+        const Adj::KGV e = s.kgv(f, p, a);
+        const double rho = e.K + e.G + e.V;
+        const GwObjective o = gw_objective(s, PU, projection);
+        const double cgw = f2m / (4.0 * N3 * N3 * std::pow(a, 6));
+        const double rgw = cgw * o.Q;
+        *J = rgw / rho;
+        for (int q = 0; q < 3; ++q)
+            for (long i = 0; i < n3; ++i) pubar[q][i] = cgw / rho * o.grad[q][i];
+        // -J / rho times the gradient of rho
+        const double c = -(*J) / rho;
+        for (long i = 0; i < n3; ++i) {
+            pbar[i] += c * p[i] / (std::pow(a, 6) * N3 * N3);
+            fbar[i] += c * pw2(cfg->kIR) * s.n2[i] * f[i] / (a * a * N3 * N3);
+        }
+        std::vector<cd> q(n3);
+        for (long i = 0; i < n3; ++i) q[i] = c / N3 * s.vprime(r[i]);
+        fft3(q, N, -1, s.f1);
+        for (long i = 0; i < n3; ++i) fbar[i] += q[i] / N3;
+        g2bar += c * e.Vg;
+        abar += -6.0 * (*J) / a + c * (-6.0 * e.K / a - 2.0 * e.G / a);
+        if (s.stat) abar = 0.0;
+    }
     // end state of the substep being undone
     std::vector<cd> fe = f, pe = p;
+    std::vector<cd> ue[3];
+    for (int q = 0; q < 3; ++q) ue[q] = U[q];
     double ae = a;
     for (long sidx_ = (long)recs.size() - 1; sidx_ >= 0; --sidx_) {
         const Adj::Rec& R = recs[sidx_];
@@ -330,17 +599,31 @@ int orc_adjoint_chi2(const psc_config* cfg, const psc_model* model, const double
         std::vector<cd> Fb(n3);
         for (long i = 0; i < n3; ++i) Fb[i] = h * pbar[i] + Fpend[i];
         s.force_adjoint(fe, ae, Fb, &rV, fbar, abar, g2bar);
+        if (s.gw) {  // pi_u' = pi_uh + h GW force at (f', a', u'), merged with the next first kick
+            std::vector<cd> Pib[3];
+            for (int q = 0; q < 3; ++q) {
+                Pib[q].resize(n3);
+                for (long i = 0; i < n3; ++i) Pib[q][i] = h * pubar[q][i] + Fpend_u[q][i];
+            }
+            abar += s.gw_force_adjoint(fe, ae, ue, Pib, ubar, fbar);
+        }
         if (s.stat) abar = 0;
         // a' = ah + bh h
         double ahbar = abar;
         bhbar += h * abar;
-        // f' = f + 2h ah^{al-3} ph
+        // f' = f + 2h ah^{al-3} ph  (and u' = u + 2h ah^{al-3} pi_uh)
         const double c = 2 * h * std::pow(R.ah, cfg->alpha - 3);
         double dot = 0;
         for (long i = 0; i < n3; ++i) {
             phbar[i] += c * fbar[i];
             dot += (std::conj(fbar[i]) * R.ph[i]).real();
         }
+        if (s.gw)
+            for (int q = 0; q < 3; ++q)
+                for (long i = 0; i < n3; ++i) {
+                    pubar[q][i] += c * ubar[q][i];
+                    dot += (std::conj(ubar[q][i]) * R.puh[q][i]).real();
+                }
         ahbar += 2 * h * (cfg->alpha - 3) * std::pow(R.ah, cfg->alpha - 4) * dot;
         // ah = a + bh h
         abar = s.stat ? 0.0 : ahbar;
@@ -348,17 +631,22 @@ int orc_adjoint_chi2(const psc_config* cfg, const psc_model* model, const double
         // ph = p + h F1 (F1 deferred to the start point)
         pbar = phbar;
         for (long i = 0; i < n3; ++i) Fpend[i] = h * phbar[i];
+        if (s.gw)
+            for (int q = 0; q < 3; ++q)
+                for (long i = 0; i < n3; ++i) Fpend_u[q][i] = h * pubar[q][i];
         // bh = b + h add
         bbar = s.stat ? 0.0 : bhbar;
         addbar = s.stat ? 0.0 : h * bhbar;
         fe = R.f;
         pe = R.p;
+        if (s.gw)
+            for (int q = 0; q < 3; ++q) ue[q] = R.u[q];
         ae = R.a;
     }
-    // t = 0: the pending force and add0 = A(f0, p0, a0)
+    // t = 0: the pending forces and add0 = A(f0, p0, a0)
     std::vector<cd> rV0 = s.A_adjoint(f0, p0, a0, addbar, fbar, pbar, abar, g2bar);
     s.force_adjoint(f0, a0, Fpend, &rV0, fbar, abar, g2bar);
-    *J = jj;
+    if (s.gw) abar += s.gw_force_adjoint(f0, a0, U0, Fpend_u, ubar, fbar);
     *g2bar_out = g2bar;
     // phi0: DC mode phi0 N^3 and da/dt(0) = a0 H(phi0), dH/dphi0 = f2m V'(phi0) / (6 H)
     const double vphi = s.m2 * phi0 + s.g2 * phi0 * chi0 * chi0;
@@ -370,6 +658,31 @@ int orc_adjoint_chi2(const psc_config* cfg, const psc_model* model, const double
     }
     (void)pbar;
     return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+// Gradient of J = <chi^2> at the end of nsteps steps.  Inputs: initial half
+// spectra, a0, the homogeneous phi0, dphi0, chi0 used for H(phi0).  Outputs:
+// J, dJ/dg2, dJ/dphi0, dJ/dchi(x) (N^3, may be NULL).
+int orc_adjoint_chi2(const psc_config* cfg, const psc_model* model, const double* f_k, const double* pi_k,
+                     double a0, double phi0, double dphi0, double chi0, int nsteps, double* J,
+                     double* g2bar_out, double* phi0bar_out, double* chibar_out) {
+    return run_adjoint(cfg, model, f_k, pi_k, nullptr, nullptr, a0, phi0, dphi0, chi0, nsteps, 0, 0, J, g2bar_out,
+                       phi0bar_out, chibar_out);
+}
+
+// Gradient of the final GW energy fraction rho_GW / rho (projection
+// PSC_GW_PROJ_REFERENCE or PSC_GW_PROJ_TT).  u_k, pi_u_k: the six GW
+// components as FFTW half spectra (11, 22, 33, 12, 13, 23).
+int orc_adjoint_gw(const psc_config* cfg, const psc_model* model, const double* f_k, const double* pi_k,
+                   const double* u_k, const double* pi_u_k, double a0, double phi0, double dphi0, double chi0,
+                   int nsteps, int projection, double* J, double* g2bar_out, double* phi0bar_out,
+                   double* chibar_out) {
+    return run_adjoint(cfg, model, f_k, pi_k, u_k, pi_u_k, a0, phi0, dphi0, chi0, nsteps, 1, projection, J,
+                       g2bar_out, phi0bar_out, chibar_out);
 }
 
 }  // extern "C"

@@ -76,3 +76,64 @@ def test_adjoint_matches_central_differences(integrator, expansion):
               forward_J(cfg, model, fk, pk, PHI0, steps, site, -eps)) / (2 * eps)
         print("chi", site, dchi[site], fd)
         assert abs(dchi[site] - fd) <= 1e-6 * abs(fd) + 1e-13
+
+
+# ---- GW energy fraction ------------------------------------------------------
+
+def forward_gw(cfg, model, fk, pk, u, pu, phi0, steps, projection, dchi_site=None, eps_chi=0.0):
+    """rho_GW / rho after `steps` steps of the half-spectrum oracle with GW."""
+    N = cfg.N
+    fk = fk.copy()
+    fk[0, 0, 0, 0] = phi0 * N ** 3
+    if dchi_site is not None:
+        x = np.zeros((N, N, N))
+        x[dchi_site] = eps_chi
+        fk[1] += O.fft_r2c_3d(x)
+    V0 = 0.5 * model.m2 * phi0 ** 2
+    H = math.sqrt((DPHI0 ** 2 / 2 + V0) / 3.0)
+    stat = cfg.expansion == PSC_EXPANSION_STATIC
+    o = O.Oracle(cfg, model)
+    o.load(fk, pk, 0.0, 1.0, 0.0 if stat else H, 0.0)
+    add0 = o.kernel_a()
+    o.load(fk, pk, 0.0, 1.0, 0.0 if stat else H, add0)
+    o.load_gw(u, pu)
+    o.step(steps)
+    _, tot = o.gw_spectrum(projection)
+    return float(tot[0])
+
+
+def setup_gw(integrator, expansion, N=8, g2=2.0e3):
+    cfg, model, ic = parity_setup(N=N, nflds=2, integrator=integrator, expansion=expansion,
+                                  kIR=4.0, dt=0.01, g2=g2, omega_star=2e-2, f0=(PHI0, 0.0),
+                                  fd0=(DPHI0, 0.0), yoshida_exact=1, with_gw=1)
+    fk, pk, sc = O.generate_ic(cfg, model, ic)
+    u, pu = O.generate_gw_ic(cfg, model, ic)
+    return cfg, model, fk, pk, u, pu
+
+
+@pytest.mark.parametrize("integrator,expansion,projection", [(PSC_VV2, PSC_EXPANSION_SELF, 1),
+                                                             (PSC_Y4, PSC_EXPANSION_SELF, 0),
+                                                             (PSC_VV2, PSC_EXPANSION_STATIC, 1)])
+def test_gw_fraction_adjoint_matches_central_differences(integrator, expansion, projection):
+    steps = 5
+    cfg, model, fk, pk, u, pu = setup_gw(integrator, expansion)
+    J, dg2, dphi0, dchi = O.adjoint_gw(cfg, model, fk, pk, u, pu, 1.0, PHI0, DPHI0, 0.0, steps, projection)
+    J0 = forward_gw(cfg, model, fk, pk, u, pu, PHI0, steps, projection)
+    print("J", J, J0)
+    assert abs(J - J0) <= 1e-12 * abs(J0)
+    eps = 1e-5 * model.g2
+    fd = (forward_gw(cfg, make_model(model.m2, model.g2 + eps), fk, pk, u, pu, PHI0, steps, projection) -
+          forward_gw(cfg, make_model(model.m2, model.g2 - eps), fk, pk, u, pu, PHI0, steps, projection)) / (2 * eps)
+    print("g2", dg2, fd)
+    assert abs(dg2 - fd) <= 1e-6 * abs(fd) + 1e-18
+    eps = 1e-5 * PHI0
+    fd = (forward_gw(cfg, model, fk, pk, u, pu, PHI0 + eps, steps, projection) -
+          forward_gw(cfg, model, fk, pk, u, pu, PHI0 - eps, steps, projection)) / (2 * eps)
+    print("phi0", dphi0, fd)
+    assert abs(dphi0 - fd) <= 1e-6 * abs(fd) + 1e-18
+    for site in [(0, 0, 0), (3, 5, 1), (7, 2, 6)]:
+        eps = 1e-6
+        fd = (forward_gw(cfg, model, fk, pk, u, pu, PHI0, steps, projection, site, eps) -
+              forward_gw(cfg, model, fk, pk, u, pu, PHI0, steps, projection, site, -eps)) / (2 * eps)
+        print("chi", site, dchi[site], fd)
+        assert abs(dchi[site] - fd) <= 1e-6 * abs(fd) + 1e-15
