@@ -208,31 +208,37 @@ int psc_num_bins(int N);
 /* ---- adjoint (no counterpart in the reference) -------------------------- */
 
 #define PSC_OBJ_CHI2 0          /* <chi^2> over the lattice at the end of the run */
+#define PSC_OBJ_GW_FRACTION 1   /* rho_GW / rho at the end (the totals of psc_gw_spectrum,
+                                   output.cpp:1054-1060); needs with_gw              */
 
 typedef struct psc_adjoint_spec {
-    int objective;          /* PSC_OBJ_CHI2                                          */
+    int objective;          /* PSC_OBJ_CHI2 or PSC_OBJ_GW_FRACTION                   */
     int nsteps;             /* forward steps of cfg->dt from the loaded state        */
     int checkpoint_every;   /* steps between stored states; each segment is replayed
                                once with recording before it is swept backwards     */
     int member;             /* 0 (single-lattice contexts only)                      */
+    int projection;         /* GW objective: PSC_GW_PROJ_REFERENCE or PSC_GW_PROJ_TT */
+    int reserved_i;
     double phi0, dphi0, chi0;  /* homogeneous initial values; da/dt(0) = a0 H(phi0)
                                   as initialize.cpp:649-668                         */
-    double reserved[5];
+    double reserved[4];
 } psc_adjoint_spec;
 
 /* Gradient of the objective by the discrete adjoint of the step (reverse
- * kick-drift over the recorded substeps, Friedmann coupling included), with
- * respect to g2, phi0 (the homogeneous mode and H(phi0), fluctuation modes
- * held fixed) and the initial real-space chi field.  The run is split into
- * segments of checkpoint_every steps: a forward pass stores each segment's
- * start state, then from the last segment backwards the segment is replayed
- * with recording and swept back.  The loaded state is left unchanged.
- * out[0..5] = J, dJ/dg2, dJ/dphi0, forward milliseconds, backward milliseconds
- * (replays and sweeps), device bytes of the adjoint workspace.  dchi receives
- * N^3 values dJ/dchi(x) when not NULL.  The workspace (checkpoints, one
- * recorded segment, adjoint arrays) stays allocated for further calls until
- * psc_destroy, psc_set_shape or psc_autotune.  Requires nflds == 2, batch == 1
- * and no GW evolution. */
+ * kick-drift over the recorded substeps, Friedmann coupling included; with the
+ * GW objective also the GW kick/drift and the source adjoint back to the
+ * fields), with respect to g2, phi0 (the homogeneous mode and H(phi0),
+ * fluctuation modes held fixed) and the initial real-space chi field.  The
+ * run is split into segments of checkpoint_every steps: a forward pass stores
+ * each segment's start state, then from the last segment backwards the segment
+ * is replayed with recording and swept back.  The loaded state is left
+ * unchanged.  out[0..5] = J, dJ/dg2, dJ/dphi0, forward milliseconds, backward
+ * milliseconds (replays and sweeps), device bytes of the adjoint workspace.
+ * dchi receives N^3 values dJ/dchi(x) when not NULL.  The workspace
+ * (checkpoints, one recorded segment, adjoint arrays) stays allocated for
+ * further calls until psc_destroy, psc_set_shape or psc_autotune.  Requires
+ * nflds == 2 and batch == 1; the GW objective requires GW evolution and loaded
+ * GW data.  No counterpart in the reference. */
 int psc_adjoint(psc_ctx* ctx, const psc_adjoint_spec* spec, double* out, double* dchi);
 
 /* ---- launch-shape autotuner (no counterpart in the reference) ----------- */

@@ -23,6 +23,7 @@ PSC_ENERGY_LEN = 16
 PSC_SPECTRA_COLS = 5
 PSC_GW_SPECTRA_COLS = 5
 PSC_OBJ_CHI2 = 0
+PSC_OBJ_GW_FRACTION = 1
 
 
 class PscConfig(C.Structure):
@@ -50,7 +51,8 @@ class PscIcParams(C.Structure):
 class PscAdjointSpec(C.Structure):
     _fields_ = [
         ("objective", C.c_int), ("nsteps", C.c_int), ("checkpoint_every", C.c_int), ("member", C.c_int),
-        ("phi0", C.c_double), ("dphi0", C.c_double), ("chi0", C.c_double), ("reserved", C.c_double * 5),
+        ("projection", C.c_int), ("reserved_i", C.c_int),
+        ("phi0", C.c_double), ("dphi0", C.c_double), ("chi0", C.c_double), ("reserved", C.c_double * 4),
     ]
 
 

@@ -20,6 +20,15 @@
 //   P5 z-axis forward -> fbar += q_k / N^3; the drift adjoint
 //      pibar_h = pibar + 2 h a_h^{al-3} fbar.
 // The scalar recursion (abar, bbar, addbar, g2bar) runs in k_adj_scalars.
+//
+// With the GW-energy-fraction objective the six u_ij (three packed pairs U_q)
+// carry adjoints ubar_q, pibar_u,q.  The GW kick pi_u += h a^{1+al} (-kIR^2 n^2 u
+// + 2 S(f)) at the end point (merged with the next substep's first kick, as for
+// the fields) gives ubar += -a^{1+al} kIR^2 n^2 Pib and Sbar = 2 a^{1+al} Pib with
+// Pib = (h + h_next) pibar_u; the source adjoint reaches the fields through the
+// gradients, fbar += sum_d (-i kIR n_d) F(gbar_d) / N^3 with
+// gbar_d = sum_e w_de sbar_de G_e (w_dd = 2), in three more passes (P1g, P3g,
+// P5g) that run beside P1, P3 and P5.
 #pragma once
 #include "psc_kernels.hpp"
 
@@ -29,6 +38,7 @@ namespace psc {
 struct AdjCoef {
     double h;    // w dt / 2
     double hn;   // w dt / 2 of the substep undone before (0 at the end of the run)
+    double cs;   // 2 a'^{1+al}                         (GW source)
     double ck;   // addbar cA (al - 2) / (a'^6 N^6)    (K part of the add adjoint)
     double cg;   // addbar cA al kIR^2 / (a'^2 N^6)     (G part)
     double c1;   // a'^{1+al} kIR^2
@@ -177,22 +187,192 @@ k_adj_p5(const cpx<typename G::T>* __restrict__ W, cpx<typename G::T>* __restric
     if (threadIdx.x == 0) part[blockIdx.x] = red[0];
 }
 
-// Compensated single-block sums of the five adjoint reductions: R1 (P1
-// partials), R2..R4 (P3 partials, width 3), R5 (P5 partials, when r5).
-__device__ void adj_block_reduce(const double* __restrict__ p1, int n1, const double* __restrict__ p3, int n3,
-                                 const double* __restrict__ p5, int n5, bool r5, double* sh, double* R) {
-    Comp acc[5];
-    for (int i = threadIdx.x; i < n1; i += blockDim.x) acc[0].add(p1[i]);
+// P1 (GW): Pib = (h + h_next) pibar_u; ubar += -a^{1+al} kIR^2 n^2 Pib; the
+// source adjoint Sbar = 2 a^{1+al} Pib goes through the z-axis inverse into
+// W[1..3]; reduction sum n^2 Re(conj(Pib) u_e) for abar.
+template <class G>
+__global__ void __launch_bounds__((G::THREADS_ROW))
+k_adj_p1g(const cpx<typename G::T>* __restrict__ pubar, cpx<typename G::T>* __restrict__ ubar,
+          const cpx<typename G::T>* __restrict__ ue, cpx<typename G::T>* __restrict__ W, size_t astride,
+          const AdjCoef* __restrict__ co, const cpx<typename G::T>* __restrict__ tw, double* __restrict__ part) {
+    using T = typename G::T;
+    constexpr int N = G::N;
+    __shared__ cpx<T> lds[G::LayRow::size(G::L)];
+    const int t = threadIdx.x % G::TPL, c = threadIdx.x / G::TPL;
+    const size_t line = (size_t)blockIdx.x * G::L + c;
+    const size_t base = (line / N) * G::PS + (line % N) * N + t;
+    const T hh = (T)(co->h + co->hn), c1 = (T)co->c1, cs = (T)co->cs;
+    const int px = signed_index((int)(line / N), N), py = signed_index((int)(line % N), N);
+    const int pxy = px * px + py * py;
+    double red[1] = {0.0};
+#pragma unroll 1
+    for (int q = 0; q < 3; ++q) {
+        cpx<T> e[G::E];
+#pragma unroll
+        for (int m = 0; m < G::E; ++m) {
+            const size_t idx = (size_t)q * astride + base + G::TPL * m;
+            const int pz = signed_index(t + G::TPL * m, N);
+            const T n2 = (T)(pxy + pz * pz);
+            const cpx<T> pb = pubar[idx], u = ue[idx];
+            const cpx<T> Pib = {hh * pb.x, hh * pb.y};
+            cpx<T> ub = ubar[idx];
+            ub.x -= c1 * n2 * Pib.x;
+            ub.y -= c1 * n2 * Pib.y;
+            ubar[idx] = ub;
+            red[0] += (double)n2 * ((double)Pib.x * u.x + (double)Pib.y * u.y);
+            e[m] = cpx<T>{cs * Pib.x, cs * Pib.y};
+        }
+        line_fft<T, N, typename G::Rad, +1, typename G::LayRow>(e, t, c, lds, tw);
+#pragma unroll
+        for (int m = 0; m < G::E; ++m) W[(size_t)(q + 1) * astride + base + G::TPL * m] = e[m];
+    }
+    block_sum<1, G::THREADS_ROW>(red, reinterpret_cast<double*>(lds));
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+// P3 (GW): x-axis inverse of sbar (three packed pairs), the gradient adjoint
+// gbar_d = sum_e w_de sbar_de G_e with the recorded real-space gradients G_e,
+// x-axis forward of gbar; reduction sum sbar . s for abar.
+template <class G>
+__global__ void __launch_bounds__((G::THREADS_COL))
+k_adj_p3g(cpx<typename G::T>* __restrict__ W, const cpx<typename G::T>* __restrict__ ge, size_t astride,
+          const cpx<typename G::T>* __restrict__ tw, double* __restrict__ part) {
+    using T = typename G::T;
+    constexpr int N = G::N;
+    __shared__ cpx<T> lds[G::LayCol::size(0)];
+    const int c = threadIdx.x % G::TC, t = threadIdx.x / G::TC;
+    const size_t z = (size_t)blockIdx.x * G::TC + c;
+    const size_t o = blockIdx.y;
+    constexpr size_t stride = G::PS;
+    const size_t base = o * N + z + t * stride;
+    cpx<T> e[3][G::E];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+#pragma unroll
+        for (int m = 0; m < G::E; ++m) e[q][m] = W[(size_t)q * astride + base + (size_t)G::TPL * m * stride];
+        line_fft<T, N, typename G::Rad, +1, typename G::LayCol>(e[q], t, c, lds, tw);
+    }
+    double red[1] = {0.0};
+#pragma unroll
+    for (int m = 0; m < G::E; ++m) {
+        const size_t idx = base + (size_t)G::TPL * m * stride;
+        const cpx<T> g0 = ge[idx], g1 = ge[astride + idx], g2 = ge[2 * astride + idx];
+        const T s11 = e[0][m].x, s22 = e[0][m].y, s33 = e[1][m].x, s12 = e[1][m].y, s13 = e[2][m].x,
+                s23 = e[2][m].y;
+        const T p00 = g0.x * g0.x + g0.y * g0.y, p11 = g1.x * g1.x + g1.y * g1.y, p22 = g2.x * g2.x + g2.y * g2.y;
+        const T p01 = g0.x * g1.x + g0.y * g1.y, p02 = g0.x * g2.x + g0.y * g2.y, p12 = g1.x * g2.x + g1.y * g2.y;
+        red[0] += (double)s11 * p00 + (double)s22 * p11 + (double)s33 * p22 + (double)s12 * p01 +
+                  (double)s13 * p02 + (double)s23 * p12;
+        const T two = (T)2;
+        e[0][m] = cpx<T>{two * s11 * g0.x + s12 * g1.x + s13 * g2.x, two * s11 * g0.y + s12 * g1.y + s13 * g2.y};
+        e[1][m] = cpx<T>{two * s22 * g1.x + s12 * g0.x + s23 * g2.x, two * s22 * g1.y + s12 * g0.y + s23 * g2.y};
+        e[2][m] = cpx<T>{two * s33 * g2.x + s13 * g0.x + s23 * g1.x, two * s33 * g2.y + s13 * g0.y + s23 * g1.y};
+    }
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        line_fft<T, N, typename G::Rad, -1, typename G::LayCol>(e[q], t, c, lds, tw);
+#pragma unroll
+        for (int m = 0; m < G::E; ++m) W[(size_t)q * astride + base + (size_t)G::TPL * m * stride] = e[q][m];
+    }
+    block_sum<1, G::THREADS_COL>(red, reinterpret_cast<double*>(lds));
+    if (threadIdx.x == 0) part[(size_t)blockIdx.y * gridDim.x + blockIdx.x] = red[0];
+}
+
+// P5 (GW): z-axis forward of gbar_d, fbar += sum_d (-i kIR n_d) gbar_d / N^3
+// (Nyquist component zeroed, evolution.cpp:494-495), then the drift adjoint of
+// u: pibar_u += 2 h a_h^{al-3} ubar; reduction sum Re(conj(ubar) pi_u half).
+template <class G>
+__global__ void __launch_bounds__((G::THREADS_ROW))
+k_adj_p5g(const cpx<typename G::T>* __restrict__ W, cpx<typename G::T>* __restrict__ fbar,
+          const cpx<typename G::T>* __restrict__ ubar, cpx<typename G::T>* __restrict__ pubar,
+          const cpx<typename G::T>* __restrict__ puh, size_t astride, const AdjCoef* __restrict__ co, double kIR,
+          const cpx<typename G::T>* __restrict__ tw, double* __restrict__ part) {
+    using T = typename G::T;
+    constexpr int N = G::N;
+    __shared__ cpx<T> lds[G::LayRow::size(G::L)];
+    const int t = threadIdx.x % G::TPL, c = threadIdx.x / G::TPL;
+    const size_t line = (size_t)blockIdx.x * G::L + c;
+    const size_t base = (line / N) * G::PS + (line % N) * N + t;
+    const T kx = (T)(kIR * deriv_index((int)(line / N), N)), ky = (T)(kIR * deriv_index((int)(line % N), N));
+    const T inv = (T)(1.0 / (double)G::N3);
+    cpx<T> acc[G::E];
+#pragma unroll
+    for (int m = 0; m < G::E; ++m) acc[m] = cpx<T>{(T)0, (T)0};
+#pragma unroll 1
+    for (int d = 0; d < 3; ++d) {
+        cpx<T> e[G::E];
+#pragma unroll
+        for (int m = 0; m < G::E; ++m) e[m] = W[(size_t)d * astride + base + G::TPL * m];
+        line_fft<T, N, typename G::Rad, -1, typename G::LayRow>(e, t, c, lds, tw);
+#pragma unroll
+        for (int m = 0; m < G::E; ++m) {
+            const T kd = d == 0 ? kx : (d == 1 ? ky : (T)(kIR * deriv_index(t + G::TPL * m, N)));
+            acc[m].x += kd * e[m].y;   // (-i kd) (x + i y) = kd y - i kd x
+            acc[m].y -= kd * e[m].x;
+        }
+    }
+#pragma unroll
+    for (int m = 0; m < G::E; ++m) {
+        const size_t idx = base + G::TPL * m;
+        cpx<T> fb = fbar[idx];
+        fb.x += inv * acc[m].x;
+        fb.y += inv * acc[m].y;
+        fbar[idx] = fb;
+    }
+    double red[1] = {0.0};
+    if (!co->final_pass) {
+        const T cd = (T)co->cd;
+#pragma unroll 1
+        for (int q = 0; q < 3; ++q)
+#pragma unroll
+            for (int m = 0; m < G::E; ++m) {
+                const size_t idx = (size_t)q * astride + base + G::TPL * m;
+                const cpx<T> ub = ubar[idx], ph = puh[idx];
+                cpx<T> pb = pubar[idx];
+                pb.x += cd * ub.x;
+                pb.y += cd * ub.y;
+                pubar[idx] = pb;
+                red[0] += (double)ub.x * ph.x + (double)ub.y * ph.y;
+            }
+    }
+    block_sum<1, G::THREADS_ROW>(red, reinterpret_cast<double*>(lds));
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+// Compensated single-block sums of the adjoint reductions: R0 (P1 partials),
+// R1..R3 (P3 partials, width 3), R4 (P5 partials, when r5) and with GW
+// R5 (P1g), R6 (P3g), R7 (P5g, when r5).
+struct AdjPartials {
+    const double* p1;
+    const double* p3;
+    const double* p5;
+    const double* p1g;
+    const double* p3g;
+    const double* p5g;
+};
+
+__device__ void adj_block_reduce(const AdjPartials& P, int n1, int n3, int n5, bool r5, bool gw, double* sh,
+                                 double* R) {
+    constexpr int NR = 8;
+    Comp acc[NR];
+    for (int i = threadIdx.x; i < n1; i += blockDim.x) {
+        acc[0].add(P.p1[i]);
+        if (gw) acc[5].add(P.p1g[i]);
+    }
     for (int i = threadIdx.x; i < n3; i += blockDim.x) {
-        acc[1].add(p3[3 * (size_t)i]);
-        acc[2].add(p3[3 * (size_t)i + 1]);
-        acc[3].add(p3[3 * (size_t)i + 2]);
+        acc[1].add(P.p3[3 * (size_t)i]);
+        acc[2].add(P.p3[3 * (size_t)i + 1]);
+        acc[3].add(P.p3[3 * (size_t)i + 2]);
+        if (gw) acc[6].add(P.p3g[i]);
     }
     if (r5)
-        for (int i = threadIdx.x; i < n5; i += blockDim.x) acc[4].add(p5[i]);
+        for (int i = threadIdx.x; i < n5; i += blockDim.x) {
+            acc[4].add(P.p5[i]);
+            if (gw) acc[7].add(P.p5g[i]);
+        }
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
-    for (int q = 0; q < 5; ++q) {
+    for (int q = 0; q < NR; ++q) {
 #pragma unroll
         for (int off = 32; off >= 1; off >>= 1) {
             const double s2 = __shfl_xor(acc[q].s, off, 64), c2 = __shfl_xor(acc[q].c, off, 64);
@@ -205,7 +385,7 @@ __device__ void adj_block_reduce(const double* __restrict__ p1, int n1, const do
     }
     __syncthreads();
     if (threadIdx.x == 0)
-        for (int q = 0; q < 5; ++q) {
+        for (int q = 0; q < NR; ++q) {
             double ss = sh[(q * 4) * 2], cc = sh[(q * 4) * 2 + 1];
             for (int w = 1; w < 4; ++w) comp_merge(ss, cc, sh[(q * 4 + w) * 2], sh[(q * 4 + w) * 2 + 1]);
             R[q] = ss + cc;
@@ -225,6 +405,8 @@ struct AdjScalarOp {
     int static_bg;
     int alpha;
     int end_of_segment;
+    int gw;                      // GW adjoint passes ran (partials P1g, P3g, P5g)
+    int pad3;
     double h_pre;                // w dt / 2 of the substep to undo next
     double a_e, K_e, G_e, V_e;   // t = 0 pass: forward values at t = 0
     double kIR, f2m, N3;
@@ -232,21 +414,20 @@ struct AdjScalarOp {
 };
 
 __global__ void __launch_bounds__(256)
-k_adj_scalars(AdjState* __restrict__ st, AdjCoef* __restrict__ co, const double* __restrict__ p1,
-              const double* __restrict__ p3, const double* __restrict__ p5, const MemberState* __restrict__ me,
-              const MemberState* __restrict__ mh, AdjScalarOp op) {
-    __shared__ double sh[5 * 4 * 2];
+k_adj_scalars(AdjState* __restrict__ st, AdjCoef* __restrict__ co, AdjPartials parts,
+              const MemberState* __restrict__ me, const MemberState* __restrict__ mh, AdjScalarOp op) {
+    __shared__ double sh[8 * 4 * 2];
     const int al = op.alpha;
-    double R[5] = {0, 0, 0, 0, 0};
+    double R[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     // states loaded up front (in flight during the reduction), written back once
     AdjState S = *st;
     const MemberState E = *me;
     const double ah_rec = mh->a_half;
     __syncthreads();   // every thread has read the state before thread 0 rewrites it
     if (!S.pending) op.mode &= ~1;
-    if (op.mode & 1) adj_block_reduce(p1, op.n1, p3, op.n3, p5, op.n5, !(op.mode & 4), sh, R);
+    if (op.mode & 1) adj_block_reduce(parts, op.n1, op.n3, op.n5, !(op.mode & 4), op.gw != 0, sh, R);
     if (threadIdx.x != 0) return;
-    const double R1 = R[0], R2 = R[1], R3 = R[2], R4 = R[3], R5 = R[4];
+    const double R1 = R[0], R2 = R[1], R3 = R[2], R4 = R[3], R5 = R[4] + R[7];
     if (op.mode & 1) {
         const double a = S.a_e;
         const double cA = 1.0 / 3.0 * op.f2m * ipow(a, 1 + 2 * al);
@@ -256,6 +437,8 @@ k_adj_scalars(AdjState* __restrict__ st, AdjCoef* __restrict__ co, const double*
             S.abar = S.bbar = S.addbar = 0.0;
         } else {
             S.abar += R1 * (-(1 + al) * ipow(a, al) * op.kIR * op.kIR) + (3 + al) / a * R2;
+            // GW kick force a^{1+al} (-kIR^2 n^2 u + 2 S): d/da over the two inner products
+            if (op.gw) S.abar += (1 + al) / a * (ipow(a, 1 + al) * (-op.kIR * op.kIR) * R[5] + R[6]);
             if (!(op.mode & 4)) {
                 const double h = S.h_undo, ah = S.ah_undo;
                 double ahbar = S.abar;                            // a' = ah + bh h
@@ -297,6 +480,7 @@ k_adj_scalars(AdjState* __restrict__ st, AdjCoef* __restrict__ co, const double*
         C.cg = addbar * cA * al * op.kIR * op.kIR / (a * a) / (op.N3 * op.N3);
         C.c1 = ipow(a, 1 + al) * op.kIR * op.kIR;
         C.c3 = ipow(a, 3 + al);
+        C.cs = 2.0 * ipow(a, 1 + al);
         C.cV = addbar * cA * (al + 1) / op.N3;
         C.cd = fin ? 0.0 : 2.0 * h * ipow(a_h_pre, al - 3);
         C.final_pass = fin ? 1 : 0;

@@ -569,11 +569,13 @@ __device__ __forceinline__ int deriv_index(int i, int N) { return i == N / 2 ? 0
 
 // A (GW): drift of fields and u, then z-axis inverse of the fields and of the
 // three gradient spectra i kIR n_d P_f.
-template <class G, bool DRIFT>
+template <class G, bool DRIFT, bool REC = false>
 __global__ void __launch_bounds__((G::THREADS_ROW))
 k_z_inverse_gw(cpx<typename G::T>* __restrict__ Pf, const cpx<typename G::T>* __restrict__ Pp, cpx<typename G::T>* __restrict__ W,
                cpx<typename G::T>* __restrict__ U, const cpx<typename G::T>* __restrict__ PU, const MemberState* __restrict__ ms,
-               const cpx<typename G::T>* __restrict__ tw, size_t astride, double kIR) {
+               const cpx<typename G::T>* __restrict__ tw, size_t astride, double kIR,
+               cpx<typename G::T>* __restrict__ rec_f = nullptr, cpx<typename G::T>* __restrict__ rec_ph = nullptr,
+               cpx<typename G::T>* __restrict__ rec_u = nullptr, cpx<typename G::T>* __restrict__ rec_puh = nullptr) {
     using T = typename G::T;
     constexpr int N = G::N;
     __shared__ cpx<T> lds[G::LayRow::size(G::L)];
@@ -591,6 +593,10 @@ k_z_inverse_gw(cpx<typename G::T>* __restrict__ Pf, const cpx<typename G::T>* __
         cpx<T> v = Pf[base + G::TPL * m];
         if constexpr (DRIFT) {
             const cpx<T> p = Pp[base + G::TPL * m];
+            if constexpr (REC) {  // adjoint recording: substep start f and pi_half
+                rec_f[base + G::TPL * m] = v;
+                rec_ph[base + G::TPL * m] = p;
+            }
             v.x += cd * p.x;
             v.y += cd * p.y;
             Pf[base + G::TPL * m] = v;
@@ -625,6 +631,10 @@ k_z_inverse_gw(cpx<typename G::T>* __restrict__ Pf, const cpx<typename G::T>* __
                 u.x += cd * p.x;
                 u.y += cd * p.y;
                 U[idx] = u;
+                if constexpr (REC) {  // adjoint recording: u after the drift, pi_u half
+                    rec_u[idx] = u;
+                    rec_puh[idx] = p;
+                }
             }
     }
 }
@@ -633,9 +643,10 @@ k_z_inverse_gw(cpx<typename G::T>* __restrict__ Pf, const cpx<typename G::T>* __
 // gradient arrays through a plain x-axis inverse pass, form the packed source
 // S = (S11 + i S22, S33 + i S12, S13 + i S23) from the real-space gradients
 // and run the three x-axis forward transforms.
-template <class G>
+template <class G, bool REC = false>
 __global__ void __launch_bounds__((G::THREADS_COL))
-k_x_source(cpx<typename G::T>* __restrict__ W, const cpx<typename G::T>* __restrict__ tw, size_t astride) {
+k_x_source(cpx<typename G::T>* __restrict__ W, const cpx<typename G::T>* __restrict__ tw, size_t astride,
+           cpx<typename G::T>* __restrict__ rec_g = nullptr) {
     using T = typename G::T;
     constexpr int N = G::N;
     __shared__ cpx<T> lds[G::LayCol::size(0)];
@@ -656,6 +667,12 @@ k_x_source(cpx<typename G::T>* __restrict__ W, const cpx<typename G::T>* __restr
         const T ax = g[0][m].x * inv, bx = g[0][m].y * inv;
         const T ay = g[1][m].x * inv, by = g[1][m].y * inv;
         const T az = g[2][m].x * inv, bz = g[2][m].y * inv;
+        if constexpr (REC) {  // adjoint recording: real-space gradients d_d phi + i d_d chi
+            const size_t idx = base + (size_t)G::TPL * m * stride;
+            rec_g[idx] = cpx<T>{ax, bx};
+            rec_g[astride + idx] = cpx<T>{ay, by};
+            rec_g[2 * astride + idx] = cpx<T>{az, bz};
+        }
         g[0][m] = cpx<T>{ax * ax + bx * bx, ay * ay + by * by};
         g[1][m] = cpx<T>{az * az + bz * bz, ax * ay + bx * by};
         g[2][m] = cpx<T>{ax * az + bx * bz, ay * az + by * bz};
@@ -670,12 +687,12 @@ k_x_source(cpx<typename G::T>* __restrict__ W, const cpx<typename G::T>* __restr
 
 // E (GW): z-axis forward of V' and of the source, field kick as k_z_kick and
 // the GW kick of the three packed u arrays.
-template <class G, bool REDUCE>
+template <class G, bool REDUCE, bool REC = false>
 __global__ void __launch_bounds__((G::THREADS_ROW))
 k_z_kick_gw(const cpx<typename G::T>* __restrict__ W, const cpx<typename G::T>* __restrict__ Pf, cpx<typename G::T>* __restrict__ Pp,
             const cpx<typename G::T>* __restrict__ U, cpx<typename G::T>* __restrict__ PU, const MemberState* __restrict__ ms,
             const cpx<typename G::T>* __restrict__ tw, double* __restrict__ partKG, double c1d, double c2d,
-            int use_a_start, StepConsts sc, size_t astride) {
+            int use_a_start, StepConsts sc, size_t astride, cpx<typename G::T>* __restrict__ rec_p = nullptr) {
     using T = typename G::T;
     constexpr int N = G::N;
     __shared__ cpx<T> lds[G::LayRow::size(G::L)];
@@ -708,6 +725,7 @@ k_z_kick_gw(const cpx<typename G::T>* __restrict__ W, const cpx<typename G::T>* 
         const cpx<T> F = {-ln2 * f.x - vc * e[m].x, -ln2 * f.y - vc * e[m].y};
         p.x += c1 * F.x;
         p.y += c1 * F.y;
+        if constexpr (REC) rec_p[base + G::TPL * m] = p;  // adjoint recording: synchronised pi
         if constexpr (REDUCE) {
             const double px_ = p.x, py_ = p.y, fx = f.x, fy = f.y;
             red[0] += px_ * px_ + py_ * py_;

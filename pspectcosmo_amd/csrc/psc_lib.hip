@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <complex>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -67,6 +68,7 @@ struct EngineBase;
 // the adjoint arrays and, when steps are graph-captured, the captured segment.
 struct AdjWork {
     int K = 0, S = 0, nseg = 0;
+    bool gw = false;                  // GW objective: u, pi_u carried
     std::vector<void*> ck;            // f, pi per segment
     MemberState* ck_ms = nullptr;
     std::vector<void*> f, p, r, ph;
@@ -78,6 +80,14 @@ struct AdjWork {
     double* p1 = nullptr;
     double* p3 = nullptr;
     double* p5 = nullptr;
+    // GW: three packed arrays per entry, MS elements apart
+    std::vector<void*> ck_u, ck_pu;   // per segment
+    std::vector<void*> u, g, puh;     // u and real-space gradients per boundary, pi_u half per substep
+    void* ubar = nullptr;
+    void* pubar = nullptr;
+    double* p1g = nullptr;
+    double* p3g = nullptr;
+    double* p5g = nullptr;
     hipGraphExec_t g_rec = nullptr, g_bwd = nullptr;   // full segment: recording replay, sweep
     double bytes = 0.0;
     std::vector<void*> owned;
@@ -113,6 +123,7 @@ struct psc_ctx {
     std::vector<double> w;
     std::vector<psc_model> models;
     std::vector<char> loaded;
+    int loaded_gw = 0;
     std::string err;
     EngineBase* eng = nullptr;
     int timing = 0;
@@ -462,10 +473,10 @@ struct Engine final : EngineBase {
     }
 
     // ---- adjoint helpers --------------------------------------------------
-    int adj_workspace(psc_ctx* c, int K, int nseg, AdjWork** out) {
+    int adj_workspace(psc_ctx* c, int K, int nseg, bool gw, AdjWork** out) {
         const int S = (int)c->w.size();
         AdjWork* cur = c->adj.get();
-        if (cur && cur->K == K && cur->S == S && cur->nseg >= nseg) {
+        if (cur && cur->K == K && cur->S == S && cur->nseg >= nseg && cur->gw == gw) {
             *out = cur;
             return PSC_OK;
         }
@@ -508,6 +519,28 @@ struct Engine final : EngineBase {
         w->p1 = static_cast<double*>(dalloc(sizeof(double) * G::BLOCKS_ROW));
         w->p3 = static_cast<double*>(dalloc(sizeof(double) * 3 * G::BLOCKS_COL));
         w->p5 = static_cast<double*>(dalloc(sizeof(double) * G::BLOCKS_ROW));
+        w->gw = gw;
+        if (gw) {
+            w->ck_u.resize(nseg);
+            w->ck_pu.resize(nseg);
+            for (int sg = 0; sg < nseg; ++sg) {
+                w->ck_u[sg] = dalloc(3 * arr);
+                w->ck_pu[sg] = dalloc(3 * arr);
+            }
+            w->u.resize(SK + 1);
+            w->g.resize(SK + 1);
+            w->puh.resize(SK);
+            for (int j = 0; j <= SK; ++j) {
+                w->u[j] = dalloc(3 * arr);
+                w->g[j] = dalloc(3 * arr);
+                if (j < SK) w->puh[j] = dalloc(3 * arr);
+            }
+            w->ubar = dalloc(3 * arr);
+            w->pubar = dalloc(3 * arr);
+            w->p1g = static_cast<double*>(dalloc(sizeof(double) * G::BLOCKS_ROW));
+            w->p3g = static_cast<double*>(dalloc(sizeof(double) * G::BLOCKS_COL));
+            w->p5g = static_cast<double*>(dalloc(sizeof(double) * G::BLOCKS_ROW));
+        }
         if (!ok) return c->fail(PSC_ERR_NOMEM, "adjoint: device allocation failed");
         c->adj = std::move(w);
         *out = c->adj.get();
@@ -518,6 +551,7 @@ struct Engine final : EngineBase {
     // real-space fields at every substep boundary, pi_half per substep and the
     // scalar state after each scalar update.
     int record_segment(psc_ctx* c, int steps, AdjWork& w) {
+        if (w.gw) return record_segment_gw(c, steps, w);
         const int S = (int)c->w.size();
         const double dt = c->cfg.dt;
         const StepConsts sc = consts_of(c);
@@ -567,21 +601,97 @@ struct Engine final : EngineBase {
         return PSC_OK;
     }
 
-    // One backward substep (or the t = 0 pass): P1, y inverse, P3, y forward, P5.
-    int backward_passes(psc_ctx* c, AdjWork& w, int jf, int jph) {
+    // The same replay through the GW chain (force_chain_gw), recording also u
+    // and the real-space gradients at every boundary and pi_u half per substep.
+    int record_segment_gw(psc_ctx* c, int steps, AdjWork& w) {
+        const int S = (int)c->w.size();
+        const double dt = c->cfg.dt;
+        const StepConsts sc = consts_of(c);
+        const size_t arr = G::MS * sizeof(cpx<T>), as = G::MS;   // one member: arrays MS apart
+        const double kIR = c->cfg.kIR;
+        HIPCHK(c, hipMemcpyAsync(w.p[0], c->Pp, arr, hipMemcpyDeviceToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(w.u[0], c->U, 3 * arr, hipMemcpyDeviceToDevice, c->stream));
+        ScalarOp op{};
+        op.begin = 1;
+        op.w_begin = c->w[0];
+        op.nV = G::BLOCKS_COL;
+        op.nKG = G::BLOCKS_ROW;
+        int rc = launch_scalars(c, op, w.rms);
+        if (rc) return rc;
         dim3 rgrid(G::BLOCKS_ROW, 1), rblock(G::THREADS_ROW), cgrid(N / G::TC, N, 1), cblock(G::THREADS_COL);
+        dim3 cgrid3(N / G::TC, N, 3);
+        auto middle = [&](int jb) -> int {   // y inverse .. y forward of the GW chain at boundary jb
+            int r2 = launch_y<+1>(c, P(c->W), 4);
+            if (r2) return r2;
+            hipLaunchKernelGGL((k_x_force<G, true>), cgrid, cblock, 0, c->stream, P(c->W), c->ms, P(c->tw),
+                               c->partV, P(c->Pf), P(w.r[jb]));
+            hipLaunchKernelGGL((k_strided<G, 0, +1>), cgrid3, cblock, 0, c->stream, P(c->W) + as, P(c->tw));
+            hipLaunchKernelGGL((k_x_source<G, true>), cgrid, cblock, 0, c->stream, P(c->W), P(c->tw), as,
+                               P(w.g[jb]));
+            return launch_y<-1>(c, P(c->W), 4);
+        };
+        // prologue force at (f_0, a_0, u_0): first kicks
+        hipLaunchKernelGGL((k_z_inverse_gw<G, false>), rgrid, rblock, 0, c->stream, P(c->Pf), P(c->Pp), P(c->W),
+                           P(c->U), P(c->PU), c->ms, P(c->tw), as, kIR, nullptr, nullptr, nullptr, nullptr);
+        if ((rc = middle(0))) return rc;
+        hipLaunchKernelGGL((k_z_kick_gw<G, false>), rgrid, rblock, 0, c->stream, P(c->W), P(c->Pf), P(c->Pp),
+                           P(c->U), P(c->PU), c->ms, P(c->tw), c->partKG, c->w[0] * dt / 2.0, 0.0, 1, sc, as,
+                           nullptr);
+        for (int j = 0; j < steps * S; ++j) {
+            const int p = j % S;
+            const bool last = j == steps * S - 1;
+            const double wn = c->w[(p + 1) % S];
+            hipLaunchKernelGGL((k_z_inverse_gw<G, true, true>), rgrid, rblock, 0, c->stream, P(c->Pf), P(c->Pp),
+                               P(c->W), P(c->U), P(c->PU), c->ms, P(c->tw), as, kIR, P(w.f[j]), P(w.ph[j]),
+                               P(w.u[j + 1]), P(w.puh[j]));
+            if ((rc = middle(j + 1))) return rc;
+            hipLaunchKernelGGL((k_z_kick_gw<G, true, true>), rgrid, rblock, 0, c->stream, P(c->W), P(c->Pf),
+                               P(c->Pp), P(c->U), P(c->PU), c->ms, P(c->tw), c->partKG, c->w[p] * dt / 2.0,
+                               last ? 0.0 : wn * dt / 2.0, 0, sc, as, P(w.p[j + 1]));
+            ScalarOp o{};
+            o.finish = 1;
+            o.last_of_step = p == S - 1;
+            o.begin = last ? 0 : 1;
+            o.w_finish = c->w[p];
+            o.w_begin = wn;
+            o.nV = G::BLOCKS_COL;
+            o.nKG = G::BLOCKS_ROW;
+            if ((rc = launch_scalars(c, o, w.rms + j + 1))) return rc;
+        }
+        HIPCHK(c, hipMemcpyAsync(w.f[steps * S], c->Pf, arr, hipMemcpyDeviceToDevice, c->stream));
+        HIPCHK(c, hipGetLastError());
+        return PSC_OK;
+    }
+
+    // One backward substep (or the t = 0 pass): P1, y inverse, P3, y forward,
+    // P5; with the GW objective P1g, P3g, P5g run beside them (P5g before P5,
+    // so that the drift adjoint in P5 sees the complete fbar).
+    int backward_passes(psc_ctx* c, AdjWork& w, int jf, int jph, bool gw) {
+        dim3 rgrid(G::BLOCKS_ROW, 1), rblock(G::THREADS_ROW), cgrid(N / G::TC, N, 1), cblock(G::THREADS_COL);
+        const size_t as = G::MS;
         hipLaunchKernelGGL((k_adj_p1<G>), rgrid, rblock, 0, c->stream, P(w.f[jf]), P(w.p[jf]), P(w.fbar),
                            P(w.pbar), P(c->W), w.co, P(c->tw), w.p1);
-        int rc = launch_y<+1>(c, P(c->W), 1);
+        if (gw)
+            hipLaunchKernelGGL((k_adj_p1g<G>), rgrid, rblock, 0, c->stream, P(w.pubar), P(w.ubar), P(w.u[jf]),
+                               P(c->W), as, w.co, P(c->tw), w.p1g);
+        int rc = launch_y<+1>(c, P(c->W), gw ? 4 : 1);
         if (rc) return rc;
         hipLaunchKernelGGL((k_adj_p3<G>), cgrid, cblock, 0, c->stream, P(c->W), P(w.r[jf]), c->ms, w.co, P(c->tw),
                            w.p3);
-        if ((rc = launch_y<-1>(c, P(c->W), 1))) return rc;
+        if (gw)
+            hipLaunchKernelGGL((k_adj_p3g<G>), cgrid, cblock, 0, c->stream, P(c->W) + as, P(w.g[jf]), as,
+                               P(c->tw), w.p3g);
+        if ((rc = launch_y<-1>(c, P(c->W), gw ? 4 : 1))) return rc;
+        if (gw)
+            hipLaunchKernelGGL((k_adj_p5g<G>), rgrid, rblock, 0, c->stream, P(c->W) + as, P(w.fbar), P(w.ubar),
+                               P(w.pubar), P(w.puh[jph]), as, w.co, c->cfg.kIR, P(c->tw), w.p5g);
         hipLaunchKernelGGL((k_adj_p5<G>), rgrid, rblock, 0, c->stream, P(c->W), P(w.fbar), P(w.pbar),
                            P(w.ph[jph]), w.co, P(c->tw), w.p5);
         HIPCHK(c, hipGetLastError());
         return PSC_OK;
     }
+
+    AdjPartials partials(const AdjWork& w) const { return AdjPartials{w.p1, w.p3, w.p5, w.p1g, w.p3g, w.p5g}; }
 
     AdjScalarOp adj_op(const psc_ctx* c) {
         const StepConsts sc = consts_of(c);
@@ -594,6 +704,7 @@ struct Engine final : EngineBase {
         op.n1 = G::BLOCKS_ROW;
         op.n3 = G::BLOCKS_COL;
         op.n5 = G::BLOCKS_ROW;
+        op.gw = c->adj && c->adj->gw ? 1 : 0;
         return op;
     }
 
@@ -606,9 +717,9 @@ struct Engine final : EngineBase {
             op.mode = 1 | 2;
             op.h_pre = c->w[j % S] * c->cfg.dt / 2.0;
             op.end_of_segment = j == nsub - 1;
-            hipLaunchKernelGGL(k_adj_scalars, dim3(1), dim3(256), 0, c->stream, w.st, w.co, w.p1, w.p3, w.p5,
+            hipLaunchKernelGGL(k_adj_scalars, dim3(1), dim3(256), 0, c->stream, w.st, w.co, partials(w),
                                w.rms + j + 1, w.rms + j, op);
-            int rc = backward_passes(c, w, j + 1, j);
+            int rc = backward_passes(c, w, j + 1, j, w.gw);
             if (rc) return rc;
         }
         return PSC_OK;
@@ -877,14 +988,126 @@ __global__ void k_adj_init_chi2(const cpx<typename G::T>* __restrict__ Pf, cpx<t
     fbar[idx] = cpx<typename G::T>{(typename G::T)(((double)a.x - b.x) * inv), (typename G::T)(((double)a.y + b.y) * inv)};
 }
 
+// GW energy fraction at the end (the totals of psc_gw_spectrum,
+// output.cpp:1027-1060) and its gradient with respect to the packed pi_u pairs
+// (u11 + i u22, u33 + i u12, u13 + i u23; full spectra, host FP64): the
+// reference's half-spectrum visiting with weights 2 / 1 (output.cpp:800-1040),
+// the k = 0 term without projection, either projection.
+void gw_fraction_gradient(int N, int projection, const std::vector<double> pu[3], double* Q,
+                          std::vector<double> grad[3]) {
+    typedef std::complex<double> zc;
+    const size_t n3 = (size_t)N * N * N;
+    for (int q = 0; q < 3; ++q) grad[q].assign(2 * n3, 0.0);
+    auto full = [&](int i, int j, int k) {
+        return ((size_t)((i % N + N) % N) * N + (size_t)((j % N + N) % N)) * N + (size_t)((k % N + N) % N);
+    };
+    auto at = [&](int q, size_t i) { return zc(pu[q][2 * i], pu[q][2 * i + 1]); };
+    auto add = [&](int q, size_t i, zc v) {
+        grad[q][2 * i] += v.real();
+        grad[q][2 * i + 1] += v.imag();
+    };
+    static const int pr[6] = {0, 0, 1, 1, 2, 2};           // pair of component c
+    static const int ri[6] = {0, 1, 2, 0, 0, 1}, rj[6] = {0, 1, 2, 1, 2, 2};   // 11 22 33 12 13 23
+    // component c from its pair: even c = (P(k) + conj P(-k)) / 2, odd c = (P(k) - conj P(-k)) / 2i
+    auto unpack = [&](int c, size_t k, size_t mk) {
+        const zc x = at(pr[c], k), y = std::conj(at(pr[c], mk));
+        return (c % 2 == 0) ? 0.5 * (x + y) : zc(0, -0.5) * (x - y);
+    };
+    auto scatter = [&](int c, size_t k, size_t mk, zc g) {
+        const zc s = (c % 2 == 0) ? zc(0.5, 0) : zc(0, 0.5);
+        add(pr[c], k, s * g);
+        add(pr[c], mk, s * std::conj(g));
+    };
+    double total = 0.0;
+    auto mode = [&](int i, int j, int k, double wgt) {
+        const size_t kk = full(i, j, k), mk = full(-i, -j, -k);
+        const double kv[3] = {(double)(i <= N / 2 ? i : i - N), (double)(j <= N / 2 ? j : j - N), (double)k};
+        const double kn = std::sqrt(kv[0] * kv[0] + kv[1] * kv[1] + kv[2] * kv[2]);
+        double Pm[3][3];
+        for (int x = 0; x < 3; ++x)
+            for (int y = 0; y < 3; ++y) Pm[x][y] = (x == y ? 1.0 : 0.0) - kv[x] / kn * kv[y] / kn;
+        zc M[3][3], PM[3][3], Gm[3][3];
+        for (int c = 0; c < 6; ++c) M[ri[c]][rj[c]] = M[rj[c]][ri[c]] = unpack(c, kk, mk);
+        for (int x = 0; x < 3; ++x)
+            for (int y = 0; y < 3; ++y) {
+                PM[x][y] = 0;
+                for (int z = 0; z < 3; ++z) PM[x][y] += Pm[x][z] * M[z][y];
+            }
+        const zc tr = PM[0][0] + PM[1][1] + PM[2][2];
+        double q = 0.0;
+        if (projection == PSC_GW_PROJ_TT) {   // Lambda M = P M P - P tr(P M) / 2; gradient 2 Lambda(Lambda M)
+            zc A[3][3];
+            for (int x = 0; x < 3; ++x)
+                for (int y = 0; y < 3; ++y) {
+                    A[x][y] = 0;
+                    for (int z = 0; z < 3; ++z) A[x][y] += PM[x][z] * Pm[z][y];
+                    A[x][y] -= 0.5 * Pm[x][y] * tr;
+                    q += std::norm(A[x][y]);
+                }
+            zc trPA = 0;
+            for (int z = 0; z < 3; ++z)
+                for (int w2 = 0; w2 < 3; ++w2) trPA += Pm[z][w2] * A[w2][z];
+            for (int x = 0; x < 3; ++x)
+                for (int y = 0; y < 3; ++y) {
+                    zc v = 0;
+                    for (int z = 0; z < 3; ++z)
+                        for (int w2 = 0; w2 < 3; ++w2) v += Pm[x][z] * A[z][w2] * Pm[w2][y];
+                    Gm[x][y] = 2.0 * (v - 0.5 * trPA * Pm[x][y]);
+                }
+        } else {   // one-sided P M (output.cpp:819-831): |P M|^2 - |tr P M|^2 / 2
+            for (int x = 0; x < 3; ++x)
+                for (int y = 0; y < 3; ++y) q += std::norm(PM[x][y]);
+            q -= 0.5 * std::norm(tr);
+            for (int x = 0; x < 3; ++x)
+                for (int y = 0; y < 3; ++y) {
+                    zc v = 0;
+                    for (int z = 0; z < 3; ++z) v += Pm[x][z] * PM[z][y];
+                    Gm[x][y] = 2.0 * v - tr * Pm[x][y];
+                }
+        }
+        total += wgt * q;
+        for (int c = 0; c < 6; ++c) {
+            const int x = ri[c], y = rj[c];
+            scatter(c, kk, mk, wgt * (x == y ? Gm[x][x] : Gm[x][y] + Gm[y][x]));
+        }
+    };
+    const int H = N / 2;
+    for (int k = 0; k <= H; k += H) {
+        for (int i = 0; i <= H; i += H)
+            for (int j = 1; j < H; j++) mode(i, j, k, 2.0);
+        for (int j = 0; j <= H; j += H)
+            for (int i = 1; i < H; i++) mode(i, j, k, 2.0);
+        for (int i = 1; i < H; i++) {
+            for (int j = 1; j < H; j++) mode(i, j, k, 2.0);
+            for (int j = H + 1; j < N; j++) mode(i, j, k, 2.0);
+        }
+    }
+    for (int k = 1; k < H; k++)
+        for (int i = 0; i < N; i++)
+            for (int j = 0; j < N; j++) mode(i, j, k, 2.0);
+    for (int k = 0; k <= H; k += H)
+        for (int i = 0; i <= H; i += H)
+            for (int j = 0; j <= H; j += H)
+                if (i || j || k) mode(i, j, k, 1.0);
+    for (int c = 0; c < 6; ++c) {   // k = 0 (output.cpp:1032-1038)
+        const double wgt = ri[c] == rj[c] ? 1.0 : 2.0;
+        const zc v = unpack(c, 0, 0);
+        total += wgt * std::norm(v);
+        scatter(c, 0, 0, wgt * 2.0 * v);
+    }
+    *Q = total;
+}
+
 template <class G>
 int Engine<G>::adjoint(psc_ctx* c, const psc_adjoint_spec* sp, double* out, double* dchi) {
     const int M = sp->nsteps;
     const int K = std::min(sp->checkpoint_every, M);
     const int nseg = (M + K - 1) / K;
+    const int S = (int)c->w.size();
     const size_t arr = G::MS * sizeof(cpx<T>);
+    const bool gw = sp->objective == PSC_OBJ_GW_FRACTION;
     AdjWork* w = nullptr;
-    int rc = adj_workspace(c, K, nseg, &w);
+    int rc = adj_workspace(c, K, nseg, gw, &w);
     if (rc) return rc;
     MemberState s0;
     if ((rc = read_state(c, 0, &s0))) return rc;
@@ -898,35 +1121,99 @@ int Engine<G>::adjoint(psc_ctx* c, const psc_adjoint_spec* sp, double* out, doub
             for (int i = 0; i < 3; ++i) (void)hipEventDestroy(e[i]);
         }
     } guard{ev};
+    auto save = [&](int sg) -> int {
+        HIPCHK(c, hipMemcpyAsync(w->ck[2 * sg], c->Pf, arr, hipMemcpyDeviceToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(w->ck[2 * sg + 1], c->Pp, arr, hipMemcpyDeviceToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(w->ck_ms + sg, c->ms, sizeof(MemberState), hipMemcpyDeviceToDevice, c->stream));
+        if (gw) {
+            HIPCHK(c, hipMemcpyAsync(w->ck_u[sg], c->U, 3 * arr, hipMemcpyDeviceToDevice, c->stream));
+            HIPCHK(c, hipMemcpyAsync(w->ck_pu[sg], c->PU, 3 * arr, hipMemcpyDeviceToDevice, c->stream));
+        }
+        return PSC_OK;
+    };
+    auto restore = [&](int sg) -> int {
+        HIPCHK(c, hipMemcpyAsync(c->Pf, w->ck[2 * sg], arr, hipMemcpyDeviceToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(c->Pp, w->ck[2 * sg + 1], arr, hipMemcpyDeviceToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(c->ms, w->ck_ms + sg, sizeof(MemberState), hipMemcpyDeviceToDevice, c->stream));
+        if (gw) {
+            HIPCHK(c, hipMemcpyAsync(c->U, w->ck_u[sg], 3 * arr, hipMemcpyDeviceToDevice, c->stream));
+            HIPCHK(c, hipMemcpyAsync(c->PU, w->ck_pu[sg], 3 * arr, hipMemcpyDeviceToDevice, c->stream));
+        }
+        return PSC_OK;
+    };
     // forward run with a checkpoint at every segment start; the last segment
     // is recorded on the way
     HIPCHK(c, hipEventRecord(ev[0], c->stream));
     for (int sg = 0; sg < nseg; ++sg) {
-        HIPCHK(c, hipMemcpyAsync(w->ck[2 * sg], c->Pf, arr, hipMemcpyDeviceToDevice, c->stream));
-        HIPCHK(c, hipMemcpyAsync(w->ck[2 * sg + 1], c->Pp, arr, hipMemcpyDeviceToDevice, c->stream));
-        HIPCHK(c, hipMemcpyAsync(w->ck_ms + sg, c->ms, sizeof(MemberState), hipMemcpyDeviceToDevice, c->stream));
+        if ((rc = save(sg))) return rc;
         const int steps = std::min(K, M - sg * K);
         if ((rc = sg == nseg - 1 ? run_segment(c, steps, *w, false) : step(c, steps))) return rc;
     }
     HIPCHK(c, hipEventRecord(ev[1], c->stream));
     // the objective and its adjoint at the end
-    if ((rc = real_space(c, 0, c->Pf))) return rc;
-    double rs[6];
-    HIPCHK(c, hipMemcpyAsync(rs, c->sums, sizeof(rs), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    const double J = rs[3] / (double)G::N3;
+    double J = 0.0;
     HIPCHK(c, hipMemsetAsync(w->pbar, 0, arr, c->stream));
-    HIPCHK(c, hipMemsetAsync(w->st, 0, sizeof(AdjState), c->stream));
-    hipLaunchKernelGGL((k_adj_init_chi2<G>), dim3((unsigned)((G::N3 + 255) / 256)), dim3(256), 0, c->stream,
-                       P(c->Pf), P(w->fbar));
+    if (!gw) {
+        if ((rc = real_space(c, 0, c->Pf))) return rc;
+        double rs[6];
+        HIPCHK(c, hipMemcpyAsync(rs, c->sums, sizeof(rs), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        J = rs[3] / (double)G::N3;
+        HIPCHK(c, hipMemsetAsync(w->st, 0, sizeof(AdjState), c->stream));
+        hipLaunchKernelGGL((k_adj_init_chi2<G>), dim3((unsigned)((G::N3 + 255) / 256)), dim3(256), 0, c->stream,
+                           P(c->Pf), P(w->fbar));
+    } else {
+        // J = rho_GW / rho, rho_GW = (f*/M_P)^2 Q / (4 N^6 a^6) (output.cpp:1056-1058)
+        MemberState se;
+        if ((rc = read_state(c, 0, &se))) return rc;
+        Diag de;
+        if ((rc = diagnose(c, 0, se, &de))) return rc;
+        std::vector<double> pu[3], grad[3];
+        for (int q = 0; q < 3; ++q)
+            if ((rc = download(c, c->PU, q * c->B, pu[q]))) return rc;
+        double Q = 0.0;
+        gw_fraction_gradient(N, sp->projection, pu, &Q, grad);
+        const StepConsts sc = consts_of(c);
+        const double a = se.a, N3 = (double)G::N3;
+        const double Kt = de.K[0] + de.K[1], Gt = de.G[0] + de.G[1], Vt = de.Vt[0] + de.Vt[1];
+        const double rho = Kt + Gt + Vt;
+        const double cgw = sc.f2m / (4.0 * N3 * N3 * std::pow(a, 6));
+        J = cgw * Q / rho;
+        for (int q = 0; q < 3; ++q) {
+            for (auto& v : grad[q]) v *= cgw / rho;
+            if ((rc = upload(c, w->pubar, q, grad[q]))) return rc;
+        }
+        HIPCHK(c, hipMemsetAsync(w->ubar, 0, 3 * arr, c->stream));
+        HIPCHK(c, hipMemsetAsync(w->fbar, 0, arr, c->stream));
+        // <phi^2 chi^2 / 2> from the recorded final real-space fields (dV/dg2)
+        const int nsub_last = (M - (nseg - 1) * K) * S;
+        std::vector<double> r;
+        if ((rc = download(c, w->r[nsub_last], 0, r))) return rc;
+        double vg = 0.0;
+        for (size_t i = 0; i < G::N3; ++i) vg += 0.5 * r[2 * i] * r[2 * i] * r[2 * i + 1] * r[2 * i + 1];
+        vg /= N3;
+        // -J / rho times the gradient of rho = K + G + V at the final point: the
+        // K and G parts and V' through one pass of the adjoint kernels
+        const double cr = -J / rho;
+        AdjCoef co{};
+        co.ck = cr / (std::pow(a, 6) * N3 * N3);
+        co.cg = cr * sc.kIR * sc.kIR / (a * a * N3 * N3);
+        co.cV = cr / N3;
+        co.final_pass = 1;
+        HIPCHK(c, hipMemcpyAsync(w->co, &co, sizeof(co), hipMemcpyHostToDevice, c->stream));
+        if ((rc = backward_passes(c, *w, nsub_last, 0, false))) return rc;
+        AdjState st{};
+        st.g2bar = cr * vg;
+        st.abar = sc.expansion_static ? 0.0 : -6.0 * J / a + cr * (-6.0 * Kt / a - 2.0 * Gt / a);
+        HIPCHK(c, hipMemcpyAsync(w->st, &st, sizeof(st), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
     // segments from the last: restore the checkpoint, replay with recording
     // (the last segment is still recorded), sweep back
     for (int sg = nseg - 1; sg >= 0; --sg) {
         const int steps = std::min(K, M - sg * K);
         if (sg < nseg - 1) {
-            HIPCHK(c, hipMemcpyAsync(c->Pf, w->ck[2 * sg], arr, hipMemcpyDeviceToDevice, c->stream));
-            HIPCHK(c, hipMemcpyAsync(c->Pp, w->ck[2 * sg + 1], arr, hipMemcpyDeviceToDevice, c->stream));
-            HIPCHK(c, hipMemcpyAsync(c->ms, w->ck_ms + sg, sizeof(MemberState), hipMemcpyDeviceToDevice, c->stream));
+            if ((rc = restore(sg))) return rc;
             if ((rc = run_segment(c, steps, *w, false))) return rc;
         }
         if ((rc = run_segment(c, steps, *w, true))) return rc;
@@ -935,19 +1222,19 @@ int Engine<G>::adjoint(psc_ctx* c, const psc_adjoint_spec* sp, double* out, doub
     // and the initial d2a/dt2
     AdjScalarOp op = adj_op(c);
     op.mode = 1;
-    hipLaunchKernelGGL(k_adj_scalars, dim3(1), dim3(256), 0, c->stream, w->st, w->co, w->p1, w->p3, w->p5, w->rms,
-                       w->rms, op);
+    hipLaunchKernelGGL(k_adj_scalars, dim3(1), dim3(256), 0, c->stream, w->st, w->co, partials(*w), w->rms, w->rms,
+                       op);
     op.mode = 2 | 4;
     op.a_e = s0.a;
     op.K_e = d0.K[0] + d0.K[1];
     op.G_e = d0.G[0] + d0.G[1];
     op.V_e = d0.Vt[0] + d0.Vt[1];
-    hipLaunchKernelGGL(k_adj_scalars, dim3(1), dim3(256), 0, c->stream, w->st, w->co, w->p1, w->p3, w->p5, w->rms,
-                       w->rms, op);
-    if ((rc = backward_passes(c, *w, 0, 0))) return rc;
+    hipLaunchKernelGGL(k_adj_scalars, dim3(1), dim3(256), 0, c->stream, w->st, w->co, partials(*w), w->rms, w->rms,
+                       op);
+    if ((rc = backward_passes(c, *w, 0, 0, gw))) return rc;
     op.mode = 1 | 4;
-    hipLaunchKernelGGL(k_adj_scalars, dim3(1), dim3(256), 0, c->stream, w->st, w->co, w->p1, w->p3, w->p5, w->rms,
-                       w->rms, op);
+    hipLaunchKernelGGL(k_adj_scalars, dim3(1), dim3(256), 0, c->stream, w->st, w->co, partials(*w), w->rms, w->rms,
+                       op);
     HIPCHK(c, hipEventRecord(ev[2], c->stream));
     HIPCHK(c, hipEventSynchronize(ev[2]));
     float fwd_ms = 0, bwd_ms = 0;
@@ -966,9 +1253,7 @@ int Engine<G>::adjoint(psc_ctx* c, const psc_adjoint_spec* sp, double* out, doub
     const double phi0bar =
         (double)G::N3 * (double)f00.x + (sc.expansion_static ? 0.0 : hst.bbar * s0.a * sc.f2m * vphi / (6.0 * H));
     // restore the loaded state
-    HIPCHK(c, hipMemcpyAsync(c->Pf, w->ck[0], arr, hipMemcpyDeviceToDevice, c->stream));
-    HIPCHK(c, hipMemcpyAsync(c->Pp, w->ck[1], arr, hipMemcpyDeviceToDevice, c->stream));
-    HIPCHK(c, hipMemcpyAsync(c->ms, w->ck_ms, sizeof(MemberState), hipMemcpyDeviceToDevice, c->stream));
+    if ((rc = restore(0))) return rc;
     if (dchi) {
         // dJ/dchi(x) = Im(F^H fbar)(x); real_space() leaves F^H src / N^3 in W
         if ((rc = real_space(c, 0, w->fbar))) return rc;
@@ -1471,6 +1756,7 @@ int psc_load_gw_k(psc_ctx* c, int member, const double* u_k, const double* pi_u_
         pack_full(c->N, 2, pi_u_k + 2 * nk * (2 * a), full);
         if ((rc = upload(c, c->PU, (int)(arr_off + member), full))) return rc;
     }
+    c->loaded_gw = 1;
     return PSC_OK;
 }
 
@@ -1711,9 +1997,17 @@ int psc_set_shape(psc_ctx* c, const char* name) {
 int psc_adjoint(psc_ctx* c, const psc_adjoint_spec* sp, double* out, double* dchi) {
     if (!c) return PSC_ERR_ARG;
     if (!sp || !out) return c->fail(PSC_ERR_ARG, "null adjoint arguments");
-    if (sp->objective != PSC_OBJ_CHI2) return c->fail(PSC_ERR_UNSUPPORTED, "unknown objective");
-    if (c->cfg.nflds != 2 || c->B != 1 || c->cfg.with_gw || sp->member != 0)
-        return c->fail(PSC_ERR_UNSUPPORTED, "the adjoint needs two fields, one member and no GW");
+    if (sp->objective != PSC_OBJ_CHI2 && sp->objective != PSC_OBJ_GW_FRACTION)
+        return c->fail(PSC_ERR_UNSUPPORTED, "unknown objective");
+    if (c->cfg.nflds != 2 || c->B != 1 || sp->member != 0)
+        return c->fail(PSC_ERR_UNSUPPORTED, "the adjoint needs two fields and one member");
+    if (sp->objective == PSC_OBJ_GW_FRACTION && !c->cfg.with_gw)
+        return c->fail(PSC_ERR_UNSUPPORTED, "the GW objective needs a context with GW evolution");
+    if (sp->objective == PSC_OBJ_GW_FRACTION &&
+        (sp->projection != PSC_GW_PROJ_REFERENCE && sp->projection != PSC_GW_PROJ_TT))
+        return c->fail(PSC_ERR_ARG, "unknown projection");
+    if (sp->objective == PSC_OBJ_GW_FRACTION && !c->loaded_gw)
+        return c->fail(PSC_ERR_STATE, "no GW initial data");
     if (sp->nsteps < 1 || sp->checkpoint_every < 1) return c->fail(PSC_ERR_ARG, "bad step counts");
     if (!c->loaded[0]) return c->fail(PSC_ERR_STATE, "no initial data");
     const int saved = c->timing;

@@ -266,13 +266,16 @@ class Lattice:
     def set_shape(self, name):
         self._check(lib().psc_set_shape(self._h, name.encode()))
 
-    def adjoint(self, nsteps, checkpoint_every, phi0, dphi0, chi0=0.0, want_field=True):
-        """Gradient of the final <chi^2> (psc_adjoint): returns a dict with J,
-        dJ/dg2, dJ/dphi0, dJ/dchi(x) (N^3 array or None), forward and backward
-        milliseconds and the extra device bytes of the sweep."""
+    def adjoint(self, nsteps, checkpoint_every, phi0, dphi0, chi0=0.0, want_field=True, objective=0,
+                projection=1):
+        """Gradient of the final <chi^2> (objective 0) or of the final GW energy
+        fraction (objective 1, with `projection`) by psc_adjoint: returns a dict
+        with J, dJ/dg2, dJ/dphi0, dJ/dchi(x) (N^3 array or None), forward and
+        backward milliseconds and the device bytes of the adjoint workspace."""
         spec = PscAdjointSpec()
-        spec.objective, spec.nsteps, spec.checkpoint_every, spec.member = PSC_OBJ_CHI2, int(nsteps), \
+        spec.objective, spec.nsteps, spec.checkpoint_every, spec.member = int(objective), int(nsteps), \
             int(checkpoint_every), 0
+        spec.projection = int(projection)
         spec.phi0, spec.dphi0, spec.chi0 = phi0, dphi0, chi0
         out = np.zeros(6)
         dchi = np.zeros((self.N,) * 3) if want_field else None

@@ -187,3 +187,50 @@ def test_graph_captured_segments_and_workspace_reuse_agree_with_eager():
             assert g[k] == ref[k], k
         assert np.array_equal(g["dchi"], ref["dchi"])
     lat.close()
+
+
+# ---- GW energy fraction ------------------------------------------------------
+
+def setup_gw(integrator, expansion, N=16, precision=PSC_FP64, use_graph=0):
+    cfg, model, ic = parity_setup(N=N, nflds=2, integrator=integrator, expansion=expansion,
+                                  precision=precision, kIR=4.0, dt=0.01, g2=2.0e3, omega_star=2e-2,
+                                  f0=(PHI0, 0.0), fd0=(DPHI0, 0.0), yoshida_exact=1, with_gw=1,
+                                  use_graph=use_graph)
+    fk, pk, sc = O.generate_ic(cfg, model, ic)
+    u, pu = O.generate_gw_ic(cfg, model, ic)
+    lat = lattice(cfg, model, fk, pk, sc)
+    lat.load_gw_k(0, u, pu)
+    return cfg, model, fk, pk, u, pu, lat
+
+
+@pytest.mark.parametrize("integrator,expansion,projection,steps,every", [
+    (PSC_VV2, PSC_EXPANSION_SELF, 1, 8, 3),
+    (PSC_Y4, PSC_EXPANSION_SELF, 0, 5, 2),
+    (PSC_VV2, PSC_EXPANSION_STATIC, 1, 6, 6),
+])
+def test_gw_fraction_adjoint_matches_oracle(integrator, expansion, projection, steps, every):
+    cfg, model, fk, pk, u, pu, lat = setup_gw(integrator, expansion)
+    print("case gw", integrator, expansion, projection, steps, every, flush=True)
+    k0 = lat.get_gw_k()
+    g = lat.adjoint(steps, every, PHI0, DPHI0, objective=1, projection=projection)
+    J, dg2, dphi0, dchi = O.adjoint_gw(cfg, model, fk, pk, u, pu, 1.0, PHI0, DPHI0, 0.0, steps, projection)
+    errs = (abs(g["J"] - J) / abs(J), abs(g["dg2"] - dg2) / abs(dg2), abs(g["dphi0"] - dphi0) / abs(dphi0),
+            rel_err(g["dchi"], dchi))
+    print("gw errors J, g2, phi0, chi:", errs, "J", g["J"], J)
+    assert errs[0] <= 1e-12 and max(errs[1:]) <= 1e-9
+    k1 = lat.get_gw_k()
+    assert rel_err(k1[0], k0[0]) == 0.0 and rel_err(k1[1], k0[1]) == 0.0
+    lat.close()
+
+
+def test_gw_fraction_adjoint_fp32_and_graphs():
+    cfg, model, fk, pk, u, pu, lat = setup_gw(PSC_VV2, PSC_EXPANSION_SELF, N=32, precision=PSC_FP32,
+                                              use_graph=1)
+    J, dg2, dphi0, dchi = O.adjoint_gw(cfg, model, fk, pk, u, pu, 1.0, PHI0, DPHI0, 0.0, 6, 1)
+    for _ in range(2):   # captured segments, then reuse of the workspace
+        g = lat.adjoint(6, 3, PHI0, DPHI0, objective=1, projection=1)
+        errs = (abs(g["J"] - J) / abs(J), abs(g["dg2"] - dg2) / abs(dg2), abs(g["dphi0"] - dphi0) / abs(dphi0),
+                rel_err(g["dchi"], dchi))
+        print("gw fp32 errors J, g2, phi0, chi:", errs)
+        assert max(errs) <= 1e-3
+    lat.close()
