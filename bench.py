@@ -38,7 +38,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 from pspectcosmo_amd.abi import make_config, make_model, make_ic, PSC_Y4, PSC_FP32, PSC_FP64  # noqa: E402
-from pspectcosmo_amd.lattice import Lattice, generate_ic  # noqa: E402
+from pspectcosmo_amd.lattice import Lattice, generate_ic, generate_gw_ic  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 METRIC = "site-updates/s at 256³, FP32-mixed vs FP64 on same GPU; % HBM roofline"
@@ -242,16 +242,20 @@ PHI0, DPHI0 = 1.0, -0.7
 _ADJ_IC = {}
 
 
-def adjoint_case(precision, phi0=PHI0, g2=ADJ_G2, dchi=None, tuned=False):
+def adjoint_case(precision, phi0=PHI0, g2=ADJ_G2, dchi=None, tuned=False, gw=False):
     """Context loaded with the adjoint workload's initial state.  The
     fluctuations are generated once (with g^2 = 1e4) and held fixed, as the
     gradient holds them: g2 changes only the coupling, phi0 sets the DC mode
     and da/dt = a H(phi0), dchi (real space) is added to chi."""
     n = ADJ_N
     cfg = make_config(n, nflds=2, integrator=2, precision=precision, kIR=ADJ_KIR, dt=ADJ_DT,
-                      omegaStar=ADJ_OMEGA, use_graph=1, autotune=1 if tuned else 0)
+                      omegaStar=ADJ_OMEGA, use_graph=1, autotune=1 if tuned else 0, with_gw=1 if gw else 0)
     if not _ADJ_IC:
-        _ADJ_IC["ic"] = generate_ic(cfg, make_model(1.0, ADJ_G2), make_ic(f0=(PHI0, 0.0), fd0=(DPHI0, 0.0), seed=1))
+        ic = make_ic(f0=(PHI0, 0.0), fd0=(DPHI0, 0.0), seed=1)
+        _ADJ_IC["ic"] = generate_ic(cfg, make_model(1.0, ADJ_G2), ic)
+        cgw = make_config(n, nflds=2, integrator=2, precision=precision, kIR=ADJ_KIR, dt=ADJ_DT,
+                          omegaStar=ADJ_OMEGA, with_gw=1)
+        _ADJ_IC["gw"] = generate_gw_ic(cgw, make_model(1.0, ADJ_G2), ic)
     fk, pk, _ = _ADJ_IC["ic"]
     fk = fk.copy()
     fk[0, 0, 0, 0] = phi0 * n ** 3
@@ -261,7 +265,14 @@ def adjoint_case(precision, phi0=PHI0, g2=ADJ_G2, dchi=None, tuned=False):
     lat = Lattice(cfg)
     lat.set_model(0, make_model(1.0, g2))
     lat.load_ic_k(0, fk, pk, 1.0, H)
+    if gw:
+        lat.load_gw_k(0, *_ADJ_IC["gw"])
     return lat
+
+
+def gw_fraction_after(lat, steps):
+    lat.step(steps)
+    return float(lat.gw_spectrum(1)[1][0])
 
 
 def chi2_after(lat, steps):
@@ -298,9 +309,14 @@ def bench_adjoint128(args):
     print(f"[bench] adjoint fp32: J {g['J']:.6e} dg2 {g['dg2']:.6e} dphi0 {g['dphi0']:.6e} "
           f"fwd {fwd * 1e3:.1f} ms, adjoint fwd {g['forward_ms']:.1f} + bwd {g['backward_ms']:.1f} ms",
           file=sys.stderr, flush=True)
-    if args.no_fd:
+    if args.no_fd:   # profiling runs: both objectives in FP32, no reference runs
+        lgw = adjoint_case(PSC_FP32, tuned=tuned, gw=True)
+        lgw.adjoint(steps, every, PHI0, DPHI0, want_field=False, objective=1, projection=1)
+        ggw = lgw.adjoint(steps, every, PHI0, DPHI0, objective=1, projection=1)
+        lgw.close()
         print(json.dumps({"metric": "adjoint/forward time at 128^3, FP32-mixed", "value": g["backward_ms"] / (fwd * 1e3),
-                          "unit": "x", "profiling_run": True}))
+                          "unit": "x", "profiling_run": True, "gw_forward_ms": ggw["forward_ms"],
+                          "gw_backward_ms": ggw["backward_ms"]}))
         return
     # FP64 GPU gradient of the same run
     l64 = adjoint_case(PSC_FP64)
@@ -329,6 +345,43 @@ def bench_adjoint128(args):
     def rel(a, b):
         return abs(a - b) / abs(b)
 
+    # the GW energy fraction objective (TT projection) on the same run with the six u_ij evolved
+    lgw = adjoint_case(PSC_FP32, tuned=tuned, gw=True)
+    lgw.step(args.warmup)
+    device_sync(lgw)
+    lgw.close()
+    lgw = adjoint_case(PSC_FP32, tuned=tuned, gw=True)
+    device_sync(lgw)
+    t0 = time.perf_counter()
+    lgw.step(steps)
+    device_sync(lgw)
+    fwd_gw = time.perf_counter() - t0
+    lgw.close()
+    lgw = adjoint_case(PSC_FP32, tuned=tuned, gw=True)
+    lgw.adjoint(steps, every, PHI0, DPHI0, want_field=False, objective=1, projection=1)
+    ggw = lgw.adjoint(steps, every, PHI0, DPHI0, objective=1, projection=1)
+    lgw.close()
+    l64 = adjoint_case(PSC_FP64, gw=True)
+    ggw64 = l64.adjoint(steps, every, PHI0, DPHI0, objective=1, projection=1)
+    l64.close()
+    print(f"[bench] adjoint gw fp32: J {ggw['J']:.6e} fwd {fwd_gw * 1e3:.1f} ms, adjoint fwd "
+          f"{ggw['forward_ms']:.1f} + bwd {ggw['backward_ms']:.1f} ms", file=sys.stderr, flush=True)
+    fdgw = {}
+    for name, prec in (("fp32", PSC_FP32), ("fp64", PSC_FP64)):
+        e = 1e-3 * ADJ_G2
+        dg2 = (gw_fraction_after(adjoint_case(prec, g2=ADJ_G2 + e, gw=True), steps) -
+               gw_fraction_after(adjoint_case(prec, g2=ADJ_G2 - e, gw=True), steps)) / (2 * e)
+        e = 1e-3 * PHI0
+        dphi0 = (gw_fraction_after(adjoint_case(prec, phi0=PHI0 + e, gw=True), steps) -
+                 gw_fraction_after(adjoint_case(prec, phi0=PHI0 - e, gw=True), steps)) / (2 * e)
+        v = np.random.default_rng(7).standard_normal((ADJ_N,) * 3)
+        e = 1e-3 * math.sqrt(g["J"])
+        dd = (gw_fraction_after(adjoint_case(prec, dchi=e * v, gw=True), steps) -
+              gw_fraction_after(adjoint_case(prec, dchi=-e * v, gw=True), steps)) / (2 * e)
+        fdgw[name] = {"dg2": dg2, "dphi0": dphi0, "dchi_dir": dd}
+    v = np.random.default_rng(7).standard_normal((ADJ_N,) * 3)
+    gw_dir, gw64_dir = float((ggw["dchi"] * v).sum()), float((ggw64["dchi"] * v).sum())
+
     ratio = g["backward_ms"] / (fwd * 1e3)
     out = {"metric": "adjoint/forward time at 128^3, 500 VV steps, checkpoint every 25, FP32-mixed",
            "value": ratio, "unit": "x", "n_gpus": 1, "steps": steps, "warmup": args.warmup,
@@ -352,7 +405,24 @@ def bench_adjoint128(args):
            "rel_fp32_vs_fd32": {k: rel(a, fd["fp32"][k]) for k, a in
                                 (("dg2", g["dg2"]), ("dphi0", g["dphi0"]), ("dchi_dir", ad_dir))},
            "rel_fp64_vs_fd64": {k: rel(a, fd["fp64"][k]) for k, a in
-                                (("dg2", g64["dg2"]), ("dphi0", g64["dphi0"]), ("dchi_dir", ad64_dir))}}
+                                (("dg2", g64["dg2"]), ("dphi0", g64["dphi0"]), ("dchi_dir", ad64_dir))},
+           "gw_fraction": {
+               "objective": "rho_GW / rho at the end, TT projection, six u_ij evolved",
+               "J": ggw["J"], "forward_ms": fwd_gw * 1e3, "adjoint_forward_ms": ggw["forward_ms"],
+               "adjoint_backward_ms": ggw["backward_ms"],
+               "backward_over_forward": ggw["backward_ms"] / (fwd_gw * 1e3),
+               "total_over_forward": (ggw["forward_ms"] + ggw["backward_ms"]) / (fwd_gw * 1e3),
+               "extra_device_bytes": ggw["extra_bytes"],
+               "grad_fp32": {"dg2": ggw["dg2"], "dphi0": ggw["dphi0"], "dchi_dir": gw_dir},
+               "grad_fp64": {"dg2": ggw64["dg2"], "dphi0": ggw64["dphi0"], "dchi_dir": gw64_dir},
+               "fd": fdgw,
+               "rel_fp32_vs_fp64": {"dg2": rel(ggw["dg2"], ggw64["dg2"]), "dphi0": rel(ggw["dphi0"], ggw64["dphi0"]),
+                                    "dchi_field": float(np.linalg.norm(ggw["dchi"] - ggw64["dchi"]) /
+                                                        np.linalg.norm(ggw64["dchi"]))},
+               "rel_fp32_vs_fd32": {k: rel(a, fdgw["fp32"][k]) for k, a in
+                                    (("dg2", ggw["dg2"]), ("dphi0", ggw["dphi0"]), ("dchi_dir", gw_dir))},
+               "rel_fp64_vs_fd64": {k: rel(a, fdgw["fp64"][k]) for k, a in
+                                    (("dg2", ggw64["dg2"]), ("dphi0", ggw64["dphi0"]), ("dchi_dir", gw64_dir))}}}
     print(json.dumps(out))
 
 

@@ -339,6 +339,178 @@ k_adj_p5g(const cpx<typename G::T>* __restrict__ W, cpx<typename G::T>* __restri
     if (threadIdx.x == 0) part[blockIdx.x] = red[0];
 }
 
+// ---------------------------------------------------------------------------
+// GW-energy-fraction objective at the end of the run: Q = sum over the modes
+// the reference visits (half spectrum kz <= N/2, one of each conjugate pair,
+// weight 2, self-conjugate corners weight 1; output.cpp:800-1040) of the
+// projected |pi_u(k)|^2, plus the unprojected k = 0 term (output.cpp:1032-1038),
+// and its gradient scale * dQ / d(pi_u pairs) written into G (three packed
+// arrays).  Every element of G belongs to exactly one visited mode (as k or
+// as -k), so each thread writes its own elements.
+struct zd {
+    double x, y;
+};
+__device__ __forceinline__ zd zadd(zd a, zd b) { return {a.x + b.x, a.y + b.y}; }
+__device__ __forceinline__ zd zsub(zd a, zd b) { return {a.x - b.x, a.y - b.y}; }
+__device__ __forceinline__ zd zscale(double s, zd a) { return {s * a.x, s * a.y}; }
+__device__ __forceinline__ zd zmul(zd a, zd b) { return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x}; }
+__device__ __forceinline__ zd zconj(zd a) { return {a.x, -a.y}; }
+__device__ __forceinline__ double znorm(zd a) { return a.x * a.x + a.y * a.y; }
+
+template <class G>
+__global__ void __launch_bounds__(256)
+k_gw_objective(const cpx<typename G::T>* __restrict__ PU, cpx<typename G::T>* __restrict__ Gout, int projection,
+               double scale, double* __restrict__ part) {
+    using T = typename G::T;
+    constexpr int N = G::N, H = N / 2;
+    constexpr size_t as = G::MS;
+    const size_t li = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double q = 0.0;
+    constexpr size_t total = (size_t)N * N * (H + 1);
+    if (li < total) {
+        const int k = (int)(li % (H + 1)), j = (int)((li / (H + 1)) % N), i = (int)(li / ((size_t)(H + 1) * N));
+        double wgt = 0.0;
+        bool dc = false;
+        if (k > 0 && k < H) {
+            wgt = 2.0;
+        } else {
+            const bool ie = i == 0 || i == H, je = j == 0 || j == H;
+            if (ie && je) {
+                if (i == 0 && j == 0 && k == 0) dc = true;
+                else wgt = 1.0;
+            } else if (ie && j >= 1 && j < H) {
+                wgt = 2.0;
+            } else if (je && i >= 1 && i < H) {
+                wgt = 2.0;
+            } else if (i >= 1 && i < H && ((j >= 1 && j < H) || (j > H && j < N))) {
+                wgt = 2.0;
+            }
+        }
+        if (wgt > 0.0 || dc) {
+            const size_t kk = (size_t)i * G::PS + (size_t)j * N + k;
+            const size_t mk = (size_t)((N - i) % N) * G::PS + (size_t)((N - j) % N) * N + (N - k) % N;
+            // the six components 11, 22, 33, 12, 13, 23 from the pairs (11,22), (33,12), (13,23)
+            zd u[6];
+#pragma unroll
+            for (int pq = 0; pq < 3; ++pq) {
+                const cpx<T> a = PU[pq * as + kk], b = PU[pq * as + mk];
+                const zd x = {(double)a.x, (double)a.y}, y = {(double)b.x, -(double)b.y};
+                u[2 * pq] = zscale(0.5, zadd(x, y));
+                const zd d = zsub(x, y);
+                u[2 * pq + 1] = {0.5 * d.y, -0.5 * d.x};   // (x - y) / 2i
+            }
+            zd gc[6];
+            if (dc) {
+#pragma unroll
+                for (int c = 0; c < 6; ++c) {
+                    const double w = c < 3 ? 1.0 : 2.0;   // diagonal once, off-diagonal twice
+                    q += w * znorm(u[c]);
+                    gc[c] = zscale(2.0 * w, u[c]);
+                }
+            } else {
+                const int ci[6] = {0, 1, 2, 0, 0, 1}, cj[6] = {0, 1, 2, 1, 2, 2};
+                const double kv[3] = {(double)signed_index(i, N), (double)signed_index(j, N), (double)k};
+                const double kn2 = kv[0] * kv[0] + kv[1] * kv[1] + kv[2] * kv[2];
+                double P[3][3];
+#pragma unroll
+                for (int x = 0; x < 3; ++x)
+#pragma unroll
+                    for (int y = 0; y < 3; ++y) P[x][y] = (x == y ? 1.0 : 0.0) - kv[x] * kv[y] / kn2;
+                zd M[3][3];
+#pragma unroll
+                for (int c = 0; c < 6; ++c) M[ci[c]][cj[c]] = M[cj[c]][ci[c]] = u[c];
+                zd PM[3][3];
+#pragma unroll
+                for (int x = 0; x < 3; ++x)
+#pragma unroll
+                    for (int y = 0; y < 3; ++y)
+                        PM[x][y] = zadd(zadd(zscale(P[x][0], M[0][y]), zscale(P[x][1], M[1][y])), zscale(P[x][2], M[2][y]));
+                const zd tr = zadd(zadd(PM[0][0], PM[1][1]), PM[2][2]);
+                zd Gm[3][3];
+                if (projection == 1) {   // TT: A = P M P - P tr(P M) / 2; gradient 2 (P A P - P tr(P A) / 2)
+                    zd A[3][3];
+#pragma unroll
+                    for (int x = 0; x < 3; ++x)
+#pragma unroll
+                        for (int y = 0; y < 3; ++y) {
+                            A[x][y] = zadd(zadd(zscale(P[0][y], PM[x][0]), zscale(P[1][y], PM[x][1])),
+                                           zscale(P[2][y], PM[x][2]));
+                            A[x][y] = zsub(A[x][y], zscale(0.5 * P[x][y], tr));
+                            q += znorm(A[x][y]);
+                        }
+                    zd PA[3][3];
+#pragma unroll
+                    for (int x = 0; x < 3; ++x)
+#pragma unroll
+                        for (int y = 0; y < 3; ++y)
+                            PA[x][y] = zadd(zadd(zscale(P[x][0], A[0][y]), zscale(P[x][1], A[1][y])), zscale(P[x][2], A[2][y]));
+                    const zd trPA = zadd(zadd(PA[0][0], PA[1][1]), PA[2][2]);
+#pragma unroll
+                    for (int x = 0; x < 3; ++x)
+#pragma unroll
+                        for (int y = 0; y < 3; ++y) {
+                            const zd pap = zadd(zadd(zscale(P[0][y], PA[x][0]), zscale(P[1][y], PA[x][1])),
+                                                zscale(P[2][y], PA[x][2]));
+                            Gm[x][y] = zscale(2.0, zsub(pap, zscale(0.5 * P[x][y], trPA)));
+                        }
+                } else {                 // one-sided P M: |P M|^2 - |tr P M|^2 / 2; gradient 2 P (P M) - tr(P M) P
+#pragma unroll
+                    for (int x = 0; x < 3; ++x)
+#pragma unroll
+                        for (int y = 0; y < 3; ++y) q += znorm(PM[x][y]);
+                    q -= 0.5 * znorm(tr);
+#pragma unroll
+                    for (int x = 0; x < 3; ++x)
+#pragma unroll
+                        for (int y = 0; y < 3; ++y) {
+                            const zd ppm = zadd(zadd(zscale(P[x][0], PM[0][y]), zscale(P[x][1], PM[1][y])),
+                                                zscale(P[x][2], PM[2][y]));
+                            Gm[x][y] = zsub(zscale(2.0, ppm), zscale(P[x][y], tr));
+                        }
+                }
+                q *= wgt;
+#pragma unroll
+                for (int c = 0; c < 6; ++c)
+                    gc[c] = zscale(wgt, ci[c] == cj[c] ? Gm[ci[c]][ci[c]] : zadd(Gm[ci[c]][cj[c]], Gm[cj[c]][ci[c]]));
+            }
+            // chain rule of the unpacking: pair += g_even / 2 + i g_odd / 2 at k, conjugates at -k
+#pragma unroll
+            for (int pq = 0; pq < 3; ++pq) {
+                const zd ge = gc[2 * pq], go = gc[2 * pq + 1];
+                const zd at_k = zadd(zscale(0.5, ge), zmul({0.0, 0.5}, go));
+                const zd at_mk = zadd(zscale(0.5, zconj(ge)), zmul({0.0, 0.5}, zconj(go)));
+                if (kk == mk) {
+                    const zd v = zadd(at_k, at_mk);
+                    Gout[pq * as + kk] = cpx<T>{(T)(scale * v.x), (T)(scale * v.y)};
+                } else {
+                    Gout[pq * as + kk] = cpx<T>{(T)(scale * at_k.x), (T)(scale * at_k.y)};
+                    Gout[pq * as + mk] = cpx<T>{(T)(scale * at_mk.x), (T)(scale * at_mk.y)};
+                }
+            }
+        }
+    }
+    double r[1] = {q};
+    __shared__ double sh[4];
+    block_sum<1, 256>(r, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = r[0];
+}
+
+// <phi^2 chi^2 / 2> partial sums over recorded real-space fields (dV / dg2).
+template <class G>
+__global__ void __launch_bounds__(256)
+k_vg_sums(const cpx<typename G::T>* __restrict__ r, double* __restrict__ part) {
+    constexpr int N = G::N;
+    double acc[1] = {0.0};
+    for (size_t li = (size_t)blockIdx.x * blockDim.x + threadIdx.x; li < G::N3; li += (size_t)gridDim.x * blockDim.x) {
+        const size_t idx = (li / ((size_t)N * N)) * G::PS + li % ((size_t)N * N);
+        const double ph = r[idx].x, ch = r[idx].y;
+        acc[0] += 0.5 * ph * ph * ch * ch;
+    }
+    __shared__ double sh[4];
+    block_sum<1, 256>(acc, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = acc[0];
+}
+
 // Compensated single-block sums of the adjoint reductions: R0 (P1 partials),
 // R1..R3 (P3 partials, width 3), R4 (P5 partials, when r5) and with GW
 // R5 (P1g), R6 (P3g), R7 (P5g, when r5).

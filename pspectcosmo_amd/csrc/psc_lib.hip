@@ -4,7 +4,6 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
-#include <complex>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -88,6 +87,7 @@ struct AdjWork {
     double* p1g = nullptr;
     double* p3g = nullptr;
     double* p5g = nullptr;
+    double* pobj = nullptr;           // partials of the GW objective
     hipGraphExec_t g_rec = nullptr, g_bwd = nullptr;   // full segment: recording replay, sweep
     double bytes = 0.0;
     std::vector<void*> owned;
@@ -473,6 +473,7 @@ struct Engine final : EngineBase {
     }
 
     // ---- adjoint helpers --------------------------------------------------
+    static constexpr int kObjBlocks = (int)(((size_t)N * N * (N / 2 + 1) + 255) / 256);
     int adj_workspace(psc_ctx* c, int K, int nseg, bool gw, AdjWork** out) {
         const int S = (int)c->w.size();
         AdjWork* cur = c->adj.get();
@@ -540,6 +541,7 @@ struct Engine final : EngineBase {
             w->p1g = static_cast<double*>(dalloc(sizeof(double) * G::BLOCKS_ROW));
             w->p3g = static_cast<double*>(dalloc(sizeof(double) * G::BLOCKS_COL));
             w->p5g = static_cast<double*>(dalloc(sizeof(double) * G::BLOCKS_ROW));
+            w->pobj = static_cast<double*>(dalloc(sizeof(double) * (kObjBlocks + 2)));
         }
         if (!ok) return c->fail(PSC_ERR_NOMEM, "adjoint: device allocation failed");
         c->adj = std::move(w);
@@ -988,116 +990,6 @@ __global__ void k_adj_init_chi2(const cpx<typename G::T>* __restrict__ Pf, cpx<t
     fbar[idx] = cpx<typename G::T>{(typename G::T)(((double)a.x - b.x) * inv), (typename G::T)(((double)a.y + b.y) * inv)};
 }
 
-// GW energy fraction at the end (the totals of psc_gw_spectrum,
-// output.cpp:1027-1060) and its gradient with respect to the packed pi_u pairs
-// (u11 + i u22, u33 + i u12, u13 + i u23; full spectra, host FP64): the
-// reference's half-spectrum visiting with weights 2 / 1 (output.cpp:800-1040),
-// the k = 0 term without projection, either projection.
-void gw_fraction_gradient(int N, int projection, const std::vector<double> pu[3], double* Q,
-                          std::vector<double> grad[3]) {
-    typedef std::complex<double> zc;
-    const size_t n3 = (size_t)N * N * N;
-    for (int q = 0; q < 3; ++q) grad[q].assign(2 * n3, 0.0);
-    auto full = [&](int i, int j, int k) {
-        return ((size_t)((i % N + N) % N) * N + (size_t)((j % N + N) % N)) * N + (size_t)((k % N + N) % N);
-    };
-    auto at = [&](int q, size_t i) { return zc(pu[q][2 * i], pu[q][2 * i + 1]); };
-    auto add = [&](int q, size_t i, zc v) {
-        grad[q][2 * i] += v.real();
-        grad[q][2 * i + 1] += v.imag();
-    };
-    static const int pr[6] = {0, 0, 1, 1, 2, 2};           // pair of component c
-    static const int ri[6] = {0, 1, 2, 0, 0, 1}, rj[6] = {0, 1, 2, 1, 2, 2};   // 11 22 33 12 13 23
-    // component c from its pair: even c = (P(k) + conj P(-k)) / 2, odd c = (P(k) - conj P(-k)) / 2i
-    auto unpack = [&](int c, size_t k, size_t mk) {
-        const zc x = at(pr[c], k), y = std::conj(at(pr[c], mk));
-        return (c % 2 == 0) ? 0.5 * (x + y) : zc(0, -0.5) * (x - y);
-    };
-    auto scatter = [&](int c, size_t k, size_t mk, zc g) {
-        const zc s = (c % 2 == 0) ? zc(0.5, 0) : zc(0, 0.5);
-        add(pr[c], k, s * g);
-        add(pr[c], mk, s * std::conj(g));
-    };
-    double total = 0.0;
-    auto mode = [&](int i, int j, int k, double wgt) {
-        const size_t kk = full(i, j, k), mk = full(-i, -j, -k);
-        const double kv[3] = {(double)(i <= N / 2 ? i : i - N), (double)(j <= N / 2 ? j : j - N), (double)k};
-        const double kn = std::sqrt(kv[0] * kv[0] + kv[1] * kv[1] + kv[2] * kv[2]);
-        double Pm[3][3];
-        for (int x = 0; x < 3; ++x)
-            for (int y = 0; y < 3; ++y) Pm[x][y] = (x == y ? 1.0 : 0.0) - kv[x] / kn * kv[y] / kn;
-        zc M[3][3], PM[3][3], Gm[3][3];
-        for (int c = 0; c < 6; ++c) M[ri[c]][rj[c]] = M[rj[c]][ri[c]] = unpack(c, kk, mk);
-        for (int x = 0; x < 3; ++x)
-            for (int y = 0; y < 3; ++y) {
-                PM[x][y] = 0;
-                for (int z = 0; z < 3; ++z) PM[x][y] += Pm[x][z] * M[z][y];
-            }
-        const zc tr = PM[0][0] + PM[1][1] + PM[2][2];
-        double q = 0.0;
-        if (projection == PSC_GW_PROJ_TT) {   // Lambda M = P M P - P tr(P M) / 2; gradient 2 Lambda(Lambda M)
-            zc A[3][3];
-            for (int x = 0; x < 3; ++x)
-                for (int y = 0; y < 3; ++y) {
-                    A[x][y] = 0;
-                    for (int z = 0; z < 3; ++z) A[x][y] += PM[x][z] * Pm[z][y];
-                    A[x][y] -= 0.5 * Pm[x][y] * tr;
-                    q += std::norm(A[x][y]);
-                }
-            zc trPA = 0;
-            for (int z = 0; z < 3; ++z)
-                for (int w2 = 0; w2 < 3; ++w2) trPA += Pm[z][w2] * A[w2][z];
-            for (int x = 0; x < 3; ++x)
-                for (int y = 0; y < 3; ++y) {
-                    zc v = 0;
-                    for (int z = 0; z < 3; ++z)
-                        for (int w2 = 0; w2 < 3; ++w2) v += Pm[x][z] * A[z][w2] * Pm[w2][y];
-                    Gm[x][y] = 2.0 * (v - 0.5 * trPA * Pm[x][y]);
-                }
-        } else {   // one-sided P M (output.cpp:819-831): |P M|^2 - |tr P M|^2 / 2
-            for (int x = 0; x < 3; ++x)
-                for (int y = 0; y < 3; ++y) q += std::norm(PM[x][y]);
-            q -= 0.5 * std::norm(tr);
-            for (int x = 0; x < 3; ++x)
-                for (int y = 0; y < 3; ++y) {
-                    zc v = 0;
-                    for (int z = 0; z < 3; ++z) v += Pm[x][z] * PM[z][y];
-                    Gm[x][y] = 2.0 * v - tr * Pm[x][y];
-                }
-        }
-        total += wgt * q;
-        for (int c = 0; c < 6; ++c) {
-            const int x = ri[c], y = rj[c];
-            scatter(c, kk, mk, wgt * (x == y ? Gm[x][x] : Gm[x][y] + Gm[y][x]));
-        }
-    };
-    const int H = N / 2;
-    for (int k = 0; k <= H; k += H) {
-        for (int i = 0; i <= H; i += H)
-            for (int j = 1; j < H; j++) mode(i, j, k, 2.0);
-        for (int j = 0; j <= H; j += H)
-            for (int i = 1; i < H; i++) mode(i, j, k, 2.0);
-        for (int i = 1; i < H; i++) {
-            for (int j = 1; j < H; j++) mode(i, j, k, 2.0);
-            for (int j = H + 1; j < N; j++) mode(i, j, k, 2.0);
-        }
-    }
-    for (int k = 1; k < H; k++)
-        for (int i = 0; i < N; i++)
-            for (int j = 0; j < N; j++) mode(i, j, k, 2.0);
-    for (int k = 0; k <= H; k += H)
-        for (int i = 0; i <= H; i += H)
-            for (int j = 0; j <= H; j += H)
-                if (i || j || k) mode(i, j, k, 1.0);
-    for (int c = 0; c < 6; ++c) {   // k = 0 (output.cpp:1032-1038)
-        const double wgt = ri[c] == rj[c] ? 1.0 : 2.0;
-        const zc v = unpack(c, 0, 0);
-        total += wgt * std::norm(v);
-        scatter(c, 0, 0, wgt * 2.0 * v);
-    }
-    *Q = total;
-}
-
 template <class G>
 int Engine<G>::adjoint(psc_ctx* c, const psc_adjoint_spec* sp, double* out, double* dchi) {
     const int M = sp->nsteps;
@@ -1168,30 +1060,25 @@ int Engine<G>::adjoint(psc_ctx* c, const psc_adjoint_spec* sp, double* out, doub
         if ((rc = read_state(c, 0, &se))) return rc;
         Diag de;
         if ((rc = diagnose(c, 0, se, &de))) return rc;
-        std::vector<double> pu[3], grad[3];
-        for (int q = 0; q < 3; ++q)
-            if ((rc = download(c, c->PU, q * c->B, pu[q]))) return rc;
-        double Q = 0.0;
-        gw_fraction_gradient(N, sp->projection, pu, &Q, grad);
         const StepConsts sc = consts_of(c);
         const double a = se.a, N3 = (double)G::N3;
         const double Kt = de.K[0] + de.K[1], Gt = de.G[0] + de.G[1], Vt = de.Vt[0] + de.Vt[1];
         const double rho = Kt + Gt + Vt;
         const double cgw = sc.f2m / (4.0 * N3 * N3 * std::pow(a, 6));
-        J = cgw * Q / rho;
-        for (int q = 0; q < 3; ++q) {
-            for (auto& v : grad[q]) v *= cgw / rho;
-            if ((rc = upload(c, w->pubar, q, grad[q]))) return rc;
-        }
-        HIPCHK(c, hipMemsetAsync(w->ubar, 0, 3 * arr, c->stream));
-        HIPCHK(c, hipMemsetAsync(w->fbar, 0, arr, c->stream));
+        hipLaunchKernelGGL((k_gw_objective<G>), dim3(kObjBlocks), dim3(256), 0, c->stream, P(c->PU), P(w->pubar),
+                           sp->projection, cgw / rho, w->pobj);
+        hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(64), 0, c->stream, w->pobj, kObjBlocks, 1, c->sums);
         // <phi^2 chi^2 / 2> from the recorded final real-space fields (dV/dg2)
         const int nsub_last = (M - (nseg - 1) * K) * S;
-        std::vector<double> r;
-        if ((rc = download(c, w->r[nsub_last], 0, r))) return rc;
-        double vg = 0.0;
-        for (size_t i = 0; i < G::N3; ++i) vg += 0.5 * r[2 * i] * r[2 * i] * r[2 * i + 1] * r[2 * i + 1];
-        vg /= N3;
+        hipLaunchKernelGGL((k_vg_sums<G>), dim3(256), dim3(256), 0, c->stream, P(w->r[nsub_last]), c->part);
+        hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(64), 0, c->stream, c->part, 256, 1, c->sums + 1);
+        double qv[2];
+        HIPCHK(c, hipMemcpyAsync(qv, c->sums, sizeof(qv), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipMemsetAsync(w->ubar, 0, 3 * arr, c->stream));
+        HIPCHK(c, hipMemsetAsync(w->fbar, 0, arr, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        J = cgw * qv[0] / rho;
+        const double vg = qv[1] / N3;
         // -J / rho times the gradient of rho = K + G + V at the final point: the
         // K and G parts and V' through one pass of the adjoint kernels
         const double cr = -J / rho;
