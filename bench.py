@@ -309,6 +309,10 @@ def bench_adjoint128(args):
     print(f"[bench] adjoint fp32: J {g['J']:.6e} dg2 {g['dg2']:.6e} dphi0 {g['dphi0']:.6e} "
           f"fwd {fwd * 1e3:.1f} ms, adjoint fwd {g['forward_ms']:.1f} + bwd {g['backward_ms']:.1f} ms",
           file=sys.stderr, flush=True)
+    if args.no_fd and args.no_gw:
+        print(json.dumps({"metric": "adjoint/forward time at 128^3, FP32-mixed", "value": g["backward_ms"] / (fwd * 1e3),
+                          "unit": "x", "profiling_run": True}))
+        return
     if args.no_fd:   # profiling runs: both objectives in FP32, no reference runs
         lgw = adjoint_case(PSC_FP32, tuned=tuned, gw=True)
         lgw.adjoint(steps, every, PHI0, DPHI0, want_field=False, objective=1, projection=1)
@@ -440,6 +444,7 @@ def main():
     ap.add_argument("--no-autotune", action="store_true", help="headline: keep the default shapes")
     ap.add_argument("--seq-steps", type=int, default=150, help="ensemble: steps per member run alone")
     ap.add_argument("--no-fd", action="store_true", help="adjoint: skip the finite-difference and FP64 runs")
+    ap.add_argument("--no-gw", action="store_true", help="adjoint profiling runs: <chi^2> objective only")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs the lattice on a single GPU; --gpus "
