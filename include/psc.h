@@ -168,6 +168,15 @@ int psc_get_k(psc_ctx* ctx, int member, double* f_k, double* pi_k);
 /* Real-space fields f and pi (c2r and /N^3, output.cpp:401-425). Either may be NULL. */
 int psc_get_fields(psc_ctx* ctx, int member, double* f, double* pi);
 
+/* Energy densities on the lattice as the snapshots write them, each N^3
+ * doubles x-major (the reference's [i][j][k]); any pointer may be NULL.
+ * pot: sum of the potential terms per site; kin: a^6/2 sum_f pi_f^2
+ * (kinetic_energy_lattice, evolution.cpp:7-19); grad: a^2/2 sum_f sum_d
+ * (d_d f)^2 with spectral first derivatives whose Nyquist component is zeroed
+ * (gradient_energy_lattice, evolution.cpp:23-138); tot = pot + kin + grad
+ * (snapshots, output.cpp:1614-1625).  The a-powers are the reference's. */
+int psc_energy_densities(psc_ctx* ctx, int member, double* pot, double* kin, double* grad, double* tot);
+
 /* Energies, replacing energy_density()/energy() (output.cpp:57-115,1069-1187).
  * out[PSC_ENERGY_LEN]: K_0, G_0, K_1, G_1, V_term0, V_term1, rho_total,
  * constraint violation |H^2 - rho/3| / |H^2 + rho/3| (output.cpp:1180-1184),

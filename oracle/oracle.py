@@ -55,6 +55,7 @@ def lib():
         L.orc_get_fields.argtypes = [C.c_void_p, _D, _D]
         L.orc_field_derivative.argtypes = [C.c_void_p, C.c_int, C.c_int, _D]
         L.orc_energies.argtypes = [C.c_void_p, _D]
+        L.orc_energy_densities.argtypes = [C.c_void_p, _D, _D, _D, _D]
         L.orc_spectra.argtypes = [C.c_void_p, C.c_int, _D, C.c_int]
         L.orc_gw_spectrum.argtypes = [C.c_void_p, C.c_int, _D, C.c_int, _D]
         L.orc_gw_omega_log.argtypes = [C.c_void_p, C.c_int, C.c_int, _D, _D]
@@ -232,6 +233,13 @@ class Oracle:
         e = np.zeros(PSC_ENERGY_LEN)
         lib().orc_energies(self._h, e)
         return e
+
+    def energy_densities(self):
+        """(pot, kin, grad, tot), each (N, N, N)."""
+        N = self.N
+        out = [np.zeros((N, N, N)) for _ in range(4)]
+        assert lib().orc_energy_densities(self._h, *[o.ravel() for o in out]) == 0
+        return tuple(out)
 
     def spectra(self, fld):
         nb = num_bins(self.N)

@@ -903,6 +903,77 @@ int orc_field_derivative(orc_ctx* c, int fl, int d, double* out) {
     return 0;
 }
 
+// Energy densities on the lattice written with each snapshot: the potential
+// per site summed over the terms and the total (output.cpp:1614-1625),
+// kinetic_energy_lattice (evolution.cpp:7-19: a^6/2 sum_f pi_f^2, the
+// reference's a-power) and gradient_energy_lattice (evolution.cpp:23-138:
+// a^2/2 sum_f sum_d (d_d f)^2 with each derivative a 1D transform along axis
+// d, multiplied by i kIR k for k < N/2, zero at the Nyquist index, back and /N).
+// Restated here line by line on the axis, independent of the 3D derivative
+// used elsewhere in the oracle.  Each output is N^3 doubles, x-major.
+int orc_energy_densities(orc_ctx* c, double* pot, double* kin, double* grad, double* tot) {
+    const int N = c->N;
+    const size_t nr = c->nr;
+    const double a = c->a, kir = c->cfg.kIR;
+    std::vector<double> Kd(nr, 0.0), Gd(nr, 0.0), Vd(nr, 0.0);
+    std::vector<std::vector<double>> f(c->cfg.nflds);
+    std::vector<double> r;
+    for (int fl = 0; fl < c->cfg.nflds; ++fl) {
+        c->to_real(c->f_k[fl], f[fl]);
+        c->to_real(c->pi_k[fl], r);
+        for (size_t i = 0; i < nr; ++i) Kd[i] += r[i] * r[i];
+    }
+    const double kn = std::pow(a, 6) / 2.0;
+    for (size_t i = 0; i < nr; ++i) Kd[i] *= kn;
+    const size_t stride[3] = {(size_t)N * N, (size_t)N, 1};
+    for (int fl = 0; fl < c->cfg.nflds; ++fl) {
+        const double* src = f[fl].data();
+        for (int d = 0; d < 3; ++d) {
+#pragma omp parallel
+            {
+                Fft1D ft(N);
+                std::vector<cd> line(N);
+#pragma omp for schedule(static)
+                for (long l = 0; l < (long)N * N; ++l) {
+                    // the line along axis d through the other two indices of l
+                    const size_t u = (size_t)(l / N), v = (size_t)(l % N);
+                    size_t start;
+                    if (d == 0) start = u * N + v;
+                    else if (d == 1) start = u * N * N + v;
+                    else start = u * N * N + v * N;
+                    for (int k = 0; k < N; ++k) line[k] = cd(src[start + k * stride[d]], 0.0);
+                    ft.exec(line.data(), -1);
+                    for (int k = 0; k < N; ++k) {
+                        const int sk = k < N / 2 ? k : (k == N / 2 ? 0 : k - N);
+                        line[k] *= cd(0.0, kir * sk);
+                    }
+                    ft.exec(line.data(), +1);
+                    for (int k = 0; k < N; ++k) {
+                        const double dv = line[k].real() / N;
+                        Gd[start + k * stride[d]] += dv * dv;
+                    }
+                }
+            }
+        }
+    }
+    const double gn = a * a / 2.0;
+    for (size_t i = 0; i < nr; ++i) Gd[i] *= gn;
+    std::vector<double> fp(c->cfg.nflds);
+    for (size_t i = 0; i < nr; ++i) {
+        for (int fl = 0; fl < c->cfg.nflds; ++fl) fp[fl] = f[fl][i];
+        double v = 0.0;
+        for (int term = 0; term < c->model.num_terms(); ++term) v += c->model.V_term(term, fp.data());
+        Vd[i] = v;
+    }
+    for (size_t i = 0; i < nr; ++i) {
+        if (pot) pot[i] = Vd[i];
+        if (kin) kin[i] = Kd[i];
+        if (grad) grad[i] = Gd[i];
+        if (tot) tot[i] = Vd[i] + Kd[i] + Gd[i];
+    }
+    return 0;
+}
+
 // energy_density + energy + meansvars (output.cpp:57-115, 380-461, 1069-1187),
 // laid out as psc_energies documents.
 int orc_energies(orc_ctx* c, double* out) {

@@ -41,7 +41,7 @@ def build(verbose=False):
     run = os.path.join(HERE, "psc_run")
     cxx = os.environ.get("CXX", "g++")
     cmd = [cxx, "-O2", "-std=c++17", "-Wall", "-o", run + ".tmp", os.path.join(CSRC, "psc_run.cpp"),
-           "-L" + HERE, "-lpsc", "-Wl,-rpath,$ORIGIN"]
+           "-L" + HERE, "-lpsc", "-ldl", "-Wl,-rpath,$ORIGIN"]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)

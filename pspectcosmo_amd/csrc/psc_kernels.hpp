@@ -567,6 +567,77 @@ namespace psc {
 
 __device__ __forceinline__ int deriv_index(int i, int N) { return i == N / 2 ? 0 : signed_index(i, N); }
 
+// Snapshot energy densities (snapshots, output.cpp:1614-1625), x-axis inverse
+// pass into a dense x-major N^3 double array D (the order of the reference's
+// f[fld][i][j][k]).  MODE 0: potential V(phi, chi) summed over the terms
+// (model.h:35-73), the coupling term times scale (1 with two fields, 0 with
+// one), with the k = 0 mode of src added back as in k_x_inverse_diag; MODE 1: scale * (pi_phi^2 + pi_chi^2), likewise;
+// MODE 2: D += scale * (d phi^2 + d chi^2) for one derivative axis (no k = 0
+// mode).  W and src are one member's arrays; the grid covers one member.
+template <class G, int MODE>
+__global__ void __launch_bounds__((G::THREADS_COL))
+k_x_density(const cpx<typename G::T>* __restrict__ W, double* __restrict__ D, const MemberState* __restrict__ ms,
+            const cpx<typename G::T>* __restrict__ tw, const cpx<typename G::T>* __restrict__ src, double scale) {
+    using T = typename G::T;
+    constexpr int N = G::N;
+    __shared__ cpx<T> lds[G::LayCol::size(0)];
+    const int c = threadIdx.x % G::TC, t = threadIdx.x / G::TC;
+    const size_t z = (size_t)blockIdx.x * G::TC + c;
+    const size_t o = blockIdx.y;
+    constexpr size_t stride = G::PS;
+    const size_t base = o * N + z + t * stride;
+    cpx<T> e[G::E];
+#pragma unroll
+    for (int m = 0; m < G::E; ++m) e[m] = W[base + (size_t)G::TPL * m * stride];
+    line_fft<T, N, typename G::Rad, +1, typename G::LayCol>(e, t, c, lds, tw);
+    const double inv = 1.0 / (double)G::N3;
+    double pb = 0.0, qb = 0.0;
+    if constexpr (MODE != 2) {
+        const cpx<T> dc = src[0];
+        pb = (double)dc.x * inv;
+        qb = (double)dc.y * inv;
+    }
+    const double m2 = ms[0].m2, g2 = ms[0].g2;
+#pragma unroll
+    for (int m = 0; m < G::E; ++m) {
+        const double u = pb + (double)e[m].x * inv, v = qb + (double)e[m].y * inv;
+        const size_t x = (size_t)(t + G::TPL * m);
+        double* d = D + (x * N + o) * N + z;
+        if constexpr (MODE == 0) *d = 0.5 * m2 * (u * u) + scale * (0.5 * g2 * (u * u) * (v * v));
+        else if constexpr (MODE == 1) *d = scale * (u * u + v * v);
+        else *d += scale * (u * u + v * v);
+    }
+}
+
+// Snapshot gradient density, first pass per axis: the spectral derivative
+// i kIR n_d P_f along axis d (n_d = 0 at the Nyquist index, the 1D transforms
+// of gradient_energy_lattice, evolution.cpp:23-138, restated in 3D) formed
+// on the fly, then the z-axis inverse transform into W.  One member.
+template <class G>
+__global__ void __launch_bounds__((G::THREADS_ROW))
+k_z_inverse_deriv(const cpx<typename G::T>* __restrict__ Pf, cpx<typename G::T>* __restrict__ W,
+                  const cpx<typename G::T>* __restrict__ tw, int d, double kIR) {
+    using T = typename G::T;
+    constexpr int N = G::N;
+    __shared__ cpx<T> lds[G::LayRow::size(G::L)];
+    const int t = threadIdx.x % G::TPL, c = threadIdx.x / G::TPL;
+    const size_t line = (size_t)blockIdx.x * G::L + c;
+    const size_t base = (line / N) * G::PS + (line % N) * N + t;
+    const int ix = (int)(line / N), iy = (int)(line % N);
+    cpx<T> e[G::E];
+#pragma unroll
+    for (int m = 0; m < G::E; ++m) {
+        const int n = d == 0 ? deriv_index(ix, N) : (d == 1 ? deriv_index(iy, N) : deriv_index(t + G::TPL * m, N));
+        const T k = (T)(kIR * n);
+        const cpx<T> f = Pf[base + G::TPL * m];
+        e[m] = cpx<T>{-k * f.y, k * f.x};
+    }
+    line_fft<T, N, typename G::Rad, +1, typename G::LayRow>(e, t, c, lds, tw);
+#pragma unroll
+    for (int m = 0; m < G::E; ++m) W[base + G::TPL * m] = e[m];
+}
+
+
 // A (GW): drift of fields and u, then z-axis inverse of the fields and of the
 // three gradient spectra i kIR n_d P_f.
 template <class G, bool DRIFT, bool REC = false>

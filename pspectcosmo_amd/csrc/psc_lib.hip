@@ -220,6 +220,9 @@ struct EngineBase {
     virtual int adjoint(psc_ctx* c, const psc_adjoint_spec* sp, double* out, double* dchi) = 0;
     // Forward 3D transform of the packed real-space lattice in W[member] into dst[member].
     virtual int to_spectrum(psc_ctx* c, int member, void* dst) = 0;
+    // Snapshot energy density `kind` (0 potential, 1 kinetic, 2 gradient) of one
+    // member into the dense N^3 device array D; scale as k_x_density takes it.
+    virtual int density(psc_ctx* c, int member, int kind, double* D, double scale) = 0;
 };
 
 template <class G>
@@ -468,6 +471,36 @@ struct Engine final : EngineBase {
         if (rc) return rc;
         hipLaunchKernelGGL((k_z_forward<G>), dim3(G::BLOCKS_ROW, 1), dim3(G::THREADS_ROW), 0, c->stream, W,
                            static_cast<cpx<T>*>(dst) + off, P(c->tw));
+        HIPCHK(c, hipGetLastError());
+        return PSC_OK;
+    }
+
+    int density(psc_ctx* c, int member, int kind, double* D, double scale) override {
+        const size_t off = (size_t)member * G::MS;
+        cpx<T>* W = P(c->W) + off;
+        const cpx<T>* tw = P(c->tw);
+        dim3 cgrid(N / G::TC, N, 1), cblock(G::THREADS_COL);
+        int rc;
+        if (kind < 2) {
+            const cpx<T>* src = P(kind == 0 ? c->Pf : c->Pp) + off;
+            if ((rc = launch_z_inverse(c, false, src, nullptr, nullptr, W, c->ms + member, 1))) return rc;
+            if ((rc = launch_y<+1>(c, W, 1))) return rc;
+            if (kind == 0)
+                hipLaunchKernelGGL((k_x_density<G, 0>), cgrid, cblock, 0, c->stream, W, D, c->ms + member, tw,
+                                   src, scale);
+            else
+                hipLaunchKernelGGL((k_x_density<G, 1>), cgrid, cblock, 0, c->stream, W, D, c->ms + member, tw,
+                                   src, scale);
+        } else {
+            HIPCHK(c, hipMemsetAsync(D, 0, G::N3 * sizeof(double), c->stream));
+            for (int d = 0; d < 3; ++d) {
+                hipLaunchKernelGGL((k_z_inverse_deriv<G>), dim3(G::BLOCKS_ROW, 1), dim3(G::THREADS_ROW), 0,
+                                   c->stream, P(c->Pf) + off, W, tw, d, c->cfg.kIR);
+                if ((rc = launch_y<+1>(c, W, 1))) return rc;
+                hipLaunchKernelGGL((k_x_density<G, 2>), cgrid, cblock, 0, c->stream, W, D, c->ms + member, tw,
+                                   nullptr, scale);
+            }
+        }
         HIPCHK(c, hipGetLastError());
         return PSC_OK;
     }
@@ -1485,6 +1518,40 @@ int psc_get_fields(psc_ctx* c, int member, double* f, double* pi) {
         for (size_t i = 0; i < c->N3; ++i) {
             out[i] = full[2 * i];
             if (c->cfg.nflds > 1) out[c->N3 + i] = full[2 * i + 1];
+        }
+    }
+    return PSC_OK;
+}
+
+int psc_energy_densities(psc_ctx* c, int member, double* pot, double* kin, double* grad, double* tot) {
+    int rc = check_member(c, member);
+    if (rc) return rc;
+    MemberState s;
+    if ((rc = read_state(c, member, &s))) return rc;
+    const size_t n3 = c->N3;
+    double* D = nullptr;
+    HIPCHK(c, hipMalloc(&D, n3 * sizeof(double)));
+    std::unique_ptr<double, void (*)(double*)> hold(D, [](double* p) { (void)hipFree(p); });
+    std::vector<double> tmp;
+    double* outs[3] = {pot, kin, grad};
+    // potential (coupling term only with two fields), a^6/2 sum pi^2,
+    // a^2/2 sum |grad f|^2 (evolution.cpp:7-19, 135-137)
+    const double scales[3] = {c->cfg.nflds == 2 ? 1.0 : 0.0, std::pow(s.a, 6) / 2.0, s.a * s.a / 2.0};
+    for (int kind = 0; kind < 3; ++kind) {
+        double* dst = outs[kind];
+        if (!dst && !tot) continue;
+        if (!dst) {
+            tmp.resize(n3);
+            dst = tmp.data();
+        }
+        if ((rc = c->eng->density(c, member, kind, D, scales[kind]))) return rc;
+        HIPCHK(c, hipMemcpyAsync(dst, D, n3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        if (tot) {  // tot = pot + kin + grad in that order (output.cpp:1623)
+            if (kind == 0)
+                std::memcpy(tot, dst, n3 * sizeof(double));
+            else
+                for (size_t i = 0; i < n3; ++i) tot[i] += dst[i];
         }
     }
     return PSC_OK;

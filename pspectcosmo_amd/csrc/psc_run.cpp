@@ -7,7 +7,12 @@
 //   average_spectra_times.txt, spectra_gws.txt, energy_gws.txt
 //                                            every `infreq` (save_Infreq, output.cpp:1842-1848)
 //   info.txt                                 at the end (output_parameters, output.cpp:1750)
-// snapshots of the real-space fields every `snap` (save_Snap, output.cpp:1850-1853),
+// snapshots every `snap` (save_Snap, output.cpp:1850-1853): HDF5 files
+// Snapshots/data_<n>_snapshot.h5 with /ScalarField_<f> and, with
+// snap_energy = 1 (SNAPSHOTS_ENERGY, parameters.h:14), /PotentialEnergy,
+// /KineticEnergy, /GradientEnergy, /TotalEnergy, all N x N x N native doubles
+// (snapshots, output.cpp:1541-1648); the HDF5 C library is loaded when the
+// first snapshot is due (snap_format = raw writes plain float64 files instead),
 // and the lattice image grid.img (checkpoint, output.cpp:1652-1703) with
 // every spectra output and at the end when checkpoint = 1; continue_run = 1
 // resumes from it (initialize.cpp:590-616).
@@ -35,6 +40,7 @@
 #include <vector>
 
 #include "../../include/psc.h"
+#include "psc_hdf5.hpp"
 
 namespace {
 
@@ -95,6 +101,9 @@ const char* kKeys[][2] = {
     {"use_graph", "1: replay captured steps"},
     {"autotune", "1: time the launch shapes before the run"},
     {"dir", "output directory (default output)"},
+    {"snap_format", "h5 (the reference's HDF5 snapshots) or raw (float64 files)"},
+    {"snap_energy", "1: snapshots also hold the energy densities (SNAPSHOTS_ENERGY)"},
+    {"hdf5_lib", "path of libhdf5.so (default: the loader's search, then /opt/conda/lib)"},
     {"checkpoint", "1: write <dir>/grid.img with the spectra and at the end"},
     {"continue_run", "1: resume from <dir>/grid.img when it exists"},
 };
@@ -265,26 +274,48 @@ struct Run {
         }
     }
 
-    // Snapshot of the real-space fields.  The reference writes HDF5
-    // (snapshots, output.cpp:1541-1648); with no HDF5 library available this
-    // writes each dataset the reference names as raw little-endian float64
-    // N^3 arrays in C order, plus a text index.
+    // Snapshot (snapshots, output.cpp:1541-1648): the real-space fields and,
+    // with snap_energy, the four energy densities, as N^3 doubles x-major.
+    // HDF5 by default; snap_format = raw writes each dataset as a little-endian
+    // float64 file plus a text index.
+    bool snap_h5 = true, snap_energy = true;
+    std::string hdf5_lib;
+    psc_io::Hdf5 h5;
     void save_snap(int number) {
         const std::string sdir = dir + "/Snapshots";
         mkdir(sdir.c_str(), 0755);
         const size_t n3 = (size_t)cfg.N * cfg.N * cfg.N;
-        std::vector<double> f(n3 * cfg.nflds), p(n3 * cfg.nflds);
-        check(psc_get_fields(ctx, 0, f.data(), p.data()), "psc_get_fields");
+        std::vector<double> f(n3 * cfg.nflds);
+        check(psc_get_fields(ctx, 0, f.data(), nullptr), "psc_get_fields");
+        std::vector<std::pair<std::string, const double*>> sets;
+        for (int q = 0; q < cfg.nflds; ++q) sets.push_back({"/ScalarField_" + std::to_string(q), f.data() + q * n3});
+        std::vector<double> en;
+        if (snap_energy) {
+            en.resize(4 * n3);
+            check(psc_energy_densities(ctx, 0, en.data(), en.data() + n3, en.data() + 2 * n3, en.data() + 3 * n3),
+                  "psc_energy_densities");
+            const char* names[4] = {"/PotentialEnergy", "/KineticEnergy", "/GradientEnergy", "/TotalEnergy"};
+            for (int i = 0; i < 4; ++i) sets.push_back({names[i], en.data() + i * n3});
+        }
+        const std::string base = sdir + "/data_" + std::to_string(number) + "_snapshot";
+        if (snap_h5) {
+            std::string err = h5.load(hdf5_lib);
+            if (err.empty()) err = h5.write(base + ".h5", cfg.N, sets);
+            if (!err.empty()) {
+                std::fprintf(stderr, "psc_run: snapshot: %s\n", err.c_str());
+                std::exit(2);
+            }
+            return;
+        }
         double sc[4];
         check(psc_get_scalars(ctx, 0, sc), "psc_get_scalars");
-        const std::string base = sdir + "/data_" + std::to_string(number) + "_snapshot";
         std::ofstream idx(base + ".txt");
         idx << "t " << std::setprecision(17) << sc[0] << "\na " << sc[1] << "\nN " << cfg.N << "\n";
-        for (int q = 0; q < cfg.nflds; ++q) {
-            const std::string name = "ScalarField_" + std::to_string(q);
+        for (auto& d : sets) {
+            const std::string name = d.first.substr(1);
             std::ofstream out(base + "." + name + ".f64", std::ios::binary);
-            out.write(reinterpret_cast<const char*>(f.data() + q * n3), n3 * sizeof(double));
-            idx << "/" << name << " float64 " << cfg.N << "x" << cfg.N << "x" << cfg.N << " " << name << ".f64\n";
+            out.write(reinterpret_cast<const char*>(d.second), n3 * sizeof(double));
+            idx << d.first << " float64 " << cfg.N << "x" << cfg.N << "x" << cfg.N << " " << name << ".f64\n";
         }
     }
 
@@ -410,6 +441,14 @@ int main(int argc, char** argv) {
     r.ic.seed = (int)p.num("seed", 1);
     r.dir = p.get("dir", "output");
     r.gw_proj = p.get("gw_projection", "ref") == "tt" ? PSC_GW_PROJ_TT : PSC_GW_PROJ_REFERENCE;
+    const std::string snap_format = p.get("snap_format", "h5");
+    if (snap_format != "h5" && snap_format != "raw") {
+        std::fprintf(stderr, "psc_run: snap_format must be h5 or raw\n");
+        return 2;
+    }
+    r.snap_h5 = snap_format == "h5";
+    r.snap_energy = (int)p.num("snap_energy", 1) != 0;
+    r.hdf5_lib = p.get("hdf5_lib", "");
     const double kmax = c.kIR * c.N * std::sqrt(3.0) / 2.0;
     auto calculate_dt = [&](double a) { return 2.0 / (kmax / a) / 10; };   // pspectcosmo.cpp:213-233
     r.auto_dt = p.get("dt", "auto") == "auto";

@@ -55,6 +55,7 @@ def lib():
         L.psc_get_k.argtypes = [vp, C.c_int, _D, _D]
         L.psc_get_fields.argtypes = [vp, C.c_int, _D, _D]
         L.psc_energies.argtypes = [vp, C.c_int, _D]
+        L.psc_energy_densities.argtypes = [vp, C.c_int, _D, _D, _D, _D]
         L.psc_spectra.argtypes = [vp, C.c_int, C.c_int, _D, C.c_int]
         L.psc_load_gw_k.argtypes = [vp, C.c_int, _D, _D]
         L.psc_get_gw_k.argtypes = [vp, C.c_int, _D, _D]
@@ -83,7 +84,8 @@ def exported_symbols():
             "psc_get_fields", "psc_energies", "psc_spectra", "psc_load_gw_k", "psc_get_gw_k",
             "psc_gw_spectrum", "psc_gw_omega_log", "psc_num_bins", "psc_set_kernel_timing", "psc_get_kernel_times",
             "psc_reset_kernel_times", "psc_device_bytes", "psc_autotune", "psc_shape_name",
-            "psc_list_shapes", "psc_set_shape", "psc_adjoint", "psc_set_dt", "psc_means", "psc_load_fields"]
+            "psc_list_shapes", "psc_set_shape", "psc_adjoint", "psc_set_dt", "psc_means", "psc_load_fields",
+            "psc_energy_densities"]
 
 
 def generate_ic(cfg, model, ic):
@@ -186,6 +188,13 @@ class Lattice:
         p = np.zeros_like(f)
         self._check(lib().psc_get_fields(self._h, member, f.ravel(), p.ravel()))
         return f, p
+
+    def energy_densities(self, member=0):
+        """Snapshot energy densities (pot, kin, grad, tot), each (N, N, N)."""
+        N = self.N
+        out = [np.zeros((N, N, N)) for _ in range(4)]
+        self._check(lib().psc_energy_densities(self._h, member, *[o.ravel() for o in out]))
+        return tuple(out)
 
     def energies(self, member=0):
         e = np.zeros(PSC_ENERGY_LEN)

@@ -43,3 +43,10 @@ def test_auto_dt_is_the_default(tmp_path):
 def test_unknown_parameter_is_an_error():
     r = run("Nx=64", "--dry-run")
     assert r.returncode == 2 and "unknown parameter Nx" in r.stderr
+
+
+def test_snapshot_format_is_checked():
+    r = run("snap_format=netcdf", "--dry-run")
+    assert r.returncode == 2 and "snap_format" in r.stderr
+    for fmt in ("h5", "raw"):
+        assert run(f"snap_format={fmt}", "--dry-run").returncode == 0
