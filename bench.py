@@ -135,20 +135,19 @@ def canonical_gw_bytes_per_site_step(s_r, n, substeps=3, nflds=2):
 
 
 def bench_gw512(args):
-    from pspectcosmo_amd.lattice import generate_gw_ic
     n = 512
     model = make_model(1.0, 1e5)
     ic = make_ic(f0=(1.0, 0.0), fd0=(-0.7, 0.0), seed=1)
     cfg = make_config(n, nflds=2, integrator=PSC_Y4, precision=PSC_FP32, kIR=KIR, dt=0.001,
                       with_gw=1, use_graph=0 if args.no_graph else 1, autotune=1 if args.autotune else 0)
-    fk, pk, sc = generate_ic(cfg, model, ic)
-    u, pu = generate_gw_ic(cfg, model, ic)
-    print("[bench] gw512 ICs ready", file=sys.stderr, flush=True)
     lat = Lattice(cfg)
     lat.set_model(0, model)
-    lat.load_ic_k(0, fk, pk, sc[0], sc[1])
-    lat.load_gw_k(0, u, pu)
-    del fk, pk, u, pu
+    t0 = time.perf_counter()
+    lat.generate_ic_device(0, ic)     # scalar and GW modes generated on the device
+    lat.generate_gw_ic_device(0, ic)
+    device_sync(lat)
+    print(f"[bench] gw512 ICs ready ({(time.perf_counter() - t0) * 1e3:.0f} ms on the device)", file=sys.stderr,
+          flush=True)
     if args.autotune:
         print("[bench] autotune", lat.autotune(2), file=sys.stderr, flush=True)
     steps = args.steps
@@ -180,14 +179,25 @@ def bench_ensemble128(args):
                       use_graph=0 if args.no_graph else 1, autotune=1 if args.autotune else 0)
     lat = Lattice(cfg)
     ics = []
+    # every member's initial modes generated on the device from its own seed
+    # (one untimed call first: the first launch loads the kernels)
+    lat.set_model(0, make_model(1.0, members[0]["g2"]))
+    lat.generate_ic_device(0, make_ic(f0=(members[0]["phi0"], 0.0), fd0=(-0.7, 0.0), seed=members[0]["seed"]))
+    device_sync(lat)
+    t0 = time.perf_counter()
     for m in members:
         model = make_model(1.0, m["g2"])
         ic = make_ic(f0=(m["phi0"], 0.0), fd0=(-0.7, 0.0), seed=m["seed"])
-        fk, pk, sc = generate_ic(cfg, model, ic)
         lat.set_model(m["member"], model)
-        lat.load_ic_k(m["member"], fk, pk, sc[0], sc[1])
-        ics.append((model, fk, pk, sc))
-    print("[bench] ensemble ICs ready", file=sys.stderr, flush=True)
+        lat.generate_ic_device(m["member"], ic)
+        ics.append((model, ic))
+    device_sync(lat)
+    ic_device_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    generate_ic(cfg, ics[0][0], ics[0][1])   # the host generator, one member, for comparison
+    ic_host_s = time.perf_counter() - t0
+    print(f"[bench] ensemble ICs ready: device {ic_device_s * 1e3:.1f} ms for {B} members, host "
+          f"{ic_host_s * 1e3:.1f} ms for one", file=sys.stderr, flush=True)
     if args.autotune:
         print("[bench] autotune", lat.autotune(3), file=sys.stderr, flush=True)
     steps = args.steps
@@ -209,9 +219,9 @@ def bench_ensemble128(args):
     if args.autotune:
         one.set_shape(shape)
     seq = 0.0
-    for model, fk, pk, sc in ics:
+    for model, ic in ics:
         one.set_model(0, model)
-        one.load_ic_k(0, fk, pk, sc[0], sc[1])
+        one.generate_ic_device(0, ic)
         one.step(2)
         device_sync(one)
         t0 = time.perf_counter()
@@ -230,6 +240,8 @@ def bench_ensemble128(args):
                                            "shape": shape},
            "sequential": {"value": vseq, "steps_per_member": seq_steps},
            "batched_over_sequential": v / vseq,
+           "initial_conditions": {"device_ms_all_members": ic_device_s * 1e3,
+                                  "host_ms_one_member": ic_host_s * 1e3},
            "canonical_bytes_per_site_step": b, "step_hbm_frac_canonical": v * b / (HBM_PEAK_GBS * 1e9),
            "kernels": per_kernel}
     print(json.dumps(out))

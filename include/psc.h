@@ -142,6 +142,16 @@ int psc_set_model(psc_ctx* ctx, int member, const psc_model* model);
  * initialize.cpp:797 does with kernel_a_k_nopad (evolution.cpp:1061-1090). */
 int psc_load_ic_k(psc_ctx* ctx, int member, const double* f_k, const double* pi_k,
                   double a, double ad);
+/* Initial conditions generated on the device straight into member `member`:
+ * the modes of psc_generate_ic (initialize(), initialize.cpp:562-810, in its
+ * visiting order with the same Park-Miller draws; the model set with
+ * psc_set_model), then the state as psc_load_ic_k leaves it.  Values agree
+ * with psc_generate_ic to within a few units in the last place (the device's
+ * log, sin and cos).  scalars_out (may be NULL) as psc_generate_ic. */
+int psc_generate_ic_device(psc_ctx* ctx, int member, const psc_ic_params* ic, double* scalars_out);
+/* GW modes on the device, as psc_generate_gw_ic followed by psc_load_gw_k
+ * (initialize_GW, initialize.cpp:341-470). */
+int psc_generate_gw_ic_device(psc_ctx* ctx, int member, const psc_ic_params* ic);
 /* Load a member from real-space fields and momenta (nflds x N^3 each, the
  * layout of pspectcosmo.h:63) and t, a, da/dt, as the continue_run branch
  * of the reference reads its lattice image (initialize.cpp:590-616); the

@@ -11,6 +11,7 @@
 #include <thread>
 #include <vector>
 #include "../../include/psc.h"
+#include "psc_ic_modes.hpp"
 
 // Bit-exact agreement with other IEEE restatements of initialize(): no FMA
 // contraction in this file, and the phase's sine and cosine from one glibc
@@ -65,27 +66,10 @@ class Rng {
     bool warm_ = false;
 };
 
-struct Pot {
-    int nflds;
-    double m2, g2;
-    int terms() const { return nflds == 2 ? 2 : 1; }
-    double term(int t, const double* f) const {
-        return t == 0 ? 0.5 * m2 * pw2(f[0]) : 0.5 * g2 * pw2(f[0]) * pw2(f[1]);
-    }
-    void meff(double* m, const double* f) const {
-        m[0] = m2 + (nflds == 2 ? g2 * pw2(f[1]) : 0.0);
-        if (nflds == 2) m[1] = g2 * pw2(f[0]);
-    }
-};
 
-// Number of modes visited per field in the order of initialize.cpp:685-749
-// (shared with initialize_GW, initialize.cpp:355-428): the kz = 0 and N/2
-// planes (half of each plane, conjugates filled alongside), the interior
-// planes 0 < kz < N/2, then the 8 real corners (7 without the DC mode).
-uint64_t mode_count(int N, bool dc) {
-    const uint64_t h = N / 2 - 1;
-    return 2 * (4 * h + 2 * h * h) + h * (uint64_t)N * N + (dc ? 8 : 7);
-}
+using psc_ic::mode_count;
+using psc_ic::Pot;
+using psc_ic::background;
 
 // Visit the modes in that order; each mode consumes 4 draws from the stream
 // `rng` handed to fn(rng, i, j, k, conjugate index or -1, isReal).  The
@@ -136,39 +120,6 @@ bool valid(const psc_config* c) {
            c->nflds <= 2 && c->kIR > 0 && c->fStar > 0 && c->MPl > 0;
 }
 
-void background(const psc_config& c, const Pot& V, const psc_ic_params& ic, double* a, double* ad,
-                double* addp, double* H) {
-    double f0[2] = {ic.f0[0], c.nflds > 1 ? ic.f0[1] : 0.0};
-    double fdsq = 0.0, pot = 0.0;
-    for (int q = 0; q < c.nflds; ++q) fdsq += pw2(ic.fd0[q]);
-    for (int t = 0; t < V.terms(); ++t) pot += V.term(t, f0);
-    double h = (fdsq / 2.0 + pot) * pw2(c.fStar / c.MPl) / 3.0;
-    *H = std::sqrt(h);
-    *a = ic.a0;
-    *ad = *H * ic.a0;
-    *addp = pw2(c.fStar / c.MPl) / 3.0 * ((c.alpha - 2) * fdsq + (c.alpha + 1) * pot) * ic.a0;
-    if (c.expansion == PSC_EXPANSION_STATIC) *H = *ad = *addp = 0.0;
-}
-
-// orthogonal_unit_vectors / TT_tensors (initialize.cpp:66-122).
-void polarisation(const double k[3], double e[3][3]) {
-    const double kn = std::sqrt(k[0] * k[0] + k[1] * k[1] + k[2] * k[2]);
-    const double h[3] = {k[0] / kn, k[1] / kn, k[2] / kn};
-    const bool z = std::fabs(h[0]) < 1e-9 && std::fabs(h[1]) < 1e-9 && std::fabs(h[2] - 1.0) < 1e-9;
-    const double ref[3] = {0.0, z ? 1.0 : 0.0, z ? 0.0 : 1.0};
-    double u[3] = {h[1] * ref[2] - h[2] * ref[1], h[2] * ref[0] - h[0] * ref[2], h[0] * ref[1] - h[1] * ref[0]};
-    double un = std::sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
-    for (double& x : u) x /= un;
-    double v[3] = {h[1] * u[2] - h[2] * u[1], h[2] * u[0] - h[0] * u[2], h[0] * u[1] - h[1] * u[0]};
-    double vn = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
-    for (double& x : v) x /= vn;
-    for (int i = 0; i < 3; ++i)
-        if (u[i] < 0) { for (double& x : u) x = -x; break; }
-    for (int i = 0; i < 3; ++i)
-        if (v[i] < 0) { for (double& x : v) x = -x; break; }
-    for (int i = 0; i < 3; ++i)
-        for (int j = 0; j < 3; ++j) e[i][j] = u[i] * u[j] - v[i] * v[j] + u[i] * v[j] + v[i] * u[j];
-}
 
 }  // namespace
 
@@ -253,7 +204,7 @@ extern "C" int psc_generate_gw_ic(const psc_config* cfg, const psc_model* model,
         const int px = sidx(i, N), py = sidx(j, N);
         const double kv[3] = {px * cfg->kIR, py * cfg->kIR, k * cfg->kIR};
         double e[3][3];
-        polarisation(kv, e);
+        psc_ic::polarisation(kv, e);
         const double kk = std::sqrt(pw2(cfg->kIR) * (pw2(px) + pw2(py) + pw2(k)));
         const double rms = norm / std::sqrt(kk);
         const double al = rms / std::sqrt(2.0) * std::sqrt(std::log(1.0 / rng.uniform())) / a;

@@ -18,6 +18,7 @@
 #include "../../include/psc.h"
 #include "psc_kernels.hpp"
 #include "psc_adjoint.hpp"
+#include "psc_ic_device.hpp"
 
 using namespace psc;
 
@@ -1415,6 +1416,119 @@ int psc_load_ic_k(psc_ctx* c, int member, const double* f_k, const double* pi_k,
     s.add = kernel_a_value(c, d, a);  // initialize.cpp:797
     if ((rc = write_state(c, member, &s))) return rc;
     c->loaded[member] = 1;
+    return PSC_OK;
+}
+
+namespace {
+
+// Constants of one device generation call: psc_generate_ic's background and
+// vacuum normalisation (initialize.cpp:624-668, 685-700) or, for the GW
+// modes, initialize_GW's (initialize.cpp:341-360).  sc = a, da/dt, printed
+// d2a/dt2, H.
+int ic_setup(psc_ctx* c, int member, const psc_ic_params* ic, bool gw, IcConsts* k, double sc[4]) {
+#pragma clang fp contract(off)
+    const psc_config& cfg = c->cfg;
+    const psc_model& mdl = c->models[member];
+    const psc_ic::Pot V{cfg.nflds, mdl.m2, mdl.g2};
+    double a, ad, addp, H;
+    psc_ic::background(cfg, V, *ic, &a, &ad, &addp, &H);
+    if (!(H >= 0.0)) return c->fail(PSC_ERR_ARG, "negative H^2 from the initial values");
+    const int N = c->N;
+    const double L = 2.0 * 3.14159265358979323846264338327950288 / cfg.kIR, dx = L / N;
+    std::memset(k, 0, sizeof(*k));
+    k->N = N;
+    k->nflds = cfg.nflds;
+    k->seed = ic->seed;
+    k->a = a;
+    k->ap = std::pow(a, cfg.alpha - 3);
+    k->H = H;
+    k->N3 = (double)N * N * N;
+    k->kIR = cfg.kIR;
+    k->dp2 = cfg.kIR * cfg.kIR;
+    const unsigned long long M = psc_ic::mode_count(N, true);
+    if (!gw) {
+        k->norm = cfg.omegaStar / cfg.fStar * std::pow(N / dx, 1.5) / std::sqrt(2.0);
+        const double f0[2] = {ic->f0[0], cfg.nflds > 1 ? ic->f0[1] : 0.0};
+        V.meff(k->meff, f0);
+        for (int q = 0; q < cfg.nflds; ++q) {
+            k->first[q] = 4 * (unsigned long long)q * M;
+            k->f0[q] = ic->f0[q];
+            k->fd0[q] = ic->fd0[q];
+        }
+    } else {
+        k->norm = cfg.MPl * cfg.omegaStar / (cfg.fStar * cfg.fStar) * std::pow(N / dx, 1.5);
+        k->first[0] = 4 * M * (unsigned long long)cfg.nflds;   // after every scalar mode's draws
+    }
+    sc[0] = a;
+    sc[1] = ad;
+    sc[2] = addp;
+    sc[3] = H;
+    return PSC_OK;
+}
+
+}  // namespace
+
+int psc_generate_ic_device(psc_ctx* c, int member, const psc_ic_params* ic, double* scalars_out) {
+    int rc = check_member(c, member);
+    if (rc) return rc;
+    if (!ic || !(ic->a0 > 0)) return c->fail(PSC_ERR_ARG, "bad initial-condition parameters");
+    IcConsts k;
+    double sc[4];
+    if ((rc = ic_setup(c, member, ic, false, &k, sc))) return rc;
+    const unsigned long long count = psc_ic::mode_count(c->N, true);
+    const dim3 grid((unsigned)((count + 255) / 256)), block(256);
+    const size_t off = (size_t)member * c->MS, PS = plane_stride(c->N);
+    if (c->cfg.precision == PSC_FP32)
+        hipLaunchKernelGGL(k_ic_fields<float>, grid, block, 0, c->stream, static_cast<cpx<float>*>(c->Pf) + off,
+                           static_cast<cpx<float>*>(c->Pp) + off, k, PS, count);
+    else
+        hipLaunchKernelGGL(k_ic_fields<double>, grid, block, 0, c->stream, static_cast<cpx<double>*>(c->Pf) + off,
+                           static_cast<cpx<double>*>(c->Pp) + off, k, PS, count);
+    HIPCHK(c, hipGetLastError());
+    // the member's state as psc_load_ic_k sets it
+    MemberState s;
+    std::memset(&s, 0, sizeof(s));
+    s.a = s.a_start = s.a_half = sc[0];
+    s.ad = s.b = (c->cfg.expansion == PSC_EXPANSION_STATIC) ? 0.0 : sc[1];
+    s.m2 = c->models[member].m2;
+    s.g2 = c->models[member].g2;
+    if ((rc = write_state(c, member, &s))) return rc;
+    Diag d;
+    if ((rc = diagnose(c, member, s, &d))) return rc;
+    s.add = kernel_a_value(c, d, sc[0]);  // initialize.cpp:797
+    if ((rc = write_state(c, member, &s))) return rc;
+    c->loaded[member] = 1;
+    if (scalars_out)
+        for (int i = 0; i < 4; ++i) scalars_out[i] = sc[i];
+    return PSC_OK;
+}
+
+int psc_generate_gw_ic_device(psc_ctx* c, int member, const psc_ic_params* ic) {
+    int rc = check_member(c, member);
+    if (rc) return rc;
+    if (!c->cfg.with_gw) return c->fail(PSC_ERR_STATE, "context created without GW evolution");
+    if (!ic || !(ic->a0 > 0)) return c->fail(PSC_ERR_ARG, "bad initial-condition parameters");
+    IcConsts k;
+    double sc[4];
+    if ((rc = ic_setup(c, member, ic, true, &k, sc))) return rc;
+    const size_t arr = c->MS * c->es, astride = (size_t)c->B * c->MS;
+    for (int a = 0; a < 3; ++a) {   // the DC mode is not generated and stays zero
+        const size_t o = ((size_t)a * c->B + member) * arr;
+        HIPCHK(c, hipMemsetAsync(static_cast<char*>(c->U) + o, 0, arr, c->stream));
+        HIPCHK(c, hipMemsetAsync(static_cast<char*>(c->PU) + o, 0, arr, c->stream));
+    }
+    const unsigned long long count = psc_ic::mode_count(c->N, false);
+    const dim3 grid((unsigned)((count + 255) / 256)), block(256);
+    const size_t off = (size_t)member * c->MS, PS = plane_stride(c->N);
+    if (c->cfg.precision == PSC_FP32)
+        hipLaunchKernelGGL(k_ic_gw<float>, grid, block, 0, c->stream, static_cast<cpx<float>*>(c->U) + off,
+                           static_cast<cpx<float>*>(c->PU) + off, astride, k, PS, count);
+    else
+        hipLaunchKernelGGL(k_ic_gw<double>, grid, block, 0, c->stream, static_cast<cpx<double>*>(c->U) + off,
+                           static_cast<cpx<double>*>(c->PU) + off, astride, k, PS, count);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->loaded_gw = 1;
     return PSC_OK;
 }
 

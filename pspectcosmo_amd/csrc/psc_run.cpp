@@ -96,6 +96,7 @@ const char* kKeys[][2] = {
     {"initfield", "homogeneous field values, comma separated"},
     {"initderivs", "homogeneous field derivatives, comma separated"},
     {"seed", "Park-Miller seed"},
+    {"ic", "host (the host generator) or device (the same modes generated on the GPU)"},
     {"with_gw", "1: evolve the 6 GW components"},
     {"gw_projection", "ref (the reference's one-sided projection) or tt"},
     {"use_graph", "1: replay captured steps"},
@@ -441,6 +442,12 @@ int main(int argc, char** argv) {
     r.ic.seed = (int)p.num("seed", 1);
     r.dir = p.get("dir", "output");
     r.gw_proj = p.get("gw_projection", "ref") == "tt" ? PSC_GW_PROJ_TT : PSC_GW_PROJ_REFERENCE;
+    const std::string ic_where = p.get("ic", "host");
+    if (ic_where != "host" && ic_where != "device") {
+        std::fprintf(stderr, "psc_run: ic must be host or device\n");
+        return 2;
+    }
+    const bool ic_device = ic_where == "device";
     const std::string snap_format = p.get("snap_format", "h5");
     if (snap_format != "h5" && snap_format != "raw") {
         std::fprintf(stderr, "psc_run: snap_format must be h5 or raw\n");
@@ -482,9 +489,13 @@ int main(int argc, char** argv) {
     }
     // initial conditions (initialize(), initialize.cpp:624-800)
     const size_t nk = (size_t)c.nflds * c.N * c.N * (c.N / 2 + 1) * 2;   // FFTW half spectra
-    std::vector<double> fk(nk), pk(nk);
+    std::vector<double> fk, pk;
     double sc0[4];
-    r.check(psc_generate_ic(&c, &r.model, &r.ic, fk.data(), pk.data(), sc0), "psc_generate_ic");
+    if (!ic_device) {
+        fk.resize(nk);
+        pk.resize(nk);
+        r.check(psc_generate_ic(&c, &r.model, &r.ic, fk.data(), pk.data(), sc0), "psc_generate_ic");
+    }
     r.check(psc_create(&c, &r.ctx), "psc_create");
     r.check(psc_set_model(r.ctx, 0, &r.model), "psc_set_model");
     const bool checkpointing = (int)p.num("checkpoint", 0) != 0;
@@ -496,10 +507,14 @@ int main(int argc, char** argv) {
         resumed = true;
         std::printf("Data read. Resuming run at t = %g\n", t_start);
         if (c.with_gw) std::fprintf(stderr, "psc_run: grid.img holds no GW state; GW restart from vacuum\n");
+    } else if (ic_device) {
+        r.check(psc_generate_ic_device(r.ctx, 0, &r.ic, sc0), "psc_generate_ic_device");
     } else {
         r.check(psc_load_ic_k(r.ctx, 0, fk.data(), pk.data(), sc0[0], sc0[1]), "psc_load_ic_k");
     }
-    if (c.with_gw) {
+    if (c.with_gw && ic_device) {
+        r.check(psc_generate_gw_ic_device(r.ctx, 0, &r.ic), "psc_generate_gw_ic_device");
+    } else if (c.with_gw) {
         const size_t gk = (size_t)6 * c.N * c.N * (c.N / 2 + 1) * 2;
         std::vector<double> u(gk), pu(gk);
         r.check(psc_generate_gw_ic(&c, &r.model, &r.ic, u.data(), pu.data()), "psc_generate_gw_ic");

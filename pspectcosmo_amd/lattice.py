@@ -58,6 +58,8 @@ def lib():
         L.psc_energy_densities.argtypes = [vp, C.c_int, _D, _D, _D, _D]
         L.psc_spectra.argtypes = [vp, C.c_int, C.c_int, _D, C.c_int]
         L.psc_load_gw_k.argtypes = [vp, C.c_int, _D, _D]
+        L.psc_generate_ic_device.argtypes = [vp, C.c_int, C.POINTER(PscIcParams), _D]
+        L.psc_generate_gw_ic_device.argtypes = [vp, C.c_int, C.POINTER(PscIcParams)]
         L.psc_get_gw_k.argtypes = [vp, C.c_int, _D, _D]
         L.psc_gw_spectrum.argtypes = [vp, C.c_int, C.c_int, _D, C.c_int, _D]
         L.psc_gw_omega_log.argtypes = [vp, C.c_int, C.c_int, C.c_int, _D, _D]
@@ -85,7 +87,7 @@ def exported_symbols():
             "psc_gw_spectrum", "psc_gw_omega_log", "psc_num_bins", "psc_set_kernel_timing", "psc_get_kernel_times",
             "psc_reset_kernel_times", "psc_device_bytes", "psc_autotune", "psc_shape_name",
             "psc_list_shapes", "psc_set_shape", "psc_adjoint", "psc_set_dt", "psc_means", "psc_load_fields",
-            "psc_energy_densities"]
+            "psc_energy_densities", "psc_generate_ic_device", "psc_generate_gw_ic_device"]
 
 
 def generate_ic(cfg, model, ic):
@@ -145,6 +147,16 @@ class Lattice:
         pk = np.ascontiguousarray(pk, dtype=np.complex128)
         self._check(lib().psc_load_ic_k(self._h, member, fk.view(np.float64).ravel(),
                                         pk.view(np.float64).ravel(), a, ad))
+
+    def generate_ic_device(self, member, ic):
+        """Initial conditions generated on the device for one member; returns
+        (a, ad, add_printed, H) as generate_ic does."""
+        sc = np.zeros(4)
+        self._check(lib().psc_generate_ic_device(self._h, member, C.byref(ic), sc))
+        return sc
+
+    def generate_gw_ic_device(self, member, ic):
+        self._check(lib().psc_generate_gw_ic_device(self._h, member, C.byref(ic)))
 
     def load_fields(self, member, f, pi, t, a, ad):
         f = np.ascontiguousarray(f, dtype=np.float64)

@@ -50,3 +50,9 @@ def test_snapshot_format_is_checked():
     assert r.returncode == 2 and "snap_format" in r.stderr
     for fmt in ("h5", "raw"):
         assert run(f"snap_format={fmt}", "--dry-run").returncode == 0
+
+
+def test_ic_source_is_checked():
+    r = run("ic=gpu", "--dry-run")
+    assert r.returncode == 2 and "ic must be" in r.stderr
+    assert run("ic=device", "--dry-run").returncode == 0

@@ -140,3 +140,16 @@ def test_checkpoint_and_continue_match_an_uninterrupted_run(tmp_path):
     sa = [x for x in rows(a_dir / "spectra_scalar_1.txt") if x is not None]
     sb = [x for x in rows(b_dir / "spectra_scalar_1.txt") if x is not None]
     close(sb, sa, rtol=2e-6)
+
+
+def test_device_initial_conditions_give_the_same_run(tmp_path):
+    a_dir, b_dir = tmp_path / "host", tmp_path / "device"
+    a_dir.mkdir()
+    b_dir.mkdir()
+    common = ["dt=0.01", "nsteps=10", "freq_steps=5", "infreq_steps=10"]
+    run_driver(a_dir, *common)
+    run_driver(b_dir, *common, "ic=device")
+    for name in ("average_energies.txt", "average_scalar_0.txt", "average_scalar_1.txt", "spectra_scalar_1.txt"):
+        ra = [x for x in rows(a_dir / name) if x is not None]
+        rb = [x for x in rows(b_dir / name) if x is not None]
+        close(rb, ra, rtol=2e-6, atol=1e-300)
