@@ -77,7 +77,9 @@ typedef struct psc_config {
     int use_graph;     /* capture each step in a hipGraph and replay it                   */
     int autotune;      /* 1: adopt the launch shape cached by psc_autotune for this
                           device / N / precision / batch / GW, if any                   */
-    int reserved_i[5];
+    int pad;           /* 0: evolve_VV_k_nopad; 2: the dealiased stepper evolve_VV_k_pad
+                          (PAD_FLAG, evolution.cpp:1271-1759) with N_pad = 2 N          */
+    int reserved_i[4];
     double kIR;        /* lattice IR wavenumber, L = 2 pi / kIR (parameters.h:28,32)       */
     double dt;         /* time step in program units (parameters.h:36)                     */
     double fStar;      /* field rescaling (parameters.h:133)                               */

@@ -54,6 +54,7 @@ def lib():
         L.orc_get_scalars.argtypes = [C.c_void_p, _D]
         L.orc_get_fields.argtypes = [C.c_void_p, _D, _D]
         L.orc_field_derivative.argtypes = [C.c_void_p, C.c_int, C.c_int, _D]
+        L.orc_vprime_k.argtypes = [C.c_void_p, _D]
         L.orc_energies.argtypes = [C.c_void_p, _D]
         L.orc_energy_densities.argtypes = [C.c_void_p, _D, _D, _D, _D]
         L.orc_spectra.argtypes = [C.c_void_p, C.c_int, _D, C.c_int]
@@ -222,6 +223,13 @@ class Oracle:
         p = np.zeros_like(f)
         lib().orc_get_fields(self._h, f.ravel(), p.ravel())
         return f, p
+
+    def vprime_k(self):
+        """V'_k half spectra per field, as the stepper's force forms them."""
+        N = self.N
+        out = np.zeros((self.nf, N, N, N // 2 + 1), dtype=np.complex128)
+        assert lib().orc_vprime_k(self._h, out.view(np.float64).ravel()) == 0
+        return out
 
     def derivative(self, fld, axis):
         N = self.N

@@ -34,6 +34,7 @@
 #include <cstdio>
 #include <vector>
 #include <algorithm>
+#include <memory>
 #include <string>
 #include <omp.h>
 
@@ -252,6 +253,12 @@ struct orc_ctx {
     // GW (6 symmetric components 11,22,33,12,13,23)
     std::vector<cd> u_k[6], pi_u_k[6], pi_u_half[6], src_k[6];
     std::vector<double> grad[2][3], src[6];
+    // dealiased stepper (PAD_FLAG): fields and V' on the Np = 2N grid
+    int Np = 0, Nph = 0;
+    size_t nrp = 0, nkp = 0;
+    std::unique_ptr<Fft3D> fftp;
+    std::vector<cd> fk_pad, Vk_pad, tmp_pad;
+    std::vector<double> f_pad[2], Vp_pad[2];
 
     orc_ctx(const psc_config& c, const psc_model& m)
         : cfg(c), N(c.N), Nh(c.N / 2 + 1), fft(c.N) {
@@ -270,6 +277,19 @@ struct orc_ctx {
             Vp_k[fl].assign(nk, 0.0);
             f[fl].assign(nr, 0.0);
             Vp[fl].assign(nr, 0.0);
+        }
+        if (cfg.pad == 2) {
+            Np = 2 * N;
+            Nph = Np / 2 + 1;
+            nrp = (size_t)Np * Np * Np;
+            nkp = (size_t)Np * Np * Nph;
+            fftp = std::make_unique<Fft3D>(Np);
+            fk_pad.assign(nkp, 0.0);
+            Vk_pad.assign(nkp, 0.0);
+            for (int fl = 0; fl < cfg.nflds; ++fl) {
+                f_pad[fl].assign(nrp, 0.0);
+                Vp_pad[fl].assign(nrp, 0.0);
+            }
         }
         if (cfg.with_gw) {
             for (int c6 = 0; c6 < 6; ++c6) {
@@ -380,6 +400,120 @@ struct orc_ctx {
                ((al - 2) * K + al * G + (al + 1) * V);
     }
 
+    // padding (evolution.cpp:1271-1455): the N-grid half spectrum into the
+    // zeroed Np-grid half spectrum.  Each x or y index N/2 goes to both +N/2
+    // and Np - N/2 with half the value; the kz = N/2 plane is stored at
+    // padded kz = N/2 with half the value (its -N/2 partner is the implied
+    // conjugate); every other index keeps its signed value.
+    void pad_spectrum(const std::vector<cd>& in, std::vector<cd>& out) const {
+        std::fill(out.begin(), out.end(), cd(0.0, 0.0));
+        for (int i = 0; i < N; ++i) {
+            int ti[2], ni = 1;
+            double wi = 1.0;
+            if (i == N / 2) { ti[0] = N / 2; ti[1] = Np - N / 2; ni = 2; wi = 0.5; }
+            else ti[0] = (signed_index(i, N) + Np) % Np;
+            for (int j = 0; j < N; ++j) {
+                int tj[2], nj = 1;
+                double wj = 1.0;
+                if (j == N / 2) { tj[0] = N / 2; tj[1] = Np - N / 2; nj = 2; wj = 0.5; }
+                else tj[0] = (signed_index(j, N) + Np) % Np;
+                for (int k = 0; k < Nh; ++k) {
+                    const double w = wi * wj * (k == N / 2 ? 0.5 : 1.0);
+                    const cd v = in[((size_t)i * N + j) * Nh + k] * w;
+                    for (int a = 0; a < ni; ++a)
+                        for (int b = 0; b < nj; ++b) out[((size_t)ti[a] * Np + tj[b]) * Nph + k] = v;
+                }
+            }
+        }
+    }
+
+    // extract_padding (evolution.cpp:1458-1617): each N-grid half-spectrum
+    // mode from the padded one with the same signed indices, an index N/2
+    // (x, y or z) read at +N/2 and doubled, all divided by (Np/N)^3.
+    void extract_spectrum(const std::vector<cd>& in, std::vector<cd>& out) const {
+        const double nf = std::pow((double)Np / N, 3);
+        for (int i = 0; i < N; ++i) {
+            const int si = i == N / 2 ? N / 2 : (signed_index(i, N) + Np) % Np;
+            for (int j = 0; j < N; ++j) {
+                const int sj = j == N / 2 ? N / 2 : (signed_index(j, N) + Np) % Np;
+                for (int k = 0; k < Nh; ++k) {
+                    double c = 1.0;
+                    if (i == N / 2) c *= 2.0;
+                    if (j == N / 2) c *= 2.0;
+                    if (k == N / 2) c *= 2.0;
+                    out[((size_t)i * N + j) * Nh + k] = in[((size_t)si * Np + sj) * Nph + k] * c / nf;
+                }
+            }
+        }
+    }
+
+    // The padded real fields: padding, c2r on the Np grid, / N^3
+    // (evolve_VV_k_pad, evolution.cpp:1693-1702, 1721-1729).
+    void refresh_pad_f() {
+        for (int fl = 0; fl < cfg.nflds; ++fl) {
+            pad_spectrum(f_k[fl], fk_pad);
+            fftp->c2r(fk_pad.data(), f_pad[fl].data(), tmp_pad);
+            double* fp = f_pad[fl].data();
+#pragma omp parallel for schedule(static)
+            for (long i = 0; i < (long)nrp; ++i) fp[i] /= N3;
+        }
+    }
+
+    // kernel_field_k_pad (evolution.cpp:1640-1676): V' on the padded grid, r2c
+    // there, extract_padding, then the spectral force on the N grid.
+    void kernel_field_pad(std::vector<cd>* out) {
+        const double norm1 = std::pow(a, 1 + cfg.alpha);
+        const double norm2 = std::pow(a, 3 + cfg.alpha);
+        const int nf = cfg.nflds;
+#pragma omp parallel for schedule(static)
+        for (long i = 0; i < (long)nrp; ++i) {
+            double fp[2] = {0, 0};
+            for (int fl = 0; fl < nf; ++fl) fp[fl] = f_pad[fl][i];
+            for (int fl = 0; fl < nf; ++fl) Vp_pad[fl][i] = model.dvdf(fl, fp);
+        }
+        const double k2ir = pw2(cfg.kIR);
+        for (int fl = 0; fl < nf; ++fl) {
+            fftp->r2c(Vp_pad[fl].data(), Vk_pad.data());
+            extract_spectrum(Vk_pad, Vp_k[fl]);
+            cd* o = out[fl].data();
+            const cd* fk = f_k[fl].data();
+            const cd* vk = Vp_k[fl].data();
+#pragma omp parallel for schedule(static)
+            for (int i = 0; i < N; ++i) {
+                const int px = signed_index(i, N);
+                for (int j = 0; j < N; ++j) {
+                    const int py = signed_index(j, N);
+                    for (int k = 0; k < Nh; ++k) {
+                        const int k2 = px * px + py * py + k * k;
+                        const size_t idx = ((size_t)i * N + j) * Nh + k;
+                        o[idx] = cd(-norm1 * k2ir * k2 * fk[idx].real() - norm2 * vk[idx].real(),
+                                    -norm1 * k2ir * k2 * fk[idx].imag() - norm2 * vk[idx].imag());
+                    }
+                }
+            }
+        }
+    }
+
+    // kernel_a_k_pad (evolution.cpp:1618-1638): K and G from the N-grid
+    // spectra, V averaged over the padded lattice.
+    double kernel_a_pad() const {
+        if (cfg.expansion == PSC_EXPANSION_STATIC) return 0.0;
+        double K = 0, G = 0, V = 0;
+        for (int fl = 0; fl < cfg.nflds; ++fl) {
+            K += kinetic_energy_k(fl);
+            G += gradient_energy_k(fl);
+        }
+        double fp[2] = {0, 0};
+        for (size_t i = 0; i < nrp; ++i) {
+            for (int fl = 0; fl < cfg.nflds; ++fl) fp[fl] = f_pad[fl][i];
+            for (int term = 0; term < model.num_terms(); ++term) V += model.V_term(term, fp);
+        }
+        V /= (double)nrp;
+        const int al = cfg.alpha;
+        return 1.0 / 3.0 * pw2(cfg.fStar / cfg.MPl) * std::pow(a, 1 + 2 * al) *
+               ((al - 2) * K + al * G + (al + 1) * V);
+    }
+
     // kernel_field_k_nopad (evolution.cpp:1092-1129).  Real f must be current.
     void kernel_field(std::vector<cd>* out) {
         const double norm1 = std::pow(a, 1 + cfg.alpha);
@@ -479,8 +613,13 @@ struct orc_ctx {
         for (size_t p = 0; p < w.size(); ++p) {
             const double wp = w[p];
             b_half = b + wp * dt / 2.0 * add;                       // step 1
-            refresh_real_f();                                        // step 2
-            kernel_field(kf);
+            if (Np) {                                                // step 2
+                refresh_pad_f();
+                kernel_field_pad(kf);
+            } else {
+                refresh_real_f();
+                kernel_field(kf);
+            }
             for (int fl = 0; fl < nf; ++fl) {
                 cd* ph = pi_k_half[fl].data();
                 const cd* pk = pi_k[fl].data();
@@ -501,6 +640,7 @@ struct orc_ctx {
                 for (long i = 0; i < (long)nk; ++i) fk[i] += wp * dt * norm * ph[i];
                 to_real(f_k[fl], f[fl]);
             }
+            if (Np) refresh_pad_f();
             if (gw) {
                 for (int c6 = 0; c6 < 6; ++c6) {
                     cd* uk = u_k[c6].data();
@@ -510,7 +650,8 @@ struct orc_ctx {
                 }
             }
             a = stat ? 1.0 : a_half + b_half * wp * dt / 2.0;         // step 5
-            kernel_field(kf);                                        // step 6
+            if (Np) kernel_field_pad(kf);                            // step 6
+            else kernel_field(kf);
             for (int fl = 0; fl < nf; ++fl) {
                 cd* pk = pi_k[fl].data();
                 const cd* ph = pi_k_half[fl].data();
@@ -522,7 +663,7 @@ struct orc_ctx {
                 gw_source();
                 gw_kick(pi_u_half, pi_u_k, wp * dt / 2);
             }
-            add = kernel_a();                                        // step 7
+            add = Np ? kernel_a_pad() : kernel_a();                  // step 7
             b = b_half + wp * dt / 2.0 * add;
         }
         ad = stat ? 0.0 : b;
@@ -700,6 +841,7 @@ int orc_fft_1d(int n, double* data, int sign) {
 
 static int check_cfg(const psc_config* c) {
     if (!c || c->N < 4 || (c->N & (c->N - 1)) || c->nflds < 1 || c->nflds > 2) return PSC_ERR_ARG;
+    if ((c->pad != 0 && c->pad != 2) || (c->pad == 2 && c->with_gw)) return PSC_ERR_ARG;
     return 0;
 }
 
@@ -849,7 +991,13 @@ int orc_get_gw(orc_ctx* c, double* u_k, double* pi_u_k) {
 }
 
 // kernel_a on the current state (real f refreshed first).
+// d2a/dt2 of the loaded state: kernel_a_k_nopad, or kernel_a_k_pad for the
+// dealiased stepper (initialize.cpp:782-797).
 double orc_kernel_a(orc_ctx* c) {
+    if (c->Np) {
+        c->refresh_pad_f();
+        return c->kernel_a_pad();
+    }
     c->refresh_real_f();
     return c->kernel_a();
 }
@@ -892,6 +1040,22 @@ int orc_get_fields(orc_ctx* c, double* f, double* pi) {
             std::memcpy(pi + (size_t)fl * c->nr, r.data(), c->nr * sizeof(double));
         }
     }
+    return 0;
+}
+
+// V'_k of the loaded state (test hook): the N-grid half spectra that
+// kernel_field_k_nopad or, for the dealiased stepper, kernel_field_k_pad
+// forms, per field.
+int orc_vprime_k(orc_ctx* c, double* out) {
+    if (c->Np) {
+        c->refresh_pad_f();
+        c->kernel_field_pad(c->kf);
+    } else {
+        c->refresh_real_f();
+        c->kernel_field(c->kf);
+    }
+    for (int fl = 0; fl < c->cfg.nflds; ++fl)
+        std::memcpy(out + (size_t)fl * c->nk * 2, c->Vp_k[fl].data(), c->nk * sizeof(cd));
     return 0;
 }
 
