@@ -30,6 +30,7 @@ struct StepConsts {
     int expansion_static;
     int nflds;
     int pad;
+    double NV;        // sites of the lattice V is averaged over (N^3, or N_pad^3 dealiased)
 };
 
 // Launch shape of the FFT passes: precision T, lattice N, radix plan Rad

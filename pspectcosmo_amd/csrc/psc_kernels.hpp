@@ -318,7 +318,7 @@ k_scalars(MemberState* __restrict__ ms, const double* __restrict__ partV,
             const double a = s.a;
             const double Kn = 0.5 * tot[1] / ipow(a, 6) / (sc.N3 * sc.N3);
             const double Gn = sc.kIR * sc.kIR * tot[2] / sc.N3 / 2.0 / (a * a) / sc.N3;
-            const double Vn = tot[0] / sc.N3;
+            const double Vn = tot[0] / sc.NV;
             s.K = Kn;
             s.G = Gn;
             s.V = Vn;

@@ -19,6 +19,7 @@
 #include "psc_kernels.hpp"
 #include "psc_adjoint.hpp"
 #include "psc_ic_device.hpp"
+#include "psc_padded.hpp"
 
 using namespace psc;
 
@@ -38,12 +39,20 @@ enum Kind {
     K_X_FORCE_GW,
     K_Z_KICK_GW,
     K_X_INV_GW,
+    K_PAD_DRIFT,
+    K_PAD_Z_INV,
+    K_PAD_Y_INV,
+    K_PAD_X_FORCE,
+    K_PAD_Y_FWD,
+    K_PAD_Z_FWD,
+    K_PAD_KICK,
     K_NKINDS
 };
 const char* kKindNames[K_NKINDS] = {"z_inverse_drift", "y_inverse", "x_force", "y_forward",
                                     "z_forward_kick", "scalars", "z_inverse", "z_forward_kick_first",
                                     "diagnostics", "z_inverse_drift_gw", "x_source_gw", "z_forward_kick_gw",
-                                    "x_inverse_gw"};
+                                    "x_inverse_gw", "pad_drift", "pad_z_inverse", "pad_y_inverse", "pad_x_force",
+                                    "pad_y_forward", "pad_z_forward", "pad_kick"};
 
 // Chunks of the first reduction level for passes with many blocks.
 constexpr int kPreChunks = 512;
@@ -60,6 +69,18 @@ struct Pending {
 };
 
 struct EngineBase;
+
+// The dealiased stepper's force chain on the 2N grid (defined below).
+struct PadBase {
+    virtual ~PadBase() {}
+    // Drift (optional), padded V', extraction and both kicks; as force_chain.
+    virtual int force_chain(psc_ctx* c, bool drift, bool reduce, double c1, double c2, int use_a_start) = 0;
+    virtual int nV() const = 0;    // partial sums of V per force evaluation
+    virtual int nKG() const = 0;   // partial sums of K, G per kick
+    virtual size_t padded_elems() const = 0;
+    // <V> over the padded lattice of the current state (kernel_a_k_pad's V_E).
+    virtual int mean_V(psc_ctx* c, double* V) = 0;
+};
 
 // Device workspace of the adjoint, kept between calls with the same checkpoint
 // spacing: checkpoints (f, pi, scalar state) at every segment start, one
@@ -113,6 +134,15 @@ struct psc_ctx {
     void* tw = nullptr;
     void* U = nullptr;   // GW: packed u pairs [3][B][N^3]
     void* PU = nullptr;  // GW: packed pi_u pairs [3][B][N^3]
+    // dealiased stepper (cfg.pad == 2): padded work array, the kept padded
+    // modes R[(N+1)^3], ghosts of the kz = 0, N/2 planes [2][N][N] of f and
+    // pi, the scaled k = 0 mode, and twiddles of the 2N transforms
+    void* Wp = nullptr;
+    void* Rp = nullptr;
+    void* Gf = nullptr;
+    void* Gp = nullptr;
+    void* dcp = nullptr;
+    void* twp = nullptr;
     MemberState* ms = nullptr;
     double* partV = nullptr;
     double* partKG = nullptr;
@@ -127,6 +157,7 @@ struct psc_ctx {
     int loaded_gw = 0;
     std::string err;
     EngineBase* eng = nullptr;
+    PadBase* padeng = nullptr;
     int timing = 0;
     Timing kt[K_NKINDS];
     std::vector<hipEvent_t> ev_free;
@@ -161,6 +192,7 @@ StepConsts consts_of(const psc_ctx* c) {
     s.alpha = c->cfg.alpha;
     s.expansion_static = c->cfg.expansion == PSC_EXPANSION_STATIC;
     s.nflds = c->cfg.nflds;
+    s.NV = c->padeng ? 8.0 * (double)c->N3 : (double)c->N3;   // lattice sites V is averaged over
     return s;
 }
 
@@ -350,6 +382,7 @@ struct Engine final : EngineBase {
 
     // One force evaluation chain; DRIFT applies the substep's drift first.
     int force_chain(psc_ctx* c, bool drift, bool reduce, double c1, double c2, int use_a_start) {
+        if (c->padeng) return c->padeng->force_chain(c, drift, reduce, c1, c2, use_a_start);
         if (c->cfg.with_gw) return force_chain_gw(c, drift, reduce, c1, c2, use_a_start);
         int rc;
         if ((rc = launch_z_inverse(c, drift, P(c->Pf), P(c->Pf), P(c->Pp), P(c->W), c->ms, c->B))) return rc;
@@ -375,8 +408,8 @@ struct Engine final : EngineBase {
             op.begin = last ? 0 : 1;
             op.w_finish = c->w[p];
             op.w_begin = wn;
-            op.nV = nV_blocks(c);
-            op.nKG = G::BLOCKS_ROW;
+            op.nV = c->padeng ? c->padeng->nV() : nV_blocks(c);
+            op.nKG = c->padeng ? c->padeng->nKG() : G::BLOCKS_ROW;
             if ((rc = launch_scalars(c, op))) return rc;
         }
         return PSC_OK;
@@ -399,8 +432,8 @@ struct Engine final : EngineBase {
         ScalarOp op{};
         op.begin = 1;
         op.w_begin = c->w[0];
-        op.nV = nV_blocks(c);
-        op.nKG = G::BLOCKS_ROW;
+        op.nV = c->padeng ? c->padeng->nV() : nV_blocks(c);
+        op.nKG = c->padeng ? c->padeng->nKG() : G::BLOCKS_ROW;
         int rc = launch_scalars(c, op);
         if (rc) return rc;
         if ((rc = force_chain(c, false, false, c->w[0] * c->cfg.dt / 2.0, 0.0, 1))) return rc;
@@ -806,6 +839,80 @@ int Engine<G>::gw_spectra(psc_ctx* c, int member, int projection, int nlog) {
     return gw_spectra_impl<typename G::T, G::N>(c, member, projection, nlog);
 }
 
+// The dealiased stepper's passes on the padded shape GP (N_pad = GP::N = 2N).
+template <class GP>
+struct PadEngine final : PadBase {
+    using T = typename GP::T;
+    static constexpr int NP = GP::N, N = NP / 2;
+    static cpx<T>* P(void* p) { return static_cast<cpx<T>*>(p); }
+    static constexpr double XN = (double)N * N * N * sizeof(cpx<T>);
+    static constexpr double XP = 8.0 * XN;
+
+    int nV() const override { return GP::BLOCKS_COL; }
+    int nKG() const override { return (int)((size_t)N * N * N / 2048); }
+    size_t padded_elems() const override { return GP::MS; }
+
+    // padding with the z-axis inverse, then the y-axis inverse, then V' and
+    // the sum of V in the x pass
+    int front(psc_ctx* c) {
+        int rc = timed(c, K_PAD_Z_INV, XN + XP, [&] {
+            hipLaunchKernelGGL((k_pad_z<GP>), dim3(GP::BLOCKS_ROW), dim3(GP::THREADS_ROW), 0, c->stream, P(c->Pf),
+                               P(c->Gf), P(c->Wp), P(c->dcp), P(c->twp));
+        });
+        if (rc) return rc;
+        rc = timed(c, K_PAD_Y_INV, 2 * XP, [&] {
+            hipLaunchKernelGGL((k_strided<GP, 1, +1>), dim3(NP / GP::TC, NP, 1), dim3(GP::THREADS_COL), 0, c->stream,
+                               P(c->Wp), P(c->twp));
+        });
+        if (rc) return rc;
+        return timed(c, K_PAD_X_FORCE, 2 * XP, [&] {
+            hipLaunchKernelGGL((k_x_force<GP>), dim3(NP / GP::TC, NP, 1), dim3(GP::THREADS_COL), 0, c->stream,
+                               P(c->Wp), c->ms, P(c->twp), c->partV, P(c->dcp));
+        });
+    }
+
+    int force_chain(psc_ctx* c, bool drift, bool reduce, double c1, double c2, int use_a_start) override {
+        int rc;
+        if (drift && (rc = timed(c, K_PAD_DRIFT, 3 * XN, [&] {
+                hipLaunchKernelGGL((k_drift_pad<T, N>), dim3((unsigned)((size_t)N * N * N / 256)), dim3(256), 0,
+                                   c->stream, P(c->Pf), P(c->Pp), P(c->Gf), P(c->Gp), c->ms);
+            })))
+            return rc;
+        if ((rc = front(c))) return rc;
+        rc = timed(c, K_PAD_Y_FWD, 2 * XP, [&] {
+            hipLaunchKernelGGL((k_strided<GP, 1, -1>), dim3(NP / GP::TC, NP, 1), dim3(GP::THREADS_COL), 0, c->stream,
+                               P(c->Wp), P(c->twp));
+        });
+        if (rc) return rc;
+        rc = timed(c, K_PAD_Z_FWD, XP / 2 + XN, [&] {
+            hipLaunchKernelGGL((k_unpad_z<GP>), dim3(GP::BLOCKS_ROW), dim3(GP::THREADS_ROW), 0, c->stream, P(c->Wp),
+                               P(c->Rp), P(c->twp));
+        });
+        if (rc) return rc;
+        const StepConsts sc = consts_of(c);
+        return timed(c, K_PAD_KICK, 4 * XN, [&] {
+            if (reduce)
+                hipLaunchKernelGGL((k_kick_pad<GP, true>), dim3(nKG()), dim3(256), 0, c->stream, P(c->Rp), P(c->Pf),
+                                   P(c->Pp), P(c->Gf), P(c->Gp), c->ms, c->partKG, c1, c2, use_a_start, sc);
+            else
+                hipLaunchKernelGGL((k_kick_pad<GP, false>), dim3(nKG()), dim3(256), 0, c->stream, P(c->Rp), P(c->Pf),
+                                   P(c->Pp), P(c->Gf), P(c->Gp), c->ms, c->partKG, c1, c2, use_a_start, sc);
+        });
+    }
+
+    int mean_V(psc_ctx* c, double* V) override {
+        int rc = front(c);
+        if (rc) return rc;
+        hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(64), 0, c->stream, c->partV, nV(), 1, c->sums);
+        HIPCHK(c, hipGetLastError());
+        double v = 0.0;
+        HIPCHK(c, hipMemcpyAsync(&v, c->sums, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        *V = v / (8.0 * N * N * N);
+        return PSC_OK;
+    }
+};
+
 // Launch shapes: index 0 is the default of each (precision, N); further
 // entries are the candidates the autotuner times (same results up to the
 // rounding of a different radix plan; identical when only TC / L differ).
@@ -816,12 +923,19 @@ struct ShapeEntry {
     int precision, N;
     const char* name;
     EngineBase* (*make)();
+    PadBase* (*make_pad)();   // the dealiased stepper of the N/2 lattice on this shape
 };
 
 template <class G>
 EngineBase* make_one() { return new Engine<G>(); }
 
-#define PSC_SHAPE(P, T, N, NAME, ...) {P, N, NAME, &make_one<S<T, N, __VA_ARGS__>>}
+template <class G>
+PadBase* make_pad_one() {
+    if constexpr (G::N >= 32) return new PadEngine<G>();
+    else return nullptr;
+}
+
+#define PSC_SHAPE(P, T, N, NAME, ...) {P, N, NAME, &make_one<S<T, N, __VA_ARGS__>>, &make_pad_one<S<T, N, __VA_ARGS__>>}
 const ShapeEntry kShapes[] = {
     PSC_SHAPE(PSC_FP32, float, 16, "r4.4_tc16_l64", Radices<4, 4>, 16, 64),
     PSC_SHAPE(PSC_FP32, float, 32, "r8.4_tc32_l64", Radices<8, 4>, 32, 64),
@@ -979,6 +1093,37 @@ struct Diag {
 
 // Energies of one member from its device state (energy_density,
 // output.cpp:57-115, with kinetic/gradient from k space and V on the lattice).
+// The ghosts of the dealiased stepper as the anti-Hermitian parts they add to
+// the half spectra of each field in the kz = 0, N/2 planes:
+// A_phi(n) = (G(n) - conj G(-n)) / 2, A_chi(n) = (G(n) + conj G(-n)) / 2i;
+// fn(field, x, y, plane, A) is called for every mode of both planes.
+template <class F>
+int for_each_ghost(psc_ctx* c, const void* Gsrc, F&& fn) {
+    const int N = c->N;
+    const size_t n = (size_t)2 * N * N;
+    std::vector<double> g(2 * n);
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (c->cfg.precision == PSC_FP32) {
+        std::vector<float> tmp(2 * n);
+        HIPCHK(c, hipMemcpy(tmp.data(), Gsrc, tmp.size() * sizeof(float), hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < 2 * n; ++i) g[i] = tmp[i];
+    } else {
+        HIPCHK(c, hipMemcpy(g.data(), Gsrc, g.size() * sizeof(double), hipMemcpyDeviceToHost));
+    }
+    for (int pl = 0; pl < 2; ++pl)
+        for (int x = 0; x < N; ++x)
+            for (int y = 0; y < N; ++y) {
+                const size_t i = (size_t)pl * N * N + (size_t)x * N + y;
+                const size_t m = (size_t)pl * N * N + (size_t)((N - x) % N) * N + (N - y) % N;
+                const double gr = g[2 * i], gi = g[2 * i + 1], mr = g[2 * m], mi = -g[2 * m + 1];
+                const double ap[2] = {0.5 * (gr - mr), 0.5 * (gi - mi)};                  // (G - conj G(-n)) / 2
+                const double ac[2] = {0.5 * (gi + mi), -0.5 * (gr + mr)};                 // (G + conj G(-n)) / 2i
+                fn(0, x, y, pl, ap);
+                if (c->cfg.nflds > 1) fn(1, x, y, pl, ac);
+            }
+    return PSC_OK;
+}
+
 int diagnose(psc_ctx* c, int member, const MemberState& s, Diag* d) {
     int rc = c->eng->field_sums(c, member);
     if (rc) return rc;
@@ -999,6 +1144,26 @@ int diagnose(psc_ctx* c, int member, const MemberState& s, Diag* d) {
         d->Vt[q] = rs[4 + q] / N3;
     }
     if (c->cfg.nflds == 1) d->K[1] = d->G[1] = d->Vt[1] = d->mean[1] = d->var[1] = d->sumsq[1] = 0.0;
+    if (c->padeng) {   // the half-spectrum sums include the ghosts (|H + A|^2 = |H|^2 + |A|^2 per plane)
+        const int Nn = c->N;
+        double gk[2] = {0, 0}, gg[2] = {0, 0};
+        auto n2 = [&](int x, int y, int pl) {
+            const int px = x <= Nn / 2 ? x : x - Nn, py = y <= Nn / 2 ? y : y - Nn, pz = pl ? Nn / 2 : 0;
+            return (double)(px * px + py * py + pz * pz);
+        };
+        int rc2 = for_each_ghost(c, c->Gp, [&](int q, int, int, int, const double* A) {
+            gk[q] += A[0] * A[0] + A[1] * A[1];
+        });
+        if (rc2) return rc2;
+        rc2 = for_each_ghost(c, c->Gf, [&](int q, int x, int y, int pl, const double* A) {
+            gg[q] += n2(x, y, pl) * (A[0] * A[0] + A[1] * A[1]);
+        });
+        if (rc2) return rc2;
+        for (int q = 0; q < c->cfg.nflds; ++q) {
+            d->K[q] += 0.5 * gk[q] / std::pow(a, 6) / (N3 * N3);
+            d->G[q] += kIR * kIR * gg[q] / N3 / 2.0 / (a * a) / N3;
+        }
+    }
     return PSC_OK;
 }
 
@@ -1008,6 +1173,24 @@ double kernel_a_value(const psc_ctx* c, const Diag& d, double a) {
     const double f2m = (c->cfg.fStar / c->cfg.MPl) * (c->cfg.fStar / c->cfg.MPl);
     const double K = d.K[0] + d.K[1], G = d.G[0] + d.G[1], V = d.Vt[0] + d.Vt[1];
     return 1.0 / 3.0 * f2m * std::pow(a, 1 + 2 * al) * ((al - 2) * K + al * G + (al + 1) * V);
+}
+
+// d2a/dt2 of newly loaded data (initialize.cpp:782-797): kernel_a_k_nopad, or
+// kernel_a_k_pad with V averaged over the padded lattice; for the dealiased
+// stepper the ghosts of the new data are zero.
+int initial_add(psc_ctx* c, const Diag& d, double a, double* add) {
+    Diag e = d;
+    if (c->padeng) {
+        HIPCHK(c, hipMemsetAsync(c->Gf, 0, (size_t)2 * c->N * c->N * c->es, c->stream));
+        HIPCHK(c, hipMemsetAsync(c->Gp, 0, (size_t)2 * c->N * c->N * c->es, c->stream));
+        double V;
+        int rc = c->padeng->mean_V(c, &V);
+        if (rc) return rc;
+        e.Vt[0] = V;
+        e.Vt[1] = 0.0;
+    }
+    *add = kernel_a_value(c, e, a);
+    return PSC_OK;
 }
 
 // <chi^2> objective: fbar = (P(k) - conj P(-k)) / N^6 on the padded layout.
@@ -1274,8 +1457,12 @@ int psc_create(const psc_config* cfg, psc_ctx** out) {
         (cfg->integrator != PSC_VV2 && cfg->integrator != PSC_Y4) ||
         (cfg->precision != PSC_FP64 && cfg->precision != PSC_FP32) || !(cfg->kIR > 0) ||
         !(cfg->fStar > 0) || !(cfg->MPl > 0) || !std::isfinite(cfg->dt) ||
-        (cfg->expansion != PSC_EXPANSION_SELF && cfg->expansion != PSC_EXPANSION_STATIC))
+        (cfg->expansion != PSC_EXPANSION_SELF && cfg->expansion != PSC_EXPANSION_STATIC) ||
+        (cfg->pad != 0 && cfg->pad != 2))
         return PSC_ERR_ARG;
+    // the dealiased stepper: one member, no GW (the reference pads only the
+    // scalar force), N_pad = 2N within the transform sizes
+    if (cfg->pad == 2 && (cfg->batch != 1 || cfg->with_gw || N > 256)) return PSC_ERR_UNSUPPORTED;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return PSC_ERR_DEVICE;
     std::unique_ptr<psc_ctx> c(new psc_ctx());
@@ -1295,6 +1482,12 @@ int psc_create(const psc_config* cfg, psc_ctx** out) {
                 if (name == kShapes[idx].name) c->shape = idx;
         }
         c->eng = kShapes[c->shape].make();
+        if (cfg->pad == 2) {
+            const std::vector<int> pc = shapes_for(cfg->precision, 2 * N);
+            if (pc.empty()) return PSC_ERR_UNSUPPORTED;
+            c->padeng = kShapes[pc[0]].make_pad();
+            if (!c->padeng) return PSC_ERR_UNSUPPORTED;
+        }
     }
     integrator_weights(*cfg, c->w);
     c->models.assign(c->B, psc_model{1.0, 0.0});
@@ -1307,7 +1500,9 @@ int psc_create(const psc_config* cfg, psc_ctx** out) {
         return true;
     };
     // partial-sum sizes: max blocks of any pass x widths
-    const size_t blocks_col = (size_t)(N / 8) * N, blocks_row = (size_t)N * N / 4;
+    size_t blocks_col = (size_t)(N / 8) * N;
+    const size_t blocks_row = (size_t)N * N / 4;
+    if (c->padeng) blocks_col = std::max(blocks_col, (size_t)c->padeng->nV());
     c->part_len = std::max<size_t>(std::max(blocks_col * 6, (size_t)256 * 4),
                                    (size_t)128 * (maxnumbins(N) * 4 + kMaxLogBins));
     c->sums_len = (size_t)maxnumbins(N) * 4 + kMaxLogBins + 64;
@@ -1320,29 +1515,32 @@ int psc_create(const psc_config* cfg, psc_ctx** out) {
         !alloc((void**)&c->partV2, sizeof(double) * kPreChunks * c->B) ||
         !alloc((void**)&c->partKG2, sizeof(double) * 2 * kPreChunks * c->B) ||
         !alloc((void**)&c->part, sizeof(double) * c->part_len) ||
-        !alloc((void**)&c->sums, sizeof(double) * c->sums_len)) {
+        !alloc((void**)&c->sums, sizeof(double) * c->sums_len) ||
+        (c->padeng && (!alloc(&c->Wp, c->padeng->padded_elems() * c->es) ||
+                       !alloc(&c->Rp, (size_t)(N + 1) * (N + 1) * (N + 1) * c->es) ||
+                       !alloc(&c->Gf, (size_t)2 * N * N * c->es) || !alloc(&c->Gp, (size_t)2 * N * N * c->es) ||
+                       !alloc(&c->dcp, c->es) || !alloc(&c->twp, (size_t)2 * N * c->es)))) {
         psc_destroy(c.release());
         return PSC_ERR_NOMEM;
     }
-    // Twiddles exp(-2 pi i m / N), m < N, rounded once from long double.
-    {
-        std::vector<double> twd(2 * N);
-        for (int m = 0; m < N; ++m) {
-            long double ang = -2.0L * 3.141592653589793238462643383279502884L * m / N;
+    // Twiddles exp(-2 pi i m / n), m < n, rounded once from long double.
+    auto twiddles = [&](void* dst, int n) {
+        std::vector<double> twd(2 * n);
+        for (int m = 0; m < n; ++m) {
+            long double ang = -2.0L * 3.141592653589793238462643383279502884L * m / n;
             twd[2 * m] = (double)cosl(ang);
             twd[2 * m + 1] = (double)sinl(ang);
         }
         if (cfg->precision == PSC_FP32) {
-            std::vector<float> twf(2 * N);
-            for (int i = 0; i < 2 * N; ++i) twf[i] = (float)twd[i];
-            if (hipMemcpy(c->tw, twf.data(), twf.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess) {
-                psc_destroy(c.release());
-                return PSC_ERR_DEVICE;
-            }
-        } else if (hipMemcpy(c->tw, twd.data(), twd.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
-            psc_destroy(c.release());
-            return PSC_ERR_DEVICE;
+            std::vector<float> twf(2 * n);
+            for (int i = 0; i < 2 * n; ++i) twf[i] = (float)twd[i];
+            return hipMemcpy(dst, twf.data(), twf.size() * sizeof(float), hipMemcpyHostToDevice) == hipSuccess;
         }
+        return hipMemcpy(dst, twd.data(), twd.size() * sizeof(double), hipMemcpyHostToDevice) == hipSuccess;
+    };
+    if (!twiddles(c->tw, N) || (c->padeng && !twiddles(c->twp, 2 * N))) {
+        psc_destroy(c.release());
+        return PSC_ERR_DEVICE;
     }
     std::vector<MemberState> init(c->B);
     for (auto& s : init) {
@@ -1353,6 +1551,8 @@ int psc_create(const psc_config* cfg, psc_ctx** out) {
     if (hipMemcpy(c->ms, init.data(), sizeof(MemberState) * c->B, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemset(c->Pf, 0, arr) != hipSuccess || hipMemset(c->Pp, 0, arr) != hipSuccess ||
         hipMemset(c->W, 0, gw ? 4 * arr : arr) != hipSuccess ||
+        (c->padeng && (hipMemset(c->Gf, 0, (size_t)2 * N * N * c->es) != hipSuccess ||
+                       hipMemset(c->Gp, 0, (size_t)2 * N * N * c->es) != hipSuccess)) ||
         (gw && (hipMemset(c->U, 0, 3 * arr) != hipSuccess || hipMemset(c->PU, 0, 3 * arr) != hipSuccess))) {
         psc_destroy(c.release());
         return PSC_ERR_DEVICE;
@@ -1371,11 +1571,12 @@ int psc_destroy(psc_ctx* c) {
     }
     for (auto e : c->ev_free) hipEventDestroy(e);
     void* ptrs[] = {c->Pf, c->Pp, c->W, c->tw, c->U, c->PU, c->ms, c->partV, c->partKG, c->partV2, c->partKG2,
-                    c->part, c->sums};
+                    c->part, c->sums, c->Wp, c->Rp, c->Gf, c->Gp, c->dcp, c->twp};
     for (void* p : ptrs)
         if (p) hipFree(p);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c->eng;
+    delete c->padeng;
     delete c;
     return PSC_OK;
 }
@@ -1413,7 +1614,7 @@ int psc_load_ic_k(psc_ctx* c, int member, const double* f_k, const double* pi_k,
     if ((rc = write_state(c, member, &s))) return rc;
     Diag d;
     if ((rc = diagnose(c, member, s, &d))) return rc;
-    s.add = kernel_a_value(c, d, a);  // initialize.cpp:797
+    if ((rc = initial_add(c, d, a, &s.add))) return rc;  // initialize.cpp:797
     if ((rc = write_state(c, member, &s))) return rc;
     c->loaded[member] = 1;
     return PSC_OK;
@@ -1495,7 +1696,7 @@ int psc_generate_ic_device(psc_ctx* c, int member, const psc_ic_params* ic, doub
     if ((rc = write_state(c, member, &s))) return rc;
     Diag d;
     if ((rc = diagnose(c, member, s, &d))) return rc;
-    s.add = kernel_a_value(c, d, sc[0]);  // initialize.cpp:797
+    if ((rc = initial_add(c, d, sc[0], &s.add))) return rc;  // initialize.cpp:797
     if ((rc = write_state(c, member, &s))) return rc;
     c->loaded[member] = 1;
     if (scalars_out)
@@ -1557,7 +1758,7 @@ int psc_load_fields(psc_ctx* c, int member, const double* f, const double* pi, d
     if ((rc = write_state(c, member, &s))) return rc;
     Diag d;
     if ((rc = diagnose(c, member, s, &d))) return rc;
-    s.add = kernel_a_value(c, d, a);  // initialize.cpp:797, as for new initial data
+    if ((rc = initial_add(c, d, a, &s.add))) return rc;  // initialize.cpp:797, as for new initial data
     if ((rc = write_state(c, member, &s))) return rc;
     c->loaded[member] = 1;
     return PSC_OK;
@@ -1609,13 +1810,23 @@ int psc_get_k(psc_ctx* c, int member, double* f_k, double* pi_k) {
     int rc = check_member(c, member);
     if (rc) return rc;
     std::vector<double> full;
+    const size_t nk = (size_t)c->N * c->N * (c->N / 2 + 1);
+    auto add_ghost = [&](const void* G, double* out) {   // the dealiased stepper's plane ghosts
+        return for_each_ghost(c, G, [&](int q, int x, int y, int pl, const double* A) {
+            const size_t i = 2 * (q * nk + ((size_t)x * c->N + y) * (c->N / 2 + 1) + (pl ? c->N / 2 : 0));
+            out[i] += A[0];
+            out[i + 1] += A[1];
+        });
+    };
     if (f_k) {
         if ((rc = download(c, c->Pf, member, full))) return rc;
         unpack_half(c->N, c->cfg.nflds, full.data(), f_k);
+        if (c->padeng && (rc = add_ghost(c->Gf, f_k))) return rc;
     }
     if (pi_k) {
         if ((rc = download(c, c->Pp, member, full))) return rc;
         unpack_half(c->N, c->cfg.nflds, full.data(), pi_k);
+        if (c->padeng && (rc = add_ghost(c->Gp, pi_k))) return rc;
     }
     return PSC_OK;
 }
@@ -2069,6 +2280,7 @@ int psc_adjoint(psc_ctx* c, const psc_adjoint_spec* sp, double* out, double* dch
         return c->fail(PSC_ERR_UNSUPPORTED, "unknown objective");
     if (c->cfg.nflds != 2 || c->B != 1 || sp->member != 0)
         return c->fail(PSC_ERR_UNSUPPORTED, "the adjoint needs two fields and one member");
+    if (c->padeng) return c->fail(PSC_ERR_UNSUPPORTED, "the adjoint is built for the unpadded stepper");
     if (sp->objective == PSC_OBJ_GW_FRACTION && !c->cfg.with_gw)
         return c->fail(PSC_ERR_UNSUPPORTED, "the GW objective needs a context with GW evolution");
     if (sp->objective == PSC_OBJ_GW_FRACTION &&

@@ -1,0 +1,245 @@
+// psc_padded.hpp -- the dealiased stepper (evolve_VV_k_pad with padding,
+// extract_padding, kernel_field_k_pad, kernel_a_k_pad: evolution.cpp:1271-1759)
+// on the device, with N_pad = 2N.
+//
+// The N-grid state is the packed full spectrum P = phi_k + i chi_k (as for
+// the unpadded stepper) plus, for the kz = 0 and kz = N/2 planes, a packed
+// "ghost" Gamma = A_phi + i A_chi of the anti-Hermitian parts that the
+// reference's half spectra acquire there: extract_padding reads the +N/2
+// copy of every Nyquist index, which in those planes is not the conjugate of
+// its mirror.  The ghost never reaches the N-grid real fields (c2r drops it),
+// but it enters the padded field through the kz = N/2 split, the kick, and
+// the half-spectrum sums of K and G, so it is carried to reproduce the
+// reference's half spectra exactly.
+//
+// Per force evaluation, with X_p the padded array:
+//   k_drift_pad     f += c_drift pi (N grid and ghosts)
+//   k_pad_z         padding (Nyquist indices split into +-N/2 halves, the
+//                   kz = N/2 halves (P +- Gamma) / 2), scaled by (N_pad/N)^3 so
+//                   that the N_pad-grid transform's 1/N_pad^3 yields the
+//                   reference's c2r / N^3; z-axis inverse transform
+//   k_strided y+    y-axis inverse on the padded grid
+//   k_x_force       x-axis inverse, V' residual and sum of V, x-axis forward
+//                   (the unpadded kernel on the padded shape)
+//   k_strided y-    y-axis forward
+//   k_unpad_z       z-axis forward of the lines with |x|, |y| <= N/2, keeping
+//                   the modes |z| <= N/2 in R[(N+1)^3]
+//   k_kick_pad      extract_padding from R, the spectral force and both kicks,
+//                   the ghost kick, sums of |pi|^2 and n^2 |f|^2
+#pragma once
+
+#include "psc_kernels.hpp"
+
+namespace psc {
+
+__device__ __forceinline__ int pad_signed(int p, int NP) { return p < NP / 2 ? p : p - NP; }
+
+// N-grid index (0..N-1) of a signed index s in [-N/2, N/2] (both ends -> N/2).
+__device__ __forceinline__ int pad_wrap(int s, int N) { return s == -N / 2 ? N / 2 : (s + N) % N; }
+
+// Drift of the N-grid state and of both ghost planes (evolution.cpp:1716-1720).
+template <typename T, int N>
+__global__ void __launch_bounds__(256)
+k_drift_pad(cpx<T>* __restrict__ Pf, const cpx<T>* __restrict__ Pp, cpx<T>* __restrict__ Gf,
+            const cpx<T>* __restrict__ Gp, const MemberState* __restrict__ ms) {
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)N * N * N) return;
+    const int x = (int)(e / ((size_t)N * N)), y = (int)((e / N) % N), z = (int)(e % N);
+    const size_t idx = (size_t)x * plane_stride(N) + (size_t)y * N + z;
+    const T cd = (T)ms[0].c_drift;
+    cpx<T> f = Pf[idx];
+    const cpx<T> p = Pp[idx];
+    f.x += cd * p.x;
+    f.y += cd * p.y;
+    Pf[idx] = f;
+    if (z == 0 || z == N / 2) {
+        const size_t gi = (z == 0 ? 0 : (size_t)N * N) + (size_t)x * N + y;
+        cpx<T> g = Gf[gi];
+        const cpx<T> q = Gp[gi];
+        g.x += cd * q.x;
+        g.y += cd * q.y;
+        Gf[gi] = g;
+    }
+}
+
+// Padding and the z-axis inverse transform on the padded grid.  The k = 0
+// mode goes to dcp (times (N_pad/N)^3) and is left out of the transform, as
+// in the unpadded passes.
+template <class GP>
+__global__ void __launch_bounds__((GP::THREADS_ROW))
+k_pad_z(const cpx<typename GP::T>* __restrict__ Pf, const cpx<typename GP::T>* __restrict__ Gf,
+        cpx<typename GP::T>* __restrict__ Wp, cpx<typename GP::T>* __restrict__ dcp,
+        const cpx<typename GP::T>* __restrict__ tw) {
+    using T = typename GP::T;
+    constexpr int NP = GP::N, N = NP / 2, H = N / 2;
+    constexpr size_t PSN = plane_stride(N);
+    __shared__ cpx<T> lds[GP::LayRow::size(GP::L)];
+    const int t = threadIdx.x % GP::TPL, c = threadIdx.x / GP::TPL;
+    const size_t line = (size_t)blockIdx.x * GP::L + c;
+    const int xp = (int)(line / NP), yp = (int)(line % NP);
+    const size_t base = (size_t)xp * GP::PS + (size_t)yp * NP + t;
+    const int sx = pad_signed(xp, NP), sy = pad_signed(yp, NP);
+    cpx<T> e[GP::E];
+    const bool mx = sx <= H && sx >= -H;
+    // With L <= N_pad every line of a block has the same padded x index, so a
+    // block outside |x| <= N/2 only writes zeros and leaves before the
+    // transform's barriers; otherwise such lines transform zeros.
+    if constexpr (GP::L <= NP) {
+        if (!mx) {
+#pragma unroll
+            for (int m = 0; m < GP::E; ++m) Wp[base + GP::TPL * m] = cpx<T>{(T)0, (T)0};
+            return;
+        }
+    }
+    const bool my = mx && sy <= H && sy >= -H;
+    const int x = pad_wrap(sx, N), y = pad_wrap(sy, N);
+    const T w = (T)(8.0 * (sx == H || sx == -H ? 0.5 : 1.0) * (sy == H || sy == -H ? 0.5 : 1.0));
+#pragma unroll
+    for (int m = 0; m < GP::E; ++m) {
+        const int sz = pad_signed(t + GP::TPL * m, NP);
+        cpx<T> v = {(T)0, (T)0};
+        if (my && sz <= H && sz >= -H) {
+            const int z = pad_wrap(sz, N);
+            v = Pf[(size_t)x * PSN + (size_t)y * N + z];
+            if (sz == H || sz == -H) {   // kz = N/2: (P +- Gamma) / 2
+                const cpx<T> g = Gf[(size_t)N * N + (size_t)x * N + y];
+                const T s = sz > 0 ? (T)1 : (T)-1;
+                v.x = (T)0.5 * (v.x + s * g.x);
+                v.y = (T)0.5 * (v.y + s * g.y);
+            }
+            v.x *= w;
+            v.y *= w;
+            if (x == 0 && y == 0 && sz == 0) {
+                dcp[0] = v;
+                v = cpx<T>{(T)0, (T)0};
+            }
+        }
+        e[m] = v;
+    }
+    line_fft<T, NP, typename GP::Rad, +1, typename GP::LayRow>(e, t, c, lds, tw);
+#pragma unroll
+    for (int m = 0; m < GP::E; ++m) Wp[base + GP::TPL * m] = e[m];
+}
+
+// z-axis forward transform of the padded lines that hold N-grid modes, the
+// modes |z| <= N/2 stored in R at ((sx + H)(N + 1) + sy + H)(N + 1) + sz + H.
+template <class GP>
+__global__ void __launch_bounds__((GP::THREADS_ROW))
+k_unpad_z(const cpx<typename GP::T>* __restrict__ Wp, cpx<typename GP::T>* __restrict__ R,
+          const cpx<typename GP::T>* __restrict__ tw) {
+    using T = typename GP::T;
+    constexpr int NP = GP::N, N = NP / 2, H = N / 2;
+    __shared__ cpx<T> lds[GP::LayRow::size(GP::L)];
+    const int t = threadIdx.x % GP::TPL, c = threadIdx.x / GP::TPL;
+    const size_t line = (size_t)blockIdx.x * GP::L + c;
+    const int xp = (int)(line / NP), yp = (int)(line % NP);
+    const int sx = pad_signed(xp, NP), sy = pad_signed(yp, NP);
+    const bool mx = sx <= H && sx >= -H;
+    if constexpr (GP::L <= NP) {   // block-uniform: all lines share the padded x index
+        if (!mx) return;
+    }
+    const size_t base = (size_t)xp * GP::PS + (size_t)yp * NP + t;
+    cpx<T> e[GP::E];
+#pragma unroll
+    for (int m = 0; m < GP::E; ++m) e[m] = Wp[base + GP::TPL * m];
+    line_fft<T, NP, typename GP::Rad, -1, typename GP::LayRow>(e, t, c, lds, tw);
+    if (!mx || sy > H || sy < -H) return;
+    const size_t row = ((size_t)(sx + H) * (N + 1) + (sy + H)) * (N + 1);
+#pragma unroll
+    for (int m = 0; m < GP::E; ++m) {
+        const int sz = pad_signed(t + GP::TPL * m, NP);
+        if (sz <= H && sz >= -H) R[row + (sz + H)] = e[m];
+    }
+}
+
+// extract_padding and the kick (kernel_field_k_pad evolution.cpp:1665-1672,
+// kicks :1704-1709, :1737-1742).  For the N-grid mode n with c = 2 per index
+// equal to N/2: outside the kz = 0, N/2 planes E = c R(n with its Nyquist
+// x, y indices at sign(kz) N/2) / 8; in those planes E = c/2 (R(n+) + R(n-)) / 8
+// and the ghost source c/2 (R(n+) - R(n-)) / 8, n+- carrying every Nyquist
+// index at +-N/2.  The residual's k = 0 mode gets N^3 V'(mean) back.  Each
+// thread handles 8 modes; REDUCE sums |pi|^2 after the first kick and
+// n^2 |f|^2, ghosts included (the half-spectrum sums of the reference).
+template <class GP, bool REDUCE>
+__global__ void __launch_bounds__(256)
+k_kick_pad(const cpx<typename GP::T>* __restrict__ R, const cpx<typename GP::T>* __restrict__ Pf,
+           cpx<typename GP::T>* __restrict__ Pp, const cpx<typename GP::T>* __restrict__ Gf,
+           cpx<typename GP::T>* __restrict__ Gp, const MemberState* __restrict__ ms,
+           double* __restrict__ partKG, double c1d, double c2d, int use_a_start, StepConsts sc) {
+    using T = typename GP::T;
+    constexpr int NP = GP::N, N = NP / 2, H = N / 2;
+    constexpr size_t PSN = plane_stride(N);
+    __shared__ double red_lds[2 * 256 / 64 + 8];
+    const double a = use_a_start ? ms[0].a_start : ms[0].a;
+    const T lapc = (T)(pow(a, 1 + sc.alpha) * sc.kIR * sc.kIR);
+    const T vc = (T)pow(a, 3 + sc.alpha);
+    const T c1 = (T)c1d, c2 = (T)c2d;
+    const T inv8 = (T)0.125;
+    auto ridx = [](int ax, int ay, int az) {
+        return ((size_t)(ax + H) * (N + 1) + (ay + H)) * (N + 1) + (az + H);
+    };
+    double red[2] = {0.0, 0.0};
+#pragma unroll 1
+    for (int q = 0; q < 8; ++q) {
+        const size_t e = ((size_t)blockIdx.x * 8 + q) * 256 + threadIdx.x;
+        const int x = (int)(e / ((size_t)N * N)), y = (int)((e / N) % N), z = (int)(e % N);
+        const int sx = x <= H ? x : x - N, sy = y <= H ? y : y - N, sz = z <= H ? z : z - N;
+        const bool X = x == H, Y = y == H, Z = z == H;
+        const T cc = (T)((X ? 2 : 1) * (Y ? 2 : 1) * (Z ? 2 : 1));
+        const bool plane = z == 0 || Z;
+        cpx<T> E, GE = {(T)0, (T)0};
+        if (!plane) {
+            const int s = sz > 0 ? H : -H;
+            const cpx<T> r = R[ridx(X ? s : sx, Y ? s : sy, sz)];
+            E = cpx<T>{cc * r.x * inv8, cc * r.y * inv8};
+        } else {
+            const cpx<T> rp = R[ridx(X ? H : sx, Y ? H : sy, Z ? H : 0)];
+            const cpx<T> rm = R[ridx(X ? -H : sx, Y ? -H : sy, Z ? -H : 0)];
+            const T h = (T)0.5 * cc * inv8;
+            E = cpx<T>{h * (rp.x + rm.x), h * (rp.y + rm.y)};
+            GE = cpx<T>{h * (rp.x - rm.x), h * (rp.y - rm.y)};
+        }
+        const size_t idx = (size_t)x * PSN + (size_t)y * N + z;
+        const cpx<T> f = Pf[idx];
+        cpx<T> p = Pp[idx];
+        if (e == 0) add_mean_force(E, f, ms[0], (double)N * N * N);
+        const int n2i = sx * sx + sy * sy + sz * sz;
+        const T ln2 = lapc * (T)n2i;
+        const cpx<T> F = {-ln2 * f.x - vc * E.x, -ln2 * f.y - vc * E.y};
+        p.x += c1 * F.x;
+        p.y += c1 * F.y;
+        if constexpr (REDUCE) {
+            const double px_ = p.x, py_ = p.y, fx = f.x, fy = f.y;
+            red[0] += px_ * px_ + py_ * py_;
+            red[1] += (double)n2i * (fx * fx + fy * fy);
+        }
+        p.x += c2 * F.x;
+        p.y += c2 * F.y;
+        Pp[idx] = p;
+        if (plane) {
+            const size_t gi = (z == 0 ? 0 : (size_t)N * N) + (size_t)x * N + y;
+            const cpx<T> gf = Gf[gi];
+            cpx<T> gp = Gp[gi];
+            const cpx<T> FG = {-ln2 * gf.x - vc * GE.x, -ln2 * gf.y - vc * GE.y};
+            gp.x += c1 * FG.x;
+            gp.y += c1 * FG.y;
+            if constexpr (REDUCE) {
+                const double px_ = gp.x, py_ = gp.y, fx = gf.x, fy = gf.y;
+                red[0] += px_ * px_ + py_ * py_;
+                red[1] += (double)n2i * (fx * fx + fy * fy);
+            }
+            gp.x += c2 * FG.x;
+            gp.y += c2 * FG.y;
+            Gp[gi] = gp;
+        }
+    }
+    if constexpr (REDUCE) {
+        block_sum<2, 256>(red, red_lds);
+        if (threadIdx.x == 0) {
+            partKG[(size_t)blockIdx.x * 2] = red[0];
+            partKG[(size_t)blockIdx.x * 2 + 1] = red[1];
+        }
+    }
+}
+
+}  // namespace psc
