@@ -1994,6 +1994,32 @@ int psc_spectra(psc_ctx* c, int member, int fld, double* out, int nbins) {
     HIPCHK(c, hipStreamSynchronize(c->stream));
     const double a = s.a, kIR = c->cfg.kIR, L = 2.0 * 3.14159265358979323846 / kIR;
     const double aop = std::pow(a, c->cfg.alpha - 3);
+    if (c->padeng) {
+        // the plane ghosts of the dealiased stepper add |A|^2 to their bins
+        // (weight 1; the cross terms with the Hermitian parts cancel between
+        // n and -n, which share a bin)
+        const int N = c->N;
+        std::vector<double> af((size_t)2 * N * N * 2);
+        if ((rc = for_each_ghost(c, c->Gf, [&](int q, int x, int y, int pl, const double* A) {
+                 if (q != fld) return;
+                 const size_t i = ((size_t)pl * N * N + (size_t)x * N + y) * 2;
+                 af[i] = A[0];
+                 af[i + 1] = A[1];
+             })))
+            return rc;
+        const double hub = s.ad / a;
+        if ((rc = for_each_ghost(c, c->Gp, [&](int q, int x, int y, int pl, const double* A) {
+                 if (q != fld) return;
+                 const size_t i = ((size_t)pl * N * N + (size_t)x * N + y) * 2;
+                 const int px = x <= N / 2 ? x : x - N, py = y <= N / 2 ? y : y - N, pz = pl ? N / 2 : 0;
+                 const int l = (int)std::floor(std::sqrt((double)(px * px + py * py + pz * pz)) + 0.5);
+                 const double nr = aop * A[0] + hub * af[i], ni = aop * A[1] + hub * af[i + 1];
+                 h[(size_t)l * 4 + 1] += af[i] * af[i] + af[i + 1] * af[i + 1];
+                 h[(size_t)l * 4 + 2] += A[0] * A[0] + A[1] * A[1];
+                 h[(size_t)l * 4 + 3] += nr * nr + ni * ni;
+             })))
+            return rc;
+    }
     for (int i = 1; i < mb; ++i) {
         const double np = h[(size_t)i * 4];
         double f2 = h[(size_t)i * 4 + 1], p2 = h[(size_t)i * 4 + 2], n2 = h[(size_t)i * 4 + 3];

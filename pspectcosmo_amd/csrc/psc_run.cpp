@@ -84,6 +84,7 @@ const char* kKeys[][2] = {
     {"infreq_steps", "fixed dt: steps between spectra"},
     {"snap_steps", "fixed dt: steps between snapshots (0: none)"},
     {"integrator", "VV (2) or Y4 (4)"},
+    {"pad", "0: evolve_VV_k_nopad; 2: the dealiased stepper with N_pad = 2N (PAD_FLAG)"},
     {"yoshida_exact", "1: Yoshida weights from 2^(1/3); 0: the reference's digits"},
     {"precision", "fp64 or fp32"},
     {"expansion", "0 no expansion, 2 self-consistent"},
@@ -423,6 +424,7 @@ int main(int argc, char** argv) {
     c.precision = p.get("precision", "fp64") == "fp32" ? PSC_FP32 : PSC_FP64;
     c.batch = 1;
     c.with_gw = (int)p.num("with_gw", 0);
+    c.pad = (int)p.num("pad", 0);
     c.alpha = (int)p.num("alpha", 0);
     c.expansion = (int)p.num("expansion", 2) == 0 ? PSC_EXPANSION_STATIC : PSC_EXPANSION_SELF;
     c.use_graph = (int)p.num("use_graph", 1);
@@ -471,10 +473,10 @@ int main(int argc, char** argv) {
 
     if (dry) {
         std::printf("N=%d nflds=%d integrator=%s precision=%s expansion=%d alpha=%d kIR=%g dt=%s%g "
-                    "m2=%g g2=%g initfield=%g,%g initderivs=%g,%g seed=%d with_gw=%d dir=%s\n",
+                    "m2=%g g2=%g initfield=%g,%g initderivs=%g,%g seed=%d with_gw=%d pad=%d dir=%s\n",
                     c.N, c.nflds, c.integrator == PSC_Y4 ? "Y4" : "VV", c.precision == PSC_FP32 ? "fp32" : "fp64",
                     c.expansion, c.alpha, c.kIR, r.auto_dt ? "auto:" : "", c.dt, r.model.m2, r.model.g2,
-                    r.ic.f0[0], r.ic.f0[1], r.ic.fd0[0], r.ic.fd0[1], r.ic.seed, c.with_gw, r.dir.c_str());
+                    r.ic.f0[0], r.ic.f0[1], r.ic.fd0[0], r.ic.fd0[1], r.ic.seed, c.with_gw, c.pad, r.dir.c_str());
         if (r.auto_dt)
             std::printf("tf=%g tOutputFreq=%g tOutputInfreq=%g tSnapFreq=%g\n", tf, tOutFreq, tOutInfreq, tSnap);
         else

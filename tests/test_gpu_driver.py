@@ -153,3 +153,20 @@ def test_device_initial_conditions_give_the_same_run(tmp_path):
         ra = [x for x in rows(a_dir / name) if x is not None]
         rb = [x for x in rows(b_dir / name) if x is not None]
         close(rb, ra, rtol=2e-6, atol=1e-300)
+
+
+def test_padded_run_matches_oracle(tmp_path):
+    run_driver(tmp_path, "dt=0.01", "nsteps=10", "freq_steps=5", "infreq_steps=10", "pad=2")
+    cfg, model, ic = parity_setup(N=16, kIR=4.0, dt=0.01, g2=2e3, omega_star=2e-2, f0=(1.0, 0.0),
+                                  fd0=(-0.7, 0.0), seed=1, pad=2)
+    fk, pk, sc = O.generate_ic(cfg, model, ic)
+    o = O.Oracle(cfg, model)
+    o.load(fk, pk, 0.0, sc[0], sc[1], 0.0)
+    o.load(fk, pk, 0.0, sc[0], sc[1], o.kernel_a())
+    energies = rows(tmp_path / "average_energies.txt")
+    for r in range(2):
+        o.step(5)
+        e = o.energies()
+        close(energies[r][1:], [e[0], e[1], e[2], e[3], e[4], e[5], e[6]])
+    spec = [x for x in rows(tmp_path / "spectra_scalar_1.txt") if x is not None]
+    close(np.array(spec)[:, 1:4], o.spectra(1)[:, 1:4], rtol=2e-6, atol=1e-300)

@@ -2,7 +2,7 @@
 N_pad = 2N) on the GPU against the oracle's restatement: half spectra of
 f and pi after 10 steps including the anti-Hermitian parts the reference's
 extraction leaves in the kz = 0, N/2 planes, the background scalars, K and G
-of both fields.  FP64 <= 1e-10, FP32 <= 1e-5 (relative L2 over the
+and the binned spectra of both fields.  FP64 <= 1e-10, FP32 <= 1e-5 (relative L2 over the
 spectrum).  Also: graph-captured steps equal eager ones, the linear force
 reproduces the unpadded stepper, and unsupported combinations are rejected."""
 import numpy as np
@@ -53,6 +53,8 @@ def test_fp64_padded_matches_oracle(N, integrator):
     assert ef <= 1e-10 and ep <= 1e-10 and efl <= 1e-10
     np.testing.assert_allclose(lat.scalars(), o.scalars(), rtol=1e-12)
     np.testing.assert_allclose(lat.energies()[:4], o.energies()[:4], rtol=1e-10)
+    for f in range(2):   # the binned spectra of the half spectra, ghosts included
+        np.testing.assert_allclose(lat.spectra(f), o.spectra(f), rtol=1e-9, atol=0)
     lat.close()
 
 
