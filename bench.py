@@ -193,9 +193,11 @@ def bench_padded256(args):
     el = time.perf_counter() - t0
     dom, achieved, avg_s, bytes_, per_kernel = kernel_roofline(lat, min(steps, 10))
     v = n ** 3 * steps / el
-    # algorithmic bytes per substep in units of X = N^3 x 8 B: drift 3, padding
-    # and z inverse 1 + 8, y and x passes 3 x 16, z forward 4 + 1, kick 4
-    b = 3 * 69 * 8
+    # algorithmic bytes per substep in units of X = N^3 x 8 B (the padded x
+    # planes |x| <= N/2 hold half of the 8X padded array): drift 3, padding
+    # and z inverse 1 + 4, y inverse, x pass and y forward 8 each, z forward
+    # 2 + 1, kick 4
+    b = 3 * 39 * 8
     out = {"metric": "site-updates/s at 256^3, dealiased (N_pad = 512), FP32-mixed", "value": v,
            "unit": "site-updates/s", "n_gpus": 1, "steps": steps, "warmup": args.warmup,
            "ms_per_step": el / steps * 1e3, "higher_is_better": True, "dtype": "f32", "data": "synthetic",

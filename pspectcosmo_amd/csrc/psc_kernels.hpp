@@ -89,7 +89,10 @@ k_z_forward(const cpx<typename G::T>* __restrict__ W, cpx<typename G::T>* __rest
 
 // ---------------------------------------------------------------------------
 // B / D: strided-axis pass in place.  AXIS 1 = y (stride N), 0 = x (stride N^2).
-template <class G, int AXIS, int SIGN>
+// MAPPED (y passes of the dealiased stepper on its 2N grid): blockIdx.y runs
+// over the N/2 + 1 ... x planes |x| <= N/4 that hold modes; the others stay
+// untouched.
+template <class G, int AXIS, int SIGN, bool MAPPED = false>
 __global__ void __launch_bounds__((G::THREADS_COL))
 k_strided(cpx<typename G::T>* __restrict__ W, const cpx<typename G::T>* __restrict__ tw) {
     using T = typename G::T;
@@ -98,7 +101,7 @@ k_strided(cpx<typename G::T>* __restrict__ W, const cpx<typename G::T>* __restri
     const int c = threadIdx.x % G::TC, t = threadIdx.x / G::TC;
     const int member = blockIdx.z;
     const size_t z = (size_t)blockIdx.x * G::TC + c;
-    const size_t o = blockIdx.y;
+    const size_t o = MAPPED ? (size_t)((blockIdx.y + N - N / 4) % N) : (size_t)blockIdx.y;
     constexpr size_t stride = AXIS == 1 ? (size_t)N : G::PS;
     const size_t base = (size_t)member * G::MS + (AXIS == 1 ? o * G::PS : o * N) + z + t * stride;
     cpx<T> e[G::E];
@@ -113,7 +116,9 @@ k_strided(cpx<typename G::T>* __restrict__ W, const cpx<typename G::T>* __restri
 // C: x-axis inverse, pointwise V'(phi, chi) of model.h (dvdf, model.h:75-82;
 // kernel_field_k_nopad evolution.cpp:1103-1109) and x-axis forward, with the
 // lattice sum of V (kernel_a_k_nopad evolution.cpp:1077-1085) in FP64.
-template <class G, bool REC = false>
+// PADIN (the dealiased stepper on its 2N grid): the x rows |x| > N/4 of the
+// spectrum are zero and are neither read nor written.
+template <class G, bool REC = false, bool PADIN = false>
 __global__ void __launch_bounds__((G::THREADS_COL))
 k_x_force(cpx<typename G::T>* __restrict__ W, const MemberState* __restrict__ ms, const cpx<typename G::T>* __restrict__ tw,
           double* __restrict__ partV, const cpx<typename G::T>* __restrict__ Pf,
@@ -128,8 +133,13 @@ k_x_force(cpx<typename G::T>* __restrict__ W, const MemberState* __restrict__ ms
     constexpr size_t stride = G::PS;
     const size_t base = (size_t)member * G::MS + o * N + z + t * stride;
     cpx<T> e[G::E];
+    auto kept = [&](int m) {   // spectrum row x = t + TPL m holds data
+        const int x = t + G::TPL * m;
+        return !PADIN || x <= N / 4 || x >= N - N / 4;
+    };
 #pragma unroll
-    for (int m = 0; m < G::E; ++m) e[m] = W[base + (size_t)G::TPL * m * stride];
+    for (int m = 0; m < G::E; ++m)
+        e[m] = kept(m) ? W[base + (size_t)G::TPL * m * stride] : cpx<T>{(T)0, (T)0};
     line_fft<T, N, typename G::Rad, +1, typename G::LayCol>(e, t, c, lds, tw);
     const T inv = (T)(1.0 / (double)G::N3);
     const T m2 = (T)ms[member].m2, g2 = (T)ms[member].g2;
@@ -161,7 +171,8 @@ k_x_force(cpx<typename G::T>* __restrict__ W, const MemberState* __restrict__ ms
     if constexpr (sizeof(T) == 8) asm volatile("" : "+s"(tw_f));
     line_fft<T, N, typename G::Rad, -1, typename G::LayCol>(e, t, c, lds, tw_f);
 #pragma unroll
-    for (int m = 0; m < G::E; ++m) W[base + (size_t)G::TPL * m * stride] = e[m];
+    for (int m = 0; m < G::E; ++m)
+        if (kept(m)) W[base + (size_t)G::TPL * m * stride] = e[m];
     block_sum<1, G::THREADS_COL>(v, reinterpret_cast<double*>(lds));
     if (threadIdx.x == 0)
         partV[(size_t)member * G::BLOCKS_COL + blockIdx.y * gridDim.x + blockIdx.x] = v[0];

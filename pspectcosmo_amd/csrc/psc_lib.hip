@@ -855,19 +855,20 @@ struct PadEngine final : PadBase {
     // padding with the z-axis inverse, then the y-axis inverse, then V' and
     // the sum of V in the x pass
     int front(psc_ctx* c) {
-        int rc = timed(c, K_PAD_Z_INV, XN + XP, [&] {
+        int rc = timed(c, K_PAD_Z_INV, XN + XP * (N + 1) / (2.0 * N), [&] {
             hipLaunchKernelGGL((k_pad_z<GP>), dim3(GP::BLOCKS_ROW), dim3(GP::THREADS_ROW), 0, c->stream, P(c->Pf),
                                P(c->Gf), P(c->Wp), P(c->dcp), P(c->twp));
         });
         if (rc) return rc;
-        rc = timed(c, K_PAD_Y_INV, 2 * XP, [&] {
-            hipLaunchKernelGGL((k_strided<GP, 1, +1>), dim3(NP / GP::TC, NP, 1), dim3(GP::THREADS_COL), 0, c->stream,
-                               P(c->Wp), P(c->twp));
+        // the y passes and the stored x rows cover the padded x planes |x| <= N/2
+        rc = timed(c, K_PAD_Y_INV, XP * (N + 1) / N, [&] {
+            hipLaunchKernelGGL((k_strided<GP, 1, +1, true>), dim3(NP / GP::TC, N + 1, 1), dim3(GP::THREADS_COL), 0,
+                               c->stream, P(c->Wp), P(c->twp));
         });
         if (rc) return rc;
-        return timed(c, K_PAD_X_FORCE, 2 * XP, [&] {
-            hipLaunchKernelGGL((k_x_force<GP>), dim3(NP / GP::TC, NP, 1), dim3(GP::THREADS_COL), 0, c->stream,
-                               P(c->Wp), c->ms, P(c->twp), c->partV, P(c->dcp));
+        return timed(c, K_PAD_X_FORCE, XP * (N + 1) / N, [&] {
+            hipLaunchKernelGGL((k_x_force<GP, false, true>), dim3(NP / GP::TC, NP, 1), dim3(GP::THREADS_COL), 0,
+                               c->stream, P(c->Wp), c->ms, P(c->twp), c->partV, P(c->dcp));
         });
     }
 
@@ -879,12 +880,12 @@ struct PadEngine final : PadBase {
             })))
             return rc;
         if ((rc = front(c))) return rc;
-        rc = timed(c, K_PAD_Y_FWD, 2 * XP, [&] {
-            hipLaunchKernelGGL((k_strided<GP, 1, -1>), dim3(NP / GP::TC, NP, 1), dim3(GP::THREADS_COL), 0, c->stream,
-                               P(c->Wp), P(c->twp));
+        rc = timed(c, K_PAD_Y_FWD, XP * (N + 1) / N, [&] {
+            hipLaunchKernelGGL((k_strided<GP, 1, -1, true>), dim3(NP / GP::TC, N + 1, 1), dim3(GP::THREADS_COL), 0,
+                               c->stream, P(c->Wp), P(c->twp));
         });
         if (rc) return rc;
-        rc = timed(c, K_PAD_Z_FWD, XP / 2 + XN, [&] {
+        rc = timed(c, K_PAD_Z_FWD, XN * (N + 1) * (N + 1) / ((double)N * N) * 3, [&] {
             hipLaunchKernelGGL((k_unpad_z<GP>), dim3(GP::BLOCKS_ROW), dim3(GP::THREADS_ROW), 0, c->stream, P(c->Wp),
                                P(c->Rp), P(c->twp));
         });

@@ -18,10 +18,10 @@
 //                   kz = N/2 halves (P +- Gamma) / 2), scaled by (N_pad/N)^3 so
 //                   that the N_pad-grid transform's 1/N_pad^3 yields the
 //                   reference's c2r / N^3; z-axis inverse transform
-//   k_strided y+    y-axis inverse on the padded grid
-//   k_x_force       x-axis inverse, V' residual and sum of V, x-axis forward
-//                   (the unpadded kernel on the padded shape)
-//   k_strided y-    y-axis forward
+//   k_strided y+    y-axis inverse on the padded x planes |x| <= N/2
+//   k_x_force       x-axis inverse (rows |x| > N/2 read as zero), V' residual
+//                   and sum of V, x-axis forward (rows |x| <= N/2 stored)
+//   k_strided y-    y-axis forward on the planes |x| <= N/2
 //   k_unpad_z       z-axis forward of the lines with |x|, |y| <= N/2, keeping
 //                   the modes |z| <= N/2 in R[(N+1)^3]
 //   k_kick_pad      extract_padding from R, the spectral force and both kicks,
@@ -81,15 +81,13 @@ k_pad_z(const cpx<typename GP::T>* __restrict__ Pf, const cpx<typename GP::T>* _
     const int sx = pad_signed(xp, NP), sy = pad_signed(yp, NP);
     cpx<T> e[GP::E];
     const bool mx = sx <= H && sx >= -H;
-    // With L <= N_pad every line of a block has the same padded x index, so a
-    // block outside |x| <= N/2 only writes zeros and leaves before the
-    // transform's barriers; otherwise such lines transform zeros.
+    // Padded x planes |x| > N/2 hold no modes: the y passes skip them and the
+    // x pass reads them as zero, so nothing is written there.  With L <= N_pad
+    // every line of a block has the same padded x index and such a block
+    // leaves before the transform's barriers; otherwise its lines transform
+    // zeros and skip the store.
     if constexpr (GP::L <= NP) {
-        if (!mx) {
-#pragma unroll
-            for (int m = 0; m < GP::E; ++m) Wp[base + GP::TPL * m] = cpx<T>{(T)0, (T)0};
-            return;
-        }
+        if (!mx) return;
     }
     const bool my = mx && sy <= H && sy >= -H;
     const int x = pad_wrap(sx, N), y = pad_wrap(sy, N);
@@ -117,6 +115,7 @@ k_pad_z(const cpx<typename GP::T>* __restrict__ Pf, const cpx<typename GP::T>* _
         e[m] = v;
     }
     line_fft<T, NP, typename GP::Rad, +1, typename GP::LayRow>(e, t, c, lds, tw);
+    if (!mx) return;
 #pragma unroll
     for (int m = 0; m < GP::E; ++m) Wp[base + GP::TPL * m] = e[m];
 }
@@ -137,6 +136,11 @@ k_unpad_z(const cpx<typename GP::T>* __restrict__ Wp, cpx<typename GP::T>* __res
     const bool mx = sx <= H && sx >= -H;
     if constexpr (GP::L <= NP) {   // block-uniform: all lines share the padded x index
         if (!mx) return;
+        // and cover L consecutive padded y: leave if none of them holds modes
+        const int y0 = pad_signed((int)(((size_t)blockIdx.x * GP::L) % NP), NP);
+        const int y1 = pad_signed((int)(((size_t)blockIdx.x * GP::L + GP::L - 1) % NP), NP);
+        if (y0 > H && y1 > H) return;            // both ends in (H, N) (no wrap: L <= N and blocks align)
+        if (y0 < -H && y1 < -H) return;          // both ends in [-N, -H)
     }
     const size_t base = (size_t)xp * GP::PS + (size_t)yp * NP + t;
     cpx<T> e[GP::E];
