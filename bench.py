@@ -184,6 +184,8 @@ def bench_padded256(args):
     lat.set_model(0, model)
     lat.generate_ic_device(0, ic)
     device_sync(lat)
+    if args.autotune:   # the shapes of the 512-point padded passes
+        print("[bench] autotune", lat.autotune(2), file=sys.stderr, flush=True)
     steps = args.steps
     lat.step(args.warmup)
     device_sync(lat)

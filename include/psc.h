@@ -269,7 +269,9 @@ int psc_adjoint(psc_ctx* ctx, const psc_adjoint_spec* spec, double* out, double*
  * steps on the loaded state, keeps the fastest, restores the state, and
  * appends "key shape milliseconds" to the cache file named by the environment
  * variable PSC_TUNE_CACHE (default $HOME/.cache/pspectcosmo_amd/tune.txt).
- * report (may be NULL) receives "shape=ms;..." for every candidate. */
+ * report (may be NULL) receives "shape=ms;..." for every candidate.  With
+ * pad = 2 the candidates, psc_shape_name, psc_list_shapes and psc_set_shape
+ * refer to the 2N shapes of the padded passes. */
 int psc_autotune(psc_ctx* ctx, int nsteps, char* report, int report_len);
 /* Name of the launch shape the context uses. */
 const char* psc_shape_name(const psc_ctx* ctx);

@@ -166,6 +166,7 @@ struct psc_ctx {
     std::unique_ptr<AdjWork> adj;
     double dev_bytes = 0.0;
     int shape = 0;   // index into kShapes
+    int pad_shape = -1;   // dealiased stepper: index into kShapes of its 2N shape
     std::string report;
 
     int fail(int code, const std::string& msg) {
@@ -1176,14 +1177,20 @@ double kernel_a_value(const psc_ctx* c, const Diag& d, double a) {
     return 1.0 / 3.0 * f2m * std::pow(a, 1 + 2 * al) * ((al - 2) * K + al * G + (al + 1) * V);
 }
 
+// New data for the dealiased stepper has no plane ghosts: clear them before
+// anything reads the state (diagnose adds them to K and G).
+int reset_ghosts(psc_ctx* c) {
+    if (!c->padeng) return PSC_OK;
+    HIPCHK(c, hipMemsetAsync(c->Gf, 0, (size_t)2 * c->N * c->N * c->es, c->stream));
+    HIPCHK(c, hipMemsetAsync(c->Gp, 0, (size_t)2 * c->N * c->N * c->es, c->stream));
+    return PSC_OK;
+}
+
 // d2a/dt2 of newly loaded data (initialize.cpp:782-797): kernel_a_k_nopad, or
-// kernel_a_k_pad with V averaged over the padded lattice; for the dealiased
-// stepper the ghosts of the new data are zero.
+// kernel_a_k_pad with V averaged over the padded lattice.
 int initial_add(psc_ctx* c, const Diag& d, double a, double* add) {
     Diag e = d;
     if (c->padeng) {
-        HIPCHK(c, hipMemsetAsync(c->Gf, 0, (size_t)2 * c->N * c->N * c->es, c->stream));
-        HIPCHK(c, hipMemsetAsync(c->Gp, 0, (size_t)2 * c->N * c->N * c->es, c->stream));
         double V;
         int rc = c->padeng->mean_V(c, &V);
         if (rc) return rc;
@@ -1397,6 +1404,7 @@ std::string tune_key(const psc_ctx* c) {
     std::ostringstream k;
     k << arch << "|N" << c->N << "|" << (c->cfg.precision == PSC_FP32 ? "fp32" : "fp64") << "|B" << c->B
       << "|gw" << c->cfg.with_gw;
+    if (c->cfg.pad) k << "|pad" << c->cfg.pad;
     return k.str();
 }
 
@@ -1486,7 +1494,13 @@ int psc_create(const psc_config* cfg, psc_ctx** out) {
         if (cfg->pad == 2) {
             const std::vector<int> pc = shapes_for(cfg->precision, 2 * N);
             if (pc.empty()) return PSC_ERR_UNSUPPORTED;
-            c->padeng = kShapes[pc[0]].make_pad();
+            c->pad_shape = pc[0];
+            if (cfg->autotune) {
+                const std::string name = tune_lookup(tune_key(c.get()));
+                for (int idx : pc)
+                    if (name == kShapes[idx].name) c->pad_shape = idx;
+            }
+            c->padeng = kShapes[c->pad_shape].make_pad();
             if (!c->padeng) return PSC_ERR_UNSUPPORTED;
         }
     }
@@ -1605,6 +1619,7 @@ int psc_load_ic_k(psc_ctx* c, int member, const double* f_k, const double* pi_k,
     if ((rc = upload(c, c->Pf, member, full))) return rc;
     pack_full(c->N, c->cfg.nflds, pi_k, full);
     if ((rc = upload(c, c->Pp, member, full))) return rc;
+    if ((rc = reset_ghosts(c))) return rc;
     MemberState s;
     std::memset(&s, 0, sizeof(s));
     s.t = 0.0;
@@ -1687,6 +1702,7 @@ int psc_generate_ic_device(psc_ctx* c, int member, const psc_ic_params* ic, doub
         hipLaunchKernelGGL(k_ic_fields<double>, grid, block, 0, c->stream, static_cast<cpx<double>*>(c->Pf) + off,
                            static_cast<cpx<double>*>(c->Pp) + off, k, PS, count);
     HIPCHK(c, hipGetLastError());
+    if ((rc = reset_ghosts(c))) return rc;
     // the member's state as psc_load_ic_k sets it
     MemberState s;
     std::memset(&s, 0, sizeof(s));
@@ -1749,6 +1765,7 @@ int psc_load_fields(psc_ctx* c, int member, const double* f, const double* pi, d
         if ((rc = upload(c, c->W, member, full))) return rc;
         if ((rc = c->eng->to_spectrum(c, member, which == 0 ? c->Pf : c->Pp))) return rc;
     }
+    if ((rc = reset_ghosts(c))) return rc;
     MemberState s;
     std::memset(&s, 0, sizeof(s));
     s.t = t;
@@ -2196,14 +2213,17 @@ int psc_autotune(psc_ctx* c, int nsteps, char* report, int report_len) {
         if (!c->loaded[b]) return c->fail(PSC_ERR_STATE, "autotune needs loaded initial data");
     const size_t arr = (size_t)c->B * c->MS * c->es;
     const bool gw = c->cfg.with_gw != 0;
-    void *sf = nullptr, *sp = nullptr, *su = nullptr, *spu = nullptr;
+    const bool pad = c->padeng != nullptr;
+    const size_t garr = pad ? (size_t)2 * c->N * c->N * c->es : 0;
+    void *sf = nullptr, *sp = nullptr, *su = nullptr, *spu = nullptr, *sgf = nullptr, *sgp = nullptr;
     std::vector<MemberState> ms(c->B);
     auto cleanup = [&]() {
-        for (void* q : {sf, sp, su, spu})
+        for (void* q : {sf, sp, su, spu, sgf, sgp})
             if (q) hipFree(q);
     };
     if (hipMalloc(&sf, arr) != hipSuccess || hipMalloc(&sp, arr) != hipSuccess ||
-        (gw && (hipMalloc(&su, 3 * arr) != hipSuccess || hipMalloc(&spu, 3 * arr) != hipSuccess))) {
+        (gw && (hipMalloc(&su, 3 * arr) != hipSuccess || hipMalloc(&spu, 3 * arr) != hipSuccess)) ||
+        (pad && (hipMalloc(&sgf, garr) != hipSuccess || hipMalloc(&sgp, garr) != hipSuccess))) {
         cleanup();
         return c->fail(PSC_ERR_NOMEM, "autotune snapshot allocation failed");
     }
@@ -2215,6 +2235,10 @@ int psc_autotune(psc_ctx* c, int nsteps, char* report, int report_len) {
             HIPCHK(c, hipMemcpy(save ? su : c->U, save ? c->U : su, 3 * arr, hipMemcpyDeviceToDevice));
             HIPCHK(c, hipMemcpy(save ? spu : c->PU, save ? c->PU : spu, 3 * arr, hipMemcpyDeviceToDevice));
         }
+        if (pad) {
+            HIPCHK(c, hipMemcpy(save ? sgf : c->Gf, save ? c->Gf : sgf, garr, hipMemcpyDeviceToDevice));
+            HIPCHK(c, hipMemcpy(save ? sgp : c->Gp, save ? c->Gp : sgp, garr, hipMemcpyDeviceToDevice));
+        }
         if (save)
             HIPCHK(c, hipMemcpy(ms.data(), c->ms, sizeof(MemberState) * c->B, hipMemcpyDeviceToHost));
         else
@@ -2223,20 +2247,30 @@ int psc_autotune(psc_ctx* c, int nsteps, char* report, int report_len) {
     };
     int rc = copy(true);
     if (rc) { cleanup(); return rc; }
-    const std::vector<int> cand = shapes_for(c->cfg.precision, c->N);
+    // the dealiased stepper's time is in its 2N passes: tune those
+    const std::vector<int> cand = shapes_for(c->cfg.precision, pad ? 2 * c->N : c->N);
     const int saved_timing = c->timing;
     c->timing = 0;
     double best_ms = 1e300;
-    int best = c->shape;
+    int best = pad ? c->pad_shape : c->shape;
+    auto adopt = [&](int idx) {
+        drop_graphs(c);
+        if (pad) {
+            delete c->padeng;
+            c->padeng = kShapes[idx].make_pad();
+            c->pad_shape = idx;
+        } else {
+            delete c->eng;
+            c->eng = kShapes[idx].make();
+            c->shape = idx;
+        }
+    };
     std::ostringstream rep;
     hipEvent_t e0, e1;
     hipEventCreate(&e0);
     hipEventCreate(&e1);
     for (int idx : cand) {
-        drop_graphs(c);
-        delete c->eng;
-        c->eng = kShapes[idx].make();
-        c->shape = idx;
+        adopt(idx);
         if ((rc = copy(false))) break;
         if ((rc = c->eng->step(c, 1))) break;   // warm-up (and graph capture)
         if ((rc = copy(false))) break;
@@ -2255,10 +2289,7 @@ int psc_autotune(psc_ctx* c, int nsteps, char* report, int report_len) {
     hipEventDestroy(e0);
     hipEventDestroy(e1);
     c->timing = saved_timing;
-    drop_graphs(c);
-    delete c->eng;
-    c->shape = best;
-    c->eng = kShapes[best].make();
+    adopt(best);
     int rc2 = copy(false);
     cleanup();
     if (rc) return rc;
@@ -2272,12 +2303,15 @@ int psc_autotune(psc_ctx* c, int nsteps, char* report, int report_len) {
     return PSC_OK;
 }
 
-const char* psc_shape_name(const psc_ctx* c) { return c ? kShapes[c->shape].name : "null context"; }
+const char* psc_shape_name(const psc_ctx* c) {
+    if (!c) return "null context";
+    return kShapes[c->padeng ? c->pad_shape : c->shape].name;
+}
 
 int psc_list_shapes(const psc_ctx* c, char* names, int names_len) {
     if (!c || !names || names_len < 1) return PSC_ERR_ARG;
     std::string s;
-    for (int idx : shapes_for(c->cfg.precision, c->N)) {
+    for (int idx : shapes_for(c->cfg.precision, c->padeng ? 2 * c->N : c->N)) {
         if (!s.empty()) s += ",";
         s += kShapes[idx].name;
     }
@@ -2288,6 +2322,18 @@ int psc_list_shapes(const psc_ctx* c, char* names, int names_len) {
 
 int psc_set_shape(psc_ctx* c, const char* name) {
     if (!c || !name) return PSC_ERR_ARG;
+    if (c->padeng) {   // the dealiased stepper's 2N shape
+        for (int idx : shapes_for(c->cfg.precision, 2 * c->N))
+            if (std::strcmp(name, kShapes[idx].name) == 0) {
+                HIPCHK(c, hipStreamSynchronize(c->stream));
+                drop_graphs(c);
+                delete c->padeng;
+                c->pad_shape = idx;
+                c->padeng = kShapes[idx].make_pad();
+                return PSC_OK;
+            }
+        return c->fail(PSC_ERR_ARG, std::string("no launch shape named ") + name);
+    }
     for (int idx : shapes_for(c->cfg.precision, c->N))
         if (std::strcmp(name, kShapes[idx].name) == 0) {
             HIPCHK(c, hipStreamSynchronize(c->stream));
