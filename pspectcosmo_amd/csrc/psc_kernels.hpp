@@ -728,10 +728,10 @@ k_z_inverse_gw(cpx<typename G::T>* __restrict__ Pf, const cpx<typename G::T>* __
     }
 }
 
-// C (GW), last part: after the fields went through k_x_force and the three
-// gradient arrays through a plain x-axis inverse pass, form the packed source
-// S = (S11 + i S22, S33 + i S12, S13 + i S23) from the real-space gradients
-// and run the three x-axis forward transforms.
+// C (GW), last part: the x-axis inverse transforms of the three gradient
+// arrays, the packed source S = (S11 + i S22, S33 + i S12, S13 + i S23) from
+// the real-space gradients, and the three x-axis forward transforms, in one
+// pass over the three arrays.
 template <class G, bool REC = false>
 __global__ void __launch_bounds__((G::THREADS_COL))
 k_x_source(cpx<typename G::T>* __restrict__ W, const cpx<typename G::T>* __restrict__ tw, size_t astride,
@@ -751,6 +751,12 @@ k_x_source(cpx<typename G::T>* __restrict__ W, const cpx<typename G::T>* __restr
     for (int a = 0; a < 3; ++a)
 #pragma unroll
         for (int m = 0; m < G::E; ++m) g[a][m] = W[(size_t)(a + 1) * astride + base + (size_t)G::TPL * m * stride];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const cpx<T>* twa = tw;   // reloaded per transform, as below
+        asm volatile("" : "+s"(twa));
+        line_fft<T, N, typename G::Rad, +1, typename G::LayCol>(g[a], t, c, lds, twa);
+    }
 #pragma unroll
     for (int m = 0; m < G::E; ++m) {
         const T ax = g[0][m].x * inv, bx = g[0][m].y * inv;

@@ -354,14 +354,9 @@ struct Engine final : EngineBase {
             });
         if (rc) return rc;
         if ((rc = launch_y<+1>(c, P(c->W), 4 * c->B))) return rc;
-        // fields: x inverse, V', x forward; gradients: x inverse, then the
-        // source and its x forward transforms
+        // fields: x inverse, V', x forward; gradients: x inverse, the source
+        // and its x forward transforms in one pass
         if ((rc = launch_x_force(c))) return rc;
-        rc = timed(c, K_X_INV_GW, 6 * X, [&] {
-            hipLaunchKernelGGL((k_strided<G, 0, +1>), dim3(N / G::TC, N, 3 * c->B), dim3(G::THREADS_COL), 0,
-                               c->stream, P(c->W) + astride, P(c->tw));
-        });
-        if (rc) return rc;
         rc = timed(c, K_X_FORCE_GW, 6 * X, [&] {
             hipLaunchKernelGGL((k_x_source<G>), dim3(N / G::TC, N, c->B), dim3(G::THREADS_COL), 0, c->stream,
                                P(c->W), P(c->tw), astride);
@@ -689,13 +684,11 @@ struct Engine final : EngineBase {
         int rc = launch_scalars(c, op, w.rms);
         if (rc) return rc;
         dim3 rgrid(G::BLOCKS_ROW, 1), rblock(G::THREADS_ROW), cgrid(N / G::TC, N, 1), cblock(G::THREADS_COL);
-        dim3 cgrid3(N / G::TC, N, 3);
         auto middle = [&](int jb) -> int {   // y inverse .. y forward of the GW chain at boundary jb
             int r2 = launch_y<+1>(c, P(c->W), 4);
             if (r2) return r2;
             hipLaunchKernelGGL((k_x_force<G, true>), cgrid, cblock, 0, c->stream, P(c->W), c->ms, P(c->tw),
                                c->partV, P(c->Pf), P(w.r[jb]));
-            hipLaunchKernelGGL((k_strided<G, 0, +1>), cgrid3, cblock, 0, c->stream, P(c->W) + as, P(c->tw));
             hipLaunchKernelGGL((k_x_source<G, true>), cgrid, cblock, 0, c->stream, P(c->W), P(c->tw), as,
                                P(w.g[jb]));
             return launch_y<-1>(c, P(c->W), 4);
