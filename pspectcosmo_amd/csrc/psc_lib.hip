@@ -13,6 +13,7 @@
 #include <memory>
 #include <string>
 #include <type_traits>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/psc.h"
@@ -259,7 +260,10 @@ struct EngineBase {
     virtual int density(psc_ctx* c, int member, int kind, double* D, double scale) = 0;
 };
 
-template <class G>
+// G drives every pass but the y-axis ones, which run on GY: a plan with more
+// columns per block can suit the strided y passes where it would starve the
+// x passes (their rows hold fused force, source and reduction work).
+template <class G, class GY = G>
 struct Engine final : EngineBase {
     using T = typename G::T;
     static constexpr int N = G::N;
@@ -283,9 +287,10 @@ struct Engine final : EngineBase {
     template <int SIGN>
     int launch_y(psc_ctx* c, cpx<T>* W, int members) {
         const double X = (double)members * G::N3 * sizeof(cpx<T>);
-        dim3 grid(N / G::TC, N, members), block(G::THREADS_COL);
+        static_assert(GY::N == N && std::is_same_v<typename GY::T, T>, "y plan of another lattice");
+        dim3 grid(N / GY::TC, N, members), block(GY::THREADS_COL);
         return timed(c, SIGN > 0 ? K_Y_INV : K_Y_FWD, 2 * X, [&] {
-            hipLaunchKernelGGL((k_strided<G, 1, SIGN>), grid, block, 0, c->stream, W, P(c->tw));
+            hipLaunchKernelGGL((k_strided<GY, 1, SIGN>), grid, block, 0, c->stream, W, P(c->tw));
         });
     }
     int launch_x_force(psc_ctx* c) {
@@ -828,8 +833,8 @@ int gw_spectra_impl(psc_ctx* c, int member, int projection, int nlog) {
     return PSC_OK;
 }
 
-template <class G>
-int Engine<G>::gw_spectra(psc_ctx* c, int member, int projection, int nlog) {
+template <class G, class GY>
+int Engine<G, GY>::gw_spectra(psc_ctx* c, int member, int projection, int nlog) {
     return gw_spectra_impl<typename G::T, G::N>(c, member, projection, nlog);
 }
 
@@ -921,8 +926,8 @@ struct ShapeEntry {
     PadBase* (*make_pad)();   // the dealiased stepper of the N/2 lattice on this shape
 };
 
-template <class G>
-EngineBase* make_one() { return new Engine<G>(); }
+template <class G, class GY = G>
+EngineBase* make_one() { return new Engine<G, GY>(); }
 
 template <class G>
 PadBase* make_pad_one() {
@@ -931,6 +936,13 @@ PadBase* make_pad_one() {
 }
 
 #define PSC_SHAPE(P, T, N, NAME, ...) {P, N, NAME, &make_one<S<T, N, __VA_ARGS__>>, &make_pad_one<S<T, N, __VA_ARGS__>>}
+// A shape whose y passes run on the second plan GY; not offered to the
+// dealiased stepper, whose y passes are its own.
+#define PSC_SHAPE_Y(P, NAME, G, GY) {P, G::N, NAME, &make_one<G, GY>, nullptr}
+using F512_888 = S<float, 512, Radices<8, 8, 8>, 16, 4>;
+using F512_Y = S<float, 512, Radices<16, 16, 2>, 32, 4>;
+using D512_888 = S<double, 512, Radices<8, 8, 8>, 8, 4>;
+using D512_Y = S<double, 512, Radices<16, 16, 2>, 16, 4>;
 const ShapeEntry kShapes[] = {
     PSC_SHAPE(PSC_FP32, float, 16, "r4.4_tc16_l64", Radices<4, 4>, 16, 64),
     PSC_SHAPE(PSC_FP32, float, 32, "r8.4_tc32_l64", Radices<8, 4>, 32, 64),
@@ -942,6 +954,7 @@ const ShapeEntry kShapes[] = {
     PSC_SHAPE(PSC_FP32, float, 256, "r16.16_tc16_l16", Radices<16, 16>, 16, 16),
     PSC_SHAPE(PSC_FP32, float, 256, "r16.16_tc32_l8", Radices<16, 16>, 32, 8),
     PSC_SHAPE(PSC_FP32, float, 256, "r8.8.4_tc16_l8", Radices<8, 8, 4>, 16, 8),
+    PSC_SHAPE_Y(PSC_FP32, "r8.8.8_tc16_l4_y16.16.2_tc32", F512_888, F512_Y),
     PSC_SHAPE(PSC_FP32, float, 512, "r8.8.8_tc16_l4", Radices<8, 8, 8>, 16, 4),
     PSC_SHAPE(PSC_FP32, float, 512, "r16.16.2_tc16_l8", Radices<16, 16, 2>, 16, 8),
     PSC_SHAPE(PSC_FP32, float, 512, "r8.8.8_tc8_l8", Radices<8, 8, 8>, 8, 8),
@@ -953,17 +966,21 @@ const ShapeEntry kShapes[] = {
     PSC_SHAPE(PSC_FP64, double, 256, "r16.16_tc16_l8", Radices<16, 16>, 16, 8),
     PSC_SHAPE(PSC_FP64, double, 256, "r8.8.4_tc16_l8", Radices<8, 8, 4>, 16, 8),
     PSC_SHAPE(PSC_FP64, double, 256, "r16.16_tc8_l8", Radices<16, 16>, 8, 8),
+    PSC_SHAPE_Y(PSC_FP64, "r8.8.8_tc8_l4_y16.16.2_tc16", D512_888, D512_Y),
     PSC_SHAPE(PSC_FP64, double, 512, "r8.8.8_tc8_l4", Radices<8, 8, 8>, 8, 4),
     PSC_SHAPE(PSC_FP64, double, 512, "r16.16.2_tc8_l4", Radices<16, 16, 2>, 8, 4),
 };
 #undef PSC_SHAPE
+#undef PSC_SHAPE_Y
 constexpr int kNumShapes = sizeof(kShapes) / sizeof(kShapes[0]);
 
-// Indices of the shapes available for (precision, N), default first.
-std::vector<int> shapes_for(int precision, int N) {
+// Indices of the shapes available for (precision, N), default first; with
+// `pad`, those the dealiased stepper can run its 2N passes on.
+std::vector<int> shapes_for(int precision, int N, bool pad = false) {
     std::vector<int> v;
     for (int i = 0; i < kNumShapes; ++i)
-        if (kShapes[i].precision == precision && kShapes[i].N == N) v.push_back(i);
+        if (kShapes[i].precision == precision && kShapes[i].N == N && (!pad || kShapes[i].make_pad))
+            v.push_back(i);
     return v;
 }
 
@@ -1208,8 +1225,8 @@ __global__ void k_adj_init_chi2(const cpx<typename G::T>* __restrict__ Pf, cpx<t
     fbar[idx] = cpx<typename G::T>{(typename G::T)(((double)a.x - b.x) * inv), (typename G::T)(((double)a.y + b.y) * inv)};
 }
 
-template <class G>
-int Engine<G>::adjoint(psc_ctx* c, const psc_adjoint_spec* sp, double* out, double* dchi) {
+template <class G, class GY>
+int Engine<G, GY>::adjoint(psc_ctx* c, const psc_adjoint_spec* sp, double* out, double* dchi) {
     const int M = sp->nsteps;
     const int K = std::min(sp->checkpoint_every, M);
     const int nseg = (M + K - 1) / K;
@@ -1485,7 +1502,7 @@ int psc_create(const psc_config* cfg, psc_ctx** out) {
         }
         c->eng = kShapes[c->shape].make();
         if (cfg->pad == 2) {
-            const std::vector<int> pc = shapes_for(cfg->precision, 2 * N);
+            const std::vector<int> pc = shapes_for(cfg->precision, 2 * N, true);
             if (pc.empty()) return PSC_ERR_UNSUPPORTED;
             c->pad_shape = pc[0];
             if (cfg->autotune) {
@@ -2241,7 +2258,7 @@ int psc_autotune(psc_ctx* c, int nsteps, char* report, int report_len) {
     int rc = copy(true);
     if (rc) { cleanup(); return rc; }
     // the dealiased stepper's time is in its 2N passes: tune those
-    const std::vector<int> cand = shapes_for(c->cfg.precision, pad ? 2 * c->N : c->N);
+    const std::vector<int> cand = shapes_for(c->cfg.precision, pad ? 2 * c->N : c->N, pad);
     const int saved_timing = c->timing;
     c->timing = 0;
     double best_ms = 1e300;
@@ -2304,7 +2321,7 @@ const char* psc_shape_name(const psc_ctx* c) {
 int psc_list_shapes(const psc_ctx* c, char* names, int names_len) {
     if (!c || !names || names_len < 1) return PSC_ERR_ARG;
     std::string s;
-    for (int idx : shapes_for(c->cfg.precision, c->padeng ? 2 * c->N : c->N)) {
+    for (int idx : shapes_for(c->cfg.precision, c->padeng ? 2 * c->N : c->N, c->padeng != nullptr)) {
         if (!s.empty()) s += ",";
         s += kShapes[idx].name;
     }
@@ -2316,7 +2333,7 @@ int psc_list_shapes(const psc_ctx* c, char* names, int names_len) {
 int psc_set_shape(psc_ctx* c, const char* name) {
     if (!c || !name) return PSC_ERR_ARG;
     if (c->padeng) {   // the dealiased stepper's 2N shape
-        for (int idx : shapes_for(c->cfg.precision, 2 * c->N))
+        for (int idx : shapes_for(c->cfg.precision, 2 * c->N, true))
             if (std::strcmp(name, kShapes[idx].name) == 0) {
                 HIPCHK(c, hipStreamSynchronize(c->stream));
                 drop_graphs(c);
