@@ -23,7 +23,8 @@ namespace psc {
 
 // ---------------------------------------------------------------------------
 // A: contiguous-axis inverse pass; DRIFT fuses f += c_drift * pi (evolution.cpp:1164-1169).
-template <class G, bool DRIFT, bool REC = false>
+// REV: lines in reverse order, as in k_strided.
+template <class G, bool DRIFT, bool REC = false, bool REV = false>
 __global__ void __launch_bounds__((G::THREADS_ROW))
 k_z_inverse(const cpx<typename G::T>* __restrict__ src, cpx<typename G::T>* __restrict__ Pf, const cpx<typename G::T>* __restrict__ Pp,
             cpx<typename G::T>* __restrict__ W, const MemberState* __restrict__ ms,
@@ -33,8 +34,8 @@ k_z_inverse(const cpx<typename G::T>* __restrict__ src, cpx<typename G::T>* __re
     constexpr int N = G::N;
     __shared__ cpx<T> lds[G::LayRow::size(G::L)];
     const int t = threadIdx.x % G::TPL, c = threadIdx.x / G::TPL;
-    const int member = blockIdx.y;
-    const size_t line = (size_t)blockIdx.x * G::L + c;
+    const int member = REV ? gridDim.y - 1 - blockIdx.y : blockIdx.y;
+    const size_t line = (size_t)(REV ? gridDim.x - 1 - blockIdx.x : blockIdx.x) * G::L + c;
     const size_t base = (size_t)member * G::MS + (line / N) * G::PS + (line % N) * N + t;
     cpx<T> e[G::E];
     if constexpr (DRIFT) {
@@ -91,17 +92,21 @@ k_z_forward(const cpx<typename G::T>* __restrict__ W, cpx<typename G::T>* __rest
 // B / D: strided-axis pass in place.  AXIS 1 = y (stride N), 0 = x (stride N^2).
 // MAPPED (y passes of the dealiased stepper on its 2N grid): blockIdx.y runs
 // over the N/2 + 1 ... x planes |x| <= N/4 that hold modes; the others stay
-// untouched.
-template <class G, int AXIS, int SIGN, bool MAPPED = false>
+// untouched.  REV: the blocks take the lattice from its far end (last member,
+// last plane first), so that a pass starts on the lines the previous pass
+// wrote last, which are still in the 256 MB memory-side cache.
+template <class G, int AXIS, int SIGN, bool MAPPED = false, bool REV = false>
 __global__ void __launch_bounds__((G::THREADS_COL))
 k_strided(cpx<typename G::T>* __restrict__ W, const cpx<typename G::T>* __restrict__ tw) {
     using T = typename G::T;
     constexpr int N = G::N;
     __shared__ cpx<T> lds[G::LayCol::size(0)];
     const int c = threadIdx.x % G::TC, t = threadIdx.x / G::TC;
-    const int member = blockIdx.z;
-    const size_t z = (size_t)blockIdx.x * G::TC + c;
-    const size_t o = MAPPED ? (size_t)((blockIdx.y + N - N / 4) % N) : (size_t)blockIdx.y;
+    const unsigned bx = REV ? gridDim.x - 1 - blockIdx.x : blockIdx.x;
+    const unsigned by = REV ? gridDim.y - 1 - blockIdx.y : blockIdx.y;
+    const int member = REV ? gridDim.z - 1 - blockIdx.z : blockIdx.z;
+    const size_t z = (size_t)bx * G::TC + c;
+    const size_t o = MAPPED ? (size_t)((by + N - N / 4) % N) : (size_t)by;
     constexpr size_t stride = AXIS == 1 ? (size_t)N : G::PS;
     const size_t base = (size_t)member * G::MS + (AXIS == 1 ? o * G::PS : o * N) + z + t * stride;
     cpx<T> e[G::E];

@@ -276,21 +276,26 @@ struct Engine final : EngineBase {
         dim3 grid(G::BLOCKS_ROW, members), block(G::THREADS_ROW);
         if (drift)
             return timed(c, K_Z_INV_DRIFT, 4 * X, [&] {
-                hipLaunchKernelGGL((k_z_inverse<G, true>), grid, block, 0, c->stream, src, Pf, Pp, W,
+                hipLaunchKernelGGL((k_z_inverse<G, true, false, true>), grid, block, 0, c->stream, src, Pf, Pp, W,
                                    ms, P(c->tw));
             });
         return timed(c, K_Z_INV, 2 * X, [&] {
-            hipLaunchKernelGGL((k_z_inverse<G, false>), grid, block, 0, c->stream, src, Pf, Pp, W, ms,
+            hipLaunchKernelGGL((k_z_inverse<G, false, false, true>), grid, block, 0, c->stream, src, Pf, Pp, W, ms,
                                P(c->tw));
         });
     }
+    // Pass directions alternate so that each pass first reads what its
+    // predecessor wrote last (still in the memory-side cache): z kick
+    // forward, z inverse reversed, y inverse forward, x pass (reads every x
+    // plane per block, no order to exploit), y forward reversed.  At 256^3
+    // FP32 (134 MB arrays) this took the step from 1.007 to 0.958 ms.
     template <int SIGN>
     int launch_y(psc_ctx* c, cpx<T>* W, int members) {
         const double X = (double)members * G::N3 * sizeof(cpx<T>);
         static_assert(GY::N == N && std::is_same_v<typename GY::T, T>, "y plan of another lattice");
         dim3 grid(N / GY::TC, N, members), block(GY::THREADS_COL);
         return timed(c, SIGN > 0 ? K_Y_INV : K_Y_FWD, 2 * X, [&] {
-            hipLaunchKernelGGL((k_strided<GY, 1, SIGN>), grid, block, 0, c->stream, W, P(c->tw));
+            hipLaunchKernelGGL((k_strided<GY, 1, SIGN, false, (SIGN < 0)>), grid, block, 0, c->stream, W, P(c->tw));
         });
     }
     int launch_x_force(psc_ctx* c) {
