@@ -22,6 +22,15 @@
 namespace psc {
 
 // ---------------------------------------------------------------------------
+// Block `lin` of `total` in reverse order: the dispatcher deals consecutive
+// blocks to the eight XCDs in turn, so groups of eight are reversed and the
+// position inside a group kept; each block then lands on the XCD whose L2 saw
+// the same lines in a pass of the same mapping run forward.
+__device__ __forceinline__ unsigned reversed_block(unsigned lin, unsigned total) {
+    return total % 8 == 0 ? total - 8 - (lin & ~7u) + (lin & 7u) : total - 1 - lin;
+}
+
+// ---------------------------------------------------------------------------
 // A: contiguous-axis inverse pass; DRIFT fuses f += c_drift * pi (evolution.cpp:1164-1169).
 // REV: lines in reverse order, as in k_strided.
 template <class G, bool DRIFT, bool REC = false, bool REV = false>
@@ -34,8 +43,10 @@ k_z_inverse(const cpx<typename G::T>* __restrict__ src, cpx<typename G::T>* __re
     constexpr int N = G::N;
     __shared__ cpx<T> lds[G::LayRow::size(G::L)];
     const int t = threadIdx.x % G::TPL, c = threadIdx.x / G::TPL;
-    const int member = REV ? gridDim.y - 1 - blockIdx.y : blockIdx.y;
-    const size_t line = (size_t)(REV ? gridDim.x - 1 - blockIdx.x : blockIdx.x) * G::L + c;
+    unsigned lin = blockIdx.x + gridDim.x * blockIdx.y;
+    if constexpr (REV) lin = reversed_block(lin, gridDim.x * gridDim.y);
+    const int member = (int)(lin / gridDim.x);
+    const size_t line = (size_t)(lin % gridDim.x) * G::L + c;
     const size_t base = (size_t)member * G::MS + (line / N) * G::PS + (line % N) * N + t;
     cpx<T> e[G::E];
     if constexpr (DRIFT) {
@@ -93,8 +104,9 @@ k_z_forward(const cpx<typename G::T>* __restrict__ W, cpx<typename G::T>* __rest
 // MAPPED (y passes of the dealiased stepper on its 2N grid): blockIdx.y runs
 // over the N/2 + 1 ... x planes |x| <= N/4 that hold modes; the others stay
 // untouched.  REV: the blocks take the lattice from its far end (last member,
-// last plane first), so that a pass starts on the lines the previous pass
-// wrote last, which are still in the 256 MB memory-side cache.
+// last plane first, in the order of reversed_block), so that a pass starts on
+// the lines the previous pass wrote last, which are still in the 256 MB
+// memory-side cache.
 template <class G, int AXIS, int SIGN, bool MAPPED = false, bool REV = false>
 __global__ void __launch_bounds__((G::THREADS_COL))
 k_strided(cpx<typename G::T>* __restrict__ W, const cpx<typename G::T>* __restrict__ tw) {
@@ -102,9 +114,15 @@ k_strided(cpx<typename G::T>* __restrict__ W, const cpx<typename G::T>* __restri
     constexpr int N = G::N;
     __shared__ cpx<T> lds[G::LayCol::size(0)];
     const int c = threadIdx.x % G::TC, t = threadIdx.x / G::TC;
-    const unsigned bx = REV ? gridDim.x - 1 - blockIdx.x : blockIdx.x;
-    const unsigned by = REV ? gridDim.y - 1 - blockIdx.y : blockIdx.y;
-    const int member = REV ? gridDim.z - 1 - blockIdx.z : blockIdx.z;
+    unsigned bx = blockIdx.x, by = blockIdx.y;
+    int member = blockIdx.z;
+    if constexpr (REV) {
+        const unsigned lin = reversed_block(blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z),
+                                            gridDim.x * gridDim.y * gridDim.z);
+        bx = lin % gridDim.x;
+        by = (lin / gridDim.x) % gridDim.y;
+        member = (int)(lin / (gridDim.x * gridDim.y));
+    }
     const size_t z = (size_t)bx * G::TC + c;
     const size_t o = MAPPED ? (size_t)((by + N - N / 4) % N) : (size_t)by;
     constexpr size_t stride = AXIS == 1 ? (size_t)N : G::PS;
