@@ -59,6 +59,7 @@ struct Shape {
     using LayRow = LayoutRow<N, R1>;
     using LayCol = LayoutCol<N, TC>;
     static_assert(THREADS_COL <= 1024 && THREADS_ROW <= 1024, "block too large");
+    static_assert(TC >= 8 && L >= 4, "the partial-sum buffers hold (N / 8) N column and N^2 / 4 row blocks");
     static_assert(LayCol::size(0) * sizeof(cpx<T>) <= 131072, "strided tile exceeds 128 KiB of LDS");
     static_assert(LayRow::size(L) * sizeof(cpx<T>) <= 65536, "row tile exceeds 64 KiB of LDS");
 };

@@ -843,11 +843,13 @@ int Engine<G, GY>::gw_spectra(psc_ctx* c, int member, int projection, int nlog) 
     return gw_spectra_impl<typename G::T, G::N>(c, member, projection, nlog);
 }
 
-// The dealiased stepper's passes on the padded shape GP (N_pad = GP::N = 2N).
-template <class GP>
+// The dealiased stepper's passes on the padded shape GP (N_pad = GP::N = 2N);
+// the two z-axis row passes (padding, extraction) may run on another plan GR.
+template <class GP, class GR = GP>
 struct PadEngine final : PadBase {
     using T = typename GP::T;
     static constexpr int NP = GP::N, N = NP / 2;
+    static_assert(GR::N == NP && std::is_same_v<typename GR::T, T>, "row plan of another lattice");
     static cpx<T>* P(void* p) { return static_cast<cpx<T>*>(p); }
     static constexpr double XN = (double)N * N * N * sizeof(cpx<T>);
     static constexpr double XP = 8.0 * XN;
@@ -860,7 +862,7 @@ struct PadEngine final : PadBase {
     // the sum of V in the x pass
     int front(psc_ctx* c) {
         int rc = timed(c, K_PAD_Z_INV, XN + XP * (N + 1) / (2.0 * N), [&] {
-            hipLaunchKernelGGL((k_pad_z<GP>), dim3(GP::BLOCKS_ROW), dim3(GP::THREADS_ROW), 0, c->stream, P(c->Pf),
+            hipLaunchKernelGGL((k_pad_z<GR>), dim3(GR::BLOCKS_ROW), dim3(GR::THREADS_ROW), 0, c->stream, P(c->Pf),
                                P(c->Gf), P(c->Wp), P(c->dcp), P(c->twp));
         });
         if (rc) return rc;
@@ -890,7 +892,7 @@ struct PadEngine final : PadBase {
         });
         if (rc) return rc;
         rc = timed(c, K_PAD_Z_FWD, XN * (N + 1) * (N + 1) / ((double)N * N) * 3, [&] {
-            hipLaunchKernelGGL((k_unpad_z<GP>), dim3(GP::BLOCKS_ROW), dim3(GP::THREADS_ROW), 0, c->stream, P(c->Wp),
+            hipLaunchKernelGGL((k_unpad_z<GR>), dim3(GR::BLOCKS_ROW), dim3(GR::THREADS_ROW), 0, c->stream, P(c->Wp),
                                P(c->Rp), P(c->twp));
         });
         if (rc) return rc;
@@ -927,16 +929,16 @@ using S = Shape<T, N, R, TC, L>;
 struct ShapeEntry {
     int precision, N;
     const char* name;
-    EngineBase* (*make)();
-    PadBase* (*make_pad)();   // the dealiased stepper of the N/2 lattice on this shape
+    EngineBase* (*make)();    // the stepper of the N lattice, or null (padded stepper only)
+    PadBase* (*make_pad)();   // the dealiased stepper of the N/2 lattice, or null
 };
 
 template <class G, class GY = G>
 EngineBase* make_one() { return new Engine<G, GY>(); }
 
-template <class G>
+template <class G, class GR = G>
 PadBase* make_pad_one() {
-    if constexpr (G::N >= 32) return new PadEngine<G>();
+    if constexpr (G::N >= 32) return new PadEngine<G, GR>();
     else return nullptr;
 }
 
@@ -944,8 +946,11 @@ PadBase* make_pad_one() {
 // A shape whose y passes run on the second plan GY; not offered to the
 // dealiased stepper, whose y passes are its own.
 #define PSC_SHAPE_Y(P, NAME, G, GY) {P, G::N, NAME, &make_one<G, GY>, nullptr}
+// A shape of the padded stepper only, its z row passes on GR.
+#define PSC_SHAPE_PAD(P, NAME, G, GR) {P, G::N, NAME, nullptr, &make_pad_one<G, GR>}
 using F512_888 = S<float, 512, Radices<8, 8, 8>, 16, 4>;
 using F512_Y = S<float, 512, Radices<16, 16, 2>, 32, 4>;
+using F512_1616 = S<float, 512, Radices<16, 16, 2>, 16, 8>;
 using D512_888 = S<double, 512, Radices<8, 8, 8>, 8, 4>;
 using D512_Y = S<double, 512, Radices<16, 16, 2>, 16, 4>;
 const ShapeEntry kShapes[] = {
@@ -959,6 +964,7 @@ const ShapeEntry kShapes[] = {
     PSC_SHAPE(PSC_FP32, float, 256, "r16.16_tc16_l16", Radices<16, 16>, 16, 16),
     PSC_SHAPE(PSC_FP32, float, 256, "r16.16_tc32_l8", Radices<16, 16>, 32, 8),
     PSC_SHAPE(PSC_FP32, float, 256, "r8.8.4_tc16_l8", Radices<8, 8, 4>, 16, 8),
+    PSC_SHAPE_PAD(PSC_FP32, "r16.16.2_tc16_l8_z8.8.8_l4", F512_1616, F512_888),
     PSC_SHAPE_Y(PSC_FP32, "r8.8.8_tc16_l4_y16.16.2_tc32", F512_888, F512_Y),
     PSC_SHAPE(PSC_FP32, float, 512, "r8.8.8_tc16_l4", Radices<8, 8, 8>, 16, 4),
     PSC_SHAPE(PSC_FP32, float, 512, "r16.16.2_tc16_l8", Radices<16, 16, 2>, 16, 8),
@@ -977,6 +983,7 @@ const ShapeEntry kShapes[] = {
 };
 #undef PSC_SHAPE
 #undef PSC_SHAPE_Y
+#undef PSC_SHAPE_PAD
 constexpr int kNumShapes = sizeof(kShapes) / sizeof(kShapes[0]);
 
 // Indices of the shapes available for (precision, N), default first; with
@@ -984,7 +991,8 @@ constexpr int kNumShapes = sizeof(kShapes) / sizeof(kShapes[0]);
 std::vector<int> shapes_for(int precision, int N, bool pad = false) {
     std::vector<int> v;
     for (int i = 0; i < kNumShapes; ++i)
-        if (kShapes[i].precision == precision && kShapes[i].N == N && (!pad || kShapes[i].make_pad))
+        if (kShapes[i].precision == precision && kShapes[i].N == N &&
+            (pad ? kShapes[i].make_pad != nullptr : kShapes[i].make != nullptr))
             v.push_back(i);
     return v;
 }
@@ -1529,10 +1537,12 @@ int psc_create(const psc_config* cfg, psc_ctx** out) {
         c->dev_bytes += (double)bytes;
         return true;
     };
-    // partial-sum sizes: max blocks of any pass x widths
+    // partial-sum sizes: the most blocks any shape of the table can launch
+    // (TC >= 8, L >= 4, asserted in Shape), also after a shape switch; the
+    // dealiased stepper's x pass runs on its 2N grid
     size_t blocks_col = (size_t)(N / 8) * N;
     const size_t blocks_row = (size_t)N * N / 4;
-    if (c->padeng) blocks_col = std::max(blocks_col, (size_t)c->padeng->nV());
+    if (c->padeng) blocks_col = std::max(blocks_col, (size_t)(2 * N / 8) * (2 * N));
     c->part_len = std::max<size_t>(std::max(blocks_col * 6, (size_t)256 * 4),
                                    (size_t)128 * (maxnumbins(N) * 4 + kMaxLogBins));
     c->sums_len = (size_t)maxnumbins(N) * 4 + kMaxLogBins + 64;

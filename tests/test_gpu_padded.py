@@ -134,6 +134,31 @@ def test_padded_shapes_agree_and_autotune_restores_the_state():
     lat.close()
 
 
+def test_padded_256_shapes_agree():
+    """Every 512-point plan of the dealiased 256^3 stepper, the one with its z
+    row passes on a second plan included, against the default, FP32."""
+    cfg, model, ic = parity_setup(N=256, integrator=PSC_Y4, precision=PSC_FP32, pad=2)
+    fk, pk, sc = O.generate_ic(cfg, model, ic)
+    lat = Lattice(cfg)
+    lat.set_model(0, model)
+    shapes = lat.shapes()
+    print("padded 256 shapes", shapes, flush=True)
+    assert any("_z" in s for s in shapes) and lat.shape_name() == shapes[0]
+    ref = None
+    for name in shapes:
+        lat.set_shape(name)
+        lat.load_ic_k(0, fk, pk, sc[0], sc[1])
+        lat.step(2)
+        got = lat.get_k()
+        if ref is None:
+            ref = got
+            continue
+        e = max(rel_err(got[0], ref[0]), rel_err(got[1], ref[1]))
+        print(f"{name}: {e:.3e}")
+        assert e <= 2e-6, name
+    lat.close()
+
+
 def test_reload_after_steps_matches_a_fresh_context():
     """Loading new data clears the plane ghosts left by earlier steps before
     d2a/dt2 is formed from K, G and the padded V."""
