@@ -679,8 +679,10 @@ k_z_inverse_deriv(const cpx<typename G::T>* __restrict__ Pf, cpx<typename G::T>*
 }
 
 
-// A (GW): drift of fields and u, then z-axis inverse of the fields and of the
-// three gradient spectra i kIR n_d P_f.
+// A (GW): drift of fields and u, then z-axis inverse of the fields (W0) and
+// of the z gradient i kIR n_z P_f (W3).  The x and y gradients need no z
+// pass of their own: i kIR n_x and i kIR n_y are constant along a z line, so
+// they are formed from W0 in the y and x passes (k_y_inverse_gw, k_x_source).
 template <class G, bool DRIFT, bool REC = false>
 __global__ void __launch_bounds__((G::THREADS_ROW))
 k_z_inverse_gw(cpx<typename G::T>* __restrict__ Pf, const cpx<typename G::T>* __restrict__ Pp, cpx<typename G::T>* __restrict__ W,
@@ -695,8 +697,6 @@ k_z_inverse_gw(cpx<typename G::T>* __restrict__ Pf, const cpx<typename G::T>* __
     const int member = blockIdx.y;
     const size_t line = (size_t)blockIdx.x * G::L + c;
     const size_t base = (size_t)member * G::MS + (line / N) * G::PS + (line % N) * N + t;
-    const int ix = (int)(line / N), iy = (int)(line % N);
-    const T kx = (T)(kIR * deriv_index(ix, N)), ky = (T)(kIR * deriv_index(iy, N));
     const T cd = DRIFT ? (T)ms[member].c_drift : (T)0;
     cpx<T> e[G::E];
     cpx<T> f[G::E];
@@ -721,17 +721,13 @@ k_z_inverse_gw(cpx<typename G::T>* __restrict__ Pf, const cpx<typename G::T>* __
 #pragma unroll
     for (int m = 0; m < G::E; ++m) W[base + G::TPL * m] = e[m];
 #pragma unroll
-    for (int d = 0; d < 3; ++d) {
-#pragma unroll
-        for (int m = 0; m < G::E; ++m) {
-            const int kz = t + G::TPL * m;
-            const T kd = d == 0 ? kx : (d == 1 ? ky : (T)(kIR * deriv_index(kz, N)));
-            e[m] = cpx<T>{-kd * f[m].y, kd * f[m].x};
-        }
-        line_fft<T, N, typename G::Rad, +1, typename G::LayRow>(e, t, c, lds, tw);
-#pragma unroll
-        for (int m = 0; m < G::E; ++m) W[(size_t)(d + 1) * astride + base + G::TPL * m] = e[m];
+    for (int m = 0; m < G::E; ++m) {
+        const T kz = (T)(kIR * deriv_index(t + G::TPL * m, N));
+        e[m] = cpx<T>{-kz * f[m].y, kz * f[m].x};
     }
+    line_fft<T, N, typename G::Rad, +1, typename G::LayRow>(e, t, c, lds, tw);
+#pragma unroll
+    for (int m = 0; m < G::E; ++m) W[3 * astride + base + G::TPL * m] = e[m];
     if constexpr (DRIFT) {
 #pragma unroll
         for (int a = 0; a < 3; ++a)
@@ -751,14 +747,63 @@ k_z_inverse_gw(cpx<typename G::T>* __restrict__ Pf, const cpx<typename G::T>* __
     }
 }
 
-// C (GW), last part: the x-axis inverse transforms of the three gradient
-// arrays, the packed source S = (S11 + i S22, S33 + i S12, S13 + i S23) from
-// the real-space gradients, and the three x-axis forward transforms, in one
-// pass over the three arrays.
+// B (GW): y-axis inverse of the fields W0 and of the z gradient W3, and the
+// y gradient i kIR n_y W0 formed from the fields' column before its
+// transform, into W2.  W1 is not read (the x gradient comes from W0 in
+// k_x_source).  The fields' column is read twice (the second time from the
+// cache) rather than held through the gradient's transform: one column in
+// registers at a time keeps the 16-element plans free of spills.
+template <class G>
+__global__ void __launch_bounds__((G::THREADS_COL))
+k_y_inverse_gw(cpx<typename G::T>* __restrict__ W, const cpx<typename G::T>* __restrict__ tw, size_t astride,
+               double kIR) {
+    using T = typename G::T;
+    constexpr int N = G::N;
+    __shared__ cpx<T> lds[G::LayCol::size(0)];
+    const int c = threadIdx.x % G::TC, t = threadIdx.x / G::TC;
+    const int member = blockIdx.z;
+    const size_t z = (size_t)blockIdx.x * G::TC + c;
+    const size_t base = (size_t)member * G::MS + (size_t)blockIdx.y * G::PS + z + (size_t)t * N;
+    cpx<T> e[G::E];
+#pragma unroll
+    for (int m = 0; m < G::E; ++m) {
+        const cpx<T> f = W[base + (size_t)G::TPL * m * N];
+        const T ky = (T)(kIR * deriv_index(t + G::TPL * m, N));
+        e[m] = cpx<T>{-ky * f.y, ky * f.x};
+    }
+    line_fft<T, N, typename G::Rad, +1, typename G::LayCol>(e, t, c, lds, tw);
+#pragma unroll
+    for (int m = 0; m < G::E; ++m) W[2 * astride + base + (size_t)G::TPL * m * N] = e[m];
+#pragma unroll
+    for (int m = 0; m < G::E; ++m) e[m] = W[base + (size_t)G::TPL * m * N];
+    {   // twiddles reloaded (cached) rather than kept live across transforms
+        const cpx<T>* twa = tw;
+        asm volatile("" : "+s"(twa));
+        line_fft<T, N, typename G::Rad, +1, typename G::LayCol>(e, t, c, lds, twa);
+    }
+#pragma unroll
+    for (int m = 0; m < G::E; ++m) W[base + (size_t)G::TPL * m * N] = e[m];
+#pragma unroll
+    for (int m = 0; m < G::E; ++m) e[m] = W[3 * astride + base + (size_t)G::TPL * m * N];
+    {   // twiddles reloaded (cached) rather than kept live across transforms
+        const cpx<T>* twa = tw;
+        asm volatile("" : "+s"(twa));
+        line_fft<T, N, typename G::Rad, +1, typename G::LayCol>(e, t, c, lds, twa);
+    }
+#pragma unroll
+    for (int m = 0; m < G::E; ++m) W[3 * astride + base + (size_t)G::TPL * m * N] = e[m];
+}
+
+// C (GW), first part: the x-axis inverse transforms of the three gradients
+// (the x gradient i kIR n_x W0 formed from the fields' column, the y and z
+// gradients from W2, W3), the packed source S = (S11 + i S22, S33 + i S12,
+// S13 + i S23) from the real-space gradients, and the three x-axis forward
+// transforms into W1..W3, in one pass.  It runs before k_x_force, which
+// overwrites W0.
 template <class G, bool REC = false>
 __global__ void __launch_bounds__((G::THREADS_COL))
 k_x_source(cpx<typename G::T>* __restrict__ W, const cpx<typename G::T>* __restrict__ tw, size_t astride,
-           cpx<typename G::T>* __restrict__ rec_g = nullptr) {
+           double kIR, cpx<typename G::T>* __restrict__ rec_g = nullptr) {
     using T = typename G::T;
     constexpr int N = G::N;
     __shared__ cpx<T> lds[G::LayCol::size(0)];
@@ -771,7 +816,13 @@ k_x_source(cpx<typename G::T>* __restrict__ W, const cpx<typename G::T>* __restr
     const T inv = (T)(1.0 / (double)G::N3);
     cpx<T> g[3][G::E];
 #pragma unroll
-    for (int a = 0; a < 3; ++a)
+    for (int m = 0; m < G::E; ++m) {
+        const cpx<T> f = W[base + (size_t)G::TPL * m * stride];
+        const T kx = (T)(kIR * deriv_index(t + G::TPL * m, N));
+        g[0][m] = cpx<T>{-kx * f.y, kx * f.x};
+    }
+#pragma unroll
+    for (int a = 1; a < 3; ++a)
 #pragma unroll
         for (int m = 0; m < G::E; ++m) g[a][m] = W[(size_t)(a + 1) * astride + base + (size_t)G::TPL * m * stride];
 #pragma unroll

@@ -39,7 +39,7 @@ enum Kind {
     K_Z_INV_GW,
     K_X_FORCE_GW,
     K_Z_KICK_GW,
-    K_X_INV_GW,
+    K_Y_INV_GW,
     K_PAD_DRIFT,
     K_PAD_Z_INV,
     K_PAD_Y_INV,
@@ -52,7 +52,7 @@ enum Kind {
 const char* kKindNames[K_NKINDS] = {"z_inverse_drift", "y_inverse", "x_force", "y_forward",
                                     "z_forward_kick", "scalars", "z_inverse", "z_forward_kick_first",
                                     "diagnostics", "z_inverse_drift_gw", "x_source_gw", "z_forward_kick_gw",
-                                    "x_inverse_gw", "pad_drift", "pad_z_inverse", "pad_y_inverse", "pad_x_force",
+                                    "y_inverse_gw", "pad_drift", "pad_z_inverse", "pad_y_inverse", "pad_x_force",
                                     "pad_y_forward", "pad_z_forward", "pad_kick"};
 
 // Chunks of the first reduction level for passes with many blocks.
@@ -346,6 +346,15 @@ struct Engine final : EngineBase {
         });
     }
 
+    // y inverse of the GW chain: fields and z gradient in place, y gradient
+    // formed from the fields (read 2X, write 3X)
+    int launch_y_inverse_gw(psc_ctx* c, size_t astride) {
+        const double X = (double)c->B * G::N3 * sizeof(cpx<T>);
+        return timed(c, K_Y_INV_GW, 5 * X, [&] {
+            hipLaunchKernelGGL((k_y_inverse_gw<GY>), dim3(N / GY::TC, N, c->B), dim3(GY::THREADS_COL), 0, c->stream,
+                               P(c->W), P(c->tw), astride, c->cfg.kIR);
+        });
+    }
     int force_chain_gw(psc_ctx* c, bool drift, bool reduce, double c1, double c2, int use_a_start) {
         const size_t astride = (size_t)c->B * G::MS;
         const double X = (double)c->B * G::N3 * sizeof(cpx<T>);
@@ -353,25 +362,25 @@ struct Engine final : EngineBase {
         int rc;
         dim3 rgrid(G::BLOCKS_ROW, c->B), rblock(G::THREADS_ROW);
         if (drift)
-            rc = timed(c, K_Z_INV_GW, 16 * X, [&] {
+            rc = timed(c, K_Z_INV_GW, 14 * X, [&] {
                 hipLaunchKernelGGL((k_z_inverse_gw<G, true>), rgrid, rblock, 0, c->stream, P(c->Pf), P(c->Pp),
                                    P(c->W), P(c->U), P(c->PU), c->ms, P(c->tw), astride, c->cfg.kIR);
             });
         else
-            rc = timed(c, K_Z_INV_GW, 5 * X, [&] {
+            rc = timed(c, K_Z_INV_GW, 3 * X, [&] {
                 hipLaunchKernelGGL((k_z_inverse_gw<G, false>), rgrid, rblock, 0, c->stream, P(c->Pf), P(c->Pp),
                                    P(c->W), P(c->U), P(c->PU), c->ms, P(c->tw), astride, c->cfg.kIR);
             });
         if (rc) return rc;
-        if ((rc = launch_y<+1>(c, P(c->W), 4 * c->B))) return rc;
-        // fields: x inverse, V', x forward; gradients: x inverse, the source
-        // and its x forward transforms in one pass
-        if ((rc = launch_x_force(c))) return rc;
+        if ((rc = launch_y_inverse_gw(c, astride))) return rc;
+        // gradients: x inverse, the source and its x forward transforms in
+        // one pass, before the fields' x pass overwrites W0 with V'
         rc = timed(c, K_X_FORCE_GW, 6 * X, [&] {
             hipLaunchKernelGGL((k_x_source<G>), dim3(N / G::TC, N, c->B), dim3(G::THREADS_COL), 0, c->stream,
-                               P(c->W), P(c->tw), astride);
+                               P(c->W), P(c->tw), astride, c->cfg.kIR, nullptr);
         });
         if (rc) return rc;
+        if ((rc = launch_x_force(c))) return rc;
         if ((rc = launch_y<-1>(c, P(c->W), 4 * c->B))) return rc;
         if (reduce)
             return timed(c, K_Z_KICK_GW, 16 * X, [&] {
@@ -695,12 +704,12 @@ struct Engine final : EngineBase {
         if (rc) return rc;
         dim3 rgrid(G::BLOCKS_ROW, 1), rblock(G::THREADS_ROW), cgrid(N / G::TC, N, 1), cblock(G::THREADS_COL);
         auto middle = [&](int jb) -> int {   // y inverse .. y forward of the GW chain at boundary jb
-            int r2 = launch_y<+1>(c, P(c->W), 4);
+            int r2 = launch_y_inverse_gw(c, as);
             if (r2) return r2;
+            hipLaunchKernelGGL((k_x_source<G, true>), cgrid, cblock, 0, c->stream, P(c->W), P(c->tw), as, kIR,
+                               P(w.g[jb]));
             hipLaunchKernelGGL((k_x_force<G, true>), cgrid, cblock, 0, c->stream, P(c->W), c->ms, P(c->tw),
                                c->partV, P(c->Pf), P(w.r[jb]));
-            hipLaunchKernelGGL((k_x_source<G, true>), cgrid, cblock, 0, c->stream, P(c->W), P(c->tw), as,
-                               P(w.g[jb]));
             return launch_y<-1>(c, P(c->W), 4);
         };
         // prologue force at (f_0, a_0, u_0): first kicks
