@@ -91,3 +91,39 @@ def test_gw_graph_replay_matches_eager():
         a, b = lat.get_gw_k(0), g.get_gw_k(0)
         np.testing.assert_array_equal(a[0], b[0])
         np.testing.assert_array_equal(a[1], b[1])
+
+
+def test_gw_512_shapes_agree():
+    """At 512^3 the default shape runs the y passes, the GW y inverse among
+    them, on a second plan (radix 16/16/2, 32 columns per block); every 512
+    shape gives the same change of the GW and field spectra over four
+    Yoshida-4 steps, FP32 (the change, not the spectra themselves, which the
+    initial vacuum modes dominate)."""
+    cfg, model, ic = parity_setup(N=512, with_gw=1, integrator=PSC_Y4, precision=PSC_FP32)
+    lat = Lattice(cfg)
+    lat.set_model(0, model)
+    shapes = lat.shapes()
+    print("512 GW shapes", shapes, flush=True)
+    assert "_y" in shapes[0]
+    ref = None
+    for name in shapes:
+        lat.set_shape(name)
+        lat.generate_ic_device(0, ic)
+        lat.generate_gw_ic_device(0, ic)
+        s0, _ = lat.gw_spectrum(PSC_GW_PROJ_TT, 0)
+        f0 = lat.spectra(0, 0)
+        lat.step(4)
+        s1, _ = lat.gw_spectrum(PSC_GW_PROJ_TT, 0)
+        f1 = lat.spectra(0, 0)
+        ds, df = s1 - s0, f1 - f0
+        if ref is None:
+            ref = (ds, df)
+            print(f"{name}: GW spectrum change {np.abs(ds).max() / np.abs(s0).max():.2e} of its size, "
+                  f"field spectrum {np.abs(df).max() / np.abs(f0).max():.2e}", flush=True)
+            assert np.all(np.isfinite(s1)) and np.abs(ds).max() > 1e-9 * np.abs(s0).max()
+            continue
+        es = float(np.abs(ds - ref[0]).max() / np.abs(ref[0]).max())
+        ef = float(np.abs(df - ref[1]).max() / np.abs(ref[1]).max())
+        print(f"{name}: GW change {es:.2e} field change {ef:.2e}", flush=True)
+        assert es <= 1e-3 and ef <= 1e-3, name
+    lat.close()
