@@ -103,7 +103,9 @@ k_z_forward(const cpx<typename G::T>* __restrict__ W, cpx<typename G::T>* __rest
 // B / D: strided-axis pass in place.  AXIS 1 = y (stride N), 0 = x (stride N^2).
 // MAPPED (y passes of the dealiased stepper on its 2N grid): blockIdx.y runs
 // over the N/2 + 1 ... x planes |x| <= N/4 that hold modes; the others stay
-// untouched.  REV: the blocks take the lattice from its far end (last member,
+// untouched.  Only the rows |y| <= N/4 of a spectrum hold modes: the inverse
+// reads the others as zero (the padding pass does not write them) and the
+// forward does not store them (the extraction pass does not read them).  REV: the blocks take the lattice from its far end (last member,
 // last plane first, in the order of reversed_block), so that a pass starts on
 // the lines the previous pass wrote last, which are still in the 256 MB
 // memory-side cache.
@@ -127,12 +129,19 @@ k_strided(cpx<typename G::T>* __restrict__ W, const cpx<typename G::T>* __restri
     const size_t o = MAPPED ? (size_t)((by + N - N / 4) % N) : (size_t)by;
     constexpr size_t stride = AXIS == 1 ? (size_t)N : G::PS;
     const size_t base = (size_t)member * G::MS + (AXIS == 1 ? o * G::PS : o * N) + z + t * stride;
+    static_assert(!MAPPED || AXIS == 1, "mapped passes run along y");
+    auto kept = [&](int m) {   // spectrum row y = t + TPL m holds modes
+        const int y = t + G::TPL * m;
+        return !MAPPED || y <= N / 4 || y >= N - N / 4;
+    };
     cpx<T> e[G::E];
 #pragma unroll
-    for (int m = 0; m < G::E; ++m) e[m] = W[base + (size_t)G::TPL * m * stride];
+    for (int m = 0; m < G::E; ++m)
+        e[m] = (SIGN < 0 || kept(m)) ? W[base + (size_t)G::TPL * m * stride] : cpx<T>{(T)0, (T)0};
     line_fft<T, N, typename G::Rad, SIGN, typename G::LayCol>(e, t, c, lds, tw);
 #pragma unroll
-    for (int m = 0; m < G::E; ++m) W[base + (size_t)G::TPL * m * stride] = e[m];
+    for (int m = 0; m < G::E; ++m)
+        if (SIGN > 0 || kept(m)) W[base + (size_t)G::TPL * m * stride] = e[m];
 }
 
 // ---------------------------------------------------------------------------

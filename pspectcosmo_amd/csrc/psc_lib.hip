@@ -862,6 +862,7 @@ struct PadEngine final : PadBase {
     static cpx<T>* P(void* p) { return static_cast<cpx<T>*>(p); }
     static constexpr double XN = (double)N * N * N * sizeof(cpx<T>);
     static constexpr double XP = 8.0 * XN;
+    static constexpr double KP = (N + 1.0) / (2.0 * N);   // share of a padded axis holding modes
 
     int nV() const override { return GP::BLOCKS_COL; }
     int nKG() const override { return (int)((size_t)N * N * N / 2048); }
@@ -870,18 +871,19 @@ struct PadEngine final : PadBase {
     // padding with the z-axis inverse, then the y-axis inverse, then V' and
     // the sum of V in the x pass
     int front(psc_ctx* c) {
-        int rc = timed(c, K_PAD_Z_INV, XN + XP * (N + 1) / (2.0 * N), [&] {
+        int rc = timed(c, K_PAD_Z_INV, XN + XP * KP * KP, [&] {
             hipLaunchKernelGGL((k_pad_z<GR>), dim3(GR::BLOCKS_ROW), dim3(GR::THREADS_ROW), 0, c->stream, P(c->Pf),
                                P(c->Gf), P(c->Wp), P(c->dcp), P(c->twp));
         });
         if (rc) return rc;
-        // the y passes and the stored x rows cover the padded x planes |x| <= N/2
-        rc = timed(c, K_PAD_Y_INV, XP * (N + 1) / N, [&] {
+        // the y passes and the stored x rows cover the padded x planes |x| <= N/2;
+        // the y inverse reads the rows |y| <= N/2 of them
+        rc = timed(c, K_PAD_Y_INV, XP * KP * (KP + 1.0), [&] {
             hipLaunchKernelGGL((k_strided<GP, 1, +1, true>), dim3(NP / GP::TC, N + 1, 1), dim3(GP::THREADS_COL), 0,
                                c->stream, P(c->Wp), P(c->twp));
         });
         if (rc) return rc;
-        return timed(c, K_PAD_X_FORCE, XP * (N + 1) / N, [&] {
+        return timed(c, K_PAD_X_FORCE, XP * KP * 2.0, [&] {
             hipLaunchKernelGGL((k_x_force<GP, false, true>), dim3(NP / GP::TC, NP, 1), dim3(GP::THREADS_COL), 0,
                                c->stream, P(c->Wp), c->ms, P(c->twp), c->partV, P(c->dcp));
         });
@@ -895,7 +897,7 @@ struct PadEngine final : PadBase {
             })))
             return rc;
         if ((rc = front(c))) return rc;
-        rc = timed(c, K_PAD_Y_FWD, XP * (N + 1) / N, [&] {
+        rc = timed(c, K_PAD_Y_FWD, XP * KP * (1.0 + KP), [&] {
             hipLaunchKernelGGL((k_strided<GP, 1, -1, true>), dim3(NP / GP::TC, N + 1, 1), dim3(GP::THREADS_COL), 0,
                                c->stream, P(c->Wp), P(c->twp));
         });

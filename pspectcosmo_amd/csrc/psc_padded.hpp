@@ -82,12 +82,18 @@ k_pad_z(const cpx<typename GP::T>* __restrict__ Pf, const cpx<typename GP::T>* _
     cpx<T> e[GP::E];
     const bool mx = sx <= H && sx >= -H;
     // Padded x planes |x| > N/2 hold no modes: the y passes skip them and the
-    // x pass reads them as zero, so nothing is written there.  With L <= N_pad
-    // every line of a block has the same padded x index and such a block
-    // leaves before the transform's barriers; otherwise its lines transform
-    // zeros and skip the store.
+    // x pass reads them as zero, so nothing is written there; likewise the
+    // rows |y| > N/2 of the kept planes, which the y inverse reads as zero.
+    // With L <= N_pad every line of a block has the same padded x index and L
+    // consecutive padded y: a block without modes leaves before the
+    // transform's barriers; otherwise lines without modes transform zeros and
+    // skip the store.
     if constexpr (GP::L <= NP) {
         if (!mx) return;
+        const int y0 = pad_signed((int)(((size_t)blockIdx.x * GP::L) % NP), NP);
+        const int y1 = pad_signed((int)(((size_t)blockIdx.x * GP::L + GP::L - 1) % NP), NP);
+        if (y0 > H && y1 > H) return;            // both ends in (H, N) (no wrap: L <= N and blocks align)
+        if (y0 < -H && y1 < -H) return;          // both ends in [-N, -H)
     }
     const bool my = mx && sy <= H && sy >= -H;
     const int x = pad_wrap(sx, N), y = pad_wrap(sy, N);
@@ -115,7 +121,7 @@ k_pad_z(const cpx<typename GP::T>* __restrict__ Pf, const cpx<typename GP::T>* _
         e[m] = v;
     }
     line_fft<T, NP, typename GP::Rad, +1, typename GP::LayRow>(e, t, c, lds, tw);
-    if (!mx) return;
+    if (!my) return;
 #pragma unroll
     for (int m = 0; m < GP::E; ++m) Wp[base + GP::TPL * m] = e[m];
 }
