@@ -462,6 +462,7 @@ struct Engine final : EngineBase {
             for (int s = 0; s < nsteps; ++s)
                 if ((rc = record_step(c, s == nsteps - 1))) return rc;
         }
+        HIPCHK(c, hipGetLastError());   // a failed launch is reported here, not by a later call
         return c->timing ? drain_timing(c) : PSC_OK;
     }
 
@@ -1091,6 +1092,10 @@ void unpack_half(int N, int nflds, const double* full, double* hk) {
             }
 }
 
+// Every copy and fill runs on the context's stream and is waited for there:
+// that stream is non-blocking, so work on the null stream (hipMemcpy,
+// hipMemset) would not be ordered with its kernels and could overlap them.
+
 // Copy one unpadded N^3 complex FP64 array into slot `member` of a padded
 // device array (x-planes N (N + 1) apart), converting to the field precision.
 int upload(psc_ctx* c, void* dst_base, int member, const std::vector<double>& full) {
@@ -1100,9 +1105,11 @@ int upload(psc_ctx* c, void* dst_base, int member, const std::vector<double>& fu
     if (c->cfg.precision == PSC_FP32) {
         std::vector<float> tmp(2 * n);
         for (size_t i = 0; i < 2 * n; ++i) tmp[i] = (float)full[i];
-        HIPCHK(c, hipMemcpy2D(dst, dpitch, tmp.data(), width, width, c->N, hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy2DAsync(dst, dpitch, tmp.data(), width, width, c->N, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
     } else {
-        HIPCHK(c, hipMemcpy2D(dst, dpitch, full.data(), width, width, c->N, hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy2DAsync(dst, dpitch, full.data(), width, width, c->N, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
     }
     return PSC_OK;
 }
@@ -1112,13 +1119,14 @@ int download(psc_ctx* c, const void* src_base, int member, std::vector<double>& 
     const char* src = static_cast<const char*>(src_base) + (size_t)member * c->MS * c->es;
     const size_t spitch = plane_stride(c->N) * c->es, width = plane * c->es;
     full.resize(2 * n);
-    HIPCHK(c, hipStreamSynchronize(c->stream));
     if (c->cfg.precision == PSC_FP32) {
         std::vector<float> tmp(2 * n);
-        HIPCHK(c, hipMemcpy2D(tmp.data(), width, src, spitch, width, c->N, hipMemcpyDeviceToHost));
+        HIPCHK(c, hipMemcpy2DAsync(tmp.data(), width, src, spitch, width, c->N, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
         for (size_t i = 0; i < 2 * n; ++i) full[i] = tmp[i];
     } else {
-        HIPCHK(c, hipMemcpy2D(full.data(), width, src, spitch, width, c->N, hipMemcpyDeviceToHost));
+        HIPCHK(c, hipMemcpy2DAsync(full.data(), width, src, spitch, width, c->N, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
     }
     return PSC_OK;
 }
@@ -1138,13 +1146,14 @@ int for_each_ghost(psc_ctx* c, const void* Gsrc, F&& fn) {
     const int N = c->N;
     const size_t n = (size_t)2 * N * N;
     std::vector<double> g(2 * n);
-    HIPCHK(c, hipStreamSynchronize(c->stream));
     if (c->cfg.precision == PSC_FP32) {
         std::vector<float> tmp(2 * n);
-        HIPCHK(c, hipMemcpy(tmp.data(), Gsrc, tmp.size() * sizeof(float), hipMemcpyDeviceToHost));
+        HIPCHK(c, hipMemcpyAsync(tmp.data(), Gsrc, tmp.size() * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
         for (size_t i = 0; i < 2 * n; ++i) g[i] = tmp[i];
     } else {
-        HIPCHK(c, hipMemcpy(g.data(), Gsrc, g.size() * sizeof(double), hipMemcpyDeviceToHost));
+        HIPCHK(c, hipMemcpyAsync(g.data(), Gsrc, g.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
     }
     for (int pl = 0; pl < 2; ++pl)
         for (int x = 0; x < N; ++x)
@@ -1387,9 +1396,10 @@ int Engine<G, GY>::adjoint(psc_ctx* c, const psc_adjoint_spec* sp, double* out, 
     HIPCHK(c, hipEventElapsedTime(&fwd_ms, ev[0], ev[1]));
     HIPCHK(c, hipEventElapsedTime(&bwd_ms, ev[1], ev[2]));
     AdjState hst;
-    HIPCHK(c, hipMemcpy(&hst, w->st, sizeof(AdjState), hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpyAsync(&hst, w->st, sizeof(AdjState), hipMemcpyDeviceToHost, c->stream));
     cpx<T> f00;
-    HIPCHK(c, hipMemcpy(&f00, w->fbar, sizeof(f00), hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpyAsync(&f00, w->fbar, sizeof(f00), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     // phi0 enters through the DC mode phi0 N^3 and da/dt(0) = a0 H(phi0),
     // dH/dphi0 = f2m V'(phi0) / (6 H)  (initialize.cpp:649-668)
     const StepConsts sc = consts_of(c);
@@ -1585,9 +1595,13 @@ int psc_create(const psc_config* cfg, psc_ctx** out) {
         if (cfg->precision == PSC_FP32) {
             std::vector<float> twf(2 * n);
             for (int i = 0; i < 2 * n; ++i) twf[i] = (float)twd[i];
-            return hipMemcpy(dst, twf.data(), twf.size() * sizeof(float), hipMemcpyHostToDevice) == hipSuccess;
+            return hipMemcpyAsync(dst, twf.data(), twf.size() * sizeof(float), hipMemcpyHostToDevice, c->stream) ==
+                       hipSuccess &&
+                   hipStreamSynchronize(c->stream) == hipSuccess;
         }
-        return hipMemcpy(dst, twd.data(), twd.size() * sizeof(double), hipMemcpyHostToDevice) == hipSuccess;
+        return hipMemcpyAsync(dst, twd.data(), twd.size() * sizeof(double), hipMemcpyHostToDevice, c->stream) ==
+                   hipSuccess &&
+               hipStreamSynchronize(c->stream) == hipSuccess;
     };
     if (!twiddles(c->tw, N) || (c->padeng && !twiddles(c->twp, 2 * N))) {
         psc_destroy(c.release());
@@ -1599,12 +1613,14 @@ int psc_create(const psc_config* cfg, psc_ctx** out) {
         s.a = s.a_start = s.a_half = 1.0;
         s.m2 = 1.0;
     }
-    if (hipMemcpy(c->ms, init.data(), sizeof(MemberState) * c->B, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemset(c->Pf, 0, arr) != hipSuccess || hipMemset(c->Pp, 0, arr) != hipSuccess ||
-        hipMemset(c->W, 0, gw ? 4 * arr : arr) != hipSuccess ||
-        (c->padeng && (hipMemset(c->Gf, 0, (size_t)2 * N * N * c->es) != hipSuccess ||
-                       hipMemset(c->Gp, 0, (size_t)2 * N * N * c->es) != hipSuccess)) ||
-        (gw && (hipMemset(c->U, 0, 3 * arr) != hipSuccess || hipMemset(c->PU, 0, 3 * arr) != hipSuccess))) {
+    // the fills finish before the context is handed out: they are on the
+    // context's stream, ordered before anything the caller launches there
+    auto fill = [&](void* p, size_t bytes) { return hipMemsetAsync(p, 0, bytes, c->stream) == hipSuccess; };
+    if (hipMemcpyAsync(c->ms, init.data(), sizeof(MemberState) * c->B, hipMemcpyHostToDevice, c->stream) !=
+            hipSuccess ||
+        !fill(c->Pf, arr) || !fill(c->Pp, arr) || !fill(c->W, gw ? 4 * arr : arr) ||
+        (c->padeng && (!fill(c->Gf, (size_t)2 * N * N * c->es) || !fill(c->Gp, (size_t)2 * N * N * c->es))) ||
+        (gw && (!fill(c->U, 3 * arr) || !fill(c->PU, 3 * arr))) || hipStreamSynchronize(c->stream) != hipSuccess) {
         psc_destroy(c.release());
         return PSC_ERR_DEVICE;
     }
@@ -2264,21 +2280,24 @@ int psc_autotune(psc_ctx* c, int nsteps, char* report, int report_len) {
         return c->fail(PSC_ERR_NOMEM, "autotune snapshot allocation failed");
     }
     auto copy = [&](bool save) -> int {
-        HIPCHK(c, hipStreamSynchronize(c->stream));
-        HIPCHK(c, hipMemcpy(save ? sf : c->Pf, save ? c->Pf : sf, arr, hipMemcpyDeviceToDevice));
-        HIPCHK(c, hipMemcpy(save ? sp : c->Pp, save ? c->Pp : sp, arr, hipMemcpyDeviceToDevice));
+        auto d2d = [&](void* dst, const void* src, size_t bytes) {
+            return hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, c->stream);
+        };
+        HIPCHK(c, d2d(save ? sf : c->Pf, save ? c->Pf : sf, arr));
+        HIPCHK(c, d2d(save ? sp : c->Pp, save ? c->Pp : sp, arr));
         if (gw) {
-            HIPCHK(c, hipMemcpy(save ? su : c->U, save ? c->U : su, 3 * arr, hipMemcpyDeviceToDevice));
-            HIPCHK(c, hipMemcpy(save ? spu : c->PU, save ? c->PU : spu, 3 * arr, hipMemcpyDeviceToDevice));
+            HIPCHK(c, d2d(save ? su : c->U, save ? c->U : su, 3 * arr));
+            HIPCHK(c, d2d(save ? spu : c->PU, save ? c->PU : spu, 3 * arr));
         }
         if (pad) {
-            HIPCHK(c, hipMemcpy(save ? sgf : c->Gf, save ? c->Gf : sgf, garr, hipMemcpyDeviceToDevice));
-            HIPCHK(c, hipMemcpy(save ? sgp : c->Gp, save ? c->Gp : sgp, garr, hipMemcpyDeviceToDevice));
+            HIPCHK(c, d2d(save ? sgf : c->Gf, save ? c->Gf : sgf, garr));
+            HIPCHK(c, d2d(save ? sgp : c->Gp, save ? c->Gp : sgp, garr));
         }
         if (save)
-            HIPCHK(c, hipMemcpy(ms.data(), c->ms, sizeof(MemberState) * c->B, hipMemcpyDeviceToHost));
+            HIPCHK(c, hipMemcpyAsync(ms.data(), c->ms, sizeof(MemberState) * c->B, hipMemcpyDeviceToHost, c->stream));
         else
-            HIPCHK(c, hipMemcpy(c->ms, ms.data(), sizeof(MemberState) * c->B, hipMemcpyHostToDevice));
+            HIPCHK(c, hipMemcpyAsync(c->ms, ms.data(), sizeof(MemberState) * c->B, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
         return PSC_OK;
     };
     int rc = copy(true);

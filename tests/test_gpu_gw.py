@@ -112,10 +112,13 @@ def test_gw_512_shapes_agree():
         lat.generate_gw_ic_device(0, ic)
         s0, _ = lat.gw_spectrum(PSC_GW_PROJ_TT, 0)
         f0 = lat.spectra(0, 0)
+        print(f"{name}: initial scalars {lat.scalars(0)}, field spectrum sum {f0.sum():.12e}, "
+              f"GW spectrum sum {s0.sum():.12e}", flush=True)
         lat.step(4)
         s1, _ = lat.gw_spectrum(PSC_GW_PROJ_TT, 0)
         f1 = lat.spectra(0, 0)
         ds, df = s1 - s0, f1 - f0
+        print(f"{name}: scalars after the steps {lat.scalars(0)}", flush=True)
         if ref is None:
             ref = (ds, df)
             print(f"{name}: GW spectrum change {np.abs(ds).max() / np.abs(s0).max():.2e} of its size, "

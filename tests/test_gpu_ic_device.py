@@ -109,3 +109,29 @@ def test_rejections():
     with pytest.raises(Exception):
         lat.generate_ic_device(1, ic)          # no such member
     lat.close()
+
+
+def test_first_generation_after_another_context_at_512():
+    """A fresh context's zero fills and copies are ordered with its kernels:
+    the first device ICs generated right after creation (on memory another
+    context used and freed) equal a second generation.  With the fills on the
+    null stream, the fill of pi overlapped the IC kernel at 512^3 and zeroed
+    part of the modes it had written."""
+    cfg, model, ic = parity_setup(N=512, with_gw=1, integrator=PSC_Y4, precision=PSC_FP32)
+    print("case device IC 512^3 GW after another context", flush=True)
+    first = Lattice(cfg)
+    first.set_model(0, model)
+    first.generate_ic_device(0, make_ic(f0=(1.3, 0.1), fd0=(0.2, 0.0), seed=7))
+    first.step(1)
+    first.close()
+    lat = Lattice(cfg)
+    lat.set_model(0, model)
+    sums = []
+    for _ in range(2):
+        lat.generate_ic_device(0, ic)
+        lat.generate_gw_ic_device(0, ic)
+        s, _ = lat.gw_spectrum(0, 0)
+        sums.append((lat.spectra(0, 0).sum(), lat.spectra(1, 0).sum(), s.sum(), lat.scalars(0)[3]))
+    print("first", sums[0], "second", sums[1], flush=True)
+    assert sums[0] == sums[1]
+    lat.close()
