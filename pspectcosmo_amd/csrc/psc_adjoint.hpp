@@ -232,11 +232,14 @@ k_adj_p1g(const cpx<typename G::T>* __restrict__ pubar, cpx<typename G::T>* __re
 
 // P3 (GW): x-axis inverse of sbar (three packed pairs), the gradient adjoint
 // gbar_d = sum_e w_de sbar_de G_e with the recorded real-space gradients G_e,
-// x-axis forward of gbar; reduction sum sbar . s for abar.
+// x-axis forward of gbar; reduction sum sbar . s for abar.  The x gradient's
+// adjoint leaves as -i kIR n_x gbar_x: that factor is constant along y and z
+// lines, so it is applied here and the y forward merges the x and y shares
+// into one array (k_adj_y_fwd_gw).
 template <class G>
 __global__ void __launch_bounds__((G::THREADS_COL))
 k_adj_p3g(cpx<typename G::T>* __restrict__ W, const cpx<typename G::T>* __restrict__ ge, size_t astride,
-          const cpx<typename G::T>* __restrict__ tw, double* __restrict__ part) {
+          double kIR, const cpx<typename G::T>* __restrict__ tw, double* __restrict__ part) {
     using T = typename G::T;
     constexpr int N = G::N;
     __shared__ cpx<T> lds[G::LayCol::size(0)];
@@ -271,6 +274,13 @@ k_adj_p3g(cpx<typename G::T>* __restrict__ W, const cpx<typename G::T>* __restri
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
         line_fft<T, N, typename G::Rad, -1, typename G::LayCol>(e[q], t, c, lds, tw);
+        if (q == 0) {
+#pragma unroll
+            for (int m = 0; m < G::E; ++m) {   // (-i kx) (x + i y) = kx y - i kx x
+                const T kx = (T)(kIR * deriv_index(t + G::TPL * m, N));
+                e[0][m] = cpx<T>{kx * e[0][m].y, -kx * e[0][m].x};
+            }
+        }
 #pragma unroll
         for (int m = 0; m < G::E; ++m) W[(size_t)q * astride + base + (size_t)G::TPL * m * stride] = e[q][m];
     }
@@ -278,9 +288,52 @@ k_adj_p3g(cpx<typename G::T>* __restrict__ W, const cpx<typename G::T>* __restri
     if (threadIdx.x == 0) part[(size_t)blockIdx.y * gridDim.x + blockIdx.x] = red[0];
 }
 
+// y forward of the GW backward pass: W0 (the field adjoint) in place;
+// W1 <- FFT_y(W1) - i kIR n_y FFT_y(W2) (the x share, -i kIR n_x already
+// applied, and the y share of the gradient adjoint in one array); W3 (the z
+// share) in place.  One launch for the four arrays.
+template <class G>
+__global__ void __launch_bounds__((G::THREADS_COL))
+k_adj_y_fwd_gw(cpx<typename G::T>* __restrict__ W, size_t astride, double kIR,
+               const cpx<typename G::T>* __restrict__ tw) {
+    using T = typename G::T;
+    constexpr int N = G::N;
+    __shared__ cpx<T> lds[G::LayCol::size(0)];
+    const int c = threadIdx.x % G::TC, t = threadIdx.x / G::TC;
+    const size_t z = (size_t)blockIdx.x * G::TC + c;
+    const size_t base = (size_t)blockIdx.y * G::PS + z + (size_t)t * N;
+    cpx<T> e[G::E], g[G::E];
+#pragma unroll
+    for (int m = 0; m < G::E; ++m) e[m] = W[base + (size_t)G::TPL * m * N];
+    line_fft<T, N, typename G::Rad, -1, typename G::LayCol>(e, t, c, lds, tw);
+#pragma unroll
+    for (int m = 0; m < G::E; ++m) W[base + (size_t)G::TPL * m * N] = e[m];
+    W += astride;
+#pragma unroll
+    for (int m = 0; m < G::E; ++m) e[m] = W[base + (size_t)G::TPL * m * N];
+    line_fft<T, N, typename G::Rad, -1, typename G::LayCol>(e, t, c, lds, tw);
+#pragma unroll
+    for (int m = 0; m < G::E; ++m) g[m] = W[astride + base + (size_t)G::TPL * m * N];
+    line_fft<T, N, typename G::Rad, -1, typename G::LayCol>(g, t, c, lds, tw);
+#pragma unroll
+    for (int m = 0; m < G::E; ++m) {
+        const T ky = (T)(kIR * deriv_index(t + G::TPL * m, N));
+        e[m].x += ky * g[m].y;
+        e[m].y -= ky * g[m].x;
+        W[base + (size_t)G::TPL * m * N] = e[m];
+    }
+#pragma unroll
+    for (int m = 0; m < G::E; ++m) e[m] = W[2 * astride + base + (size_t)G::TPL * m * N];
+    line_fft<T, N, typename G::Rad, -1, typename G::LayCol>(e, t, c, lds, tw);
+#pragma unroll
+    for (int m = 0; m < G::E; ++m) W[2 * astride + base + (size_t)G::TPL * m * N] = e[m];
+}
+
 // P5 (GW): z-axis forward of gbar_d, fbar += sum_d (-i kIR n_d) gbar_d / N^3
 // (Nyquist component zeroed, evolution.cpp:494-495), then the drift adjoint of
 // u: pibar_u += 2 h a_h^{al-3} ubar; reduction sum Re(conj(ubar) pi_u half).
+// W0 holds the x and y shares with their factors applied (k_adj_p3g,
+// k_adj_y_fwd_gw), W2 the z share.
 template <class G>
 __global__ void __launch_bounds__((G::THREADS_ROW))
 k_adj_p5g(const cpx<typename G::T>* __restrict__ W, cpx<typename G::T>* __restrict__ fbar,
@@ -293,22 +346,21 @@ k_adj_p5g(const cpx<typename G::T>* __restrict__ W, cpx<typename G::T>* __restri
     const int t = threadIdx.x % G::TPL, c = threadIdx.x / G::TPL;
     const size_t line = (size_t)blockIdx.x * G::L + c;
     const size_t base = (line / N) * G::PS + (line % N) * N + t;
-    const T kx = (T)(kIR * deriv_index((int)(line / N), N)), ky = (T)(kIR * deriv_index((int)(line % N), N));
     const T inv = (T)(1.0 / (double)G::N3);
     cpx<T> acc[G::E];
 #pragma unroll
-    for (int m = 0; m < G::E; ++m) acc[m] = cpx<T>{(T)0, (T)0};
-#pragma unroll 1
-    for (int d = 0; d < 3; ++d) {
+    for (int m = 0; m < G::E; ++m) acc[m] = W[base + G::TPL * m];
+    line_fft<T, N, typename G::Rad, -1, typename G::LayRow>(acc, t, c, lds, tw);
+    {
         cpx<T> e[G::E];
 #pragma unroll
-        for (int m = 0; m < G::E; ++m) e[m] = W[(size_t)d * astride + base + G::TPL * m];
+        for (int m = 0; m < G::E; ++m) e[m] = W[2 * astride + base + G::TPL * m];
         line_fft<T, N, typename G::Rad, -1, typename G::LayRow>(e, t, c, lds, tw);
 #pragma unroll
         for (int m = 0; m < G::E; ++m) {
-            const T kd = d == 0 ? kx : (d == 1 ? ky : (T)(kIR * deriv_index(t + G::TPL * m, N)));
-            acc[m].x += kd * e[m].y;   // (-i kd) (x + i y) = kd y - i kd x
-            acc[m].y -= kd * e[m].x;
+            const T kz = (T)(kIR * deriv_index(t + G::TPL * m, N));
+            acc[m].x += kz * e[m].y;   // (-i kz) (x + i y) = kz y - i kz x
+            acc[m].y -= kz * e[m].x;
         }
     }
 #pragma unroll

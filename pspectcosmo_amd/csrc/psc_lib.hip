@@ -763,8 +763,12 @@ struct Engine final : EngineBase {
                            w.p3);
         if (gw)
             hipLaunchKernelGGL((k_adj_p3g<G>), cgrid, cblock, 0, c->stream, P(c->W) + as, P(w.g[jf]), as,
-                               P(c->tw), w.p3g);
-        if ((rc = launch_y<-1>(c, P(c->W), gw ? 4 : 1))) return rc;
+                               c->cfg.kIR, P(c->tw), w.p3g);
+        if (gw)
+            hipLaunchKernelGGL((k_adj_y_fwd_gw<GY>), dim3(N / GY::TC, N, 1), dim3(GY::THREADS_COL), 0, c->stream,
+                               P(c->W), as, c->cfg.kIR, P(c->tw));
+        else if ((rc = launch_y<-1>(c, P(c->W), 1)))
+            return rc;
         if (gw)
             hipLaunchKernelGGL((k_adj_p5g<G>), rgrid, rblock, 0, c->stream, P(c->W) + as, P(w.fbar), P(w.ubar),
                                P(w.pubar), P(w.puh[jph]), as, w.co, c->cfg.kIR, P(c->tw), w.p5g);
