@@ -146,13 +146,22 @@ k_adj_p3(cpx<typename G::T>* __restrict__ W, const cpx<typename G::T>* __restric
     }
 }
 
-// P5
-template <class G>
+// P5: z-axis forward of W0, fbar += W0 / N^3, then the drift adjoint
+// pbar += cd fbar; reduction sum Re(conj(fbar) pi_half).  GW: W0 also holds
+// the x and y shares of the gradient adjoint (k_adj_p3g, k_adj_y_fwd_gw), W3
+// its z share, which enters as -i kIR n_z FFT_z(W3) (Nyquist component zeroed,
+// evolution.cpp:494-495), so that fbar is complete before the drift adjoint;
+// then the drift adjoint of u: pibar_u += cd ubar, reduction
+// sum Re(conj(ubar) pi_u half) into part_g.
+template <class G, bool GW = false>
 __global__ void __launch_bounds__((G::THREADS_ROW))
 k_adj_p5(const cpx<typename G::T>* __restrict__ W, cpx<typename G::T>* __restrict__ fbar,
          cpx<typename G::T>* __restrict__ pbar, const cpx<typename G::T>* __restrict__ ph,
          const AdjCoef* __restrict__ co,
-         const cpx<typename G::T>* __restrict__ tw, double* __restrict__ part) {
+         const cpx<typename G::T>* __restrict__ tw, double* __restrict__ part,
+         size_t astride = 0, double kIR = 0.0, const cpx<typename G::T>* __restrict__ ubar = nullptr,
+         cpx<typename G::T>* __restrict__ pubar = nullptr, const cpx<typename G::T>* __restrict__ puh = nullptr,
+         double* __restrict__ part_g = nullptr) {
     using T = typename G::T;
     constexpr int N = G::N;
     __shared__ cpx<T> lds[G::LayRow::size(G::L)];
@@ -163,10 +172,22 @@ k_adj_p5(const cpx<typename G::T>* __restrict__ W, cpx<typename G::T>* __restric
 #pragma unroll
     for (int m = 0; m < G::E; ++m) e[m] = W[base + G::TPL * m];
     line_fft<T, N, typename G::Rad, -1, typename G::LayRow>(e, t, c, lds, tw);
+    if constexpr (GW) {
+        cpx<T> g[G::E];
+#pragma unroll
+        for (int m = 0; m < G::E; ++m) g[m] = W[3 * astride + base + G::TPL * m];
+        line_fft<T, N, typename G::Rad, -1, typename G::LayRow>(g, t, c, lds, tw);
+#pragma unroll
+        for (int m = 0; m < G::E; ++m) {
+            const T kz = (T)(kIR * deriv_index(t + G::TPL * m, N));
+            e[m].x += kz * g[m].y;   // (-i kz) (x + i y) = kz y - i kz x
+            e[m].y -= kz * g[m].x;
+        }
+    }
     const T inv = (T)(1.0 / (double)G::N3);
     const T cd = (T)co->cd;
     const bool fin = co->final_pass != 0;
-    double red[1] = {0.0};
+    double red[2] = {0.0, 0.0};
 #pragma unroll
     for (int m = 0; m < G::E; ++m) {
         const size_t idx = base + G::TPL * m;
@@ -183,8 +204,27 @@ k_adj_p5(const cpx<typename G::T>* __restrict__ W, cpx<typename G::T>* __restric
             pbar[idx] = pb;
         }
     }
-    block_sum<1, G::THREADS_ROW>(red, reinterpret_cast<double*>(lds));
-    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+    if constexpr (GW) {
+        if (!fin) {
+#pragma unroll 1
+            for (int q = 0; q < 3; ++q)
+#pragma unroll
+                for (int m = 0; m < G::E; ++m) {
+                    const size_t idx = (size_t)q * astride + base + G::TPL * m;
+                    const cpx<T> ub = ubar[idx], pu = puh[idx];
+                    cpx<T> pb = pubar[idx];
+                    pb.x += cd * ub.x;
+                    pb.y += cd * ub.y;
+                    pubar[idx] = pb;
+                    red[1] += (double)ub.x * pu.x + (double)ub.y * pu.y;
+                }
+        }
+    }
+    block_sum<2, G::THREADS_ROW>(red, reinterpret_cast<double*>(lds));
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = red[0];
+        if constexpr (GW) part_g[blockIdx.x] = red[1];
+    }
 }
 
 // P1 (GW): Pib = (h + h_next) pibar_u; ubar += -a^{1+al} kIR^2 n^2 Pib; the
@@ -230,16 +270,19 @@ k_adj_p1g(const cpx<typename G::T>* __restrict__ pubar, cpx<typename G::T>* __re
     if (threadIdx.x == 0) part[blockIdx.x] = red[0];
 }
 
-// P3 (GW): x-axis inverse of sbar (three packed pairs), the gradient adjoint
-// gbar_d = sum_e w_de sbar_de G_e with the recorded real-space gradients G_e,
-// x-axis forward of gbar; reduction sum sbar . s for abar.  The x gradient's
-// adjoint leaves as -i kIR n_x gbar_x: that factor is constant along y and z
-// lines, so it is applied here and the y forward merges the x and y shares
-// into one array (k_adj_y_fwd_gw).
+// P3 (GW): x-axis inverse of sbar (three packed pairs in W1..W3), the
+// gradient adjoint gbar_d = sum_e w_de sbar_de G_e with the recorded
+// real-space gradients G_e, x-axis forward of gbar; reduction sum sbar . s
+// for abar.  Each gbar_d reaches fbar as -i kIR n_d FFT_z FFT_y (gbar_d) / N^3,
+// the field adjoint W0 as FFT_z FFT_y (W0) / N^3 (k_adj_p5).  n_x is constant
+// along y and z lines, so -i kIR n_x gbar_x is added into W0 here; the y
+// forward adds the y share into W0 as well (k_adj_y_fwd_gw), and the z share
+// goes on in W3.  Arguments: W = W1, W0 = the field adjoint array.
 template <class G>
 __global__ void __launch_bounds__((G::THREADS_COL))
-k_adj_p3g(cpx<typename G::T>* __restrict__ W, const cpx<typename G::T>* __restrict__ ge, size_t astride,
-          double kIR, const cpx<typename G::T>* __restrict__ tw, double* __restrict__ part) {
+k_adj_p3g(cpx<typename G::T>* __restrict__ W, cpx<typename G::T>* __restrict__ W0,
+          const cpx<typename G::T>* __restrict__ ge, size_t astride, double kIR,
+          const cpx<typename G::T>* __restrict__ tw, double* __restrict__ part) {
     using T = typename G::T;
     constexpr int N = G::N;
     __shared__ cpx<T> lds[G::LayCol::size(0)];
@@ -276,22 +319,26 @@ k_adj_p3g(cpx<typename G::T>* __restrict__ W, const cpx<typename G::T>* __restri
         line_fft<T, N, typename G::Rad, -1, typename G::LayCol>(e[q], t, c, lds, tw);
         if (q == 0) {
 #pragma unroll
-            for (int m = 0; m < G::E; ++m) {   // (-i kx) (x + i y) = kx y - i kx x
+            for (int m = 0; m < G::E; ++m) {   // W0 += (-i kx) (x + i y) = kx y - i kx x
+                const size_t idx = base + (size_t)G::TPL * m * stride;
                 const T kx = (T)(kIR * deriv_index(t + G::TPL * m, N));
-                e[0][m] = cpx<T>{kx * e[0][m].y, -kx * e[0][m].x};
+                cpx<T> w0 = W0[idx];
+                w0.x += kx * e[0][m].y;
+                w0.y -= kx * e[0][m].x;
+                W0[idx] = w0;
             }
-        }
+        } else {
 #pragma unroll
-        for (int m = 0; m < G::E; ++m) W[(size_t)q * astride + base + (size_t)G::TPL * m * stride] = e[q][m];
+            for (int m = 0; m < G::E; ++m) W[(size_t)q * astride + base + (size_t)G::TPL * m * stride] = e[q][m];
+        }
     }
     block_sum<1, G::THREADS_COL>(red, reinterpret_cast<double*>(lds));
     if (threadIdx.x == 0) part[(size_t)blockIdx.y * gridDim.x + blockIdx.x] = red[0];
 }
 
-// y forward of the GW backward pass: W0 (the field adjoint) in place;
-// W1 <- FFT_y(W1) - i kIR n_y FFT_y(W2) (the x share, -i kIR n_x already
-// applied, and the y share of the gradient adjoint in one array); W3 (the z
-// share) in place.  One launch for the four arrays.
+// y forward of the GW backward pass: W0 <- FFT_y(W0) - i kIR n_y FFT_y(W2)
+// (W0 holds the field adjoint and the x share of the gradient adjoint, W2 its
+// y share), W3 (the z share) in place.  W1 is not read.
 template <class G>
 __global__ void __launch_bounds__((G::THREADS_COL))
 k_adj_y_fwd_gw(cpx<typename G::T>* __restrict__ W, size_t astride, double kIR,
@@ -307,13 +354,7 @@ k_adj_y_fwd_gw(cpx<typename G::T>* __restrict__ W, size_t astride, double kIR,
     for (int m = 0; m < G::E; ++m) e[m] = W[base + (size_t)G::TPL * m * N];
     line_fft<T, N, typename G::Rad, -1, typename G::LayCol>(e, t, c, lds, tw);
 #pragma unroll
-    for (int m = 0; m < G::E; ++m) W[base + (size_t)G::TPL * m * N] = e[m];
-    W += astride;
-#pragma unroll
-    for (int m = 0; m < G::E; ++m) e[m] = W[base + (size_t)G::TPL * m * N];
-    line_fft<T, N, typename G::Rad, -1, typename G::LayCol>(e, t, c, lds, tw);
-#pragma unroll
-    for (int m = 0; m < G::E; ++m) g[m] = W[astride + base + (size_t)G::TPL * m * N];
+    for (int m = 0; m < G::E; ++m) g[m] = W[2 * astride + base + (size_t)G::TPL * m * N];
     line_fft<T, N, typename G::Rad, -1, typename G::LayCol>(g, t, c, lds, tw);
 #pragma unroll
     for (int m = 0; m < G::E; ++m) {
@@ -323,73 +364,12 @@ k_adj_y_fwd_gw(cpx<typename G::T>* __restrict__ W, size_t astride, double kIR,
         W[base + (size_t)G::TPL * m * N] = e[m];
     }
 #pragma unroll
-    for (int m = 0; m < G::E; ++m) e[m] = W[2 * astride + base + (size_t)G::TPL * m * N];
+    for (int m = 0; m < G::E; ++m) e[m] = W[3 * astride + base + (size_t)G::TPL * m * N];
     line_fft<T, N, typename G::Rad, -1, typename G::LayCol>(e, t, c, lds, tw);
 #pragma unroll
-    for (int m = 0; m < G::E; ++m) W[2 * astride + base + (size_t)G::TPL * m * N] = e[m];
+    for (int m = 0; m < G::E; ++m) W[3 * astride + base + (size_t)G::TPL * m * N] = e[m];
 }
 
-// P5 (GW): z-axis forward of gbar_d, fbar += sum_d (-i kIR n_d) gbar_d / N^3
-// (Nyquist component zeroed, evolution.cpp:494-495), then the drift adjoint of
-// u: pibar_u += 2 h a_h^{al-3} ubar; reduction sum Re(conj(ubar) pi_u half).
-// W0 holds the x and y shares with their factors applied (k_adj_p3g,
-// k_adj_y_fwd_gw), W2 the z share.
-template <class G>
-__global__ void __launch_bounds__((G::THREADS_ROW))
-k_adj_p5g(const cpx<typename G::T>* __restrict__ W, cpx<typename G::T>* __restrict__ fbar,
-          const cpx<typename G::T>* __restrict__ ubar, cpx<typename G::T>* __restrict__ pubar,
-          const cpx<typename G::T>* __restrict__ puh, size_t astride, const AdjCoef* __restrict__ co, double kIR,
-          const cpx<typename G::T>* __restrict__ tw, double* __restrict__ part) {
-    using T = typename G::T;
-    constexpr int N = G::N;
-    __shared__ cpx<T> lds[G::LayRow::size(G::L)];
-    const int t = threadIdx.x % G::TPL, c = threadIdx.x / G::TPL;
-    const size_t line = (size_t)blockIdx.x * G::L + c;
-    const size_t base = (line / N) * G::PS + (line % N) * N + t;
-    const T inv = (T)(1.0 / (double)G::N3);
-    cpx<T> acc[G::E];
-#pragma unroll
-    for (int m = 0; m < G::E; ++m) acc[m] = W[base + G::TPL * m];
-    line_fft<T, N, typename G::Rad, -1, typename G::LayRow>(acc, t, c, lds, tw);
-    {
-        cpx<T> e[G::E];
-#pragma unroll
-        for (int m = 0; m < G::E; ++m) e[m] = W[2 * astride + base + G::TPL * m];
-        line_fft<T, N, typename G::Rad, -1, typename G::LayRow>(e, t, c, lds, tw);
-#pragma unroll
-        for (int m = 0; m < G::E; ++m) {
-            const T kz = (T)(kIR * deriv_index(t + G::TPL * m, N));
-            acc[m].x += kz * e[m].y;   // (-i kz) (x + i y) = kz y - i kz x
-            acc[m].y -= kz * e[m].x;
-        }
-    }
-#pragma unroll
-    for (int m = 0; m < G::E; ++m) {
-        const size_t idx = base + G::TPL * m;
-        cpx<T> fb = fbar[idx];
-        fb.x += inv * acc[m].x;
-        fb.y += inv * acc[m].y;
-        fbar[idx] = fb;
-    }
-    double red[1] = {0.0};
-    if (!co->final_pass) {
-        const T cd = (T)co->cd;
-#pragma unroll 1
-        for (int q = 0; q < 3; ++q)
-#pragma unroll
-            for (int m = 0; m < G::E; ++m) {
-                const size_t idx = (size_t)q * astride + base + G::TPL * m;
-                const cpx<T> ub = ubar[idx], ph = puh[idx];
-                cpx<T> pb = pubar[idx];
-                pb.x += cd * ub.x;
-                pb.y += cd * ub.y;
-                pubar[idx] = pb;
-                red[0] += (double)ub.x * ph.x + (double)ub.y * ph.y;
-            }
-    }
-    block_sum<1, G::THREADS_ROW>(red, reinterpret_cast<double*>(lds));
-    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
-}
 
 // ---------------------------------------------------------------------------
 // GW-energy-fraction objective at the end of the run: Q = sum over the modes

@@ -747,8 +747,9 @@ struct Engine final : EngineBase {
     }
 
     // One backward substep (or the t = 0 pass): P1, y inverse, P3, y forward,
-    // P5; with the GW objective P1g, P3g, P5g run beside them (P5g before P5,
-    // so that the drift adjoint in P5 sees the complete fbar).
+    // P5; with the GW objective P1g and P3g run beside P1 and P3, the y
+    // forward also merges the gradient adjoint's x and y shares into W0, and
+    // P5 adds its z share to fbar before the drift adjoint.
     int backward_passes(psc_ctx* c, AdjWork& w, int jf, int jph, bool gw) {
         dim3 rgrid(G::BLOCKS_ROW, 1), rblock(G::THREADS_ROW), cgrid(N / G::TC, N, 1), cblock(G::THREADS_COL);
         const size_t as = G::MS;
@@ -762,7 +763,7 @@ struct Engine final : EngineBase {
         hipLaunchKernelGGL((k_adj_p3<G>), cgrid, cblock, 0, c->stream, P(c->W), P(w.r[jf]), c->ms, w.co, P(c->tw),
                            w.p3);
         if (gw)
-            hipLaunchKernelGGL((k_adj_p3g<G>), cgrid, cblock, 0, c->stream, P(c->W) + as, P(w.g[jf]), as,
+            hipLaunchKernelGGL((k_adj_p3g<G>), cgrid, cblock, 0, c->stream, P(c->W) + as, P(c->W), P(w.g[jf]), as,
                                c->cfg.kIR, P(c->tw), w.p3g);
         if (gw)
             hipLaunchKernelGGL((k_adj_y_fwd_gw<GY>), dim3(N / GY::TC, N, 1), dim3(GY::THREADS_COL), 0, c->stream,
@@ -770,10 +771,12 @@ struct Engine final : EngineBase {
         else if ((rc = launch_y<-1>(c, P(c->W), 1)))
             return rc;
         if (gw)
-            hipLaunchKernelGGL((k_adj_p5g<G>), rgrid, rblock, 0, c->stream, P(c->W) + as, P(w.fbar), P(w.ubar),
-                               P(w.pubar), P(w.puh[jph]), as, w.co, c->cfg.kIR, P(c->tw), w.p5g);
-        hipLaunchKernelGGL((k_adj_p5<G>), rgrid, rblock, 0, c->stream, P(c->W), P(w.fbar), P(w.pbar),
-                           P(w.ph[jph]), w.co, P(c->tw), w.p5);
+            hipLaunchKernelGGL((k_adj_p5<G, true>), rgrid, rblock, 0, c->stream, P(c->W), P(w.fbar), P(w.pbar),
+                               P(w.ph[jph]), w.co, P(c->tw), w.p5, as, c->cfg.kIR, P(w.ubar), P(w.pubar),
+                               P(w.puh[jph]), w.p5g);
+        else
+            hipLaunchKernelGGL((k_adj_p5<G>), rgrid, rblock, 0, c->stream, P(c->W), P(w.fbar), P(w.pbar),
+                               P(w.ph[jph]), w.co, P(c->tw), w.p5);
         HIPCHK(c, hipGetLastError());
         return PSC_OK;
     }
