@@ -168,6 +168,7 @@ struct psc_ctx {
     double dev_bytes = 0.0;
     int shape = 0;   // index into kShapes
     int pad_shape = -1;   // dealiased stepper: index into kShapes of its 2N shape
+    int member_group = 0;   // members per group of the force chain (0: all)
     std::string report;
 
     int fail(int code, const std::string& msg) {
@@ -298,26 +299,32 @@ struct Engine final : EngineBase {
             hipLaunchKernelGGL((k_strided<GY, 1, SIGN, false, (SIGN < 0)>), grid, block, 0, c->stream, W, P(c->tw));
         });
     }
-    int launch_x_force(psc_ctx* c) {
-        const double X = (double)c->B * G::N3 * sizeof(cpx<T>);
-        dim3 grid(N / G::TC, N, c->B), block(G::THREADS_COL);
+    // Members m0 .. m0 + n - 1 (all of them by default).
+    int launch_x_force(psc_ctx* c, int m0 = 0, int n = -1) {
+        if (n < 0) n = c->B;
+        const size_t off = (size_t)m0 * G::MS;
+        const double X = (double)n * G::N3 * sizeof(cpx<T>);
+        dim3 grid(N / G::TC, N, n), block(G::THREADS_COL);
         return timed(c, K_X_FORCE, 2 * X, [&] {
-            hipLaunchKernelGGL((k_x_force<G>), grid, block, 0, c->stream, P(c->W), c->ms, P(c->tw),
-                               c->partV, P(c->Pf));
+            hipLaunchKernelGGL((k_x_force<G>), grid, block, 0, c->stream, P(c->W) + off, c->ms + m0, P(c->tw),
+                               c->partV + (size_t)m0 * G::BLOCKS_COL, P(c->Pf) + off);
         });
     }
-    int launch_z_kick(psc_ctx* c, bool reduce, double c1, double c2, int use_a_start) {
-        const double X = (double)c->B * G::N3 * sizeof(cpx<T>);
-        dim3 grid(G::BLOCKS_ROW, c->B), block(G::THREADS_ROW);
+    int launch_z_kick(psc_ctx* c, bool reduce, double c1, double c2, int use_a_start, int m0 = 0, int n = -1) {
+        if (n < 0) n = c->B;
+        const size_t off = (size_t)m0 * G::MS;
+        double* pkg = c->partKG + (size_t)m0 * G::BLOCKS_ROW * 2;
+        const double X = (double)n * G::N3 * sizeof(cpx<T>);
+        dim3 grid(G::BLOCKS_ROW, n), block(G::THREADS_ROW);
         const StepConsts sc = consts_of(c);
         if (reduce)
             return timed(c, K_Z_FWD_KICK, 4 * X, [&] {
-                hipLaunchKernelGGL((k_z_kick<G, true>), grid, block, 0, c->stream, P(c->W), P(c->Pf),
-                                   P(c->Pp), c->ms, P(c->tw), c->partKG, c1, c2, use_a_start, sc);
+                hipLaunchKernelGGL((k_z_kick<G, true>), grid, block, 0, c->stream, P(c->W) + off, P(c->Pf) + off,
+                                   P(c->Pp) + off, c->ms + m0, P(c->tw), pkg, c1, c2, use_a_start, sc);
             });
         return timed(c, K_Z_FWD_KICK_NR, 4 * X, [&] {
-            hipLaunchKernelGGL((k_z_kick<G, false>), grid, block, 0, c->stream, P(c->W), P(c->Pf),
-                               P(c->Pp), c->ms, P(c->tw), c->partKG, c1, c2, use_a_start, sc);
+            hipLaunchKernelGGL((k_z_kick<G, false>), grid, block, 0, c->stream, P(c->W) + off, P(c->Pf) + off,
+                               P(c->Pp) + off, c->ms + m0, P(c->tw), pkg, c1, c2, use_a_start, sc);
         });
     }
     // Reduce the substep partials to the scalar update.  Passes with more than
@@ -399,12 +406,23 @@ struct Engine final : EngineBase {
     int force_chain(psc_ctx* c, bool drift, bool reduce, double c1, double c2, int use_a_start) {
         if (c->padeng) return c->padeng->force_chain(c, drift, reduce, c1, c2, use_a_start);
         if (c->cfg.with_gw) return force_chain_gw(c, drift, reduce, c1, c2, use_a_start);
-        int rc;
-        if ((rc = launch_z_inverse(c, drift, P(c->Pf), P(c->Pf), P(c->Pp), P(c->W), c->ms, c->B))) return rc;
-        if ((rc = launch_y<+1>(c, P(c->W), c->B))) return rc;
-        if ((rc = launch_x_force(c))) return rc;
-        if ((rc = launch_y<-1>(c, P(c->W), c->B))) return rc;
-        return launch_z_kick(c, reduce, c1, c2, use_a_start);
+        // An ensemble runs the chain on groups of members whose arrays fit the
+        // memory-side cache together, so that each pass of a group reads what
+        // the group's previous pass has just written.
+        const int gs = c->member_group > 0 ? std::min(c->member_group, c->B) : c->B;
+        for (int m0 = 0; m0 < c->B; m0 += gs) {
+            const int n = std::min(gs, c->B - m0);
+            const size_t off = (size_t)m0 * G::MS;
+            int rc;
+            if ((rc = launch_z_inverse(c, drift, P(c->Pf) + off, P(c->Pf) + off, P(c->Pp) + off, P(c->W) + off,
+                                       c->ms + m0, n)))
+                return rc;
+            if ((rc = launch_y<+1>(c, P(c->W) + off, n))) return rc;
+            if ((rc = launch_x_force(c, m0, n))) return rc;
+            if ((rc = launch_y<-1>(c, P(c->W) + off, n))) return rc;
+            if ((rc = launch_z_kick(c, reduce, c1, c2, use_a_start, m0, n))) return rc;
+        }
+        return PSC_OK;
     }
 
     // All substeps of one step; `final` drops the trailing first-kick of the
@@ -1532,6 +1550,13 @@ int psc_create(const psc_config* cfg, psc_ctx** out) {
     c->N3 = (size_t)N * N * N;
     c->MS = member_stride(N);
     c->es = cfg->precision == PSC_FP32 ? 2 * sizeof(float) : 2 * sizeof(double);
+    {   // ensemble groups of about 128 MiB of W each (measured on 64 x 128^3:
+        // groups of 8 members FP32, 4 FP64, 8-10 % faster than one launch over
+        // all members); members larger than 64 MiB run all at once
+        const double member_bytes = (double)c->MS * c->es;
+        const int g = (int)std::lround(134217728.0 / member_bytes);
+        c->member_group = g >= 2 ? g : 0;
+    }
     {
         const std::vector<int> cand = shapes_for(cfg->precision, N);
         if (cand.empty()) return PSC_ERR_UNSUPPORTED;
