@@ -66,6 +66,24 @@ struct Shape {
 
 __device__ __forceinline__ int signed_index(int i, int N) { return i <= N / 2 ? i : i - N; }
 
+// Non-temporal (streaming) access for data the next passes do not read:
+// such lines then leave room in the caches for the arrays the next pass
+// does read.
+template <typename T>
+__device__ __forceinline__ cpx<T> load_stream(const cpx<T>* p) {
+    using V = T __attribute__((ext_vector_type(2)));
+    const V v = __builtin_nontemporal_load(reinterpret_cast<const V*>(p));
+    return cpx<T>{v.x, v.y};
+}
+template <typename T>
+__device__ __forceinline__ void store_stream(cpx<T>* p, cpx<T> a) {
+    using V = T __attribute__((ext_vector_type(2)));
+    V v;
+    v.x = a.x;
+    v.y = a.y;
+    __builtin_nontemporal_store(v, reinterpret_cast<V*>(p));
+}
+
 // a^n for a small integer exponent (powers of the scale factor) by repeated
 // squaring: a few multiplications instead of a general pow on the single
 // thread that runs each scalar update.

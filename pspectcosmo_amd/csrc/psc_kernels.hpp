@@ -53,15 +53,17 @@ k_z_inverse(const cpx<typename G::T>* __restrict__ src, cpx<typename G::T>* __re
         const T cd = (T)ms[member].c_drift;
 #pragma unroll
         for (int m = 0; m < G::E; ++m) {
-            cpx<T> f = Pf[base + G::TPL * m];
-            const cpx<T> p = Pp[base + G::TPL * m];
+            // f and pi are read next by the z kick, three passes later: streamed,
+            // so that the caches keep W for the y inverse
+            cpx<T> f = load_stream(Pf + base + G::TPL * m);
+            const cpx<T> p = load_stream(Pp + base + G::TPL * m);
             if constexpr (REC) {  // adjoint recording: substep start f and pi_half
                 rec_f[base + G::TPL * m] = f;
                 rec_ph[base + G::TPL * m] = p;
             }
             f.x += cd * p.x;
             f.y += cd * p.y;
-            Pf[base + G::TPL * m] = f;
+            store_stream(Pf + base + G::TPL * m, f);
             e[m] = f;
         }
     } else {
@@ -255,8 +257,8 @@ k_z_kick(const cpx<typename G::T>* __restrict__ W, const cpx<typename G::T>* __r
         const int kz = t + G::TPL * m;
         const int pz = signed_index(kz, N);
         const T n2 = (T)(pxy + pz * pz);
-        const cpx<T> f = Pf[base + G::TPL * m];
-        cpx<T> p = Pp[base + G::TPL * m];
+        const cpx<T> f = load_stream(Pf + base + G::TPL * m);
+        cpx<T> p = load_stream(Pp + base + G::TPL * m);
         if (m == 0 && line == 0 && t == 0) add_mean_force(e[0], f, ms[member], (double)G::N3);
         const T ln2 = lapc * n2;
         const cpx<T> F = {-ln2 * f.x - vc * e[m].x, -ln2 * f.y - vc * e[m].y};
