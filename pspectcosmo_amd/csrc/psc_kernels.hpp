@@ -242,7 +242,9 @@ k_z_kick(const cpx<typename G::T>* __restrict__ W, const cpx<typename G::T>* __r
     const size_t base = (size_t)member * G::MS + (line / N) * G::PS + (line % N) * N + t;
     cpx<T> e[G::E];
 #pragma unroll
-    for (int m = 0; m < G::E; ++m) e[m] = W[base + G::TPL * m];
+    for (int m = 0; m < G::E; ++m)   // W is not read again; streamed in FP64 (measured 14 % faster
+        // there, 1 % slower in FP32, where more of the arrays fit the memory-side cache)
+        e[m] = sizeof(T) == 8 ? load_stream(W + base + G::TPL * m) : W[base + G::TPL * m];
     line_fft<T, N, typename G::Rad, -1, typename G::LayRow>(e, t, c, lds, tw);
     const double a = use_a_start ? ms[member].a_start : ms[member].a;
     const T lapc = (T)(pow(a, 1 + sc.alpha) * sc.kIR * sc.kIR);
