@@ -33,7 +33,7 @@ __device__ __forceinline__ unsigned reversed_block(unsigned lin, unsigned total)
 // ---------------------------------------------------------------------------
 // A: contiguous-axis inverse pass; DRIFT fuses f += c_drift * pi (evolution.cpp:1164-1169).
 // REV: lines in reverse order, as in k_strided.
-template <class G, bool DRIFT, bool REC = false, bool REV = false>
+template <class G, bool DRIFT, bool REC = false, bool REV = false, bool STREAM = false>
 __global__ void __launch_bounds__((G::THREADS_ROW))
 k_z_inverse(const cpx<typename G::T>* __restrict__ src, cpx<typename G::T>* __restrict__ Pf, const cpx<typename G::T>* __restrict__ Pp,
             cpx<typename G::T>* __restrict__ W, const MemberState* __restrict__ ms,
@@ -53,17 +53,28 @@ k_z_inverse(const cpx<typename G::T>* __restrict__ src, cpx<typename G::T>* __re
         const T cd = (T)ms[member].c_drift;
 #pragma unroll
         for (int m = 0; m < G::E; ++m) {
-            // f and pi are read next by the z kick, three passes later: streamed,
-            // so that the caches keep W for the y inverse
-            cpx<T> f = load_stream(Pf + base + G::TPL * m);
-            const cpx<T> p = load_stream(Pp + base + G::TPL * m);
+            // f and pi are read next by the z kick, three passes later: with
+            // STREAM (arrays larger than the memory-side cache holds for
+            // three passes) they do not allocate, so that the caches keep W
+            // for the y inverse
+            cpx<T> f, p;
+            if constexpr (STREAM) {
+                f = load_stream(Pf + base + G::TPL * m);
+                p = load_stream(Pp + base + G::TPL * m);
+            } else {
+                f = Pf[base + G::TPL * m];
+                p = Pp[base + G::TPL * m];
+            }
             if constexpr (REC) {  // adjoint recording: substep start f and pi_half
                 rec_f[base + G::TPL * m] = f;
                 rec_ph[base + G::TPL * m] = p;
             }
             f.x += cd * p.x;
             f.y += cd * p.y;
-            store_stream(Pf + base + G::TPL * m, f);
+            if constexpr (STREAM)
+                store_stream(Pf + base + G::TPL * m, f);
+            else
+                Pf[base + G::TPL * m] = f;
             e[m] = f;
         }
     } else {
@@ -227,7 +238,7 @@ __device__ __forceinline__ void add_mean_force(cpx<T>& vk, const cpx<T>& f0, con
 // unnormalised spectrum of V'.  c1/c2 are w dt / 2 of the two kicks applied
 // with this force; REDUCE accumulates sum |pi|^2 after the first kick and
 // sum n^2 |f|^2 (kinetic_energy_k / gradient_energy_k, evolution.cpp:923-1035).
-template <class G, bool REDUCE, bool REC = false>
+template <class G, bool REDUCE, bool REC = false, bool STREAM = false>
 __global__ void __launch_bounds__((G::THREADS_ROW))
 k_z_kick(const cpx<typename G::T>* __restrict__ W, const cpx<typename G::T>* __restrict__ Pf, cpx<typename G::T>* __restrict__ Pp,
          const MemberState* __restrict__ ms, const cpx<typename G::T>* __restrict__ tw,
@@ -244,7 +255,7 @@ k_z_kick(const cpx<typename G::T>* __restrict__ W, const cpx<typename G::T>* __r
 #pragma unroll
     for (int m = 0; m < G::E; ++m)   // W is not read again; streamed in FP64 (measured 14 % faster
         // there, 1 % slower in FP32, where more of the arrays fit the memory-side cache)
-        e[m] = sizeof(T) == 8 ? load_stream(W + base + G::TPL * m) : W[base + G::TPL * m];
+        e[m] = (STREAM && sizeof(T) == 8) ? load_stream(W + base + G::TPL * m) : W[base + G::TPL * m];
     line_fft<T, N, typename G::Rad, -1, typename G::LayRow>(e, t, c, lds, tw);
     const double a = use_a_start ? ms[member].a_start : ms[member].a;
     const T lapc = (T)(pow(a, 1 + sc.alpha) * sc.kIR * sc.kIR);
@@ -259,8 +270,8 @@ k_z_kick(const cpx<typename G::T>* __restrict__ W, const cpx<typename G::T>* __r
         const int kz = t + G::TPL * m;
         const int pz = signed_index(kz, N);
         const T n2 = (T)(pxy + pz * pz);
-        const cpx<T> f = load_stream(Pf + base + G::TPL * m);
-        cpx<T> p = load_stream(Pp + base + G::TPL * m);
+        const cpx<T> f = STREAM ? load_stream(Pf + base + G::TPL * m) : Pf[base + G::TPL * m];
+        cpx<T> p = STREAM ? load_stream(Pp + base + G::TPL * m) : Pp[base + G::TPL * m];
         if (m == 0 && line == 0 && t == 0) add_mean_force(e[0], f, ms[member], (double)G::N3);
         const T ln2 = lapc * n2;
         const cpx<T> F = {-ln2 * f.x - vc * e[m].x, -ln2 * f.y - vc * e[m].y};
