@@ -50,6 +50,7 @@ def lib():
         L.orc_kernel_a.argtypes = [C.c_void_p]
         L.orc_step.argtypes = [C.c_void_p, C.c_int]
         L.orc_set_dt.argtypes = [C.c_void_p, C.c_double]
+        L.orc_plant_gradient_power.argtypes = [C.c_void_p, C.c_int]
         L.orc_get_k.argtypes = [C.c_void_p, _D, _D]
         L.orc_get_scalars.argtypes = [C.c_void_p, _D]
         L.orc_get_fields.argtypes = [C.c_void_p, _D, _D]
@@ -204,6 +205,11 @@ class Oracle:
 
     def set_dt(self, dt):
         lib().orc_set_dt(self._h, float(dt))
+
+    def plant_gradient_power(self, offset):
+        """Plant a wrong power of a on the gradient force (tests that must
+        detect such a mistake)."""
+        lib().orc_plant_gradient_power(self._h, int(offset))
 
     def get_k(self):
         N = self.N

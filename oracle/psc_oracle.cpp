@@ -250,6 +250,10 @@ struct orc_ctx {
     std::vector<double> f[2], pif[2], Vp[2];
     std::vector<cd> tmp;
     double t = 0, a = 1, ad = 0, add = 0;
+    // Planted mistake for the tests that must detect it: added to the
+    // exponent 1 + alpha of the gradient force a^{1+alpha} kIR^2 n^2 f.  Always
+    // 0 outside orc_plant_gradient_power.
+    int grad_power_offset = 0;
     // GW (6 symmetric components 11,22,33,12,13,23)
     std::vector<cd> u_k[6], pi_u_k[6], pi_u_half[6], src_k[6];
     std::vector<double> grad[2][3], src[6];
@@ -462,7 +466,7 @@ struct orc_ctx {
     // kernel_field_k_pad (evolution.cpp:1640-1676): V' on the padded grid, r2c
     // there, extract_padding, then the spectral force on the N grid.
     void kernel_field_pad(std::vector<cd>* out) {
-        const double norm1 = std::pow(a, 1 + cfg.alpha);
+        const double norm1 = std::pow(a, 1 + cfg.alpha + grad_power_offset);
         const double norm2 = std::pow(a, 3 + cfg.alpha);
         const int nf = cfg.nflds;
 #pragma omp parallel for schedule(static)
@@ -516,7 +520,7 @@ struct orc_ctx {
 
     // kernel_field_k_nopad (evolution.cpp:1092-1129).  Real f must be current.
     void kernel_field(std::vector<cd>* out) {
-        const double norm1 = std::pow(a, 1 + cfg.alpha);
+        const double norm1 = std::pow(a, 1 + cfg.alpha + grad_power_offset);
         const double norm2 = std::pow(a, 3 + cfg.alpha);
         const int nf = cfg.nflds;
 #pragma omp parallel for schedule(static)
@@ -1011,6 +1015,14 @@ int orc_step(orc_ctx* c, int nsteps) {
 // pspectcosmo.cpp:622-627).
 int orc_set_dt(orc_ctx* c, double dt) {
     c->cfg.dt = dt;
+    return 0;
+}
+
+// Test infrastructure: plants a wrong power of a on the gradient force
+// (a^{1+alpha+offset} in place of a^{1+alpha}, evolution.cpp:1099-1100,
+// 1123-1124) so that a test can show it detects such a mistake.
+int orc_plant_gradient_power(orc_ctx* c, int offset) {
+    c->grad_power_offset = offset;
     return 0;
 }
 

@@ -198,3 +198,62 @@ def test_spectra_bin_counts_and_parseval():
     tot = (w * np.abs(k[0]) ** 2).sum() - abs(k[0, 0, 0, 0]) ** 2
     bins = np.arange(1, num_bins(N) + 1)
     assert abs((s[:, 1] / bins).sum() * N ** 6 - tot) < 1e-10 * tot
+
+
+def _friedmann_run(dt, integrator, exact, T=1.0, plant=0):
+    """Two fields with gradient energy ~12 % of rho (kIR = 1, omegaStar = 0.1)
+    and expansion on; da/dt set from the full rho = K + G + V so that the
+    Friedmann constraint holds to rounding at t = 0 (the reference sets it
+    from the homogeneous energy only, initialize.cpp:649-668).  Returns the
+    constraint violation |H^2 - rho/3| / |H^2 + rho/3| (output.cpp:1180-1184)
+    at t = 0 and t = T, the energies at t = 0 and a(T)."""
+    cfg, model, ic = parity_setup(N=16, nflds=2, integrator=integrator, dt=dt, kIR=1.0, g2=50.0,
+                                  omega_star=0.1, yoshida_exact=exact, f0=(1.0, 0.3),
+                                  fd0=(-0.7, 0.2))
+    fk, pk, sc = O.generate_ic(cfg, model, ic)
+    o = O.Oracle(cfg, model)
+    o.plant_gradient_power(plant)
+    a0 = sc[0]
+    o.load(fk, pk, 0.0, a0, sc[1], 0.0)
+    rho = o.energies()[6]
+    ad0 = a0 * math.sqrt(rho / 3.0)   # fStar = MPl
+    o.load(fk, pk, 0.0, a0, ad0, 0.0)
+    add0 = o.kernel_a()
+    o.load(fk, pk, 0.0, a0, ad0, add0)
+    e0 = o.energies()
+    o.step(int(round(T / dt)))
+    e1 = o.energies()
+    return e0[7], e1[7], e0, e1[8]
+
+
+@pytest.mark.parametrize("integrator,exact,order", [(PSC_VV2, 0, 2), (PSC_Y4, 1, 4)])
+def test_friedmann_constraint_drift_order(integrator, exact, order):
+    """With fluctuations (gradient energy >= 10 % of rho) and expansion, the
+    Friedmann constraint is conserved by the continuum equations
+    (pi' = a^{1+al} lap f - a^{3+al} V', f' = a^{al-3} pi, a'' from
+    kernel_a_k_nopad, evolution.cpp:1061-1129), so the lattice violation at
+    a fixed time T drifts only through the integrator's truncation error and
+    falls by 2^order when dt halves."""
+    viol = []
+    for dt in (0.01, 0.005, 0.0025):
+        c0, c1, e0, aT = _friedmann_run(dt, integrator, exact)
+        G = e0[1] + e0[3]
+        assert G >= 0.10 * e0[6], (G, e0[6])
+        assert c0 < 1e-14
+        assert aT > 1.9   # a grows by ~2, so powers of a are exercised
+        viol.append(c1)
+    r1, r2 = viol[0] / viol[1], viol[1] / viol[2]
+    for r in (r1, r2):
+        assert 2 ** order * 0.9 < r < 2 ** order * 1.1, (viol, r1, r2)
+    assert viol[-1] < (6e-5 if order == 2 else 5e-8), viol
+
+
+@pytest.mark.parametrize("integrator,exact", [(PSC_VV2, 0), (PSC_Y4, 1)])
+def test_friedmann_constraint_catches_wrong_gradient_power(integrator, exact):
+    """A wrong power of a on the gradient force (a^{3+al} in place of the
+    reference's a^{1+al}, evolution.cpp:1099-1100,1123-1124), planted in the
+    oracle, breaks energy conservation: the violation is then large and does
+    not fall with dt, so the drift test above would fail."""
+    viol = [_friedmann_run(dt, integrator, exact, plant=2)[1] for dt in (0.01, 0.005)]
+    assert min(viol) > 1e-2, viol
+    assert viol[0] / viol[1] < 1.5, viol
