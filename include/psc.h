@@ -29,7 +29,7 @@
 extern "C" {
 #endif
 
-#define PSC_ABI_VERSION 1
+#define PSC_ABI_VERSION 2
 
 /* ---- status codes ------------------------------------------------------ */
 #define PSC_OK 0
@@ -88,13 +88,26 @@ typedef struct psc_config {
     double reserved_d[3];
 } psc_config;
 
-/* Potential V = m2 phi^2 / 2 + g2 phi^2 chi^2 / 2 (the chi term only when
- * nflds == 2).  The shipped single-field model.h (model.h:35-104) is the
- * nflds == 1, m2 == 1 case. */
+/* Couplings of the potential, per ensemble member.  The potential itself is
+ * compiled in from a model header, as the reference compiles model.h
+ * (model.h:34-104): each model header states which couplings it reads.  The
+ * built-in models are
+ *   preheat      (libpsc.so)             V = m2 phi^2 / 2 + g2 phi^2 chi^2 / 2;
+ *                with one field the reference's shipped phi^2 model.h;
+ *   lambda_phi4  (libpsc_lambda_phi4.so) V = lambda phi^4 / 4 + g2 phi^2 chi^2 / 2. */
 typedef struct psc_model {
-    double m2;
-    double g2;
+    double m2;         /* mass term                                            */
+    double g2;         /* phi^2 chi^2 coupling                                 */
+    double lambda;     /* phi^4 self-coupling                                  */
+    double c[5];       /* further couplings of a user model                    */
 } psc_model;
+
+/* Coupling indices (psc_adjoint_spec.coupling): the derivative of the
+ * adjoint's objective is taken with respect to this member of psc_model. */
+#define PSC_COUPLING_G2 0
+#define PSC_COUPLING_M2 1
+#define PSC_COUPLING_LAMBDA 2
+#define PSC_COUPLING_C0 3      /* c[i] is PSC_COUPLING_C0 + i                  */
 
 /* Initial-condition parameters (parameters.h:62,124-125; initialize.cpp:624-668). */
 typedef struct psc_ic_params {
@@ -111,6 +124,13 @@ typedef struct psc_ic_params {
 
 int psc_abi_version(void);
 const char* psc_status_string(int status);
+/* Name and formula of the model the library was built with
+ * (modelinfo_, model.h:10-19). */
+const char* psc_model_name(void);
+const char* psc_model_description(void);
+/* Number of potential terms of the model with nflds fields
+ * (num_potential_terms, model.h:34). */
+int psc_num_potential_terms(int nflds);
 
 /* Fill the initial half spectra for all fields, replacing initialize()
  * (initialize.cpp:562-810, non-inflation Rayleigh branch initialize.cpp:181-197).
@@ -190,10 +210,15 @@ int psc_get_fields(psc_ctx* ctx, int member, double* f, double* pi);
 int psc_energy_densities(psc_ctx* ctx, int member, double* pot, double* kin, double* grad, double* tot);
 
 /* Energies, replacing energy_density()/energy() (output.cpp:57-115,1069-1187).
- * out[PSC_ENERGY_LEN]: K_0, G_0, K_1, G_1, V_term0, V_term1, rho_total,
+ * out[PSC_ENERGY_LEN]: K_0, G_0, K_1, G_1, V_term0, V_term1 (the first two
+ * potential terms; psc_potential_terms has all of them), rho_total,
  * constraint violation |H^2 - rho/3| / |H^2 + rho/3| (output.cpp:1180-1184),
  * a, da/dt, d2a/dt2, t, then <f_0>, <f_1>, var f_0, var f_1 (output.cpp:380-461). */
 int psc_energies(psc_ctx* ctx, int member, double* out);
+
+/* Lattice averages of every potential term (potential_energy(term, NULL),
+ * model.h:38-52): out[t] for t < psc_num_potential_terms(nflds). */
+int psc_potential_terms(psc_ctx* ctx, int member, double* out);
 
 /* Means and variances as written by meansvars() (output.cpp:380-461), per
  * field q: out[6q .. 6q+5] = <f>, a^{alpha-3} <pi>, <f^2>, a^{2(alpha-3)} <pi^2>,
@@ -239,7 +264,8 @@ typedef struct psc_adjoint_spec {
                                once with recording before it is swept backwards     */
     int member;             /* 0 (single-lattice contexts only)                      */
     int projection;         /* GW objective: PSC_GW_PROJ_REFERENCE or PSC_GW_PROJ_TT */
-    int reserved_i;
+    int coupling;           /* PSC_COUPLING_*: out[1] is dJ/d(that coupling); the
+                               model must provide its derivatives                 */
     double phi0, dphi0, chi0;  /* homogeneous initial values; da/dt(0) = a0 H(phi0)
                                   as initialize.cpp:649-668                         */
     double reserved[4];
@@ -248,12 +274,13 @@ typedef struct psc_adjoint_spec {
 /* Gradient of the objective by the discrete adjoint of the step (reverse
  * kick-drift over the recorded substeps, Friedmann coupling included; with the
  * GW objective also the GW kick/drift and the source adjoint back to the
- * fields), with respect to g2, phi0 (the homogeneous mode and H(phi0),
+ * fields), with respect to a coupling of the model (spec->coupling, g2 by
+ * default), phi0 (the homogeneous mode and H(phi0),
  * fluctuation modes held fixed) and the initial real-space chi field.  The
  * run is split into segments of checkpoint_every steps: a forward pass stores
  * each segment's start state, then from the last segment backwards the segment
  * is replayed with recording and swept back.  The loaded state is left
- * unchanged.  out[0..5] = J, dJ/dg2, dJ/dphi0, forward milliseconds, backward
+ * unchanged.  out[0..5] = J, dJ/d(coupling), dJ/dphi0, forward milliseconds, backward
  * milliseconds (replays and sweeps), device bytes of the adjoint workspace.
  * dchi receives N^3 values dJ/dchi(x) when not NULL.  The workspace
  * (checkpoints, one recorded segment, adjoint arrays) stays allocated for

@@ -39,6 +39,7 @@
 #include <omp.h>
 
 #include "../include/psc.h"
+#include "../pspectcosmo_amd/csrc/psc_model.hpp"
 
 typedef std::complex<double> cd;
 
@@ -191,32 +192,31 @@ inline double pw2(double x) { return x * x; }
 inline int signed_index(int i, int N) { return i <= N / 2 ? i : i - N; }
 
 // ---------------------------------------------------------------------------
-// Model hooks: V = m2 phi^2/2 + g2 phi^2 chi^2/2, in the shape of model.h
-// (potential_energy model.h:35-73 per term, dvdf :75-82, d2Vdf2 :86-93 with the
-// Hessian term index fld + fld2, effective_mass :96-98).
+// Model hooks of the build's model header (model.h:35-98: potential_energy
+// per term, dvdf, d2Vdf2 with the Hessian index fld + fld2, effective_mass),
+// evaluated in FP64 for one lattice's couplings.  With one field chi is 0.
 struct Model {
     int nflds;
-    double m2, g2;
-    int num_terms() const { return nflds == 2 ? 2 : 1; }
+    psc::Couplings<double> cp;
+    int num_terms() const { return psc::Model::num_potential_terms(nflds); }
     double V_term(int term, const double* f) const {
-        if (term == 0) return 0.5 * m2 * pw2(f[0]);
-        return 0.5 * g2 * pw2(f[0]) * pw2(f[1]);
+        const double x[2] = {f[0], nflds == 2 ? f[1] : 0.0};
+        return psc::Model::potential_energy<double>(term, x, cp);
     }
     double dvdf(int fld, const double* f) const {
-        if (nflds == 1) return m2 * f[0];
-        if (fld == 0) return m2 * f[0] + g2 * f[0] * pw2(f[1]);
-        return g2 * pw2(f[0]) * f[1];
+        const double x[2] = {f[0], nflds == 2 ? f[1] : 0.0};
+        return psc::Model::dvdf<double>(fld, x, cp);
     }
     double d2Vdf2(int term, const double* f) const {
-        if (nflds == 1) return m2;
-        if (term == 0) return m2 + g2 * pw2(f[1]);
-        if (term == 1) return 2.0 * g2 * f[0] * f[1];
-        return g2 * pw2(f[0]);
+        const double x[2] = {f[0], nflds == 2 ? f[1] : 0.0};
+        return psc::Model::d2Vdf2<double>(term, x, cp);
     }
     void effective_mass(double* m, const double* f) const {
-        if (nflds == 1) { m[0] = m2; return; }
-        m[0] = m2 + g2 * pw2(f[1]);
-        m[1] = g2 * pw2(f[0]);
+        const double x[2] = {f[0], nflds == 2 ? f[1] : 0.0};
+        double ms[2];
+        psc::Model::effective_mass<double>(ms, x, cp);
+        m[0] = ms[0];
+        if (nflds == 2) m[1] = ms[1];
     }
 };
 
@@ -267,8 +267,7 @@ struct orc_ctx {
     orc_ctx(const psc_config& c, const psc_model& m)
         : cfg(c), N(c.N), Nh(c.N / 2 + 1), fft(c.N) {
         model.nflds = c.nflds;
-        model.m2 = m.m2;
-        model.g2 = m.g2;
+        model.cp = psc::Couplings<double>(m);
         nr = (size_t)N * N * N;
         nk = (size_t)N * N * Nh;
         N3 = (double)N * N * N;
@@ -806,6 +805,34 @@ extern "C" {
 
 int orc_abi_version(void) { return PSC_ABI_VERSION; }
 
+// The model header of this build, evaluated at one point (for the tests that
+// pin it: derivatives against differences of V, closed forms).  out:
+// [0..3] potential terms (0 beyond num_potential_terms), [4..5] dvdf,
+// [6..8] d2Vdf2 (phi phi, phi chi, chi chi), [9..10] effective masses,
+// [11..12] dvdf_delta(mean = f, d = df) in FP64, [13..14] the same in FP32,
+// [15] dV / d(coupling), [16..17] dV'_fld / d(coupling), [18] number of terms.
+const char* orc_model_name(void) { return psc::Model::name; }
+int orc_model_eval(const psc_model* m, int nflds, const double* f, const double* df, int coupling, double* out) {
+    const psc::Couplings<double> cp(*m);
+    const psc::Couplings<float> cpf(*m);
+    const double x[2] = {f[0], nflds == 2 ? f[1] : 0.0}, d[2] = {df[0], nflds == 2 ? df[1] : 0.0};
+    const int nt = psc::Model::num_potential_terms(nflds);
+    for (int t = 0; t < psc::kMaxTerms; ++t) out[t] = t < nt ? psc::Model::potential_energy<double>(t, x, cp) : 0.0;
+    for (int q = 0; q < 2; ++q) out[4 + q] = psc::Model::dvdf<double>(q, x, cp);
+    for (int t = 0; t < 3; ++t) out[6 + t] = psc::Model::d2Vdf2<double>(t, x, cp);
+    psc::Model::effective_mass<double>(out + 9, x, cp);
+    psc::dvdf_delta<psc::Model, double>(x, d, out + 11, cp);
+    const float xf[2] = {(float)x[0], (float)x[1]}, dff[2] = {(float)d[0], (float)d[1]};
+    float of[2];
+    psc::dvdf_delta<psc::Model, float>(xf, dff, of, cpf);
+    out[13] = of[0];
+    out[14] = of[1];
+    out[15] = psc::dV_dcoupling<psc::Model, double>(coupling, x, cp);
+    for (int q = 0; q < 2; ++q) out[16 + q] = psc::d2V_dfdcoupling<psc::Model, double>(coupling, q, x, cp);
+    out[18] = nt;
+    return psc::has_coupling<psc::Model>(coupling) ? 0 : PSC_ERR_UNSUPPORTED;
+}
+
 int orc_set_threads(int n) {
     if (n > 0) omp_set_num_threads(n);
     return omp_get_max_threads();
@@ -909,7 +936,7 @@ int orc_generate_gw_ic(const psc_config* cfg, const psc_model* model, const psc_
     const int N = cfg->N, Nh = N / 2 + 1;
     const size_t nk = (size_t)N * N * Nh;
     const double L = 2.0 * kPi / cfg->kIR, dx = L / N;
-    Model m{cfg->nflds, model->m2, model->g2};
+    Model m{cfg->nflds, psc::Couplings<double>(*model)};
     double a, ad, addp, H;
     hubble_init(*cfg, m, *ic, &a, &ad, &addp, &H);
     ParkMiller rng(ic->seed);

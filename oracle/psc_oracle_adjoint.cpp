@@ -25,6 +25,7 @@
 #include <omp.h>
 
 #include "../include/psc.h"
+#include "../pspectcosmo_amd/csrc/psc_model.hpp"
 
 typedef std::complex<double> cd;
 
@@ -96,7 +97,9 @@ inline double pw2(double x) { return x * x; }
 
 struct Adj {
     psc_config cfg;
-    double m2, g2;
+    psc_model par;
+    psc::Couplings<double> cp;
+    int coupling;   // PSC_COUPLING_* of the parameter derivative
     int N;
     long n3;
     double N3;
@@ -104,7 +107,7 @@ struct Adj {
     std::vector<double> n2;  // |n|^2 per packed index
     std::vector<double> w;
     bool stat;
-    Adj(const psc_config& c, double m2_, double g2_) : cfg(c), m2(m2_), g2(g2_), N(c.N), f1(c.N) {
+    Adj(const psc_config& c, const psc_model& m, int k) : cfg(c), par(m), cp(m), coupling(k), N(c.N), f1(c.N) {
         n3 = (long)N * N * N;
         N3 = (double)n3;
         n2.resize(n3);
@@ -131,9 +134,10 @@ struct Adj {
         for (auto& v : r) v /= N3;
         return r;
     }
+    // dvdf of the model (model.h:75-82) at the packed point r = phi + i chi
     cd vprime(cd r) const {
-        const double p = r.real(), q = r.imag();
-        return cd(m2 * p + g2 * p * q * q, g2 * p * p * q);
+        const double f[2] = {r.real(), r.imag()};
+        return cd(psc::Model::dvdf<double>(0, f, cp), psc::Model::dvdf<double>(1, f, cp));
     }
     // F(f, a) = -a^{1+al} kIR^2 n^2 f - a^{3+al} F(V'(r))
     std::vector<cd> force(const std::vector<cd>& f, double a) const {
@@ -146,7 +150,7 @@ struct Adj {
         return F;
     }
     struct KGV {
-        double K, G, V, Vg;  // Vg = <phi^2 chi^2 / 2>
+        double K, G, V, Vg;  // Vg = <dV / d(coupling)>
     };
     KGV kgv(const std::vector<cd>& f, const std::vector<cd>& p, double a) const {
         double sk = 0, sg = 0, sv = 0, svg = 0;
@@ -156,9 +160,9 @@ struct Adj {
         }
         std::vector<cd> r = real_space(f);
         for (auto& v : r) {
-            const double ph = v.real(), ch = v.imag();
-            sv += 0.5 * m2 * ph * ph + 0.5 * g2 * ph * ph * ch * ch;
-            svg += 0.5 * ph * ph * ch * ch;
+            const double f[2] = {v.real(), v.imag()};
+            sv += psc::potential_sum<psc::Model, double>(cfg.nflds, f, cp);
+            svg += psc::dV_dcoupling<psc::Model, double>(coupling, f, cp);
         }
         KGV o;
         o.K = 0.5 * sk / std::pow(a, 6) / (N3 * N3);
@@ -319,11 +323,14 @@ struct Adj {
         std::vector<cd> q(n3);
         double dv = 0, dg = 0;
         for (long i = 0; i < n3; ++i) {
-            const double ph = r[i].real(), ch = r[i].imag();
+            const double f[2] = {r[i].real(), r[i].imag()};
             const cd v = vprime(r[i]);
             dv += vb[i].real() * v.real() + vb[i].imag() * v.imag();
-            dg += vb[i].real() * ph * ch * ch + vb[i].imag() * ph * ph * ch;
-            const double hpp = m2 + g2 * ch * ch, hpc = 2 * g2 * ph * ch, hcc = g2 * ph * ph;
+            dg += vb[i].real() * psc::d2V_dfdcoupling<psc::Model, double>(coupling, 0, f, cp) +
+                  vb[i].imag() * psc::d2V_dfdcoupling<psc::Model, double>(coupling, 1, f, cp);
+            // Hessian, d2Vdf2 with the index fld + fld2 (model.h:86-93)
+            const double hpp = psc::Model::d2Vdf2<double>(0, f, cp), hpc = psc::Model::d2Vdf2<double>(1, f, cp),
+                         hcc = psc::Model::d2Vdf2<double>(2, f, cp);
             q[i] = cd(hpp * vb[i].real() + hpc * vb[i].imag(), hpc * vb[i].real() + hcc * vb[i].imag());
             if (rV) q[i] += (*rV)[i];
         }
@@ -508,10 +515,11 @@ GwObjective gw_objective(const Adj& s, const std::vector<cd> PU[3], int projecti
 // Shared forward/backward of the adjoint.  objective 0: <chi^2>; 1: rho_GW / rho.
 int run_adjoint(const psc_config* cfg, const psc_model* model, const double* f_k, const double* pi_k,
                 const double* u_k, const double* pi_u_k, double a0, double phi0, double dphi0, double chi0,
-                int nsteps, int objective, int projection, double* J, double* g2bar_out, double* phi0bar_out,
-                double* chibar_out) {
+                int nsteps, int objective, int projection, int coupling, double* J, double* g2bar_out,
+                double* phi0bar_out, double* chibar_out) {
     if (cfg->nflds != 2) return PSC_ERR_ARG;
-    Adj s(*cfg, model->m2, model->g2);
+    if (!psc::has_coupling<psc::Model>(coupling)) return PSC_ERR_UNSUPPORTED;
+    Adj s(*cfg, *model, coupling);
     s.gw = objective == 1;
     const long n3 = s.n3;
     const double N3 = s.N3;
@@ -527,7 +535,8 @@ int run_adjoint(const psc_config* cfg, const psc_model* model, const double* f_k
             pack(N, 2, pi_u_k + (size_t)2 * q * nkh * 2, PU[q]);
         }
     // background at t = 0 (initialize.cpp:649-668) and kernel_a on the state
-    const double V0 = 0.5 * s.m2 * phi0 * phi0 + 0.5 * s.g2 * phi0 * phi0 * chi0 * chi0;
+    const double fh0[2] = {phi0, chi0};
+    const double V0 = psc::potential_sum<psc::Model, double>(cfg->nflds, fh0, s.cp);
     const double f2m = pw2(cfg->fStar / cfg->MPl);
     const double H = std::sqrt((dphi0 * dphi0 / 2.0 + V0) * f2m / 3.0);
     double a = a0, b = s.stat ? 0.0 : H * a0;
@@ -647,9 +656,12 @@ int run_adjoint(const psc_config* cfg, const psc_model* model, const double* f_k
     std::vector<cd> rV0 = s.A_adjoint(f0, p0, a0, addbar, fbar, pbar, abar, g2bar);
     s.force_adjoint(f0, a0, Fpend, &rV0, fbar, abar, g2bar);
     if (s.gw) abar += s.gw_force_adjoint(f0, a0, U0, Fpend_u, ubar, fbar);
-    *g2bar_out = g2bar;
+    // the coupling also enters da/dt(0) = a0 H: dH/dc = f2m dV/dc(phi0, chi0) / (6 H)
+    const double fh[2] = {phi0, chi0};
+    *g2bar_out = g2bar + (s.stat ? 0.0 : bbar * a0 * f2m * psc::dV_dcoupling<psc::Model, double>(coupling, fh, s.cp) /
+                                             (6.0 * H));
     // phi0: DC mode phi0 N^3 and da/dt(0) = a0 H(phi0), dH/dphi0 = f2m V'(phi0) / (6 H)
-    const double vphi = s.m2 * phi0 + s.g2 * phi0 * chi0 * chi0;
+    const double vphi = psc::Model::dvdf<double>(0, fh, s.cp);
     *phi0bar_out = N3 * fbar[0].real() + (s.stat ? 0.0 : bbar * a0 * f2m * vphi / (6.0 * H));
     if (chibar_out) {
         std::vector<cd> g = fbar;
@@ -666,12 +678,12 @@ extern "C" {
 
 // Gradient of J = <chi^2> at the end of nsteps steps.  Inputs: initial half
 // spectra, a0, the homogeneous phi0, dphi0, chi0 used for H(phi0).  Outputs:
-// J, dJ/dg2, dJ/dphi0, dJ/dchi(x) (N^3, may be NULL).
+// J, dJ/d(coupling) (PSC_COUPLING_*), dJ/dphi0, dJ/dchi(x) (N^3, may be NULL).
 int orc_adjoint_chi2(const psc_config* cfg, const psc_model* model, const double* f_k, const double* pi_k,
-                     double a0, double phi0, double dphi0, double chi0, int nsteps, double* J,
+                     double a0, double phi0, double dphi0, double chi0, int nsteps, int coupling, double* J,
                      double* g2bar_out, double* phi0bar_out, double* chibar_out) {
-    return run_adjoint(cfg, model, f_k, pi_k, nullptr, nullptr, a0, phi0, dphi0, chi0, nsteps, 0, 0, J, g2bar_out,
-                       phi0bar_out, chibar_out);
+    return run_adjoint(cfg, model, f_k, pi_k, nullptr, nullptr, a0, phi0, dphi0, chi0, nsteps, 0, 0, coupling, J,
+                       g2bar_out, phi0bar_out, chibar_out);
 }
 
 // Gradient of the final GW energy fraction rho_GW / rho (projection
@@ -679,10 +691,10 @@ int orc_adjoint_chi2(const psc_config* cfg, const psc_model* model, const double
 // components as FFTW half spectra (11, 22, 33, 12, 13, 23).
 int orc_adjoint_gw(const psc_config* cfg, const psc_model* model, const double* f_k, const double* pi_k,
                    const double* u_k, const double* pi_u_k, double a0, double phi0, double dphi0, double chi0,
-                   int nsteps, int projection, double* J, double* g2bar_out, double* phi0bar_out,
-                   double* chibar_out) {
-    return run_adjoint(cfg, model, f_k, pi_k, u_k, pi_u_k, a0, phi0, dphi0, chi0, nsteps, 1, projection, J,
-                       g2bar_out, phi0bar_out, chibar_out);
+                   int nsteps, int projection, int coupling, double* J, double* g2bar_out,
+                   double* phi0bar_out, double* chibar_out) {
+    return run_adjoint(cfg, model, f_k, pi_k, u_k, pi_u_k, a0, phi0, dphi0, chi0, nsteps, 1, projection, coupling,
+                       J, g2bar_out, phi0bar_out, chibar_out);
 }
 
 }  // extern "C"

@@ -24,6 +24,11 @@ PSC_SPECTRA_COLS = 5
 PSC_GW_SPECTRA_COLS = 5
 PSC_OBJ_CHI2 = 0
 PSC_OBJ_GW_FRACTION = 1
+PSC_COUPLING_G2 = 0
+PSC_COUPLING_M2 = 1
+PSC_COUPLING_LAMBDA = 2
+PSC_COUPLING_C0 = 3
+PSC_MAX_TERMS = 4
 
 
 class PscConfig(C.Structure):
@@ -38,7 +43,7 @@ class PscConfig(C.Structure):
 
 
 class PscModel(C.Structure):
-    _fields_ = [("m2", C.c_double), ("g2", C.c_double)]
+    _fields_ = [("m2", C.c_double), ("g2", C.c_double), ("lambda_", C.c_double), ("c", C.c_double * 5)]
 
 
 class PscIcParams(C.Structure):
@@ -51,7 +56,7 @@ class PscIcParams(C.Structure):
 class PscAdjointSpec(C.Structure):
     _fields_ = [
         ("objective", C.c_int), ("nsteps", C.c_int), ("checkpoint_every", C.c_int), ("member", C.c_int),
-        ("projection", C.c_int), ("reserved_i", C.c_int),
+        ("projection", C.c_int), ("coupling", C.c_int),
         ("phi0", C.c_double), ("dphi0", C.c_double), ("chi0", C.c_double), ("reserved", C.c_double * 4),
     ]
 
@@ -68,9 +73,13 @@ def make_config(N, nflds=2, integrator=PSC_VV2, precision=PSC_FP64, batch=1, wit
     return c
 
 
-def make_model(m2=1.0, g2=0.0):
+def make_model(m2=1.0, g2=0.0, lam=0.0, c=()):
+    """Couplings of psc_model; which of them a model reads is stated by its
+    header (preheat: m2, g2; lambda_phi4: lam, g2)."""
     m = PscModel()
-    m.m2, m.g2 = m2, g2
+    m.m2, m.g2, m.lambda_ = m2, g2, lam
+    for i, v in enumerate(c):
+        m.c[i] = v
     return m
 
 

@@ -15,7 +15,8 @@
 //      ybar = -a'^{3+al} Fbar, z inverse.
 //   P2 y-axis inverse (k_strided).
 //   P3 x-axis inverse -> vbar = F^H ybar; q = Hess V(r') vbar + cV V'(r')
-//      (the V part of the add adjoint); x forward.  Reductions for abar, g2bar.
+//      (the V part of the add adjoint); x forward.  Reductions for abar and
+//      the coupling's adjoint.
 //   P4 y-axis forward (k_strided).
 //   P5 z-axis forward -> fbar += q_k / N^3; the drift adjoint
 //      pibar_h = pibar + 2 h a_h^{al-3} fbar.
@@ -51,7 +52,7 @@ struct AdjCoef {
 
 // Scalar adjoint state.
 struct AdjState {
-    double abar, bbar, addbar, g2bar;
+    double abar, bbar, addbar, g2bar;   // g2bar: the adjoint of the chosen coupling
     double addbar_used;   // addbar of the substep being undone (V part of its add adjoint)
     double a_e;           // forward a at that substep's end point
     double h_undo;        // w dt / 2 of that substep
@@ -107,7 +108,7 @@ template <class G>
 __global__ void __launch_bounds__((G::THREADS_COL))
 k_adj_p3(cpx<typename G::T>* __restrict__ W, const cpx<typename G::T>* __restrict__ re,
          const MemberState* __restrict__ ms, const AdjCoef* __restrict__ co,
-         const cpx<typename G::T>* __restrict__ tw, double* __restrict__ part) {
+         const cpx<typename G::T>* __restrict__ tw, double* __restrict__ part, int coupling) {
     using T = typename G::T;
     constexpr int N = G::N;
     __shared__ cpx<T> lds[G::LayCol::size(0)];
@@ -120,18 +121,21 @@ k_adj_p3(cpx<typename G::T>* __restrict__ W, const cpx<typename G::T>* __restric
 #pragma unroll
     for (int m = 0; m < G::E; ++m) e[m] = W[base + (size_t)G::TPL * m * stride];
     line_fft<T, N, typename G::Rad, +1, typename G::LayCol>(e, t, c, lds, tw);
-    const T m2 = (T)ms->m2, g2 = (T)ms->g2, cV = (T)co->cV;
+    const Couplings<T> cp = couplings<T>(*ms);
+    const T cV = (T)co->cV;
     double red[3] = {0.0, 0.0, 0.0};
 #pragma unroll
     for (int m = 0; m < G::E; ++m) {
         const cpx<T> r = re[base + (size_t)G::TPL * m * stride];
-        const T ph = r.x, ch = r.y;
+        const T f[2] = {r.x, r.y};
         const cpx<T> vb = e[m];
-        const T vpx = m2 * ph + g2 * ph * (ch * ch), vpy = g2 * (ph * ph) * ch;
+        // V' and the Hessian (dvdf, d2Vdf2 with index fld + fld2, model.h:75-93)
+        const T vpx = Model::dvdf<T>(0, f, cp), vpy = Model::dvdf<T>(1, f, cp);
         red[0] += (double)vb.x * vpx + (double)vb.y * vpy;
-        red[1] += (double)vb.x * (ph * ch * ch) + (double)vb.y * (ph * ph * ch);
-        red[2] += 0.5 * (double)(ph * ph) * (double)(ch * ch);
-        const T hpp = m2 + g2 * ch * ch, hpc = (T)2 * g2 * ph * ch, hcc = g2 * ph * ph;
+        red[1] += (double)vb.x * d2V_dfdcoupling<Model, T>(coupling, 0, f, cp) +
+                  (double)vb.y * d2V_dfdcoupling<Model, T>(coupling, 1, f, cp);
+        red[2] += (double)dV_dcoupling<Model, T>(coupling, f, cp);
+        const T hpp = Model::d2Vdf2<T>(0, f, cp), hpc = Model::d2Vdf2<T>(1, f, cp), hcc = Model::d2Vdf2<T>(2, f, cp);
         e[m] = cpx<T>{hpp * vb.x + hpc * vb.y + cV * vpx, hpc * vb.x + hcc * vb.y + cV * vpy};
     }
     line_fft<T, N, typename G::Rad, -1, typename G::LayCol>(e, t, c, lds, tw);
@@ -527,16 +531,18 @@ k_gw_objective(const cpx<typename G::T>* __restrict__ PU, cpx<typename G::T>* __
     if (threadIdx.x == 0) part[blockIdx.x] = r[0];
 }
 
-// <phi^2 chi^2 / 2> partial sums over recorded real-space fields (dV / dg2).
+// Partial sums of dV/d(coupling) over recorded real-space fields.
 template <class G>
 __global__ void __launch_bounds__(256)
-k_vg_sums(const cpx<typename G::T>* __restrict__ r, double* __restrict__ part) {
+k_vg_sums(const cpx<typename G::T>* __restrict__ r, const MemberState* __restrict__ ms, int coupling,
+          double* __restrict__ part) {
     constexpr int N = G::N;
+    const Couplings<double> cp(ms->par);
     double acc[1] = {0.0};
     for (size_t li = (size_t)blockIdx.x * blockDim.x + threadIdx.x; li < G::N3; li += (size_t)gridDim.x * blockDim.x) {
         const size_t idx = (li / ((size_t)N * N)) * G::PS + li % ((size_t)N * N);
-        const double ph = r[idx].x, ch = r[idx].y;
-        acc[0] += 0.5 * ph * ph * ch * ch;
+        const double f[2] = {(double)r[idx].x, (double)r[idx].y};
+        acc[0] += dV_dcoupling<Model, double>(coupling, f, cp);
     }
     __shared__ double sh[4];
     block_sum<1, 256>(acc, sh);
@@ -555,7 +561,7 @@ struct AdjPartials {
     const double* p5g;
 };
 
-__device__ void adj_block_reduce(const AdjPartials& P, int n1, int n3, int n5, bool r5, bool gw, double* sh,
+static __device__ void adj_block_reduce(const AdjPartials& P, int n1, int n3, int n5, bool r5, bool gw, double* sh,
                                  double* R) {
     constexpr int NR = 8;
     Comp acc[NR];
@@ -617,7 +623,7 @@ struct AdjScalarOp {
     int n1, n3, n5, pad2;        // partial counts of P1, P3, P5
 };
 
-__global__ void __launch_bounds__(256)
+static __global__ void __launch_bounds__(256)
 k_adj_scalars(AdjState* __restrict__ st, AdjCoef* __restrict__ co, AdjPartials parts,
               const MemberState* __restrict__ me, const MemberState* __restrict__ mh, AdjScalarOp op) {
     __shared__ double sh[8 * 4 * 2];

@@ -2,7 +2,9 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <cstddef>
+#include "../../include/psc.h"
 #include "psc_fft.hpp"
+#include "psc_model.hpp"
 
 namespace psc {
 
@@ -15,10 +17,19 @@ struct MemberState {
     double b, b_half, a_start, a_half;
     double c_drift;
     double K, G, V;   // reductions of the last completed substep (normalised)
-    double m2, g2;    // model couplings (model.h)
-    double pad0, pad1;
+    int nflds;        // fields evolved (1 or 2; with one, chi is held at 0)
+    int pad_i;
+    double pad[3];
+    psc_model par;    // couplings of the model (model.h)
+    double reserved[8];
 };
-static_assert(sizeof(MemberState) == 128, "MemberState layout");
+static_assert(sizeof(MemberState) == 256, "MemberState layout");
+
+// The couplings of a member in the field precision.
+template <typename T>
+__device__ __forceinline__ Couplings<T> couplings(const MemberState& s) {
+    return Couplings<T>(s.par);
+}
 
 // Constants of a run shared by all members.
 struct StepConsts {

@@ -128,7 +128,7 @@ extern "C" int psc_generate_ic(const psc_config* cfg, const psc_model* model, co
     if (!valid(cfg) || !model || !ic || !f_k || !pi_k || ic->a0 <= 0) return PSC_ERR_ARG;
     const int N = cfg->N, Nh = N / 2 + 1;
     const size_t nk = (size_t)N * N * Nh;
-    const Pot V{cfg->nflds, model->m2, model->g2};
+    const Pot V(cfg->nflds, *model);
     double a, ad, addp, H;
     background(*cfg, V, *ic, &a, &ad, &addp, &H);
     if (!(H >= 0.0)) return PSC_ERR_ARG;  // initialize.cpp:661-664
@@ -190,7 +190,7 @@ extern "C" int psc_generate_gw_ic(const psc_config* cfg, const psc_model* model,
     static const int ci[6] = {0, 1, 2, 0, 0, 1}, cj[6] = {0, 1, 2, 1, 2, 2};
     const int N = cfg->N, Nh = N / 2 + 1;
     const size_t nk = (size_t)N * N * Nh;
-    const Pot V{cfg->nflds, model->m2, model->g2};
+    const Pot V(cfg->nflds, *model);
     double a, ad, addp, H;
     background(*cfg, V, *ic, &a, &ad, &addp, &H);
     // The GW draws follow the 4 draws of every scalar mode of every field.

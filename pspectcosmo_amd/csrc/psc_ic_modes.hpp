@@ -19,11 +19,7 @@
 
 #include "../../include/psc.h"
 
-#if defined(__HIPCC__)
-#define PSC_HD __host__ __device__
-#else
-#define PSC_HD
-#endif
+#include "psc_model.hpp"
 // No contraction into fused multiply-adds inside the mode arithmetic, so that
 // host and device round every product and sum alike.
 #if defined(__clang__)
@@ -166,20 +162,24 @@ PSC_HD inline void polarisation(const double k[3], double e[3][3]) {
         for (int j = 0; j < 3; ++j) e[i][j] = u[i] * u[j] - v[i] * v[j] + u[i] * v[j] + v[i] * u[j];
 }
 
-// V = m2 phi^2 / 2 + g2 phi^2 chi^2 / 2 per term (model.h:35-73) and the
-// effective masses d2V/df2 at the homogeneous values (model.h:96-98).
+// The potential terms (potential_energy, model.h:35-73) and the effective
+// masses at the homogeneous values (effective_mass, model.h:96-98) of the
+// build's model for one member's couplings.
 struct Pot {
     int nflds;
-    double m2, g2;
-    int terms() const { return nflds == 2 ? 2 : 1; }
+    psc::Couplings<double> cp;
+    Pot(int nf, const psc_model& m) : nflds(nf), cp(m) {}
+    int terms() const { return psc::Model::num_potential_terms(nflds); }
     double term(int t, const double* f) const {
         PSC_NO_CONTRACT
-        return t == 0 ? 0.5 * m2 * pw2(f[0]) : 0.5 * g2 * pw2(f[0]) * pw2(f[1]);
+        return psc::Model::potential_energy<double>(t, f, cp);
     }
     void meff(double* m, const double* f) const {
         PSC_NO_CONTRACT
-        m[0] = m2 + (nflds == 2 ? g2 * pw2(f[1]) : 0.0);
-        if (nflds == 2) m[1] = g2 * pw2(f[0]);
+        double ms[2];
+        psc::Model::effective_mass<double>(ms, f, cp);
+        m[0] = ms[0];
+        if (nflds == 2) m[1] = ms[1];
     }
 };
 

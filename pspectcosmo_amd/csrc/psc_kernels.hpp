@@ -187,25 +187,26 @@ k_x_force(cpx<typename G::T>* __restrict__ W, const MemberState* __restrict__ ms
         e[m] = kept(m) ? W[base + (size_t)G::TPL * m * stride] : cpx<T>{(T)0, (T)0};
     line_fft<T, N, typename G::Rad, +1, typename G::LayCol>(e, t, c, lds, tw);
     const T inv = (T)(1.0 / (double)G::N3);
-    const T m2 = (T)ms[member].m2, g2 = (T)ms[member].g2;
-    const T half = (T)0.5;
+    const Couplings<T> cp = couplings<T>(ms[member]);
+    const int nf = ms[member].nflds;
     // homogeneous values from the k = 0 mode left out of the inverse transform
     const cpx<T> dc = Pf[(size_t)member * G::MS];
-    const T pb = (T)((double)dc.x / (double)G::N3), qb = (T)((double)dc.y / (double)G::N3);
+    const T mean[2] = {(T)((double)dc.x / (double)G::N3), nf > 1 ? (T)((double)dc.y / (double)G::N3) : (T)0};
     double v[1] = {0.0};
 #pragma unroll
     for (int m = 0; m < G::E; ++m) {
-        const T p = e[m].x * inv, q = e[m].y * inv;   // fluctuations
-        const T phi = pb + p, chi = qb + q;
-        if constexpr (REC) rec_r[base + (size_t)G::TPL * m * stride] = cpx<T>{phi, chi};  // adjoint recording
-        const T chi2 = chi * chi, phi2 = phi * phi;
-        // V'(phi, chi) - V'(pb, qb), written so that every term carries a
-        // fluctuation factor (no cancellation against the mean); the k = 0
-        // mode of V'(pb, qb) is added back in the z pass
-        e[m].x = m2 * p + g2 * (p * chi2 + pb * q * ((T)2 * qb + q));
-        e[m].y = g2 * (p * ((T)2 * pb + p) * chi + pb * pb * q);
+        const T d[2] = {e[m].x * inv, nf > 1 ? e[m].y * inv : (T)0};   // fluctuations
+        const T f[2] = {mean[0] + d[0], mean[1] + d[1]};
+        if constexpr (REC) rec_r[base + (size_t)G::TPL * m * stride] = cpx<T>{f[0], f[1]};  // adjoint recording
+        // V'(f) - V'(mean) (dvdf, model.h:75-82), written so that every term
+        // carries a fluctuation factor (no cancellation against the mean);
+        // the k = 0 mode of V'(mean) is added back in the z pass
+        T dv[2];
+        dvdf_delta<Model, T>(mean, d, dv, cp);
+        e[m].x = dv[0];
+        e[m].y = nf > 1 ? dv[1] : (T)0;
         // V per site in the field precision, summed in FP64
-        v[0] += (double)(half * m2 * phi2 + half * g2 * phi2 * chi2);
+        v[0] += (double)potential_sum<Model, T>(nf, f, cp);
     }
     // In FP64 the forward transform reloads its twiddles (cached) instead of
     // keeping the inverse transform's live across the V' step: an opaque copy
@@ -227,9 +228,10 @@ k_x_force(cpx<typename G::T>* __restrict__ W, const MemberState* __restrict__ ms
 // homogeneous values pb + i qb = f(k = 0) / N^3, computed in FP64.
 template <typename T>
 __device__ __forceinline__ void add_mean_force(cpx<T>& vk, const cpx<T>& f0, const MemberState& s, double N3) {
-    const double pb = (double)f0.x / N3, qb = (double)f0.y / N3;
-    vk.x += (T)(N3 * (s.m2 * pb + s.g2 * pb * qb * qb));
-    vk.y += (T)(N3 * (s.g2 * pb * pb * qb));
+    const Couplings<double> cp(s.par);
+    const double mean[2] = {(double)f0.x / N3, s.nflds > 1 ? (double)f0.y / N3 : 0.0};
+    vk.x += (T)(N3 * Model::dvdf<double>(0, mean, cp));
+    if (s.nflds > 1) vk.y += (T)(N3 * Model::dvdf<double>(1, mean, cp));
 }
 
 // ---------------------------------------------------------------------------
@@ -326,7 +328,7 @@ __device__ __forceinline__ void comp_merge(double& s, double& c, double s2, doub
 constexpr int kScalarThreads = 256;
 constexpr int kScalarDirect = 512;
 
-__global__ void __launch_bounds__(kScalarThreads)
+static __global__ void __launch_bounds__(kScalarThreads)
 k_scalars(MemberState* __restrict__ ms, const double* __restrict__ partV,
           const double* __restrict__ partKG, ScalarOp op, StepConsts sc, MemberState* __restrict__ rec = nullptr) {
     constexpr int WAVES = kScalarThreads / 64;
@@ -415,8 +417,11 @@ k_scalars(MemberState* __restrict__ ms, const double* __restrict__ partV,
 // ---------------------------------------------------------------------------
 // Diagnostics: x-axis inverse pass that normalises by 1/N^3, writes the real
 // fields (phi, chi) back into W, and reduces sum phi, sum chi, sum phi^2,
-// sum chi^2, sum V_0, sum V_1 (meansvars output.cpp:427-447, potential terms
-// model.h:35-73).
+// sum chi^2 (meansvars output.cpp:427-447), the sum of each potential term
+// (potential_energy, model.h:35-73) and of each field's effective mass
+// (effective_mass, model.h:96-98, averaged as output.cpp:569-580): the
+// kDiagWidth partial sums of a block.
+constexpr int kDiagWidth = 4 + kMaxTerms + 2;
 template <class G>
 __global__ void __launch_bounds__((G::THREADS_COL))
 k_x_inverse_diag(cpx<typename G::T>* __restrict__ W, const MemberState* __restrict__ ms,
@@ -436,10 +441,14 @@ k_x_inverse_diag(cpx<typename G::T>* __restrict__ W, const MemberState* __restri
     for (int m = 0; m < G::E; ++m) e[m] = W[base + (size_t)G::TPL * m * stride];
     line_fft<T, N, typename G::Rad, +1, typename G::LayCol>(e, t, c, lds, tw);
     const double inv = 1.0 / (double)G::N3;
-    const double m2 = ms[member].m2, g2 = ms[member].g2;
+    const Couplings<double> cp(ms[member].par);
+    const int nf = ms[member].nflds;
+    const int nterms = Model::num_potential_terms(nf);
     const cpx<T> dc = src[(size_t)member * G::MS];   // k = 0 mode, left out of the z pass
     const double pb = (double)dc.x * inv, qb = (double)dc.y * inv;
-    double r[6] = {0, 0, 0, 0, 0, 0};
+    double r[kDiagWidth];
+#pragma unroll
+    for (int q = 0; q < kDiagWidth; ++q) r[q] = 0.0;
 #pragma unroll
     for (int m = 0; m < G::E; ++m) {
         const double phi = pb + (double)e[m].x * inv, chi = qb + (double)e[m].y * inv;
@@ -449,16 +458,22 @@ k_x_inverse_diag(cpx<typename G::T>* __restrict__ W, const MemberState* __restri
         r[1] += chi;
         r[2] += phi * phi;
         r[3] += chi * chi;
-        r[4] += 0.5 * m2 * (phi * phi);
-        r[5] += 0.5 * g2 * (phi * phi) * (chi * chi);
+        const double f[2] = {phi, nf > 1 ? chi : 0.0};
+#pragma unroll
+        for (int term = 0; term < kMaxTerms; ++term)
+            if (term < nterms) r[4 + term] += Model::potential_energy<double>(term, f, cp);
+        double msq[2];
+        Model::effective_mass<double>(msq, f, cp);
+        r[4 + kMaxTerms] += msq[0];
+        r[5 + kMaxTerms] += msq[1];
     }
 #pragma unroll
     for (int m = 0; m < G::E; ++m) W[base + (size_t)G::TPL * m * stride] = e[m];
-    block_sum<6, G::THREADS_COL>(r, reinterpret_cast<double*>(lds));
+    block_sum<kDiagWidth, G::THREADS_COL>(r, reinterpret_cast<double*>(lds));
     if (threadIdx.x == 0) {
-        double* dst = part + ((size_t)member * G::BLOCKS_COL + blockIdx.y * gridDim.x + blockIdx.x) * 6;
+        double* dst = part + ((size_t)member * G::BLOCKS_COL + blockIdx.y * gridDim.x + blockIdx.x) * kDiagWidth;
 #pragma unroll
-        for (int q = 0; q < 6; ++q) dst[q] = r[q];
+        for (int q = 0; q < kDiagWidth; ++q) dst[q] = r[q];
     }
 }
 
@@ -594,7 +609,7 @@ __device__ void prereduce_chunk(const double* __restrict__ in, int n, double* __
 // Both first-level reductions of a substep in one launch: blocks
 // [0, chunks) cut the V partials (width 1), blocks [chunks, 2 chunks) the
 // K, G partials (width 2); a pass with few blocks (n <= chunks) is skipped.
-__global__ void __launch_bounds__(256)
+static __global__ void __launch_bounds__(256)
 k_prereduce(const double* __restrict__ inV, int nV, double* __restrict__ outV, const double* __restrict__ inKG,
             int nKG, double* __restrict__ outKG, int chunks) {
     const int member = blockIdx.y;
@@ -605,8 +620,17 @@ k_prereduce(const double* __restrict__ inV, int nV, double* __restrict__ outV, c
     }
 }
 
+// Device-to-device copy of n complex elements, launched in place of a memcpy
+// where the copy is part of a captured graph (so that the graph holds kernel
+// nodes only).
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_copy(cpx<T>* __restrict__ dst, const cpx<T>* __restrict__ src, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) dst[i] = src[i];
+}
+
 // Fixed-order sum of per-block partial vectors: out[q] = sum_b part[b][q].
-__global__ void k_sum_partials(const double* __restrict__ part, int nblocks, int width,
+static __global__ void k_sum_partials(const double* __restrict__ part, int nblocks, int width,
                                double* __restrict__ out) {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= width) return;
@@ -635,8 +659,8 @@ __device__ __forceinline__ int deriv_index(int i, int N) { return i == N / 2 ? 0
 // Snapshot energy densities (snapshots, output.cpp:1614-1625), x-axis inverse
 // pass into a dense x-major N^3 double array D (the order of the reference's
 // f[fld][i][j][k]).  MODE 0: potential V(phi, chi) summed over the terms
-// (model.h:35-73), the coupling term times scale (1 with two fields, 0 with
-// one), with the k = 0 mode of src added back as in k_x_inverse_diag; MODE 1: scale * (pi_phi^2 + pi_chi^2), likewise;
+// (model.h:35-73; scale unused), with the k = 0 mode of src added back as in
+// k_x_inverse_diag; MODE 1: scale * (pi_phi^2 + pi_chi^2), likewise;
 // MODE 2: D += scale * (d phi^2 + d chi^2) for one derivative axis (no k = 0
 // mode).  W and src are one member's arrays; the grid covers one member.
 template <class G, int MODE>
@@ -662,13 +686,17 @@ k_x_density(const cpx<typename G::T>* __restrict__ W, double* __restrict__ D, co
         pb = (double)dc.x * inv;
         qb = (double)dc.y * inv;
     }
-    const double m2 = ms[0].m2, g2 = ms[0].g2;
+    const Couplings<double> cp(ms[0].par);
+    const int nf = ms[0].nflds;
 #pragma unroll
     for (int m = 0; m < G::E; ++m) {
-        const double u = pb + (double)e[m].x * inv, v = qb + (double)e[m].y * inv;
+        const double u = pb + (double)e[m].x * inv, v = nf > 1 ? qb + (double)e[m].y * inv : 0.0;
         const size_t x = (size_t)(t + G::TPL * m);
         double* d = D + (x * N + o) * N + z;
-        if constexpr (MODE == 0) *d = 0.5 * m2 * (u * u) + scale * (0.5 * g2 * (u * u) * (v * v));
+        if constexpr (MODE == 0) {
+            const double f[2] = {u, v};
+            *d = potential_sum<Model, double>(nf, f, cp);
+        }
         else if constexpr (MODE == 1) *d = scale * (u * u + v * v);
         else *d += scale * (u * u + v * v);
     }

@@ -92,8 +92,10 @@ const char* kKeys[][2] = {
     {"fStar", "field unit"},
     {"omegaStar", "time unit"},
     {"MPl", "Planck mass"},
-    {"m2", "V = m2 phi^2 / 2 + g2 phi^2 chi^2 / 2"},
-    {"g2", "coupling"},
+    {"m2", "mass coupling of the model (psc_model.m2)"},
+    {"g2", "phi^2 chi^2 coupling (psc_model.g2)"},
+    {"lambda", "phi^4 coupling (psc_model.lambda)"},
+    {"couplings", "further couplings c[0..4] of a user model, comma separated"},
     {"initfield", "homogeneous field values, comma separated"},
     {"initderivs", "homogeneous field derivatives, comma separated"},
     {"seed", "Park-Miller seed"},
@@ -203,13 +205,15 @@ struct Run {
         }
         double e[PSC_ENERGY_LEN];
         check(psc_energies(ctx, 0, e), "psc_energies");
-        const int nterms = cfg.nflds == 2 ? 2 : 1;
+        const int nterms = psc_num_potential_terms(cfg.nflds);
+        double vt[8] = {0};
+        check(psc_potential_terms(ctx, 0, vt), "psc_potential_terms");
         std::vector<double> vals;
         for (int f = 0; f < cfg.nflds; ++f) {
             vals.push_back(e[2 * f]);
             vals.push_back(e[2 * f + 1]);
         }
-        for (int k = 0; k < nterms; ++k) vals.push_back(e[4 + k]);
+        for (int k = 0; k < nterms; ++k) vals.push_back(vt[k]);
         const double total = e[6];
         vals.push_back(total);
         energy << std::fixed << std::setprecision(kFreqPrec) << t;
@@ -366,7 +370,8 @@ struct Run {
         std::ofstream info(path("info"), std::ios::out);
         const double L = 2 * M_PI / cfg.kIR;
         info << "--------------------------\nModel Specific Information\n--------------------------\n";
-        info << "V = m2 phi^2 / 2 + g2 phi^2 chi^2 / 2, m2 = " << model.m2 << ", g2 = " << model.g2 << "\n";
+        info << psc_model_name() << " model\n" << psc_model_description() << "\n";   // modelinfo_ (model.h:10-19)
+        info << "m2 = " << model.m2 << ", g2 = " << model.g2 << ", lambda = " << model.lambda << "\n";
         info << "\n--------------------------\nGeneral Program Information\n-----------------------------\n";
         info << "Grid size=" << cfg.N << "^3\n";
         info << "Number of fields=" << cfg.nflds << "\n";
@@ -435,6 +440,11 @@ int main(int argc, char** argv) {
     c.MPl = p.num("MPl", 1.0);
     r.model.m2 = p.num("m2", 1.0);
     r.model.g2 = p.num("g2", 2.5e5);
+    r.model.lambda = p.num("lambda", 0.0);
+    {
+        const std::vector<double> cs = p.list("couplings", {});
+        for (size_t i = 0; i < cs.size() && i < 5; ++i) r.model.c[i] = cs[i];
+    }
     const std::vector<double> f0 = p.list("initfield", {1.0, 0.0}), fd0 = p.list("initderivs", {-0.7, 0.0});
     for (int q = 0; q < 2; ++q) {
         r.ic.f0[q] = q < (int)f0.size() ? f0[q] : 0.0;

@@ -25,21 +25,25 @@ def test_header_and_binding_agree():
     assert header_functions() == sorted(L.exported_symbols())
 
 
-def test_library_exports_every_header_symbol():
-    lib = C.CDLL(L.LIB_PATH)
+@pytest.mark.parametrize("model_name", ["preheat", "lambda_phi4"])
+def test_library_exports_every_header_symbol(model_name):
+    lib = C.CDLL(L.lib_path(model_name))
     missing = [s for s in header_functions() if not hasattr(lib, s)]
     assert not missing, missing
+    assert lib.psc_abi_version() == 2
+    lib.psc_model_name.restype = C.c_char_p
+    assert lib.psc_model_name().decode() == model_name
 
 
 def test_struct_sizes_match_header():
-    # psc_config: 16 ints + 8 doubles; psc_model: 2 doubles; psc_ic_params: 5 doubles + 2 ints + 4 doubles
+    # psc_config: 16 ints + 8 doubles; psc_model: 8 doubles; psc_ic_params: 5 doubles + 2 ints + 4 doubles
     assert C.sizeof(PscConfig) == 16 * 4 + 8 * 8
-    assert C.sizeof(PscModel) == 16
+    assert C.sizeof(PscModel) == 64
     assert C.sizeof(PscIcParams) == 5 * 8 + 2 * 4 + 4 * 8
 
 
 def test_abi_version():
-    assert L.lib().psc_abi_version() == 1
+    assert L.lib().psc_abi_version() == 2
     assert L.lib().psc_num_bins(64) == 56
 
 
@@ -48,6 +52,19 @@ def test_host_ic_matches_oracle_bitwise(N, nflds, seed):
     cfg, model, ic = parity_setup(N=N, nflds=nflds, seed=seed)
     fk, pk, sc = L.generate_ic(cfg, model, ic)
     ofk, opk, osc = O.generate_ic(cfg, model, ic)
+    np.testing.assert_array_equal(fk, ofk)
+    np.testing.assert_array_equal(pk, opk)
+    np.testing.assert_array_equal(sc, osc[:4])
+
+
+@pytest.mark.parametrize("nflds", [1, 2])
+def test_host_ic_matches_oracle_bitwise_lambda_phi4(nflds):
+    """Both sides built with the lambda phi^4 model header: the effective
+    masses 3 lambda phi0^2 (+ g2 chi0^2), g2 phi0^2 set the mode frequencies."""
+    cfg, _, ic = parity_setup(N=16, nflds=nflds, seed=5, f0=(1.2, 0.1))
+    model = make_model(m2=0.0, g2=3.0e3, lam=50.0)
+    fk, pk, sc = L.generate_ic(cfg, model, ic, model_name="lambda_phi4")
+    ofk, opk, osc = O.generate_ic(cfg, model, ic, model_name="lambda_phi4")
     np.testing.assert_array_equal(fk, ofk)
     np.testing.assert_array_equal(pk, opk)
     np.testing.assert_array_equal(sc, osc[:4])
