@@ -52,6 +52,9 @@ extern "C" {
 #define PSC_EXPANSION_STATIC 0 /* a = 1, da/dt = 0 (Minkowski test mode)       */
 #define PSC_EXPANSION_SELF 2   /* self-consistent Friedmann, parameters.h:65   */
 
+#define PSC_DT_FIXED 0         /* psc_config.dt_rule                           */
+#define PSC_DT_REFERENCE 1
+
 #define PSC_GW_PROJ_REFERENCE 0 /* one-sided P.u as in output.cpp:819-831      */
 #define PSC_GW_PROJ_TT 1        /* Lambda_ij,lm u_lm, true TT projection       */
 
@@ -79,7 +82,11 @@ typedef struct psc_config {
                           device / N / precision / batch / GW, if any                   */
     int pad;           /* 0: evolve_VV_k_nopad; 2: the dealiased stepper evolve_VV_k_pad
                           (PAD_FLAG, evolution.cpp:1271-1759) with N_pad = 2 N          */
-    int reserved_i[4];
+    int dt_rule;       /* PSC_DT_FIXED: every step takes dt; PSC_DT_REFERENCE: each step
+                          takes the reference's calculate_dt, capped and clamped to the
+                          output time of psc_set_output_times (pspectcosmo.cpp:213-233,
+                          622-627), computed on the device                           */
+    int reserved_i[3];
     double kIR;        /* lattice IR wavenumber, L = 2 pi / kIR (parameters.h:28,32)       */
     double dt;         /* time step in program units (parameters.h:36)                     */
     double fStar;      /* field rescaling (parameters.h:133)                               */
@@ -182,16 +189,32 @@ int psc_load_fields(psc_ctx* ctx, int member, const double* f, const double* pi,
                     double ad);
 /* Overwrite the background scalars t, a, da/dt, d2a/dt2 of one member. */
 int psc_set_scalars(psc_ctx* ctx, int member, double t, double a, double ad, double add);
-/* out[0..3] = t, a, da/dt, d2a/dt2 (pspectcosmo.h:44-45). */
+/* out[0..3] = t, a, da/dt, d2a/dt2 (pspectcosmo.h:44-45).  Reads the state
+ * without synchronising the momenta (see psc_step). */
 int psc_get_scalars(psc_ctx* ctx, int member, double* out);
+/* Steps of nonzero dt a member has taken since its data were loaded (the
+ * driver's step counter, pspectcosmo.cpp:683). */
+int psc_get_step_count(psc_ctx* ctx, int member, long long* steps);
 
-/* Advance every member by nsteps fixed steps of cfg->dt; replaces
- * `evolve_VV_k_nopad(dt)` called from the main loop (pspectcosmo.cpp:636). */
+/* Advance every member by nsteps steps; replaces `evolve_VV_k_nopad(dt)`
+ * called from the main loop (pspectcosmo.cpp:636).  With PSC_DT_FIXED each
+ * step takes dt; with PSC_DT_REFERENCE each step's dt is the reference's rule
+ * evaluated on the device, and a member that has reached its output time
+ * takes steps of zero for the rest of the call.  The call leaves the momenta
+ * one look-ahead kick ahead (the first kick of the next step, which the
+ * reference recomputes from the same fields, evolution.cpp:1144-1152), so
+ * that a following psc_step continues without a force evaluation; every
+ * function that reads or changes the state first restores the synchronised
+ * momenta, which the last step also stored.  psc_step(n) and n calls of
+ * psc_step(1) give bit-identical states. */
 int psc_step(psc_ctx* ctx, int nsteps);
-/* Change the time step used by later psc_step calls, for the reference's
- * adaptive rule dt = 2 / (k_max / a) / 10 clamped to output times
- * (calculate_dt, pspectcosmo.cpp:213-233, 622-627).  Drops captured steps. */
+/* Change the fixed time step (PSC_DT_FIXED) for later psc_step calls. */
 int psc_set_dt(psc_ctx* ctx, double dt);
+/* PSC_DT_REFERENCE: the next output time t_out that steps land on (the
+ * earlier of the reference's next_Output_Freq_time and
+ * next_Output_Infreq_time, pspectcosmo.cpp:624-627) and the largest dt
+ * (tOutputFreq), for every member.  A huge t_out leaves the rule unclamped. */
+int psc_set_output_times(psc_ctx* ctx, double t_out, double dt_cap);
 /* Block until all device work of the context is done. */
 int psc_sync(psc_ctx* ctx);
 

@@ -120,9 +120,9 @@ k_adj_p3(cpx<typename G::T>* __restrict__ W, const cpx<typename G::T>* __restric
     cpx<T> e[G::E];
 #pragma unroll
     for (int m = 0; m < G::E; ++m) e[m] = W[base + (size_t)G::TPL * m * stride];
-    line_fft<T, N, typename G::Rad, +1, typename G::LayCol>(e, t, c, lds, tw);
-    const Couplings<T> cp = couplings<T>(*ms);
+    const Couplings<T> cp = couplings<T>(*ms);   // read before the transform hides their latency
     const T cV = (T)co->cV;
+    line_fft<T, N, typename G::Rad, +1, typename G::LayCol>(e, t, c, lds, tw);
     double red[3] = {0.0, 0.0, 0.0};
 #pragma unroll
     for (int m = 0; m < G::E; ++m) {

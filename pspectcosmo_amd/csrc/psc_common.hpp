@@ -12,18 +12,46 @@ namespace psc {
 // the reference globals (pspectcosmo.h:44-45); b, b_half, a_start, a_half and
 // c_drift are the temporaries of evolve_VV_k_nopad (evolution.cpp:1133-1194)
 // carried across kernel launches so a captured step needs no host round trip.
+// The step size and the kick coefficients live here too, so that a captured
+// step does not depend on dt and the reference's per-step dt rule
+// (calculate_dt, pspectcosmo.cpp:213-233, 622-627) runs on the device.
 struct MemberState {
     double t, a, ad, add;
     double b, b_half, a_start, a_half;
     double c_drift;
     double K, G, V;   // reductions of the last completed substep (normalised)
     int nflds;        // fields evolved (1 or 2; with one, chi is held at 0)
-    int pad_i;
-    double pad[3];
+    int ahead;        // pi (and pi_u, the ghosts) hold one look-ahead kick: the
+                      // synchronised values are in the sync buffers, the
+                      // synchronised a is a_start
+    double dt;        // the step being taken
+    double kick1;     // the coming z kick's coefficients: its substep's second
+    double kick2;     // kick w dt / 2, and the next substep's first (the next
+                      // step's when the substep is the last: the look-ahead)
     psc_model par;    // couplings of the model (model.h)
-    double reserved[8];
+    double dt_next;   // dt of the next step (set when the last substep begins)
+    double t_out;     // adaptive rule: the output time the steps land on
+    double dt_cap;    // adaptive rule: largest dt (tOutputFreq)
+    double steps;     // steps taken with dt > 0 since the data were loaded
+    double reserved[4];
 };
 static_assert(sizeof(MemberState) == 256, "MemberState layout");
+
+// Kick coefficients of a z-kick launch.
+struct KickArgs {
+    double c1, c2;      // explicit coefficients (from_state == 0: the adjoint's replays)
+    int from_state;     // 1: c1, c2 = the member's kick1, kick2
+    int prologue;       // first kick of a call's first substep only, at a_start
+                        // (c2 = 0); members already ahead are left alone
+    int no_lookahead;   // c2 = 0: the last substep of a call that ends synchronised
+    int pad;
+};
+
+// The coefficients a kick launch applies to one member.
+__device__ __forceinline__ void kick_coefs(const KickArgs& ka, const MemberState& s, double& c1, double& c2) {
+    c1 = ka.from_state ? s.kick1 : ka.c1;
+    c2 = (ka.prologue || ka.no_lookahead) ? 0.0 : (ka.from_state ? s.kick2 : ka.c2);
+}
 
 // The couplings of a member in the field precision.
 template <typename T>
@@ -33,16 +61,32 @@ __device__ __forceinline__ Couplings<T> couplings(const MemberState& s) {
 
 // Constants of a run shared by all members.
 struct StepConsts {
-    double dt;
+    double dt;        // fixed step (PSC_DT_FIXED)
     double kIR;
     double f2m;       // (fStar / MPl)^2
     double N3;        // N^3
     int alpha;
     int expansion_static;
     int nflds;
-    int pad;
+    int dt_rule;      // PSC_DT_FIXED or PSC_DT_REFERENCE
     double NV;        // sites of the lattice V is averaged over (N^3, or N_pad^3 dealiased)
+    double kmax;      // kIR sqrt(3) N / 2, the momentum of calculate_dt (pspectcosmo.cpp:622)
 };
+
+// dt of a step that starts at (a, t): the fixed step, or the reference's rule
+// dt = 2 / (kmax / a) / 10, capped at tOutputFreq and clamped to land on the
+// next output time (pspectcosmo.cpp:213-233, 622-627); a member that has
+// reached its output time (within the reference's 1e-10) takes steps of 0.
+__host__ __device__ inline double step_dt(const MemberState& s, double a, double t, const StepConsts& sc) {
+    if (sc.dt_rule == 0) return s.dt;
+    double dt = 2.0 / (sc.kmax / a) / 10.0;
+    dt = dt < s.dt_cap ? dt : s.dt_cap;
+    if (t + dt > s.t_out) {
+        const double rem = s.t_out - t;
+        dt = rem < 1e-10 ? 0.0 : rem;
+    }
+    return dt;
+}
 
 // Launch shape of the FFT passes: precision T, lattice N, radix plan Rad
 // (first radix R1, TPL = N / R1 threads per line), TC consecutive z columns

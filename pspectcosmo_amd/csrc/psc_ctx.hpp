@@ -70,7 +70,7 @@ struct EngineBase;
 struct PadBase {
     virtual ~PadBase() {}
     // Drift (optional), padded V', extraction and both kicks; as force_chain.
-    virtual int force_chain(psc_ctx* c, bool drift, bool reduce, double c1, double c2, int use_a_start) = 0;
+    virtual int force_chain(psc_ctx* c, bool drift, bool reduce, const KickArgs& ka, bool sync_out) = 0;
     virtual int nV() const = 0;    // partial sums of V per force evaluation
     virtual int nKG() const = 0;   // partial sums of K, G per kick
     virtual size_t padded_elems() const = 0;
@@ -162,7 +162,17 @@ struct psc_ctx {
     Timing kt[K_NKINDS];
     std::vector<hipEvent_t> ev_free;
     std::vector<Pending> pending;
-    hipGraphExec_t g_mid = nullptr, g_last = nullptr;
+    hipGraphExec_t g_mid = nullptr, g_last = nullptr, g_last_sync = nullptr;
+    // synchronised momenta stored by the last step of a call (pi, pi_u pairs,
+    // the dealiased stepper's ghost planes) and the context's state: 0 every
+    // member synchronised, 1 every member ahead by the look-ahead kick, 2 some
+    // may be (a call with the adaptive rule: members at their output time end
+    // synchronised)
+    void* Ps = nullptr;
+    void* PUs = nullptr;
+    void* Gps = nullptr;
+    int ahead = 0;
+    double t_out = 1e300, dt_cap = 1e300;   // targets of the adaptive rule (psc_set_output_times)
     std::unique_ptr<AdjWork> adj;
     double dev_bytes = 0.0;
     int shape = 0;   // index into kShapes
@@ -196,6 +206,8 @@ inline StepConsts consts_of(const psc_ctx* c) {
     s.expansion_static = c->cfg.expansion == PSC_EXPANSION_STATIC;
     s.nflds = c->cfg.nflds;
     s.NV = c->padeng ? 8.0 * (double)c->N3 : (double)c->N3;   // lattice sites V is averaged over
+    s.dt_rule = c->cfg.dt_rule;
+    s.kmax = c->cfg.kIR * std::sqrt(3.0) * c->N / 2.0;
     return s;
 }
 
@@ -240,9 +252,14 @@ inline int drain_timing(psc_ctx* c) {
     return PSC_OK;
 }
 
+// Kinds of a recorded step (Engine::record_step).
+enum { kStepMid = 0, kStepLastAhead = 1, kStepLastSync = 2 };
+
 struct EngineBase {
     virtual ~EngineBase() = default;
+    // nsteps steps, ending one look-ahead kick ahead (psc_step) or synchronised
     virtual int step(psc_ctx* c, int nsteps) = 0;
+    virtual int step_synced(psc_ctx* c, int nsteps) = 0;
     // Real-space IFFT of src (Pf or Pp of one member) into W[member] as
     // normalised (phi, chi), with diagnostic sums into c->sums[0..5].
     virtual int real_space(psc_ctx* c, int member, const void* src) = 0;

@@ -19,6 +19,23 @@ struct Engine final : EngineBase {
     using T = typename G::T;
     static constexpr int N = G::N;
     static int nV_blocks(const psc_ctx*) { return G::BLOCKS_COL; }
+    // explicit kick coefficients (the adjoint's replays); prologue: the first
+    // kick at a_start only
+    static KickArgs kick_args(double c1, double c2, bool prologue = false) {
+        KickArgs ka{};
+        ka.c1 = c1;
+        ka.c2 = c2;
+        ka.prologue = prologue ? 1 : 0;
+        return ka;
+    }
+    // the fields of a scalar update that begins substep q of the step
+    static void set_begin_weights(ScalarOp& op, int q, const std::vector<double>& w) {
+        const int S = (int)w.size();
+        op.begin = 1;
+        op.w_begin = w[q];
+        op.begin_last = q == S - 1;
+        op.w_after = w[(q + 1) % S];
+    }
     // copy of n complex elements as a kernel (captured graphs hold kernels only)
     static int copy(psc_ctx* c, void* dst, const void* src, size_t n) {
         const unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 8192);
@@ -77,7 +94,9 @@ struct Engine final : EngineBase {
                                c->partV + (size_t)m0 * G::BLOCKS_COL, P(c->Pf) + off);
         });
     }
-    int launch_z_kick(psc_ctx* c, bool reduce, double c1, double c2, int use_a_start, int m0 = 0, int n = -1) {
+    // z kick of members m0 .. m0 + n - 1; SYNC_OUT also stores the
+    // synchronised momenta (the last substep of a call that ends ahead)
+    int launch_z_kick(psc_ctx* c, bool reduce, const KickArgs& ka, bool sync_out, int m0 = 0, int n = -1) {
         if (n < 0) n = c->B;
         const size_t off = (size_t)m0 * G::MS;
         double* pkg = c->partKG + (size_t)m0 * G::BLOCKS_ROW * 2;
@@ -85,25 +104,24 @@ struct Engine final : EngineBase {
         dim3 grid(G::BLOCKS_ROW, n), block(G::THREADS_ROW);
         const StepConsts sc = consts_of(c);
         const bool st = streams(n);
-        if (reduce)
-            return timed(c, K_Z_FWD_KICK, 4 * X, [&] {
-                if (st)
-                    hipLaunchKernelGGL((k_z_kick<G, true, false, true>), grid, block, 0, c->stream, P(c->W) + off,
-                                       P(c->Pf) + off, P(c->Pp) + off, c->ms + m0, P(c->tw), pkg, c1, c2,
-                                       use_a_start, sc, nullptr);
-                else
-                    hipLaunchKernelGGL((k_z_kick<G, true>), grid, block, 0, c->stream, P(c->W) + off,
-                                       P(c->Pf) + off, P(c->Pp) + off, c->ms + m0, P(c->tw), pkg, c1, c2,
-                                       use_a_start, sc, nullptr);
-            });
-        return timed(c, K_Z_FWD_KICK_NR, 4 * X, [&] {
-            if (st)
-                hipLaunchKernelGGL((k_z_kick<G, false, false, true>), grid, block, 0, c->stream, P(c->W) + off,
-                                   P(c->Pf) + off, P(c->Pp) + off, c->ms + m0, P(c->tw), pkg, c1, c2, use_a_start,
-                                   sc, nullptr);
-            else
-                hipLaunchKernelGGL((k_z_kick<G, false>), grid, block, 0, c->stream, P(c->W) + off, P(c->Pf) + off,
-                                   P(c->Pp) + off, c->ms + m0, P(c->tw), pkg, c1, c2, use_a_start, sc, nullptr);
+        auto go = [&](auto red, auto stream, auto sync) {
+            hipLaunchKernelGGL((k_z_kick<G, decltype(red)::value, false, decltype(stream)::value, decltype(sync)::value>),
+                               grid, block, 0, c->stream, P(c->W) + off, P(c->Pf) + off, P(c->Pp) + off, c->ms + m0,
+                               P(c->tw), pkg, ka, sc, nullptr, P(c->Ps) + off);
+        };
+        using TT = std::true_type;
+        using FF = std::false_type;
+        return timed(c, reduce ? K_Z_FWD_KICK : K_Z_FWD_KICK_NR, (sync_out ? 5 : 4) * X, [&] {
+            if (!reduce) {   // the call's first kick
+                if (st) go(FF{}, TT{}, FF{});
+                else go(FF{}, FF{}, FF{});
+            } else if (sync_out) {
+                if (st) go(TT{}, TT{}, TT{});
+                else go(TT{}, FF{}, TT{});
+            } else {
+                if (st) go(TT{}, TT{}, FF{});
+                else go(TT{}, FF{}, FF{});
+            }
         });
     }
     // Reduce the substep partials to the scalar update.  Passes with more than
@@ -141,7 +159,7 @@ struct Engine final : EngineBase {
                                P(c->W), P(c->tw), astride, c->cfg.kIR);
         });
     }
-    int force_chain_gw(psc_ctx* c, bool drift, bool reduce, double c1, double c2, int use_a_start) {
+    int force_chain_gw(psc_ctx* c, bool drift, bool reduce, const KickArgs& ka, bool sync_out) {
         const size_t astride = (size_t)c->B * G::MS;
         const double X = (double)c->B * G::N3 * sizeof(cpx<T>);
         const StepConsts sc = consts_of(c);
@@ -168,23 +186,26 @@ struct Engine final : EngineBase {
         if (rc) return rc;
         if ((rc = launch_x_force(c))) return rc;
         if ((rc = launch_y<-1>(c, P(c->W), 4 * c->B))) return rc;
-        if (reduce)
-            return timed(c, K_Z_KICK_GW, 16 * X, [&] {
-                hipLaunchKernelGGL((k_z_kick_gw<G, true>), rgrid, rblock, 0, c->stream, P(c->W), P(c->Pf),
-                                   P(c->Pp), P(c->U), P(c->PU), c->ms, P(c->tw), c->partKG, c1, c2, use_a_start,
-                                   sc, astride);
-            });
-        return timed(c, K_Z_KICK_GW, 16 * X, [&] {
-            hipLaunchKernelGGL((k_z_kick_gw<G, false>), rgrid, rblock, 0, c->stream, P(c->W), P(c->Pf),
-                               P(c->Pp), P(c->U), P(c->PU), c->ms, P(c->tw), c->partKG, c1, c2, use_a_start, sc,
-                               astride);
+        return timed(c, K_Z_KICK_GW, (sync_out ? 20 : 16) * X, [&] {
+            if (reduce && sync_out)
+                hipLaunchKernelGGL((k_z_kick_gw<G, true, false, true>), rgrid, rblock, 0, c->stream, P(c->W), P(c->Pf),
+                                   P(c->Pp), P(c->U), P(c->PU), c->ms, P(c->tw), c->partKG, ka, sc, astride, nullptr,
+                                   P(c->Ps), P(c->PUs));
+            else if (reduce)
+                hipLaunchKernelGGL((k_z_kick_gw<G, true>), rgrid, rblock, 0, c->stream, P(c->W), P(c->Pf), P(c->Pp),
+                                   P(c->U), P(c->PU), c->ms, P(c->tw), c->partKG, ka, sc, astride, nullptr, nullptr,
+                                   nullptr);
+            else
+                hipLaunchKernelGGL((k_z_kick_gw<G, false>), rgrid, rblock, 0, c->stream, P(c->W), P(c->Pf), P(c->Pp),
+                                   P(c->U), P(c->PU), c->ms, P(c->tw), c->partKG, ka, sc, astride, nullptr, nullptr,
+                                   nullptr);
         });
     }
 
     // One force evaluation chain; DRIFT applies the substep's drift first.
-    int force_chain(psc_ctx* c, bool drift, bool reduce, double c1, double c2, int use_a_start) {
-        if (c->padeng) return c->padeng->force_chain(c, drift, reduce, c1, c2, use_a_start);
-        if (c->cfg.with_gw) return force_chain_gw(c, drift, reduce, c1, c2, use_a_start);
+    int force_chain(psc_ctx* c, bool drift, bool reduce, const KickArgs& ka, bool sync_out) {
+        if (c->padeng) return c->padeng->force_chain(c, drift, reduce, ka, sync_out);
+        if (c->cfg.with_gw) return force_chain_gw(c, drift, reduce, ka, sync_out);
         // An ensemble runs the chain on groups of members whose arrays fit the
         // memory-side cache together, so that each pass of a group reads what
         // the group's previous pass has just written.
@@ -199,27 +220,36 @@ struct Engine final : EngineBase {
             if ((rc = launch_y<+1>(c, P(c->W) + off, n))) return rc;
             if ((rc = launch_x_force(c, m0, n))) return rc;
             if ((rc = launch_y<-1>(c, P(c->W) + off, n))) return rc;
-            if ((rc = launch_z_kick(c, reduce, c1, c2, use_a_start, m0, n))) return rc;
+            if ((rc = launch_z_kick(c, reduce, ka, sync_out, m0, n))) return rc;
         }
         return PSC_OK;
     }
 
-    // All substeps of one step; `final` drops the trailing first-kick of the
-    // next substep so the momenta are synchronised on return.
-    int record_step(psc_ctx* c, bool final) {
+    // All substeps of one step.  kind: kStepMid (an interior step of a call),
+    // kStepLastAhead (the last step of a call: its last substep also applies
+    // the next step's first kick, the look-ahead, and stores the synchronised
+    // momenta), kStepLastSync (the last step of a call that ends synchronised).
+    // Kick coefficients come from the member states (set by the scalar update
+    // that begins each substep), so the recorded step does not depend on dt.
+    int record_step(psc_ctx* c, int kind) {
         const int S = (int)c->w.size();
-        const double dt = c->cfg.dt;
         for (int p = 0; p < S; ++p) {
-            const bool last = final && p == S - 1;
-            const double wn = c->w[(p + 1) % S];
-            int rc = force_chain(c, true, true, c->w[p] * dt / 2.0, last ? 0.0 : wn * dt / 2.0, 0);
+            const bool last = p == S - 1;
+            KickArgs ka{};
+            ka.from_state = 1;
+            ka.no_lookahead = (kind == kStepLastSync && last) ? 1 : 0;
+            int rc = force_chain(c, true, true, ka, kind == kStepLastAhead && last);
             if (rc) return rc;
             ScalarOp op{};
             op.finish = 1;
-            op.last_of_step = p == S - 1;
-            op.begin = last ? 0 : 1;
+            op.last_of_step = last;
+            op.begin = (kind == kStepLastSync && last) ? 0 : 1;
             op.w_finish = c->w[p];
-            op.w_begin = wn;
+            const int q = (p + 1) % S;   // the substep begun next
+            op.w_begin = c->w[q];
+            op.begin_last = q == S - 1;
+            op.w_after = c->w[(q + 1) % S];
+            op.end_call = last && kind != kStepMid ? (kind == kStepLastAhead ? 1 : 2) : 0;
             op.nV = c->padeng ? c->padeng->nV() : nV_blocks(c);
             op.nKG = c->padeng ? c->padeng->nKG() : G::BLOCKS_ROW;
             if ((rc = launch_scalars(c, op))) return rc;
@@ -227,10 +257,10 @@ struct Engine final : EngineBase {
         return PSC_OK;
     }
 
-    int build_graph(psc_ctx* c, bool final, hipGraphExec_t* out) {
+    int build_graph(psc_ctx* c, int kind, hipGraphExec_t* out) {
         hipGraph_t g;
         HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        int rc = record_step(c, final);
+        int rc = record_step(c, kind);
         hipError_t e = hipStreamEndCapture(c->stream, &g);
         if (rc) return rc;
         if (e != hipSuccess) return c->fail(PSC_ERR_DEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
@@ -243,27 +273,52 @@ struct Engine final : EngineBase {
         return PSC_OK;
     }
 
-    int step(psc_ctx* c, int nsteps) override {
-        if (nsteps <= 0) return PSC_OK;
+    // The call's start: members that are not ahead begin substep 0 (dt from
+    // the rule) and take its first kick, from a force at the start point.
+    // The force is the one the previous call's last substep evaluated at the
+    // same fields and a (evolution.cpp:1144-1152), so a call that continues
+    // from an ahead state (c->ahead == 1) skips this entirely.
+    int prologue(psc_ctx* c) {
         ScalarOp op{};
         op.begin = 1;
+        op.prologue = 1;
+        const int S = (int)c->w.size();
         op.w_begin = c->w[0];
+        op.begin_last = S == 1;
+        op.w_after = c->w[1 % S];
         op.nV = c->padeng ? c->padeng->nV() : nV_blocks(c);
         op.nKG = c->padeng ? c->padeng->nKG() : G::BLOCKS_ROW;
         int rc = launch_scalars(c, op);
         if (rc) return rc;
-        if ((rc = force_chain(c, false, false, c->w[0] * c->cfg.dt / 2.0, 0.0, 1))) return rc;
+        KickArgs ka{};
+        ka.from_state = 1;
+        ka.prologue = 1;
+        return force_chain(c, false, false, ka, false);
+    }
+
+    int step(psc_ctx* c, int nsteps) override { return run_steps(c, nsteps, false); }
+    int step_synced(psc_ctx* c, int nsteps) override { return run_steps(c, nsteps, true); }
+
+    // nsteps steps; the call ends ahead (the look-ahead kick applied, the
+    // synchronised momenta stored) or, with end_synced, synchronised.
+    int run_steps(psc_ctx* c, int nsteps, bool end_synced) {
+        if (nsteps <= 0) return PSC_OK;
+        int rc;
+        if (c->ahead != 1 && (rc = prologue(c))) return rc;
+        const int last = end_synced ? kStepLastSync : kStepLastAhead;
         const bool graph = c->cfg.use_graph && !c->timing;
         if (graph) {
-            if (!c->g_mid && (rc = build_graph(c, false, &c->g_mid))) return rc;
-            if (!c->g_last && (rc = build_graph(c, true, &c->g_last))) return rc;
+            hipGraphExec_t& gl = end_synced ? c->g_last_sync : c->g_last;
+            if (nsteps > 1 && !c->g_mid && (rc = build_graph(c, kStepMid, &c->g_mid))) return rc;
+            if (!gl && (rc = build_graph(c, last, &gl))) return rc;
             for (int s = 0; s < nsteps - 1; ++s) HIPCHK(c, hipGraphLaunch(c->g_mid, c->stream));
-            HIPCHK(c, hipGraphLaunch(c->g_last, c->stream));
+            HIPCHK(c, hipGraphLaunch(gl, c->stream));
         } else {
             for (int s = 0; s < nsteps; ++s)
-                if ((rc = record_step(c, s == nsteps - 1))) return rc;
+                if ((rc = record_step(c, s == nsteps - 1 ? last : kStepMid))) return rc;
         }
         HIPCHK(c, hipGetLastError());   // a failed launch is reported here, not by a later call
+        c->ahead = end_synced ? 0 : (c->cfg.dt_rule == PSC_DT_FIXED ? 1 : 2);
         return c->timing ? drain_timing(c) : PSC_OK;
     }
 
@@ -445,8 +500,8 @@ struct Engine final : EngineBase {
         int rc = copy(c, w.p[0], c->Pp, G::MS);
         if (rc) return rc;
         ScalarOp op{};
-        op.begin = 1;
-        op.w_begin = c->w[0];
+        set_begin_weights(op, 0, c->w);
+        op.prologue = 1;
         op.nV = G::BLOCKS_COL;
         op.nKG = G::BLOCKS_ROW;
         if ((rc = launch_scalars(c, op, w.rms))) return rc;
@@ -459,7 +514,7 @@ struct Engine final : EngineBase {
                            P(c->Pf), P(w.r[0]));
         if ((rc = launch_y<-1>(c, P(c->W), 1))) return rc;
         hipLaunchKernelGGL((k_z_kick<G, false>), rgrid, rblock, 0, c->stream, P(c->W), P(c->Pf), P(c->Pp), c->ms,
-                           P(c->tw), c->partKG, c->w[0] * dt / 2.0, 0.0, 1, sc, nullptr);
+                           P(c->tw), c->partKG, kick_args(c->w[0] * dt / 2.0, 0.0, true), sc, nullptr, nullptr);
         for (int j = 0; j < steps * S; ++j) {
             const int p = j % S;
             const bool last = j == steps * S - 1;
@@ -471,14 +526,13 @@ struct Engine final : EngineBase {
                                c->partV, P(c->Pf), P(w.r[j + 1]));
             if ((rc = launch_y<-1>(c, P(c->W), 1))) return rc;
             hipLaunchKernelGGL((k_z_kick<G, true, true>), rgrid, rblock, 0, c->stream, P(c->W), P(c->Pf),
-                               P(c->Pp), c->ms, P(c->tw), c->partKG, c->w[p] * dt / 2.0, last ? 0.0 : wn * dt / 2.0,
-                               0, sc, P(w.p[j + 1]));
+                               P(c->Pp), c->ms, P(c->tw), c->partKG,
+                               kick_args(c->w[p] * dt / 2.0, last ? 0.0 : wn * dt / 2.0), sc, P(w.p[j + 1]), nullptr);
             ScalarOp o{};
             o.finish = 1;
             o.last_of_step = p == S - 1;
-            o.begin = last ? 0 : 1;
+            if (!last) set_begin_weights(o, (p + 1) % S, c->w);
             o.w_finish = c->w[p];
-            o.w_begin = wn;
             o.nV = G::BLOCKS_COL;
             o.nKG = G::BLOCKS_ROW;
             if ((rc = launch_scalars(c, o, w.rms + j + 1))) return rc;
@@ -499,8 +553,8 @@ struct Engine final : EngineBase {
         int rc = copy(c, w.p[0], c->Pp, as);
         if (rc || (rc = copy(c, w.u[0], c->U, 3 * as))) return rc;
         ScalarOp op{};
-        op.begin = 1;
-        op.w_begin = c->w[0];
+        set_begin_weights(op, 0, c->w);
+        op.prologue = 1;
         op.nV = G::BLOCKS_COL;
         op.nKG = G::BLOCKS_ROW;
         if ((rc = launch_scalars(c, op, w.rms))) return rc;
@@ -519,8 +573,8 @@ struct Engine final : EngineBase {
                            P(c->U), P(c->PU), c->ms, P(c->tw), as, kIR, nullptr, nullptr, nullptr, nullptr);
         if ((rc = middle(0))) return rc;
         hipLaunchKernelGGL((k_z_kick_gw<G, false>), rgrid, rblock, 0, c->stream, P(c->W), P(c->Pf), P(c->Pp),
-                           P(c->U), P(c->PU), c->ms, P(c->tw), c->partKG, c->w[0] * dt / 2.0, 0.0, 1, sc, as,
-                           nullptr);
+                           P(c->U), P(c->PU), c->ms, P(c->tw), c->partKG, kick_args(c->w[0] * dt / 2.0, 0.0, true), sc,
+                           as, nullptr, nullptr, nullptr);
         for (int j = 0; j < steps * S; ++j) {
             const int p = j % S;
             const bool last = j == steps * S - 1;
@@ -530,14 +584,14 @@ struct Engine final : EngineBase {
                                P(w.u[j + 1]), P(w.puh[j]));
             if ((rc = middle(j + 1))) return rc;
             hipLaunchKernelGGL((k_z_kick_gw<G, true, true>), rgrid, rblock, 0, c->stream, P(c->W), P(c->Pf),
-                               P(c->Pp), P(c->U), P(c->PU), c->ms, P(c->tw), c->partKG, c->w[p] * dt / 2.0,
-                               last ? 0.0 : wn * dt / 2.0, 0, sc, as, P(w.p[j + 1]));
+                               P(c->Pp), P(c->U), P(c->PU), c->ms, P(c->tw), c->partKG,
+                               kick_args(c->w[p] * dt / 2.0, last ? 0.0 : wn * dt / 2.0), sc, as, P(w.p[j + 1]),
+                               nullptr, nullptr);
             ScalarOp o{};
             o.finish = 1;
             o.last_of_step = p == S - 1;
-            o.begin = last ? 0 : 1;
+            if (!last) set_begin_weights(o, (p + 1) % S, c->w);
             o.w_finish = c->w[p];
-            o.w_begin = wn;
             o.nV = G::BLOCKS_COL;
             o.nKG = G::BLOCKS_ROW;
             if ((rc = launch_scalars(c, o, w.rms + j + 1))) return rc;
@@ -702,7 +756,7 @@ struct PadEngine final : PadBase {
         });
     }
 
-    int force_chain(psc_ctx* c, bool drift, bool reduce, double c1, double c2, int use_a_start) override {
+    int force_chain(psc_ctx* c, bool drift, bool reduce, const KickArgs& ka, bool sync_out) override {
         int rc;
         if (drift && (rc = timed(c, K_PAD_DRIFT, 3 * XN, [&] {
                 hipLaunchKernelGGL((k_drift_pad<T, N>), dim3((unsigned)((size_t)N * N * N / 256)), dim3(256), 0,
@@ -721,13 +775,17 @@ struct PadEngine final : PadBase {
         });
         if (rc) return rc;
         const StepConsts sc = consts_of(c);
-        return timed(c, K_PAD_KICK, 4 * XN, [&] {
-            if (reduce)
+        return timed(c, K_PAD_KICK, (sync_out ? 5 : 4) * XN, [&] {
+            if (reduce && sync_out)
+                hipLaunchKernelGGL((k_kick_pad<GP, true, true>), dim3(nKG()), dim3(256), 0, c->stream, P(c->Rp),
+                                   P(c->Pf), P(c->Pp), P(c->Gf), P(c->Gp), c->ms, c->partKG, ka, sc, P(c->Ps),
+                                   P(c->Gps));
+            else if (reduce)
                 hipLaunchKernelGGL((k_kick_pad<GP, true>), dim3(nKG()), dim3(256), 0, c->stream, P(c->Rp), P(c->Pf),
-                                   P(c->Pp), P(c->Gf), P(c->Gp), c->ms, c->partKG, c1, c2, use_a_start, sc);
+                                   P(c->Pp), P(c->Gf), P(c->Gp), c->ms, c->partKG, ka, sc, nullptr, nullptr);
             else
                 hipLaunchKernelGGL((k_kick_pad<GP, false>), dim3(nKG()), dim3(256), 0, c->stream, P(c->Rp), P(c->Pf),
-                                   P(c->Pp), P(c->Gf), P(c->Gp), c->ms, c->partKG, c1, c2, use_a_start, sc);
+                                   P(c->Pp), P(c->Gf), P(c->Gp), c->ms, c->partKG, ka, sc, nullptr, nullptr);
         });
     }
 
@@ -795,6 +853,7 @@ int Engine<G, GY>::adjoint(psc_ctx* c, const psc_adjoint_spec* sp, double* out, 
         return PSC_OK;
     };
     auto restore = [&](int sg) -> int {
+        c->ahead = 0;   // checkpoints hold synchronised states
         HIPCHK(c, hipMemcpyAsync(c->Pf, w->ck[2 * sg], arr, hipMemcpyDeviceToDevice, c->stream));
         HIPCHK(c, hipMemcpyAsync(c->Pp, w->ck[2 * sg + 1], arr, hipMemcpyDeviceToDevice, c->stream));
         HIPCHK(c, hipMemcpyAsync(c->ms, w->ck_ms + sg, sizeof(MemberState), hipMemcpyDeviceToDevice, c->stream));
@@ -810,7 +869,7 @@ int Engine<G, GY>::adjoint(psc_ctx* c, const psc_adjoint_spec* sp, double* out, 
     for (int sg = 0; sg < nseg; ++sg) {
         if ((rc = save(sg))) return rc;
         const int steps = std::min(K, M - sg * K);
-        if ((rc = sg == nseg - 1 ? run_segment(c, steps, *w, false) : step(c, steps))) return rc;
+        if ((rc = sg == nseg - 1 ? run_segment(c, steps, *w, false) : step_synced(c, steps))) return rc;
     }
     HIPCHK(c, hipEventRecord(ev[1], c->stream));
     // the objective and its adjoint at the end

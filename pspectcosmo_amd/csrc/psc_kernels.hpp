@@ -185,12 +185,14 @@ k_x_force(cpx<typename G::T>* __restrict__ W, const MemberState* __restrict__ ms
 #pragma unroll
     for (int m = 0; m < G::E; ++m)
         e[m] = kept(m) ? W[base + (size_t)G::TPL * m * stride] : cpx<T>{(T)0, (T)0};
-    line_fft<T, N, typename G::Rad, +1, typename G::LayCol>(e, t, c, lds, tw);
-    const T inv = (T)(1.0 / (double)G::N3);
+    // the member's couplings and the homogeneous values (the k = 0 mode left
+    // out of the inverse transform) are read before the transform, so that
+    // their latency is hidden behind it
     const Couplings<T> cp = couplings<T>(ms[member]);
     const int nf = ms[member].nflds;
-    // homogeneous values from the k = 0 mode left out of the inverse transform
     const cpx<T> dc = Pf[(size_t)member * G::MS];
+    line_fft<T, N, typename G::Rad, +1, typename G::LayCol>(e, t, c, lds, tw);
+    const T inv = (T)(1.0 / (double)G::N3);
     const T mean[2] = {(T)((double)dc.x / (double)G::N3), nf > 1 ? (T)((double)dc.y / (double)G::N3) : (T)0};
     double v[1] = {0.0};
 #pragma unroll
@@ -240,17 +242,18 @@ __device__ __forceinline__ void add_mean_force(cpx<T>& vk, const cpx<T>& f0, con
 // unnormalised spectrum of V'.  c1/c2 are w dt / 2 of the two kicks applied
 // with this force; REDUCE accumulates sum |pi|^2 after the first kick and
 // sum n^2 |f|^2 (kinetic_energy_k / gradient_energy_k, evolution.cpp:923-1035).
-template <class G, bool REDUCE, bool REC = false, bool STREAM = false>
+template <class G, bool REDUCE, bool REC = false, bool STREAM = false, bool SYNC_OUT = false>
 __global__ void __launch_bounds__((G::THREADS_ROW))
 k_z_kick(const cpx<typename G::T>* __restrict__ W, const cpx<typename G::T>* __restrict__ Pf, cpx<typename G::T>* __restrict__ Pp,
          const MemberState* __restrict__ ms, const cpx<typename G::T>* __restrict__ tw,
-         double* __restrict__ partKG, double c1d, double c2d, int use_a_start, StepConsts sc,
-         cpx<typename G::T>* __restrict__ rec_p = nullptr) {
+         double* __restrict__ partKG, KickArgs ka, StepConsts sc,
+         cpx<typename G::T>* __restrict__ rec_p = nullptr, cpx<typename G::T>* __restrict__ Ps = nullptr) {
     using T = typename G::T;
     constexpr int N = G::N;
     __shared__ cpx<T> lds[G::LayRow::size(G::L)];
     const int t = threadIdx.x % G::TPL, c = threadIdx.x / G::TPL;
     const int member = blockIdx.y;
+    if (ka.prologue && ms[member].ahead) return;   // the call's first kick was applied as a look-ahead
     const size_t line = (size_t)blockIdx.x * G::L + c;
     const size_t base = (size_t)member * G::MS + (line / N) * G::PS + (line % N) * N + t;
     cpx<T> e[G::E];
@@ -259,9 +262,11 @@ k_z_kick(const cpx<typename G::T>* __restrict__ W, const cpx<typename G::T>* __r
         // there, 1 % slower in FP32, where more of the arrays fit the memory-side cache)
         e[m] = (STREAM && sizeof(T) == 8) ? load_stream(W + base + G::TPL * m) : W[base + G::TPL * m];
     line_fft<T, N, typename G::Rad, -1, typename G::LayRow>(e, t, c, lds, tw);
-    const double a = use_a_start ? ms[member].a_start : ms[member].a;
+    const double a = ka.prologue ? ms[member].a_start : ms[member].a;
     const T lapc = (T)(pow(a, 1 + sc.alpha) * sc.kIR * sc.kIR);
     const T vc = (T)pow(a, 3 + sc.alpha);
+    double c1d, c2d;
+    kick_coefs(ka, ms[member], c1d, c2d);
     const T c1 = (T)c1d, c2 = (T)c2d;
     const int ix = (int)(line / N), iy = (int)(line % N);
     const int px = signed_index(ix, N), py = signed_index(iy, N);
@@ -280,6 +285,7 @@ k_z_kick(const cpx<typename G::T>* __restrict__ W, const cpx<typename G::T>* __r
         p.x += c1 * F.x;
         p.y += c1 * F.y;
         if constexpr (REC) rec_p[base + G::TPL * m] = p;  // adjoint recording: synchronised pi
+        if constexpr (SYNC_OUT) Ps[base + G::TPL * m] = p;  // synchronised pi at the end of a call
         if constexpr (REDUCE) {
             const double px_ = p.x, py_ = p.y, fx = f.x, fy = f.y;
             red[0] += px_ * px_ + py_ * py_;
@@ -308,8 +314,14 @@ struct ScalarOp {
     int finish;        // reduce partials and finish a substep
     int last_of_step;  // the finished substep closes a step
     int begin;         // begin a substep with weight w_begin
-    int pad;
+    int prologue;      // the begin starts a call: members already ahead have begun
+                       // it at the end of the last call and are left alone
     double w_finish, w_begin;
+    double w_after;    // weight of the substep after the one begun (of the next
+                       // step, w_0, when the one begun is the last)
+    int begin_last;    // the substep begun is the last of its step
+    int end_call;      // the finished substep ends a call: 1 with the look-ahead
+                       // kick applied (ahead when its coefficient is nonzero), 2 without
     int nV, nKG;       // partial counts per member
 };
 
@@ -393,20 +405,31 @@ k_scalars(MemberState* __restrict__ ms, const double* __restrict__ partV,
             double add = 1.0 / 3.0 * sc.f2m * ipow(a, 1 + 2 * al) * ((al - 2) * Kn + al * Gn + (al + 1) * Vn);
             if (sc.expansion_static) add = 0.0;
             s.add = add;
-            s.b = s.b_half + op.w_finish * sc.dt / 2.0 * add;
+            s.b = s.b_half + op.w_finish * s.dt / 2.0 * add;
             if (op.last_of_step) {
                 s.ad = sc.expansion_static ? 0.0 : s.b;
-                s.t += sc.dt;
+                s.t += s.dt;
+                if (s.dt != 0.0) s.steps += 1.0;
+                s.dt = s.dt_next;
             }
+            if (op.end_call) s.ahead = (op.end_call == 1 && s.kick2 != 0.0) ? 1 : 0;
         }
     }
-    if (op.begin && threadIdx.x == 0) {
-        const double w = op.w_begin, dt = sc.dt;
+    if (op.begin && threadIdx.x == 0 && !(op.prologue && s.ahead)) {
+        if (op.prologue) s.dt = step_dt(s, s.a, s.t, sc);   // the first step of a call
+        const double w = op.w_begin, dt = s.dt;
         s.b_half = s.b + w * dt / 2.0 * s.add;
         s.a_start = s.a;
         s.a_half = sc.expansion_static ? 1.0 : s.a + s.b_half * w * dt / 2.0;
         s.c_drift = w * dt * ipow(s.a_half, sc.alpha - 3);
         s.a = sc.expansion_static ? 1.0 : s.a_half + s.b_half * w * dt / 2.0;
+        s.kick1 = w * dt / 2.0;
+        if (op.begin_last) {   // the step ends with this substep: dt of the next step
+            s.dt_next = step_dt(s, s.a, s.t + dt, sc);
+            s.kick2 = op.w_after * s.dt_next / 2.0;
+        } else {
+            s.kick2 = op.w_after * dt / 2.0;
+        }
     }
     if (threadIdx.x == 0) {
         ms[member] = s;
@@ -617,6 +640,28 @@ k_prereduce(const double* __restrict__ inV, int nV, double* __restrict__ outV, c
         if (nV > chunks) prereduce_chunk<1>(inV, nV, outV, chunks, member, blockIdx.x);
     } else if (nKG > chunks) {
         prereduce_chunk<2>(inKG, nKG, outKG, chunks, member, blockIdx.x - chunks);
+    }
+}
+
+// Restores the synchronised state of the members a call left ahead by one
+// look-ahead kick: the momenta from the sync buffer (k_sync_copy, `arrays`
+// packed arrays per member, member stride `mstride`), then the scale factor
+// a = a_start and the flag (k_sync_state, after the copies).
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_sync_copy(cpx<T>* __restrict__ dst, const cpx<T>* __restrict__ src, const MemberState* __restrict__ ms,
+            size_t n, size_t mstride, int arrays, int members) {
+    const int member = blockIdx.y % members, arr = blockIdx.y / members;
+    if (!ms[member].ahead) return;
+    const size_t off = ((size_t)arr * members + member) * mstride;
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256)
+        dst[off + i] = src[off + i];
+}
+static __global__ void k_sync_state(MemberState* __restrict__ ms, int members) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m < members && ms[m].ahead) {
+        ms[m].a = ms[m].a_start;
+        ms[m].ahead = 0;
     }
 }
 
@@ -913,24 +958,28 @@ k_x_source(cpx<typename G::T>* __restrict__ W, const cpx<typename G::T>* __restr
 
 // E (GW): z-axis forward of V' and of the source, field kick as k_z_kick and
 // the GW kick of the three packed u arrays.
-template <class G, bool REDUCE, bool REC = false>
+template <class G, bool REDUCE, bool REC = false, bool SYNC_OUT = false>
 __global__ void __launch_bounds__((G::THREADS_ROW))
 k_z_kick_gw(const cpx<typename G::T>* __restrict__ W, const cpx<typename G::T>* __restrict__ Pf, cpx<typename G::T>* __restrict__ Pp,
             const cpx<typename G::T>* __restrict__ U, cpx<typename G::T>* __restrict__ PU, const MemberState* __restrict__ ms,
-            const cpx<typename G::T>* __restrict__ tw, double* __restrict__ partKG, double c1d, double c2d,
-            int use_a_start, StepConsts sc, size_t astride, cpx<typename G::T>* __restrict__ rec_p = nullptr) {
+            const cpx<typename G::T>* __restrict__ tw, double* __restrict__ partKG, KickArgs ka,
+            StepConsts sc, size_t astride, cpx<typename G::T>* __restrict__ rec_p = nullptr,
+            cpx<typename G::T>* __restrict__ Ps = nullptr, cpx<typename G::T>* __restrict__ PUs = nullptr) {
     using T = typename G::T;
     constexpr int N = G::N;
     __shared__ cpx<T> lds[G::LayRow::size(G::L)];
     const int t = threadIdx.x % G::TPL, c = threadIdx.x / G::TPL;
     const int member = blockIdx.y;
+    if (ka.prologue && ms[member].ahead) return;   // the call's first kick was applied as a look-ahead
     const size_t line = (size_t)blockIdx.x * G::L + c;
     const size_t base = (size_t)member * G::MS + (line / N) * G::PS + (line % N) * N + t;
-    const double a = use_a_start ? ms[member].a_start : ms[member].a;
+    const double a = ka.prologue ? ms[member].a_start : ms[member].a;
     const T norm1 = (T)pow(a, 1 + sc.alpha);
     const T lapc = (T)(pow(a, 1 + sc.alpha) * sc.kIR * sc.kIR);
     const T vc = (T)pow(a, 3 + sc.alpha);
     const T k2c = (T)(sc.kIR * sc.kIR);
+    double c1d, c2d;
+    kick_coefs(ka, ms[member], c1d, c2d);
     const T c1 = (T)c1d, c2 = (T)c2d;
     const int ix = (int)(line / N), iy = (int)(line % N);
     const int px = signed_index(ix, N), py = signed_index(iy, N);
@@ -960,6 +1009,7 @@ k_z_kick_gw(const cpx<typename G::T>* __restrict__ W, const cpx<typename G::T>* 
         p.x += c1 * F.x;
         p.y += c1 * F.y;
         if constexpr (REC) rec_p[base + G::TPL * m] = p;  // adjoint recording: synchronised pi
+        if constexpr (SYNC_OUT) Ps[base + G::TPL * m] = p;  // synchronised pi at the end of a call
         if constexpr (REDUCE) {
             const double px_ = p.x, py_ = p.y, fx = f.x, fy = f.y;
             red[0] += px_ * px_ + py_ * py_;
@@ -994,6 +1044,7 @@ k_z_kick_gw(const cpx<typename G::T>* __restrict__ W, const cpx<typename G::T>* 
             const cpx<T> F = {norm1 * (-k2 * u.x + (T)2 * e[m].x), norm1 * (-k2 * u.y + (T)2 * e[m].y)};
             p.x += c1 * F.x;
             p.y += c1 * F.y;
+            if constexpr (SYNC_OUT) PUs[idx] = p;
             p.x += c2 * F.x;
             p.y += c2 * F.y;
             PU[idx] = p;

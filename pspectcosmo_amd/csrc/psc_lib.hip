@@ -65,10 +65,57 @@ void integrator_weights(const psc_config& c, std::vector<double>& w) {
 int maxnumbins(int N) { return (int)(std::sqrt(3.0) * (N / 2)) + 2; }
 constexpr int kMaxLogBins = 1024;
 
-int check_member(psc_ctx* c, int member) {
+int check_member_only(psc_ctx* c, int member) {
     if (!c) return PSC_ERR_ARG;
     if (member < 0 || member >= c->B) return c->fail(PSC_ERR_ARG, "member index out of range");
     return PSC_OK;
+}
+
+// Restores the synchronised momenta and scale factor of every member a call
+// left one look-ahead kick ahead (see psc_step): the copies from the sync
+// buffers, then the states.  Every function that reads or changes the state
+// runs it first.
+int ensure_synced(psc_ctx* c) {
+    if (!c->ahead) return PSC_OK;
+    auto copy = [&](void* dst, const void* src, size_t n, size_t mstride, int arrays, int members) {
+        const dim3 grid((unsigned)std::min<size_t>((n + 255) / 256, 4096), (unsigned)(arrays * members));
+        if (c->cfg.precision == PSC_FP32)
+            hipLaunchKernelGGL(k_sync_copy<float>, grid, dim3(256), 0, c->stream, static_cast<cpx<float>*>(dst),
+                               static_cast<const cpx<float>*>(src), c->ms, n, mstride, arrays, members);
+        else
+            hipLaunchKernelGGL(k_sync_copy<double>, grid, dim3(256), 0, c->stream, static_cast<cpx<double>*>(dst),
+                               static_cast<const cpx<double>*>(src), c->ms, n, mstride, arrays, members);
+    };
+    copy(c->Pp, c->Ps, c->MS, c->MS, 1, c->B);
+    if (c->cfg.with_gw) copy(c->PU, c->PUs, c->MS, c->MS, 3, c->B);
+    if (c->padeng) copy(c->Gp, c->Gps, (size_t)2 * c->N * c->N, 0, 1, 1);
+    hipLaunchKernelGGL(k_sync_state, dim3((c->B + 63) / 64), dim3(64), 0, c->stream, c->ms, c->B);
+    HIPCHK(c, hipGetLastError());
+    c->ahead = 0;
+    return PSC_OK;
+}
+
+// A member index check for functions that read or change the state: the
+// state is synchronised first.
+int check_member(psc_ctx* c, int member) {
+    int rc = check_member_only(c, member);
+    if (rc) return rc;
+    return ensure_synced(c);
+}
+
+// The state of newly loaded data (t, a, da/dt; d2a/dt2 follows from it).
+MemberState fresh_state(const psc_ctx* c, int member, double t, double a, double ad) {
+    MemberState s;
+    std::memset(&s, 0, sizeof(s));
+    s.t = t;
+    s.a = s.a_start = s.a_half = a;
+    s.ad = s.b = (c->cfg.expansion == PSC_EXPANSION_STATIC) ? 0.0 : ad;
+    s.par = c->models[member];
+    s.nflds = c->cfg.nflds;
+    s.dt = s.dt_next = c->cfg.dt;
+    s.t_out = c->t_out;
+    s.dt_cap = c->dt_cap;
+    return s;
 }
 
 // Copy one member's MemberState to the host.
@@ -371,7 +418,8 @@ void tune_store(const std::string& key, const std::string& name, double ms) {
 void drop_graphs(psc_ctx* c) {
     if (c->g_mid) (void)hipGraphExecDestroy(c->g_mid);
     if (c->g_last) (void)hipGraphExecDestroy(c->g_last);
-    c->g_mid = c->g_last = nullptr;
+    if (c->g_last_sync) (void)hipGraphExecDestroy(c->g_last_sync);
+    c->g_mid = c->g_last = c->g_last_sync = nullptr;
     c->adj.reset();
 }
 
@@ -423,7 +471,7 @@ int psc_create(const psc_config* cfg, psc_ctx** out) {
         (cfg->precision != PSC_FP64 && cfg->precision != PSC_FP32) || !(cfg->kIR > 0) ||
         !(cfg->fStar > 0) || !(cfg->MPl > 0) || !std::isfinite(cfg->dt) ||
         (cfg->expansion != PSC_EXPANSION_SELF && cfg->expansion != PSC_EXPANSION_STATIC) ||
-        (cfg->pad != 0 && cfg->pad != 2))
+        (cfg->pad != 0 && cfg->pad != 2) || (cfg->dt_rule != PSC_DT_FIXED && cfg->dt_rule != PSC_DT_REFERENCE))
         return PSC_ERR_ARG;
     // the dealiased stepper: one member, no GW (the reference pads only the
     // scalar force), N_pad = 2N within the transform sizes
@@ -487,8 +535,8 @@ int psc_create(const psc_config* cfg, psc_ctx** out) {
                                    (size_t)128 * (maxnumbins(N) * 4 + kMaxLogBins));
     c->sums_len = (size_t)maxnumbins(N) * 4 + kMaxLogBins + 64;
     const bool gw = cfg->with_gw != 0;
-    if ((gw && (!alloc(&c->U, 3 * arr) || !alloc(&c->PU, 3 * arr))) ||
-        !alloc(&c->Pf, arr) || !alloc(&c->Pp, arr) || !alloc(&c->W, gw ? 4 * arr : arr) ||
+    if ((gw && (!alloc(&c->U, 3 * arr) || !alloc(&c->PU, 3 * arr) || !alloc(&c->PUs, 3 * arr))) ||
+        !alloc(&c->Pf, arr) || !alloc(&c->Pp, arr) || !alloc(&c->Ps, arr) || !alloc(&c->W, gw ? 4 * arr : arr) ||
         !alloc(&c->tw, (size_t)N * c->es) || !alloc((void**)&c->ms, sizeof(MemberState) * c->B) ||
         !alloc((void**)&c->partV, sizeof(double) * blocks_col * c->B) ||
         !alloc((void**)&c->partKG, sizeof(double) * 2 * blocks_row * c->B) ||
@@ -499,6 +547,7 @@ int psc_create(const psc_config* cfg, psc_ctx** out) {
         (c->padeng && (!alloc(&c->Wp, c->padeng->padded_elems() * c->es) ||
                        !alloc(&c->Rp, (size_t)(N + 1) * (N + 1) * (N + 1) * c->es) ||
                        !alloc(&c->Gf, (size_t)2 * N * N * c->es) || !alloc(&c->Gp, (size_t)2 * N * N * c->es) ||
+                       !alloc(&c->Gps, (size_t)2 * N * N * c->es) ||
                        !alloc(&c->dcp, c->es) || !alloc(&c->twp, (size_t)2 * N * c->es)))) {
         psc_destroy(c.release());
         return PSC_ERR_NOMEM;
@@ -532,6 +581,8 @@ int psc_create(const psc_config* cfg, psc_ctx** out) {
         s.a = s.a_start = s.a_half = 1.0;
         s.par = c->models[0];
         s.nflds = cfg->nflds;
+        s.dt = s.dt_next = cfg->dt;
+        s.t_out = s.dt_cap = 1e300;
     }
     // the fills finish before the context is handed out: they are on the
     // context's stream, ordered before anything the caller launches there
@@ -557,8 +608,8 @@ int psc_destroy(psc_ctx* c) {
         hipEventDestroy(p.b);
     }
     for (auto e : c->ev_free) hipEventDestroy(e);
-    void* ptrs[] = {c->Pf, c->Pp, c->W, c->tw, c->U, c->PU, c->ms, c->partV, c->partKG, c->partV2, c->partKG2,
-                    c->part, c->sums, c->Wp, c->Rp, c->Gf, c->Gp, c->dcp, c->twp};
+    void* ptrs[] = {c->Pf, c->Pp, c->Ps, c->W, c->tw, c->U, c->PU, c->PUs, c->ms, c->partV, c->partKG, c->partV2,
+                    c->partKG2, c->part, c->sums, c->Wp, c->Rp, c->Gf, c->Gp, c->Gps, c->dcp, c->twp};
     for (void* p : ptrs)
         if (p) hipFree(p);
     if (c->stream) hipStreamDestroy(c->stream);
@@ -591,13 +642,7 @@ int psc_load_ic_k(psc_ctx* c, int member, const double* f_k, const double* pi_k,
     pack_full(c->N, c->cfg.nflds, pi_k, full);
     if ((rc = upload(c, c->Pp, member, full))) return rc;
     if ((rc = reset_ghosts(c))) return rc;
-    MemberState s;
-    std::memset(&s, 0, sizeof(s));
-    s.t = 0.0;
-    s.a = s.a_start = s.a_half = a;
-    s.ad = s.b = (c->cfg.expansion == PSC_EXPANSION_STATIC) ? 0.0 : ad;
-    s.par = c->models[member];
-    s.nflds = c->cfg.nflds;
+    MemberState s = fresh_state(c, member, 0.0, a, ad);
     if ((rc = write_state(c, member, &s))) return rc;
     Diag d;
     if ((rc = diagnose(c, member, s, &d))) return rc;
@@ -675,12 +720,7 @@ int psc_generate_ic_device(psc_ctx* c, int member, const psc_ic_params* ic, doub
     HIPCHK(c, hipGetLastError());
     if ((rc = reset_ghosts(c))) return rc;
     // the member's state as psc_load_ic_k sets it
-    MemberState s;
-    std::memset(&s, 0, sizeof(s));
-    s.a = s.a_start = s.a_half = sc[0];
-    s.ad = s.b = (c->cfg.expansion == PSC_EXPANSION_STATIC) ? 0.0 : sc[1];
-    s.par = c->models[member];
-    s.nflds = c->cfg.nflds;
+    MemberState s = fresh_state(c, member, 0.0, sc[0], sc[1]);
     if ((rc = write_state(c, member, &s))) return rc;
     Diag d;
     if ((rc = diagnose(c, member, s, &d))) return rc;
@@ -737,13 +777,7 @@ int psc_load_fields(psc_ctx* c, int member, const double* f, const double* pi, d
         if ((rc = c->eng->to_spectrum(c, member, which == 0 ? c->Pf : c->Pp))) return rc;
     }
     if ((rc = reset_ghosts(c))) return rc;
-    MemberState s;
-    std::memset(&s, 0, sizeof(s));
-    s.t = t;
-    s.a = s.a_start = s.a_half = a;
-    s.ad = s.b = (c->cfg.expansion == PSC_EXPANSION_STATIC) ? 0.0 : ad;
-    s.par = c->models[member];
-    s.nflds = c->cfg.nflds;
+    MemberState s = fresh_state(c, member, t, a, ad);
     if ((rc = write_state(c, member, &s))) return rc;
     Diag d;
     if ((rc = diagnose(c, member, s, &d))) return rc;
@@ -766,15 +800,26 @@ int psc_set_scalars(psc_ctx* c, int member, double t, double a, double ad, doubl
 }
 
 int psc_get_scalars(psc_ctx* c, int member, double* out) {
-    int rc = check_member(c, member);
+    int rc = check_member_only(c, member);   // no synchronisation: a read of t and a
     if (rc) return rc;
     MemberState s;
     if ((rc = read_state(c, member, &s))) return rc;
+    if (s.ahead) s.a = s.a_start;   // the synchronised scale factor of a member ahead
     out[0] = s.t;
     out[1] = s.a;
     out[2] = s.ad;
     out[3] = s.add;
     if (s.a < 0) return c->fail(PSC_ERR_NEGATIVE_A, "scale factor is negative");
+    return PSC_OK;
+}
+
+int psc_get_step_count(psc_ctx* c, int member, long long* steps) {
+    int rc = check_member_only(c, member);
+    if (rc) return rc;
+    if (!steps) return c->fail(PSC_ERR_ARG, "null output");
+    MemberState s;
+    if ((rc = read_state(c, member, &s))) return rc;
+    *steps = (long long)s.steps;
     return PSC_OK;
 }
 
@@ -874,10 +919,31 @@ int psc_energy_densities(psc_ctx* c, int member, double* pot, double* kin, doubl
 int psc_set_dt(psc_ctx* c, double dt) {
     if (!c) return PSC_ERR_ARG;
     if (!(dt > 0.0) || !std::isfinite(dt)) return c->fail(PSC_ERR_ARG, "dt must be positive");
-    if (dt != c->cfg.dt) {
-        HIPCHK(c, hipStreamSynchronize(c->stream));
-        c->cfg.dt = dt;
-        drop_graphs(c);
+    int rc = ensure_synced(c);   // a look-ahead kick applied with the old dt is undone
+    if (rc) return rc;
+    c->cfg.dt = dt;
+    for (int m = 0; m < c->B; ++m) {
+        MemberState s;
+        if ((rc = read_state(c, m, &s))) return rc;
+        s.dt = s.dt_next = dt;
+        if ((rc = write_state(c, m, &s))) return rc;
+    }
+    return PSC_OK;
+}
+
+int psc_set_output_times(psc_ctx* c, double t_out, double dt_cap) {
+    if (!c) return PSC_ERR_ARG;
+    if (!(dt_cap > 0.0) || std::isnan(t_out)) return c->fail(PSC_ERR_ARG, "bad output times");
+    int rc = ensure_synced(c);   // a look-ahead kick applied with a dt of the old targets is undone
+    if (rc) return rc;
+    c->t_out = t_out;
+    c->dt_cap = dt_cap;
+    for (int m = 0; m < c->B; ++m) {
+        MemberState s;
+        if ((rc = read_state(c, m, &s))) return rc;
+        s.t_out = t_out;
+        s.dt_cap = dt_cap;
+        if ((rc = write_state(c, m, &s))) return rc;
     }
     return PSC_OK;
 }
@@ -1178,6 +1244,8 @@ int psc_autotune(psc_ctx* c, int nsteps, char* report, int report_len) {
     if (nsteps < 1) return c->fail(PSC_ERR_ARG, "nsteps must be >= 1");
     for (int b = 0; b < c->B; ++b)
         if (!c->loaded[b]) return c->fail(PSC_ERR_STATE, "autotune needs loaded initial data");
+    int rc0 = ensure_synced(c);   // the snapshot is of the synchronised state
+    if (rc0) return rc0;
     const size_t arr = (size_t)c->B * c->MS * c->es;
     const bool gw = c->cfg.with_gw != 0;
     const bool pad = c->padeng != nullptr;
@@ -1213,6 +1281,7 @@ int psc_autotune(psc_ctx* c, int nsteps, char* report, int report_len) {
         else
             HIPCHK(c, hipMemcpyAsync(c->ms, ms.data(), sizeof(MemberState) * c->B, hipMemcpyHostToDevice, c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));
+        if (!save) c->ahead = 0;
         return PSC_OK;
     };
     int rc = copy(true);
@@ -1336,6 +1405,9 @@ int psc_adjoint(psc_ctx* c, const psc_adjoint_spec* sp, double* out, double* dch
         return c->fail(PSC_ERR_UNSUPPORTED, std::string("model ") + Model::name +
                                                 " has no derivatives for coupling " + std::to_string(sp->coupling));
     if (!c->loaded[0]) return c->fail(PSC_ERR_STATE, "no initial data");
+    if (c->cfg.dt_rule != PSC_DT_FIXED) return c->fail(PSC_ERR_UNSUPPORTED, "the adjoint takes fixed steps");
+    int rs = ensure_synced(c);
+    if (rs) return rs;
     const int saved = c->timing;
     c->timing = 0;
     int rc = c->eng->adjoint(c, sp, out, dchi);

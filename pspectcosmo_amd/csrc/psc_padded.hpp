@@ -170,19 +170,23 @@ k_unpad_z(const cpx<typename GP::T>* __restrict__ Wp, cpx<typename GP::T>* __res
 // index at +-N/2.  The residual's k = 0 mode gets N^3 V'(mean) back.  Each
 // thread handles 8 modes; REDUCE sums |pi|^2 after the first kick and
 // n^2 |f|^2, ghosts included (the half-spectrum sums of the reference).
-template <class GP, bool REDUCE>
+template <class GP, bool REDUCE, bool SYNC_OUT = false>
 __global__ void __launch_bounds__(256)
 k_kick_pad(const cpx<typename GP::T>* __restrict__ R, const cpx<typename GP::T>* __restrict__ Pf,
            cpx<typename GP::T>* __restrict__ Pp, const cpx<typename GP::T>* __restrict__ Gf,
            cpx<typename GP::T>* __restrict__ Gp, const MemberState* __restrict__ ms,
-           double* __restrict__ partKG, double c1d, double c2d, int use_a_start, StepConsts sc) {
+           double* __restrict__ partKG, KickArgs ka, StepConsts sc,
+           cpx<typename GP::T>* __restrict__ Ps = nullptr, cpx<typename GP::T>* __restrict__ Gps = nullptr) {
     using T = typename GP::T;
     constexpr int NP = GP::N, N = NP / 2, H = N / 2;
     constexpr size_t PSN = plane_stride(N);
     __shared__ double red_lds[2 * 256 / 64 + 8];
-    const double a = use_a_start ? ms[0].a_start : ms[0].a;
+    if (ka.prologue && ms[0].ahead) return;   // the call's first kick was applied as a look-ahead
+    const double a = ka.prologue ? ms[0].a_start : ms[0].a;
     const T lapc = (T)(pow(a, 1 + sc.alpha) * sc.kIR * sc.kIR);
     const T vc = (T)pow(a, 3 + sc.alpha);
+    double c1d, c2d;
+    kick_coefs(ka, ms[0], c1d, c2d);
     const T c1 = (T)c1d, c2 = (T)c2d;
     const T inv8 = (T)0.125;
     auto ridx = [](int ax, int ay, int az) {
@@ -218,6 +222,7 @@ k_kick_pad(const cpx<typename GP::T>* __restrict__ R, const cpx<typename GP::T>*
         const cpx<T> F = {-ln2 * f.x - vc * E.x, -ln2 * f.y - vc * E.y};
         p.x += c1 * F.x;
         p.y += c1 * F.y;
+        if constexpr (SYNC_OUT) Ps[idx] = p;   // synchronised pi at the end of a call
         if constexpr (REDUCE) {
             const double px_ = p.x, py_ = p.y, fx = f.x, fy = f.y;
             red[0] += px_ * px_ + py_ * py_;
@@ -233,6 +238,7 @@ k_kick_pad(const cpx<typename GP::T>* __restrict__ R, const cpx<typename GP::T>*
             const cpx<T> FG = {-ln2 * gf.x - vc * GE.x, -ln2 * gf.y - vc * GE.y};
             gp.x += c1 * FG.x;
             gp.y += c1 * FG.y;
+            if constexpr (SYNC_OUT) Gps[gi] = gp;
             if constexpr (REDUCE) {
                 const double px_ = gp.x, py_ = gp.y, fx = gf.x, fy = gf.y;
                 red[0] += px_ * px_ + py_ * py_;

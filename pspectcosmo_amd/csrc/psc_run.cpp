@@ -472,7 +472,8 @@ int main(int argc, char** argv) {
     auto calculate_dt = [&](double a) { return 2.0 / (kmax / a) / 10; };   // pspectcosmo.cpp:213-233
     r.auto_dt = p.get("dt", "auto") == "auto";
     c.dt = r.auto_dt ? calculate_dt(r.ic.a0) : p.num("dt", 0.005);
-    if (r.auto_dt) c.use_graph = 0;   // dt changes every step
+    // the reference's per-step dt runs on the device, inside the captured steps
+    c.dt_rule = r.auto_dt ? PSC_DT_REFERENCE : PSC_DT_FIXED;
     const long long nsteps = (long long)p.num("nsteps", 200);
     const double tf = p.num("tf", r.auto_dt ? 1.0 : nsteps * c.dt);
     const double tOutFreq = p.num("tOutputFreq", 0.01), tOutInfreq = p.num("tOutputInfreq", 0.1);
@@ -586,17 +587,35 @@ int main(int argc, char** argv) {
         auto next_after = [&](double every) { return every * (std::floor(t / every + 1e-9) + 1); };
         double next_freq = next_after(tOutFreq), next_infreq = next_after(tOutInfreq);
         double next_snap = tSnap > 0 ? next_after(tSnap) : 1e300;
+        // Each call runs the steps up to the next output time: the device takes
+        // the reference's dt per step, capped at tOutputFreq and clamped to
+        // land on the output time, after which its steps are empty.  The
+        // number of steps of a call is estimated from the current dt (a grows,
+        // so dt does not shrink); a call that falls short is followed by
+        // another.
+        double target = -1.0;
         while (t < tf) {
+            const double t_out = std::min(next_freq, next_infreq);
+            if (t_out != target) {
+                r.check(psc_set_output_times(r.ctx, t_out, tOutFreq), "psc_set_output_times");
+                target = t_out;
+            }
             double dt = std::min(calculate_dt(a), tOutFreq);
-            if (t + dt > next_freq || t + dt > next_infreq) dt = std::min(next_freq - t, next_infreq - t);
-            r.check(psc_set_dt(r.ctx, dt), "psc_set_dt");
-            r.check(psc_step(r.ctx, 1), "psc_step");
-            dt_last = dt;
-            ++steps;
+            if (t + dt > t_out) dt = t_out - t;
+            const double n = std::ceil((t_out - t) / dt - 1e-9);
+            // the reference stops at the first step that reaches tf: near tf
+            // the steps go one per call
+            const double n_tf = std::ceil((tf - t) / dt - 1e-9);
+            const int calls = (int)std::max(1.0, std::min(n_tf <= n ? n_tf - 1 : n, 1e6));
+            r.check(psc_step(r.ctx, calls), "psc_step");
             double s[4];
             r.check(psc_get_scalars(r.ctx, 0, s), "psc_get_scalars");
             t = s[0];
             a = s[1];
+            long long done_steps = 0;
+            r.check(psc_get_step_count(r.ctx, 0, &done_steps), "psc_get_step_count");
+            steps = done_steps;
+            dt_last = dt;
             if (std::fabs(t - next_freq) < 1e-10) {
                 next_freq += tOutFreq;
                 r.save_freq();

@@ -87,6 +87,8 @@ def lib(model_name=None):
         L.psc_model_description.restype = C.c_char_p
         L.psc_num_potential_terms.argtypes = [C.c_int]
         L.psc_potential_terms.argtypes = [vp, C.c_int, _D]
+        L.psc_set_output_times.argtypes = [vp, C.c_double, C.c_double]
+        L.psc_get_step_count.argtypes = [vp, C.c_int, P(C.c_longlong)]
         _libs[path] = L
     return _libs[path]
 
@@ -101,7 +103,8 @@ def exported_symbols():
             "psc_reset_kernel_times", "psc_device_bytes", "psc_autotune", "psc_shape_name",
             "psc_list_shapes", "psc_set_shape", "psc_adjoint", "psc_set_dt", "psc_means", "psc_load_fields",
             "psc_energy_densities", "psc_generate_ic_device", "psc_generate_gw_ic_device",
-            "psc_model_name", "psc_model_description", "psc_num_potential_terms", "psc_potential_terms"]
+            "psc_model_name", "psc_model_description", "psc_num_potential_terms", "psc_potential_terms",
+            "psc_set_output_times", "psc_get_step_count"]
 
 
 def generate_ic(cfg, model, ic, model_name=None):
@@ -189,6 +192,14 @@ class Lattice:
 
     def step(self, n=1):
         self._check(self._L.psc_step(self._h, int(n)))
+
+    def set_output_times(self, t_out, dt_cap):
+        self._check(self._L.psc_set_output_times(self._h, float(t_out), float(dt_cap)))
+
+    def step_count(self, member=0):
+        n = C.c_longlong()
+        self._check(self._L.psc_get_step_count(self._h, member, C.byref(n)))
+        return n.value
 
     def set_dt(self, dt):
         self._check(self._L.psc_set_dt(self._h, float(dt)))
