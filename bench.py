@@ -24,7 +24,13 @@ Other workloads (own JSON line each):
   --workload padded256   the dealiased stepper (V' on 512^3) of the 256^3
                          two-field lattice, Yoshida-4, FP32-mixed
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+The default run adds, under "workloads", a shorter measurement of each other
+configuration (gw512, ensemble128, adjoint128, padded256), the attainable
+copy bandwidth, the stable variant of the headline lattice (kIR = 1, finite
+values), the rate of one-step psc_step calls against one call of many steps,
+and the reference's adaptive dt rule on the device against fixed dt.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-workloads]
                        [--workload preheat256|gw512|ensemble128|adjoint128|padded256] [--autotune]
 """
 import argparse
@@ -101,19 +107,37 @@ def kernel_roofline(lat, steps):
     return dom, achieved, avg_s, d["bytes"], per_kernel
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(fk, pk, sc, model, budget_s=12.0):
+    """The oracle's FP64 step with the FFTW3 API (libfftw3, else MKL's FFTW3
+    interface; planned per call with FFTW_ESTIMATE as the reference does) on
+    every host core the process may use; the oracle's own radix-2 FFT when no
+    FFTW3 API loads."""
     from oracle import oracle as O
     try:
         cores = len(os.sched_getaffinity(0))
     except AttributeError:
         cores = os.cpu_count() or 1
-    cores = max(1, min(cores, 16))
     O.set_threads(cores)
+    backend = O.set_fft_backend(1, cores)
+    if backend is None:
+        backend = O.set_fft_backend(0)
     cfg = make_config(N, nflds=2, integrator=PSC_Y4, precision=PSC_FP64, kIR=KIR, dt=DT)
     orc = O.Oracle(cfg, model)
     orc.load(fk, pk, 0.0, sc[0], sc[1], 0.0)
     add0 = orc.kernel_a()
     orc.load(fk, pk, 0.0, sc[0], sc[1], add0)
+    orc.step(1)   # warm-up step (thread pools, page faults), untimed
     n = 0
     t0 = time.perf_counter()
     while True:
@@ -122,9 +146,12 @@ def cpu_baseline(fk, pk, sc, model, budget_s=12.0):
         el = time.perf_counter() - t0
         if el >= budget_s or n >= 50:
             break
+    O.set_fft_backend(0)
     return {"value": N ** 3 * n / el, "unit": "site-updates/s", "cores": cores, "kind": "port",
-            "sample": f"{n} Yoshida-4 steps of the same 256^3 two-field lattice, FP64, oracle "
-                      f"restatement with its own radix-2 FFT and OpenMP ({el:.1f} s)"}
+            "fft_backend": backend, "nproc": os.cpu_count(), "cpu_model": cpu_model(),
+            "sample": f"{n} Yoshida-4 steps of the same 256^3 two-field lattice after one warm-up step, FP64, "
+                      f"the oracle's restatement of the reference step with the FFTW3 API ({backend}) on "
+                      f"{cores} threads ({el:.1f} s)"}
 
 
 def canonical_gw_bytes_per_site_step(s_r, n, substeps=3, nflds=2):
@@ -169,7 +196,8 @@ def bench_gw512(args):
                                   "Yoshida-4, dt = 0.001", "shape": lat.shape_name()},
            "canonical_bytes_per_site_step": b, "step_hbm_frac_canonical": v * b / (HBM_PEAK_GBS * 1e9),
            "device_bytes": lat.device_bytes(), "kernels": per_kernel}
-    print(json.dumps(out))
+    lat.close()
+    return out
 
 
 def bench_padded256(args):
@@ -210,7 +238,7 @@ def bench_padded256(args):
            "canonical_bytes_per_site_step": b, "step_hbm_frac_canonical": v * b / (HBM_PEAK_GBS * 1e9),
            "device_bytes": lat.device_bytes(), "kernels": per_kernel}
     lat.close()
-    print(json.dumps(out))
+    return out
 
 
 def bench_ensemble128(args):
@@ -287,7 +315,8 @@ def bench_ensemble128(args):
                                   "host_ms_one_member": ic_host_s * 1e3},
            "canonical_bytes_per_site_step": b, "step_hbm_frac_canonical": v * b / (HBM_PEAK_GBS * 1e9),
            "kernels": per_kernel}
-    print(json.dumps(out))
+    one.close()
+    return out
 
 
 ADJ_N, ADJ_STEPS, ADJ_EVERY, ADJ_KIR, ADJ_G2, ADJ_DT, ADJ_OMEGA = 128, 500, 25, 2.0, 1e4, 0.004, 1e-4
@@ -485,6 +514,180 @@ def bench_adjoint128(args):
     print(json.dumps(out))
 
 
+def bench_copy():
+    """Attainable HBM copy bandwidth (pspectcosmo_amd/copy_probe: 1 GiB read and
+    1 GiB written per launch, 20 launches)."""
+    import subprocess
+    exe = os.path.join(ROOT, "pspectcosmo_amd", "copy_probe")
+    r = subprocess.run([exe, "1024", "20"], capture_output=True, text=True, timeout=120)
+    if r.returncode != 0:
+        return {"error": r.stderr.strip()[-200:]}
+    d = json.loads(r.stdout.strip().splitlines()[-1])
+    d["frac_of_peak"] = d["copy_GBps"] / HBM_PEAK_GBS
+    return d
+
+
+STABLE_KIR = 1.0
+
+
+def stable_lattice(n, integrator, precision, dt, kIR=STABLE_KIR, dt_rule=0, use_graph=1):
+    """The headline model on a lattice with kIR = 1, where dt = 0.002 is
+    within the stability bound (k_max dt = 0.44 at 256^3, m_chi dt = 1.0), so
+    the fields stay finite: ICs generated on the device."""
+    cfg = make_config(n, nflds=2, integrator=integrator, precision=precision, kIR=kIR, dt=dt,
+                      use_graph=use_graph, dt_rule=dt_rule, omegaStar=1e-4)
+    lat = Lattice(cfg)
+    lat.set_model(0, make_model(1.0, G2))
+    lat.generate_ic_device(0, make_ic(f0=(1.0, 0.0), fd0=(-0.7, 0.0), seed=1))
+    return lat
+
+
+def timed_steps(lat, n, calls=1):
+    """Seconds for n steps in `calls` equal calls of psc_step."""
+    device_sync(lat)
+    t0 = time.perf_counter()
+    for _ in range(calls):
+        lat.step(n // calls)
+    device_sync(lat)
+    return time.perf_counter() - t0
+
+
+def bench_stable(steps=200):
+    """The headline configuration with kIR = 1 (finite values) at 256^3,
+    Yoshida-4, dt = 0.002, FP32-mixed and FP64."""
+    out = {"workload": "256^3 two-field lattice, g^2 = 2.5e5, kIR = 1, Yoshida-4, dt = 0.002 (k_max dt = 0.44), "
+                       "omegaStar = 1e-4, device ICs"}
+    for name, prec in (("fp32", PSC_FP32), ("fp64", PSC_FP64)):
+        lat = stable_lattice(N, PSC_Y4, prec, DT)
+        lat.step(5)
+        el = timed_steps(lat, steps)
+        e = lat.energies(0)
+        out[name] = {"value": N ** 3 * steps / el, "ms_per_step": el / steps * 1e3,
+                     "finite": bool(np.isfinite(e).all()), "a": float(e[8]),
+                     "constraint_violation": float(e[7])}
+        lat.close()
+    out["fp32_over_fp64"] = out["fp32"]["value"] / out["fp64"]["value"]
+    return out
+
+
+def bench_single_steps():
+    """psc_step(1) called n times against one psc_step(n): the look-ahead
+    kick carries across calls, so the one-step calls cost one extra write of
+    the synchronised momenta per call (FP32-mixed, stable variant)."""
+    out = {}
+    for n_lat, n in ((128, 200), (256, 60)):
+        for integ, name in ((2, "vv"), (PSC_Y4, "y4")):
+            lat = stable_lattice(n_lat, integ, PSC_FP32, DT)
+            lat.step(3)
+            one = timed_steps(lat, n, 1)
+            single = timed_steps(lat, n, n)
+            one = min(one, timed_steps(lat, n, 1))
+            single = min(single, timed_steps(lat, n, n))
+            lat.close()
+            out[f"{n_lat}_{name}"] = {"steps": n, "ms_per_step_one_call": one / n * 1e3,
+                                      "ms_per_step_single_calls": single / n * 1e3, "ratio": single / one}
+    return out
+
+
+def bench_adaptive(intervals=10, per_interval=20):
+    """The reference's dt rule (calculate_dt with the tOutputFreq cap and the
+    clamp to output times, pspectcosmo.cpp:213-233, 622-627) evaluated on the
+    device inside the captured steps, driven the way psc_run drives it (one
+    call per output interval, one read of t and a after it), against fixed dt
+    with the same cadence: 256^3, FP32-mixed, Yoshida-4, stable variant."""
+    kmax = STABLE_KIR * math.sqrt(3.0) * N / 2.0
+    dt0 = 2.0 / kmax / 10.0
+    cap = per_interval * dt0 * 1.02   # about per_interval steps per output interval
+    res = {}
+    for rule in (1, 0):
+        lat = stable_lattice(N, PSC_Y4, PSC_FP32, dt0, dt_rule=rule)
+        t, a, steps_real, calls = 0.0, 1.0, 0, 0
+        n_intervals = intervals + 1   # the first interval is a warm-up
+        t_start = None
+        s0 = 0
+        for k in range(n_intervals):
+            if k == 1:
+                device_sync(lat)
+                t_start = time.perf_counter()
+                s0 = lat.step_count(0) if rule else steps_real
+            if rule:
+                t_out = (k + 1) * cap
+                lat.set_output_times(t_out, cap)
+                while abs(t - t_out) >= 1e-10:
+                    dt = min(2.0 / (kmax / a) / 10.0, cap, t_out - t)
+                    lat.step(max(1, int(math.ceil((t_out - t) / dt - 1e-9))))
+                    calls += 1
+                    sc = lat.scalars(0)
+                    t, a = sc[0], sc[1]
+            else:
+                lat.step(per_interval)
+                steps_real += per_interval
+                lat.scalars(0)
+        device_sync(lat)
+        el = time.perf_counter() - t_start
+        n_real = (lat.step_count(0) if rule else steps_real) - s0
+        res["reference_rule" if rule else "fixed"] = {"value": N ** 3 * n_real / el, "steps": n_real,
+                                                      "ms_per_step": el / n_real * 1e3}
+        if rule:
+            res["reference_rule"]["calls"] = calls
+        lat.close()
+    res["rule_over_fixed"] = res["reference_rule"]["value"] / res["fixed"]["value"]
+    return res
+
+
+def adjoint_ratios(args):
+    """Adjoint cost of the adjoint128 workload (500 VV steps at 128^3,
+    FP32-mixed, a checkpoint every 25 steps) for both objectives:
+    backward / forward (replays and sweep against one forward) and
+    total / forward (with the checkpointed forward)."""
+    steps, every = ADJ_STEPS, ADJ_EVERY
+    out = {}
+    for obj, gw in ((0, False), (1, True)):
+        lat = adjoint_case(PSC_FP32, gw=gw)
+        lat.step(10)
+        device_sync(lat)
+        lat.close()
+        lat = adjoint_case(PSC_FP32, gw=gw)
+        fwd = timed_steps(lat, steps)
+        lat.close()
+        lat = adjoint_case(PSC_FP32, gw=gw)
+        kw = dict(objective=obj, projection=1) if gw else {}
+        lat.adjoint(steps, every, PHI0, DPHI0, want_field=False, **kw)
+        g = lat.adjoint(steps, every, PHI0, DPHI0, want_field=False, **kw)
+        lat.close()
+        out["gw_fraction" if gw else "chi2"] = {
+            "forward_ms": fwd * 1e3, "adjoint_forward_ms": g["forward_ms"], "adjoint_backward_ms": g["backward_ms"],
+            "backward_over_forward": g["backward_ms"] / (fwd * 1e3),
+            "total_over_forward": (g["forward_ms"] + g["backward_ms"]) / (fwd * 1e3),
+            "extra_device_bytes": g["extra_bytes"], "J": g["J"], "dJ_dg2": g["dg2"]}
+    return out
+
+
+def run_workloads(args):
+    """Shorter runs of every other configuration, each in its own contexts."""
+    w = {}
+    quick = argparse.Namespace(**vars(args))
+    quick.autotune = False
+    for name, fn in (("copy_bandwidth", lambda: bench_copy()),
+                     ("stable_variant", lambda: bench_stable()),
+                     ("single_step_calls", lambda: bench_single_steps()),
+                     ("adaptive_dt", lambda: bench_adaptive()),
+                     ("gw512", lambda: bench_gw512(argparse.Namespace(**{**vars(quick), "steps": 20, "warmup": 2}))),
+                     ("ensemble128", lambda: bench_ensemble128(argparse.Namespace(
+                         **{**vars(quick), "steps": 60, "warmup": 3, "seq_steps": 20}))),
+                     ("padded256", lambda: bench_padded256(argparse.Namespace(**{**vars(quick), "steps": 10, "warmup": 2}))),
+                     ("adjoint128", lambda: adjoint_ratios(args))):
+        t0 = time.perf_counter()
+        print(f"[bench] workload {name} ...", file=sys.stderr, flush=True)
+        try:
+            w[name] = fn()
+        except Exception as e:   # a failed sub-measurement is reported, not fatal to the headline
+            w[name] = {"error": f"{type(e).__name__}: {e}"}
+        if isinstance(w[name], dict):
+            w[name]["bench_seconds"] = time.perf_counter() - t0
+    return w
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -501,6 +704,7 @@ def main():
     ap.add_argument("--seq-steps", type=int, default=150, help="ensemble: steps per member run alone")
     ap.add_argument("--no-fd", action="store_true", help="adjoint: skip the finite-difference and FP64 runs")
     ap.add_argument("--no-gw", action="store_true", help="adjoint profiling runs: <chi^2> objective only")
+    ap.add_argument("--no-workloads", action="store_true", help="headline only (profiling runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs the lattice on a single GPU; --gpus "
@@ -508,13 +712,13 @@ def main():
               file=sys.stderr)
         sys.exit(2)
     if args.workload == "gw512":
-        return bench_gw512(args)
+        return print(json.dumps(bench_gw512(args)))
     if args.workload == "padded256":
-        return bench_padded256(args)
+        return print(json.dumps(bench_padded256(args)))
     if args.workload == "adjoint128":
         return bench_adjoint128(args)
     if args.workload == "ensemble128":
-        return bench_ensemble128(args)
+        return print(json.dumps(bench_ensemble128(args)))
 
     model = make_model(1.0, G2)
     ic = make_ic(f0=(1.0, 0.0), fd0=(-0.7, 0.0), seed=1)
@@ -583,6 +787,8 @@ def main():
         "kernels_fp32": per_kernel,
         "kernels_fp64": per_kernel64,
     }
+    if not args.no_workloads and not args.fp32_only:
+        out["workloads"] = run_workloads(args)
     if not args.no_cpu:
         print("[bench] cpu baseline ...", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(fk, pk, sc, model)

@@ -76,6 +76,8 @@ def lib(model_name=None):
         L.orc_adjoint_gw.argtypes = [P(PscConfig), P(PscModel), _D, _D, _D, _D, C.c_double, C.c_double,
                                      C.c_double, C.c_double, C.c_int, C.c_int, C.c_int, P(C.c_double),
                                      P(C.c_double), P(C.c_double), C.c_void_p]
+        L.orc_set_fft_backend.argtypes = [C.c_int, C.c_int]
+        L.orc_fft_backend_name.restype = C.c_char_p
         L.orc_model_name.restype = C.c_char_p
         L.orc_model_eval.argtypes = [P(PscModel), C.c_int, _D, _D, C.c_int, _D]
         _libs[path] = L
@@ -100,6 +102,13 @@ def model_eval(model, nflds, f, df=(0.0, 0.0), coupling=0, model_name=None):
 
 def set_threads(n):
     return lib().orc_set_threads(int(n))
+
+
+def set_fft_backend(backend, threads=1):
+    """0: the oracle's radix-2 FFT; 1: the FFTW3 API (libfftw3 or MKL) with
+    `threads` threads.  Returns the backend's name, or None if it is absent."""
+    rc = lib().orc_set_fft_backend(int(backend), int(threads))
+    return lib().orc_fft_backend_name().decode() if rc == 0 else None
 
 
 def park_miller_iterate(state, steps):

@@ -37,6 +37,8 @@
 #include <memory>
 #include <string>
 #include <omp.h>
+#include <dlfcn.h>
+#include <cstdlib>
 
 #include "../include/psc.h"
 #include "../pspectcosmo_amd/csrc/psc_model.hpp"
@@ -116,6 +118,53 @@ struct Fft1D {
     }
 };
 
+// The FFTW3 API the reference calls (fftw_plan_dft_r2c_3d / c2r_3d with
+// FFTW_ESTIMATE, fftw_execute, fftw_init_threads, fftw_plan_with_nthreads;
+// evolution.cpp:1112,1146, pspectcosmo.cpp:250-251), loaded at run time from
+// libfftw3 or, where that is absent, from MKL's FFTW3 interface (libmkl_rt).
+// Selected with orc_set_fft_backend, for the CPU baseline; the parity tests
+// use the radix-2 transforms above, which tests pin.
+struct FftwApi {
+    typedef void* plan;
+    plan (*r2c_3d)(int, int, int, double*, void*, unsigned) = nullptr;
+    plan (*c2r_3d)(int, int, int, void*, double*, unsigned) = nullptr;
+    void (*execute)(plan) = nullptr;
+    void (*destroy)(plan) = nullptr;
+    int (*init_threads)() = nullptr;
+    void (*with_nthreads)(int) = nullptr;
+    std::string name;
+    bool on = false;
+    static constexpr unsigned kEstimate = 1u << 6;   // FFTW_ESTIMATE
+    bool load() {
+        if (r2c_3d) return true;
+        // MKL threads on the GNU OpenMP runtime this library uses
+        setenv("MKL_THREADING_LAYER", "GNU", 0);
+        const char* cands[] = {"libfftw3.so.3", "libmkl_rt.so.1", "/opt/conda/lib/libmkl_rt.so.1"};
+        for (const char* c : cands) {
+            void* h = dlopen(c, RTLD_NOW | RTLD_GLOBAL);
+            if (!h) continue;
+            r2c_3d = (plan(*)(int, int, int, double*, void*, unsigned))dlsym(h, "fftw_plan_dft_r2c_3d");
+            c2r_3d = (plan(*)(int, int, int, void*, double*, unsigned))dlsym(h, "fftw_plan_dft_c2r_3d");
+            execute = (void (*)(plan))dlsym(h, "fftw_execute");
+            destroy = (void (*)(plan))dlsym(h, "fftw_destroy_plan");
+            // threads: libfftw3 keeps them in libfftw3_threads
+            void* ht = h;
+            if (!dlsym(h, "fftw_init_threads")) ht = dlopen("libfftw3_threads.so.3", RTLD_NOW | RTLD_GLOBAL);
+            if (ht) {
+                init_threads = (int (*)())dlsym(ht, "fftw_init_threads");
+                with_nthreads = (void (*)(int))dlsym(ht, "fftw_plan_with_nthreads");
+            }
+            if (r2c_3d && c2r_3d && execute && destroy) {
+                name = c;
+                return true;
+            }
+            r2c_3d = nullptr;
+        }
+        return false;
+    }
+};
+FftwApi g_fftw;
+
 struct Fft3D {
     int N, Nh;
     Fft1D f;
@@ -123,6 +172,12 @@ struct Fft3D {
 
     // Unnormalised forward real-to-complex transform; out in FFTW layout.
     void r2c(const double* in, cd* out) const {
+        if (g_fftw.on) {   // planned per call with FFTW_ESTIMATE, as the reference does
+            FftwApi::plan p = g_fftw.r2c_3d(N, N, N, const_cast<double*>(in), out, FftwApi::kEstimate);
+            g_fftw.execute(p);
+            g_fftw.destroy(p);
+            return;
+        }
         const int n = N, nh = Nh;
 #pragma omp parallel
         {
@@ -158,6 +213,12 @@ struct Fft3D {
     void c2r(const cd* in, double* out, std::vector<cd>& tmp) const {
         const int n = N, nh = Nh;
         tmp.assign(in, in + (size_t)n * n * nh);
+        if (g_fftw.on) {   // the c2r input is destroyed: the copy (the reference's Cache_Data_k)
+            FftwApi::plan p = g_fftw.c2r_3d(N, N, N, tmp.data(), out, FftwApi::kEstimate);
+            g_fftw.execute(p);
+            g_fftw.destroy(p);
+            return;
+        }
 #pragma omp parallel
         {
             std::vector<cd> buf(n);
@@ -804,6 +865,25 @@ const int kGwJ[6] = {0, 1, 2, 1, 2, 2};
 extern "C" {
 
 int orc_abi_version(void) { return PSC_ABI_VERSION; }
+
+// FFT backend of the 3D transforms: 0 the oracle's radix-2 FFT (default),
+// 1 the FFTW3 API (libfftw3, else MKL's FFTW3 interface) with `threads`
+// threads.  Returns 0, or PSC_ERR_UNSUPPORTED when no FFTW3 API loads.
+int orc_set_fft_backend(int backend, int threads) {
+    if (backend == 0) {
+        g_fftw.on = false;
+        return 0;
+    }
+    if (!g_fftw.load()) return PSC_ERR_UNSUPPORTED;
+    if (g_fftw.init_threads && g_fftw.with_nthreads) {
+        static bool inited = false;
+        if (!inited) inited = g_fftw.init_threads() != 0;
+        g_fftw.with_nthreads(threads > 0 ? threads : 1);
+    }
+    g_fftw.on = true;
+    return 0;
+}
+const char* orc_fft_backend_name(void) { return g_fftw.on ? g_fftw.name.c_str() : "radix-2 (oracle)"; }
 
 // The model header of this build, evaluated at one point (for the tests that
 // pin it: derivatives against differences of V, closed forms).  out:

@@ -111,7 +111,8 @@ struct Engine final : EngineBase {
         };
         using TT = std::true_type;
         using FF = std::false_type;
-        return timed(c, reduce ? K_Z_FWD_KICK : K_Z_FWD_KICK_NR, (sync_out ? 5 : 4) * X, [&] {
+        const int kind = !reduce ? K_Z_FWD_KICK_NR : (sync_out ? K_Z_FWD_KICK_SYNC : K_Z_FWD_KICK);
+        return timed(c, kind, (sync_out ? 5 : 4) * X, [&] {
             if (!reduce) {   // the call's first kick
                 if (st) go(FF{}, TT{}, FF{});
                 else go(FF{}, FF{}, FF{});
@@ -186,7 +187,7 @@ struct Engine final : EngineBase {
         if (rc) return rc;
         if ((rc = launch_x_force(c))) return rc;
         if ((rc = launch_y<-1>(c, P(c->W), 4 * c->B))) return rc;
-        return timed(c, K_Z_KICK_GW, (sync_out ? 20 : 16) * X, [&] {
+        return timed(c, sync_out ? K_Z_KICK_GW_SYNC : K_Z_KICK_GW, (sync_out ? 20 : 16) * X, [&] {
             if (reduce && sync_out)
                 hipLaunchKernelGGL((k_z_kick_gw<G, true, false, true>), rgrid, rblock, 0, c->stream, P(c->W), P(c->Pf),
                                    P(c->Pp), P(c->U), P(c->PU), c->ms, P(c->tw), c->partKG, ka, sc, astride, nullptr,
@@ -775,7 +776,7 @@ struct PadEngine final : PadBase {
         });
         if (rc) return rc;
         const StepConsts sc = consts_of(c);
-        return timed(c, K_PAD_KICK, (sync_out ? 5 : 4) * XN, [&] {
+        return timed(c, sync_out ? K_PAD_KICK_SYNC : K_PAD_KICK, (sync_out ? 5 : 4) * XN, [&] {
             if (reduce && sync_out)
                 hipLaunchKernelGGL((k_kick_pad<GP, true, true>), dim3(nKG()), dim3(256), 0, c->stream, P(c->Rp),
                                    P(c->Pf), P(c->Pp), P(c->Gf), P(c->Gp), c->ms, c->partKG, ka, sc, P(c->Ps),

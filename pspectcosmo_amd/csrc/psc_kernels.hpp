@@ -285,7 +285,9 @@ k_z_kick(const cpx<typename G::T>* __restrict__ W, const cpx<typename G::T>* __r
         p.x += c1 * F.x;
         p.y += c1 * F.y;
         if constexpr (REC) rec_p[base + G::TPL * m] = p;  // adjoint recording: synchronised pi
-        if constexpr (SYNC_OUT) Ps[base + G::TPL * m] = p;  // synchronised pi at the end of a call
+        // synchronised pi at the end of a call, read only if the state is
+        // synchronised later: a streaming store
+        if constexpr (SYNC_OUT) store_stream(Ps + base + G::TPL * m, p);
         if constexpr (REDUCE) {
             const double px_ = p.x, py_ = p.y, fx = f.x, fy = f.y;
             red[0] += px_ * px_ + py_ * py_;
@@ -1009,7 +1011,9 @@ k_z_kick_gw(const cpx<typename G::T>* __restrict__ W, const cpx<typename G::T>* 
         p.x += c1 * F.x;
         p.y += c1 * F.y;
         if constexpr (REC) rec_p[base + G::TPL * m] = p;  // adjoint recording: synchronised pi
-        if constexpr (SYNC_OUT) Ps[base + G::TPL * m] = p;  // synchronised pi at the end of a call
+        // synchronised pi at the end of a call, read only if the state is
+        // synchronised later: a streaming store
+        if constexpr (SYNC_OUT) store_stream(Ps + base + G::TPL * m, p);
         if constexpr (REDUCE) {
             const double px_ = p.x, py_ = p.y, fx = f.x, fy = f.y;
             red[0] += px_ * px_ + py_ * py_;
@@ -1044,7 +1048,7 @@ k_z_kick_gw(const cpx<typename G::T>* __restrict__ W, const cpx<typename G::T>* 
             const cpx<T> F = {norm1 * (-k2 * u.x + (T)2 * e[m].x), norm1 * (-k2 * u.y + (T)2 * e[m].y)};
             p.x += c1 * F.x;
             p.y += c1 * F.y;
-            if constexpr (SYNC_OUT) PUs[idx] = p;
+            if constexpr (SYNC_OUT) store_stream(PUs + idx, p);
             p.x += c2 * F.x;
             p.y += c2 * F.y;
             PU[idx] = p;

@@ -158,3 +158,26 @@ def test_pi_k_relation_of_modes():
     om2 = kIR ** 2 * (n[:, None, None] ** 2 + n[None, :, None] ** 2 + kz[None, None, :] ** 2) + 1.0
     r = (np.abs(pk[0]) ** 2 / om2)[:, :, 1:N // 2].mean() / (np.abs(fk[0]) ** 2)[:, :, 1:N // 2].mean()
     assert abs(r - 1.0) < 0.1
+
+
+def test_fftw_api_backend_agrees_with_radix2():
+    """The FFTW3-API backend of the CPU baseline (libfftw3, else MKL's FFTW3
+    interface) against the oracle's pinned radix-2 transforms: one Yoshida-4
+    step of a 16^3 two-field lattice."""
+    from oracle import oracle as O
+    from pspectcosmo_amd.abi import PSC_Y4
+    from tests._util import parity_setup, rel_err
+    import pytest
+    cfg, model, ic = parity_setup(N=16, integrator=PSC_Y4)
+    fk, pk, sc = O.generate_ic(cfg, model, ic)
+    res = []
+    for backend in (0, 1):
+        name = O.set_fft_backend(backend, 2)
+        if name is None:
+            pytest.skip("no FFTW3 API (libfftw3 or MKL) on this host")
+        o = O.Oracle(cfg, model)
+        o.load(fk, pk, 0.0, sc[0], sc[1], sc[4])
+        o.step(2)
+        res.append(o.get_k())
+    O.set_fft_backend(0)
+    assert rel_err(res[1][0], res[0][0]) < 1e-13 and rel_err(res[1][1], res[0][1]) < 1e-13
