@@ -110,6 +110,7 @@ const char* kKeys[][2] = {
     {"hdf5_lib", "path of libhdf5.so (default: the loader's search, then /opt/conda/lib)"},
     {"checkpoint", "1: write <dir>/grid.img with the spectra and at the end"},
     {"continue_run", "1: resume from <dir>/grid.img when it exists"},
+    {"store_lattice_times", "times at which the lattice image moves to a new file grid<T>.img, comma separated"},
 };
 
 void usage() {
@@ -327,13 +328,39 @@ struct Run {
 
     // Lattice image in the reference's layout (checkpoint, output.cpp:1690-1697):
     // int run_number, double t, a, da/dt, then f and pi, nflds x N^3 doubles each.
+    // Image file switching at store_lattice_times (checkpoint,
+    // output.cpp:1653-1683): with no times (or a first time of 0) every image
+    // overwrites grid.img; otherwise the images go to grid<T>.img, T the
+    // integer part of the listed time they run up to, and the first image
+    // written after t passes that time moves on to the next name (one move
+    // per image), the last file being named for tf.
+    std::vector<double> lattice_times;
+    size_t lattice_current = 0;
+    bool lattice_first = true;
+    std::string lattice_name;
+    double tf = 0.0;
+
+    std::string grid_name(double t) {
+        const bool multi = !lattice_times.empty() && lattice_times[0] != 0.0;
+        if (!multi) return dir + "/grid.img";
+        if (lattice_first) {
+            lattice_first = false;
+            lattice_name = dir + "/grid" + std::to_string((int)lattice_times[0]) + ".img";
+        } else if (lattice_current < lattice_times.size() && t > lattice_times[lattice_current]) {
+            ++lattice_current;
+            const double upto = lattice_current < lattice_times.size() ? lattice_times[lattice_current] : tf;
+            lattice_name = dir + "/grid" + std::to_string((int)upto) + ".img";
+        }
+        return lattice_name;
+    }
+
     void write_grid(int run_number) {
         const size_t n3 = (size_t)cfg.N * cfg.N * cfg.N;
         std::vector<double> f(n3 * cfg.nflds), p(n3 * cfg.nflds);
         check(psc_get_fields(ctx, 0, f.data(), p.data()), "psc_get_fields");
         double sc[4];
         check(psc_get_scalars(ctx, 0, sc), "psc_get_scalars");
-        const std::string name = dir + "/grid.img";
+        const std::string name = grid_name(sc[0]);
         std::ofstream g(name + ".tmp", std::ios::binary);
         g.write(reinterpret_cast<const char*>(&run_number), sizeof(run_number));
         g.write(reinterpret_cast<const char*>(&sc[0]), sizeof(double));
@@ -476,6 +503,8 @@ int main(int argc, char** argv) {
     c.dt_rule = r.auto_dt ? PSC_DT_REFERENCE : PSC_DT_FIXED;
     const long long nsteps = (long long)p.num("nsteps", 200);
     const double tf = p.num("tf", r.auto_dt ? 1.0 : nsteps * c.dt);
+    r.tf = tf;
+    r.lattice_times = p.list("store_lattice_times", {});
     const double tOutFreq = p.num("tOutputFreq", 0.01), tOutInfreq = p.num("tOutputInfreq", 0.1);
     const double tSnap = p.num("tSnapFreq", 0.0);
     const long long freq_steps = (long long)p.num("freq_steps", 20), infreq_steps = (long long)p.num("infreq_steps", 20);

@@ -170,3 +170,120 @@ def test_padded_run_matches_oracle(tmp_path):
         close(energies[r][1:], [e[0], e[1], e[2], e[3], e[4], e[5], e[6]])
     spec = [x for x in rows(tmp_path / "spectra_scalar_1.txt") if x is not None]
     close(np.array(spec)[:, 1:4], o.spectra(1)[:, 1:4], rtol=2e-6, atol=1e-300)
+
+
+N3 = 16 ** 3
+
+
+def read_image(path, nflds=2):
+    """A lattice image in the reference's layout (checkpoint,
+    output.cpp:1690-1697): int run_number, double t, a, da/dt, then
+    f[nflds][N][N][N] and pi_f[nflds][N][N][N] as doubles."""
+    raw = open(path, "rb").read()
+    assert len(raw) == 4 + 3 * 8 + 2 * nflds * N3 * 8
+    run = int(np.frombuffer(raw, np.int32, 1, 0)[0])
+    t, a, ad = np.frombuffer(raw, np.float64, 3, 4)
+    f = np.frombuffer(raw, np.float64, nflds * N3, 28).reshape(nflds, 16, 16, 16)
+    p = np.frombuffer(raw, np.float64, nflds * N3, 28 + nflds * N3 * 8).reshape(nflds, 16, 16, 16)
+    return run, t, a, ad, f, p
+
+
+def write_image(path, run, t, a, ad, f, p):
+    with open(path, "wb") as out:
+        out.write(np.int32(run).tobytes())
+        out.write(np.array([t, a, ad], np.float64).tobytes())
+        out.write(np.ascontiguousarray(f, np.float64).tobytes())
+        out.write(np.ascontiguousarray(p, np.float64).tobytes())
+
+
+def test_lattice_image_holds_the_oracle_state(tmp_path):
+    """grid.img written after 5 steps, read with the reference's layout,
+    against the oracle's real-space fields, momenta and scalars at t = 0.05."""
+    run_driver(tmp_path, "dt=0.01", "nsteps=5", "freq_steps=5", "infreq_steps=5", "checkpoint=1")
+    run, t, a, ad, f, p = read_image(tmp_path / "grid.img")
+    _, o = oracle(0.01)
+    o.step(5)
+    of, op = o.fields()
+    s = o.scalars()
+    assert run == 0
+    assert abs(t - 0.05) < 1e-15
+    np.testing.assert_allclose([a, ad], s[1:3], rtol=1e-13)
+    ef = np.linalg.norm(f - of) / np.linalg.norm(of)
+    ep = np.linalg.norm(p - op) / np.linalg.norm(op)
+    print(f"grid.img vs oracle: field {ef:.2e} momentum {ep:.2e}")
+    assert ef <= 1e-12 and ep <= 1e-12
+    assert np.linalg.norm((f - f.mean(axis=(1, 2, 3), keepdims=True)) - (of - of.mean(axis=(1, 2, 3), keepdims=True))) \
+        <= 1e-10 * np.linalg.norm(of - of.mean(axis=(1, 2, 3), keepdims=True))
+
+
+def test_resume_from_an_image_written_by_the_oracle(tmp_path):
+    """An image written here from the oracle's state at t = 0.05 (the
+    reference's layout, run number 0), resumed by the driver to step 10:
+    its outputs at t = 0.1 against the oracle run straight to t = 0.1, and the
+    run number of the image it leaves is 1 (initialize.cpp:592-593)."""
+    _, o = oracle(0.01)
+    o.step(5)
+    of, op = o.fields()
+    s = o.scalars()
+    write_image(tmp_path / "grid.img", 0, s[0], s[1], s[2], of, op)
+    r = run_driver(tmp_path, "dt=0.01", "nsteps=10", "freq_steps=5", "infreq_steps=5", "checkpoint=1",
+                   "continue_run=1")
+    assert "Resuming run at t = 0.05" in r.stdout
+    o.step(5)
+    energies = rows(tmp_path / "average_energies.txt")
+    assert len(energies) == 1
+    e = o.energies()
+    assert abs(energies[0][0] - 0.1) < 1e-3
+    close(energies[0][1:], e[:7])
+    for q, m in enumerate(oracle_means(o)):
+        close(rows(tmp_path / f"average_scalar_{q}.txt")[0][1:], m, rtol=2e-6, atol=1e-12)
+    run, t, a, ad, f, p = read_image(tmp_path / "grid.img")
+    assert run == 1 and abs(t - 0.1) < 1e-12
+    of, op = o.fields()
+    assert np.linalg.norm(f - of) <= 1e-12 * np.linalg.norm(of)
+    assert np.linalg.norm(p - op) <= 1e-12 * np.linalg.norm(op)
+
+
+def test_lattice_images_switch_files_at_store_lattice_times(tmp_path):
+    """store_lattice_times = 1.2, 2.2 (checkpoint, output.cpp:1653-1683):
+    images up to t = 1.2 go to grid1.img, then to grid2.img up to 2.2, then
+    to a file named for tf = 3; each holds the last image written to it."""
+    run_driver(tmp_path, "dt=0.1", "nsteps=30", "freq_steps=5", "infreq_steps=5", "checkpoint=1",
+               "kIR=0.25", "g2=10", "store_lattice_times=1.2,2.2")
+    assert not (tmp_path / "grid.img").exists()
+    times = {}
+    for name in ("grid1.img", "grid2.img", "grid3.img"):
+        run, t, a, ad, f, p = read_image(tmp_path / name)
+        times[name] = t
+        assert run == 0 and np.all(np.isfinite(f))
+    assert abs(times["grid1.img"] - 1.0) < 1e-9
+    assert abs(times["grid2.img"] - 2.0) < 1e-9
+    assert abs(times["grid3.img"] - 3.0) < 1e-9
+
+
+def run_failing(tmp_path, *args):
+    print("psc_run (expected to stop)", *args, flush=True)
+    return subprocess.run([PSC_RUN, *COMMON, *args, f"dir={tmp_path}"], stdout=subprocess.PIPE,
+                          stderr=subprocess.PIPE, text=True, timeout=120)
+
+
+def test_unstable_run_exits_with_status_1(tmp_path):
+    """dt = 1 at k_max = 55 (omega dt >> 2): the leapfrog blows up, the
+    fields overflow and the first output after that stops the run with the
+    reference's exit status 1 (output.cpp:450-454)."""
+    r = run_failing(tmp_path, "dt=1.0", "nsteps=400", "freq_steps=10", "infreq_steps=400")
+    print(r.stderr[-300:])
+    assert r.returncode == 1
+    assert "not numerical" in r.stderr
+
+
+def test_negative_scale_factor_exits_with_status_1(tmp_path):
+    """An image whose da/dt is set to -1000 before it is resumed: a < 0 after
+    the next step stops the run with exit status 1 (output.cpp:480-483)."""
+    run_driver(tmp_path, "dt=0.01", "nsteps=5", "freq_steps=5", "infreq_steps=5", "checkpoint=1")
+    run, t, a, ad, f, p = read_image(tmp_path / "grid.img")
+    write_image(tmp_path / "grid.img", run, t, a, -1000.0, f, p)
+    r = run_failing(tmp_path, "dt=0.01", "nsteps=10", "freq_steps=5", "infreq_steps=5", "continue_run=1")
+    print(r.stderr[-300:])
+    assert r.returncode == 1
+    assert "negative" in r.stderr

@@ -130,3 +130,58 @@ def test_gw_512_shapes_agree():
         print(f"{name}: GW change {es:.2e} field change {ef:.2e}", flush=True)
         assert es <= 1e-3 and ef <= 1e-3, name
     lat.close()
+
+
+@pytest.mark.parametrize("precision,tol", [(PSC_FP64, 1e-12), (PSC_FP32, 2e-6)])
+def test_free_gw_exact_at_512(precision, tol):
+    """The 512^3 GW instantiations against the exact discrete solution: with
+    phi = chi = 0 the source d_i f d_j f vanishes, and on a static background
+    each component of each mode follows the velocity-Verlet map of
+    u'' = -kIR^2 n^2 u (lapl, evolution.cpp:343-372) exactly:
+    u' = A u + dt pi, pi' = C u + A pi with A = 1 - w^2 dt^2 / 2,
+    C = -w^2 dt (1 - w^2 dt^2 / 4).  Random modes in all six components
+    (the kz = 0 and N/2 planes left empty so that no Hermitian pairing is
+    needed), two steps at k_max dt = 1.1."""
+    from pspectcosmo_amd.abi import PSC_EXPANSION_STATIC
+    N, kIR, dt, steps = 512, 0.5, 0.005, 2
+    cfg, model, ic = parity_setup(N=N, with_gw=1, integrator=PSC_VV2, precision=precision, kIR=kIR, dt=dt,
+                                  expansion=PSC_EXPANSION_STATIC)
+    print(f"case free GW 512^3 precision {precision}", flush=True)
+    rng = np.random.default_rng(3)
+    shape = (6, N, N, N // 2 + 1)
+    u = np.empty(shape, np.complex128)
+    pu = np.empty(shape, np.complex128)
+    for c in range(6):
+        for arr in (u, pu):
+            arr[c].real = rng.standard_normal(shape[1:], dtype=np.float32)
+            arr[c].imag = rng.standard_normal(shape[1:], dtype=np.float32)
+            arr[c][:, :, 0] = 0
+            arr[c][:, :, N // 2] = 0
+    fk = np.zeros((2, N, N, N // 2 + 1), np.complex128)
+    lat = Lattice(cfg)
+    lat.set_model(0, model)
+    lat.load_ic_k(0, fk, fk, 1.0, 0.0)
+    del fk
+    lat.load_gw_k(0, u, pu)
+    print("loaded", flush=True)
+    lat.step(steps)
+    gu, gp = lat.get_gw_k(0)
+    lat.close()
+    i = np.arange(N)
+    n = np.where(i <= N // 2, i, i - N).astype(np.float64)
+    nz = np.arange(N // 2 + 1, dtype=np.float64)
+    w2 = kIR ** 2 * (n[:, None, None] ** 2 + n[None, :, None] ** 2 + nz[None, None, :] ** 2)
+    A = 1 - w2 * dt * dt / 2
+    C = -w2 * dt * (1 - w2 * dt * dt / 4)
+    num_u = num_p = den_u = den_p = 0.0
+    for c in range(6):
+        f, p = u[c], pu[c]
+        for _ in range(steps):
+            f, p = A * f + dt * p, C * f + A * p
+        num_u += np.linalg.norm(gu[c] - f) ** 2
+        num_p += np.linalg.norm(gp[c] - p) ** 2
+        den_u += np.linalg.norm(f) ** 2
+        den_p += np.linalg.norm(p) ** 2
+    eu, ep = float(np.sqrt(num_u / den_u)), float(np.sqrt(num_p / den_p))
+    print(f"free GW 512^3: u {eu:.3e} pi_u {ep:.3e}")
+    assert eu < tol and ep < tol

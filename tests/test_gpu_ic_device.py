@@ -135,3 +135,53 @@ def test_first_generation_after_another_context_at_512():
     print("first", sums[0], "second", sums[1], flush=True)
     assert sums[0] == sums[1]
     lat.close()
+
+
+@pytest.mark.parametrize("nflds,N,seed", [(2, 32, 11), (1, 16, 3), (2, 64, 7)])
+def test_device_ic_matches_oracle(nflds, N, seed):
+    """The device generator against the oracle's restatement of initialize()
+    directly (not through the host generator): modes, background scalars,
+    and three steps of both from their own starts."""
+    from oracle import oracle as O
+    cfg, model, ic = parity_setup(N=N, nflds=nflds, precision=PSC_FP64, omega_star=2e-2, kIR=4.0, g2=2e3,
+                                  seed=seed)
+    print(f"case device IC vs oracle {N}^3 nflds {nflds} seed {seed}", flush=True)
+    ofk, opk, osc = O.generate_ic(cfg, model, ic)
+    d = Lattice(cfg)
+    d.set_model(0, model)
+    sd = d.generate_ic_device(0, ic)
+    dfk, dpk = d.get_k()
+    ef, ep = max_rel(dfk, ofk), max_rel(dpk, opk)
+    print(f"device vs oracle: field {ef:.2e} momentum {ep:.2e} (max error / max value)")
+    assert ef <= 1e-14 and ep <= 1e-14
+    np.testing.assert_allclose(sd, osc[:4], rtol=1e-15, atol=0)
+    s = d.scalars()
+    np.testing.assert_allclose(s[1:3], osc[0:2], rtol=1e-15, atol=0)
+    np.testing.assert_allclose(s[3], osc[4], rtol=1e-12, atol=0)   # d2a/dt2 from the device's own sums
+    o = O.Oracle(cfg, model)
+    o.load(ofk, opk, 0.0, osc[0], osc[1], osc[4])
+    o.step(3)
+    d.step(3)
+    gk, gp = d.get_k()
+    ok, op = o.get_k()
+    print(f"after 3 steps: field {rel_err(gk, ok):.2e} momentum {rel_err(gp, op):.2e}")
+    assert rel_err(gk, ok) <= 1e-12 and rel_err(gp, op) <= 1e-12
+    np.testing.assert_allclose(d.scalars(), o.scalars(), rtol=1e-12)
+    d.close()
+
+
+def test_device_gw_ic_matches_oracle():
+    from oracle import oracle as O
+    cfg, model, ic = parity_setup(N=32, precision=PSC_FP64, omega_star=2e-2, kIR=4.0, g2=2e3, seed=5, with_gw=1,
+                                  integrator=PSC_Y4)
+    print("case device GW IC vs oracle 32^3 fp64", flush=True)
+    ou, opu = O.generate_gw_ic(cfg, model, ic)
+    d = Lattice(cfg)
+    d.set_model(0, model)
+    d.generate_ic_device(0, ic)
+    d.generate_gw_ic_device(0, ic)
+    du, dpu = d.get_gw_k()
+    eu, ep = max_rel(du, ou), max_rel(dpu, opu)
+    print(f"u {eu:.2e} pi_u {ep:.2e}")
+    assert eu <= 1e-14 and ep <= 1e-14
+    d.close()
