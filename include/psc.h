@@ -232,6 +232,16 @@ int psc_get_fields(psc_ctx* ctx, int member, double* f, double* pi);
  * (snapshots, output.cpp:1614-1625).  The a-powers are the reference's. */
 int psc_energy_densities(psc_ctx* ctx, int member, double* pot, double* kin, double* grad, double* tot);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (serving loops, harnesses that forbid hipMalloc): the same densities,
+ * bit for bit, staged through a fixed 32 MiB device array of the library (one
+ * per GPU, held until the process ends) in blocks of y rows instead of a
+ * per-call N^3 allocation; it calls no hipMalloc or hipFree.  tot needs pot,
+ * kin and grad (there is no host staging either).  Calls that overlap on one
+ * GPU share the array and race; PSC_ERR_UNSUPPORTED when N^2 doubles exceed
+ * it (N > 2048). */
+int psc_energy_densities_noalloc(psc_ctx* ctx, int member, double* pot, double* kin, double* grad, double* tot);
+
 /* Energies, replacing energy_density()/energy() (output.cpp:57-115,1069-1187).
  * out[PSC_ENERGY_LEN]: K_0, G_0, K_1, G_1, V_term0, V_term1 (the first two
  * potential terms; psc_potential_terms has all of them), rho_total,

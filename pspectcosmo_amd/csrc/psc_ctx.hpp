@@ -131,6 +131,7 @@ struct psc_ctx {
     int N = 0, B = 1;
     size_t N3 = 0, es = 0;
     size_t MS = 0;   // member stride of the padded device arrays (elements)
+    int device = 0;  // the device current at psc_create, which holds the arrays
     hipStream_t stream = nullptr;
     void* Pf = nullptr;
     void* Pp = nullptr;
@@ -278,8 +279,9 @@ struct EngineBase {
     // Forward 3D transform of the packed real-space lattice in W[member] into dst[member].
     virtual int to_spectrum(psc_ctx* c, int member, void* dst) = 0;
     // Snapshot energy density `kind` (0 potential, 1 kinetic, 2 gradient) of one
-    // member into the dense N^3 device array D; scale as k_x_density takes it.
-    virtual int density(psc_ctx* c, int member, int kind, double* D, double scale) = 0;
+    // member's y rows o0 .. o0 + rows - 1 into the device array D (rows = N:
+    // the dense N^3 lattice), laid out and scaled as k_x_density describes.
+    virtual int density(psc_ctx* c, int member, int kind, double* D, double scale, int o0, int rows) = 0;
 };
 
 // ---- helpers of psc_lib.hip used by the engines ---------------------------

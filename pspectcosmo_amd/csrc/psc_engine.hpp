@@ -382,11 +382,12 @@ struct Engine final : EngineBase {
         return PSC_OK;
     }
 
-    int density(psc_ctx* c, int member, int kind, double* D, double scale) override {
+    int density(psc_ctx* c, int member, int kind, double* D, double scale, int o0, int rows) override {
+        if (o0 < 0 || rows < 1 || o0 + rows > N) return c->fail(PSC_ERR_ARG, "bad density row range");
         const size_t off = (size_t)member * G::MS;
         cpx<T>* W = P(c->W) + off;
         const cpx<T>* tw = P(c->tw);
-        dim3 cgrid(N / G::TC, N, 1), cblock(G::THREADS_COL);
+        dim3 cgrid(N / G::TC, rows, 1), cblock(G::THREADS_COL);
         int rc;
         if (kind < 2) {
             const cpx<T>* src = P(kind == 0 ? c->Pf : c->Pp) + off;
@@ -394,18 +395,21 @@ struct Engine final : EngineBase {
             if ((rc = launch_y<+1>(c, W, 1))) return rc;
             if (kind == 0)
                 hipLaunchKernelGGL((k_x_density<G, 0>), cgrid, cblock, 0, c->stream, W, D, c->ms + member, tw,
-                                   src, scale);
+                                   src, scale, o0, rows);
             else
                 hipLaunchKernelGGL((k_x_density<G, 1>), cgrid, cblock, 0, c->stream, W, D, c->ms + member, tw,
-                                   src, scale);
+                                   src, scale, o0, rows);
         } else {
-            HIPCHK(c, hipMemsetAsync(D, 0, G::N3 * sizeof(double), c->stream));
             for (int d = 0; d < 3; ++d) {
                 hipLaunchKernelGGL((k_z_inverse_deriv<G>), dim3(G::BLOCKS_ROW, 1), dim3(G::THREADS_ROW), 0,
                                    c->stream, P(c->Pf) + off, W, tw, d, c->cfg.kIR);
                 if ((rc = launch_y<+1>(c, W, 1))) return rc;
-                hipLaunchKernelGGL((k_x_density<G, 2>), cgrid, cblock, 0, c->stream, W, D, c->ms + member, tw,
-                                   nullptr, scale);
+                if (d == 0)
+                    hipLaunchKernelGGL((k_x_density<G, 3>), cgrid, cblock, 0, c->stream, W, D, c->ms + member, tw,
+                                       nullptr, scale, o0, rows);
+                else
+                    hipLaunchKernelGGL((k_x_density<G, 2>), cgrid, cblock, 0, c->stream, W, D, c->ms + member, tw,
+                                       nullptr, scale, o0, rows);
             }
         }
         HIPCHK(c, hipGetLastError());

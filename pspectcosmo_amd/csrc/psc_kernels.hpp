@@ -704,22 +704,26 @@ namespace psc {
 __device__ __forceinline__ int deriv_index(int i, int N) { return i == N / 2 ? 0 : signed_index(i, N); }
 
 // Snapshot energy densities (snapshots, output.cpp:1614-1625), x-axis inverse
-// pass into a dense x-major N^3 double array D (the order of the reference's
-// f[fld][i][j][k]).  MODE 0: potential V(phi, chi) summed over the terms
+// pass into an x-major double array D (the order of the reference's
+// f[fld][i][j][k]) holding the y rows o0 .. o0 + rows - 1 of every x plane:
+// D[(x rows + r) N + z] is the site (x, o0 + r, z), so rows = N, o0 = 0 is the
+// whole N^3 lattice.  MODE 0: potential V(phi, chi) summed over the terms
 // (model.h:35-73; scale unused), with the k = 0 mode of src added back as in
 // k_x_inverse_diag; MODE 1: scale * (pi_phi^2 + pi_chi^2), likewise;
-// MODE 2: D += scale * (d phi^2 + d chi^2) for one derivative axis (no k = 0
-// mode).  W and src are one member's arrays; the grid covers one member.
+// MODE 3: D = scale * (d phi^2 + d chi^2) for the first derivative axis and
+// MODE 2: D += scale * (d phi^2 + d chi^2) for the others (no k = 0 mode).
+// W and src are one member's arrays; the grid is (N / TC, rows).
 template <class G, int MODE>
 __global__ void __launch_bounds__((G::THREADS_COL))
 k_x_density(const cpx<typename G::T>* __restrict__ W, double* __restrict__ D, const MemberState* __restrict__ ms,
-            const cpx<typename G::T>* __restrict__ tw, const cpx<typename G::T>* __restrict__ src, double scale) {
+            const cpx<typename G::T>* __restrict__ tw, const cpx<typename G::T>* __restrict__ src, double scale,
+            int o0, int rows) {
     using T = typename G::T;
     constexpr int N = G::N;
     __shared__ cpx<T> lds[G::LayCol::size(0)];
     const int c = threadIdx.x % G::TC, t = threadIdx.x / G::TC;
     const size_t z = (size_t)blockIdx.x * G::TC + c;
-    const size_t o = blockIdx.y;
+    const size_t o = (size_t)o0 + blockIdx.y;
     constexpr size_t stride = G::PS;
     const size_t base = o * N + z + t * stride;
     cpx<T> e[G::E];
@@ -728,7 +732,7 @@ k_x_density(const cpx<typename G::T>* __restrict__ W, double* __restrict__ D, co
     line_fft<T, N, typename G::Rad, +1, typename G::LayCol>(e, t, c, lds, tw);
     const double inv = 1.0 / (double)G::N3;
     double pb = 0.0, qb = 0.0;
-    if constexpr (MODE != 2) {
+    if constexpr (MODE < 2) {
         const cpx<T> dc = src[0];
         pb = (double)dc.x * inv;
         qb = (double)dc.y * inv;
@@ -739,12 +743,12 @@ k_x_density(const cpx<typename G::T>* __restrict__ W, double* __restrict__ D, co
     for (int m = 0; m < G::E; ++m) {
         const double u = pb + (double)e[m].x * inv, v = nf > 1 ? qb + (double)e[m].y * inv : 0.0;
         const size_t x = (size_t)(t + G::TPL * m);
-        double* d = D + (x * N + o) * N + z;
+        double* d = D + (x * (size_t)rows + blockIdx.y) * N + z;
         if constexpr (MODE == 0) {
             const double f[2] = {u, v};
             *d = potential_sum<Model, double>(nf, f, cp);
         }
-        else if constexpr (MODE == 1) *d = scale * (u * u + v * v);
+        else if constexpr (MODE == 1 || MODE == 3) *d = scale * (u * u + v * v);
         else *d += scale * (u * u + v * v);
     }
 }

@@ -12,6 +12,7 @@
 #include <sstream>
 #include <sys/stat.h>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -423,6 +424,48 @@ void drop_graphs(psc_ctx* c) {
     c->adj.reset();
 }
 
+// Scales of the three snapshot densities: potential (coupling term only with
+// two fields), a^6/2 sum pi^2, a^2/2 sum |grad f|^2 (evolution.cpp:7-19,
+// 135-137).
+void density_scales(const psc_ctx* c, const MemberState& s, double scales[3]) {
+    scales[0] = c->cfg.nflds == 2 ? 1.0 : 0.0;
+    scales[1] = std::pow(s.a, 6) / 2.0;
+    scales[2] = s.a * s.a / 2.0;
+}
+
+// The fixed staging array of psc_energy_densities_noalloc: 32 MiB of doubles
+// per device, held until the process ends.  Host code never passes `scratch`
+// itself to a kernel or a copy (on the host it names a stand-in, not device
+// memory): it uses the device address that hipGetSymbolAddress returns, one
+// per device, kept in scratch_addr.
+constexpr size_t kScratchDoubles = (size_t)4 << 20;
+__device__ __align__(256) double scratch[kScratchDoubles];
+
+constexpr int kMaxDevices = 64;
+std::mutex scratch_mutex;
+double* scratch_addr[kMaxDevices] = {};
+
+// The device address of `scratch` on device `dev` (looked up once, with dev
+// current), or nullptr when the lookup fails.
+double* scratch_on_device(int dev) {
+    if (dev < 0 || dev >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lock(scratch_mutex);
+    if (!scratch_addr[dev]) {
+        int cur = -1;
+        if (hipGetDevice(&cur) != hipSuccess) return nullptr;
+        if (cur != dev && hipSetDevice(dev) != hipSuccess) return nullptr;
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (cur != dev) (void)hipSetDevice(cur);
+        if (e != hipSuccess || !p) {
+            (void)hipGetLastError();
+            return nullptr;
+        }
+        scratch_addr[dev] = static_cast<double*>(p);
+    }
+    return scratch_addr[dev];
+}
+
 }  // namespace
 
 // =============================================================================
@@ -519,6 +562,8 @@ int psc_create(const psc_config* cfg, psc_ctx** out) {
     c->models.assign(c->B, psc_model{1.0, 0.0, 0.0, {0.0, 0.0, 0.0, 0.0, 0.0}});
     c->loaded.assign(c->B, 0);
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return PSC_ERR_DEVICE;
+    if (hipGetDevice(&c->device) != hipSuccess) return PSC_ERR_DEVICE;
+    (void)scratch_on_device(c->device);   // set-up of psc_energy_densities_noalloc's array; its call re-checks
     const size_t arr = (size_t)c->B * c->MS * c->es;
     auto alloc = [&](void** p, size_t bytes) {
         if (hipMalloc(p, bytes) != hipSuccess) return false;
@@ -893,9 +938,8 @@ int psc_energy_densities(psc_ctx* c, int member, double* pot, double* kin, doubl
     std::unique_ptr<double, void (*)(double*)> hold(D, [](double* p) { (void)hipFree(p); });
     std::vector<double> tmp;
     double* outs[3] = {pot, kin, grad};
-    // potential (coupling term only with two fields), a^6/2 sum pi^2,
-    // a^2/2 sum |grad f|^2 (evolution.cpp:7-19, 135-137)
-    const double scales[3] = {c->cfg.nflds == 2 ? 1.0 : 0.0, std::pow(s.a, 6) / 2.0, s.a * s.a / 2.0};
+    double scales[3];
+    density_scales(c, s, scales);
     for (int kind = 0; kind < 3; ++kind) {
         double* dst = outs[kind];
         if (!dst && !tot) continue;
@@ -903,7 +947,7 @@ int psc_energy_densities(psc_ctx* c, int member, double* pot, double* kin, doubl
             tmp.resize(n3);
             dst = tmp.data();
         }
-        if ((rc = c->eng->density(c, member, kind, D, scales[kind]))) return rc;
+        if ((rc = c->eng->density(c, member, kind, D, scales[kind], 0, c->N))) return rc;
         HIPCHK(c, hipMemcpyAsync(dst, D, n3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));
         if (tot) {  // tot = pot + kin + grad in that order (output.cpp:1623)
@@ -911,6 +955,46 @@ int psc_energy_densities(psc_ctx* c, int member, double* pot, double* kin, doubl
                 std::memcpy(tot, dst, n3 * sizeof(double));
             else
                 for (size_t i = 0; i < n3; ++i) tot[i] += dst[i];
+        }
+    }
+    return PSC_OK;
+}
+
+int psc_energy_densities_noalloc(psc_ctx* c, int member, double* pot, double* kin, double* grad, double* tot) {
+    int rc = check_member(c, member);
+    if (rc) return rc;
+    if (tot && (!pot || !kin || !grad))
+        return c->fail(PSC_ERR_ARG, "tot needs pot, kin and grad: this call has no host staging");
+    const size_t row = (size_t)c->N * c->N;   // one y row of every x plane, in doubles
+    if (row > kScratchDoubles)
+        return c->fail(PSC_ERR_UNSUPPORTED, "N^2 doubles exceed the 32 MiB staging array");
+    double* S = scratch_on_device(c->device);
+    if (!S) return c->fail(PSC_ERR_DEVICE, "hipGetSymbolAddress of the staging array failed");
+    MemberState s;
+    if ((rc = read_state(c, member, &s))) return rc;
+    double scales[3];
+    density_scales(c, s, scales);
+    const int N = c->N;
+    const int rows = (int)std::min<size_t>((size_t)N, kScratchDoubles / row);
+    double* outs[3] = {pot, kin, grad};
+    for (int kind = 0; kind < 3; ++kind) {
+        double* dst = outs[kind];
+        if (!dst) continue;
+        // y rows o0 .. o0 + r - 1 of every x plane: N blocks of r N doubles,
+        // one per plane, placed at (x N + o0) N in the output
+        for (int o0 = 0; o0 < N; o0 += rows) {
+            const int r = std::min(rows, N - o0);
+            if ((rc = c->eng->density(c, member, kind, S, scales[kind], o0, r))) return rc;
+            const size_t w = (size_t)r * N * sizeof(double);
+            HIPCHK(c, hipMemcpy2DAsync(dst + (size_t)o0 * N, row * sizeof(double), S, w, w, (size_t)N,
+                                       hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+        }
+        if (tot) {  // tot = pot + kin + grad in that order (output.cpp:1623)
+            if (kind == 0)
+                std::memcpy(tot, dst, c->N3 * sizeof(double));
+            else
+                for (size_t i = 0; i < c->N3; ++i) tot[i] += dst[i];
         }
     }
     return PSC_OK;

@@ -65,6 +65,7 @@ def lib(model_name=None):
         L.psc_get_fields.argtypes = [vp, C.c_int, _D, _D]
         L.psc_energies.argtypes = [vp, C.c_int, _D]
         L.psc_energy_densities.argtypes = [vp, C.c_int, _D, _D, _D, _D]
+        L.psc_energy_densities_noalloc.argtypes = [vp, C.c_int, _D, _D, _D, _D]
         L.psc_spectra.argtypes = [vp, C.c_int, C.c_int, _D, C.c_int]
         L.psc_load_gw_k.argtypes = [vp, C.c_int, _D, _D]
         L.psc_generate_ic_device.argtypes = [vp, C.c_int, C.POINTER(PscIcParams), _D]
@@ -104,7 +105,7 @@ def exported_symbols():
             "psc_list_shapes", "psc_set_shape", "psc_adjoint", "psc_set_dt", "psc_means", "psc_load_fields",
             "psc_energy_densities", "psc_generate_ic_device", "psc_generate_gw_ic_device",
             "psc_model_name", "psc_model_description", "psc_num_potential_terms", "psc_potential_terms",
-            "psc_set_output_times", "psc_get_step_count"]
+            "psc_set_output_times", "psc_get_step_count", "psc_energy_densities_noalloc"]
 
 
 def generate_ic(cfg, model, ic, model_name=None):
@@ -233,6 +234,14 @@ class Lattice:
         N = self.N
         out = [np.zeros((N, N, N)) for _ in range(4)]
         self._check(self._L.psc_energy_densities(self._h, member, *[o.ravel() for o in out]))
+        return tuple(out)
+
+    def energy_densities_noalloc(self, member=0):
+        """The same densities through the library's fixed staging array (no
+        device allocation in the call)."""
+        N = self.N
+        out = [np.zeros((N, N, N)) for _ in range(4)]
+        self._check(self._L.psc_energy_densities_noalloc(self._h, member, *[o.ravel() for o in out]))
         return tuple(out)
 
     def energies(self, member=0):
