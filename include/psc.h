@@ -124,7 +124,12 @@ typedef struct psc_ic_params {
     int seed;          /* Park-Miller seed (parameters.h:62); < 1 is the reference's
                           debug mode where every draw returns 0.33 (initialize.cpp:15) */
     int reserved_i;
-    double reserved_d[4];
+    double kcutoff;    /* low-pass of the initial modes (parameters.h:31): modes with
+                          k^2 > kcutoff^2 are zero and take no draws, so the later
+                          modes take the draws they would have used
+                          (initialize.cpp:138-145, 281-294); 0, or a value of at
+                          least N kIR, cuts nothing                               */
+    double reserved_d[3];
 } psc_ic_params;
 
 /* ---- host-side utilities (no device needed) --------------------------- */

@@ -744,13 +744,30 @@ struct IcState {
     const Model* model;
     ParkMiller* rng;
     double a, hubble, norm;
+    double k2cutoff;
 };
+
+// k2cutoff of initialize.cpp:139 and :282: kcutoff^2 when kcutoff is below
+// 2 pi N / L, else 0 (no cut); kcutoff = 0 cuts nothing either.
+double k2cutoff_of(const psc_config* cfg, const psc_ic_params* ic) {
+    const double L = 2.0 * kPi / cfg->kIR;
+    return ic->kcutoff < 2.0 * kPi * (double)cfg->N / L ? pw2(ic->kcutoff) : 0.0;
+}
 
 // set_mode (initialize.cpp:128-220), SIMULATE_INFLATION == false branch.
 // norm = (omegaStar / fStar) (N / dx)^{3/2} / sqrt(2): the continuum vacuum
 // normalisation of the commented DeltaF_DeltaPi variant (parameters.h:132),
 // the same N^3 / L^{3/2} scaling as initialize.cpp:260-263.
+//
+// The low-pass of initialize.cpp:138-145: with k2cutoff > 0 a mode with
+// p2 > k2cutoff is set to zero before any draw, so the later modes take the
+// draws it would have used.
 void set_mode(IcState& s, double p2, double m2, cd* fk, cd* pk, bool isReal) {
+    if (s.k2cutoff > 0.0 && p2 > s.k2cutoff) {
+        *fk = cd(0.0, 0.0);
+        *pk = cd(0.0, 0.0);
+        return;
+    }
     double omega;
     if (p2 + m2 > 0.0)
         omega = std::sqrt(p2 + pw2(s.a) * m2);
@@ -968,7 +985,7 @@ int orc_generate_ic(const psc_config* cfg, const psc_model* model, const psc_ic_
     hubble_init(*cfg, ctx.model, *ic, &a, &ad, &addp, &H);
     ParkMiller rng(ic->seed);
     IcState s{cfg, &ctx.model, &rng, a, H,
-              cfg->omegaStar / cfg->fStar * std::pow(N / dx, 1.5) / std::sqrt(2.0)};
+              cfg->omegaStar / cfg->fStar * std::pow(N / dx, 1.5) / std::sqrt(2.0), k2cutoff_of(cfg, ic)};
     double f0[2] = {ic->f0[0], cfg->nflds > 1 ? ic->f0[1] : 0.0};
     double mass_sq[2];
     ctx.model.effective_mass(mass_sq, f0);
@@ -1020,9 +1037,14 @@ int orc_generate_gw_ic(const psc_config* cfg, const psc_model* model, const psc_
     double a, ad, addp, H;
     hubble_init(*cfg, m, *ic, &a, &ad, &addp, &H);
     ParkMiller rng(ic->seed);
-    // 4 draws per visited scalar mode, DC corner included (initialize.cpp:181-190).
+    const double k2cut = k2cutoff_of(cfg, ic);
+    // 4 draws per visited scalar mode below the cutoff, DC corner included
+    // (initialize.cpp:138-145, 181-190).
     long scalar_modes = 0;
-    visit_modes(N, true, [&](int, int, int, long, bool) { ++scalar_modes; });
+    visit_modes(N, true, [&](int i, int j, int k, long, bool) {
+        const double p2 = pw2(cfg->kIR) * (pw2(signed_index(i, N)) + pw2(signed_index(j, N)) + pw2(k));
+        if (!(k2cut > 0.0 && p2 > k2cut)) ++scalar_modes;
+    });
     for (long d = 0; d < 4 * scalar_modes * cfg->nflds; ++d) rng.uniform();
     const double norm = cfg->MPl * cfg->omegaStar / pw2(cfg->fStar) * std::pow(N / dx, 1.5);
     cd* u = reinterpret_cast<cd*>(u_out);
@@ -1034,6 +1056,7 @@ int orc_generate_gw_ic(const psc_config* cfg, const psc_model* model, const psc_
         int px = signed_index(i, N), py = signed_index(j, N);
         double kv[3] = {px * cfg->kIR, py * cfg->kIR, k * cfg->kIR};
         double k2 = pw2(cfg->kIR) * (pw2(px) + pw2(py) + pw2(k));
+        if (k2cut > 0.0 && k2 > k2cut) return;   // initialize.cpp:281-294: zero, no draws
         double e[3][3];
         tt_tensor(kv, e);
         double kk = std::sqrt(k2);

@@ -51,7 +51,7 @@ class PscModel(C.Structure):
 class PscIcParams(C.Structure):
     _fields_ = [
         ("f0", C.c_double * 2), ("fd0", C.c_double * 2), ("a0", C.c_double),
-        ("seed", C.c_int), ("reserved_i", C.c_int), ("reserved_d", C.c_double * 4),
+        ("seed", C.c_int), ("reserved_i", C.c_int), ("kcutoff", C.c_double), ("reserved_d", C.c_double * 3),
     ]
 
 
@@ -86,11 +86,11 @@ def make_model(m2=1.0, g2=0.0, lam=0.0, c=()):
     return m
 
 
-def make_ic(f0=(1.0, 0.0), fd0=(0.0, 0.0), a0=1.0, seed=1):
+def make_ic(f0=(1.0, 0.0), fd0=(0.0, 0.0), a0=1.0, seed=1, kcutoff=0.0):
     p = PscIcParams()
     p.f0[0], p.f0[1] = f0
     p.fd0[0], p.fd0[1] = fd0
-    p.a0, p.seed = a0, seed
+    p.a0, p.seed, p.kcutoff = a0, seed, kcutoff
     return p
 
 

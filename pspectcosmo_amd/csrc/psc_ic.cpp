@@ -67,32 +67,35 @@ class Rng {
 };
 
 
-using psc_ic::mode_count;
 using psc_ic::Pot;
 using psc_ic::background;
 
-// Visit the modes in that order; each mode consumes 4 draws from the stream
-// `rng` handed to fn(rng, i, j, k, conjugate index or -1, isReal).  The
-// stream starts at draw `first`.  The interior planes (the bulk) run on worker
-// threads, each (kz, i) row starting from its own skip-ahead position, which
-// leaves every mode with exactly the draws of the sequential order.
+// Visit the modes in that order; each mode below the cutoff consumes 4 draws
+// from the stream `rng` handed to fn(rng, i, j, k, conjugate index or -1,
+// isReal), and a cut mode is skipped (no call, no draws).  The stream starts
+// at draw `first`.  The interior planes (the bulk) run on worker threads,
+// each (kz, i) row starting from its own skip-ahead position (the kept modes
+// before it, from `kept`), which leaves every mode with exactly the draws of
+// the sequential order.
 template <class F>
-void for_each_mode(int N, bool dc, int seed, uint64_t first, F&& fn) {
+void for_each_mode(int N, bool dc, int seed, uint64_t first, const psc_ic::KeptIndex& kept, F&& fn) {
     const int Nh = N / 2 + 1;
     auto at = [&](int i, int j, int k) { return ((long)i * N + j) * Nh + k; };
+    auto visit = [&](Rng& r, int i, int j, int k, long conj, bool real) {
+        const int64_t px = sidx(i, N), py = sidx(j, N);
+        if (kept.cut.keeps(px * px + py * py + (int64_t)k * k)) fn(r, i, j, k, conj, real);
+    };
     Rng rng = Rng::at(seed, first);
-    uint64_t v = 0;
     for (int k = 0; k <= N / 2; k += N / 2) {
         for (int i = 0; i <= N / 2; i += N / 2)
-            for (int j = 1; j < N / 2; ++j, ++v) fn(rng, i, j, k, at(i, N - j, k), false);
+            for (int j = 1; j < N / 2; ++j) visit(rng, i, j, k, at(i, N - j, k), false);
         for (int j = 0; j <= N / 2; j += N / 2)
-            for (int i = 1; i < N / 2; ++i, ++v) fn(rng, i, j, k, at(N - i, j, k), false);
+            for (int i = 1; i < N / 2; ++i) visit(rng, i, j, k, at(N - i, j, k), false);
         for (int i = 1; i < N / 2; ++i) {
-            for (int j = 1; j < N / 2; ++j, ++v) fn(rng, i, j, k, at(N - i, N - j, k), false);
-            for (int j = N / 2 + 1; j < N; ++j, ++v) fn(rng, i, j, k, at(N - i, N - j, k), false);
+            for (int j = 1; j < N / 2; ++j) visit(rng, i, j, k, at(N - i, N - j, k), false);
+            for (int j = N / 2 + 1; j < N; ++j) visit(rng, i, j, k, at(N - i, N - j, k), false);
         }
     }
-    const uint64_t v0 = v;
     const long rows = (long)(N / 2 - 1) * N;  // (kz, i) rows of N modes each
     unsigned nt = std::max(1u, std::min(32u, std::thread::hardware_concurrency()));
     if (rows < 256) nt = 1;
@@ -101,17 +104,17 @@ void for_each_mode(int N, bool dc, int seed, uint64_t first, F&& fn) {
         pool.emplace_back([&, w] {
             for (long r = w; r < rows; r += nt) {
                 const int k = 1 + (int)(r / N), i = (int)(r % N);
-                Rng rr = Rng::at(seed, first + 4 * (v0 + (uint64_t)r * N));
-                for (int j = 0; j < N; ++j) fn(rr, i, j, k, -1L, false);
+                Rng rr = Rng::at(seed, first + 4 * (kept.planes + kept.rowpre[r]));
+                for (int j = 0; j < N; ++j) visit(rr, i, j, k, -1L, false);
             }
         });
     for (auto& t : pool) t.join();
-    Rng rc = Rng::at(seed, first + 4 * (v0 + (uint64_t)rows * N));
+    Rng rc = Rng::at(seed, first + 4 * (kept.planes + kept.interior));
     for (int k = 0; k <= N / 2; k += N / 2)
         for (int i = 0; i <= N / 2; i += N / 2)
             for (int j = 0; j <= N / 2; j += N / 2) {
                 if (!dc && i == 0 && j == 0 && k == 0) continue;
-                fn(rc, i, j, k, -1L, true);
+                visit(rc, i, j, k, -1L, true);
             }
 }
 
@@ -141,12 +144,16 @@ extern "C" int psc_generate_ic(const psc_config* cfg, const psc_model* model, co
     V.meff(meff, f0);
     const double dp2 = pw2(cfg->kIR), ap = std::pow(a, cfg->alpha - 3);
     const double N3 = (double)N * N * N;
-    const uint64_t M = mode_count(N, true);
+    psc_ic::KeptIndex kept;
+    std::vector<uint64_t> rowpre;
+    const uint64_t M = psc_ic::build_kept_index(N, true, psc_ic::make_cut(*cfg, ic->kcutoff), kept, rowpre);
     for (int q = 0; q < cfg->nflds; ++q) {
         double* fq = f_k + 2 * nk * q;
         double* pq = pi_k + 2 * nk * q;
-        // four draws per mode (initialize.cpp:181-190), fields in sequence
-        for_each_mode(N, true, ic->seed, 4 * (uint64_t)q * M, [&](Rng& rng, int i, int j, int k, long conj, bool real) {
+        std::memset(fq, 0, sizeof(double) * 2 * nk);   // the cut modes stay zero
+        std::memset(pq, 0, sizeof(double) * 2 * nk);
+        // four draws per kept mode (initialize.cpp:181-190), fields in sequence
+        for_each_mode(N, true, ic->seed, 4 * (uint64_t)q * M, kept, [&](Rng& rng, int i, int j, int k, long conj, bool real) {
             const int px = sidx(i, N), py = sidx(j, N);
             const double p2 = dp2 * (pw2(px) + pw2(py) + pw2(k));
             const double om = (p2 + meff[q] > 0.0) ? std::sqrt(p2 + pw2(a) * meff[q]) : std::sqrt(p2);
@@ -193,14 +200,18 @@ extern "C" int psc_generate_gw_ic(const psc_config* cfg, const psc_model* model,
     const Pot V(cfg->nflds, *model);
     double a, ad, addp, H;
     background(*cfg, V, *ic, &a, &ad, &addp, &H);
-    // The GW draws follow the 4 draws of every scalar mode of every field.
-    const uint64_t skip = 4 * mode_count(N, true) * (uint64_t)cfg->nflds;
+    // The GW draws follow the 4 draws of every kept scalar mode of every field.
+    const psc_ic::Cut cut = psc_ic::make_cut(*cfg, ic->kcutoff);
+    psc_ic::KeptIndex kept_f, kept;
+    std::vector<uint64_t> rowpre_f, rowpre;
+    const uint64_t skip = 4 * psc_ic::build_kept_index(N, true, cut, kept_f, rowpre_f) * (uint64_t)cfg->nflds;
+    psc_ic::build_kept_index(N, false, cut, kept, rowpre);
     const double L = 2.0 * kPi / cfg->kIR, dx = L / N;
     const double norm = cfg->MPl * cfg->omegaStar / pw2(cfg->fStar) * std::pow(N / dx, 1.5);
     const double ap = std::pow(a, cfg->alpha - 3);
     std::memset(u_k, 0, sizeof(double) * 2 * nk * 6);
     std::memset(pi_u_k, 0, sizeof(double) * 2 * nk * 6);
-    for_each_mode(N, false, ic->seed, skip, [&](Rng& rng, int i, int j, int k, long conj, bool real) {
+    for_each_mode(N, false, ic->seed, skip, kept, [&](Rng& rng, int i, int j, int k, long conj, bool real) {
         const int px = sidx(i, N), py = sidx(j, N);
         const double kv[3] = {px * cfg->kIR, py * cfg->kIR, k * cfg->kIR};
         double e[3][3];

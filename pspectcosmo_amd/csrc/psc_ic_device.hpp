@@ -23,7 +23,15 @@ struct IcConsts {
     double norm, dp2, a, ap, H, N3;
     double meff[2], f0[2], fd0[2];
     double kIR;
+    psc_ic::KeptIndex kept;        // the kcutoff low-pass; kept.rowpre is a device table
 };
+
+// Draw offset / 4 of mode m (work item t): its visiting position, or with a
+// cutoff the number of kept modes visited before it.
+__device__ __forceinline__ unsigned long long ic_rank(const IcConsts& p, bool dc, const psc_ic::Mode& m,
+                                                      unsigned long long t) {
+    return p.kept.cut.on() ? p.kept.rank(p.N, dc, m, t) : m.v;
+}
 
 __device__ __forceinline__ int ic_sidx(int i, int N) { return i <= N / 2 ? i : i - N; }
 
@@ -61,13 +69,21 @@ k_ic_fields(cpx<T>* __restrict__ Pf, cpx<T>* __restrict__ Pp, IcConsts p, size_t
     const int N = p.N;
     const psc_ic::Mode m = psc_ic::mode_at(N, true, t);
     const int px = ic_sidx(m.i, N), py = ic_sidx(m.j, N);
+    const size_t at = (size_t)m.i * PS + (size_t)m.j * N + m.k;
+    const size_t cj = (size_t)((N - m.i) % N) * PS + (size_t)((N - m.j) % N) * N + (N - m.k) % N;
+    if (!p.kept.cut.keeps((int64_t)px * px + (int64_t)py * py + (int64_t)m.k * m.k)) {   // cut: zero, no draws
+        Pf[at] = Pp[at] = cpx<T>{(T)0, (T)0};
+        if (!m.real) Pf[cj] = Pp[cj] = cpx<T>{(T)0, (T)0};
+        return;
+    }
+    const unsigned long long rank = ic_rank(p, true, m, t);
     const double p2 = p.dp2 * ((double)px * (double)px + (double)py * (double)py + (double)m.k * (double)m.k);
     double fr[2] = {0.0, 0.0}, fi[2] = {0.0, 0.0}, pr[2] = {0.0, 0.0}, pim[2] = {0.0, 0.0};
     for (int q = 0; q < p.nflds; ++q) {
         const double om = (p2 + p.meff[q] > 0.0) ? sqrt(p2 + (p.a * p.a) * p.meff[q]) : sqrt(p2);
         const double rms = om > 0.0 ? p.norm / sqrt(om) : 0.0;
         double rl, il, rr, ir;
-        ic_amplitudes(psc_ic::Stream::at(p.seed, p.first[q] + 4 * m.v), rms, p.a, rl, il, rr, ir);
+        ic_amplitudes(psc_ic::Stream::at(p.seed, p.first[q] + 4 * rank), rms, p.a, rl, il, rr, ir);
         fr[q] = rl + rr;
         fi[q] = il + ir;
         pr[q] = p.ap * (-om * (il - ir) - p.H * fr[q]);
@@ -81,11 +97,9 @@ k_ic_fields(cpx<T>* __restrict__ Pf, cpx<T>* __restrict__ Pp, IcConsts p, size_t
             fi[q] = pim[q] = 0.0;
         }
     }
-    const size_t at = (size_t)m.i * PS + (size_t)m.j * N + m.k;
     Pf[at] = cpx<T>{(T)(fr[0] - fi[1]), (T)(fi[0] + fr[1])};
     Pp[at] = cpx<T>{(T)(pr[0] - pim[1]), (T)(pim[0] + pr[1])};
     if (!m.real) {
-        const size_t cj = (size_t)((N - m.i) % N) * PS + (size_t)((N - m.j) % N) * N + (N - m.k) % N;
         Pf[cj] = cpx<T>{(T)(fr[0] + fi[1]), (T)(fr[1] - fi[0])};
         Pp[cj] = cpx<T>{(T)(pr[0] + pim[1]), (T)(pr[1] - pim[0])};
     }
@@ -105,13 +119,15 @@ k_ic_gw(cpx<T>* __restrict__ U, cpx<T>* __restrict__ PU, size_t astride, IcConst
     const int N = p.N;
     const psc_ic::Mode m = psc_ic::mode_at(N, false, t);
     const int px = ic_sidx(m.i, N), py = ic_sidx(m.j, N);
+    // a cut mode stays zero (the arrays are cleared before the launch) and takes no draws
+    if (!p.kept.cut.keeps((int64_t)px * px + (int64_t)py * py + (int64_t)m.k * m.k)) return;
     const double kv[3] = {px * p.kIR, py * p.kIR, m.k * p.kIR};
     double e[3][3];
     psc_ic::polarisation(kv, e);
     const double kk = sqrt((p.kIR * p.kIR) * ((double)px * (double)px + (double)py * (double)py + (double)m.k * (double)m.k));
     const double rms = p.norm / sqrt(kk);
     double rl, il, rr, ir;
-    ic_amplitudes(psc_ic::Stream::at(p.seed, p.first[0] + 4 * m.v), rms, p.a, rl, il, rr, ir);
+    ic_amplitudes(psc_ic::Stream::at(p.seed, p.first[0] + 4 * ic_rank(p, false, m, t)), rms, p.a, rl, il, rr, ir);
     const int ci[6] = {0, 1, 2, 0, 0, 1}, cj[6] = {0, 1, 2, 1, 2, 2};
     double ur[6], ui[6], pr[6], pim[6];
     for (int c = 0; c < 6; ++c) {

@@ -16,6 +16,7 @@
 
 #include <cmath>
 #include <cstdint>
+#include <vector>
 
 #include "../../include/psc.h"
 
@@ -98,6 +99,135 @@ PSC_HD inline Mode mode_at(int N, bool dc, uint64_t t) {
     m.j = c & 1 ? N / 2 : 0;
     m.real = true;
     return m;
+}
+
+// The low-pass of the initial modes (kcutoff, initialize.cpp:138-145 and
+// 281-294): a mode with dp2 * s > k2cutoff, s = px^2 + py^2 + pz^2, is zero
+// and takes no draws.  s is an integer (exact in double) and dp2 * s rounds
+// monotonically in s, so the cut is s > smax for the largest s with
+// dp2 * s <= k2cutoff, and the number of kept modes before any mode (its
+// draw offset / 4) is integer counting: in closed form along a row, through
+// a table of row offsets over the interior (the host fills it).  smax < 0:
+// nothing is cut.
+struct Cut {
+    int64_t smax = -1;
+    PSC_HD bool on() const { return smax >= 0; }
+    PSC_HD bool keeps(int64_t s) const { return smax < 0 || s <= smax; }
+    // integers y in [lo, hi] with y^2 <= R (all of them without a cut)
+    PSC_HD int64_t count(int64_t R, int64_t lo, int64_t hi) const {
+        if (hi < lo) return 0;
+        if (smax < 0) return hi - lo + 1;
+        if (R < 0) return 0;
+        int64_t r = (int64_t)sqrt((double)R);
+        while (r * r > R) --r;
+        while ((r + 1) * (r + 1) <= R) ++r;
+        const int64_t a = lo > -r ? lo : -r, b = hi < r ? hi : r;
+        return b >= a ? b - a + 1 : 0;
+    }
+};
+
+// k2cutoff = kcutoff^2 when kcutoff < 2 pi N / L (L = 2 pi / kIR), else no
+// cut (initialize.cpp:139, 282); kcutoff <= 0 cuts nothing either.
+inline Cut make_cut(const psc_config& c, double kcutoff) {
+    const double pi = 3.14159265358979323846;
+    const double L = 2.0 * pi / c.kIR;
+    const double k2 = kcutoff < 2.0 * pi * (double)c.N / L ? kcutoff * kcutoff : 0.0;
+    Cut cut;
+    if (!(k2 > 0.0)) return cut;
+    const double dp2 = c.kIR * c.kIR;
+    const int64_t top = 3 * (int64_t)(c.N / 2) * (c.N / 2);
+    int64_t s = (int64_t)(k2 / dp2);
+    if (s > top) s = top;
+    while (s < top && dp2 * (double)(s + 1) <= k2) ++s;
+    while (s >= 0 && dp2 * (double)s > k2) --s;
+    cut.smax = s < 0 ? 0 : s;   // s = 0 (the DC mode) always passes: 0 <= k2
+    return cut;
+}
+
+// Kept modes among the first r visits of the real plane kz (r <= 4h + 2h^2,
+// the plane part of mode_at's order).
+PSC_HD inline uint64_t plane_kept_before(int N, const Cut& cut, int kz, uint64_t r) {
+    const int64_t h = N / 2 - 1, K = (int64_t)kz * kz;
+    auto clampi = [](int64_t v, int64_t lo, int64_t hi) { return v < lo ? lo : (v > hi ? hi : v); };
+    uint64_t n = 0;
+    for (int a = 0; a < 2; ++a) {   // i = 0, N/2; j = 1..h
+        const int64_t ii = a ? N / 2 : 0;
+        n += cut.count(cut.smax - ii * ii - K, 1, clampi((int64_t)r - a * h, 0, h));
+    }
+    if ((int64_t)r <= 2 * h) return n;
+    for (int a = 0; a < 2; ++a) {   // j = 0, N/2; i = 1..h
+        const int64_t jj = a ? N / 2 : 0;
+        n += cut.count(cut.smax - jj * jj - K, 1, clampi((int64_t)r - 2 * h - a * h, 0, h));
+    }
+    if ((int64_t)r <= 4 * h) return n;
+    const int64_t q = (int64_t)r - 4 * h, full = q / (2 * h), rem = q % (2 * h);
+    for (int64_t i = 1; i <= full; ++i) {   // j = 1..h (py = j), then py = -h..-1
+        const int64_t R = cut.smax - i * i - K;
+        n += cut.count(R, 1, h) + cut.count(R, -h, -1);
+    }
+    if (rem > 0) {
+        const int64_t i = full + 1, R = cut.smax - i * i - K;
+        n += rem <= h ? cut.count(R, 1, rem) : cut.count(R, 1, h) + cut.count(R, -h, -h + (rem - h) - 1);
+    }
+    return n;
+}
+
+// Kept modes of the interior row (kz, i), and of its first j visits.
+PSC_HD inline uint64_t row_kept_before(int N, const Cut& cut, int kz, int i, int j) {
+    const int64_t px = i <= N / 2 ? i : i - N;
+    const int64_t R = cut.smax - px * px - (int64_t)kz * kz;
+    if (j <= N / 2 + 1) return cut.count(R, 0, j - 1);
+    return cut.count(R, 0, N / 2) + cut.count(R, -(N / 2 - 1), (int64_t)j - 1 - N);
+}
+
+// The draw-offset / 4 of every mode (the number of kept modes visited before
+// it) for mode_at's order: planes, interior rows through `rowpre`, corners.
+struct KeptIndex {
+    Cut cut;
+    uint64_t plane0 = 0;     // kept modes of the kz = 0 plane
+    uint64_t planes = 0;     // of both planes
+    uint64_t interior = 0;   // of the interior
+    const uint64_t* rowpre = nullptr;   // [(kz - 1) N + i]: kept interior modes before the row
+    PSC_HD uint64_t rank(int N, bool dc, const Mode& m, uint64_t t) const {
+        const int h = N / 2 - 1;
+        const uint64_t S = 4 * (uint64_t)h + 2 * (uint64_t)h * h;
+        if (t < 2 * S) return (t < S ? 0 : plane0) + plane_kept_before(N, cut, m.k, t % S);
+        if (t < 2 * S + (uint64_t)h * N * N)
+            return planes + rowpre[(size_t)(m.k - 1) * N + m.i] + row_kept_before(N, cut, m.k, m.i, m.j);
+        uint64_t n = planes + interior;
+        for (int c = dc ? 0 : 1; c < 8; ++c) {
+            const int k = (c >> 2) & 1 ? N / 2 : 0, i = (c >> 1) & 1 ? N / 2 : 0, j = c & 1 ? N / 2 : 0;
+            if (k == m.k && i == m.i && j == m.j) break;
+            if (cut.keeps((int64_t)k * k + (int64_t)i * i + (int64_t)j * j)) ++n;
+        }
+        return n;
+    }
+};
+
+// Fills the interior row table and the plane/interior totals; returns the
+// number of kept modes in the whole visit (with or without the DC corner).
+inline uint64_t build_kept_index(int N, bool dc, const Cut& cut, KeptIndex& k, std::vector<uint64_t>& rowpre) {
+    const int h = N / 2 - 1;
+    const uint64_t S = 4 * (uint64_t)h + 2 * (uint64_t)h * h;
+    k.cut = cut;
+    k.plane0 = plane_kept_before(N, cut, 0, S);
+    k.planes = k.plane0 + plane_kept_before(N, cut, N / 2, S);
+    rowpre.assign((size_t)h * N, 0);
+    uint64_t acc = 0;
+    for (int kz = 1; kz <= h; ++kz)
+        for (int i = 0; i < N; ++i) {
+            rowpre[(size_t)(kz - 1) * N + i] = acc;
+            acc += row_kept_before(N, cut, kz, i, N);
+        }
+    k.interior = acc;
+    k.rowpre = rowpre.data();
+    uint64_t corners = 0;
+    for (int c = dc ? 0 : 1; c < 8; ++c) {
+        const int64_t v = N / 2;
+        const int64_t s = ((c >> 2) & 1 ? v * v : 0) + ((c >> 1) & 1 ? v * v : 0) + (c & 1 ? v * v : 0);
+        if (cut.keeps(s)) ++corners;
+    }
+    return k.planes + k.interior + corners;
 }
 
 // rand_uniform (initialize.cpp:13-28) positioned at draw d (0-based): the

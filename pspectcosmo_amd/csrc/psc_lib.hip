@@ -723,7 +723,13 @@ int ic_setup(psc_ctx* c, int member, const psc_ic_params* ic, bool gw, IcConsts*
     k->N3 = (double)N * N * N;
     k->kIR = cfg.kIR;
     k->dp2 = cfg.kIR * cfg.kIR;
-    const unsigned long long M = psc_ic::mode_count(N, true);
+    // kept modes per field (all of them without a kcutoff) and, with a cut,
+    // the interior row offsets, copied into W for the kernel to read
+    const psc_ic::Cut cut = psc_ic::make_cut(cfg, ic->kcutoff);
+    psc_ic::KeptIndex kf;
+    std::vector<uint64_t> rowpre_f, rowpre_gw;
+    const unsigned long long M = psc_ic::build_kept_index(N, true, cut, kf, rowpre_f);
+    const std::vector<uint64_t>* rowpre = &rowpre_f;
     if (!gw) {
         k->norm = cfg.omegaStar / cfg.fStar * std::pow(N / dx, 1.5) / std::sqrt(2.0);
         const double f0[2] = {ic->f0[0], cfg.nflds > 1 ? ic->f0[1] : 0.0};
@@ -733,9 +739,19 @@ int ic_setup(psc_ctx* c, int member, const psc_ic_params* ic, bool gw, IcConsts*
             k->f0[q] = ic->f0[q];
             k->fd0[q] = ic->fd0[q];
         }
+        k->kept = kf;
     } else {
         k->norm = cfg.MPl * cfg.omegaStar / (cfg.fStar * cfg.fStar) * std::pow(N / dx, 1.5);
-        k->first[0] = 4 * M * (unsigned long long)cfg.nflds;   // after every scalar mode's draws
+        k->first[0] = 4 * M * (unsigned long long)cfg.nflds;   // after every kept scalar mode's draws
+        psc_ic::build_kept_index(N, false, cut, k->kept, rowpre_gw);
+        rowpre = &rowpre_gw;
+    }
+    k->kept.rowpre = nullptr;
+    if (cut.on()) {   // W is free between steps; the table is N (N/2 - 1) offsets
+        HIPCHK(c, hipMemcpyAsync(c->W, rowpre->data(), rowpre->size() * sizeof(uint64_t), hipMemcpyHostToDevice,
+                                 c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        k->kept.rowpre = static_cast<const uint64_t*>(c->W);
     }
     sc[0] = a;
     sc[1] = ad;

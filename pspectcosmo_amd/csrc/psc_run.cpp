@@ -99,6 +99,7 @@ const char* kKeys[][2] = {
     {"initfield", "homogeneous field values, comma separated"},
     {"initderivs", "homogeneous field derivatives, comma separated"},
     {"seed", "Park-Miller seed"},
+    {"kcutoff", "low-pass of the initial modes: k above it start at zero (0: none)"},
     {"ic", "host (the host generator) or device (the same modes generated on the GPU)"},
     {"with_gw", "1: evolve the 6 GW components"},
     {"gw_projection", "ref (the reference's one-sided projection) or tt"},
@@ -479,6 +480,7 @@ int main(int argc, char** argv) {
     }
     r.ic.a0 = 1.0;
     r.ic.seed = (int)p.num("seed", 1);
+    r.ic.kcutoff = p.num("kcutoff", 0.0);
     r.dir = p.get("dir", "output");
     r.gw_proj = p.get("gw_projection", "ref") == "tt" ? PSC_GW_PROJ_TT : PSC_GW_PROJ_REFERENCE;
     const std::string ic_where = p.get("ic", "host");

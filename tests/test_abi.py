@@ -83,3 +83,22 @@ def test_create_rejects_bad_config():
     cfg = make_config(48)  # not a power of two
     h = C.c_void_p()
     assert L.lib().psc_create(C.byref(cfg), C.byref(h)) == 1
+
+
+@pytest.mark.parametrize("N,kcutoff", [(16, 150.0), (32, 300.0), (64, 800.0)])
+def test_host_ic_with_kcutoff_matches_oracle_bitwise(N, kcutoff):
+    """The kcutoff low-pass (initialize.cpp:138-145): the host generator's
+    row-parallel skip-ahead over the kept modes against the oracle's
+    sequential restatement, scalar fields and GW modes."""
+    cfg, model, ic = parity_setup(N=N, nflds=2, seed=9, with_gw=1)
+    ic.kcutoff = kcutoff
+    fk, pk, sc = L.generate_ic(cfg, model, ic)
+    ofk, opk, osc = O.generate_ic(cfg, model, ic)
+    cut = np.count_nonzero(ofk[0] == 0)
+    assert 0.2 * ofk[0].size < cut < 0.9 * ofk[0].size   # kIR = 25: a real part of the modes is cut
+    np.testing.assert_array_equal(fk, ofk)
+    np.testing.assert_array_equal(pk, opk)
+    u, pu = L.generate_gw_ic(cfg, model, ic)
+    ou, opu = O.generate_gw_ic(cfg, model, ic)
+    np.testing.assert_array_equal(u, ou)
+    np.testing.assert_array_equal(pu, opu)

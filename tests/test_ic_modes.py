@@ -23,9 +23,28 @@ PROG = r'''
 #include "psc_ic_modes.hpp"
 #include <cstdio>
 #include <cstdlib>
+#include <string>
 int main(int argc, char** argv) {
     const int N = std::atoi(argv[1]);
     const bool dc = std::atoi(argv[2]) != 0;
+    if (argc > 3 && std::string(argv[3]) == "rank") {   // rank mode: kIR, kcutoff
+        psc_config c{};
+        c.N = N;
+        c.kIR = std::atof(argv[4]);
+        const psc_ic::Cut cut = psc_ic::make_cut(c, std::atof(argv[5]));
+        psc_ic::KeptIndex kept;
+        std::vector<uint64_t> rowpre;
+        const unsigned long long total = psc_ic::build_kept_index(N, dc, cut, kept, rowpre);
+        std::printf("%lld %llu\n", (long long)cut.smax, total);
+        const unsigned long long n = psc_ic::mode_count(N, dc);
+        for (unsigned long long t = 0; t < n; ++t) {
+            const psc_ic::Mode m = psc_ic::mode_at(N, dc, t);
+            const long long px = m.i <= N / 2 ? m.i : m.i - N, py = m.j <= N / 2 ? m.j : m.j - N;
+            std::printf("%llu %llu %d\n", m.v, (unsigned long long)kept.rank(N, dc, m, t),
+                        cut.keeps(px * px + py * py + (long long)m.k * m.k) ? 1 : 0);
+        }
+        return 0;
+    }
     if (argc > 3) {   // stream mode: seed, then draw positions
         const int seed = std::atoi(argv[3]);
         for (int a = 4; a < argc; ++a) {
@@ -125,3 +144,36 @@ def test_stream_positions_match_the_sequential_stream(prog):
     assert float(far[0]) == seq7[1000000] and float(far[1]) == seq7[1000001]
     dbg = subprocess.check_output([prog, "16", "1", "0", "5"], text=True).split()
     assert float(dbg[0]) == 0.33 and float(dbg[1]) == 0.33
+
+
+@pytest.mark.parametrize("N,dc,kIR,kcutoff", [(16, 1, 1.0, 5.0), (16, 0, 1.0, 5.0), (32, 1, 0.5, 4.3),
+                                              (32, 0, 0.5, 4.3), (16, 1, 1.0, 0.0), (16, 1, 1.0, 100.0),
+                                              (16, 1, 1.0, 0.5)])
+def test_kept_rank_counts_the_kept_modes_before_each(prog, N, dc, kIR, kcutoff):
+    """With the kcutoff low-pass every kept mode's draw offset / 4 is the
+    number of kept modes before it in the visiting order (initialize.cpp:
+    138-145 returns before drawing), here counted by brute force over the loop
+    nest, with the cut decided in double exactly as the reference does:
+    kIR^2 (px^2 + py^2 + pz^2) > kcutoff^2, for kcutoff < 2 pi N / L."""
+    out = subprocess.check_output([prog, str(N), str(dc), "rank", repr(kIR), repr(kcutoff)], text=True).splitlines()
+    smax, total = (int(x) for x in out[0].split())
+    rows = np.loadtxt(out[1:], dtype=np.int64)
+    ref = reference_order(N, dc)
+    L = 2.0 * np.pi / kIR
+    k2cut = kcutoff * kcutoff if kcutoff < 2.0 * np.pi * N / L else 0.0
+    sidx = lambda i: i if i <= N // 2 else i - N
+    keep = []
+    for (i, j, k, *_rest) in ref:
+        p2 = (kIR * kIR) * float(sidx(i) ** 2 + sidx(j) ** 2 + k * k)
+        keep.append(not (k2cut > 0 and p2 > k2cut))
+    keep = np.array(keep)
+    before = np.concatenate([[0], np.cumsum(keep)[:-1]])
+    v = rows[:, 0]
+    assert np.array_equal(rows[:, 2].astype(bool), keep[v])
+    kept_rows = rows[rows[:, 2] == 1]
+    assert np.array_equal(kept_rows[:, 1], before[kept_rows[:, 0]])
+    assert total == int(keep.sum())
+    if k2cut == 0:
+        assert smax == -1 and total == len(ref)
+    else:
+        assert 0 < total < len(ref) or kcutoff < kIR

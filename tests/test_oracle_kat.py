@@ -181,3 +181,78 @@ def test_fftw_api_backend_agrees_with_radix2():
         res.append(o.get_k())
     O.set_fft_backend(0)
     assert rel_err(res[1][0], res[0][0]) < 1e-13 and rel_err(res[1][1], res[0][1]) < 1e-13
+
+
+def _visit_order(N):
+    """The loop nest of initialize() (initialize.cpp:685-775), DC corner
+    included: (i, j, k, real)."""
+    h = N // 2
+    out = []
+    for k in (0, h):
+        for i in (0, h):
+            out += [(i, j, k, False) for j in range(1, h)]
+        for j in (0, h):
+            out += [(i, j, k, False) for i in range(1, h)]
+        for i in range(1, h):
+            out += [(i, j, k, False) for j in range(1, h)]
+            out += [(i, j, k, False) for j in range(h + 1, N)]
+    for k in range(1, h):
+        out += [(i, j, k, False) for i in range(N) for j in range(N)]
+    out += [(i, j, k, True) for k in (0, h) for i in (0, h) for j in (0, h)]
+    return out
+
+
+@pytest.mark.parametrize("kcutoff", [0.0, 5.0, 3.2])
+def test_initial_modes_decode_to_the_park_miller_stream(kcutoff):
+    """Each generated mode is f = L + R, pi = ap (-om Im(L - R) - H Re f) +
+    i ap (om Re(L - R) - H Im f) with L = al e^{2 pi i u3}, R = ar e^{2 pi i u4},
+    a_{l,r} = rms sqrt(log(1/u_{1,2})) / (sqrt 2 a) (set_mode,
+    initialize.cpp:146-220).  Inverting that for every complex mode of both
+    fields recovers the four uniforms it used; they must be the Park-Miller
+    stream (pinned above) taken four at a time in the visiting order, field
+    after field, skipping the modes above kcutoff, which are zero
+    (initialize.cpp:138-145).  A wrong visiting order, a cut mode that still
+    consumes draws, or a wrong amplitude or phase convention fails here."""
+    N, kIR = 16, 1.0
+    cfg, model, ic = parity_setup(N=N, kIR=kIR, omega_star=1e-2, g2=50.0, seed=4242, f0=(1.0, 0.3),
+                                  fd0=(-0.7, 0.1))
+    ic.kcutoff = kcutoff
+    fk, pk, sc = O.generate_ic(cfg, model, ic)
+    a, H = sc[0], sc[3]
+    ap = a ** (cfg.alpha - 3)
+    dx = (2 * math.pi / kIR) / N
+    norm = cfg.omegaStar / cfg.fStar * (N / dx) ** 1.5 / math.sqrt(2.0)
+    meff = O.model_eval(model, 2, [1.0, 0.3])["meff"]
+    order = _visit_order(N)
+    sidx = lambda i: i if i <= N // 2 else i - N
+    k2cut = kcutoff ** 2 if 0 < kcutoff < N * kIR else 0.0
+    kept = [not (k2cut > 0 and kIR ** 2 * (sidx(i) ** 2 + sidx(j) ** 2 + k * k) > k2cut) for i, j, k, _ in order]
+    per_field = sum(kept)
+    if kcutoff > 0:
+        assert per_field < len(order)
+    draws = O.rng_draws(ic.seed, 4 * per_field * 2)
+    checked = 0
+    for q in range(2):
+        n = 0
+        for (i, j, k, real), keep in zip(order, kept):
+            f, p = fk[q, i, j, k], pk[q, i, j, k]
+            if not keep:
+                assert f == 0 and p == 0
+                continue
+            d = 4 * (q * per_field + n)
+            n += 1
+            if real:
+                continue   # imaginary parts zeroed: not invertible (the DC is overwritten anyway)
+            p2 = kIR ** 2 * (sidx(i) ** 2 + sidx(j) ** 2 + k * k)
+            om = math.sqrt(p2 + a * a * meff[q]) if p2 + meff[q] > 0 else math.sqrt(p2)
+            rms = norm / math.sqrt(om)
+            D = complex((p.imag / ap + H * f.imag) / om, -(p.real / ap + H * f.real) / om)
+            Lm, Rm = (f + D) / 2, (f - D) / 2
+            u = [math.exp(-(abs(Lm) * a * math.sqrt(2.0) / rms) ** 2),
+                 math.exp(-(abs(Rm) * a * math.sqrt(2.0) / rms) ** 2),
+                 (np.angle(Lm) / (2 * math.pi)) % 1.0, (np.angle(Rm) / (2 * math.pi)) % 1.0]
+            for m in range(4):
+                e = abs(u[m] - draws[d + m])
+                assert min(e, 1 - e) < 1e-9, (q, i, j, k, m, u[m], draws[d + m])
+            checked += 1
+    assert checked > 2 * 0.5 * per_field
