@@ -110,6 +110,13 @@ struct Shape {
     static constexpr size_t PS = (size_t)N * (N + 1);
     static constexpr size_t MS = (size_t)N * PS;      // member stride
     static constexpr int BLOCKS_COL = (N / TC) * N;   // per member
+    // Most waves per SIMD k_x_force may run at.  At N <= 256 in FP32 the cap
+    // of 3 lets the compiler keep more of the two transforms in registers
+    // (130 VGPRs instead of 114): measured 11-12 % faster x passes for the
+    // 16-column shapes and 7 % for r16.8_tc32 at 128, no change for the
+    // others; FP64 and N = 512 are unchanged by any cap (and at 512 a cap
+    // would cost the second block per CU), so they keep none (8).
+    static constexpr int XF_WAVES = (sizeof(T_) == 4 && N_ <= 256) ? 3 : 8;
     static constexpr int BLOCKS_ROW = (N * N) / L;    // per member
     using LayRow = LayoutRow<N, R1>;
     using LayCol = LayoutCol<N, TC>;

@@ -164,7 +164,7 @@ k_strided(cpx<typename G::T>* __restrict__ W, const cpx<typename G::T>* __restri
 // PADIN (the dealiased stepper on its 2N grid): the x rows |x| > N/4 of the
 // spectrum are zero and are neither read nor written.
 template <class G, bool REC = false, bool PADIN = false>
-__global__ void __launch_bounds__((G::THREADS_COL))
+__global__ void __launch_bounds__((G::THREADS_COL)) __attribute__((amdgpu_waves_per_eu(1, G::XF_WAVES)))
 k_x_force(cpx<typename G::T>* __restrict__ W, const MemberState* __restrict__ ms, const cpx<typename G::T>* __restrict__ tw,
           double* __restrict__ partV, const cpx<typename G::T>* __restrict__ Pf,
           cpx<typename G::T>* __restrict__ rec_r = nullptr) {
