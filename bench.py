@@ -326,14 +326,15 @@ PHI0, DPHI0 = 1.0, -0.7
 _ADJ_IC = {}
 
 
-def adjoint_case(precision, phi0=PHI0, g2=ADJ_G2, dchi=None, tuned=False, gw=False):
+def adjoint_case(precision, phi0=PHI0, g2=ADJ_G2, dchi=None, tuned=False, gw=False, graph=True):
     """Context loaded with the adjoint workload's initial state.  The
     fluctuations are generated once (with g^2 = 1e4) and held fixed, as the
     gradient holds them: g2 changes only the coupling, phi0 sets the DC mode
     and da/dt = a H(phi0), dchi (real space) is added to chi."""
     n = ADJ_N
     cfg = make_config(n, nflds=2, integrator=2, precision=precision, kIR=ADJ_KIR, dt=ADJ_DT,
-                      omegaStar=ADJ_OMEGA, use_graph=1, autotune=1 if tuned else 0, with_gw=1 if gw else 0)
+                      omegaStar=ADJ_OMEGA, use_graph=1 if graph else 0, autotune=1 if tuned else 0,
+                      with_gw=1 if gw else 0)
     if not _ADJ_IC:
         ic = make_ic(f0=(PHI0, 0.0), fd0=(DPHI0, 0.0), seed=1)
         _ADJ_IC["ic"] = generate_ic(cfg, make_model(1.0, ADJ_G2), ic)
@@ -368,14 +369,14 @@ def chi2_after(lat, steps):
 def bench_adjoint128(args):
     steps, every = ADJ_STEPS, ADJ_EVERY
     tuned = not args.no_autotune
-    lat = adjoint_case(PSC_FP32, tuned=tuned)
+    lat = adjoint_case(PSC_FP32, tuned=tuned, graph=not args.no_graph)
     if tuned:
         print("[bench] autotune", lat.autotune(3), file=sys.stderr, flush=True)
     # plain forward of the same run (graph-captured steps)
     lat.step(args.warmup)
     device_sync(lat)
     lat.close()
-    lat = adjoint_case(PSC_FP32, tuned=tuned)
+    lat = adjoint_case(PSC_FP32, tuned=tuned, graph=not args.no_graph)
     shape = lat.shape_name()
     device_sync(lat)
     t0 = time.perf_counter()
@@ -383,7 +384,7 @@ def bench_adjoint128(args):
     device_sync(lat)
     fwd = time.perf_counter() - t0
     lat.close()
-    lat = adjoint_case(PSC_FP32, tuned=tuned)
+    lat = adjoint_case(PSC_FP32, tuned=tuned, graph=not args.no_graph)
     lat.adjoint(steps, every, PHI0, DPHI0, want_field=False)   # warm-up: workspace, captured segments
     t0 = time.perf_counter()
     g = lat.adjoint(steps, every, PHI0, DPHI0)
@@ -398,7 +399,7 @@ def bench_adjoint128(args):
                           "unit": "x", "profiling_run": True}))
         return
     if args.no_fd:   # profiling runs: both objectives in FP32, no reference runs
-        lgw = adjoint_case(PSC_FP32, tuned=tuned, gw=True)
+        lgw = adjoint_case(PSC_FP32, tuned=tuned, gw=True, graph=not args.no_graph)
         lgw.adjoint(steps, every, PHI0, DPHI0, want_field=False, objective=1, projection=1)
         ggw = lgw.adjoint(steps, every, PHI0, DPHI0, objective=1, projection=1)
         lgw.close()
@@ -637,29 +638,34 @@ def bench_adaptive(intervals=10, per_interval=20):
 
 def adjoint_ratios(args):
     """Adjoint cost of the adjoint128 workload (500 VV steps at 128^3,
-    FP32-mixed, a checkpoint every 25 steps) for both objectives:
-    backward / forward (replays and sweep against one forward) and
-    total / forward (with the checkpointed forward)."""
-    steps, every = ADJ_STEPS, ADJ_EVERY
+    FP32-mixed) for both objectives: backward / forward (replays and sweep
+    against one forward) and total / forward (with the checkpointed forward),
+    with a checkpoint every 25 steps (the configuration) and with one segment
+    of all 500 steps (checkpoint_every = nsteps: the forward records every
+    step on the way, so nothing is replayed; the same gradient bit for bit,
+    for the workspace of a 500-step recording)."""
+    steps = ADJ_STEPS
     out = {}
-    for obj, gw in ((0, False), (1, True)):
-        lat = adjoint_case(PSC_FP32, gw=gw)
-        lat.step(10)
-        device_sync(lat)
-        lat.close()
-        lat = adjoint_case(PSC_FP32, gw=gw)
-        fwd = timed_steps(lat, steps)
-        lat.close()
-        lat = adjoint_case(PSC_FP32, gw=gw)
-        kw = dict(objective=obj, projection=1) if gw else {}
-        lat.adjoint(steps, every, PHI0, DPHI0, want_field=False, **kw)
-        g = lat.adjoint(steps, every, PHI0, DPHI0, want_field=False, **kw)
-        lat.close()
-        out["gw_fraction" if gw else "chi2"] = {
-            "forward_ms": fwd * 1e3, "adjoint_forward_ms": g["forward_ms"], "adjoint_backward_ms": g["backward_ms"],
-            "backward_over_forward": g["backward_ms"] / (fwd * 1e3),
-            "total_over_forward": (g["forward_ms"] + g["backward_ms"]) / (fwd * 1e3),
-            "extra_device_bytes": g["extra_bytes"], "J": g["J"], "dJ_dg2": g["dg2"]}
+    for every, tag in ((ADJ_EVERY, ""), (steps, "_single_segment")):
+        for obj, gw in ((0, False), (1, True)):
+            lat = adjoint_case(PSC_FP32, gw=gw)
+            lat.step(10)
+            device_sync(lat)
+            lat.close()
+            lat = adjoint_case(PSC_FP32, gw=gw)
+            fwd = timed_steps(lat, steps)
+            lat.close()
+            lat = adjoint_case(PSC_FP32, gw=gw)
+            kw = dict(objective=obj, projection=1) if gw else {}
+            lat.adjoint(steps, every, PHI0, DPHI0, want_field=False, **kw)
+            g = lat.adjoint(steps, every, PHI0, DPHI0, want_field=False, **kw)
+            lat.close()
+            out[("gw_fraction" if gw else "chi2") + tag] = {
+                "checkpoint_every": every, "forward_ms": fwd * 1e3, "adjoint_forward_ms": g["forward_ms"],
+                "adjoint_backward_ms": g["backward_ms"],
+                "backward_over_forward": g["backward_ms"] / (fwd * 1e3),
+                "total_over_forward": (g["forward_ms"] + g["backward_ms"]) / (fwd * 1e3),
+                "extra_device_bytes": g["extra_bytes"], "J": g["J"], "dJ_dg2": g["dg2"]}
     return out
 
 
