@@ -89,8 +89,9 @@ def run_gpu(precision, fk, pk, sc, model, steps, warmup, graph=True, autotune=Fa
     return lat, t1 - t0
 
 
-def kernel_roofline(lat, steps):
-    """Per-kernel durations from HIP events on the library's stream."""
+def kernel_roofline(lat, steps, moved=None):
+    """Per-kernel durations from HIP events on the library's stream; with a
+    dict `moved`, also the algorithmic bytes all kernels move per step."""
     lat.reset_kernel_times()
     lat.set_kernel_timing(True)
     lat.step(steps)
@@ -104,6 +105,8 @@ def kernel_roofline(lat, steps):
     per_kernel = {k: {"avg_us": v["ms"] / v["count"] * 1e3, "count": v["count"],
                       "GBps": v["bytes"] / (v["ms"] / v["count"] / 1e3) / 1e9 if v["ms"] > 0 else 0}
                   for k, v in kinds.items()}
+    if moved is not None:
+        moved["bytes_per_step"] = sum(v["bytes"] * v["count"] for v in kinds.values()) / steps
     return dom, achieved, avg_s, d["bytes"], per_kernel
 
 
@@ -186,7 +189,8 @@ def bench_gw512(args):
     lat.step(steps)
     device_sync(lat)
     el = time.perf_counter() - t0
-    _, _, _, _, per_kernel = kernel_roofline(lat, min(steps, 5))
+    moved = {}
+    _, _, _, _, per_kernel = kernel_roofline(lat, min(steps, 5), moved)
     v = n ** 3 * steps / el
     b = canonical_gw_bytes_per_site_step(4, n)
     out = {"metric": "site-updates/s at 512^3 with GW, FP32-mixed", "value": v, "unit": "site-updates/s",
@@ -194,7 +198,9 @@ def bench_gw512(args):
            "higher_is_better": True, "dtype": "f32", "data": "synthetic",
            "config": {"workload": "512^3 two-field lattice + 6 GW components, g^2 = 1e5, kIR = 25, "
                                   "Yoshida-4, dt = 0.001", "shape": lat.shape_name()},
-           "canonical_bytes_per_site_step": b, "step_hbm_frac_canonical": v * b / (HBM_PEAK_GBS * 1e9),
+           "canonical_bytes_per_site_step": b, "canonical_rate_over_peak": v * b / (HBM_PEAK_GBS * 1e9),
+           "moved_bytes_per_step": moved["bytes_per_step"],
+           "moved_over_peak": moved["bytes_per_step"] / (el / steps) / (HBM_PEAK_GBS * 1e9),
            "device_bytes": lat.device_bytes(), "kernels": per_kernel}
     lat.close()
     return out
@@ -221,7 +227,8 @@ def bench_padded256(args):
     lat.step(steps)
     device_sync(lat)
     el = time.perf_counter() - t0
-    dom, achieved, avg_s, bytes_, per_kernel = kernel_roofline(lat, min(steps, 10))
+    moved = {}
+    dom, achieved, avg_s, bytes_, per_kernel = kernel_roofline(lat, min(steps, 10), moved)
     v = n ** 3 * steps / el
     # algorithmic bytes per substep in units of X = N^3 x 8 B (the padded x
     # planes |x| <= N/2 hold half of the 8X padded array): drift 3, padding
@@ -235,7 +242,9 @@ def bench_padded256(args):
                                   "Yoshida-4", "shape": lat.shape_name()},
            "roofline": {"kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": achieved / HBM_PEAK_GBS, "avg_us": avg_s * 1e6, "bytes": bytes_},
-           "canonical_bytes_per_site_step": b, "step_hbm_frac_canonical": v * b / (HBM_PEAK_GBS * 1e9),
+           "canonical_bytes_per_site_step": b, "canonical_rate_over_peak": v * b / (HBM_PEAK_GBS * 1e9),
+           "moved_bytes_per_step": moved["bytes_per_step"],
+           "moved_over_peak": moved["bytes_per_step"] / (el / steps) / (HBM_PEAK_GBS * 1e9),
            "device_bytes": lat.device_bytes(), "kernels": per_kernel}
     lat.close()
     return out
@@ -278,7 +287,8 @@ def bench_ensemble128(args):
     lat.step(steps)
     device_sync(lat)
     el = time.perf_counter() - t0
-    _, _, _, _, per_kernel = kernel_roofline(lat, min(steps, 20))
+    moved = {}
+    _, _, _, _, per_kernel = kernel_roofline(lat, min(steps, 20), moved)
     shape = lat.shape_name()
     lat.close()
     v = B * n ** 3 * steps / el
@@ -313,7 +323,9 @@ def bench_ensemble128(args):
            "batched_over_sequential": v / vseq,
            "initial_conditions": {"device_ms_all_members": ic_device_s * 1e3,
                                   "host_ms_one_member": ic_host_s * 1e3},
-           "canonical_bytes_per_site_step": b, "step_hbm_frac_canonical": v * b / (HBM_PEAK_GBS * 1e9),
+           "canonical_bytes_per_site_step": b, "canonical_rate_over_peak": v * b / (HBM_PEAK_GBS * 1e9),
+           "moved_bytes_per_step": moved["bytes_per_step"],
+           "moved_over_peak": moved["bytes_per_step"] / (el / steps) / (HBM_PEAK_GBS * 1e9),
            "kernels": per_kernel}
     one.close()
     return out
@@ -743,13 +755,14 @@ def main():
         print(json.dumps({"metric": METRIC, "value": v32, "unit": "site-updates/s",
                           "steps": args.steps, "profiling_run": True}))
         return
-    dom, achieved, avg_s, bpl, per_kernel = kernel_roofline(lat32, min(args.steps, 50))
+    moved32, moved64 = {}, {}
+    dom, achieved, avg_s, bpl, per_kernel = kernel_roofline(lat32, min(args.steps, 50), moved32)
     lat32.close()
     s64 = args.fp64_steps or args.steps
     lat64, el64 = run_gpu(PSC_FP64, fk, pk, sc, model, s64, args.warmup, graph, tune)
     shape64 = lat64.shape_name()
     v64 = N ** 3 * s64 / el64
-    _, _, _, _, per_kernel64 = kernel_roofline(lat64, min(s64, 20))
+    _, _, _, _, per_kernel64 = kernel_roofline(lat64, min(s64, 20), moved64)
     lat64.close()
     print(f"[bench] fp64: {el64:.3f} s for {s64} steps", file=sys.stderr, flush=True)
 
@@ -781,11 +794,21 @@ def main():
         },
         "fp64": {"value": v64, "ms_per_step": el64 / s64 * 1e3, "steps": s64},
         "fp32_over_fp64": v32 / v64,
-        "step_hbm_frac_canonical": {
-            "fp32": v32 * canonical_bytes_per_site_step(4) / (HBM_PEAK_GBS * 1e9),
-            "fp64": v64 * canonical_bytes_per_site_step(8) / (HBM_PEAK_GBS * 1e9),
-            "bytes_per_site_step": {"fp32": canonical_bytes_per_site_step(4),
-                                    "fp64": canonical_bytes_per_site_step(8)},
+        # whole-step rates: the algorithmic bytes the kernels move per step
+        # over the step time (an HBM fraction, below 1), and the canonical
+        # 16C + 2R count per field over the step time (a rate in canonical
+        # bytes: the pipeline moves fewer bytes than it counts, and the
+        # memory-side cache serves part of them, so it can exceed 1)
+        "step_rates": {
+            "moved_bytes_per_step": {"fp32": moved32["bytes_per_step"], "fp64": moved64["bytes_per_step"]},
+            "moved_over_peak": {
+                "fp32": moved32["bytes_per_step"] / (el32 / args.steps) / (HBM_PEAK_GBS * 1e9),
+                "fp64": moved64["bytes_per_step"] / (el64 / s64) / (HBM_PEAK_GBS * 1e9)},
+            "canonical_bytes_per_site_step": {"fp32": canonical_bytes_per_site_step(4),
+                                              "fp64": canonical_bytes_per_site_step(8)},
+            "canonical_rate_over_peak": {
+                "fp32": v32 * canonical_bytes_per_site_step(4) / (HBM_PEAK_GBS * 1e9),
+                "fp64": v64 * canonical_bytes_per_site_step(8) / (HBM_PEAK_GBS * 1e9)},
         },
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
