@@ -676,10 +676,14 @@ struct Engine final : EngineBase {
     }
 
     // Records (bwd = false) or sweeps (bwd = true) a segment of `steps` steps;
-    // full segments go through a graph captured on first use when steps are
-    // graph-captured.
+    // full segments of the <chi^2> adjoint go through a graph captured on
+    // first use when steps are graph-captured.  The GW objective's segments
+    // are launched eagerly: a captured GW segment once ended in a
+    // malformed-packet abort whose cause in this library was not found, and
+    // eager launches measured no slower there (backward 240 ms eager, 239 ms
+    // captured, 128^3 FP32).
     int run_segment(psc_ctx* c, int steps, AdjWork& w, bool bwd) {
-        if (steps != w.K || !c->cfg.use_graph)
+        if (steps != w.K || !c->cfg.use_graph || w.gw)
             return bwd ? backward_segment(c, steps, w) : record_segment(c, steps, w);
         hipGraphExec_t& ge = bwd ? w.g_bwd : w.g_rec;
         if (!ge) {

@@ -227,7 +227,7 @@ def test_gw_fraction_adjoint_fp32_and_graphs():
     cfg, model, fk, pk, u, pu, lat = setup_gw(PSC_VV2, PSC_EXPANSION_SELF, N=32, precision=PSC_FP32,
                                               use_graph=1)
     J, dg2, dphi0, dchi = O.adjoint_gw(cfg, model, fk, pk, u, pu, 1.0, PHI0, DPHI0, 0.0, 6, 1)
-    for _ in range(2):   # captured segments, then reuse of the workspace
+    for _ in range(2):   # a context with captured steps (the GW segments run eagerly), then reuse of the workspace
         g = lat.adjoint(6, 3, PHI0, DPHI0, objective=1, projection=1)
         errs = (abs(g["J"] - J) / abs(J), abs(g["dg2"] - dg2) / abs(dg2), abs(g["dphi0"] - dphi0) / abs(dphi0),
                 rel_err(g["dchi"], dchi))
