@@ -67,7 +67,7 @@ extern "C" {
 
 /* Run configuration: the runtime form of parameters.h (parameters.h:10-134). */
 typedef struct psc_config {
-    int N;             /* lattice points per edge, power of two 16..512 (parameters.h:22) */
+    int N;             /* lattice points per edge, 2^k or 3*2^k in 16..512 (parameters.h:22) */
     int nflds;         /* 1 or 2 scalar fields (parameters.h:20)                          */
     int integrator;    /* PSC_VV2 or PSC_Y4 (parameters.h:10 `VV`)                        */
     int precision;     /* PSC_FP64 or PSC_FP32                                             */

@@ -81,11 +81,11 @@ struct ParkMiller {
 // FFT: iterative radix-2 complex transform of a power-of-two length, used to
 // build the unnormalised 3D r2c / c2r transforms in FFTW layout that the
 // reference calls (fftw_plan_dft_r2c_3d / c2r_3d, evolution.cpp:1112,1146).
-struct Fft1D {
+struct Fft1D2 {
     int n = 0;
     std::vector<cd> tw;   // exp(-2 pi i k / n), k < n/2
     std::vector<int> rev;
-    explicit Fft1D(int n_) : n(n_), tw(n_ / 2), rev(n_) {
+    explicit Fft1D2(int n_) : n(n_), tw(n_ / 2), rev(n_) {
         int lg = 0;
         while ((1 << lg) < n) ++lg;
         for (int k = 0; k < n / 2; ++k) {
@@ -115,6 +115,40 @@ struct Fft1D {
                 }
             }
         }
+    }
+};
+
+// Lengths n = 3 m (m a power of two; FFTW handles any n for the reference):
+// one decimation-in-time radix-3 step over three radix-2 transforms of the
+// interleaved sub-sequences a[3 j + r],
+//   X[k + q m] = sum_r exp(s 2 pi i r (k + q m) / n) Y_r[k],  q = 0, 1, 2.
+struct Fft1D {
+    int n = 0, m = 0;
+    bool three = false;
+    Fft1D2 f2;
+    std::vector<cd> tw3;   // exp(-2 pi i k / n), k < n
+    explicit Fft1D(int n_) : n(n_), m(n_ % 3 == 0 ? n_ / 3 : n_), three(n_ % 3 == 0), f2(n_ % 3 == 0 ? n_ / 3 : n_) {
+        if (three) {
+            tw3.resize(n);
+            for (int k = 0; k < n; ++k) {
+                const double ang = -2.0 * kPi * (double)k / (double)n;
+                tw3[k] = cd(std::cos(ang), std::sin(ang));
+            }
+        }
+    }
+    void exec(cd* a, int sign) const {
+        if (!three) return f2.exec(a, sign);
+        std::vector<cd> y((size_t)n);
+        for (int r = 0; r < 3; ++r) {
+            for (int j = 0; j < m; ++j) y[(size_t)r * m + j] = a[3 * j + r];
+            f2.exec(y.data() + (size_t)r * m, sign);
+        }
+        auto w = [&](int k) { return sign > 0 ? std::conj(tw3[k % n]) : tw3[k % n]; };
+        for (int k = 0; k < m; ++k)
+            for (int q = 0; q < 3; ++q) {
+                const int kk = k + q * m;
+                a[kk] = y[k] + w(kk) * y[(size_t)m + k] + w(2 * kk) * y[2 * (size_t)m + k];
+            }
     }
 };
 
@@ -968,7 +1002,10 @@ int orc_fft_1d(int n, double* data, int sign) {
 }
 
 static int check_cfg(const psc_config* c) {
-    if (!c || c->N < 4 || (c->N & (c->N - 1)) || c->nflds < 1 || c->nflds > 2) return PSC_ERR_ARG;
+    // N = 2^k or 3 * 2^k (the lengths of the oracle's FFT)
+    if (!c || c->N < 4) return PSC_ERR_ARG;
+    const int m = c->N % 3 == 0 ? c->N / 3 : c->N;
+    if ((m & (m - 1)) || (c->N % 2) || c->nflds < 1 || c->nflds > 2) return PSC_ERR_ARG;
     if ((c->pad != 0 && c->pad != 2) || (c->pad == 2 && c->with_gw)) return PSC_ERR_ARG;
     return 0;
 }

@@ -1030,5 +1030,7 @@ PadBase* make_pad_one() {
 #define PSC_SHAPE_Y(P, NAME, G, GY) {P, G::N, NAME, &make_one<G, GY>, nullptr}
 // A shape of the padded stepper only, its z row passes on GR.
 #define PSC_SHAPE_PAD(P, NAME, G, GR) {P, G::N, NAME, nullptr, &make_pad_one<G, GR>}
+// A shape of the unpadded stepper only.
+#define PSC_SHAPE_ONLY(P, T, N, NAME, ...) {P, N, NAME, &make_one<S<T, N, __VA_ARGS__>>, nullptr}
 
 }  // namespace psc

@@ -78,7 +78,8 @@ constexpr int bitrev(int i, int bits) {
     return r;
 }
 
-// In-register DFT of length R (power of two <= 64) by a radix-2 DIT network:
+// In-register DFT of length R (power of two <= 64; 3 * 2^k through Dft3M) by a
+// radix-2 DIT network:
 // out[k] = sum_r in[r] exp(SIGN 2 pi i r k / R).
 template <int R, int SIGN, typename T, int LEN = 2>
 struct DftLevel {
@@ -118,13 +119,115 @@ struct BitRevPermute {
     }
 };
 
+// cos(2 pi m / n) and sin(2 pi m / n) at compile time (Taylor series after
+// reduction to [-pi/4, pi/4]; exact at the multiples of pi/4), for the
+// twiddles of the radix-3 lengths.
+constexpr double kTwoPi = 6.283185307179586476925286766559;
+constexpr double taylor_cos(double x) {
+    double term = 1.0, sum = 1.0;
+    for (int k = 1; k < 14; ++k) {
+        term *= -x * x / ((2.0 * k - 1.0) * (2.0 * k));
+        sum += term;
+    }
+    return sum;
+}
+constexpr double taylor_sin(double x) {
+    double term = x, sum = x;
+    for (int k = 1; k < 14; ++k) {
+        term *= -x * x / ((2.0 * k) * (2.0 * k + 1.0));
+        sum += term;
+    }
+    return sum;
+}
+// The angle 2 pi m / n is 2 pi (8 m mod 8n) / (8n): octant o = floor(8 m / n)
+// and remainder, so that the series argument stays within pi/4.
+constexpr double unit_cos(long m, long n) {
+    m = ((m % n) + n) % n;
+    const long o = (8 * m) / n;                          // octant 0..7
+    const double r = kTwoPi * (double)(8 * m - o * n) / (8.0 * (double)n);   // in [0, pi/4)
+    switch (o) {
+        case 0: return taylor_cos(r);
+        case 1: return taylor_sin(kTwoPi / 8 - r);
+        case 2: return -taylor_sin(r);
+        case 3: return -taylor_cos(kTwoPi / 8 - r);
+        case 4: return -taylor_cos(r);
+        case 5: return -taylor_sin(kTwoPi / 8 - r);
+        case 6: return taylor_sin(r);
+        default: return taylor_cos(kTwoPi / 8 - r);
+    }
+}
+constexpr double unit_sin(long m, long n) { return unit_cos(4 * m - n, 4 * n); }   // sin x = cos(x - pi/2)
+
+// v * exp(SIGN 2 pi i M / R) for compile-time M, R.
+template <int SIGN, int M, int R, typename T>
+__device__ __forceinline__ cpx<T> rot_n(cpx<T> v) {
+    constexpr int m = ((M % R) + R) % R;
+    if constexpr (m == 0) {
+        return v;
+    } else if constexpr (4 * m == R) {
+        return SIGN > 0 ? cpx<T>{-v.y, v.x} : cpx<T>{v.y, -v.x};
+    } else if constexpr (2 * m == R) {
+        return cpx<T>{-v.x, -v.y};
+    } else if constexpr (4 * m == 3 * R) {
+        return SIGN > 0 ? cpx<T>{v.y, -v.x} : cpx<T>{-v.y, v.x};
+    } else {
+        constexpr T c = (T)unit_cos(m, R);
+        constexpr T s = (T)(SIGN * unit_sin(m, R));
+        return {v.x * c - v.y * s, v.x * s + v.y * c};
+    }
+}
+
+constexpr bool is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
+
+template <int R, int SIGN, typename T>
+__device__ __forceinline__ void dft(cpx<T>* v);
+
+// In-register DFT of length R = 3 M (M a power of two): the three
+// interleaved sub-sequences x[r0 + 3 r1] are transformed with length M, each
+// output k1 of sub-sequence r0 is turned by exp(SIGN 2 pi i r0 k1 / R), and a
+// length-3 DFT over r0 gives the outputs k1, k1 + M, k1 + 2M.
+template <int R, int SIGN, typename T>
+struct Dft3M {
+    static constexpr int M = R / 3;
+    template <int K>
+    static __device__ __forceinline__ void combine(cpx<T> (&y)[3][M], cpx<T>* v) {
+        if constexpr (K < M) {
+            const cpx<T> a = y[0][K];
+            const cpx<T> b = rot_n<SIGN, K, R>(y[1][K]);
+            const cpx<T> c = rot_n<SIGN, 2 * K, R>(y[2][K]);
+            const cpx<T> s = cadd(b, c), d = csub(b, c);
+            constexpr T h = (T)0.86602540378443864676;   // sqrt(3) / 2
+            const cpx<T> m = {a.x - (T)0.5 * s.x, a.y - (T)0.5 * s.y};
+            // SIGN sqrt(3)/2 i d
+            const cpx<T> u = SIGN > 0 ? cpx<T>{-h * d.y, h * d.x} : cpx<T>{h * d.y, -h * d.x};
+            v[K] = cadd(a, s);
+            v[K + M] = cadd(m, u);
+            v[K + 2 * M] = csub(m, u);
+            combine<K + 1>(y, v);
+        }
+    }
+    static __device__ __forceinline__ void run(cpx<T>* v) {
+        cpx<T> y[3][M];
+#pragma unroll
+        for (int r0 = 0; r0 < 3; ++r0)
+#pragma unroll
+            for (int r1 = 0; r1 < M; ++r1) y[r0][r1] = v[r0 + 3 * r1];
+#pragma unroll
+        for (int r0 = 0; r0 < 3; ++r0) dft<M, SIGN>(y[r0]);
+        combine<0>(y, v);
+    }
+};
+
 template <int R, int SIGN, typename T>
 __device__ __forceinline__ void dft(cpx<T>* v) {
     if constexpr (R == 1) {
         return;
-    } else {
+    } else if constexpr (is_pow2(R)) {
         BitRevPermute<R>::run(v);
         DftLevel<R, SIGN, T, 2>::run(v);
+    } else {
+        static_assert(R % 3 == 0 && is_pow2(R / 3), "radix must be 2^k or 3 * 2^k");
+        Dft3M<R, SIGN, T>::run(v);
     }
 }
 

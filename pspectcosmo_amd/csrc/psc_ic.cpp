@@ -119,7 +119,7 @@ void for_each_mode(int N, bool dc, int seed, uint64_t first, const psc_ic::KeptI
 }
 
 bool valid(const psc_config* c) {
-    return c && c->N >= 16 && c->N <= 512 && (c->N & (c->N - 1)) == 0 && c->nflds >= 1 &&
+    return c && psc_ic::lattice_size_ok(c->N) && c->nflds >= 1 &&
            c->nflds <= 2 && c->kIR > 0 && c->fStar > 0 && c->MPl > 0;
 }
 

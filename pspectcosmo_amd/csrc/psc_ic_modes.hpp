@@ -33,6 +33,13 @@ namespace psc_ic {
 
 PSC_HD inline double pw2(double x) { return x * x; }
 
+// Lattice sizes of the library: 2^k or 3 * 2^k, 16 <= N <= 512 (the lengths
+// its transforms take; a size without launch shapes is refused at psc_create).
+PSC_HD inline bool lattice_size_ok(int N) {
+    const int m = N % 3 == 0 ? N / 3 : N;
+    return N >= 16 && N <= 512 && N % 2 == 0 && (m & (m - 1)) == 0;
+}
+
 // Modes visited per field: 2 (4h + 2h^2) plane modes, h N^2 interior modes,
 // 8 corners (7 without the DC mode).
 PSC_HD inline uint64_t mode_count(int N, bool dc) {

@@ -33,6 +33,7 @@ const std::vector<ShapeEntry>& shape_table() {
         register_shapes_f64_small(v);
         register_shapes_f64_256(v);
         register_shapes_f64_512(v);
+        register_shapes_r3(v);
         return v;
     }();
     return table;
@@ -509,7 +510,7 @@ int psc_create(const psc_config* cfg, psc_ctx** out) {
     if (!cfg || !out) return PSC_ERR_ARG;
     *out = nullptr;
     const int N = cfg->N;
-    if (N < 16 || N > 512 || (N & (N - 1)) || cfg->nflds < 1 || cfg->nflds > 2 || cfg->batch < 1 ||
+    if (!psc_ic::lattice_size_ok(N) || cfg->nflds < 1 || cfg->nflds > 2 || cfg->batch < 1 ||
         (cfg->integrator != PSC_VV2 && cfg->integrator != PSC_Y4) ||
         (cfg->precision != PSC_FP64 && cfg->precision != PSC_FP32) || !(cfg->kIR > 0) ||
         !(cfg->fStar > 0) || !(cfg->MPl > 0) || !std::isfinite(cfg->dt) ||

@@ -71,7 +71,7 @@ struct Params {
 };
 
 const char* kKeys[][2] = {
-    {"N", "points per edge (16..512, power of two)"},
+    {"N", "points per edge: 2^k or 3 * 2^k, 16..512"},
     {"nflds", "1 or 2"},
     {"kIR", "infrared momentum; L = 2 pi / kIR"},
     {"dt", "time step, or `auto` for the reference's adaptive rule"},

@@ -47,7 +47,7 @@ def test_abi_version():
     assert L.lib().psc_num_bins(64) == 56
 
 
-@pytest.mark.parametrize("N,nflds,seed", [(16, 2, 1), (32, 1, 7), (16, 2, 0), (64, 2, 3)])
+@pytest.mark.parametrize("N,nflds,seed", [(16, 2, 1), (32, 1, 7), (16, 2, 0), (64, 2, 3), (48, 2, 4), (96, 1, 2)])
 def test_host_ic_matches_oracle_bitwise(N, nflds, seed):
     cfg, model, ic = parity_setup(N=N, nflds=nflds, seed=seed)
     fk, pk, sc = L.generate_ic(cfg, model, ic)
@@ -80,7 +80,7 @@ def test_host_gw_ic_matches_oracle_bitwise(N):
 
 
 def test_create_rejects_bad_config():
-    cfg = make_config(48)  # not a power of two
+    cfg = make_config(40)  # neither 2^k nor 3 * 2^k
     h = C.c_void_p()
     assert L.lib().psc_create(C.byref(cfg), C.byref(h)) == 1
 

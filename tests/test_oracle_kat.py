@@ -63,14 +63,14 @@ def brute_dft3(x):
     return np.einsum("ai,bj,ck,ijk->abc", W, W, W, x)
 
 
-@pytest.mark.parametrize("N", [4, 8])
+@pytest.mark.parametrize("N", [4, 8, 6, 12])
 def test_fft_r2c_brute_force(N):
     x = np.random.default_rng(N).standard_normal((N, N, N))
     ref = brute_dft3(x)[:, :, : N // 2 + 1]
     assert np.abs(O.fft_r2c_3d(x) - ref).max() < 1e-12 * np.abs(ref).max()
 
 
-@pytest.mark.parametrize("N", [16, 32])
+@pytest.mark.parametrize("N", [16, 32, 24, 48])
 def test_fft_r2c_c2r_vs_pocketfft(N):
     x = np.random.default_rng(N).standard_normal((N, N, N))
     k = O.fft_r2c_3d(x)
@@ -95,10 +95,12 @@ def test_fft_closed_forms():
     assert np.abs(k).max() < 1e-9
 
 
-def test_fft_1d_vs_numpy():
-    x = np.random.default_rng(3).standard_normal(64) + 1j * np.random.default_rng(4).standard_normal(64)
+@pytest.mark.parametrize("n", [64, 3, 6, 24, 96, 384])
+def test_fft_1d_vs_numpy(n):
+    """Power-of-two lengths and 3 * 2^k (one radix-3 step over radix-2 halves)."""
+    x = np.random.default_rng(3).standard_normal(n) + 1j * np.random.default_rng(4).standard_normal(n)
     assert np.abs(O.fft_1d(x, -1) - np.fft.fft(x)).max() < 1e-12
-    assert np.abs(O.fft_1d(x, +1) - np.fft.ifft(x) * 64).max() < 1e-12
+    assert np.abs(O.fft_1d(x, +1) - np.fft.ifft(x) * n).max() < 1e-12
 
 
 @pytest.mark.parametrize("nflds", [1, 2])
