@@ -22,16 +22,19 @@ Other workloads (own JSON line each):
                          500 steps, a checkpoint every 25 steps: adjoint/forward
                          time, extra memory, central differences (eps = 1e-3)
   --workload padded256   the dealiased stepper (V' on 512^3) of the 256^3
-                         two-field lattice, Yoshida-4, FP32-mixed
+                         two-field lattice, Yoshida-4, FP32-mixed; with
+                         --n-pad 384 on the 3N/2 padded lattice
 
 The default run adds, under "workloads", a shorter measurement of each other
-configuration (gw512, ensemble128, adjoint128, padded256), the attainable
+configuration (gw512, ensemble128, adjoint128, padded256 with N_pad = 512
+and 384), the attainable
 copy bandwidth, the stable variant of the headline lattice (kIR = 1, finite
 values), the rate of one-step psc_step calls against one call of many steps,
 and the reference's adaptive dt rule on the device against fixed dt.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-workloads]
-                       [--workload preheat256|gw512|ensemble128|adjoint128|padded256] [--autotune]
+                       [--workload preheat256|gw512|ensemble128|adjoint128|padded256] [--n-pad 384]
+                       [--autotune]
 """
 import argparse
 import json
@@ -207,18 +210,19 @@ def bench_gw512(args):
 
 
 def bench_padded256(args):
-    """The dealiased stepper (pad = 2, N_pad = 512) on the 256^3 two-field
-    lattice, FP32-mixed, Yoshida-4."""
+    """The dealiased stepper (pad = 2, N_pad = 512, or args.n_pad = 384 for
+    3N/2) on the 256^3 two-field lattice, FP32-mixed, Yoshida-4."""
     n = 256
+    n_pad = getattr(args, "n_pad", 0) or 2 * n
     model = make_model(1.0, G2)
     ic = make_ic(f0=(1.0, 0.0), fd0=(-0.7, 0.0), seed=1)
-    cfg = make_config(n, nflds=2, integrator=PSC_Y4, precision=PSC_FP32, kIR=KIR, dt=DT, pad=2,
+    cfg = make_config(n, nflds=2, integrator=PSC_Y4, precision=PSC_FP32, kIR=KIR, dt=DT, pad=2, n_pad=n_pad,
                       use_graph=0 if args.no_graph else 1)
     lat = Lattice(cfg)
     lat.set_model(0, model)
     lat.generate_ic_device(0, ic)
     device_sync(lat)
-    if args.autotune:   # the shapes of the 512-point padded passes
+    if args.autotune:   # the shapes of the padded passes
         print("[bench] autotune", lat.autotune(2), file=sys.stderr, flush=True)
     steps = args.steps
     lat.step(args.warmup)
@@ -230,16 +234,17 @@ def bench_padded256(args):
     moved = {}
     dom, achieved, avg_s, bytes_, per_kernel = kernel_roofline(lat, min(steps, 10), moved)
     v = n ** 3 * steps / el
-    # algorithmic bytes per substep in units of X = N^3 x 8 B (the padded x
-    # planes |x| <= N/2 hold half of the 8X padded array): drift 3, padding
-    # and z inverse 1 + 4, y inverse, x pass and y forward 8 each, z forward
-    # 2 + 1, kick 4
-    b = 3 * 39 * 8
-    out = {"metric": "site-updates/s at 256^3, dealiased (N_pad = 512), FP32-mixed", "value": v,
+    # algorithmic bytes per substep in units of X = N^3 x 8 B, with r = N_pad / N
+    # (the padded array is r^3 X, its x planes |x| <= N/2 hold r^2 X): drift 3,
+    # padding and z inverse 1 + r^2, y inverse, x pass and y forward 2 r^2 each,
+    # z forward r + 1, kick 4; 39 X at r = 2, 26.25 X at r = 3/2
+    r = n_pad / n
+    b = 3 * (9 + 7 * r * r + r) * 8
+    out = {"metric": f"site-updates/s at 256^3, dealiased (N_pad = {n_pad}), FP32-mixed", "value": v,
            "unit": "site-updates/s", "n_gpus": 1, "steps": steps, "warmup": args.warmup,
            "ms_per_step": el / steps * 1e3, "higher_is_better": True, "dtype": "f32", "data": "synthetic",
-           "config": {"workload": "256^3 two-field lattice, padded V' on 512^3, g^2 = 2.5e5, kIR = 25, "
-                                  "Yoshida-4", "shape": lat.shape_name()},
+           "config": {"workload": f"256^3 two-field lattice, padded V' on {n_pad}^3, g^2 = 2.5e5, kIR = 25, "
+                                  "Yoshida-4", "N_pad": n_pad, "shape": lat.shape_name()},
            "roofline": {"kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": achieved / HBM_PEAK_GBS, "avg_us": avg_s * 1e6, "bytes": bytes_},
            "canonical_bytes_per_site_step": b, "canonical_rate_over_peak": v * b / (HBM_PEAK_GBS * 1e9),
@@ -694,6 +699,8 @@ def run_workloads(args):
                      ("ensemble128", lambda: bench_ensemble128(argparse.Namespace(
                          **{**vars(quick), "steps": 60, "warmup": 3, "seq_steps": 20}))),
                      ("padded256", lambda: bench_padded256(argparse.Namespace(**{**vars(quick), "steps": 10, "warmup": 2}))),
+                     ("padded256_n384", lambda: bench_padded256(argparse.Namespace(
+                         **{**vars(quick), "steps": 10, "warmup": 2, "n_pad": 384}))),
                      ("adjoint128", lambda: adjoint_ratios(args))):
         t0 = time.perf_counter()
         print(f"[bench] workload {name} ...", file=sys.stderr, flush=True)
@@ -718,6 +725,7 @@ def main():
     ap.add_argument("--workload", default="preheat256",
                     choices=["preheat256", "gw512", "ensemble128", "adjoint128", "padded256"])
     ap.add_argument("--autotune", action="store_true", help="time the launch shapes first")
+    ap.add_argument("--n-pad", dest="n_pad", type=int, default=0, help="padded256: N_pad (512 or 384)")
     ap.add_argument("--no-autotune", action="store_true", help="headline: keep the default shapes")
     ap.add_argument("--seq-steps", type=int, default=150, help="ensemble: steps per member run alone")
     ap.add_argument("--no-fd", action="store_true", help="adjoint: skip the finite-difference and FP64 runs")

@@ -81,12 +81,15 @@ typedef struct psc_config {
     int autotune;      /* 1: adopt the launch shape cached by psc_autotune for this
                           device / N / precision / batch / GW, if any                   */
     int pad;           /* 0: evolve_VV_k_nopad; 2: the dealiased stepper evolve_VV_k_pad
-                          (PAD_FLAG, evolution.cpp:1271-1759) with N_pad = 2 N          */
+                          (PAD_FLAG, evolution.cpp:1271-1759) on the N_pad lattice     */
     int dt_rule;       /* PSC_DT_FIXED: every step takes dt; PSC_DT_REFERENCE: each step
                           takes the reference's calculate_dt, capped and clamped to the
                           output time of psc_set_output_times (pspectcosmo.cpp:213-233,
                           622-627), computed on the device                           */
-    int reserved_i[3];
+    int n_pad;         /* N_pad of the dealiased stepper (parameters.h:23), larger than N
+                          (pspectcosmo.cpp:244): 2N or 3N/2 (N_pad <= 512); 0 = 2N;
+                          must be 0 with pad = 0                                      */
+    int reserved_i[2];
     double kIR;        /* lattice IR wavenumber, L = 2 pi / kIR (parameters.h:28,32)       */
     double dt;         /* time step in program units (parameters.h:36)                     */
     double fStar;      /* field rescaling (parameters.h:133)                               */
@@ -336,7 +339,7 @@ int psc_adjoint(psc_ctx* ctx, const psc_adjoint_spec* spec, double* out, double*
  * variable PSC_TUNE_CACHE (default $HOME/.cache/pspectcosmo_amd/tune.txt).
  * report (may be NULL) receives "shape=ms;..." for every candidate.  With
  * pad = 2 the candidates, psc_shape_name, psc_list_shapes and psc_set_shape
- * refer to the 2N shapes of the padded passes. */
+ * refer to the N_pad shapes of the padded passes. */
 int psc_autotune(psc_ctx* ctx, int nsteps, char* report, int report_len);
 /* Name of the launch shape the context uses. */
 const char* psc_shape_name(const psc_ctx* ctx);

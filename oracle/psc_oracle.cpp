@@ -352,7 +352,8 @@ struct orc_ctx {
     // GW (6 symmetric components 11,22,33,12,13,23)
     std::vector<cd> u_k[6], pi_u_k[6], pi_u_half[6], src_k[6];
     std::vector<double> grad[2][3], src[6];
-    // dealiased stepper (PAD_FLAG): fields and V' on the Np = 2N grid
+    // dealiased stepper (PAD_FLAG): fields and V' on the Np grid (N_pad,
+    // parameters.h:23; 2N when the config leaves it 0)
     int Np = 0, Nph = 0;
     size_t nrp = 0, nkp = 0;
     std::unique_ptr<Fft3D> fftp;
@@ -377,7 +378,7 @@ struct orc_ctx {
             Vp[fl].assign(nr, 0.0);
         }
         if (cfg.pad == 2) {
-            Np = 2 * N;
+            Np = cfg.n_pad ? cfg.n_pad : 2 * N;
             Nph = Np / 2 + 1;
             nrp = (size_t)Np * Np * Np;
             nkp = (size_t)Np * Np * Nph;
@@ -1007,6 +1008,13 @@ static int check_cfg(const psc_config* c) {
     const int m = c->N % 3 == 0 ? c->N / 3 : c->N;
     if ((m & (m - 1)) || (c->N % 2) || c->nflds < 1 || c->nflds > 2) return PSC_ERR_ARG;
     if ((c->pad != 0 && c->pad != 2) || (c->pad == 2 && c->with_gw)) return PSC_ERR_ARG;
+    // N_pad (parameters.h:23) larger than N (pspectcosmo.cpp:244), a length of
+    // the FFT; 0 = 2N
+    if (c->pad == 0 && c->n_pad != 0) return PSC_ERR_ARG;
+    if (c->pad == 2 && c->n_pad != 0) {
+        const int mp = c->n_pad % 3 == 0 ? c->n_pad / 3 : c->n_pad;
+        if (c->n_pad <= c->N || (mp & (mp - 1)) || c->n_pad % 2) return PSC_ERR_ARG;
+    }
     return 0;
 }
 

@@ -38,7 +38,7 @@ class PscConfig(C.Structure):
         ("N", C.c_int), ("nflds", C.c_int), ("integrator", C.c_int), ("precision", C.c_int),
         ("batch", C.c_int), ("with_gw", C.c_int), ("alpha", C.c_int), ("expansion", C.c_int),
         ("yoshida_exact", C.c_int), ("use_graph", C.c_int), ("autotune", C.c_int),
-        ("pad", C.c_int), ("dt_rule", C.c_int), ("reserved_i", C.c_int * 3),
+        ("pad", C.c_int), ("dt_rule", C.c_int), ("n_pad", C.c_int), ("reserved_i", C.c_int * 2),
         ("kIR", C.c_double), ("dt", C.c_double), ("fStar", C.c_double),
         ("omegaStar", C.c_double), ("MPl", C.c_double), ("reserved_d", C.c_double * 3),
     ]
@@ -65,13 +65,14 @@ class PscAdjointSpec(C.Structure):
 
 def make_config(N, nflds=2, integrator=PSC_VV2, precision=PSC_FP64, batch=1, with_gw=0,
                 alpha=0, expansion=PSC_EXPANSION_SELF, yoshida_exact=0, use_graph=0, autotune=0,
-                kIR=25.0, dt=0.005, fStar=1.0, omegaStar=1.0, MPl=1.0, pad=0, dt_rule=0):
+                kIR=25.0, dt=0.005, fStar=1.0, omegaStar=1.0, MPl=1.0, pad=0, dt_rule=0, n_pad=0):
     c = PscConfig()
     c.N, c.nflds, c.integrator, c.precision = N, nflds, integrator, precision
     c.batch, c.with_gw, c.alpha, c.expansion = batch, with_gw, alpha, expansion
     c.yoshida_exact, c.use_graph, c.autotune = yoshida_exact, use_graph, autotune
     c.kIR, c.dt, c.fStar, c.omegaStar, c.MPl = kIR, dt, fStar, omegaStar, MPl
     c.pad = pad
+    c.n_pad = n_pad
     c.dt_rule = dt_rule
     return c
 

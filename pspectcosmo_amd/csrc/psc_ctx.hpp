@@ -78,6 +78,7 @@ struct PadBase {
     virtual int nV() const = 0;    // partial sums of V per force evaluation
     virtual int nKG() const = 0;   // partial sums of K, G per kick
     virtual size_t padded_elems() const = 0;
+    virtual int n_pad() const = 0;   // N_pad, the padded lattice's points per edge
     // <V> over the padded lattice of the current state (kernel_a_k_pad's V_E).
     virtual int mean_V(psc_ctx* c, double* V) = 0;
 };
@@ -181,7 +182,8 @@ struct psc_ctx {
     std::unique_ptr<AdjWork> adj;
     double dev_bytes = 0.0;
     int shape = 0;   // index into kShapes
-    int pad_shape = -1;   // dealiased stepper: index into kShapes of its 2N shape
+    int pad_shape = -1;   // dealiased stepper: index into kShapes of its padded shape
+    int NP = 0;           // dealiased stepper: N_pad (2N or 3N/2), else 0
     int member_group = 0;   // members per group of the force chain (0: all)
     std::string report;
 
@@ -210,7 +212,7 @@ inline StepConsts consts_of(const psc_ctx* c) {
     s.alpha = c->cfg.alpha;
     s.expansion_static = c->cfg.expansion == PSC_EXPANSION_STATIC;
     s.nflds = c->cfg.nflds;
-    s.NV = c->padeng ? 8.0 * (double)c->N3 : (double)c->N3;   // lattice sites V is averaged over
+    s.NV = c->padeng ? (double)c->NP * c->NP * c->NP : (double)c->N3;   // lattice sites V is averaged over
     s.dt_rule = c->cfg.dt_rule;
     s.kmax = c->cfg.kIR * std::sqrt(3.0) * c->N / 2.0;
     return s;

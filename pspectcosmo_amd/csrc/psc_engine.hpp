@@ -728,19 +728,23 @@ int Engine<G, GY>::gw_spectra(psc_ctx* c, int member, int projection, int nlog) 
     return gw_spectra_impl<typename G::T, G::N>(c, member, projection, nlog);
 }
 
-// The dealiased stepper's passes on the padded shape GP (N_pad = GP::N = 2N);
-// the two z-axis row passes (padding, extraction) may run on another plan GR.
-template <class GP, class GR = GP>
+// The dealiased stepper of the N lattice on the padded shape GP (N_pad =
+// GP::N: 2N or 3N/2); the two z-axis row passes (padding, extraction) may run
+// on another plan GR.
+template <class GP, class GR, int N>
 struct PadEngine final : PadBase {
     using T = typename GP::T;
-    static constexpr int NP = GP::N, N = NP / 2;
+    static constexpr int NP = GP::N;
+    static constexpr int H = N / 2;   // the kept modes |n_d| <= H
     static_assert(GR::N == NP && std::is_same_v<typename GR::T, T>, "row plan of another lattice");
+    static_assert(pad_pair_ok<GP, N>() && pad_pair_ok<GR, N>(), "padded shape and lattice do not pair");
     static cpx<T>* P(void* p) { return static_cast<cpx<T>*>(p); }
     static constexpr double XN = (double)N * N * N * sizeof(cpx<T>);
-    static constexpr double XP = 8.0 * XN;
-    static constexpr double KP = (N + 1.0) / (2.0 * N);   // share of a padded axis holding modes
+    static constexpr double XP = (double)NP * NP * NP * sizeof(cpx<T>);
+    static constexpr double KP = (N + 1.0) / NP;   // share of a padded axis holding modes
 
     int nV() const override { return GP::BLOCKS_COL; }
+    int n_pad() const override { return NP; }
     int nKG() const override { return (int)((size_t)N * N * N / 2048); }
     size_t padded_elems() const override { return GP::MS; }
 
@@ -748,19 +752,19 @@ struct PadEngine final : PadBase {
     // the sum of V in the x pass
     int front(psc_ctx* c) {
         int rc = timed(c, K_PAD_Z_INV, XN + XP * KP * KP, [&] {
-            hipLaunchKernelGGL((k_pad_z<GR>), dim3(GR::BLOCKS_ROW), dim3(GR::THREADS_ROW), 0, c->stream, P(c->Pf),
+            hipLaunchKernelGGL((k_pad_z<GR, N>), dim3(GR::BLOCKS_ROW), dim3(GR::THREADS_ROW), 0, c->stream, P(c->Pf),
                                P(c->Gf), P(c->Wp), P(c->dcp), P(c->twp));
         });
         if (rc) return rc;
         // the y passes and the stored x rows cover the padded x planes |x| <= N/2;
         // the y inverse reads the rows |y| <= N/2 of them
         rc = timed(c, K_PAD_Y_INV, XP * KP * (KP + 1.0), [&] {
-            hipLaunchKernelGGL((k_strided<GP, 1, +1, true>), dim3(NP / GP::TC, N + 1, 1), dim3(GP::THREADS_COL), 0,
-                               c->stream, P(c->Wp), P(c->twp));
+            hipLaunchKernelGGL((k_strided<GP, 1, +1, true, false, H>), dim3(NP / GP::TC, N + 1, 1),
+                               dim3(GP::THREADS_COL), 0, c->stream, P(c->Wp), P(c->twp));
         });
         if (rc) return rc;
         return timed(c, K_PAD_X_FORCE, XP * KP * 2.0, [&] {
-            hipLaunchKernelGGL((k_x_force<GP, false, true>), dim3(NP / GP::TC, NP, 1), dim3(GP::THREADS_COL), 0,
+            hipLaunchKernelGGL((k_x_force<GP, false, true, H>), dim3(NP / GP::TC, NP, 1), dim3(GP::THREADS_COL), 0,
                                c->stream, P(c->Wp), c->ms, P(c->twp), c->partV, P(c->dcp));
         });
     }
@@ -774,26 +778,26 @@ struct PadEngine final : PadBase {
             return rc;
         if ((rc = front(c))) return rc;
         rc = timed(c, K_PAD_Y_FWD, XP * KP * (1.0 + KP), [&] {
-            hipLaunchKernelGGL((k_strided<GP, 1, -1, true>), dim3(NP / GP::TC, N + 1, 1), dim3(GP::THREADS_COL), 0,
-                               c->stream, P(c->Wp), P(c->twp));
+            hipLaunchKernelGGL((k_strided<GP, 1, -1, true, false, H>), dim3(NP / GP::TC, N + 1, 1),
+                               dim3(GP::THREADS_COL), 0, c->stream, P(c->Wp), P(c->twp));
         });
         if (rc) return rc;
         rc = timed(c, K_PAD_Z_FWD, XN * (N + 1) * (N + 1) / ((double)N * N) * 3, [&] {
-            hipLaunchKernelGGL((k_unpad_z<GR>), dim3(GR::BLOCKS_ROW), dim3(GR::THREADS_ROW), 0, c->stream, P(c->Wp),
+            hipLaunchKernelGGL((k_unpad_z<GR, N>), dim3(GR::BLOCKS_ROW), dim3(GR::THREADS_ROW), 0, c->stream, P(c->Wp),
                                P(c->Rp), P(c->twp));
         });
         if (rc) return rc;
         const StepConsts sc = consts_of(c);
         return timed(c, sync_out ? K_PAD_KICK_SYNC : K_PAD_KICK, (sync_out ? 5 : 4) * XN, [&] {
             if (reduce && sync_out)
-                hipLaunchKernelGGL((k_kick_pad<GP, true, true>), dim3(nKG()), dim3(256), 0, c->stream, P(c->Rp),
+                hipLaunchKernelGGL((k_kick_pad<GP, N, true, true>), dim3(nKG()), dim3(256), 0, c->stream, P(c->Rp),
                                    P(c->Pf), P(c->Pp), P(c->Gf), P(c->Gp), c->ms, c->partKG, ka, sc, P(c->Ps),
                                    P(c->Gps));
             else if (reduce)
-                hipLaunchKernelGGL((k_kick_pad<GP, true>), dim3(nKG()), dim3(256), 0, c->stream, P(c->Rp), P(c->Pf),
+                hipLaunchKernelGGL((k_kick_pad<GP, N, true>), dim3(nKG()), dim3(256), 0, c->stream, P(c->Rp), P(c->Pf),
                                    P(c->Pp), P(c->Gf), P(c->Gp), c->ms, c->partKG, ka, sc, nullptr, nullptr);
             else
-                hipLaunchKernelGGL((k_kick_pad<GP, false>), dim3(nKG()), dim3(256), 0, c->stream, P(c->Rp), P(c->Pf),
+                hipLaunchKernelGGL((k_kick_pad<GP, N, false>), dim3(nKG()), dim3(256), 0, c->stream, P(c->Rp), P(c->Pf),
                                    P(c->Pp), P(c->Gf), P(c->Gp), c->ms, c->partKG, ka, sc, nullptr, nullptr);
         });
     }
@@ -806,7 +810,7 @@ struct PadEngine final : PadBase {
         double v = 0.0;
         HIPCHK(c, hipMemcpyAsync(&v, c->sums, sizeof(double), hipMemcpyDeviceToHost, c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));
-        *V = v / (8.0 * N * N * N);
+        *V = v / ((double)NP * NP * NP);
         return PSC_OK;
     }
 };
@@ -1012,16 +1016,30 @@ struct ShapeEntry {
     int precision, N;
     const char* name;
     EngineBase* (*make)();    // the stepper of the N lattice, or null (padded stepper only)
-    PadBase* (*make_pad)();   // the dealiased stepper of the N/2 lattice, or null
+    PadBase* (*make_pad)(int n);   // the dealiased stepper of the n lattice on this padded
+                                   // shape (n = N/2, or 2N/3 when 3 divides N), or null
 };
 
 template <class G, class GY = G>
 EngineBase* make_one() { return new Engine<G, GY>(); }
 
+// N_pad = 2N (the reference's shipped pair, parameters.h:22-23) and, for the
+// lengths 3 * 2^k, N_pad = 3N/2 (the two-thirds rule of the quadratic terms).
+constexpr bool lattice_length(int n) {   // 2^k (16 .. 512) or 3 * 2^k (48 .. 384)
+    return n == 16 || n == 32 || n == 64 || n == 128 || n == 256 || n == 512 || n == 48 || n == 96 ||
+           n == 192 || n == 384;
+}
+
 template <class G, class GR = G>
-PadBase* make_pad_one() {
-    if constexpr (G::N >= 32) return new PadEngine<G, GR>();
-    else return nullptr;
+PadBase* make_pad_one(int n) {
+    constexpr int NP = G::N;
+    if constexpr (lattice_length(NP / 2) && NP % 2 == 0 && pad_pair_ok<G, NP / 2>() && pad_pair_ok<GR, NP / 2>())
+        if (2 * n == NP) return new PadEngine<G, GR, NP / 2>();
+    if constexpr (NP % 3 == 0 && lattice_length(2 * NP / 3) && pad_pair_ok<G, 2 * NP / 3>() &&
+                  pad_pair_ok<GR, 2 * NP / 3>())
+        if (3 * n == 2 * NP) return new PadEngine<G, GR, 2 * NP / 3>();
+    (void)n;
+    return nullptr;
 }
 
 #define PSC_SHAPE(P, T, N, NAME, ...) {P, N, NAME, &make_one<S<T, N, __VA_ARGS__>>, &make_pad_one<S<T, N, __VA_ARGS__>>}

@@ -114,15 +114,16 @@ k_z_forward(const cpx<typename G::T>* __restrict__ W, cpx<typename G::T>* __rest
 
 // ---------------------------------------------------------------------------
 // B / D: strided-axis pass in place.  AXIS 1 = y (stride N), 0 = x (stride N^2).
-// MAPPED (y passes of the dealiased stepper on its 2N grid): blockIdx.y runs
-// over the N/2 + 1 ... x planes |x| <= N/4 that hold modes; the others stay
-// untouched.  Only the rows |y| <= N/4 of a spectrum hold modes: the inverse
+// MAPPED (y passes of the dealiased stepper on its padded grid): blockIdx.y
+// runs over the 2 MH + 1 x planes |x| <= MH that hold modes (MH = half the
+// unpadded lattice: N/4 for N_pad = 2N, N/3 for N_pad = 3N/2); the others stay
+// untouched.  Only the rows |y| <= MH of a spectrum hold modes: the inverse
 // reads the others as zero (the padding pass does not write them) and the
 // forward does not store them (the extraction pass does not read them).  REV: the blocks take the lattice from its far end (last member,
 // last plane first, in the order of reversed_block), so that a pass starts on
 // the lines the previous pass wrote last, which are still in the 256 MB
 // memory-side cache.
-template <class G, int AXIS, int SIGN, bool MAPPED = false, bool REV = false>
+template <class G, int AXIS, int SIGN, bool MAPPED = false, bool REV = false, int MH = G::N / 4>
 __global__ void __launch_bounds__((G::THREADS_COL))
 k_strided(cpx<typename G::T>* __restrict__ W, const cpx<typename G::T>* __restrict__ tw) {
     using T = typename G::T;
@@ -139,13 +140,13 @@ k_strided(cpx<typename G::T>* __restrict__ W, const cpx<typename G::T>* __restri
         member = (int)(lin / (gridDim.x * gridDim.y));
     }
     const size_t z = (size_t)bx * G::TC + c;
-    const size_t o = MAPPED ? (size_t)((by + N - N / 4) % N) : (size_t)by;
+    const size_t o = MAPPED ? (size_t)((by + N - MH) % N) : (size_t)by;
     constexpr size_t stride = AXIS == 1 ? (size_t)N : G::PS;
     const size_t base = (size_t)member * G::MS + (AXIS == 1 ? o * G::PS : o * N) + z + t * stride;
     static_assert(!MAPPED || AXIS == 1, "mapped passes run along y");
     auto kept = [&](int m) {   // spectrum row y = t + TPL m holds modes
         const int y = t + G::TPL * m;
-        return !MAPPED || y <= N / 4 || y >= N - N / 4;
+        return !MAPPED || y <= MH || y >= N - MH;
     };
     cpx<T> e[G::E];
 #pragma unroll
@@ -161,9 +162,10 @@ k_strided(cpx<typename G::T>* __restrict__ W, const cpx<typename G::T>* __restri
 // C: x-axis inverse, pointwise V'(phi, chi) of model.h (dvdf, model.h:75-82;
 // kernel_field_k_nopad evolution.cpp:1103-1109) and x-axis forward, with the
 // lattice sum of V (kernel_a_k_nopad evolution.cpp:1077-1085) in FP64.
-// PADIN (the dealiased stepper on its 2N grid): the x rows |x| > N/4 of the
-// spectrum are zero and are neither read nor written.
-template <class G, bool REC = false, bool PADIN = false>
+// PADIN (the dealiased stepper on its padded grid): the x rows |x| > PH of the
+// spectrum (PH = half the unpadded lattice) are zero and are neither read nor
+// written.
+template <class G, bool REC = false, bool PADIN = false, int PH = G::N / 4>
 __global__ void __launch_bounds__((G::THREADS_COL)) __attribute__((amdgpu_waves_per_eu(1, G::XF_WAVES)))
 k_x_force(cpx<typename G::T>* __restrict__ W, const MemberState* __restrict__ ms, const cpx<typename G::T>* __restrict__ tw,
           double* __restrict__ partV, const cpx<typename G::T>* __restrict__ Pf,
@@ -180,7 +182,7 @@ k_x_force(cpx<typename G::T>* __restrict__ W, const MemberState* __restrict__ ms
     cpx<T> e[G::E];
     auto kept = [&](int m) {   // spectrum row x = t + TPL m holds data
         const int x = t + G::TPL * m;
-        return !PADIN || x <= N / 4 || x >= N - N / 4;
+        return !PADIN || x <= PH || x >= N - PH;
     };
 #pragma unroll
     for (int m = 0; m < G::E; ++m)

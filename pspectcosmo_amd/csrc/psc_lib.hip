@@ -41,12 +41,21 @@ const std::vector<ShapeEntry>& shape_table() {
 #define kShapes (shape_table())
 inline int kNumShapes() { return (int)shape_table().size(); }
 
-std::vector<int> shapes_for(int precision, int N, bool pad = false) {
+std::vector<int> shapes_for(int precision, int N) {
     std::vector<int> v;
     for (int i = 0; i < kNumShapes(); ++i)
-        if (kShapes[i].precision == precision && kShapes[i].N == N &&
-            (pad ? kShapes[i].make_pad != nullptr : kShapes[i].make != nullptr))
-            v.push_back(i);
+        if (kShapes[i].precision == precision && kShapes[i].N == N && kShapes[i].make != nullptr) v.push_back(i);
+    return v;
+}
+
+// Shapes of the padded length NP that run the dealiased stepper of the N lattice.
+std::vector<int> pad_shapes_for(int precision, int NP, int N) {
+    std::vector<int> v;
+    for (int i = 0; i < kNumShapes(); ++i)
+        if (kShapes[i].precision == precision && kShapes[i].N == NP && kShapes[i].make_pad != nullptr) {
+            std::unique_ptr<PadBase> probe(kShapes[i].make_pad(N));
+            if (probe) v.push_back(i);
+        }
     return v;
 }
 
@@ -393,7 +402,7 @@ std::string tune_key(const psc_ctx* c) {
     std::ostringstream k;
     k << arch << "|N" << c->N << "|" << (c->cfg.precision == PSC_FP32 ? "fp32" : "fp64") << "|B" << c->B
       << "|gw" << c->cfg.with_gw;
-    if (c->cfg.pad) k << "|pad" << c->cfg.pad;
+    if (c->cfg.pad) k << "|pad" << c->NP;
     return k.str();
 }
 
@@ -515,16 +524,23 @@ int psc_create(const psc_config* cfg, psc_ctx** out) {
         (cfg->precision != PSC_FP64 && cfg->precision != PSC_FP32) || !(cfg->kIR > 0) ||
         !(cfg->fStar > 0) || !(cfg->MPl > 0) || !std::isfinite(cfg->dt) ||
         (cfg->expansion != PSC_EXPANSION_SELF && cfg->expansion != PSC_EXPANSION_STATIC) ||
-        (cfg->pad != 0 && cfg->pad != 2) || (cfg->dt_rule != PSC_DT_FIXED && cfg->dt_rule != PSC_DT_REFERENCE))
+        (cfg->pad != 0 && cfg->pad != 2) || (cfg->pad == 0 && cfg->n_pad != 0) ||
+        (cfg->dt_rule != PSC_DT_FIXED && cfg->dt_rule != PSC_DT_REFERENCE))
         return PSC_ERR_ARG;
-    // the dealiased stepper: one member, no GW (the reference pads only the
-    // scalar force), N_pad = 2N within the transform sizes
-    if (cfg->pad == 2 && (cfg->batch != 1 || cfg->with_gw || N > 256)) return PSC_ERR_UNSUPPORTED;
+    // the dealiased stepper: N_pad (parameters.h:23) larger than N
+    // (pspectcosmo.cpp:244-247); 0 selects 2N
+    const int NP = cfg->pad ? (cfg->n_pad ? cfg->n_pad : 2 * N) : 0;
+    if (cfg->pad && NP <= N) return PSC_ERR_ARG;
+    // built: N_pad = 2N and 3N/2 within the transform lengths (up to 512), one
+    // member, no GW (the reference pads only the scalar force)
+    if (cfg->pad && (cfg->batch != 1 || cfg->with_gw || NP > 512 || (NP != 2 * N && 2 * NP != 3 * N)))
+        return PSC_ERR_UNSUPPORTED;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return PSC_ERR_DEVICE;
     std::unique_ptr<psc_ctx> c(new psc_ctx());
     c->cfg = *cfg;
     c->N = N;
+    c->NP = NP;
     c->B = cfg->batch;
     c->N3 = (size_t)N * N * N;
     c->MS = member_stride(N);
@@ -546,8 +562,8 @@ int psc_create(const psc_config* cfg, psc_ctx** out) {
                 if (name == kShapes[idx].name) c->shape = idx;
         }
         c->eng = kShapes[c->shape].make();
-        if (cfg->pad == 2) {
-            const std::vector<int> pc = shapes_for(cfg->precision, 2 * N, true);
+        if (cfg->pad) {
+            const std::vector<int> pc = pad_shapes_for(cfg->precision, NP, N);
             if (pc.empty()) return PSC_ERR_UNSUPPORTED;
             c->pad_shape = pc[0];
             if (cfg->autotune) {
@@ -555,7 +571,7 @@ int psc_create(const psc_config* cfg, psc_ctx** out) {
                 for (int idx : pc)
                     if (name == kShapes[idx].name) c->pad_shape = idx;
             }
-            c->padeng = kShapes[c->pad_shape].make_pad();
+            c->padeng = kShapes[c->pad_shape].make_pad(N);
             if (!c->padeng) return PSC_ERR_UNSUPPORTED;
         }
     }
@@ -573,10 +589,10 @@ int psc_create(const psc_config* cfg, psc_ctx** out) {
     };
     // partial-sum sizes: the most blocks any shape of the table can launch
     // (TC >= 8, L >= 4, asserted in Shape), also after a shape switch; the
-    // dealiased stepper's x pass runs on its 2N grid
+    // dealiased stepper's x pass runs on its padded grid
     size_t blocks_col = (size_t)(N / 8) * N;
     const size_t blocks_row = (size_t)N * N / 4;
-    if (c->padeng) blocks_col = std::max(blocks_col, (size_t)(2 * N / 8) * (2 * N));
+    if (c->padeng) blocks_col = std::max(blocks_col, (size_t)(NP / 8) * NP);
     c->part_len = std::max<size_t>(std::max(blocks_col * kDiagWidth, (size_t)256 * 4),
                                    (size_t)128 * (maxnumbins(N) * 4 + kMaxLogBins));
     c->sums_len = (size_t)maxnumbins(N) * 4 + kMaxLogBins + 64;
@@ -594,7 +610,7 @@ int psc_create(const psc_config* cfg, psc_ctx** out) {
                        !alloc(&c->Rp, (size_t)(N + 1) * (N + 1) * (N + 1) * c->es) ||
                        !alloc(&c->Gf, (size_t)2 * N * N * c->es) || !alloc(&c->Gp, (size_t)2 * N * N * c->es) ||
                        !alloc(&c->Gps, (size_t)2 * N * N * c->es) ||
-                       !alloc(&c->dcp, c->es) || !alloc(&c->twp, (size_t)2 * N * c->es)))) {
+                       !alloc(&c->dcp, c->es) || !alloc(&c->twp, (size_t)NP * c->es)))) {
         psc_destroy(c.release());
         return PSC_ERR_NOMEM;
     }
@@ -617,7 +633,7 @@ int psc_create(const psc_config* cfg, psc_ctx** out) {
                    hipSuccess &&
                hipStreamSynchronize(c->stream) == hipSuccess;
     };
-    if (!twiddles(c->tw, N) || (c->padeng && !twiddles(c->twp, 2 * N))) {
+    if (!twiddles(c->tw, N) || (c->padeng && !twiddles(c->twp, NP))) {
         psc_destroy(c.release());
         return PSC_ERR_DEVICE;
     }
@@ -1387,8 +1403,9 @@ int psc_autotune(psc_ctx* c, int nsteps, char* report, int report_len) {
     };
     int rc = copy(true);
     if (rc) { cleanup(); return rc; }
-    // the dealiased stepper's time is in its 2N passes: tune those
-    const std::vector<int> cand = shapes_for(c->cfg.precision, pad ? 2 * c->N : c->N, pad);
+    // the dealiased stepper's time is in its padded passes: tune those
+    const std::vector<int> cand =
+        pad ? pad_shapes_for(c->cfg.precision, c->NP, c->N) : shapes_for(c->cfg.precision, c->N);
     const int saved_timing = c->timing;
     c->timing = 0;
     double best_ms = 1e300;
@@ -1397,7 +1414,7 @@ int psc_autotune(psc_ctx* c, int nsteps, char* report, int report_len) {
         drop_graphs(c);
         if (pad) {
             delete c->padeng;
-            c->padeng = kShapes[idx].make_pad();
+            c->padeng = kShapes[idx].make_pad(c->N);
             c->pad_shape = idx;
         } else {
             delete c->eng;
@@ -1451,7 +1468,7 @@ const char* psc_shape_name(const psc_ctx* c) {
 int psc_list_shapes(const psc_ctx* c, char* names, int names_len) {
     if (!c || !names || names_len < 1) return PSC_ERR_ARG;
     std::string s;
-    for (int idx : shapes_for(c->cfg.precision, c->padeng ? 2 * c->N : c->N, c->padeng != nullptr)) {
+    for (int idx : c->padeng ? pad_shapes_for(c->cfg.precision, c->NP, c->N) : shapes_for(c->cfg.precision, c->N)) {
         if (!s.empty()) s += ",";
         s += kShapes[idx].name;
     }
@@ -1462,14 +1479,14 @@ int psc_list_shapes(const psc_ctx* c, char* names, int names_len) {
 
 int psc_set_shape(psc_ctx* c, const char* name) {
     if (!c || !name) return PSC_ERR_ARG;
-    if (c->padeng) {   // the dealiased stepper's 2N shape
-        for (int idx : shapes_for(c->cfg.precision, 2 * c->N, true))
+    if (c->padeng) {   // the dealiased stepper's padded shape
+        for (int idx : pad_shapes_for(c->cfg.precision, c->NP, c->N))
             if (std::strcmp(name, kShapes[idx].name) == 0) {
                 HIPCHK(c, hipStreamSynchronize(c->stream));
                 drop_graphs(c);
                 delete c->padeng;
                 c->pad_shape = idx;
-                c->padeng = kShapes[idx].make_pad();
+                c->padeng = kShapes[idx].make_pad(c->N);
                 return PSC_OK;
             }
         return c->fail(PSC_ERR_ARG, std::string("no launch shape named ") + name);

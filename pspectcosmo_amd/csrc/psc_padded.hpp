@@ -1,6 +1,7 @@
 // psc_padded.hpp -- the dealiased stepper (evolve_VV_k_pad with padding,
 // extract_padding, kernel_field_k_pad, kernel_a_k_pad: evolution.cpp:1271-1759)
-// on the device, with N_pad = 2N.
+// on the device, with N_pad = 2N or 3N/2 (any padded length that the
+// Stockham transforms take, 2^k or 3 * 2^k).
 //
 // The N-grid state is the packed full spectrum P = phi_k + i chi_k (as for
 // the unpadded stepper) plus, for the kz = 0 and kz = N/2 planes, a packed
@@ -34,6 +35,22 @@ namespace psc {
 
 __device__ __forceinline__ int pad_signed(int p, int NP) { return p < NP / 2 ? p : p - NP; }
 
+// (N_pad / N)^3, the factor between the padded and the unpadded transform
+// normalisations (extract_padding's normalize_factor, evolution.cpp:1602).
+template <int NP, int N>
+constexpr double pad_volume_ratio() {
+    return ((double)NP / N) * ((double)NP / N) * ((double)NP / N);
+}
+
+// The padded kernels take the padded shape GP and the unpadded lattice N:
+// every mode |n_d| <= N/2 has its own padded index (N_pad > N), and a block of
+// L row lines lies within one padded x plane (L divides N_pad) or covers whole
+// planes (N_pad divides L).
+template <class GP, int N>
+constexpr bool pad_pair_ok() {
+    return GP::N > N && N % 2 == 0 && (GP::N % GP::L == 0 || GP::L % GP::N == 0);
+}
+
 // N-grid index (0..N-1) of a signed index s in [-N/2, N/2] (both ends -> N/2).
 __device__ __forceinline__ int pad_wrap(int s, int N) { return s == -N / 2 ? N / 2 : (s + N) % N; }
 
@@ -65,13 +82,14 @@ k_drift_pad(cpx<T>* __restrict__ Pf, const cpx<T>* __restrict__ Pp, cpx<T>* __re
 // Padding and the z-axis inverse transform on the padded grid.  The k = 0
 // mode goes to dcp (times (N_pad/N)^3) and is left out of the transform, as
 // in the unpadded passes.
-template <class GP>
+template <class GP, int N>
 __global__ void __launch_bounds__((GP::THREADS_ROW))
 k_pad_z(const cpx<typename GP::T>* __restrict__ Pf, const cpx<typename GP::T>* __restrict__ Gf,
         cpx<typename GP::T>* __restrict__ Wp, cpx<typename GP::T>* __restrict__ dcp,
         const cpx<typename GP::T>* __restrict__ tw) {
     using T = typename GP::T;
-    constexpr int NP = GP::N, N = NP / 2, H = N / 2;
+    constexpr int NP = GP::N, H = N / 2;
+    static_assert(pad_pair_ok<GP, N>(), "padded shape and lattice do not pair");
     constexpr size_t PSN = plane_stride(N);
     __shared__ cpx<T> lds[GP::LayRow::size(GP::L)];
     const int t = threadIdx.x % GP::TPL, c = threadIdx.x / GP::TPL;
@@ -84,20 +102,21 @@ k_pad_z(const cpx<typename GP::T>* __restrict__ Pf, const cpx<typename GP::T>* _
     // Padded x planes |x| > N/2 hold no modes: the y passes skip them and the
     // x pass reads them as zero, so nothing is written there; likewise the
     // rows |y| > N/2 of the kept planes, which the y inverse reads as zero.
-    // With L <= N_pad every line of a block has the same padded x index and L
-    // consecutive padded y: a block without modes leaves before the
-    // transform's barriers; otherwise lines without modes transform zeros and
-    // skip the store.
-    if constexpr (GP::L <= NP) {
+    // When L divides N_pad, every line of a block has the same padded x index
+    // and the block covers L consecutive padded y: a block without modes
+    // leaves before the transform's barriers; otherwise lines without modes
+    // transform zeros and skip the store.
+    if constexpr (NP % GP::L == 0) {
         if (!mx) return;
         const int y0 = pad_signed((int)(((size_t)blockIdx.x * GP::L) % NP), NP);
         const int y1 = pad_signed((int)(((size_t)blockIdx.x * GP::L + GP::L - 1) % NP), NP);
-        if (y0 > H && y1 > H) return;            // both ends in (H, N) (no wrap: L <= N and blocks align)
-        if (y0 < -H && y1 < -H) return;          // both ends in [-N, -H)
+        if (y0 > H && y1 > H) return;            // both ends in (H, N_pad/2) (no wrap: blocks align)
+        if (y0 < -H && y1 < -H) return;          // both ends in [-N_pad/2, -H)
     }
     const bool my = mx && sy <= H && sy >= -H;
     const int x = pad_wrap(sx, N), y = pad_wrap(sy, N);
-    const T w = (T)(8.0 * (sx == H || sx == -H ? 0.5 : 1.0) * (sy == H || sy == -H ? 0.5 : 1.0));
+    const T w = (T)(pad_volume_ratio<NP, N>() * (sx == H || sx == -H ? 0.5 : 1.0) *
+                    (sy == H || sy == -H ? 0.5 : 1.0));
 #pragma unroll
     for (int m = 0; m < GP::E; ++m) {
         const int sz = pad_signed(t + GP::TPL * m, NP);
@@ -128,25 +147,26 @@ k_pad_z(const cpx<typename GP::T>* __restrict__ Pf, const cpx<typename GP::T>* _
 
 // z-axis forward transform of the padded lines that hold N-grid modes, the
 // modes |z| <= N/2 stored in R at ((sx + H)(N + 1) + sy + H)(N + 1) + sz + H.
-template <class GP>
+template <class GP, int N>
 __global__ void __launch_bounds__((GP::THREADS_ROW))
 k_unpad_z(const cpx<typename GP::T>* __restrict__ Wp, cpx<typename GP::T>* __restrict__ R,
           const cpx<typename GP::T>* __restrict__ tw) {
     using T = typename GP::T;
-    constexpr int NP = GP::N, N = NP / 2, H = N / 2;
+    constexpr int NP = GP::N, H = N / 2;
+    static_assert(pad_pair_ok<GP, N>(), "padded shape and lattice do not pair");
     __shared__ cpx<T> lds[GP::LayRow::size(GP::L)];
     const int t = threadIdx.x % GP::TPL, c = threadIdx.x / GP::TPL;
     const size_t line = (size_t)blockIdx.x * GP::L + c;
     const int xp = (int)(line / NP), yp = (int)(line % NP);
     const int sx = pad_signed(xp, NP), sy = pad_signed(yp, NP);
     const bool mx = sx <= H && sx >= -H;
-    if constexpr (GP::L <= NP) {   // block-uniform: all lines share the padded x index
+    if constexpr (NP % GP::L == 0) {   // block-uniform: all lines share the padded x index
         if (!mx) return;
         // and cover L consecutive padded y: leave if none of them holds modes
         const int y0 = pad_signed((int)(((size_t)blockIdx.x * GP::L) % NP), NP);
         const int y1 = pad_signed((int)(((size_t)blockIdx.x * GP::L + GP::L - 1) % NP), NP);
-        if (y0 > H && y1 > H) return;            // both ends in (H, N) (no wrap: L <= N and blocks align)
-        if (y0 < -H && y1 < -H) return;          // both ends in [-N, -H)
+        if (y0 > H && y1 > H) return;            // both ends in (H, N_pad/2) (no wrap: blocks align)
+        if (y0 < -H && y1 < -H) return;          // both ends in [-N_pad/2, -H)
     }
     const size_t base = (size_t)xp * GP::PS + (size_t)yp * NP + t;
     cpx<T> e[GP::E];
@@ -164,13 +184,13 @@ k_unpad_z(const cpx<typename GP::T>* __restrict__ Wp, cpx<typename GP::T>* __res
 
 // extract_padding and the kick (kernel_field_k_pad evolution.cpp:1665-1672,
 // kicks :1704-1709, :1737-1742).  For the N-grid mode n with c = 2 per index
-// equal to N/2: outside the kz = 0, N/2 planes E = c R(n with its Nyquist
-// x, y indices at sign(kz) N/2) / 8; in those planes E = c/2 (R(n+) + R(n-)) / 8
-// and the ghost source c/2 (R(n+) - R(n-)) / 8, n+- carrying every Nyquist
-// index at +-N/2.  The residual's k = 0 mode gets N^3 V'(mean) back.  Each
+// equal to N/2 and r = (N_pad/N)^3: outside the kz = 0, N/2 planes
+// E = c R(n with its Nyquist x, y indices at sign(kz) N/2) / r; in those planes
+// E = c/2 (R(n+) + R(n-)) / r and the ghost source c/2 (R(n+) - R(n-)) / r,
+// n+- carrying every Nyquist index at +-N/2.  The residual's k = 0 mode gets N^3 V'(mean) back.  Each
 // thread handles 8 modes; REDUCE sums |pi|^2 after the first kick and
 // n^2 |f|^2, ghosts included (the half-spectrum sums of the reference).
-template <class GP, bool REDUCE, bool SYNC_OUT = false>
+template <class GP, int N, bool REDUCE, bool SYNC_OUT = false>
 __global__ void __launch_bounds__(256)
 k_kick_pad(const cpx<typename GP::T>* __restrict__ R, const cpx<typename GP::T>* __restrict__ Pf,
            cpx<typename GP::T>* __restrict__ Pp, const cpx<typename GP::T>* __restrict__ Gf,
@@ -178,7 +198,8 @@ k_kick_pad(const cpx<typename GP::T>* __restrict__ R, const cpx<typename GP::T>*
            double* __restrict__ partKG, KickArgs ka, StepConsts sc,
            cpx<typename GP::T>* __restrict__ Ps = nullptr, cpx<typename GP::T>* __restrict__ Gps = nullptr) {
     using T = typename GP::T;
-    constexpr int NP = GP::N, N = NP / 2, H = N / 2;
+    constexpr int NP = GP::N, H = N / 2;
+    static_assert(pad_pair_ok<GP, N>(), "padded shape and lattice do not pair");
     constexpr size_t PSN = plane_stride(N);
     __shared__ double red_lds[2 * 256 / 64 + 8];
     if (ka.prologue && ms[0].ahead) return;   // the call's first kick was applied as a look-ahead
@@ -188,7 +209,7 @@ k_kick_pad(const cpx<typename GP::T>* __restrict__ R, const cpx<typename GP::T>*
     double c1d, c2d;
     kick_coefs(ka, ms[0], c1d, c2d);
     const T c1 = (T)c1d, c2 = (T)c2d;
-    const T inv8 = (T)0.125;
+    const T invr = (T)(1.0 / pad_volume_ratio<NP, N>());
     auto ridx = [](int ax, int ay, int az) {
         return ((size_t)(ax + H) * (N + 1) + (ay + H)) * (N + 1) + (az + H);
     };
@@ -205,11 +226,11 @@ k_kick_pad(const cpx<typename GP::T>* __restrict__ R, const cpx<typename GP::T>*
         if (!plane) {
             const int s = sz > 0 ? H : -H;
             const cpx<T> r = R[ridx(X ? s : sx, Y ? s : sy, sz)];
-            E = cpx<T>{cc * r.x * inv8, cc * r.y * inv8};
+            E = cpx<T>{cc * r.x * invr, cc * r.y * invr};
         } else {
             const cpx<T> rp = R[ridx(X ? H : sx, Y ? H : sy, Z ? H : 0)];
             const cpx<T> rm = R[ridx(X ? -H : sx, Y ? -H : sy, Z ? -H : 0)];
-            const T h = (T)0.5 * cc * inv8;
+            const T h = (T)0.5 * cc * invr;
             E = cpx<T>{h * (rp.x + rm.x), h * (rp.y + rm.y)};
             GE = cpx<T>{h * (rp.x - rm.x), h * (rp.y - rm.y)};
         }

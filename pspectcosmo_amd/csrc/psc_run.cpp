@@ -84,7 +84,8 @@ const char* kKeys[][2] = {
     {"infreq_steps", "fixed dt: steps between spectra"},
     {"snap_steps", "fixed dt: steps between snapshots (0: none)"},
     {"integrator", "VV (2) or Y4 (4)"},
-    {"pad", "0: evolve_VV_k_nopad; 2: the dealiased stepper with N_pad = 2N (PAD_FLAG)"},
+    {"pad", "0: evolve_VV_k_nopad; 2: the dealiased stepper (PAD_FLAG)"},
+    {"N_pad", "padded lattice of the dealiased stepper: 2N or 3N/2 (0: 2N)"},
     {"yoshida_exact", "1: Yoshida weights from 2^(1/3); 0: the reference's digits"},
     {"precision", "fp64 or fp32"},
     {"expansion", "0 no expansion, 2 self-consistent"},
@@ -458,6 +459,7 @@ int main(int argc, char** argv) {
     c.batch = 1;
     c.with_gw = (int)p.num("with_gw", 0);
     c.pad = (int)p.num("pad", 0);
+    c.n_pad = (int)p.num("N_pad", 0);
     c.alpha = (int)p.num("alpha", 0);
     c.expansion = (int)p.num("expansion", 2) == 0 ? PSC_EXPANSION_STATIC : PSC_EXPANSION_SELF;
     c.use_graph = (int)p.num("use_graph", 1);

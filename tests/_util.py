@@ -25,13 +25,13 @@ def reference_dt(N, kIR, a=1.0):
 def parity_setup(N=32, nflds=2, integrator=PSC_VV2, precision=PSC_FP64, omega_star=1e-4,
                  kIR=25.0, dt=None, g2=G2_PREHEAT, m2=1.0, seed=1, batch=1, use_graph=0,
                  expansion=PSC_EXPANSION_SELF, yoshida_exact=0, f0=(1.0, 0.0), fd0=(-0.7, 0.0),
-                 with_gw=0, pad=0):
+                 with_gw=0, pad=0, n_pad=0):
     if dt is None:
         dt = reference_dt(N, kIR)
     cfg = make_config(N, nflds=nflds, integrator=integrator, precision=precision, batch=batch,
                       kIR=kIR, dt=dt, omegaStar=omega_star, fStar=1.0, MPl=1.0,
                       use_graph=use_graph, expansion=expansion, yoshida_exact=yoshida_exact,
-                      with_gw=with_gw, pad=pad)
+                      with_gw=with_gw, pad=pad, n_pad=n_pad)
     model = make_model(m2, g2 if nflds == 2 else 0.0)
     ic = make_ic(f0=f0, fd0=fd0, seed=seed)
     return cfg, model, ic

@@ -155,10 +155,12 @@ def test_device_initial_conditions_give_the_same_run(tmp_path):
         close(rb, ra, rtol=2e-6, atol=1e-300)
 
 
-def test_padded_run_matches_oracle(tmp_path):
-    run_driver(tmp_path, "dt=0.01", "nsteps=10", "freq_steps=5", "infreq_steps=10", "pad=2")
-    cfg, model, ic = parity_setup(N=16, kIR=4.0, dt=0.01, g2=2e3, omega_star=2e-2, f0=(1.0, 0.0),
-                                  fd0=(-0.7, 0.0), seed=1, pad=2)
+@pytest.mark.parametrize("N,n_pad", [(16, 0), (32, 48)])
+def test_padded_run_matches_oracle(tmp_path, N, n_pad):
+    run_driver(tmp_path, "dt=0.01", "nsteps=10", "freq_steps=5", "infreq_steps=10", "pad=2", f"N={N}",
+               f"N_pad={n_pad}")
+    cfg, model, ic = parity_setup(N=N, kIR=4.0, dt=0.01, g2=2e3, omega_star=2e-2, f0=(1.0, 0.0),
+                                  fd0=(-0.7, 0.0), seed=1, pad=2, n_pad=n_pad)
     fk, pk, sc = O.generate_ic(cfg, model, ic)
     o = O.Oracle(cfg, model)
     o.load(fk, pk, 0.0, sc[0], sc[1], 0.0)
