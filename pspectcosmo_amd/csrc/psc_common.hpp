@@ -117,6 +117,13 @@ struct Shape {
     // others; FP64 and N = 512 are unchanged by any cap (and at 512 a cap
     // would cost the second block per CU), so they keep none (8).
     static constexpr int XF_WAVES = (sizeof(T_) == 4 && N_ <= 256) ? 3 : 8;
+    // The force x pass runs in its multi-tile form (k_x_force_mt: blocks loop
+    // over tiles, the next two tiles' columns prefetched into registers) where
+    // one block fills a CU: measured 23 % faster at 512 points FP32 (and 15-30 %
+    // for the padded 512- and 384-point passes and the plain 384 one), 6 % in
+    // FP64 at 512; no gain at 256 points and below, where several blocks per CU
+    // already overlap their loads with each other's transforms.
+    static constexpr bool XMT = sizeof(T_) == 4 ? N_ >= 384 : N_ >= 512;
     static constexpr int BLOCKS_ROW = (N * N) / L;    // per member
     using LayRow = LayoutRow<N, R1>;
     using LayCol = LayoutCol<N, TC>;

@@ -1,15 +1,29 @@
 // psc_engine.hpp -- the stepper's kernel sequencing for one launch shape:
 // Engine<G, GY> runs evolve_VV_k_nopad (evolution.cpp:1131-1198) with the
 // GW chain, diagnostics, spectra and the adjoint on the passes of shape G
-// (the y passes on GY); PadEngine<GP, GR> runs the dealiased stepper's force
-// chain (evolve_VV_k_pad, evolution.cpp:1271-1759) on the 2N shape GP.  The
-// shape instantiations live in psc_shapes_*.hip, one group per translation
-// unit, so that they compile in parallel.
+// (the y passes on GY); PadEngine<GP, GR, N> runs the dealiased stepper's
+// force chain (evolve_VV_k_pad, evolution.cpp:1271-1759) of the N lattice on
+// the padded shape GP.  The shape instantiations live in psc_shapes_*.hip, one
+// group per translation unit, so that they compile in parallel.
 #pragma once
+#include <algorithm>
+
 #include "psc_ctx.hpp"
 #include "psc_padded.hpp"
 
 namespace psc {
+
+// Grid of a multi-tile kernel: as many blocks as the current device keeps
+// resident at once (compute units x blocks per unit), at most one per tile.
+inline int resident_blocks(const void* kernel, int threads, int tiles) {
+    int dev = 0, cus = 0, per_cu = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1)
+        cus = 256;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, 0) != hipSuccess || per_cu < 1)
+        per_cu = 1;
+    return std::min(tiles, cus * per_cu);
+}
 
 // G drives every pass but the y-axis ones, which run on GY: a plan with more
 // columns per block can suit the strided y passes where it would starve the
@@ -18,7 +32,11 @@ template <class G, class GY = G>
 struct Engine final : EngineBase {
     using T = typename G::T;
     static constexpr int N = G::N;
-    static int nV_blocks(const psc_ctx*) { return G::BLOCKS_COL; }
+    // blocks per member of the force x pass: one per tile, or for the
+    // multi-tile form one per resident slot of the device
+    const int xblocks = G::XMT ? resident_blocks((const void*)k_x_force_mt<G>, G::THREADS_COL, G::BLOCKS_COL)
+                               : G::BLOCKS_COL;
+    int nV_blocks(const psc_ctx*) const { return xblocks; }
     // explicit kick coefficients (the adjoint's replays); prologue: the first
     // kick at a_start only
     static KickArgs kick_args(double c1, double c2, bool prologue = false) {
@@ -88,10 +106,15 @@ struct Engine final : EngineBase {
         if (n < 0) n = c->B;
         const size_t off = (size_t)m0 * G::MS;
         const double X = (double)n * G::N3 * sizeof(cpx<T>);
-        dim3 grid(N / G::TC, N, n), block(G::THREADS_COL);
         return timed(c, K_X_FORCE, 2 * X, [&] {
-            hipLaunchKernelGGL((k_x_force<G>), grid, block, 0, c->stream, P(c->W) + off, c->ms + m0, P(c->tw),
-                               c->partV + (size_t)m0 * G::BLOCKS_COL, P(c->Pf) + off);
+            if constexpr (G::XMT)
+                hipLaunchKernelGGL((k_x_force_mt<G>), dim3(xblocks, 1, n), dim3(G::THREADS_COL), 0, c->stream,
+                                   P(c->W) + off, c->ms + m0, P(c->tw), c->partV + (size_t)m0 * xblocks,
+                                   P(c->Pf) + off);
+            else
+                hipLaunchKernelGGL((k_x_force<G>), dim3(N / G::TC, N, n), dim3(G::THREADS_COL), 0, c->stream,
+                                   P(c->W) + off, c->ms + m0, P(c->tw), c->partV + (size_t)m0 * G::BLOCKS_COL,
+                                   P(c->Pf) + off);
         });
     }
     // z kick of members m0 .. m0 + n - 1; SYNC_OUT also stores the
@@ -742,8 +765,12 @@ struct PadEngine final : PadBase {
     static constexpr double XN = (double)N * N * N * sizeof(cpx<T>);
     static constexpr double XP = (double)NP * NP * NP * sizeof(cpx<T>);
     static constexpr double KP = (N + 1.0) / NP;   // share of a padded axis holding modes
+    // blocks of the padded x pass (multi-tile form: one per resident slot)
+    const int xblocks = GP::XMT ? resident_blocks((const void*)k_x_force_mt<GP, true, H>, GP::THREADS_COL,
+                                                  GP::BLOCKS_COL)
+                                : GP::BLOCKS_COL;
 
-    int nV() const override { return GP::BLOCKS_COL; }
+    int nV() const override { return xblocks; }
     int n_pad() const override { return NP; }
     int nKG() const override { return (int)((size_t)N * N * N / 2048); }
     size_t padded_elems() const override { return GP::MS; }
@@ -764,8 +791,12 @@ struct PadEngine final : PadBase {
         });
         if (rc) return rc;
         return timed(c, K_PAD_X_FORCE, XP * KP * 2.0, [&] {
-            hipLaunchKernelGGL((k_x_force<GP, false, true, H>), dim3(NP / GP::TC, NP, 1), dim3(GP::THREADS_COL), 0,
-                               c->stream, P(c->Wp), c->ms, P(c->twp), c->partV, P(c->dcp));
+            if constexpr (GP::XMT)
+                hipLaunchKernelGGL((k_x_force_mt<GP, true, H>), dim3(xblocks, 1, 1), dim3(GP::THREADS_COL), 0,
+                                   c->stream, P(c->Wp), c->ms, P(c->twp), c->partV, P(c->dcp));
+            else
+                hipLaunchKernelGGL((k_x_force<GP, false, true, H>), dim3(NP / GP::TC, NP, 1), dim3(GP::THREADS_COL),
+                                   0, c->stream, P(c->Wp), c->ms, P(c->twp), c->partV, P(c->dcp));
         });
     }
 

@@ -158,6 +158,31 @@ k_strided(cpx<typename G::T>* __restrict__ W, const cpx<typename G::T>* __restri
         if (SIGN > 0 || kept(m)) W[base + (size_t)G::TPL * m * stride] = e[m];
 }
 
+// The pointwise step of the x pass on a transformed column: the fields
+// f = mean + fluctuation, V'(f) - V'(mean) in place of the spectrum (the k = 0
+// mode of V'(mean) is added back in the z pass) and the sum of V in FP64.
+template <class G, bool REC>
+__device__ __forceinline__ void x_force_pointwise(cpx<typename G::T> (&e)[G::E], const typename G::T (&mean)[2], int nf,
+                                                  const Couplings<typename G::T>& cp, double& vsum,
+                                                  cpx<typename G::T>* __restrict__ rec_r, size_t base, size_t stride) {
+    using T = typename G::T;
+    const T inv = (T)(1.0 / (double)G::N3);
+#pragma unroll
+    for (int m = 0; m < G::E; ++m) {
+        const T d[2] = {e[m].x * inv, nf > 1 ? e[m].y * inv : (T)0};   // fluctuations
+        const T f[2] = {mean[0] + d[0], mean[1] + d[1]};
+        if constexpr (REC) rec_r[base + (size_t)G::TPL * m * stride] = cpx<T>{f[0], f[1]};  // adjoint recording
+        // V'(f) - V'(mean) (dvdf, model.h:75-82), written so that every term
+        // carries a fluctuation factor (no cancellation against the mean)
+        T dv[2];
+        dvdf_delta<Model, T>(mean, d, dv, cp);
+        e[m].x = dv[0];
+        e[m].y = nf > 1 ? dv[1] : (T)0;
+        // V per site in the field precision, summed in FP64
+        vsum += (double)potential_sum<Model, T>(nf, f, cp);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // C: x-axis inverse, pointwise V'(phi, chi) of model.h (dvdf, model.h:75-82;
 // kernel_field_k_nopad evolution.cpp:1103-1109) and x-axis forward, with the
@@ -194,24 +219,9 @@ k_x_force(cpx<typename G::T>* __restrict__ W, const MemberState* __restrict__ ms
     const int nf = ms[member].nflds;
     const cpx<T> dc = Pf[(size_t)member * G::MS];
     line_fft<T, N, typename G::Rad, +1, typename G::LayCol>(e, t, c, lds, tw);
-    const T inv = (T)(1.0 / (double)G::N3);
     const T mean[2] = {(T)((double)dc.x / (double)G::N3), nf > 1 ? (T)((double)dc.y / (double)G::N3) : (T)0};
     double v[1] = {0.0};
-#pragma unroll
-    for (int m = 0; m < G::E; ++m) {
-        const T d[2] = {e[m].x * inv, nf > 1 ? e[m].y * inv : (T)0};   // fluctuations
-        const T f[2] = {mean[0] + d[0], mean[1] + d[1]};
-        if constexpr (REC) rec_r[base + (size_t)G::TPL * m * stride] = cpx<T>{f[0], f[1]};  // adjoint recording
-        // V'(f) - V'(mean) (dvdf, model.h:75-82), written so that every term
-        // carries a fluctuation factor (no cancellation against the mean);
-        // the k = 0 mode of V'(mean) is added back in the z pass
-        T dv[2];
-        dvdf_delta<Model, T>(mean, d, dv, cp);
-        e[m].x = dv[0];
-        e[m].y = nf > 1 ? dv[1] : (T)0;
-        // V per site in the field precision, summed in FP64
-        v[0] += (double)potential_sum<Model, T>(nf, f, cp);
-    }
+    x_force_pointwise<G, REC>(e, mean, nf, cp, v[0], rec_r, base, stride);
     // In FP64 the forward transform reloads its twiddles (cached) instead of
     // keeping the inverse transform's live across the V' step: an opaque copy
     // of the table pointer.  That lowers the register count enough for two
@@ -226,6 +236,70 @@ k_x_force(cpx<typename G::T>* __restrict__ W, const MemberState* __restrict__ ms
     block_sum<1, G::THREADS_COL>(v, reinterpret_cast<double*>(lds));
     if (threadIdx.x == 0)
         partV[(size_t)member * G::BLOCKS_COL + blockIdx.y * gridDim.x + blockIdx.x] = v[0];
+}
+
+// C, multi-tile form for the long columns (512 points): each block loops over
+// the tiles b, b + G, b + 2G, ... of its member (G = gridDim.x, one block per
+// resident slot) and holds the next two tiles' columns in registers while it
+// transforms the current one.  On gfx950 the block barriers of the transforms
+// wait only on LDS, so the loads stay in flight across them: the load latency
+// that one block per CU would otherwise expose is hidden behind the previous
+// tiles' transforms.  The same arithmetic per column as k_x_force (bit-equal
+// results); one V partial per block, at partV[member * gridDim.x + block].
+template <class G, bool PADIN = false, int PH = G::N / 4>
+__global__ void __launch_bounds__((G::THREADS_COL)) __attribute__((amdgpu_waves_per_eu(1, G::XF_WAVES)))
+k_x_force_mt(cpx<typename G::T>* __restrict__ W, const MemberState* __restrict__ ms,
+             const cpx<typename G::T>* __restrict__ tw, double* __restrict__ partV,
+             const cpx<typename G::T>* __restrict__ Pf) {
+    using T = typename G::T;
+    constexpr int N = G::N;
+    constexpr int NT = (N / G::TC) * N;   // tiles (TC columns x one y row) per member
+    constexpr size_t stride = G::PS;
+    __shared__ cpx<T> lds[G::LayCol::size(0)];
+    const int c = threadIdx.x % G::TC, t = threadIdx.x / G::TC;
+    const int member = blockIdx.z;
+    const int GS = (int)gridDim.x;
+    auto tile_base = [&](int q) {
+        return (size_t)member * G::MS + (size_t)(q / (N / G::TC)) * N + (size_t)(q % (N / G::TC)) * G::TC + c +
+               (size_t)t * stride;
+    };
+    auto kept = [&](int m) {   // spectrum row x = t + TPL m holds data
+        const int x = t + G::TPL * m;
+        return !PADIN || x <= PH || x >= N - PH;
+    };
+    auto load = [&](cpx<T> (&dst)[G::E], int q) {
+        const size_t b = tile_base(q);
+#pragma unroll
+        for (int m = 0; m < G::E; ++m) dst[m] = kept(m) ? W[b + (size_t)G::TPL * m * stride] : cpx<T>{(T)0, (T)0};
+    };
+    const Couplings<T> cp = couplings<T>(ms[member]);
+    const int nf = ms[member].nflds;
+    const cpx<T> dc = Pf[(size_t)member * G::MS];
+    const T mean[2] = {(T)((double)dc.x / (double)G::N3), nf > 1 ? (T)((double)dc.y / (double)G::N3) : (T)0};
+    double v = 0.0;
+    cpx<T> n1[G::E], n2[G::E];
+    int q = blockIdx.x;
+    if (q < NT) load(n1, q);
+    if (q + GS < NT) load(n2, q + GS);
+    for (; q < NT; q += GS) {
+        cpx<T> e[G::E];
+#pragma unroll
+        for (int m = 0; m < G::E; ++m) {
+            e[m] = n1[m];
+            n1[m] = n2[m];
+        }
+        if (q + 2 * GS < NT) load(n2, q + 2 * GS);
+        line_fft<T, N, typename G::Rad, +1, typename G::LayCol>(e, t, c, lds, tw);
+        x_force_pointwise<G, false>(e, mean, nf, cp, v, nullptr, 0, stride);
+        line_fft<T, N, typename G::Rad, -1, typename G::LayCol>(e, t, c, lds, tw);
+        const size_t base = tile_base(q);
+#pragma unroll
+        for (int m = 0; m < G::E; ++m)
+            if (kept(m)) W[base + (size_t)G::TPL * m * stride] = e[m];
+    }
+    double vv[1] = {v};
+    block_sum<1, G::THREADS_COL>(vv, reinterpret_cast<double*>(lds));
+    if (threadIdx.x == 0) partV[(size_t)member * gridDim.x + blockIdx.x] = vv[0];
 }
 
 // The k = 0 mode of V' that the x pass leaves out: N^3 V'(pb, qb) with the
@@ -961,6 +1035,81 @@ k_x_source(cpx<typename G::T>* __restrict__ W, const cpx<typename G::T>* __restr
         line_fft<T, N, typename G::Rad, -1, typename G::LayCol>(g[a], t, c, lds, twa);
 #pragma unroll
         for (int m = 0; m < G::E; ++m) W[(size_t)(a + 1) * astride + base + (size_t)G::TPL * m * stride] = g[a][m];
+    }
+}
+
+// C (GW), first part, multi-tile form for the long columns (512 points): the
+// arithmetic of k_x_source per tile, each block looping over the tiles b,
+// b + G, ... of its member.  The next tile's three columns are loaded into
+// the registers of each array as soon as that array's forward transform is
+// stored, so the loads of tile q + G overlap the remaining transforms of tile
+// q and the first inverse transforms of tile q + G, with no registers beyond
+// the three columns.
+template <class G>
+__global__ void __launch_bounds__((G::THREADS_COL))
+k_x_source_mt(cpx<typename G::T>* __restrict__ W, const cpx<typename G::T>* __restrict__ tw, size_t astride,
+              double kIR) {
+    using T = typename G::T;
+    constexpr int N = G::N;
+    constexpr int NT = (N / G::TC) * N;
+    constexpr size_t stride = G::PS;
+    __shared__ cpx<T> lds[G::LayCol::size(0)];
+    const int c = threadIdx.x % G::TC, t = threadIdx.x / G::TC;
+    const int member = blockIdx.z;
+    const int GS = (int)gridDim.x;
+    const T inv = (T)(1.0 / (double)G::N3);
+    auto tile_base = [&](int q) {
+        return (size_t)member * G::MS + (size_t)(q / (N / G::TC)) * N + (size_t)(q % (N / G::TC)) * G::TC + c +
+               (size_t)t * stride;
+    };
+    // array 0: the fields (W0), whose x gradient is formed after the load;
+    // arrays 1, 2: the y and z gradients (W2, W3)
+    auto in_off = [&](int a) { return a == 0 ? (size_t)0 : (size_t)(a + 1) * astride; };
+    cpx<T> g[3][G::E];
+    int q = blockIdx.x;
+    if (q < NT) {
+        const size_t b = tile_base(q);
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int m = 0; m < G::E; ++m) g[a][m] = W[in_off(a) + b + (size_t)G::TPL * m * stride];
+    }
+    for (; q < NT; q += GS) {
+        const size_t base = tile_base(q);
+#pragma unroll
+        for (int m = 0; m < G::E; ++m) {
+            const T kx = (T)(kIR * deriv_index(t + G::TPL * m, N));
+            g[0][m] = cpx<T>{-kx * g[0][m].y, kx * g[0][m].x};
+        }
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const cpx<T>* twa = tw;   // reloaded per transform (cached), as in k_x_source
+            asm volatile("" : "+s"(twa));
+            line_fft<T, N, typename G::Rad, +1, typename G::LayCol>(g[a], t, c, lds, twa);
+        }
+#pragma unroll
+        for (int m = 0; m < G::E; ++m) {
+            const T ax = g[0][m].x * inv, bx = g[0][m].y * inv;
+            const T ay = g[1][m].x * inv, by = g[1][m].y * inv;
+            const T az = g[2][m].x * inv, bz = g[2][m].y * inv;
+            g[0][m] = cpx<T>{ax * ax + bx * bx, ay * ay + by * by};
+            g[1][m] = cpx<T>{az * az + bz * bz, ax * ay + bx * by};
+            g[2][m] = cpx<T>{ax * az + bx * bz, ay * az + by * bz};
+        }
+        const bool more = q + GS < NT;
+        const size_t nb = more ? tile_base(q + GS) : 0;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const cpx<T>* twa = tw;
+            asm volatile("" : "+s"(twa));
+            line_fft<T, N, typename G::Rad, -1, typename G::LayCol>(g[a], t, c, lds, twa);
+#pragma unroll
+            for (int m = 0; m < G::E; ++m) W[(size_t)(a + 1) * astride + base + (size_t)G::TPL * m * stride] = g[a][m];
+            if (more) {
+#pragma unroll
+                for (int m = 0; m < G::E; ++m) g[a][m] = W[in_off(a) + nb + (size_t)G::TPL * m * stride];
+            }
+        }
     }
 }
 
