@@ -33,9 +33,15 @@ struct MemberState {
     double t_out;     // adaptive rule: the output time the steps land on
     double dt_cap;    // adaptive rule: largest dt (tOutputFreq)
     double steps;     // steps taken with dt > 0 since the data were loaded
-    double reserved[4];
+    // FP32 path: the homogeneous mode (k = 0) of f and pi is a compensated
+    // pair, the FP32 array element plus these FP64 remainders (re, im), so
+    // that thousands of drifts and kicks do not accumulate FP32 rounding in
+    // it; dc_ps belongs to the synchronised pi of a state left ahead.  Zero
+    // in the FP64 path.
+    double dc_f[2], dc_p[2], dc_ps[2];
+    double reserved[6];
 };
-static_assert(sizeof(MemberState) == 256, "MemberState layout");
+static_assert(sizeof(MemberState) == 320, "MemberState layout");
 
 // Kick coefficients of a z-kick launch.
 struct KickArgs {

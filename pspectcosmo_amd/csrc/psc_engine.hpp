@@ -65,7 +65,7 @@ struct Engine final : EngineBase {
     static cpx<T>* P(void* p) { return static_cast<cpx<T>*>(p); }
 
     int launch_z_inverse(psc_ctx* c, bool drift, const cpx<T>* src, cpx<T>* Pf, const cpx<T>* Pp,
-                         cpx<T>* W, const MemberState* ms, int members) {
+                         cpx<T>* W, MemberState* ms, int members) {
         const double X = (double)members * G::N3 * sizeof(cpx<T>);
         dim3 grid(G::BLOCKS_ROW, members), block(G::THREADS_ROW);
         if (drift)

@@ -36,7 +36,7 @@ __device__ __forceinline__ unsigned reversed_block(unsigned lin, unsigned total)
 template <class G, bool DRIFT, bool REC = false, bool REV = false, bool STREAM = false>
 __global__ void __launch_bounds__((G::THREADS_ROW))
 k_z_inverse(const cpx<typename G::T>* __restrict__ src, cpx<typename G::T>* __restrict__ Pf, const cpx<typename G::T>* __restrict__ Pp,
-            cpx<typename G::T>* __restrict__ W, const MemberState* __restrict__ ms,
+            cpx<typename G::T>* __restrict__ W, MemberState* __restrict__ ms,
             const cpx<typename G::T>* __restrict__ tw, cpx<typename G::T>* __restrict__ rec_f = nullptr,
             cpx<typename G::T>* __restrict__ rec_ph = nullptr) {
     using T = typename G::T;
@@ -69,8 +69,12 @@ k_z_inverse(const cpx<typename G::T>* __restrict__ src, cpx<typename G::T>* __re
                 rec_f[base + G::TPL * m] = f;
                 rec_ph[base + G::TPL * m] = p;
             }
-            f.x += cd * p.x;
-            f.y += cd * p.y;
+            if (sizeof(T) == 4 && m == 0 && line == 0 && t == 0) {
+                dc_drift(f, p, ms[member]);   // the homogeneous mode, compensated
+            } else {
+                f.x += cd * p.x;
+                f.y += cd * p.y;
+            }
             if constexpr (STREAM)
                 store_stream(Pf + base + G::TPL * m, f);
             else
@@ -312,6 +316,85 @@ __device__ __forceinline__ void add_mean_force(cpx<T>& vk, const cpx<T>& f0, con
     if (s.nflds > 1) vk.y += (T)(N3 * Model::dvdf<double>(1, mean, cp));
 }
 
+// The homogeneous mode (k = 0) of the FP32 path, a compensated pair (the FP32
+// array element plus the FP64 remainder in MemberState): the drift
+// f += c_drift pi in FP64 (evolution.cpp:1164-1170).
+template <typename T>
+__device__ __forceinline__ void dc_drift(cpx<T>& f, const cpx<T>& p, MemberState& s) {
+    const double fx = (double)f.x + s.dc_f[0] + s.c_drift * ((double)p.x + s.dc_p[0]);
+    const double fy = (double)f.y + s.dc_f[1] + s.c_drift * ((double)p.y + s.dc_p[1]);
+    f.x = (T)fx;
+    f.y = (T)fy;
+    s.dc_f[0] = fx - (double)f.x;
+    s.dc_f[1] = fy - (double)f.y;
+}
+
+// ... and its two kicks in FP64: force F = -a^{3+alpha} (E + N^3 V'(mean)) with
+// E the k = 0 residual of the transformed V' and the mean from the compensated
+// field (n = 0: no Laplacian term); q1 = pi after the first kick, q2 after
+// both (kernel_field_k_nopad evolution.cpp:1115-1127, kicks :1153-1158,
+// :1184-1189).
+template <typename T>
+__device__ __forceinline__ void dc_kick(const cpx<T>& e0, const cpx<T>& f, const cpx<T>& p, const MemberState& s,
+                                        double N3, double vc, double c1, double c2, double (&q1)[2],
+                                        double (&q2)[2]) {
+    const Couplings<double> cp(s.par);
+    const double mean[2] = {((double)f.x + s.dc_f[0]) / N3, s.nflds > 1 ? ((double)f.y + s.dc_f[1]) / N3 : 0.0};
+    const double F[2] = {-vc * ((double)e0.x + N3 * Model::dvdf<double>(0, mean, cp)),
+                         -vc * ((double)e0.y + (s.nflds > 1 ? N3 * Model::dvdf<double>(1, mean, cp) : 0.0))};
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+        q1[i] = (i ? (double)p.y : (double)p.x) + s.dc_p[i] + c1 * F[i];
+        q2[i] = q1[i] + c2 * F[i];
+    }
+}
+
+// Both kicks of one field mode with the force F = -a^{1+alpha} kIR^2 n^2 f -
+// a^{3+alpha} E (kernel_field_k_nopad evolution.cpp:1115-1127, kicks
+// :1153-1158, :1184-1189); E is the mode's V' spectrum.  dc marks the
+// homogeneous mode (its N^3 V'(mean) added in FP64; in the FP32 path its kicks
+// on the compensated pair).  Between the kicks: the adjoint recording (REC),
+// the synchronised store of a call's last kick (SYNC_OUT, streaming) and the
+// sums of |pi|^2 and n^2 |f|^2 (REDUCE).  The caller stores p.
+template <typename T, bool REC, bool SYNC_OUT, bool REDUCE>
+__device__ __forceinline__ void kick_mode(cpx<T>& E, const cpx<T>& f, cpx<T>& p, int n2i, bool dc, MemberState& s,
+                                          double N3, T lapc, T vc, double vcd, double c1d, double c2d,
+                                          double (&red)[2], cpx<T>* rec_p, cpx<T>* ps) {
+    const T c1 = (T)c1d, c2 = (T)c2d;
+    if (sizeof(T) == 8 && dc) add_mean_force(E, f, s, N3);
+    const bool dc32 = sizeof(T) == 4 && dc;
+    double q1[2] = {0.0, 0.0}, q2[2] = {0.0, 0.0};
+    if (dc32) dc_kick(E, f, p, s, N3, vcd, c1d, c2d, q1, q2);
+    const T ln2 = lapc * (T)n2i;
+    const cpx<T> F = {-ln2 * f.x - vc * E.x, -ln2 * f.y - vc * E.y};
+    p.x += c1 * F.x;
+    p.y += c1 * F.y;
+    if (dc32) p = cpx<T>{(T)q1[0], (T)q1[1]};
+    if constexpr (REC) *rec_p = p;   // adjoint recording: synchronised pi
+    // synchronised pi at the end of a call, read only if the state is
+    // synchronised later: a streaming store
+    if constexpr (SYNC_OUT) {
+        store_stream(ps, p);
+        if (dc32) {
+            s.dc_ps[0] = q1[0] - (double)p.x;
+            s.dc_ps[1] = q1[1] - (double)p.y;
+        }
+    }
+    if constexpr (REDUCE) {
+        const double px_ = dc32 ? q1[0] : (double)p.x, py_ = dc32 ? q1[1] : (double)p.y;
+        const double fx = f.x, fy = f.y;
+        red[0] += px_ * px_ + py_ * py_;
+        red[1] += (double)n2i * (fx * fx + fy * fy);
+    }
+    p.x += c2 * F.x;
+    p.y += c2 * F.y;
+    if (dc32) {
+        p = cpx<T>{(T)q2[0], (T)q2[1]};
+        s.dc_p[0] = q2[0] - (double)p.x;
+        s.dc_p[1] = q2[1] - (double)p.y;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // E: z-axis forward pass with the kick fused (kernel_field_k_nopad
 // evolution.cpp:1115-1127 and the kicks :1153-1158, :1184-1189).  W holds the
@@ -321,7 +404,7 @@ __device__ __forceinline__ void add_mean_force(cpx<T>& vk, const cpx<T>& f0, con
 template <class G, bool REDUCE, bool REC = false, bool STREAM = false, bool SYNC_OUT = false>
 __global__ void __launch_bounds__((G::THREADS_ROW))
 k_z_kick(const cpx<typename G::T>* __restrict__ W, const cpx<typename G::T>* __restrict__ Pf, cpx<typename G::T>* __restrict__ Pp,
-         const MemberState* __restrict__ ms, const cpx<typename G::T>* __restrict__ tw,
+         MemberState* __restrict__ ms, const cpx<typename G::T>* __restrict__ tw,
          double* __restrict__ partKG, KickArgs ka, StepConsts sc,
          cpx<typename G::T>* __restrict__ rec_p = nullptr, cpx<typename G::T>* __restrict__ Ps = nullptr) {
     using T = typename G::T;
@@ -340,10 +423,10 @@ k_z_kick(const cpx<typename G::T>* __restrict__ W, const cpx<typename G::T>* __r
     line_fft<T, N, typename G::Rad, -1, typename G::LayRow>(e, t, c, lds, tw);
     const double a = ka.prologue ? ms[member].a_start : ms[member].a;
     const T lapc = (T)(pow(a, 1 + sc.alpha) * sc.kIR * sc.kIR);
-    const T vc = (T)pow(a, 3 + sc.alpha);
+    const double vcd = pow(a, 3 + sc.alpha);
+    const T vc = (T)vcd;
     double c1d, c2d;
     kick_coefs(ka, ms[member], c1d, c2d);
-    const T c1 = (T)c1d, c2 = (T)c2d;
     const int ix = (int)(line / N), iy = (int)(line % N);
     const int px = signed_index(ix, N), py = signed_index(iy, N);
     const int pxy = px * px + py * py;
@@ -352,26 +435,13 @@ k_z_kick(const cpx<typename G::T>* __restrict__ W, const cpx<typename G::T>* __r
     for (int m = 0; m < G::E; ++m) {
         const int kz = t + G::TPL * m;
         const int pz = signed_index(kz, N);
-        const T n2 = (T)(pxy + pz * pz);
         const cpx<T> f = STREAM ? load_stream(Pf + base + G::TPL * m) : Pf[base + G::TPL * m];
         cpx<T> p = STREAM ? load_stream(Pp + base + G::TPL * m) : Pp[base + G::TPL * m];
-        if (m == 0 && line == 0 && t == 0) add_mean_force(e[0], f, ms[member], (double)G::N3);
-        const T ln2 = lapc * n2;
-        const cpx<T> F = {-ln2 * f.x - vc * e[m].x, -ln2 * f.y - vc * e[m].y};
-        p.x += c1 * F.x;
-        p.y += c1 * F.y;
-        if constexpr (REC) rec_p[base + G::TPL * m] = p;  // adjoint recording: synchronised pi
-        // synchronised pi at the end of a call, read only if the state is
-        // synchronised later: a streaming store
-        if constexpr (SYNC_OUT) store_stream(Ps + base + G::TPL * m, p);
-        if constexpr (REDUCE) {
-            const double px_ = p.x, py_ = p.y, fx = f.x, fy = f.y;
-            red[0] += px_ * px_ + py_ * py_;
-            red[1] += (double)(pxy + pz * pz) * (fx * fx + fy * fy);
-        }
-        p.x += c2 * F.x;
-        p.y += c2 * F.y;
-        Pp[base + G::TPL * m] = p;
+        const size_t idx = base + G::TPL * m;
+        kick_mode<T, REC, SYNC_OUT, REDUCE>(e[m], f, p, pxy + pz * pz, m == 0 && line == 0 && t == 0, ms[member],
+                                            (double)G::N3, lapc, vc, vcd, c1d, c2d, red, REC ? rec_p + idx : nullptr,
+                                            SYNC_OUT ? Ps + idx : nullptr);
+        Pp[idx] = p;
     }
     if constexpr (REDUCE) {
         block_sum<2, G::THREADS_ROW>(red, reinterpret_cast<double*>(lds));
@@ -724,7 +794,8 @@ k_prereduce(const double* __restrict__ inV, int nV, double* __restrict__ outV, c
 // Restores the synchronised state of the members a call left ahead by one
 // look-ahead kick: the momenta from the sync buffer (k_sync_copy, `arrays`
 // packed arrays per member, member stride `mstride`), then the scale factor
-// a = a_start and the flag (k_sync_state, after the copies).
+// a = a_start, the homogeneous mode's remainder of the synchronised pi and
+// the flag (k_sync_state, after the copies).
 template <typename T>
 __global__ void __launch_bounds__(256)
 k_sync_copy(cpx<T>* __restrict__ dst, const cpx<T>* __restrict__ src, const MemberState* __restrict__ ms,
@@ -739,6 +810,8 @@ static __global__ void k_sync_state(MemberState* __restrict__ ms, int members) {
     const int m = blockIdx.x * blockDim.x + threadIdx.x;
     if (m < members && ms[m].ahead) {
         ms[m].a = ms[m].a_start;
+        ms[m].dc_p[0] = ms[m].dc_ps[0];
+        ms[m].dc_p[1] = ms[m].dc_ps[1];
         ms[m].ahead = 0;
     }
 }
@@ -865,7 +938,7 @@ k_z_inverse_deriv(const cpx<typename G::T>* __restrict__ Pf, cpx<typename G::T>*
 template <class G, bool DRIFT, bool REC = false>
 __global__ void __launch_bounds__((G::THREADS_ROW))
 k_z_inverse_gw(cpx<typename G::T>* __restrict__ Pf, const cpx<typename G::T>* __restrict__ Pp, cpx<typename G::T>* __restrict__ W,
-               cpx<typename G::T>* __restrict__ U, const cpx<typename G::T>* __restrict__ PU, const MemberState* __restrict__ ms,
+               cpx<typename G::T>* __restrict__ U, const cpx<typename G::T>* __restrict__ PU, MemberState* __restrict__ ms,
                const cpx<typename G::T>* __restrict__ tw, size_t astride, double kIR,
                cpx<typename G::T>* __restrict__ rec_f = nullptr, cpx<typename G::T>* __restrict__ rec_ph = nullptr,
                cpx<typename G::T>* __restrict__ rec_u = nullptr, cpx<typename G::T>* __restrict__ rec_puh = nullptr) {
@@ -888,8 +961,12 @@ k_z_inverse_gw(cpx<typename G::T>* __restrict__ Pf, const cpx<typename G::T>* __
                 rec_f[base + G::TPL * m] = v;
                 rec_ph[base + G::TPL * m] = p;
             }
-            v.x += cd * p.x;
-            v.y += cd * p.y;
+            if (sizeof(T) == 4 && m == 0 && line == 0 && t == 0) {
+                dc_drift(v, p, ms[member]);   // the homogeneous mode, compensated
+            } else {
+                v.x += cd * p.x;
+                v.y += cd * p.y;
+            }
             Pf[base + G::TPL * m] = v;
         }
         f[m] = v;
@@ -1118,7 +1195,7 @@ k_x_source_mt(cpx<typename G::T>* __restrict__ W, const cpx<typename G::T>* __re
 template <class G, bool REDUCE, bool REC = false, bool SYNC_OUT = false>
 __global__ void __launch_bounds__((G::THREADS_ROW))
 k_z_kick_gw(const cpx<typename G::T>* __restrict__ W, const cpx<typename G::T>* __restrict__ Pf, cpx<typename G::T>* __restrict__ Pp,
-            const cpx<typename G::T>* __restrict__ U, cpx<typename G::T>* __restrict__ PU, const MemberState* __restrict__ ms,
+            const cpx<typename G::T>* __restrict__ U, cpx<typename G::T>* __restrict__ PU, MemberState* __restrict__ ms,
             const cpx<typename G::T>* __restrict__ tw, double* __restrict__ partKG, KickArgs ka,
             StepConsts sc, size_t astride, cpx<typename G::T>* __restrict__ rec_p = nullptr,
             cpx<typename G::T>* __restrict__ Ps = nullptr, cpx<typename G::T>* __restrict__ PUs = nullptr) {
@@ -1133,7 +1210,8 @@ k_z_kick_gw(const cpx<typename G::T>* __restrict__ W, const cpx<typename G::T>* 
     const double a = ka.prologue ? ms[member].a_start : ms[member].a;
     const T norm1 = (T)pow(a, 1 + sc.alpha);
     const T lapc = (T)(pow(a, 1 + sc.alpha) * sc.kIR * sc.kIR);
-    const T vc = (T)pow(a, 3 + sc.alpha);
+    const double vcd = pow(a, 3 + sc.alpha);
+    const T vc = (T)vcd;
     const T k2c = (T)(sc.kIR * sc.kIR);
     double c1d, c2d;
     kick_coefs(ka, ms[member], c1d, c2d);
@@ -1157,26 +1235,13 @@ k_z_kick_gw(const cpx<typename G::T>* __restrict__ W, const cpx<typename G::T>* 
 #pragma unroll
     for (int m = 0; m < G::E; ++m) {
         const int pz = signed_index(t + G::TPL * m, N);
-        const T n2 = (T)(pxy + pz * pz);
         const cpx<T> f = fa[m];
         cpx<T> p = pa[m];
-        if (m == 0 && line == 0 && t == 0) add_mean_force(e[0], f, ms[member], (double)G::N3);
-        const T ln2 = lapc * n2;
-        const cpx<T> F = {-ln2 * f.x - vc * e[m].x, -ln2 * f.y - vc * e[m].y};
-        p.x += c1 * F.x;
-        p.y += c1 * F.y;
-        if constexpr (REC) rec_p[base + G::TPL * m] = p;  // adjoint recording: synchronised pi
-        // synchronised pi at the end of a call, read only if the state is
-        // synchronised later: a streaming store
-        if constexpr (SYNC_OUT) store_stream(Ps + base + G::TPL * m, p);
-        if constexpr (REDUCE) {
-            const double px_ = p.x, py_ = p.y, fx = f.x, fy = f.y;
-            red[0] += px_ * px_ + py_ * py_;
-            red[1] += (double)(pxy + pz * pz) * (fx * fx + fy * fy);
-        }
-        p.x += c2 * F.x;
-        p.y += c2 * F.y;
-        Pp[base + G::TPL * m] = p;
+        const size_t idx = base + G::TPL * m;
+        kick_mode<T, REC, SYNC_OUT, REDUCE>(e[m], f, p, pxy + pz * pz, m == 0 && line == 0 && t == 0, ms[member],
+                                            (double)G::N3, lapc, vc, vcd, c1d, c2d, red, REC ? rec_p + idx : nullptr,
+                                            SYNC_OUT ? Ps + idx : nullptr);
+        Pp[idx] = p;
     }
 #pragma unroll
     for (int ga = 0; ga < 3; ++ga) {

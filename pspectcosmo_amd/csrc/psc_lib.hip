@@ -699,12 +699,24 @@ int psc_load_ic_k(psc_ctx* c, int member, const double* f_k, const double* pi_k,
     if (rc) return rc;
     if (!f_k || !pi_k || !(a > 0)) return c->fail(PSC_ERR_ARG, "bad initial data");
     std::vector<double> full;
+    // FP32: the homogeneous mode's FP64 remainders beside its FP32 values
+    double lo[2][2] = {{0.0, 0.0}, {0.0, 0.0}};
+    auto remainder = [&](double (&r)[2]) {
+        if (c->cfg.precision != PSC_FP32) return;
+        for (int i = 0; i < 2; ++i) r[i] = full[i] - (double)(float)full[i];
+    };
     pack_full(c->N, c->cfg.nflds, f_k, full);
+    remainder(lo[0]);
     if ((rc = upload(c, c->Pf, member, full))) return rc;
     pack_full(c->N, c->cfg.nflds, pi_k, full);
+    remainder(lo[1]);
     if ((rc = upload(c, c->Pp, member, full))) return rc;
     if ((rc = reset_ghosts(c))) return rc;
     MemberState s = fresh_state(c, member, 0.0, a, ad);
+    for (int i = 0; i < 2; ++i) {
+        s.dc_f[i] = lo[0][i];
+        s.dc_p[i] = lo[1][i];
+    }
     if ((rc = write_state(c, member, &s))) return rc;
     Diag d;
     if ((rc = diagnose(c, member, s, &d))) return rc;
@@ -930,13 +942,20 @@ int psc_get_k(psc_ctx* c, int member, double* f_k, double* pi_k) {
             out[i + 1] += A[1];
         });
     };
+    // FP32: the homogeneous mode is its FP32 value plus the FP64 remainder
+    MemberState s;
+    if ((rc = read_state(c, member, &s))) return rc;
     if (f_k) {
         if ((rc = download(c, c->Pf, member, full))) return rc;
+        full[0] += s.dc_f[0];
+        full[1] += s.dc_f[1];
         unpack_half(c->N, c->cfg.nflds, full.data(), f_k);
         if (c->padeng && (rc = add_ghost(c->Gf, f_k))) return rc;
     }
     if (pi_k) {
         if ((rc = download(c, c->Pp, member, full))) return rc;
+        full[0] += s.dc_p[0];
+        full[1] += s.dc_p[1];
         unpack_half(c->N, c->cfg.nflds, full.data(), pi_k);
         if (c->padeng && (rc = add_ghost(c->Gp, pi_k))) return rc;
     }

@@ -58,7 +58,7 @@ __device__ __forceinline__ int pad_wrap(int s, int N) { return s == -N / 2 ? N /
 template <typename T, int N>
 __global__ void __launch_bounds__(256)
 k_drift_pad(cpx<T>* __restrict__ Pf, const cpx<T>* __restrict__ Pp, cpx<T>* __restrict__ Gf,
-            const cpx<T>* __restrict__ Gp, const MemberState* __restrict__ ms) {
+            const cpx<T>* __restrict__ Gp, MemberState* __restrict__ ms) {
     const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= (size_t)N * N * N) return;
     const int x = (int)(e / ((size_t)N * N)), y = (int)((e / N) % N), z = (int)(e % N);
@@ -66,8 +66,12 @@ k_drift_pad(cpx<T>* __restrict__ Pf, const cpx<T>* __restrict__ Pp, cpx<T>* __re
     const T cd = (T)ms[0].c_drift;
     cpx<T> f = Pf[idx];
     const cpx<T> p = Pp[idx];
-    f.x += cd * p.x;
-    f.y += cd * p.y;
+    if (sizeof(T) == 4 && e == 0) {
+        dc_drift(f, p, ms[0]);   // the homogeneous mode, compensated
+    } else {
+        f.x += cd * p.x;
+        f.y += cd * p.y;
+    }
     Pf[idx] = f;
     if (z == 0 || z == N / 2) {
         const size_t gi = (z == 0 ? 0 : (size_t)N * N) + (size_t)x * N + y;
@@ -194,7 +198,7 @@ template <class GP, int N, bool REDUCE, bool SYNC_OUT = false>
 __global__ void __launch_bounds__(256)
 k_kick_pad(const cpx<typename GP::T>* __restrict__ R, const cpx<typename GP::T>* __restrict__ Pf,
            cpx<typename GP::T>* __restrict__ Pp, const cpx<typename GP::T>* __restrict__ Gf,
-           cpx<typename GP::T>* __restrict__ Gp, const MemberState* __restrict__ ms,
+           cpx<typename GP::T>* __restrict__ Gp, MemberState* __restrict__ ms,
            double* __restrict__ partKG, KickArgs ka, StepConsts sc,
            cpx<typename GP::T>* __restrict__ Ps = nullptr, cpx<typename GP::T>* __restrict__ Gps = nullptr) {
     using T = typename GP::T;
@@ -205,7 +209,8 @@ k_kick_pad(const cpx<typename GP::T>* __restrict__ R, const cpx<typename GP::T>*
     if (ka.prologue && ms[0].ahead) return;   // the call's first kick was applied as a look-ahead
     const double a = ka.prologue ? ms[0].a_start : ms[0].a;
     const T lapc = (T)(pow(a, 1 + sc.alpha) * sc.kIR * sc.kIR);
-    const T vc = (T)pow(a, 3 + sc.alpha);
+    const double vcd = pow(a, 3 + sc.alpha);
+    const T vc = (T)vcd;
     double c1d, c2d;
     kick_coefs(ka, ms[0], c1d, c2d);
     const T c1 = (T)c1d, c2 = (T)c2d;
@@ -237,21 +242,11 @@ k_kick_pad(const cpx<typename GP::T>* __restrict__ R, const cpx<typename GP::T>*
         const size_t idx = (size_t)x * PSN + (size_t)y * N + z;
         const cpx<T> f = Pf[idx];
         cpx<T> p = Pp[idx];
-        if (e == 0) add_mean_force(E, f, ms[0], (double)N * N * N);
         const int n2i = sx * sx + sy * sy + sz * sz;
-        const T ln2 = lapc * (T)n2i;
-        const cpx<T> F = {-ln2 * f.x - vc * E.x, -ln2 * f.y - vc * E.y};
-        p.x += c1 * F.x;
-        p.y += c1 * F.y;
-        if constexpr (SYNC_OUT) Ps[idx] = p;   // synchronised pi at the end of a call
-        if constexpr (REDUCE) {
-            const double px_ = p.x, py_ = p.y, fx = f.x, fy = f.y;
-            red[0] += px_ * px_ + py_ * py_;
-            red[1] += (double)n2i * (fx * fx + fy * fy);
-        }
-        p.x += c2 * F.x;
-        p.y += c2 * F.y;
+        kick_mode<T, false, SYNC_OUT, REDUCE>(E, f, p, n2i, e == 0, ms[0], (double)N * N * N, lapc, vc, vcd, c1d,
+                                              c2d, red, nullptr, SYNC_OUT ? Ps + idx : nullptr);
         Pp[idx] = p;
+        const T ln2 = lapc * (T)n2i;
         if (plane) {
             const size_t gi = (z == 0 ? 0 : (size_t)N * N) + (size_t)x * N + y;
             const cpx<T> gf = Gf[gi];

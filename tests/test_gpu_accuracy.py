@@ -11,7 +11,10 @@ linear window):
   the first test) after 1000 steps: fields, momenta, per-bin spectra,
   energies and the Friedmann-constraint violation of both paths.
 
-The measured errors are printed; the bounds are the stated targets."""
+The FP32 path carries the homogeneous mode (k = 0) as a compensated pair (the
+FP32 value plus an FP64 remainder), so the total field error is set by the
+fluctuations.  The measured errors are printed; the bounds are the stated
+targets, or about twice the measurement where that is tighter."""
 import numpy as np
 import pytest
 
@@ -92,12 +95,13 @@ def test_fp32_mixed_against_fp64_over_1000_steps():
         relp = np.abs(sa[w, 2] - sb[w, 2]) / sb[w, 2]
         print(f"spectrum field {f}: per-bin |f_k|^2 rel error max {rel.max():.2e} median {np.median(rel):.2e}; "
               f"|pi_k|^2 max {relp.max():.2e}")
-        assert rel.max() <= 1e-3
+        assert rel.max() <= 2e-4   # measured 7.3e-5 (phi), 1.5e-5 (chi)
     print("energies fp32", a["e"][:8], "fp64", b["e"][:8])
     print(f"constraint violation: fp64 {b['e'][7]:.3e} (t = 0: {b['e0'][7]:.3e}), fp32 {a['e'][7]:.3e}")
     print(f"scale factor: fp64 a = {b['sc'][1]:.15f}, fp32 a = {a['sc'][1]:.15f}")
-    assert ef <= 1e-5 and ep <= 1e-5
-    assert efl <= 1e-5 and max(per_field_fl) <= 1e-5   # the fluctuations, not only the DC-dominated total
+    # measured on MI355X: field 8.7e-11, momentum 3.4e-7, fluctuations 2.0e-6
+    assert ef <= 1e-9 and ep <= 1e-6
+    assert efl <= 5e-6 and max(per_field_fl) <= 5e-6   # the fluctuations, not only the DC-dominated total
     np.testing.assert_allclose(a["e"][:7], b["e"][:7], rtol=1e-4)
 
 
@@ -112,12 +116,13 @@ def test_fp32_mixed_over_the_config_window():
     Friedmann-constraint drift of both.
 
     Measured on MI355X: FP64 vs oracle 3e-20 (fluctuations 2e-15); at t = 2
-    FP32 vs FP64 total field 1.2e-5, fluctuations 1.0e-4 (phi 2.4e-5, chi
-    1.6e-4), momentum fluctuations 1.3e-4, spectrum bins <= 9e-4 (median
-    2e-5), a 5.5e-7, constraint 3.8e-5 vs 4.5e-5.  Over 3000 substeps the
-    FP32 rounding of phi = phi0 + dphi (absolute eps*phi0 in dphi) and the
-    ~1000-radian phase of the m_chi = 500 modes set that floor; the bounds
-    sit at about twice the measurement."""
+    FP32 vs FP64 total field 1.5e-9 (the homogeneous mode is a compensated
+    FP32 + FP64 pair; without it 1.2e-5), fluctuations 2.5e-5 (phi 6.0e-6, chi
+    4.0e-5), momentum 6.1e-8 (fluctuations 3.1e-5), spectrum bins <= 4.1e-5
+    (median 5e-6), a 4.2e-11 relative, constraint 4.512e-5 vs 4.514e-5.  Over
+    3000 substeps the ~1000-radian phase of the m_chi = 500 modes sets the
+    fluctuation floor; the bounds sit at about twice the measurement, and the
+    total field error is held to the 1e-5 target with a wide margin."""
     steps = 1000
     cfg64, model, ic = parity_setup(N=N, integrator=PSC_Y4, precision=PSC_FP64, use_graph=1, kIR=1.0, dt=0.002,
                                     omega_star=1e-6)
@@ -163,11 +168,12 @@ def test_fp32_mixed_over_the_config_window():
     print(f"constraint violation: t = 0 {b['e0'][7]:.3e}; t = 2 fp64 {b['e'][7]:.3e}, fp32 {a['e'][7]:.3e}; "
           f"a: fp64 {b['sc'][1]:.12f}, fp32 {a['sc'][1]:.12f}", flush=True)
     assert t_end_ok(b["sc"][0])
-    assert rel_err(a["k"][0], b["k"][0]) <= 2.5e-5 and rel_err(a["k"][1], b["k"][1]) <= 1e-6
-    assert efl <= 2.5e-4 and epl <= 2.5e-4
-    assert worst <= 2e-3
-    assert abs(a["sc"][1] - b["sc"][1]) <= 1.5e-6 * b["sc"][1]
-    assert abs(a["e"][7] - b["e"][7]) <= 2e-5
+    assert rel_err(a["k"][0], b["k"][0]) <= 1e-5            # the stated target
+    assert rel_err(a["k"][0], b["k"][0]) <= 5e-9 and rel_err(a["k"][1], b["k"][1]) <= 2e-7
+    assert efl <= 6e-5 and epl <= 6e-5
+    assert worst <= 1e-4
+    assert abs(a["sc"][1] - b["sc"][1]) <= 1e-10 * b["sc"][1]
+    assert abs(a["e"][7] - b["e"][7]) <= 1e-7
 
 
 def t_end_ok(t):
