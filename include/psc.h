@@ -176,7 +176,9 @@ const char* psc_last_error(const psc_ctx* ctx);
 int psc_set_model(psc_ctx* ctx, int member, const psc_model* model);
 
 /* Copy initial half spectra to the device and compute d2a/dt2 from them as
- * initialize.cpp:797 does with kernel_a_k_nopad (evolution.cpp:1061-1090). */
+ * initialize.cpp:797 does with kernel_a_k_nopad (evolution.cpp:1061-1090).
+ * In an FP32 context the homogeneous mode (k = 0) keeps its FP64 value: the
+ * FP32 array holds its rounding and the context the remainder. */
 int psc_load_ic_k(psc_ctx* ctx, int member, const double* f_k, const double* pi_k,
                   double a, double ad);
 /* Initial conditions generated on the device straight into member `member`:
@@ -226,7 +228,8 @@ int psc_set_output_times(psc_ctx* ctx, double t_out, double dt_cap);
 /* Block until all device work of the context is done. */
 int psc_sync(psc_ctx* ctx);
 
-/* Read the state back as FP64 half spectra (f_k, pi_k of pspectcosmo.h:73-74). */
+/* Read the state back as FP64 half spectra (f_k, pi_k of pspectcosmo.h:73-74);
+ * in an FP32 context the homogeneous mode includes its FP64 remainder. */
 int psc_get_k(psc_ctx* ctx, int member, double* f_k, double* pi_k);
 /* Real-space fields f and pi (c2r and /N^3, output.cpp:401-425). Either may be NULL. */
 int psc_get_fields(psc_ctx* ctx, int member, double* f, double* pi);
