@@ -87,7 +87,7 @@ typedef struct psc_config {
                           output time of psc_set_output_times (pspectcosmo.cpp:213-233,
                           622-627), computed on the device                           */
     int n_pad;         /* N_pad of the dealiased stepper (parameters.h:23), larger than N
-                          (pspectcosmo.cpp:244): 2N or 3N/2 (N_pad <= 512); 0 = 2N;
+                          (pspectcosmo.cpp:244): 2N or 3N/2 (N_pad <= 768); 0 = 2N;
                           must be 0 with pad = 0                                      */
     int reserved_i[2];
     double kIR;        /* lattice IR wavenumber, L = 2 pi / kIR (parameters.h:28,32)       */

@@ -20,7 +20,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libpsc.so")
 UNITS = ["psc_lib.hip", "psc_ic.cpp", "psc_shapes_f32_small.hip", "psc_shapes_f32_256.hip",
          "psc_shapes_f32_512a.hip", "psc_shapes_f32_512b.hip", "psc_shapes_f64_small.hip",
-         "psc_shapes_f64_256.hip", "psc_shapes_f64_512.hip", "psc_shapes_r3.hip"]
+         "psc_shapes_f64_256.hip", "psc_shapes_f64_512.hip", "psc_shapes_r3.hip", "psc_shapes_pad768.hip"]
 # built-in models: name -> header (relative to csrc/)
 MODELS = {"preheat": "models/preheat.hpp", "lambda_phi4": "models/lambda_phi4.hpp"}
 DEFAULT_MODEL = "preheat"

@@ -1056,7 +1056,7 @@ EngineBase* make_one() { return new Engine<G, GY>(); }
 
 // N_pad = 2N (the reference's shipped pair, parameters.h:22-23) and, for the
 // lengths 3 * 2^k, N_pad = 3N/2 (the two-thirds rule of the quadratic terms).
-constexpr bool lattice_length(int n) {   // 2^k (16 .. 512) or 3 * 2^k (48 .. 384)
+constexpr bool lattice_length(int n) {   // 2^k (16 .. 512) or 3 * 2^k (48 .. 384) of an unpadded lattice
     return n == 16 || n == 32 || n == 64 || n == 128 || n == 256 || n == 512 || n == 48 || n == 96 ||
            n == 192 || n == 384;
 }

@@ -261,6 +261,8 @@ __device__ __forceinline__ void fft_stage(cpx<T>* e, int t, int c, cpx<T>* lds,
                                           const cpx<T>* __restrict__ tw) {
     constexpr int Q = N / (R * TPL);
     static_assert(Q >= 1, "radix too large for the thread count");
+    static_assert(N % (R * TPL) == 0, "every thread must take a whole number of butterflies per stage "
+                                      "(the later radices must divide the first)");
     cpx<T> v[Q][R];
     if constexpr (FIRST) {
         static_assert(Q == 1, "first stage must have one butterfly per thread");

@@ -245,7 +245,8 @@ k_x_force(cpx<typename G::T>* __restrict__ W, const MemberState* __restrict__ ms
 // C, multi-tile form for the long columns (512 points): each block loops over
 // the tiles b, b + G, b + 2G, ... of its member (G = gridDim.x, one block per
 // resident slot) and holds the next two tiles' columns in registers while it
-// transforms the current one.  On gfx950 the block barriers of the transforms
+// transforms the current one (the next one only where the columns are too
+// long for two in the register budget, as at 768 points).  On gfx950 the block barriers of the transforms
 // wait only on LDS, so the loads stay in flight across them: the load latency
 // that one block per CU would otherwise expose is hidden behind the previous
 // tiles' transforms.  The same arithmetic per column as k_x_force (bit-equal
@@ -281,18 +282,31 @@ k_x_force_mt(cpx<typename G::T>* __restrict__ W, const MemberState* __restrict__
     const cpx<T> dc = Pf[(size_t)member * G::MS];
     const T mean[2] = {(T)((double)dc.x / (double)G::N3), nf > 1 ? (T)((double)dc.y / (double)G::N3) : (T)0};
     double v = 0.0;
-    cpx<T> n1[G::E], n2[G::E];
+    // tiles prefetched: two while the three column sets (current and two
+    // ahead) take at most 40 % of the VGPRs one block per CU may use
+    // (measured: no spills for the 512- and 384-point shapes, spills at 768)
+    constexpr int SET = G::E * (int)sizeof(cpx<T>) / 4;
+    constexpr int BUDGET = G::THREADS_COL >= 256 ? 512 / (G::THREADS_COL / 256) : 512;
+    constexpr int DEPTH = 3 * SET * 5 <= BUDGET * 2 ? 2 : 1;
+    cpx<T> n1[G::E], n2[DEPTH > 1 ? G::E : 1];
     int q = blockIdx.x;
     if (q < NT) load(n1, q);
-    if (q + GS < NT) load(n2, q + GS);
+    if constexpr (DEPTH > 1)
+        if (q + GS < NT) load(n2, q + GS);
     for (; q < NT; q += GS) {
         cpx<T> e[G::E];
+        if constexpr (DEPTH > 1) {
 #pragma unroll
-        for (int m = 0; m < G::E; ++m) {
-            e[m] = n1[m];
-            n1[m] = n2[m];
+            for (int m = 0; m < G::E; ++m) {
+                e[m] = n1[m];
+                n1[m] = n2[m];
+            }
+            if (q + 2 * GS < NT) load(n2, q + 2 * GS);
+        } else {
+#pragma unroll
+            for (int m = 0; m < G::E; ++m) e[m] = n1[m];
+            if (q + GS < NT) load(n1, q + GS);
         }
-        if (q + 2 * GS < NT) load(n2, q + 2 * GS);
         line_fft<T, N, typename G::Rad, +1, typename G::LayCol>(e, t, c, lds, tw);
         x_force_pointwise<G, false>(e, mean, nf, cp, v, nullptr, 0, stride);
         line_fft<T, N, typename G::Rad, -1, typename G::LayCol>(e, t, c, lds, tw);

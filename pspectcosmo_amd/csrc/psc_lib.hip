@@ -34,6 +34,7 @@ const std::vector<ShapeEntry>& shape_table() {
         register_shapes_f64_256(v);
         register_shapes_f64_512(v);
         register_shapes_r3(v);
+        register_shapes_pad768(v);
         return v;
     }();
     return table;
@@ -531,9 +532,9 @@ int psc_create(const psc_config* cfg, psc_ctx** out) {
     // (pspectcosmo.cpp:244-247); 0 selects 2N
     const int NP = cfg->pad ? (cfg->n_pad ? cfg->n_pad : 2 * N) : 0;
     if (cfg->pad && NP <= N) return PSC_ERR_ARG;
-    // built: N_pad = 2N and 3N/2 within the transform lengths (up to 512), one
-    // member, no GW (the reference pads only the scalar force)
-    if (cfg->pad && (cfg->batch != 1 || cfg->with_gw || NP > 512 || (NP != 2 * N && 2 * NP != 3 * N)))
+    // built: N_pad = 2N and 3N/2 on the padded transform lengths (up to 768),
+    // one member, no GW (the reference pads only the scalar force)
+    if (cfg->pad && (cfg->batch != 1 || cfg->with_gw || NP > 768 || (NP != 2 * N && 2 * NP != 3 * N)))
         return PSC_ERR_UNSUPPORTED;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return PSC_ERR_DEVICE;
@@ -694,6 +695,54 @@ int psc_set_model(psc_ctx* c, int member, const psc_model* m) {
     return write_state(c, member, &s);
 }
 
+// The dealiased stepper's half spectra in its own form: in the kz = 0 and
+// kz = N/2 planes each field's value splits into the Hermitian part
+// (h(n) + conj h(-n)) / 2, which the packed spectrum holds, and the
+// anti-Hermitian part (h(n) - conj h(-n)) / 2, which goes to the plane ghost
+// Gamma = A_phi + i A_chi (the parts psc_get_k adds back).  `herm` receives
+// the half spectra with their Hermitian parts, G the ghost (2 N^2 complex,
+// plane-major, then x, then y).
+void split_plane_ghosts(int N, int nflds, const double* hk, std::vector<double>& herm, std::vector<double>& G) {
+    const int Nh = N / 2 + 1;
+    const size_t nk = (size_t)N * N * Nh;
+    herm.assign(hk, hk + 2 * nk * nflds);
+    G.assign((size_t)2 * 2 * N * N, 0.0);
+    for (int pl = 0; pl < 2; ++pl) {
+        const int k = pl ? N / 2 : 0;
+        for (int x = 0; x < N; ++x)
+            for (int y = 0; y < N; ++y) {
+                const size_t at = ((size_t)x * N + y) * Nh + k;
+                const size_t mat = ((size_t)((N - x) % N) * N + (N - y) % N) * Nh + k;
+                double A[2][2] = {{0.0, 0.0}, {0.0, 0.0}};
+                for (int q = 0; q < nflds; ++q) {
+                    const double* h = hk + 2 * nk * q;
+                    const double hr = h[2 * at], hi = h[2 * at + 1], mr = h[2 * mat], mi = -h[2 * mat + 1];
+                    herm[2 * nk * q + 2 * at] = 0.5 * (hr + mr);
+                    herm[2 * nk * q + 2 * at + 1] = 0.5 * (hi + mi);
+                    A[q][0] = 0.5 * (hr - mr);
+                    A[q][1] = 0.5 * (hi - mi);
+                }
+                const size_t g = (size_t)pl * N * N + (size_t)x * N + y;
+                G[2 * g] = A[0][0] - A[1][1];       // A_phi + i A_chi
+                G[2 * g + 1] = A[0][1] + A[1][0];
+            }
+    }
+}
+
+// Upload a plane ghost (2 N^2 complex FP64) in the field precision.
+int upload_ghost(psc_ctx* c, void* dst, const std::vector<double>& G) {
+    if (c->cfg.precision == PSC_FP32) {
+        std::vector<float> tmp(G.size());
+        for (size_t i = 0; i < G.size(); ++i) tmp[i] = (float)G[i];
+        HIPCHK(c, hipMemcpyAsync(dst, tmp.data(), tmp.size() * sizeof(float), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    } else {
+        HIPCHK(c, hipMemcpyAsync(dst, G.data(), G.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
+    return PSC_OK;
+}
+
 int psc_load_ic_k(psc_ctx* c, int member, const double* f_k, const double* pi_k, double a, double ad) {
     int rc = check_member(c, member);
     if (rc) return rc;
@@ -705,13 +754,19 @@ int psc_load_ic_k(psc_ctx* c, int member, const double* f_k, const double* pi_k,
         if (c->cfg.precision != PSC_FP32) return;
         for (int i = 0; i < 2; ++i) r[i] = full[i] - (double)(float)full[i];
     };
-    pack_full(c->N, c->cfg.nflds, f_k, full);
+    // the dealiased stepper keeps the anti-Hermitian parts of the kz = 0, N/2
+    // planes in its ghosts (a state read by psc_get_k loads back unchanged)
+    std::vector<double> herm, gf, gp;
+    if (c->padeng) split_plane_ghosts(c->N, c->cfg.nflds, f_k, herm, gf);
+    pack_full(c->N, c->cfg.nflds, c->padeng ? herm.data() : f_k, full);
     remainder(lo[0]);
     if ((rc = upload(c, c->Pf, member, full))) return rc;
-    pack_full(c->N, c->cfg.nflds, pi_k, full);
+    if (c->padeng) split_plane_ghosts(c->N, c->cfg.nflds, pi_k, herm, gp);
+    pack_full(c->N, c->cfg.nflds, c->padeng ? herm.data() : pi_k, full);
     remainder(lo[1]);
     if ((rc = upload(c, c->Pp, member, full))) return rc;
     if ((rc = reset_ghosts(c))) return rc;
+    if (c->padeng && ((rc = upload_ghost(c, c->Gf, gf)) || (rc = upload_ghost(c, c->Gp, gp)))) return rc;
     MemberState s = fresh_state(c, member, 0.0, a, ad);
     for (int i = 0; i < 2; ++i) {
         s.dc_f[i] = lo[0][i];

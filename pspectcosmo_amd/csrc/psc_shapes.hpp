@@ -28,5 +28,6 @@ void register_shapes_f64_small(std::vector<ShapeEntry>& v);
 void register_shapes_f64_256(std::vector<ShapeEntry>& v);
 void register_shapes_f64_512(std::vector<ShapeEntry>& v);
 void register_shapes_r3(std::vector<ShapeEntry>& v);
+void register_shapes_pad768(std::vector<ShapeEntry>& v);
 
 }  // namespace psc

@@ -1,0 +1,16 @@
+// psc_shapes_pad768.hip -- the 768-point padded lattice of the dealiased
+// stepper: N_pad = 3N/2 for N = 512 and N_pad = 2N for N = 384.  A padded
+// shape only (768 is not an unpadded lattice length); the first stage of every
+// line is the in-register length-12 DFT, then three radix-4 stages (the later
+// radices divide the first, so 64 threads per line keep whole butterflies).
+// FP64 keeps its column tile at 8 columns (96 KiB of LDS).
+#include "psc_shapes.hpp"
+
+namespace psc {
+void register_shapes_pad768(std::vector<ShapeEntry>& v) {
+    using F768 = S<float, 768, Radices<12, 4, 4, 4>, 16, 4>;
+    using D768 = S<double, 768, Radices<12, 4, 4, 4>, 8, 4>;
+    v.push_back(PSC_SHAPE_PAD(PSC_FP32, "r12.4.4.4_tc16_l4", F768, F768));
+    v.push_back(PSC_SHAPE_PAD(PSC_FP64, "r12.4.4.4_tc8_l4", D768, D768));
+}
+}  // namespace psc
