@@ -23,17 +23,18 @@ Other workloads (own JSON line each):
                          time, extra memory, central differences (eps = 1e-3)
   --workload padded256   the dealiased stepper (V' on 512^3) of the 256^3
                          two-field lattice, Yoshida-4, FP32-mixed; with
-                         --n-pad 384 on the 3N/2 padded lattice
+                         --n-pad 384 on the 3N/2 padded lattice; with --n 512
+                         --n-pad 768 the 512^3 lattice on 768^3
 
 The default run adds, under "workloads", a shorter measurement of each other
 configuration (gw512, ensemble128, adjoint128, padded256 with N_pad = 512
-and 384), the attainable
+and 384, padded512 with N_pad = 768), the attainable
 copy bandwidth, the stable variant of the headline lattice (kIR = 1, finite
 values), the rate of one-step psc_step calls against one call of many steps,
 and the reference's adaptive dt rule on the device against fixed dt.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-workloads]
-                       [--workload preheat256|gw512|ensemble128|adjoint128|padded256] [--n-pad 384]
+                       [--workload preheat256|gw512|ensemble128|adjoint128|padded256] [--n-pad 384] [--n 512]
                        [--autotune]
 """
 import argparse
@@ -211,8 +212,9 @@ def bench_gw512(args):
 
 def bench_padded256(args):
     """The dealiased stepper (pad = 2, N_pad = 512, or args.n_pad = 384 for
-    3N/2) on the 256^3 two-field lattice, FP32-mixed, Yoshida-4."""
-    n = 256
+    3N/2) on the 256^3 two-field lattice, FP32-mixed, Yoshida-4; args.n picks
+    another lattice (512 with N_pad = 768)."""
+    n = getattr(args, "n", 256)
     n_pad = getattr(args, "n_pad", 0) or 2 * n
     model = make_model(1.0, G2)
     ic = make_ic(f0=(1.0, 0.0), fd0=(-0.7, 0.0), seed=1)
@@ -240,10 +242,10 @@ def bench_padded256(args):
     # z forward r + 1, kick 4; 39 X at r = 2, 26.25 X at r = 3/2
     r = n_pad / n
     b = 3 * (9 + 7 * r * r + r) * 8
-    out = {"metric": f"site-updates/s at 256^3, dealiased (N_pad = {n_pad}), FP32-mixed", "value": v,
+    out = {"metric": f"site-updates/s at {n}^3, dealiased (N_pad = {n_pad}), FP32-mixed", "value": v,
            "unit": "site-updates/s", "n_gpus": 1, "steps": steps, "warmup": args.warmup,
            "ms_per_step": el / steps * 1e3, "higher_is_better": True, "dtype": "f32", "data": "synthetic",
-           "config": {"workload": f"256^3 two-field lattice, padded V' on {n_pad}^3, g^2 = 2.5e5, kIR = 25, "
+           "config": {"workload": f"{n}^3 two-field lattice, padded V' on {n_pad}^3, g^2 = 2.5e5, kIR = 25, "
                                   "Yoshida-4", "N_pad": n_pad, "shape": lat.shape_name()},
            "roofline": {"kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": achieved / HBM_PEAK_GBS, "avg_us": avg_s * 1e6, "bytes": bytes_},
@@ -701,6 +703,8 @@ def run_workloads(args):
                      ("padded256", lambda: bench_padded256(argparse.Namespace(**{**vars(quick), "steps": 10, "warmup": 2}))),
                      ("padded256_n384", lambda: bench_padded256(argparse.Namespace(
                          **{**vars(quick), "steps": 10, "warmup": 2, "n_pad": 384}))),
+                     ("padded512_n768", lambda: bench_padded256(argparse.Namespace(
+                         **{**vars(quick), "steps": 4, "warmup": 1, "n": 512, "n_pad": 768}))),
                      ("adjoint128", lambda: adjoint_ratios(args))):
         t0 = time.perf_counter()
         print(f"[bench] workload {name} ...", file=sys.stderr, flush=True)
@@ -726,6 +730,7 @@ def main():
                     choices=["preheat256", "gw512", "ensemble128", "adjoint128", "padded256"])
     ap.add_argument("--autotune", action="store_true", help="time the launch shapes first")
     ap.add_argument("--n-pad", dest="n_pad", type=int, default=0, help="padded256: N_pad (512 or 384)")
+    ap.add_argument("--n", type=int, default=256, help="padded256: the lattice (256, or 512 with --n-pad 768)")
     ap.add_argument("--no-autotune", action="store_true", help="headline: keep the default shapes")
     ap.add_argument("--seq-steps", type=int, default=150, help="ensemble: steps per member run alone")
     ap.add_argument("--no-fd", action="store_true", help="adjoint: skip the finite-difference and FP64 runs")
