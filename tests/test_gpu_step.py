@@ -1,7 +1,7 @@
 """GPU parity of the HIP lattice step against the FP64 CPU oracle.
 
-Tolerances: FP64 path <= 1e-10 relative field error after 10 steps (the bar
-of the north-star), FP32-mixed path <= 1e-5 relative to the FP64 oracle."""
+Tolerances: FP64 path <= 1e-10 relative field error after 10 steps, FP32-mixed
+path <= 1e-5 relative to the FP64 oracle."""
 import numpy as np
 import pytest
 
