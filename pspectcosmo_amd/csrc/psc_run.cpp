@@ -356,11 +356,20 @@ struct Run {
         return lattice_name;
     }
 
+    // Beside each image, <image>.k holds the k-space state the reference's
+    // image cannot: the FP64 half spectra of f and pi (with the dealiased
+    // stepper's plane parts), d2a/dt2, and with GW the u_ij and pi_u_ij.  It
+    // names the header of its image (run number, t, a, da/dt) and is read on
+    // continue_run only when the image has that header, so an image replaced
+    // by hand is read as the reference reads it.
+    static constexpr char kSideMagic[8] = {'P', 'S', 'C', 'K', 'I', 'M', 'G', '1'};
+    size_t half_len() const { return (size_t)cfg.N * cfg.N * (cfg.N / 2 + 1) * 2; }
+
     void write_grid(int run_number) {
         const size_t n3 = (size_t)cfg.N * cfg.N * cfg.N;
         std::vector<double> f(n3 * cfg.nflds), p(n3 * cfg.nflds);
         check(psc_get_fields(ctx, 0, f.data(), p.data()), "psc_get_fields");
-        double sc[4];
+        double sc[8];
         check(psc_get_scalars(ctx, 0, sc), "psc_get_scalars");
         const std::string name = grid_name(sc[0]);
         std::ofstream g(name + ".tmp", std::ios::binary);
@@ -371,8 +380,63 @@ struct Run {
         g.write(reinterpret_cast<const char*>(f.data()), f.size() * sizeof(double));
         g.write(reinterpret_cast<const char*>(p.data()), p.size() * sizeof(double));
         g.close();
+        // the k-space state
+        std::vector<double> fk(half_len() * 2), pk(half_len() * 2);
+        check(psc_get_k(ctx, 0, fk.data(), pk.data()), "psc_get_k");
+        std::ofstream k(name + ".k.tmp", std::ios::binary);
+        const int hdr[6] = {cfg.N, cfg.nflds, cfg.pad, cfg.pad ? (cfg.n_pad ? cfg.n_pad : 2 * cfg.N) : 0, cfg.with_gw,
+                            run_number};
+        k.write(kSideMagic, sizeof(kSideMagic));
+        k.write(reinterpret_cast<const char*>(hdr), sizeof(hdr));
+        k.write(reinterpret_cast<const char*>(sc), 4 * sizeof(double));   // t, a, da/dt, d2a/dt2
+        k.write(reinterpret_cast<const char*>(fk.data()), half_len() * cfg.nflds * sizeof(double));
+        k.write(reinterpret_cast<const char*>(pk.data()), half_len() * cfg.nflds * sizeof(double));
+        if (cfg.with_gw) {
+            std::vector<double> u(6 * half_len()), pu(6 * half_len());
+            check(psc_get_gw_k(ctx, 0, u.data(), pu.data()), "psc_get_gw_k");
+            k.write(reinterpret_cast<const char*>(u.data()), u.size() * sizeof(double));
+            k.write(reinterpret_cast<const char*>(pu.data()), pu.size() * sizeof(double));
+        }
+        k.close();
+        std::rename((name + ".k.tmp").c_str(), (name + ".k").c_str());
         std::rename((name + ".tmp").c_str(), name.c_str());
     }
+
+    // The k-space state of grid.img, when <dir>/grid.img.k belongs to it;
+    // sets gw_restored when it also held the GW state.
+    bool read_side(int run_number, double t0, double a, double ad) {
+        std::ifstream k(dir + "/grid.img.k", std::ios::binary);
+        if (!k) return false;
+        char magic[8];
+        int hdr[6];
+        double sc[4];
+        k.read(magic, sizeof(magic));
+        k.read(reinterpret_cast<char*>(hdr), sizeof(hdr));
+        k.read(reinterpret_cast<char*>(sc), sizeof(sc));
+        const int np = cfg.pad ? (cfg.n_pad ? cfg.n_pad : 2 * cfg.N) : 0;
+        if (!k || std::memcmp(magic, kSideMagic, sizeof(magic)) != 0 || hdr[0] != cfg.N || hdr[1] != cfg.nflds ||
+            hdr[2] != cfg.pad || hdr[3] != np || hdr[5] != run_number || sc[0] != t0 || sc[1] != a || sc[2] != ad)
+            return false;
+        std::vector<double> fk(half_len() * 2, 0.0), pk(half_len() * 2, 0.0);
+        k.read(reinterpret_cast<char*>(fk.data()), half_len() * cfg.nflds * sizeof(double));
+        k.read(reinterpret_cast<char*>(pk.data()), half_len() * cfg.nflds * sizeof(double));
+        std::vector<double> u, pu;
+        if (hdr[4] && cfg.with_gw) {
+            u.resize(6 * half_len());
+            pu.resize(6 * half_len());
+            k.read(reinterpret_cast<char*>(u.data()), u.size() * sizeof(double));
+            k.read(reinterpret_cast<char*>(pu.data()), pu.size() * sizeof(double));
+        }
+        if (!k) return false;
+        check(psc_load_ic_k(ctx, 0, fk.data(), pk.data(), a, ad), "psc_load_ic_k");
+        check(psc_set_scalars(ctx, 0, t0, a, ad, sc[3]), "psc_set_scalars");
+        if (!u.empty()) {
+            check(psc_load_gw_k(ctx, 0, u.data(), pu.data()), "psc_load_gw_k");
+            gw_restored = true;
+        }
+        return true;
+    }
+    bool gw_restored = false;
 
     // Reads grid.img into the context; returns false when there is none.
     bool read_grid(int& run_number, double& t0) {
@@ -390,6 +454,10 @@ struct Run {
         if (!g) {
             std::fprintf(stderr, "psc_run: %s/grid.img is truncated or from another lattice size\n", dir.c_str());
             std::exit(2);
+        }
+        if (read_side(run_number, t0, a, ad)) {
+            std::printf("k-space state read from grid.img.k\n");
+            return true;
         }
         check(psc_load_fields(ctx, 0, f.data(), p.data(), t0, a, ad), "psc_load_fields");
         return true;
@@ -552,13 +620,16 @@ int main(int argc, char** argv) {
         ++run_number;
         resumed = true;
         std::printf("Data read. Resuming run at t = %g\n", t_start);
-        if (c.with_gw) std::fprintf(stderr, "psc_run: grid.img holds no GW state; GW restart from vacuum\n");
+        if (c.with_gw && !r.gw_restored)
+            std::fprintf(stderr, "psc_run: grid.img holds no GW state; GW restart from vacuum\n");
     } else if (ic_device) {
         r.check(psc_generate_ic_device(r.ctx, 0, &r.ic, sc0), "psc_generate_ic_device");
     } else {
         r.check(psc_load_ic_k(r.ctx, 0, fk.data(), pk.data(), sc0[0], sc0[1]), "psc_load_ic_k");
     }
-    if (c.with_gw && ic_device) {
+    if (c.with_gw && r.gw_restored) {
+        // the GW state came with the k-space image
+    } else if (c.with_gw && ic_device) {
         r.check(psc_generate_gw_ic_device(r.ctx, 0, &r.ic), "psc_generate_gw_ic_device");
     } else if (c.with_gw) {
         const size_t gk = (size_t)6 * c.N * c.N * (c.N / 2 + 1) * 2;

@@ -134,12 +134,53 @@ def test_checkpoint_and_continue_match_an_uninterrupted_run(tmp_path):
     assert (b_dir / "grid.img").stat().st_size == 4 + 3 * 8 + 2 * 2 * 16 ** 3 * 8
     r = run_driver(b_dir, *common, "nsteps=10", "checkpoint=1", "continue_run=1")
     assert "Resuming run at t = 0.05" in r.stdout
+    assert "k-space state read from grid.img.k" in r.stdout
     ea, eb = rows(a_dir / "average_energies.txt"), rows(b_dir / "average_energies.txt")
     assert len(ea) == len(eb) == 2
     close(eb[1], ea[1], rtol=2e-6)
     sa = [x for x in rows(a_dir / "spectra_scalar_1.txt") if x is not None]
     sb = [x for x in rows(b_dir / "spectra_scalar_1.txt") if x is not None]
     close(sb, sa, rtol=2e-6)
+
+
+def test_continue_reads_the_reference_image_without_its_k_space_file(tmp_path):
+    """Without grid.img.k (or with one that belongs to another image) the
+    resume reads the reference's real-space image, as the reference does."""
+    a_dir, b_dir = tmp_path / "a", tmp_path / "b"
+    a_dir.mkdir()
+    b_dir.mkdir()
+    common = ["dt=0.01", "freq_steps=5", "infreq_steps=5"]
+    run_driver(a_dir, *common, "nsteps=10")
+    run_driver(b_dir, *common, "nsteps=5", "checkpoint=1")
+    os.remove(b_dir / "grid.img.k")
+    r = run_driver(b_dir, *common, "nsteps=10", "continue_run=1")
+    assert "Resuming run at t = 0.05" in r.stdout and "k-space state" not in r.stdout
+    close(rows(b_dir / "average_energies.txt")[1], rows(a_dir / "average_energies.txt")[1], rtol=2e-6)
+
+
+@pytest.mark.parametrize("extra", [("with_gw=1",), ("pad=2", "N=32", "N_pad=48")],
+                         ids=["gw", "padded_n48"])
+def test_continue_restores_the_k_space_state(tmp_path, extra):
+    """The k-space image carries what the reference's image cannot: the GW
+    u_ij and pi_u_ij (which the real-space image would restart from vacuum) and
+    the dealiased stepper's plane parts; the resumed run matches the
+    uninterrupted one to the printed digits."""
+    a_dir, b_dir = tmp_path / "a", tmp_path / "b"
+    a_dir.mkdir()
+    b_dir.mkdir()
+    common = ["dt=0.01", "freq_steps=5", "infreq_steps=5", *extra]
+    run_driver(a_dir, *common, "nsteps=10")
+    run_driver(b_dir, *common, "nsteps=5", "checkpoint=1")
+    r = run_driver(b_dir, *common, "nsteps=10", "checkpoint=1", "continue_run=1")
+    assert "k-space state read from grid.img.k" in r.stdout
+    assert "restart from vacuum" not in r.stderr
+    names = ["average_energies.txt", "average_scalar_0.txt", "average_scalar_1.txt"]
+    if "with_gw=1" in extra:
+        names += ["energy_gws.txt"]
+    for name in names:
+        ra, rb = rows(a_dir / name), rows(b_dir / name)
+        assert len(ra) == len(rb) == 2, name
+        close(rb[1], ra[1], rtol=2e-6, atol=1e-300)
 
 
 def test_device_initial_conditions_give_the_same_run(tmp_path):
