@@ -49,7 +49,7 @@ enum Kind {
 };
 inline const char* const kKindNames[K_NKINDS] = {"z_inverse_drift", "y_inverse", "x_force", "y_forward",
                                     "z_forward_kick", "scalars", "z_inverse", "z_forward_kick_first",
-                                    "diagnostics", "z_inverse_drift_gw", "x_source_gw", "z_forward_kick_gw",
+                                    "diagnostics", "z_inverse_drift_gw", "x_force_gw", "z_forward_kick_gw",
                                     "y_inverse_gw", "pad_drift", "pad_z_inverse", "pad_y_inverse", "pad_x_force",
                                     "pad_y_forward", "pad_z_forward", "pad_kick", "z_forward_kick_sync",
                                     "z_forward_kick_gw_sync", "pad_kick_sync"};

@@ -36,7 +36,8 @@ struct Engine final : EngineBase {
     // multi-tile form one per resident slot of the device
     const int xblocks = G::XMT ? resident_blocks((const void*)k_x_force_mt<G>, G::THREADS_COL, G::BLOCKS_COL)
                                : G::BLOCKS_COL;
-    int nV_blocks(const psc_ctx*) const { return xblocks; }
+    // the GW chain's merged x pass (k_x_gw) writes one partial per tile
+    int nV_blocks(const psc_ctx* c) const { return c->cfg.with_gw ? G::BLOCKS_COL : xblocks; }
     // explicit kick coefficients (the adjoint's replays); prologue: the first
     // kick at a_start only
     static KickArgs kick_args(double c1, double c2, bool prologue = false) {
@@ -201,14 +202,13 @@ struct Engine final : EngineBase {
             });
         if (rc) return rc;
         if ((rc = launch_y_inverse_gw(c, astride))) return rc;
-        // gradients: x inverse, the source and its x forward transforms in
-        // one pass, before the fields' x pass overwrites W0 with V'
-        rc = timed(c, K_X_FORCE_GW, 6 * X, [&] {
-            hipLaunchKernelGGL((k_x_source<G>), dim3(N / G::TC, N, c->B), dim3(G::THREADS_COL), 0, c->stream,
-                               P(c->W), P(c->tw), astride, c->cfg.kIR, nullptr);
+        // the fields' x pass (V') and the gradients' (the source and its x
+        // forward transforms) in one pass over the columns
+        rc = timed(c, K_X_FORCE_GW, 7 * X, [&] {
+            hipLaunchKernelGGL((k_x_gw<G>), dim3(N / G::TC, N, c->B), dim3(G::THREADS_COL), 0, c->stream, P(c->W),
+                               c->ms, P(c->tw), c->partV, P(c->Pf), astride, c->cfg.kIR);
         });
         if (rc) return rc;
-        if ((rc = launch_x_force(c))) return rc;
         if ((rc = launch_y<-1>(c, P(c->W), 4 * c->B))) return rc;
         return timed(c, sync_out ? K_Z_KICK_GW_SYNC : K_Z_KICK_GW, (sync_out ? 20 : 16) * X, [&] {
             if (reduce && sync_out)

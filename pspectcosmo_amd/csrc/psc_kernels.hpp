@@ -1129,6 +1129,91 @@ k_x_source(cpx<typename G::T>* __restrict__ W, const cpx<typename G::T>* __restr
     }
 }
 
+// C (GW), both parts in one pass: per tile, the fields' column (W0) is read
+// once; its x inverse, V' and x forward are those of k_x_force (W0 <- V'
+// spectrum, the V partial sums at k_x_force's places), and the x gradient
+// formed from the same column (kept in registers), with the y and z gradients
+// (W2, W3), gives the source of k_x_source (W1..W3).  Moves 7X against 8X for
+// the two kernels; the same arithmetic per column.
+template <class G>
+__global__ void __launch_bounds__((G::THREADS_COL))
+k_x_gw(cpx<typename G::T>* __restrict__ W, const MemberState* __restrict__ ms, const cpx<typename G::T>* __restrict__ tw,
+       double* __restrict__ partV, const cpx<typename G::T>* __restrict__ Pf, size_t astride, double kIR) {
+    using T = typename G::T;
+    constexpr int N = G::N;
+    __shared__ cpx<T> lds[G::LayCol::size(0)];
+    const int c = threadIdx.x % G::TC, t = threadIdx.x / G::TC;
+    const int member = blockIdx.z;
+    const size_t z = (size_t)blockIdx.x * G::TC + c;
+    const size_t o = blockIdx.y;
+    constexpr size_t stride = G::PS;
+    const size_t base = (size_t)member * G::MS + o * N + z + t * stride;
+    cpx<T> f0[G::E], e[G::E];
+#pragma unroll
+    for (int m = 0; m < G::E; ++m) {
+        f0[m] = W[base + (size_t)G::TPL * m * stride];
+        e[m] = f0[m];
+    }
+    const Couplings<T> cp = couplings<T>(ms[member]);
+    const int nf = ms[member].nflds;
+    const cpx<T> dc = Pf[(size_t)member * G::MS];
+    // the fields: x inverse, V' - V'(mean), x forward (k_x_force)
+    {
+        const cpx<T>* twa = tw;
+        asm volatile("" : "+s"(twa));
+        line_fft<T, N, typename G::Rad, +1, typename G::LayCol>(e, t, c, lds, twa);
+    }
+    const T mean[2] = {(T)((double)dc.x / (double)G::N3), nf > 1 ? (T)((double)dc.y / (double)G::N3) : (T)0};
+    double v[1] = {0.0};
+    x_force_pointwise<G, false>(e, mean, nf, cp, v[0], nullptr, 0, stride);
+    {
+        const cpx<T>* twa = tw;
+        asm volatile("" : "+s"(twa));
+        line_fft<T, N, typename G::Rad, -1, typename G::LayCol>(e, t, c, lds, twa);
+    }
+#pragma unroll
+    for (int m = 0; m < G::E; ++m) W[base + (size_t)G::TPL * m * stride] = e[m];
+    // the source from the x gradient of the same column and the y, z gradients
+    // (k_x_source)
+    cpx<T> g[3][G::E];
+#pragma unroll
+    for (int a = 1; a < 3; ++a)
+#pragma unroll
+        for (int m = 0; m < G::E; ++m) g[a][m] = W[(size_t)(a + 1) * astride + base + (size_t)G::TPL * m * stride];
+    const T inv = (T)(1.0 / (double)G::N3);
+#pragma unroll
+    for (int m = 0; m < G::E; ++m) {
+        const T kx = (T)(kIR * deriv_index(t + G::TPL * m, N));
+        g[0][m] = cpx<T>{-kx * f0[m].y, kx * f0[m].x};
+    }
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const cpx<T>* twa = tw;
+        asm volatile("" : "+s"(twa));
+        line_fft<T, N, typename G::Rad, +1, typename G::LayCol>(g[a], t, c, lds, twa);
+    }
+#pragma unroll
+    for (int m = 0; m < G::E; ++m) {
+        const T ax = g[0][m].x * inv, bx = g[0][m].y * inv;
+        const T ay = g[1][m].x * inv, by = g[1][m].y * inv;
+        const T az = g[2][m].x * inv, bz = g[2][m].y * inv;
+        g[0][m] = cpx<T>{ax * ax + bx * bx, ay * ay + by * by};
+        g[1][m] = cpx<T>{az * az + bz * bz, ax * ay + bx * by};
+        g[2][m] = cpx<T>{ax * az + bx * bz, ay * az + by * bz};
+    }
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const cpx<T>* twa = tw;
+        asm volatile("" : "+s"(twa));
+        line_fft<T, N, typename G::Rad, -1, typename G::LayCol>(g[a], t, c, lds, twa);
+#pragma unroll
+        for (int m = 0; m < G::E; ++m) W[(size_t)(a + 1) * astride + base + (size_t)G::TPL * m * stride] = g[a][m];
+    }
+    block_sum<1, G::THREADS_COL>(v, reinterpret_cast<double*>(lds));
+    if (threadIdx.x == 0)
+        partV[(size_t)member * G::BLOCKS_COL + blockIdx.y * gridDim.x + blockIdx.x] = v[0];
+}
+
 // C (GW), first part, multi-tile form for the long columns (512 points): the
 // arithmetic of k_x_source per tile, each block looping over the tiles b,
 // b + G, ... of its member.  The next tile's three columns are loaded into
