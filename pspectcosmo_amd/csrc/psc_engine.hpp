@@ -190,10 +190,11 @@ struct Engine final : EngineBase {
         const StepConsts sc = consts_of(c);
         int rc;
         dim3 rgrid(G::BLOCKS_ROW, c->B), rblock(G::THREADS_ROW);
+        // u is drifted by the u kick at the end of the chain (k_z_kick_u)
         if (drift)
-            rc = timed(c, K_Z_INV_GW, 14 * X, [&] {
-                hipLaunchKernelGGL((k_z_inverse_gw<G, true>), rgrid, rblock, 0, c->stream, P(c->Pf), P(c->Pp),
-                                   P(c->W), P(c->U), P(c->PU), c->ms, P(c->tw), astride, c->cfg.kIR);
+            rc = timed(c, K_Z_INV_GW, 5 * X, [&] {
+                hipLaunchKernelGGL((k_z_inverse_gw<G, true, false, false>), rgrid, rblock, 0, c->stream, P(c->Pf),
+                                   P(c->Pp), P(c->W), P(c->U), P(c->PU), c->ms, P(c->tw), astride, c->cfg.kIR);
             });
         else
             rc = timed(c, K_Z_INV_GW, 3 * X, [&] {
@@ -210,19 +211,25 @@ struct Engine final : EngineBase {
         });
         if (rc) return rc;
         if ((rc = launch_y<-1>(c, P(c->W), 4 * c->B))) return rc;
-        return timed(c, sync_out ? K_Z_KICK_GW_SYNC : K_Z_KICK_GW, (sync_out ? 20 : 16) * X, [&] {
-            if (reduce && sync_out)
-                hipLaunchKernelGGL((k_z_kick_gw<G, true, false, true>), rgrid, rblock, 0, c->stream, P(c->W), P(c->Pf),
-                                   P(c->Pp), P(c->U), P(c->PU), c->ms, P(c->tw), c->partKG, ka, sc, astride, nullptr,
-                                   P(c->Ps), P(c->PUs));
-            else if (reduce)
-                hipLaunchKernelGGL((k_z_kick_gw<G, true>), rgrid, rblock, 0, c->stream, P(c->W), P(c->Pf), P(c->Pp),
-                                   P(c->U), P(c->PU), c->ms, P(c->tw), c->partKG, ka, sc, astride, nullptr, nullptr,
-                                   nullptr);
-            else
-                hipLaunchKernelGGL((k_z_kick_gw<G, false>), rgrid, rblock, 0, c->stream, P(c->W), P(c->Pf), P(c->Pp),
-                                   P(c->U), P(c->PU), c->ms, P(c->tw), c->partKG, ka, sc, astride, nullptr, nullptr,
-                                   nullptr);
+        // the fields' kick (with the reduction) and the u kick, one source
+        // array per block
+        if ((rc = launch_z_kick(c, reduce, ka, sync_out))) return rc;
+        return timed(c, sync_out ? K_Z_KICK_GW_SYNC : K_Z_KICK_GW, (sync_out ? 15 : 12) * X + (drift ? 3 * X : 0), [&] {
+            dim3 ugrid(G::BLOCKS_ROW, c->B, 3);
+            auto go = [&](auto sync, auto dr) {
+                hipLaunchKernelGGL((k_z_kick_u<G, decltype(sync)::value, decltype(dr)::value>), ugrid, rblock, 0,
+                                   c->stream, P(c->W), P(c->U), P(c->PU), c->ms, P(c->tw), ka, sc, astride,
+                                   decltype(sync)::value ? P(c->PUs) : nullptr);
+            };
+            using TT = std::true_type;
+            using FF = std::false_type;
+            if (sync_out) {
+                if (drift) go(TT{}, TT{});
+                else go(TT{}, FF{});
+            } else {
+                if (drift) go(FF{}, TT{});
+                else go(FF{}, FF{});
+            }
         });
     }
 

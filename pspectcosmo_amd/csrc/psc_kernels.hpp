@@ -945,11 +945,11 @@ k_z_inverse_deriv(const cpx<typename G::T>* __restrict__ Pf, cpx<typename G::T>*
 }
 
 
-// A (GW): drift of fields and u, then z-axis inverse of the fields (W0) and
-// of the z gradient i kIR n_z P_f (W3).  The x and y gradients need no z
+// A (GW): drift of fields and (DRIFT_U) of u, then z-axis inverse of the
+// fields (W0) and of the z gradient i kIR n_z P_f (W3).  The x and y gradients need no z
 // pass of their own: i kIR n_x and i kIR n_y are constant along a z line, so
-// they are formed from W0 in the y and x passes (k_y_inverse_gw, k_x_source).
-template <class G, bool DRIFT, bool REC = false>
+// they are formed from W0 in the y and x passes (k_y_inverse_gw, k_x_gw).
+template <class G, bool DRIFT, bool REC = false, bool DRIFT_U = true>
 __global__ void __launch_bounds__((G::THREADS_ROW))
 k_z_inverse_gw(cpx<typename G::T>* __restrict__ Pf, const cpx<typename G::T>* __restrict__ Pp, cpx<typename G::T>* __restrict__ W,
                cpx<typename G::T>* __restrict__ U, const cpx<typename G::T>* __restrict__ PU, MemberState* __restrict__ ms,
@@ -998,7 +998,7 @@ k_z_inverse_gw(cpx<typename G::T>* __restrict__ Pf, const cpx<typename G::T>* __
     line_fft<T, N, typename G::Rad, +1, typename G::LayRow>(e, t, c, lds, tw);
 #pragma unroll
     for (int m = 0; m < G::E; ++m) W[3 * astride + base + G::TPL * m] = e[m];
-    if constexpr (DRIFT) {
+    if constexpr (DRIFT && DRIFT_U) {   // else the u kick (k_z_kick_u) drifts u
 #pragma unroll
         for (int a = 0; a < 3; ++a)
 #pragma unroll
@@ -1020,7 +1020,7 @@ k_z_inverse_gw(cpx<typename G::T>* __restrict__ Pf, const cpx<typename G::T>* __
 // B (GW): y-axis inverse of the fields W0 and of the z gradient W3, and the
 // y gradient i kIR n_y W0 formed from the fields' column before its
 // transform, into W2.  W1 is not read (the x gradient comes from W0 in
-// k_x_source).  The fields' column is read twice (the second time from the
+// the x pass).  The fields' column is read twice (the second time from the
 // cache) rather than held through the gradient's transform: one column in
 // registers at a time keeps the 16-element plans free of spills.
 template <class G>
@@ -1380,6 +1380,70 @@ k_z_kick_gw(const cpx<typename G::T>* __restrict__ W, const cpx<typename G::T>* 
             dst[0] = red[0];
             dst[1] = red[1];
         }
+    }
+}
+
+// E (GW), the u part alone: z-axis forward of one packed source array
+// (W1..W3) and the GW kick of its u pair, one array per block (blockIdx.z);
+// the fields' kick runs beside it as k_z_kick.  Same arithmetic as the u part
+// of k_z_kick_gw, at lower register pressure (measured 3301 -> 3171 us for
+// the whole kick at 512^3 FP32; k_z_kick_gw stays for the adjoint's
+// recording replay).  DRIFT applies the substep's drift of u first, in place
+// of k_z_inverse_gw: nothing reads u between the two, and the kick loads u
+// and pi_u anyway, so the drift costs one store instead of two loads and a
+// store in the z inverse (50X -> 44X per substep).
+template <class G, bool SYNC_OUT = false, bool DRIFT = false>
+__global__ void __launch_bounds__((G::THREADS_ROW))
+k_z_kick_u(const cpx<typename G::T>* __restrict__ W, cpx<typename G::T>* __restrict__ U,
+           cpx<typename G::T>* __restrict__ PU, const MemberState* __restrict__ ms,
+           const cpx<typename G::T>* __restrict__ tw, KickArgs ka, StepConsts sc, size_t astride,
+           cpx<typename G::T>* __restrict__ PUs = nullptr) {
+    using T = typename G::T;
+    constexpr int N = G::N;
+    __shared__ cpx<T> lds[G::LayRow::size(G::L)];
+    const int t = threadIdx.x % G::TPL, c = threadIdx.x / G::TPL;
+    const int member = blockIdx.y;
+    const int ga = blockIdx.z;
+    if (ka.prologue && ms[member].ahead) return;   // the call's first kick was applied as a look-ahead
+    const size_t line = (size_t)blockIdx.x * G::L + c;
+    const size_t base = (size_t)member * G::MS + (line / N) * G::PS + (line % N) * N + t;
+    const double a = ka.prologue ? ms[member].a_start : ms[member].a;
+    const T norm1 = (T)pow(a, 1 + sc.alpha);
+    const T k2c = (T)(sc.kIR * sc.kIR);
+    double c1d, c2d;
+    kick_coefs(ka, ms[member], c1d, c2d);
+    const T c1 = (T)c1d, c2 = (T)c2d;
+    const int px = signed_index((int)(line / N), N), py = signed_index((int)(line % N), N);
+    const int pxy = px * px + py * py;
+    const T cd = DRIFT ? (T)ms[member].c_drift : (T)0;
+    cpx<T> e[G::E], fa[G::E], pa[G::E];
+#pragma unroll
+    for (int m = 0; m < G::E; ++m) {
+        const size_t idx = (size_t)ga * astride + base + G::TPL * m;
+        e[m] = W[(size_t)(ga + 1) * astride + base + G::TPL * m];
+        fa[m] = U[idx];
+        pa[m] = PU[idx];
+    }
+    line_fft<T, N, typename G::Rad, -1, typename G::LayRow>(e, t, c, lds, tw);
+#pragma unroll
+    for (int m = 0; m < G::E; ++m) {
+        const int pz = signed_index(t + G::TPL * m, N);
+        const T k2 = k2c * (T)(pxy + pz * pz);
+        const size_t idx = (size_t)ga * astride + base + G::TPL * m;
+        cpx<T> u = fa[m];
+        cpx<T> p = pa[m];
+        if constexpr (DRIFT) {   // the substep's drift of u (evolution.cpp:654-663)
+            u.x += cd * p.x;
+            u.y += cd * p.y;
+            U[idx] = u;
+        }
+        const cpx<T> F = {norm1 * (-k2 * u.x + (T)2 * e[m].x), norm1 * (-k2 * u.y + (T)2 * e[m].y)};
+        p.x += c1 * F.x;
+        p.y += c1 * F.y;
+        if constexpr (SYNC_OUT) store_stream(PUs + idx, p);
+        p.x += c2 * F.x;
+        p.y += c2 * F.y;
+        PU[idx] = p;
     }
 }
 
