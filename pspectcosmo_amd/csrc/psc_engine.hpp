@@ -614,14 +614,16 @@ struct Engine final : EngineBase {
             const int p = j % S;
             const bool last = j == steps * S - 1;
             const double wn = c->w[(p + 1) % S];
-            hipLaunchKernelGGL((k_z_inverse_gw<G, true, true>), rgrid, rblock, 0, c->stream, P(c->Pf), P(c->Pp),
-                               P(c->W), P(c->U), P(c->PU), c->ms, P(c->tw), as, kIR, P(w.f[j]), P(w.ph[j]),
-                               P(w.u[j + 1]), P(w.puh[j]));
+            // the u drift and its recording (u after the drift, pi_u half) run in
+            // the kick, which loads u and pi_u anyway, as in the forward chain
+            hipLaunchKernelGGL((k_z_inverse_gw<G, true, true, false>), rgrid, rblock, 0, c->stream, P(c->Pf),
+                               P(c->Pp), P(c->W), P(c->U), P(c->PU), c->ms, P(c->tw), as, kIR, P(w.f[j]),
+                               P(w.ph[j]), nullptr, nullptr);
             if ((rc = middle(j + 1))) return rc;
-            hipLaunchKernelGGL((k_z_kick_gw<G, true, true>), rgrid, rblock, 0, c->stream, P(c->W), P(c->Pf),
-                               P(c->Pp), P(c->U), P(c->PU), c->ms, P(c->tw), c->partKG,
+            hipLaunchKernelGGL((k_z_kick_gw<G, true, true, false, true>), rgrid, rblock, 0, c->stream, P(c->W),
+                               P(c->Pf), P(c->Pp), P(c->U), P(c->PU), c->ms, P(c->tw), c->partKG,
                                kick_args(c->w[p] * dt / 2.0, last ? 0.0 : wn * dt / 2.0), sc, as, P(w.p[j + 1]),
-                               nullptr, nullptr);
+                               nullptr, nullptr, P(w.u[j + 1]), P(w.puh[j]));
             ScalarOp o{};
             o.finish = 1;
             o.last_of_step = p == S - 1;

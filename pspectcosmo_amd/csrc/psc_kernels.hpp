@@ -1290,14 +1290,17 @@ k_x_source_mt(cpx<typename G::T>* __restrict__ W, const cpx<typename G::T>* __re
 }
 
 // E (GW): z-axis forward of V' and of the source, field kick as k_z_kick and
-// the GW kick of the three packed u arrays.
-template <class G, bool REDUCE, bool REC = false, bool SYNC_OUT = false>
+// the GW kick of the three packed u arrays (DRIFT_U: after the substep's u
+// drift, as k_z_kick_u; the recording replay of the adjoint also records u
+// after the drift and pi_u half here).
+template <class G, bool REDUCE, bool REC = false, bool SYNC_OUT = false, bool DRIFT_U = false>
 __global__ void __launch_bounds__((G::THREADS_ROW))
 k_z_kick_gw(const cpx<typename G::T>* __restrict__ W, const cpx<typename G::T>* __restrict__ Pf, cpx<typename G::T>* __restrict__ Pp,
-            const cpx<typename G::T>* __restrict__ U, cpx<typename G::T>* __restrict__ PU, MemberState* __restrict__ ms,
+            cpx<typename G::T>* __restrict__ U, cpx<typename G::T>* __restrict__ PU, MemberState* __restrict__ ms,
             const cpx<typename G::T>* __restrict__ tw, double* __restrict__ partKG, KickArgs ka,
             StepConsts sc, size_t astride, cpx<typename G::T>* __restrict__ rec_p = nullptr,
-            cpx<typename G::T>* __restrict__ Ps = nullptr, cpx<typename G::T>* __restrict__ PUs = nullptr) {
+            cpx<typename G::T>* __restrict__ Ps = nullptr, cpx<typename G::T>* __restrict__ PUs = nullptr,
+            cpx<typename G::T>* __restrict__ rec_u = nullptr, cpx<typename G::T>* __restrict__ rec_puh = nullptr) {
     using T = typename G::T;
     constexpr int N = G::N;
     __shared__ cpx<T> lds[G::LayRow::size(G::L)];
@@ -1315,6 +1318,7 @@ k_z_kick_gw(const cpx<typename G::T>* __restrict__ W, const cpx<typename G::T>* 
     double c1d, c2d;
     kick_coefs(ka, ms[member], c1d, c2d);
     const T c1 = (T)c1d, c2 = (T)c2d;
+    const T cd = DRIFT_U ? (T)ms[member].c_drift : (T)0;
     const int ix = (int)(line / N), iy = (int)(line % N);
     const int px = signed_index(ix, N), py = signed_index(iy, N);
     const int pxy = px * px + py * py;
@@ -1362,8 +1366,17 @@ k_z_kick_gw(const cpx<typename G::T>* __restrict__ W, const cpx<typename G::T>* 
             const int pz = signed_index(t + G::TPL * m, N);
             const T k2 = k2c * (T)(pxy + pz * pz);
             const size_t idx = (size_t)ga * astride + base + G::TPL * m;
-            const cpx<T> u = fa[m];
+            cpx<T> u = fa[m];
             cpx<T> p = pa[m];
+            if constexpr (DRIFT_U) {   // the substep's u drift, as k_z_kick_u
+                u.x += cd * p.x;
+                u.y += cd * p.y;
+                U[idx] = u;
+                if constexpr (REC) {   // adjoint recording: u after the drift, pi_u half
+                    rec_u[idx] = u;
+                    rec_puh[idx] = p;
+                }
+            }
             const cpx<T> F = {norm1 * (-k2 * u.x + (T)2 * e[m].x), norm1 * (-k2 * u.y + (T)2 * e[m].y)};
             p.x += c1 * F.x;
             p.y += c1 * F.y;
