@@ -1214,81 +1214,6 @@ k_x_gw(cpx<typename G::T>* __restrict__ W, const MemberState* __restrict__ ms, c
         partV[(size_t)member * G::BLOCKS_COL + blockIdx.y * gridDim.x + blockIdx.x] = v[0];
 }
 
-// C (GW), first part, multi-tile form for the long columns (512 points): the
-// arithmetic of k_x_source per tile, each block looping over the tiles b,
-// b + G, ... of its member.  The next tile's three columns are loaded into
-// the registers of each array as soon as that array's forward transform is
-// stored, so the loads of tile q + G overlap the remaining transforms of tile
-// q and the first inverse transforms of tile q + G, with no registers beyond
-// the three columns.
-template <class G>
-__global__ void __launch_bounds__((G::THREADS_COL))
-k_x_source_mt(cpx<typename G::T>* __restrict__ W, const cpx<typename G::T>* __restrict__ tw, size_t astride,
-              double kIR) {
-    using T = typename G::T;
-    constexpr int N = G::N;
-    constexpr int NT = (N / G::TC) * N;
-    constexpr size_t stride = G::PS;
-    __shared__ cpx<T> lds[G::LayCol::size(0)];
-    const int c = threadIdx.x % G::TC, t = threadIdx.x / G::TC;
-    const int member = blockIdx.z;
-    const int GS = (int)gridDim.x;
-    const T inv = (T)(1.0 / (double)G::N3);
-    auto tile_base = [&](int q) {
-        return (size_t)member * G::MS + (size_t)(q / (N / G::TC)) * N + (size_t)(q % (N / G::TC)) * G::TC + c +
-               (size_t)t * stride;
-    };
-    // array 0: the fields (W0), whose x gradient is formed after the load;
-    // arrays 1, 2: the y and z gradients (W2, W3)
-    auto in_off = [&](int a) { return a == 0 ? (size_t)0 : (size_t)(a + 1) * astride; };
-    cpx<T> g[3][G::E];
-    int q = blockIdx.x;
-    if (q < NT) {
-        const size_t b = tile_base(q);
-#pragma unroll
-        for (int a = 0; a < 3; ++a)
-#pragma unroll
-            for (int m = 0; m < G::E; ++m) g[a][m] = W[in_off(a) + b + (size_t)G::TPL * m * stride];
-    }
-    for (; q < NT; q += GS) {
-        const size_t base = tile_base(q);
-#pragma unroll
-        for (int m = 0; m < G::E; ++m) {
-            const T kx = (T)(kIR * deriv_index(t + G::TPL * m, N));
-            g[0][m] = cpx<T>{-kx * g[0][m].y, kx * g[0][m].x};
-        }
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            const cpx<T>* twa = tw;   // reloaded per transform (cached), as in k_x_source
-            asm volatile("" : "+s"(twa));
-            line_fft<T, N, typename G::Rad, +1, typename G::LayCol>(g[a], t, c, lds, twa);
-        }
-#pragma unroll
-        for (int m = 0; m < G::E; ++m) {
-            const T ax = g[0][m].x * inv, bx = g[0][m].y * inv;
-            const T ay = g[1][m].x * inv, by = g[1][m].y * inv;
-            const T az = g[2][m].x * inv, bz = g[2][m].y * inv;
-            g[0][m] = cpx<T>{ax * ax + bx * bx, ay * ay + by * by};
-            g[1][m] = cpx<T>{az * az + bz * bz, ax * ay + bx * by};
-            g[2][m] = cpx<T>{ax * az + bx * bz, ay * az + by * bz};
-        }
-        const bool more = q + GS < NT;
-        const size_t nb = more ? tile_base(q + GS) : 0;
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            const cpx<T>* twa = tw;
-            asm volatile("" : "+s"(twa));
-            line_fft<T, N, typename G::Rad, -1, typename G::LayCol>(g[a], t, c, lds, twa);
-#pragma unroll
-            for (int m = 0; m < G::E; ++m) W[(size_t)(a + 1) * astride + base + (size_t)G::TPL * m * stride] = g[a][m];
-            if (more) {
-#pragma unroll
-                for (int m = 0; m < G::E; ++m) g[a][m] = W[in_off(a) + nb + (size_t)G::TPL * m * stride];
-            }
-        }
-    }
-}
-
 // E (GW): z-axis forward of V' and of the source, field kick as k_z_kick and
 // the GW kick of the three packed u arrays (DRIFT_U: after the substep's u
 // drift, as k_z_kick_u; the recording replay of the adjoint also records u
