@@ -193,8 +193,13 @@ struct Engine final : EngineBase {
         // u is drifted by the u kick at the end of the chain (k_z_kick_u)
         if (drift)
             rc = timed(c, K_Z_INV_GW, 5 * X, [&] {
-                hipLaunchKernelGGL((k_z_inverse_gw<G, true, false, false>), rgrid, rblock, 0, c->stream, P(c->Pf),
-                                   P(c->Pp), P(c->W), P(c->U), P(c->PU), c->ms, P(c->tw), astride, c->cfg.kIR);
+                if (streams(c->B))
+                    hipLaunchKernelGGL((k_z_inverse_gw<G, true, false, false, true>), rgrid, rblock, 0, c->stream,
+                                       P(c->Pf), P(c->Pp), P(c->W), P(c->U), P(c->PU), c->ms, P(c->tw), astride,
+                                       c->cfg.kIR);
+                else
+                    hipLaunchKernelGGL((k_z_inverse_gw<G, true, false, false>), rgrid, rblock, 0, c->stream, P(c->Pf),
+                                       P(c->Pp), P(c->W), P(c->U), P(c->PU), c->ms, P(c->tw), astride, c->cfg.kIR);
             });
         else
             rc = timed(c, K_Z_INV_GW, 3 * X, [&] {
@@ -216,19 +221,23 @@ struct Engine final : EngineBase {
         if ((rc = launch_z_kick(c, reduce, ka, sync_out))) return rc;
         return timed(c, sync_out ? K_Z_KICK_GW_SYNC : K_Z_KICK_GW, (sync_out ? 15 : 12) * X + (drift ? 3 * X : 0), [&] {
             dim3 ugrid(G::BLOCKS_ROW, c->B, 3);
-            auto go = [&](auto sync, auto dr) {
-                hipLaunchKernelGGL((k_z_kick_u<G, decltype(sync)::value, decltype(dr)::value>), ugrid, rblock, 0,
-                                   c->stream, P(c->W), P(c->U), P(c->PU), c->ms, P(c->tw), ka, sc, astride,
-                                   decltype(sync)::value ? P(c->PUs) : nullptr);
+            auto go = [&](auto sync, auto dr, auto st) {
+                hipLaunchKernelGGL((k_z_kick_u<G, decltype(sync)::value, decltype(dr)::value, decltype(st)::value>),
+                                   ugrid, rblock, 0, c->stream, P(c->W), P(c->U), P(c->PU), c->ms, P(c->tw), ka, sc,
+                                   astride, decltype(sync)::value ? P(c->PUs) : nullptr);
             };
             using TT = std::true_type;
             using FF = std::false_type;
+            auto go_st = [&](auto sync, auto dr) {
+                if (streams(c->B)) go(sync, dr, TT{});
+                else go(sync, dr, FF{});
+            };
             if (sync_out) {
-                if (drift) go(TT{}, TT{});
-                else go(TT{}, FF{});
+                if (drift) go_st(TT{}, TT{});
+                else go_st(TT{}, FF{});
             } else {
-                if (drift) go(FF{}, TT{});
-                else go(FF{}, FF{});
+                if (drift) go_st(FF{}, TT{});
+                else go_st(FF{}, FF{});
             }
         });
     }

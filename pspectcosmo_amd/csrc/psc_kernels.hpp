@@ -946,10 +946,12 @@ k_z_inverse_deriv(const cpx<typename G::T>* __restrict__ Pf, cpx<typename G::T>*
 
 
 // A (GW): drift of fields and (DRIFT_U) of u, then z-axis inverse of the
-// fields (W0) and of the z gradient i kIR n_z P_f (W3).  The x and y gradients need no z
+// fields (W0) and of the z gradient i kIR n_z P_f (W3).  STREAM: f and pi
+// with non-temporal accesses (arrays beyond the memory-side cache; 1215 ->
+// 1180 us at 512^3 FP32).  The x and y gradients need no z
 // pass of their own: i kIR n_x and i kIR n_y are constant along a z line, so
 // they are formed from W0 in the y and x passes (k_y_inverse_gw, k_x_gw).
-template <class G, bool DRIFT, bool REC = false, bool DRIFT_U = true>
+template <class G, bool DRIFT, bool REC = false, bool DRIFT_U = true, bool STREAM = false>
 __global__ void __launch_bounds__((G::THREADS_ROW))
 k_z_inverse_gw(cpx<typename G::T>* __restrict__ Pf, const cpx<typename G::T>* __restrict__ Pp, cpx<typename G::T>* __restrict__ W,
                cpx<typename G::T>* __restrict__ U, const cpx<typename G::T>* __restrict__ PU, MemberState* __restrict__ ms,
@@ -968,9 +970,9 @@ k_z_inverse_gw(cpx<typename G::T>* __restrict__ Pf, const cpx<typename G::T>* __
     cpx<T> f[G::E];
 #pragma unroll
     for (int m = 0; m < G::E; ++m) {
-        cpx<T> v = Pf[base + G::TPL * m];
+        cpx<T> v = STREAM ? load_stream(Pf + base + G::TPL * m) : Pf[base + G::TPL * m];
         if constexpr (DRIFT) {
-            const cpx<T> p = Pp[base + G::TPL * m];
+            const cpx<T> p = STREAM ? load_stream(Pp + base + G::TPL * m) : Pp[base + G::TPL * m];
             if constexpr (REC) {  // adjoint recording: substep start f and pi_half
                 rec_f[base + G::TPL * m] = v;
                 rec_ph[base + G::TPL * m] = p;
@@ -981,7 +983,8 @@ k_z_inverse_gw(cpx<typename G::T>* __restrict__ Pf, const cpx<typename G::T>* __
                 v.x += cd * p.x;
                 v.y += cd * p.y;
             }
-            Pf[base + G::TPL * m] = v;
+            if constexpr (STREAM) store_stream(Pf + base + G::TPL * m, v);
+            else Pf[base + G::TPL * m] = v;
         }
         f[m] = v;
         e[m] = v;
@@ -1329,8 +1332,10 @@ k_z_kick_gw(const cpx<typename G::T>* __restrict__ W, const cpx<typename G::T>* 
 // recording replay).  DRIFT applies the substep's drift of u first, in place
 // of k_z_inverse_gw: nothing reads u between the two, and the kick loads u
 // and pi_u anyway, so the drift costs one store instead of two loads and a
-// store in the z inverse (50X -> 44X per substep).
-template <class G, bool SYNC_OUT = false, bool DRIFT = false>
+// store in the z inverse (50X -> 44X per substep).  STREAM reads and writes the
+// arrays with non-temporal accesses (arrays beyond the memory-side cache:
+// 2986 -> 2782 us at 512^3 FP32).
+template <class G, bool SYNC_OUT = false, bool DRIFT = false, bool STREAM = false>
 __global__ void __launch_bounds__((G::THREADS_ROW))
 k_z_kick_u(const cpx<typename G::T>* __restrict__ W, cpx<typename G::T>* __restrict__ U,
            cpx<typename G::T>* __restrict__ PU, const MemberState* __restrict__ ms,
@@ -1358,9 +1363,10 @@ k_z_kick_u(const cpx<typename G::T>* __restrict__ W, cpx<typename G::T>* __restr
 #pragma unroll
     for (int m = 0; m < G::E; ++m) {
         const size_t idx = (size_t)ga * astride + base + G::TPL * m;
-        e[m] = W[(size_t)(ga + 1) * astride + base + G::TPL * m];
-        fa[m] = U[idx];
-        pa[m] = PU[idx];
+        e[m] = STREAM ? load_stream(W + (size_t)(ga + 1) * astride + base + G::TPL * m)
+                      : W[(size_t)(ga + 1) * astride + base + G::TPL * m];
+        fa[m] = STREAM ? load_stream(U + idx) : U[idx];
+        pa[m] = STREAM ? load_stream(PU + idx) : PU[idx];
     }
     line_fft<T, N, typename G::Rad, -1, typename G::LayRow>(e, t, c, lds, tw);
 #pragma unroll
@@ -1373,7 +1379,8 @@ k_z_kick_u(const cpx<typename G::T>* __restrict__ W, cpx<typename G::T>* __restr
         if constexpr (DRIFT) {   // the substep's drift of u (evolution.cpp:654-663)
             u.x += cd * p.x;
             u.y += cd * p.y;
-            U[idx] = u;
+            if constexpr (STREAM) store_stream(U + idx, u);
+            else U[idx] = u;
         }
         const cpx<T> F = {norm1 * (-k2 * u.x + (T)2 * e[m].x), norm1 * (-k2 * u.y + (T)2 * e[m].y)};
         p.x += c1 * F.x;
@@ -1381,7 +1388,8 @@ k_z_kick_u(const cpx<typename G::T>* __restrict__ W, cpx<typename G::T>* __restr
         if constexpr (SYNC_OUT) store_stream(PUs + idx, p);
         p.x += c2 * F.x;
         p.y += c2 * F.y;
-        PU[idx] = p;
+        if constexpr (STREAM) store_stream(PU + idx, p);
+        else PU[idx] = p;
     }
 }
 
