@@ -790,6 +790,9 @@ struct PadEngine final : PadBase {
 
     int nV() const override { return xblocks; }
     int n_pad() const override { return NP; }
+    // the padded array is read once after each write and exceeds the
+    // memory-side cache: its z passes use non-temporal accesses
+    static constexpr bool stream_padded() { return (double)GR::MS * sizeof(cpx<T>) > 96.0 * (1 << 20); }
     int nKG() const override { return (int)((size_t)N * N * N / 2048); }
     size_t padded_elems() const override { return GP::MS; }
 
@@ -797,8 +800,12 @@ struct PadEngine final : PadBase {
     // the sum of V in the x pass
     int front(psc_ctx* c) {
         int rc = timed(c, K_PAD_Z_INV, XN + XP * KP * KP, [&] {
-            hipLaunchKernelGGL((k_pad_z<GR, N>), dim3(GR::BLOCKS_ROW), dim3(GR::THREADS_ROW), 0, c->stream, P(c->Pf),
-                               P(c->Gf), P(c->Wp), P(c->dcp), P(c->twp));
+            if constexpr (stream_padded())
+                hipLaunchKernelGGL((k_pad_z<GR, N, true>), dim3(pad_z_blocks<GR, N>()), dim3(GR::THREADS_ROW), 0,
+                                   c->stream, P(c->Pf), P(c->Gf), P(c->Wp), P(c->dcp), P(c->twp));
+            else
+                hipLaunchKernelGGL((k_pad_z<GR, N>), dim3(pad_z_blocks<GR, N>()), dim3(GR::THREADS_ROW), 0, c->stream,
+                                   P(c->Pf), P(c->Gf), P(c->Wp), P(c->dcp), P(c->twp));
         });
         if (rc) return rc;
         // the y passes and the stored x rows cover the padded x planes |x| <= N/2;
@@ -832,8 +839,12 @@ struct PadEngine final : PadBase {
         });
         if (rc) return rc;
         rc = timed(c, K_PAD_Z_FWD, XN * (N + 1) * (N + 1) / ((double)N * N) * 3, [&] {
-            hipLaunchKernelGGL((k_unpad_z<GR, N>), dim3(GR::BLOCKS_ROW), dim3(GR::THREADS_ROW), 0, c->stream, P(c->Wp),
-                               P(c->Rp), P(c->twp));
+            if constexpr (stream_padded())
+                hipLaunchKernelGGL((k_unpad_z<GR, N, true>), dim3(pad_z_blocks<GR, N>()), dim3(GR::THREADS_ROW), 0,
+                                   c->stream, P(c->Wp), P(c->Rp), P(c->twp));
+            else
+                hipLaunchKernelGGL((k_unpad_z<GR, N>), dim3(pad_z_blocks<GR, N>()), dim3(GR::THREADS_ROW), 0,
+                                   c->stream, P(c->Wp), P(c->Rp), P(c->twp));
         });
         if (rc) return rc;
         const StepConsts sc = consts_of(c);

@@ -85,8 +85,22 @@ k_drift_pad(cpx<T>* __restrict__ Pf, const cpx<T>* __restrict__ Pp, cpx<T>* __re
 
 // Padding and the z-axis inverse transform on the padded grid.  The k = 0
 // mode goes to dcp (times (N_pad/N)^3) and is left out of the transform, as
-// in the unpadded passes.
+// in the unpadded passes.  Only the padded lines that hold N-grid modes,
+// (sx, sy) in [-N/2, N/2]^2, are transformed and stored: the launch covers
+// exactly those (N + 1)^2 lines, L per block (pad_z_blocks), line li at
+// sx = li / (N + 1) - N/2, sy = li % (N + 1) - N/2; the last block's lines
+// past the end transform zeros (the transform's barriers need every thread)
+// and store nothing.  The padded x planes |x| > N/2 and the rows |y| > N/2
+// of the kept planes are never written: the y passes skip them and read them
+// as zero.  (A launch over every padded line, most of whose blocks left at
+// once, measured 140 us against 116 us at N = 256, N_pad = 512.)  STREAM
+// stores the padded lines with non-temporal stores (88 us there): they are
+// read once, by the y inverse, and exceed the memory-side cache.
 template <class GP, int N>
+constexpr unsigned pad_z_blocks() {
+    return (unsigned)(((size_t)(N + 1) * (N + 1) + GP::L - 1) / GP::L);
+}
+template <class GP, int N, bool STREAM = false>
 __global__ void __launch_bounds__((GP::THREADS_ROW))
 k_pad_z(const cpx<typename GP::T>* __restrict__ Pf, const cpx<typename GP::T>* __restrict__ Gf,
         cpx<typename GP::T>* __restrict__ Wp, cpx<typename GP::T>* __restrict__ dcp,
@@ -97,30 +111,15 @@ k_pad_z(const cpx<typename GP::T>* __restrict__ Pf, const cpx<typename GP::T>* _
     constexpr size_t PSN = plane_stride(N);
     __shared__ cpx<T> lds[GP::LayRow::size(GP::L)];
     const int t = threadIdx.x % GP::TPL, c = threadIdx.x / GP::TPL;
-    const size_t line = (size_t)blockIdx.x * GP::L + c;
-    const int xp = (int)(line / NP), yp = (int)(line % NP);
+    const int li = (int)blockIdx.x * GP::L + c;
+    const bool my = li < (N + 1) * (N + 1);
+    const int sx = my ? li / (N + 1) - H : 0, sy = my ? li % (N + 1) - H : 0;
+    const int xp = pad_wrap(sx, NP), yp = pad_wrap(sy, NP);
     const size_t base = (size_t)xp * GP::PS + (size_t)yp * NP + t;
-    const int sx = pad_signed(xp, NP), sy = pad_signed(yp, NP);
-    cpx<T> e[GP::E];
-    const bool mx = sx <= H && sx >= -H;
-    // Padded x planes |x| > N/2 hold no modes: the y passes skip them and the
-    // x pass reads them as zero, so nothing is written there; likewise the
-    // rows |y| > N/2 of the kept planes, which the y inverse reads as zero.
-    // When L divides N_pad, every line of a block has the same padded x index
-    // and the block covers L consecutive padded y: a block without modes
-    // leaves before the transform's barriers; otherwise lines without modes
-    // transform zeros and skip the store.
-    if constexpr (NP % GP::L == 0) {
-        if (!mx) return;
-        const int y0 = pad_signed((int)(((size_t)blockIdx.x * GP::L) % NP), NP);
-        const int y1 = pad_signed((int)(((size_t)blockIdx.x * GP::L + GP::L - 1) % NP), NP);
-        if (y0 > H && y1 > H) return;            // both ends in (H, N_pad/2) (no wrap: blocks align)
-        if (y0 < -H && y1 < -H) return;          // both ends in [-N_pad/2, -H)
-    }
-    const bool my = mx && sy <= H && sy >= -H;
     const int x = pad_wrap(sx, N), y = pad_wrap(sy, N);
     const T w = (T)(pad_volume_ratio<NP, N>() * (sx == H || sx == -H ? 0.5 : 1.0) *
                     (sy == H || sy == -H ? 0.5 : 1.0));
+    cpx<T> e[GP::E];
 #pragma unroll
     for (int m = 0; m < GP::E; ++m) {
         const int sz = pad_signed(t + GP::TPL * m, NP);
@@ -146,12 +145,17 @@ k_pad_z(const cpx<typename GP::T>* __restrict__ Pf, const cpx<typename GP::T>* _
     line_fft<T, NP, typename GP::Rad, +1, typename GP::LayRow>(e, t, c, lds, tw);
     if (!my) return;
 #pragma unroll
-    for (int m = 0; m < GP::E; ++m) Wp[base + GP::TPL * m] = e[m];
+    for (int m = 0; m < GP::E; ++m) {
+        if constexpr (STREAM) store_stream(Wp + base + GP::TPL * m, e[m]);
+        else Wp[base + GP::TPL * m] = e[m];
+    }
 }
 
 // z-axis forward transform of the padded lines that hold N-grid modes, the
 // modes |z| <= N/2 stored in R at ((sx + H)(N + 1) + sy + H)(N + 1) + sz + H.
-template <class GP, int N>
+// Launched over the (N + 1)^2 kept lines as k_pad_z (pad_z_blocks); STREAM
+// reads the padded lines with non-temporal loads (read once).
+template <class GP, int N, bool STREAM = false>
 __global__ void __launch_bounds__((GP::THREADS_ROW))
 k_unpad_z(const cpx<typename GP::T>* __restrict__ Wp, cpx<typename GP::T>* __restrict__ R,
           const cpx<typename GP::T>* __restrict__ tw) {
@@ -160,24 +164,16 @@ k_unpad_z(const cpx<typename GP::T>* __restrict__ Wp, cpx<typename GP::T>* __res
     static_assert(pad_pair_ok<GP, N>(), "padded shape and lattice do not pair");
     __shared__ cpx<T> lds[GP::LayRow::size(GP::L)];
     const int t = threadIdx.x % GP::TPL, c = threadIdx.x / GP::TPL;
-    const size_t line = (size_t)blockIdx.x * GP::L + c;
-    const int xp = (int)(line / NP), yp = (int)(line % NP);
-    const int sx = pad_signed(xp, NP), sy = pad_signed(yp, NP);
-    const bool mx = sx <= H && sx >= -H;
-    if constexpr (NP % GP::L == 0) {   // block-uniform: all lines share the padded x index
-        if (!mx) return;
-        // and cover L consecutive padded y: leave if none of them holds modes
-        const int y0 = pad_signed((int)(((size_t)blockIdx.x * GP::L) % NP), NP);
-        const int y1 = pad_signed((int)(((size_t)blockIdx.x * GP::L + GP::L - 1) % NP), NP);
-        if (y0 > H && y1 > H) return;            // both ends in (H, N_pad/2) (no wrap: blocks align)
-        if (y0 < -H && y1 < -H) return;          // both ends in [-N_pad/2, -H)
-    }
-    const size_t base = (size_t)xp * GP::PS + (size_t)yp * NP + t;
+    const int li = (int)blockIdx.x * GP::L + c;
+    const bool my = li < (N + 1) * (N + 1);
+    const int sx = my ? li / (N + 1) - H : 0, sy = my ? li % (N + 1) - H : 0;
+    const size_t base = (size_t)pad_wrap(sx, NP) * GP::PS + (size_t)pad_wrap(sy, NP) * NP + t;
     cpx<T> e[GP::E];
 #pragma unroll
-    for (int m = 0; m < GP::E; ++m) e[m] = Wp[base + GP::TPL * m];
+    for (int m = 0; m < GP::E; ++m)   // lines past the end transform the (valid) line 0
+        e[m] = STREAM ? load_stream(Wp + base + GP::TPL * m) : Wp[base + GP::TPL * m];
     line_fft<T, NP, typename GP::Rad, -1, typename GP::LayRow>(e, t, c, lds, tw);
-    if (!mx || sy > H || sy < -H) return;
+    if (!my) return;
     const size_t row = ((size_t)(sx + H) * (N + 1) + (sy + H)) * (N + 1);
 #pragma unroll
     for (int m = 0; m < GP::E; ++m) {
