@@ -793,6 +793,11 @@ struct PadEngine final : PadBase {
     // the padded array is read once after each write and exceeds the
     // memory-side cache: its z passes use non-temporal accesses
     static constexpr bool stream_padded() { return (double)GR::MS * sizeof(cpx<T>) > 96.0 * (1 << 20); }
+    // the y passes too, where a tile row covers a whole 128-byte line (with
+    // 64-byte rows the two halves of a line come from neighbouring blocks and
+    // only ordinary accesses let the cache join them: 8-column 768-point y
+    // passes 1214 / 1275 -> 1460 / 1601 us non-temporal)
+    static constexpr bool stream_padded_y() { return stream_padded() && GP::TC * sizeof(cpx<T>) >= 128; }
     int nKG() const override { return (int)((size_t)N * N * N / 2048); }
     size_t padded_elems() const override { return GP::MS; }
 
@@ -811,7 +816,7 @@ struct PadEngine final : PadBase {
         // the y passes and the stored x rows cover the padded x planes |x| <= N/2;
         // the y inverse reads the rows |y| <= N/2 of them
         rc = timed(c, K_PAD_Y_INV, XP * KP * (KP + 1.0), [&] {
-            hipLaunchKernelGGL((k_strided<GP, 1, +1, true, false, H>), dim3(NP / GP::TC, N + 1, 1),
+            hipLaunchKernelGGL((k_strided<GP, 1, +1, true, false, H, stream_padded_y()>), dim3(NP / GP::TC, N + 1, 1),
                                dim3(GP::THREADS_COL), 0, c->stream, P(c->Wp), P(c->twp));
         });
         if (rc) return rc;
@@ -834,7 +839,7 @@ struct PadEngine final : PadBase {
             return rc;
         if ((rc = front(c))) return rc;
         rc = timed(c, K_PAD_Y_FWD, XP * KP * (1.0 + KP), [&] {
-            hipLaunchKernelGGL((k_strided<GP, 1, -1, true, false, H>), dim3(NP / GP::TC, N + 1, 1),
+            hipLaunchKernelGGL((k_strided<GP, 1, -1, true, false, H, stream_padded_y()>), dim3(NP / GP::TC, N + 1, 1),
                                dim3(GP::THREADS_COL), 0, c->stream, P(c->Wp), P(c->twp));
         });
         if (rc) return rc;
