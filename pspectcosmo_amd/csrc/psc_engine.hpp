@@ -798,6 +798,8 @@ struct PadEngine final : PadBase {
     // only ordinary accesses let the cache join them: 8-column 768-point y
     // passes 1214 / 1275 -> 1460 / 1601 us non-temporal)
     static constexpr bool stream_padded_y() { return stream_padded() && GP::TC * sizeof(cpx<T>) >= 128; }
+    // the extracted V' residual, (N + 1)^3 elements, read once by the kick
+    static constexpr bool stream_residual() { return (double)(N + 1) * (N + 1) * (N + 1) * sizeof(cpx<T>) > 96.0 * (1 << 20); }
     int nKG() const override { return (int)((size_t)N * N * N / 2048); }
     size_t padded_elems() const override { return GP::MS; }
 
@@ -854,16 +856,19 @@ struct PadEngine final : PadBase {
         if (rc) return rc;
         const StepConsts sc = consts_of(c);
         return timed(c, sync_out ? K_PAD_KICK_SYNC : K_PAD_KICK, (sync_out ? 5 : 4) * XN, [&] {
+            constexpr bool st = stream_residual();
             if (reduce && sync_out)
-                hipLaunchKernelGGL((k_kick_pad<GP, N, true, true>), dim3(nKG()), dim3(256), 0, c->stream, P(c->Rp),
+                hipLaunchKernelGGL((k_kick_pad<GP, N, true, true, st>), dim3(nKG()), dim3(256), 0, c->stream, P(c->Rp),
                                    P(c->Pf), P(c->Pp), P(c->Gf), P(c->Gp), c->ms, c->partKG, ka, sc, P(c->Ps),
                                    P(c->Gps));
             else if (reduce)
-                hipLaunchKernelGGL((k_kick_pad<GP, N, true>), dim3(nKG()), dim3(256), 0, c->stream, P(c->Rp), P(c->Pf),
-                                   P(c->Pp), P(c->Gf), P(c->Gp), c->ms, c->partKG, ka, sc, nullptr, nullptr);
+                hipLaunchKernelGGL((k_kick_pad<GP, N, true, false, st>), dim3(nKG()), dim3(256), 0, c->stream,
+                                   P(c->Rp), P(c->Pf), P(c->Pp), P(c->Gf), P(c->Gp), c->ms, c->partKG, ka, sc, nullptr,
+                                   nullptr);
             else
-                hipLaunchKernelGGL((k_kick_pad<GP, N, false>), dim3(nKG()), dim3(256), 0, c->stream, P(c->Rp), P(c->Pf),
-                                   P(c->Pp), P(c->Gf), P(c->Gp), c->ms, c->partKG, ka, sc, nullptr, nullptr);
+                hipLaunchKernelGGL((k_kick_pad<GP, N, false, false, st>), dim3(nKG()), dim3(256), 0, c->stream,
+                                   P(c->Rp), P(c->Pf), P(c->Pp), P(c->Gf), P(c->Gp), c->ms, c->partKG, ka, sc, nullptr,
+                                   nullptr);
         });
     }
 
