@@ -698,8 +698,10 @@ def run_workloads(args):
                      ("single_step_calls", lambda: bench_single_steps()),
                      ("adaptive_dt", lambda: bench_adaptive()),
                      ("gw512", lambda: bench_gw512(argparse.Namespace(**{**vars(quick), "steps": 20, "warmup": 2}))),
+                     # the ensemble's launch shape is autotuned like the headline's (a few ms)
                      ("ensemble128", lambda: bench_ensemble128(argparse.Namespace(
-                         **{**vars(quick), "steps": 60, "warmup": 3, "seq_steps": 20}))),
+                         **{**vars(quick), "steps": 60, "warmup": 3, "seq_steps": 20,
+                            "autotune": not args.no_autotune}))),
                      ("padded256", lambda: bench_padded256(argparse.Namespace(**{**vars(quick), "steps": 10, "warmup": 2}))),
                      ("padded256_n384", lambda: bench_padded256(argparse.Namespace(
                          **{**vars(quick), "steps": 10, "warmup": 2, "n_pad": 384}))),
