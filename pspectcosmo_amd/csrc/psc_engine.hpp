@@ -772,12 +772,13 @@ int Engine<G, GY>::gw_spectra(psc_ctx* c, int member, int projection, int nlog) 
 // The dealiased stepper of the N lattice on the padded shape GP (N_pad =
 // GP::N: 2N or 3N/2); the two z-axis row passes (padding, extraction) may run
 // on another plan GR.
-template <class GP, class GR, int N>
+template <class GP, class GR, int N, class GPY = GP>
 struct PadEngine final : PadBase {
     using T = typename GP::T;
     static constexpr int NP = GP::N;
     static constexpr int H = N / 2;   // the kept modes |n_d| <= H
     static_assert(GR::N == NP && std::is_same_v<typename GR::T, T>, "row plan of another lattice");
+    static_assert(GPY::N == NP && std::is_same_v<typename GPY::T, T>, "y plan of another lattice");
     static_assert(pad_pair_ok<GP, N>() && pad_pair_ok<GR, N>(), "padded shape and lattice do not pair");
     static cpx<T>* P(void* p) { return static_cast<cpx<T>*>(p); }
     static constexpr double XN = (double)N * N * N * sizeof(cpx<T>);
@@ -797,7 +798,7 @@ struct PadEngine final : PadBase {
     // 64-byte rows the two halves of a line come from neighbouring blocks and
     // only ordinary accesses let the cache join them: 8-column 768-point y
     // passes 1214 / 1275 -> 1460 / 1601 us non-temporal)
-    static constexpr bool stream_padded_y() { return stream_padded() && GP::TC * sizeof(cpx<T>) >= 128; }
+    static constexpr bool stream_padded_y() { return stream_padded() && GPY::TC * sizeof(cpx<T>) >= 128; }
     // the extracted V' residual, (N + 1)^3 elements, read once by the kick
     static constexpr bool stream_residual() { return (double)(N + 1) * (N + 1) * (N + 1) * sizeof(cpx<T>) > 96.0 * (1 << 20); }
     int nKG() const override { return (int)((size_t)N * N * N / 2048); }
@@ -818,8 +819,8 @@ struct PadEngine final : PadBase {
         // the y passes and the stored x rows cover the padded x planes |x| <= N/2;
         // the y inverse reads the rows |y| <= N/2 of them
         rc = timed(c, K_PAD_Y_INV, XP * KP * (KP + 1.0), [&] {
-            hipLaunchKernelGGL((k_strided<GP, 1, +1, true, false, H, stream_padded_y()>), dim3(NP / GP::TC, N + 1, 1),
-                               dim3(GP::THREADS_COL), 0, c->stream, P(c->Wp), P(c->twp));
+            hipLaunchKernelGGL((k_strided<GPY, 1, +1, true, false, H, stream_padded_y()>), dim3(NP / GPY::TC, N + 1, 1),
+                               dim3(GPY::THREADS_COL), 0, c->stream, P(c->Wp), P(c->twp));
         });
         if (rc) return rc;
         return timed(c, K_PAD_X_FORCE, XP * KP * 2.0, [&] {
@@ -841,8 +842,8 @@ struct PadEngine final : PadBase {
             return rc;
         if ((rc = front(c))) return rc;
         rc = timed(c, K_PAD_Y_FWD, XP * KP * (1.0 + KP), [&] {
-            hipLaunchKernelGGL((k_strided<GP, 1, -1, true, false, H, stream_padded_y()>), dim3(NP / GP::TC, N + 1, 1),
-                               dim3(GP::THREADS_COL), 0, c->stream, P(c->Wp), P(c->twp));
+            hipLaunchKernelGGL((k_strided<GPY, 1, -1, true, false, H, stream_padded_y()>), dim3(NP / GPY::TC, N + 1, 1),
+                               dim3(GPY::THREADS_COL), 0, c->stream, P(c->Wp), P(c->twp));
         });
         if (rc) return rc;
         rc = timed(c, K_PAD_Z_FWD, XN * (N + 1) * (N + 1) / ((double)N * N) * 3, [&] {
@@ -1100,14 +1101,14 @@ constexpr bool lattice_length(int n) {   // 2^k (16 .. 512) or 3 * 2^k (48 .. 38
            n == 192 || n == 384;
 }
 
-template <class G, class GR = G>
+template <class G, class GR = G, class GY = G>
 PadBase* make_pad_one(int n) {
     constexpr int NP = G::N;
     if constexpr (lattice_length(NP / 2) && NP % 2 == 0 && pad_pair_ok<G, NP / 2>() && pad_pair_ok<GR, NP / 2>())
-        if (2 * n == NP) return new PadEngine<G, GR, NP / 2>();
+        if (2 * n == NP) return new PadEngine<G, GR, NP / 2, GY>();
     if constexpr (NP % 3 == 0 && lattice_length(2 * NP / 3) && pad_pair_ok<G, 2 * NP / 3>() &&
                   pad_pair_ok<GR, 2 * NP / 3>())
-        if (3 * n == 2 * NP) return new PadEngine<G, GR, 2 * NP / 3>();
+        if (3 * n == 2 * NP) return new PadEngine<G, GR, 2 * NP / 3, GY>();
     (void)n;
     return nullptr;
 }
@@ -1118,6 +1119,8 @@ PadBase* make_pad_one(int n) {
 #define PSC_SHAPE_Y(P, NAME, G, GY) {P, G::N, NAME, &make_one<G, GY>, nullptr}
 // A shape of the padded stepper only, its z row passes on GR.
 #define PSC_SHAPE_PAD(P, NAME, G, GR) {P, G::N, NAME, nullptr, &make_pad_one<G, GR>}
+// ... and its y passes on GY.
+#define PSC_SHAPE_PAD_Y(P, NAME, G, GR, GY) {P, G::N, NAME, nullptr, &make_pad_one<G, GR, GY>}
 // A shape of the unpadded stepper only.
 #define PSC_SHAPE_ONLY(P, T, N, NAME, ...) {P, N, NAME, &make_one<S<T, N, __VA_ARGS__>>, nullptr}
 
